@@ -1,0 +1,203 @@
+/*
+ * milton_pt.h -- C ABI of the MI355X path-tracing core for Milton.
+ *
+ * The library replaces Milton's unidirectional path-tracing path: the
+ * renderer selected as "pathTracer" (mlt/src/milton/loaders/
+ * MiltonJSONSceneLoader.cpp:718-719), i.e. PathTracer::_evaluate
+ * (renderers/renderers/PathTracer.cpp:28-116) and everything below it
+ * (kd-tree traversal, primitive intersection, BSDF and area-light sampling,
+ * Russian roulette, film accumulation).  Paths in citations are relative to
+ * mlt/src/milton/ of the reference.
+ *
+ * Conventions
+ *  - Every entry point returns 0 on success and a negative MPT_E* code on
+ *    failure; mpt_last_error() returns the message of the last failure on the
+ *    context (the reference has no error codes: it turns load failures into
+ *    scene-parse errors, MiltonJSONSceneLoader.cpp:1782-1810).
+ *  - Host arrays are owned by the caller and copied in or out.  Device memory
+ *    is owned by the library and released by the matching destroy call.
+ *  - A context is used by one host thread at a time.
+ *  - All structs are plain data with explicit widths.
+ *
+ * Geometry conventions (shared with the CPU oracle):
+ *  - triangles are two-sided Moller-Trumbore triangles of one mesh
+ *    (shapes/MeshTriangle.cpp:92-169); per-vertex normals are the normalised
+ *    sum of unit face normals (shapes/Mesh.cpp:335-377);
+ *  - a sphere is (cx, cy, cz, r) (shapes/Sphere.cpp:16-45);
+ *  - a quad is the reference "plane" shape (shapes/Plane.cpp:25-106): the
+ *    unit square y=0, |x|,|z|<=0.5 mapped to center + x*ex + z*ez; its normal
+ *    is normalize(ez x ex) and its area |ex x ez|.  Quads with an emissive
+ *    material are the area lights sampled by next-event estimation;
+ *  - global primitive ids: triangles [0,nt), spheres [nt, nt+ns), quads
+ *    [nt+ns, nt+ns+nq); MPT_MISS marks no hit.
+ */
+#ifndef MILTON_PT_H_
+#define MILTON_PT_H_
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MPT_ABI_VERSION 1u
+#define MPT_MISS 0xFFFFFFFFu
+
+enum {
+  MPT_OK = 0,
+  MPT_E_INVALID = -1,     /* bad argument or scene description */
+  MPT_E_DEVICE = -2,      /* HIP runtime failure */
+  MPT_E_NOMEM = -3,       /* device or host allocation failed */
+  MPT_E_BUDGET = -4,      /* request exceeds the admission budget */
+  MPT_E_NOTFOUND = -5,    /* unknown job handle */
+  MPT_E_CANCELLED = -6    /* job was cancelled */
+};
+
+enum { MPT_BSDF_DIFFUSE = 0, MPT_BSDF_PHONG = 1, MPT_BSDF_DIELECTRIC = 2 };
+
+/* Replaces the Material property map keys read on the path:
+ * "bsdf" type (materials/Material.cpp:67-92), "kd"/"ks" (DiffuseBSDF.cpp:45,
+ * ModifiedPhongBSDF.cpp:51-62), "n" (ModifiedPhongBSDF.cpp:73), "ior" and
+ * "transparency" (DielectricBSDF.cpp:48-53), and the oriented emitter's
+ * radiance Le = power / (pi * area) (Emitter.cpp:34-48,
+ * emitters/OrientedEmitter.cpp:17-23).  Le == 0 means not an emitter. */
+typedef struct mpt_material {
+  uint32_t type;
+  float kd[3];
+  float ks[3];
+  float exponent;
+  float ior;
+  float transparency;
+  float Le[3];
+  uint32_t pad[3];
+} mpt_material;
+
+/* Replaces the scene graph the loader flattens (loaders/SceneGraph.cpp:23-135)
+ * and Scene::init (core/Scene.cpp:31-71). */
+typedef struct mpt_scene_desc {
+  uint32_t version; /* MPT_ABI_VERSION */
+  uint32_t num_vertices;
+  const float *vertices; /* 3 floats per vertex */
+  uint32_t num_triangles;
+  const uint32_t *indices;      /* 3 per triangle */
+  const uint32_t *tri_material; /* 1 per triangle */
+  uint32_t num_spheres;
+  const float *spheres; /* cx, cy, cz, r */
+  const uint32_t *sphere_material;
+  uint32_t num_quads;
+  const float *quads; /* center[3], ex[3], ez[3] */
+  const uint32_t *quad_material;
+  uint32_t num_materials;
+  const mpt_material *materials;
+  float env_radiance[3]; /* constant background radiance (Scene.cpp:116-118) */
+} mpt_scene_desc;
+
+/* Replaces kdTreeAccel::BuildParams (accel/kdTreeAccel.h:68-116). */
+typedef struct mpt_kd_params {
+  uint32_t leaf_size;     /* a node with <= leaf_size references becomes a leaf (kdMinPrimitives) */
+  uint32_t max_depth;     /* kdMaxDepth (at most 48) */
+  float cost_traversal;   /* kdCostTraversal */
+  float cost_intersect;   /* parsed but unused by the reference (kdCostIntersect) */
+  float empty_bias;       /* kdEmptyBias */
+  uint32_t num_threads;   /* kdNoThreads; 0 = all host cores */
+  uint32_t bins;          /* SAH bins per axis (0 = default) */
+} mpt_kd_params;
+
+/* Replaces ThinLensCamera's keys with aperture 0 (cameras/PinholeCamera.cpp:22,
+ * cameras/ThinLensCamera.cpp:62-86). */
+typedef struct mpt_camera {
+  float eye[3];
+  float look[3];
+  float up[3];
+  float height_angle; /* degrees, "heightAngle" */
+  float aspect;       /* "aspectRatio" */
+} mpt_camera;
+
+/* One render job: the film ("width"/"height", RenderOutput.cpp:34), samples
+ * per pixel ("noSuperSamples", SuperSampleGenerator.cpp:26-28), a maximum
+ * path depth (a cap the reference does not have) and the RNG seed ("-seed="). */
+typedef struct mpt_job {
+  mpt_camera camera;
+  uint32_t width, height, spp, max_depth;
+  uint32_t seed, job_id;
+} mpt_job;
+
+typedef struct mpt_ray {
+  float o[3];
+  float d[3]; /* unit length */
+} mpt_ray;
+
+typedef struct mpt_hit {
+  float t;       /* INFINITY on a miss */
+  uint32_t prim; /* global primitive id or MPT_MISS */
+} mpt_hit;
+
+typedef struct mpt_render_stats {
+  uint64_t paths;
+  uint64_t camera_rays;
+  uint64_t extension_rays; /* closest-hit rays, camera rays included */
+  uint64_t shadow_rays;
+  double render_ms;        /* device time of the whole render (HIP events) */
+  double extend_ms;        /* device time in the closest-hit kernel */
+  double shade_ms;
+  double shadow_ms;
+  uint64_t extend_launches;
+} mpt_render_stats;
+
+typedef struct mpt_context mpt_context;
+typedef struct mpt_scene mpt_scene;
+typedef struct mpt_film mpt_film;
+
+/* Opens a context on a HIP device.  (No reference counterpart: the
+ * reference has no device.) */
+int mpt_open(int device, mpt_context **ctx);
+int mpt_close(mpt_context *ctx);
+const char *mpt_last_error(const mpt_context *ctx);
+int mpt_abi_version(void);
+
+/* Uploads a scene and builds its kd-tree.  Replaces Scene::init
+ * (core/Scene.cpp:31-71) with ShapeSet::init / Mesh::init /
+ * kdTreeAccel::init (shapes/ShapeSet.cpp:36-80, shapes/Mesh.cpp:211-251,
+ * accel/kdTreeAccel.cpp:403-472).  kd may be NULL for defaults. */
+int mpt_scene_create(mpt_context *ctx, const mpt_scene_desc *desc, const mpt_kd_params *kd,
+                     mpt_scene **scene);
+int mpt_scene_destroy(mpt_scene *scene);
+/* out: [nodes, leaves, references, max_depth, device_bytes, build_ms, max_leaf, triangles] */
+int mpt_scene_stats(const mpt_scene *scene, uint64_t out[8]);
+
+/* Closest hit for n host rays.  Replaces Scene::getIntersection
+ * (core/Scene.cpp:104-108) -> kdTreeAccel::getIntersection
+ * (accel/kdTreeAccel.cpp:893-1016).  Hits with t <= 1e-7 are ignored. */
+int mpt_trace_closest(mpt_scene *scene, const mpt_ray *rays, uint64_t n, mpt_hit *hits);
+/* Any hit with 1e-7 < t < tmax[i].  Replaces Scene::intersects
+ * (core/Scene.cpp:110-114) -> kdTreeAccel::intersects (:1018-1094). */
+int mpt_trace_any(mpt_scene *scene, const mpt_ray *rays, const float *tmax, uint64_t n,
+                  uint8_t *occluded);
+
+/* Renders a job and copies the film (numerator / denominator per pixel,
+ * RGB float, row-major) to film_rgb (W*H*3 floats).  Replaces
+ * PointSampleRenderer::render (renderers/PointSampleRenderer.cpp:24-81) over
+ * RayTracer::sample -> PathTracer::_evaluate -> RenderOutput::addSample.
+ * stats may be NULL. */
+int mpt_render(mpt_scene *scene, const mpt_job *job, float *film_rgb, mpt_render_stats *stats);
+
+/* Device-resident film for benchmarking and serving: render into it without
+ * a host copy, then download when needed. */
+int mpt_film_create(mpt_context *ctx, uint32_t width, uint32_t height, mpt_film **film);
+int mpt_film_destroy(mpt_film *film);
+int mpt_render_film(mpt_scene *scene, const mpt_job *job, mpt_film *film, mpt_render_stats *stats);
+int mpt_film_download(mpt_film *film, float *film_rgb);
+
+/* Per-path radiance of pixels [pix_begin, pix_end) of a job, all samples,
+ * ordered (pixel, sample): L_out holds (pix_end-pix_begin)*spp*3 floats.
+ * Test hook: the value PathTracer::_evaluate returns for each camera ray. */
+int mpt_render_paths(mpt_scene *scene, const mpt_job *job, uint32_t pix_begin, uint32_t pix_end,
+                     float *L_out);
+
+/* Synchronises the context's stream. */
+int mpt_sync(mpt_context *ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

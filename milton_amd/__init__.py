@@ -1,0 +1,371 @@
+"""Python binding of the MI355X path-tracing core (ctypes over include/milton_pt.h).
+
+The native library milton_amd/libmilton_pt.so holds the HIP kernels, the kd-tree
+builder and the C ABI.  This module loads it and mirrors the reference's
+objects on the path: a Scene built from primitives and materials
+(core/Scene.cpp), kd-tree BuildParams (accel/kdTreeAccel.h:68-116) and a
+PathTracer renderer with init()/render() (renderers/renderers/PathTracer.cpp,
+renderers/PointSampleRenderer.cpp).  Importing never falls back to another
+implementation: without the built library every entry point raises.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libmilton_pt.so")
+
+MPT_ABI_VERSION = 1
+MPT_MISS = 0xFFFFFFFF
+MPT_BSDF_DIFFUSE, MPT_BSDF_PHONG, MPT_BSDF_DIELECTRIC = 0, 1, 2
+MPT_SCENE_ROOM, MPT_SCENE_SOUP, MPT_SCENE_TERRAIN = 1, 2, 3
+ERRORS = {-1: "invalid", -2: "device", -3: "nomem", -4: "budget", -5: "notfound", -6: "cancelled"}
+
+_f32p = ctypes.POINTER(ctypes.c_float)
+_u32p = ctypes.POINTER(ctypes.c_uint32)
+
+
+class Material(ctypes.Structure):
+    _fields_ = [("type", ctypes.c_uint32), ("kd", ctypes.c_float * 3), ("ks", ctypes.c_float * 3),
+                ("exponent", ctypes.c_float), ("ior", ctypes.c_float), ("transparency", ctypes.c_float),
+                ("Le", ctypes.c_float * 3), ("pad", ctypes.c_uint32 * 3)]
+
+
+class SceneDesc(ctypes.Structure):
+    _fields_ = [("version", ctypes.c_uint32),
+                ("num_vertices", ctypes.c_uint32), ("vertices", _f32p),
+                ("num_triangles", ctypes.c_uint32), ("indices", _u32p), ("tri_material", _u32p),
+                ("num_spheres", ctypes.c_uint32), ("spheres", _f32p), ("sphere_material", _u32p),
+                ("num_quads", ctypes.c_uint32), ("quads", _f32p), ("quad_material", _u32p),
+                ("num_materials", ctypes.c_uint32), ("materials", ctypes.POINTER(Material)),
+                ("env_radiance", ctypes.c_float * 3)]
+
+
+class KdParams(ctypes.Structure):
+    """kdTreeAccel::BuildParams (accel/kdTreeAccel.h:68-116)."""
+    _fields_ = [("leaf_size", ctypes.c_uint32), ("max_depth", ctypes.c_uint32),
+                ("cost_traversal", ctypes.c_float), ("cost_intersect", ctypes.c_float),
+                ("empty_bias", ctypes.c_float), ("num_threads", ctypes.c_uint32), ("bins", ctypes.c_uint32)]
+
+
+class Camera(ctypes.Structure):
+    _fields_ = [("eye", ctypes.c_float * 3), ("look", ctypes.c_float * 3), ("up", ctypes.c_float * 3),
+                ("height_angle", ctypes.c_float), ("aspect", ctypes.c_float)]
+
+
+class Job(ctypes.Structure):
+    _fields_ = [("camera", Camera), ("width", ctypes.c_uint32), ("height", ctypes.c_uint32),
+                ("spp", ctypes.c_uint32), ("max_depth", ctypes.c_uint32),
+                ("seed", ctypes.c_uint32), ("job_id", ctypes.c_uint32)]
+
+
+class Ray(ctypes.Structure):
+    _fields_ = [("o", ctypes.c_float * 3), ("d", ctypes.c_float * 3)]
+
+
+class RenderStats(ctypes.Structure):
+    _fields_ = [("paths", ctypes.c_uint64), ("camera_rays", ctypes.c_uint64),
+                ("extension_rays", ctypes.c_uint64), ("shadow_rays", ctypes.c_uint64),
+                ("render_ms", ctypes.c_double), ("extend_ms", ctypes.c_double),
+                ("shade_ms", ctypes.c_double), ("shadow_ms", ctypes.c_double),
+                ("extend_launches", ctypes.c_uint64)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+MATERIAL_DTYPE = np.dtype([("type", "<u4"), ("kd", "<f4", 3), ("ks", "<f4", 3), ("exponent", "<f4"),
+                           ("ior", "<f4"), ("transparency", "<f4"), ("Le", "<f4", 3), ("pad", "<u4", 3)])
+assert MATERIAL_DTYPE.itemsize == ctypes.sizeof(Material) == 64
+RAY_DTYPE = np.dtype([("o", "<f4", 3), ("d", "<f4", 3)])
+HIT_DTYPE = np.dtype([("t", "<f4"), ("prim", "<u4")])
+
+_lib = None
+
+
+class MptError(RuntimeError):
+    pass
+
+
+def lib():
+    """The loaded native library; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise MptError(f"{LIB_PATH} is missing: build it with `python -m milton_amd.build`")
+    L = ctypes.CDLL(LIB_PATH)
+    vp, u32, u64, i32 = ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint64, ctypes.c_int
+    sig = {
+        "mpt_abi_version": ([], i32),
+        "mpt_open": ([i32, ctypes.POINTER(vp)], i32),
+        "mpt_close": ([vp], i32),
+        "mpt_last_error": ([vp], ctypes.c_char_p),
+        "mpt_sync": ([vp], i32),
+        "mpt_scene_create": ([vp, ctypes.POINTER(SceneDesc), ctypes.POINTER(KdParams), ctypes.POINTER(vp)], i32),
+        "mpt_scene_destroy": ([vp], i32),
+        "mpt_scene_stats": ([vp, ctypes.POINTER(ctypes.c_uint64)], i32),
+        "mpt_trace_closest": ([vp, vp, u64, vp], i32),
+        "mpt_trace_any": ([vp, vp, vp, u64, vp], i32),
+        "mpt_render": ([vp, ctypes.POINTER(Job), vp, ctypes.POINTER(RenderStats)], i32),
+        "mpt_film_create": ([vp, u32, u32, ctypes.POINTER(vp)], i32),
+        "mpt_film_destroy": ([vp], i32),
+        "mpt_render_film": ([vp, ctypes.POINTER(Job), vp, ctypes.POINTER(RenderStats)], i32),
+        "mpt_film_download": ([vp, vp], i32),
+        "mpt_render_paths": ([vp, ctypes.POINTER(Job), u32, u32, vp], i32),
+        "mpt_gen_create": ([u32, u32, u32, ctypes.POINTER(vp)], i32),
+        "mpt_gen_describe": ([vp, ctypes.POINTER(SceneDesc), ctypes.POINTER(Camera)], i32),
+        "mpt_gen_destroy": ([vp], i32),
+        "mpt_gen_rays": ([vp, u64, u32, u32, u32, vp], i32),
+    }
+    for name, (args, res) in sig.items():
+        fn = getattr(L, name)
+        fn.argtypes = args
+        fn.restype = res
+    _lib = L
+    return L
+
+
+def _ptr(a):
+    return ctypes.c_void_p(a.ctypes.data) if a is not None else None
+
+
+# ---------------------------------------------------------------------------
+# Scenes as numpy arrays (shared by the device path and the oracle).
+# ---------------------------------------------------------------------------
+class SceneArrays:
+    """Primitive and material arrays in the layout of mpt_scene_desc."""
+
+    def __init__(self, vertices, indices, tri_material, spheres, sphere_material, quads, quad_material,
+                 materials, env, camera=None):
+        self.vertices = np.ascontiguousarray(vertices, np.float32).reshape(-1, 3)
+        self.indices = np.ascontiguousarray(indices, np.uint32).reshape(-1, 3)
+        self.tri_material = np.ascontiguousarray(tri_material, np.uint32).reshape(-1)
+        self.spheres = np.ascontiguousarray(spheres, np.float32).reshape(-1, 4)
+        self.sphere_material = np.ascontiguousarray(sphere_material, np.uint32).reshape(-1)
+        self.quads = np.ascontiguousarray(quads, np.float32).reshape(-1, 9)
+        self.quad_material = np.ascontiguousarray(quad_material, np.uint32).reshape(-1)
+        self.materials = np.ascontiguousarray(materials, MATERIAL_DTYPE).reshape(-1)
+        self.env = np.ascontiguousarray(env, np.float32).reshape(3)
+        self.camera = camera
+
+    @property
+    def num_prims(self):
+        return len(self.indices) + len(self.spheres) + len(self.quads)
+
+    def desc(self):
+        d = SceneDesc()
+        d.version = MPT_ABI_VERSION
+        d.num_vertices = len(self.vertices)
+        d.vertices = self.vertices.ctypes.data_as(_f32p)
+        d.num_triangles = len(self.indices)
+        d.indices = self.indices.ctypes.data_as(_u32p)
+        d.tri_material = self.tri_material.ctypes.data_as(_u32p)
+        d.num_spheres = len(self.spheres)
+        d.spheres = self.spheres.ctypes.data_as(_f32p)
+        d.sphere_material = self.sphere_material.ctypes.data_as(_u32p)
+        d.num_quads = len(self.quads)
+        d.quads = self.quads.ctypes.data_as(_f32p)
+        d.quad_material = self.quad_material.ctypes.data_as(_u32p)
+        d.num_materials = len(self.materials)
+        d.materials = self.materials.ctypes.data_as(ctypes.POINTER(Material))
+        for k in range(3):
+            d.env_radiance[k] = float(self.env[k])
+        return d
+
+    def materials_f32(self):
+        """Materials as the oracle's float[16] rows."""
+        m = self.materials
+        out = np.zeros((len(m), 16), np.float32)
+        out[:, 0] = m["type"]
+        out[:, 1:4] = m["kd"]
+        out[:, 4:7] = m["ks"]
+        out[:, 7] = m["exponent"]
+        out[:, 8] = m["ior"]
+        out[:, 9] = m["transparency"]
+        out[:, 10:13] = m["Le"]
+        return out
+
+
+def generate(kind, count=0, seed=1):
+    """Seeded benchmark scene (MPT_SCENE_*) as SceneArrays with its camera."""
+    L = lib()
+    g = ctypes.c_void_p()
+    rc = L.mpt_gen_create(kind, count, seed, ctypes.byref(g))
+    if rc:
+        raise MptError(f"mpt_gen_create failed: {ERRORS.get(rc, rc)}")
+    try:
+        d, cam = SceneDesc(), Camera()
+        L.mpt_gen_describe(g, ctypes.byref(d), ctypes.byref(cam))
+
+        def arr(p, n, dt, w):
+            if n == 0:
+                return np.zeros((0, w), dt)
+            return np.ctypeslib.as_array(p, shape=(n * w,)).astype(dt).reshape(n, w).copy()
+
+        mats = np.frombuffer(ctypes.string_at(d.materials, 64 * d.num_materials), MATERIAL_DTYPE).copy()
+        sa = SceneArrays(arr(d.vertices, d.num_vertices, np.float32, 3),
+                         arr(d.indices, d.num_triangles, np.uint32, 3),
+                         arr(d.tri_material, d.num_triangles, np.uint32, 1),
+                         arr(d.spheres, d.num_spheres, np.float32, 4),
+                         arr(d.sphere_material, d.num_spheres, np.uint32, 1),
+                         arr(d.quads, d.num_quads, np.float32, 9),
+                         arr(d.quad_material, d.num_quads, np.uint32, 1),
+                         mats, np.array(d.env_radiance[:], np.float32),
+                         camera=camera_dict(cam))
+        sa._gen = None
+        return sa
+    finally:
+        L.mpt_gen_destroy(g)
+
+
+def generate_rays(kind, count, n, seed=7, film=(1920, 1080), scene_seed=1):
+    """Seeded parity rays for a generated scene (half camera rays, half random)."""
+    L = lib()
+    g = ctypes.c_void_p()
+    rc = L.mpt_gen_create(kind, count, scene_seed, ctypes.byref(g))
+    if rc:
+        raise MptError(f"mpt_gen_create failed: {ERRORS.get(rc, rc)}")
+    try:
+        rays = np.zeros(n, RAY_DTYPE)
+        rc = L.mpt_gen_rays(g, n, seed, film[0], film[1], _ptr(rays))
+        if rc:
+            raise MptError("mpt_gen_rays failed")
+        return rays
+    finally:
+        L.mpt_gen_destroy(g)
+
+
+def camera_dict(cam):
+    return {"eye": list(cam.eye), "look": list(cam.look), "up": list(cam.up),
+            "height_angle": cam.height_angle, "aspect": cam.aspect}
+
+
+def make_job(camera, width, height, spp, max_depth, seed=1, job_id=0):
+    j = Job()
+    for k in range(3):
+        j.camera.eye[k] = camera["eye"][k]
+        j.camera.look[k] = camera["look"][k]
+        j.camera.up[k] = camera["up"][k]
+    j.camera.height_angle = camera["height_angle"]
+    j.camera.aspect = camera["aspect"]
+    j.width, j.height, j.spp, j.max_depth, j.seed, j.job_id = width, height, spp, max_depth, seed, job_id
+    return j
+
+
+# ---------------------------------------------------------------------------
+# Device objects.
+# ---------------------------------------------------------------------------
+class Context:
+    """A HIP device context (mpt_open)."""
+
+    def __init__(self, device=0):
+        self.L = lib()
+        self.h = ctypes.c_void_p()
+        rc = self.L.mpt_open(device, ctypes.byref(self.h))
+        if rc:
+            raise MptError(f"mpt_open({device}) failed: {ERRORS.get(rc, rc)}")
+
+    def check(self, rc, what):
+        if rc:
+            msg = self.L.mpt_last_error(self.h)
+            raise MptError(f"{what} failed ({ERRORS.get(rc, rc)}): {msg.decode() if msg else ''}")
+
+    def close(self):
+        if self.h:
+            self.L.mpt_close(self.h)
+            self.h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Scene:
+    """A device-resident scene with its kd-tree (Scene::init + kdTreeAccel::init)."""
+
+    def __init__(self, ctx, arrays, kd=None):
+        self.ctx, self.arrays = ctx, arrays
+        self.h = ctypes.c_void_p()
+        d = arrays.desc()
+        kp = kd if kd is not None else KdParams()
+        ctx.check(ctx.L.mpt_scene_create(ctx.h, ctypes.byref(d), ctypes.byref(kp), ctypes.byref(self.h)),
+                  "mpt_scene_create")
+
+    def stats(self):
+        out = (ctypes.c_uint64 * 8)()
+        self.ctx.L.mpt_scene_stats(self.h, out)
+        keys = ["nodes", "leaves", "refs", "max_depth", "device_bytes", "build_ms", "max_leaf", "triangles"]
+        return dict(zip(keys, list(out)))
+
+    def trace_closest(self, rays):
+        rays = np.ascontiguousarray(rays, RAY_DTYPE)
+        hits = np.zeros(len(rays), HIT_DTYPE)
+        self.ctx.check(self.ctx.L.mpt_trace_closest(self.h, _ptr(rays), len(rays), _ptr(hits)),
+                       "mpt_trace_closest")
+        return hits
+
+    def trace_any(self, rays, tmax):
+        rays = np.ascontiguousarray(rays, RAY_DTYPE)
+        tmax = np.ascontiguousarray(tmax, np.float32)
+        occ = np.zeros(len(rays), np.uint8)
+        self.ctx.check(self.ctx.L.mpt_trace_any(self.h, _ptr(rays), _ptr(tmax), len(rays), _ptr(occ)),
+                       "mpt_trace_any")
+        return occ
+
+    def render(self, job):
+        film = np.zeros((job.height, job.width, 3), np.float32)
+        st = RenderStats()
+        self.ctx.check(self.ctx.L.mpt_render(self.h, ctypes.byref(job), _ptr(film), ctypes.byref(st)), "mpt_render")
+        return film, st.as_dict()
+
+    def render_paths(self, job, pix_begin, pix_end):
+        L = np.zeros(((pix_end - pix_begin) * job.spp, 3), np.float32)
+        self.ctx.check(self.ctx.L.mpt_render_paths(self.h, ctypes.byref(job), pix_begin, pix_end, _ptr(L)),
+                       "mpt_render_paths")
+        return L
+
+    def close(self):
+        if self.h:
+            self.ctx.L.mpt_scene_destroy(self.h)
+            self.h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Film:
+    """Device-resident film (numerator/denominator per pixel)."""
+
+    def __init__(self, ctx, width, height):
+        self.ctx, self.width, self.height = ctx, width, height
+        self.h = ctypes.c_void_p()
+        ctx.check(ctx.L.mpt_film_create(ctx.h, width, height, ctypes.byref(self.h)), "mpt_film_create")
+
+    def render(self, scene, job):
+        st = RenderStats()
+        self.ctx.check(self.ctx.L.mpt_render_film(scene.h, ctypes.byref(job), self.h, ctypes.byref(st)),
+                       "mpt_render_film")
+        return st.as_dict()
+
+    def download(self):
+        out = np.zeros((self.height, self.width, 3), np.float32)
+        self.ctx.check(self.ctx.L.mpt_film_download(self.h, _ptr(out)), "mpt_film_download")
+        return out
+
+    def close(self):
+        if self.h:
+            self.ctx.L.mpt_film_destroy(self.h)
+            self.h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,436 @@
+// Device functions of the path-tracing core (gfx950).
+//
+// Traversal replaces kdTreeAccel::getIntersection / intersects
+// (accel/kdTreeAccel.cpp:893-1094) over the flattened layout of kdbuild.h.
+// Intersection replaces MeshTriangle::getIntersection (shapes/MeshTriangle.cpp:
+// 92-169), Sphere (shapes/Sphere.cpp:16-73) and Plane (shapes/Plane.cpp:25-49).
+// Shading replaces SurfacePoint::init + Material::initSurfacePoint
+// (core/SurfacePoint.cpp:49-78, materials/Material.cpp:157-177) and the BSDFs
+// (materials/bsdfs/DiffuseBSDF.cpp, ModifiedPhongBSDF.cpp, DielectricBSDF.cpp).
+#ifndef MPT_DEVICE_H_
+#define MPT_DEVICE_H_
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/milton_pt.h"
+
+namespace mpt_dev {
+
+constexpr uint32_t kMiss = 0xFFFFFFFFu;
+constexpr float kEps = 1e-7f;            // EPSILON, common/arch.h:137
+constexpr float kAirIor = 1.0003f;       // IndexOfRefraction::AIR
+constexpr int kBlock = 256;              // threads per block of every traversal kernel
+constexpr int kStack = 16;               // short-stack entries per lane (LDS ring)
+constexpr float kPi = 3.14159265358979323846f;
+
+enum { KIND_TRIANGLE = 0, KIND_SPHERE = 1, KIND_QUAD = 2 };
+
+// Read-only scene as seen by the kernels.
+struct Scene {
+  const uint2 *nodes;        // kd nodes (kdbuild.h)
+  const float4 *recs;        // 3 float4 per leaf reference
+  float box_lo[3], box_hi[3];
+  const uint4 *tris;         // (a, b, c, material)
+  const float4 *verts;       // xyz
+  const float4 *vnormals;    // xyz
+  const float4 *spheres;     // (centre, r)
+  const uint32_t *sphere_mat;
+  const float4 *quads;       // 4 per quad: (c, area), (ex, 0), (ez, 0), (n, material bits)
+  const mpt_material *mats;
+  const uint32_t *lights;    // quad indices of emissive quads
+  uint32_t num_lights;
+  uint32_t nt, ns, nq;
+  float env[3];
+};
+
+__device__ __forceinline__ float3 f3(float x, float y, float z) { return make_float3(x, y, z); }
+__device__ __forceinline__ float3 add(float3 a, float3 b) { return f3(a.x + b.x, a.y + b.y, a.z + b.z); }
+__device__ __forceinline__ float3 sub(float3 a, float3 b) { return f3(a.x - b.x, a.y - b.y, a.z - b.z); }
+__device__ __forceinline__ float3 mul(float3 a, float s) { return f3(a.x * s, a.y * s, a.z * s); }
+__device__ __forceinline__ float3 mul(float3 a, float3 b) { return f3(a.x * b.x, a.y * b.y, a.z * b.z); }
+__device__ __forceinline__ float dot(float3 a, float3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__device__ __forceinline__ float3 cross(float3 a, float3 b) {
+  return f3(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
+}
+__device__ __forceinline__ float3 normalize(float3 a) { return mul(a, rsqrtf(dot(a, a))); }
+__device__ __forceinline__ float len(float3 a) { return sqrtf(dot(a, a)); }
+__device__ __forceinline__ float3 xyz(float4 a) { return f3(a.x, a.y, a.z); }
+__device__ __forceinline__ float comp(float3 a, uint32_t i) { return i == 0 ? a.x : (i == 1 ? a.y : a.z); }
+__device__ __forceinline__ bool is_black(float3 a) { return a.x == 0.0f && a.y == 0.0f && a.z == 0.0f; }
+__device__ __forceinline__ float max3(float3 a) { return fmaxf(a.x, fmaxf(a.y, a.z)); }
+
+// Per-ray constants: reciprocal direction for the kd planes and the shear of
+// the watertight triangle test (Woop, Benthin, Wald, JCGT 2013): row vectors
+// mx, my, mz map (v - o) to the ray's sheared frame.
+struct RaySetup {
+  float3 o, d, inv;
+  float3 mx, my, mz;
+  float dkz;   // |d[kz]|, relates the sheared determinant to Moller-Trumbore's
+};
+
+__device__ __forceinline__ float safe_inv(float x) {
+  const float ax = fmaxf(fabsf(x), 1e-20f);
+  return 1.0f / copysignf(ax, x);
+}
+
+__device__ __forceinline__ RaySetup setup_ray(float3 o, float3 d) {
+  RaySetup r;
+  r.o = o;
+  r.d = d;
+  r.inv = f3(safe_inv(d.x), safe_inv(d.y), safe_inv(d.z));
+  const float ax = fabsf(d.x), ay = fabsf(d.y), az = fabsf(d.z);
+  uint32_t kz = (ax >= ay && ax >= az) ? 0 : (ay >= az ? 1 : 2);
+  uint32_t kx = kz == 2 ? 0 : kz + 1;
+  uint32_t ky = kx == 2 ? 0 : kx + 1;
+  const float dz = comp(d, kz);
+  if (dz < 0.0f) {
+    const uint32_t t = kx;
+    kx = ky;
+    ky = t;
+  }
+  const float sx = comp(d, kx) / dz, sy = comp(d, ky) / dz, sz = 1.0f / dz;
+  r.mx = f3(kx == 0 ? 1.0f : (kz == 0 ? -sx : 0.0f), kx == 1 ? 1.0f : (kz == 1 ? -sx : 0.0f),
+            kx == 2 ? 1.0f : (kz == 2 ? -sx : 0.0f));
+  r.my = f3(ky == 0 ? 1.0f : (kz == 0 ? -sy : 0.0f), ky == 1 ? 1.0f : (kz == 1 ? -sy : 0.0f),
+            ky == 2 ? 1.0f : (kz == 2 ? -sy : 0.0f));
+  r.mz = f3(kz == 0 ? sz : 0.0f, kz == 1 ? sz : 0.0f, kz == 2 ? sz : 0.0f);
+  r.dkz = fabsf(dz);
+  return r;
+}
+
+// Two-sided watertight triangle test.  Matches the reference's acceptance
+// rules: edges and vertices count as hits (inclusive u, v, u + v <= 1), and a
+// triangle whose Moller-Trumbore determinant |d . ((v1-v0) x (v2-v0))| is at
+// most EPSILON is rejected (EQ(det, 0), MeshTriangle.cpp:144).  The sheared
+// determinant U + V + W equals that product divided by d[kz].
+__device__ __forceinline__ float tri_t(const RaySetup &r, float3 v0, float3 v1, float3 v2) {
+  const float3 A = sub(v0, r.o), B = sub(v1, r.o), C = sub(v2, r.o);
+  const float Ax = dot(A, r.mx), Ay = dot(A, r.my);
+  const float Bx = dot(B, r.mx), By = dot(B, r.my);
+  const float Cx = dot(C, r.mx), Cy = dot(C, r.my);
+  const float U = Cx * By - Cy * Bx;
+  const float V = Ax * Cy - Ay * Cx;
+  const float W = Bx * Ay - By * Ax;
+  if ((U < 0.0f || V < 0.0f || W < 0.0f) && (U > 0.0f || V > 0.0f || W > 0.0f)) return INFINITY;
+  const float det = U + V + W;
+  if (fabsf(det) * r.dkz <= kEps) return INFINITY;
+  const float T = U * dot(A, r.mz) + V * dot(B, r.mz) + W * dot(C, r.mz);
+  return T / det;
+}
+
+// Sphere (centre c, radius rad): nearest root above EPSILON; a ray leaving the
+// sphere it starts on skips the root at its own origin.
+__device__ __forceinline__ float sphere_t(const RaySetup &r, float3 c, float rad, bool from_self) {
+  const float3 f = sub(r.o, c);
+  const float b = -dot(f, r.d);
+  const float3 l = add(f, mul(r.d, b));
+  const float disc = rad * rad - dot(l, l);
+  if (disc < 0.0f) return INFINITY;
+  const float cc = dot(f, f) - rad * rad;
+  const float q = b + copysignf(sqrtf(disc), b);
+  float t0 = q != 0.0f ? cc / q : 0.0f, t1 = q;
+  if (t0 > t1) {
+    const float t = t0;
+    t0 = t1;
+    t1 = t;
+  }
+  const float lo = from_self ? fmaxf(kEps, 1e-3f * rad) : kEps;
+  if (t0 > lo) return t0;
+  if (t1 > lo) return t1;
+  return INFINITY;
+}
+
+// Quad: the reference unit plane mapped by (ex, ez) around c, two-sided.
+__device__ __forceinline__ float quad_t(const RaySetup &r, float3 c, float3 ex, float3 ez) {
+  const float3 n = cross(ez, ex);
+  const float dn = dot(n, r.d);
+  const float t = dot(n, sub(c, r.o)) / dn;
+  const float3 p = sub(add(r.o, mul(r.d, t)), c);
+  const float u = dot(p, ex) / dot(ex, ex), v = dot(p, ez) / dot(ez, ez);
+  return (fabsf(u) <= 0.5f && fabsf(v) <= 0.5f) ? t : INFINITY;
+}
+
+// kd traversal with a kStack-entry ring stack in LDS and kd-restart on
+// overflow (Foley & Sugerman 2005).  `stk` points at this lane's column of a
+// [kStack][kBlock] uint2 array.  skip_origin is the primitive the ray leaves
+// (self-intersection guard in place of the reference's FP64 t > EPSILON),
+// skip_target a shadow ray's light.  ANY=true returns at the first hit.
+template <bool ANY>
+__device__ __forceinline__ bool trace(const Scene &sc, const RaySetup &r, float tlimit, uint32_t skip_origin,
+                                      uint32_t skip_target, uint2 *stk, float &t_out, uint32_t &id_out) {
+  // Root slab test (accel/AABB.inl:25-56).
+  float tmin = 0.0f, tmax = tlimit;
+  {
+    const float tx0 = (sc.box_lo[0] - r.o.x) * r.inv.x, tx1 = (sc.box_hi[0] - r.o.x) * r.inv.x;
+    const float ty0 = (sc.box_lo[1] - r.o.y) * r.inv.y, ty1 = (sc.box_hi[1] - r.o.y) * r.inv.y;
+    const float tz0 = (sc.box_lo[2] - r.o.z) * r.inv.z, tz1 = (sc.box_hi[2] - r.o.z) * r.inv.z;
+    tmin = fmaxf(tmin, fmaxf(fminf(tx0, tx1), fmaxf(fminf(ty0, ty1), fminf(tz0, tz1))));
+    tmax = fminf(tmax, fminf(fmaxf(tx0, tx1), fminf(fmaxf(ty0, ty1), fmaxf(tz0, tz1))));
+    // widen by a few ulps so rays grazing the root box still enter it
+    tmin = fmaxf(0.0f, tmin * (1.0f - 1e-6f) - 1e-7f);
+    tmax = tmax * (1.0f + 1e-6f) + 1e-7f;
+  }
+  t_out = INFINITY;
+  id_out = kMiss;
+  if (tmin > tmax) return false;
+  const float troot = tmax;
+  float best = tlimit;
+  uint32_t best_id = kMiss;
+  uint32_t node = 0;
+  int depth = 0;     // valid entries on the ring
+  int top = 0;       // next write slot
+  bool dropped = false;
+  for (;;) {
+    uint2 nd = sc.nodes[node];
+    while ((nd.x & 3u) != 3u) {
+      const uint32_t axis = nd.x & 3u;
+      const float split = __uint_as_float(nd.y);
+      const float oa = comp(r.o, axis), da = comp(r.d, axis), ia = comp(r.inv, axis);
+      const float tp = (split - oa) * ia;
+      const uint32_t child = nd.x >> 2;
+      const bool below = (oa < split) || (oa == split && da <= 0.0f);
+      const uint32_t first = below ? child : child + 1;
+      const uint32_t second = below ? child + 1 : child;
+      if (tp > tmax || tp <= 0.0f) {
+        node = first;
+      } else if (tp < tmin) {
+        node = second;
+      } else {
+        stk[top * kBlock] = make_uint2(second, __float_as_uint(tmax));
+        top = (top + 1) & (kStack - 1);
+        if (depth == kStack) dropped = true;
+        else ++depth;
+        node = first;
+        tmax = tp;
+      }
+      nd = sc.nodes[node];
+    }
+    const uint32_t count = nd.x >> 2;
+    const float4 *rec = sc.recs + 3ull * nd.y;
+    for (uint32_t k = 0; k < count; ++k, rec += 3) {
+      const float4 a = rec[0], b = rec[1];
+      const uint32_t id = __float_as_uint(a.w);
+      const uint32_t kind = __float_as_uint(b.w);
+      if (id == skip_target) continue;
+      float t;
+      if (kind == KIND_TRIANGLE) {
+        if (id == skip_origin) continue;
+        const float4 c = rec[2];
+        t = tri_t(r, xyz(a), xyz(b), xyz(c));
+      } else if (kind == KIND_SPHERE) {
+        t = sphere_t(r, xyz(a), b.x, id == skip_origin);
+      } else {
+        if (id == skip_origin) continue;
+        const float4 c = rec[2];
+        t = quad_t(r, xyz(a), xyz(b), xyz(c));
+      }
+      if (t > kEps && t < best) {
+        best = t;
+        best_id = id;
+        if (ANY) {
+          t_out = t;
+          id_out = id;
+          return true;
+        }
+      }
+    }
+    if (best <= tmax) break;
+    if (depth == 0) {
+      if (!dropped || tmax >= troot) break;
+      // restart from the root past the cells already visited
+      dropped = false;
+      node = 0;
+      tmin = tmax;
+      tmax = troot;
+      continue;
+    }
+    --depth;
+    top = (top + kStack - 1) & (kStack - 1);
+    const uint2 e = stk[top * kBlock];
+    node = e.x;
+    tmin = tmax;
+    tmax = __uint_as_float(e.y);
+    if (tmin > best) break;
+  }
+  if (best_id != kMiss) {
+    t_out = best;
+    id_out = best_id;
+    return true;
+  }
+  return false;
+}
+
+// ---------------------------------------------------------------------------
+// Shading.
+// ---------------------------------------------------------------------------
+// Vector::getOrthonormalBasis (common/math/Vector.inl:527-538).
+__device__ __forceinline__ void onb(float3 n, float3 &U, float3 &V) {
+  const float3 up = fabsf(n.y) > 0.8f ? f3(1, 0, 0) : f3(0, 1, 0);
+  U = normalize(cross(up, n));
+  V = normalize(cross(n, U));
+}
+
+// Vector3::cosRandom (common/math/Vector.cpp:21-29) through convertHemisphere
+// (common/math/sisd/Vector3.inl:18-22): theta = acos(sqrt(u1)), phi = 2 pi u2.
+// The hardware sine/cosine take revolutions, so phi enters as u2 directly.
+__device__ __forceinline__ float3 cos_random(float3 n, float u1, float u2) {
+  const float ct = sqrtf(u1), st = sqrtf(fmaxf(0.0f, 1.0f - u1));
+  const float cp = __builtin_amdgcn_cosf(u2), sp = __builtin_amdgcn_sinf(u2);
+  float3 U, V;
+  onb(n, U, V);
+  return add(add(mul(U, st * cp), mul(n, ct)), mul(V, st * sp));
+}
+
+__device__ __forceinline__ float3 reflect(float3 w, float3 n) { return add(mul(n, -2.0f * dot(w, n)), w); }
+
+// Vector::refractVector (common/math/Vector.inl:468-494).
+__device__ __forceinline__ bool refract(float3 v, float3 normal, float in, float out, float3 &wt) {
+  float3 norm;
+  float n;
+  if (dot(v, normal) < 0.0f) {
+    norm = normal;
+    n = in / out;
+  } else {
+    norm = mul(normal, -1.0f);
+    n = out / in;
+  }
+  const float3 w = mul(v, -1.0f);
+  const float d = dot(w, norm);
+  const float det = 1.0f - n * n * (1.0f - d * d);
+  if (det < 0.0f) return false;
+  wt = add(mul(w, -n), mul(norm, n * d - sqrtf(det)));
+  return true;
+}
+
+// Fresnel::getFr (materials/bsdfs/Fresnel.h:45-90), unpolarised.
+__device__ __forceinline__ float fresnel(float3 wi, float3 N, float n1, float n2) {
+  float ci = dot(N, wi);
+  if (ci < 0.0f) {
+    ci = -ci;
+  } else {
+    const float t = n1;
+    n1 = n2;
+    n2 = t;
+  }
+  const float snell = (n1 / n2) * sqrtf(fmaxf(0.0f, 1.0f - ci * ci));
+  if (fabsf(snell) >= 1.0f) return 1.0f;
+  const float ct = sqrtf(fmaxf(0.0f, 1.0f - snell * snell));
+  const float Rs = (n1 * ci - n2 * ct) / (n1 * ci + n2 * ct);
+  const float Rp = (n1 * ct - n2 * ci) / (n1 * ct + n2 * ci);
+  return (Rs * Rs + Rp * Rp) * 0.5f;
+}
+
+__device__ __forceinline__ float3 kd_of(const mpt_material &m) { return f3(m.kd[0], m.kd[1], m.kd[2]); }
+__device__ __forceinline__ float3 ks_of(const mpt_material &m) { return f3(m.ks[0], m.ks[1], m.ks[2]); }
+__device__ __forceinline__ float3 le_of(const mpt_material &m) { return f3(m.Le[0], m.Le[1], m.Le[2]); }
+__device__ __forceinline__ float avg3(float3 a) { return (a.x + a.y + a.z) * (1.0f / 3.0f); }
+__device__ __forceinline__ bool emissive(const mpt_material &m) { return m.Le[0] > 0 || m.Le[1] > 0 || m.Le[2] > 0; }
+
+// f(wi, wo) of diffuse (DiffuseBSDF.cpp:40-48) and modified Phong
+// (ModifiedPhongBSDF.cpp:164-181); zero for the dielectric's delta lobes.
+__device__ __forceinline__ float3 bsdf_eval(const mpt_material &m, float3 n, float3 wi, float3 wo) {
+  if (m.type == MPT_BSDF_DIFFUSE) {
+    if (dot(n, wi) >= 0.0f || dot(n, wo) <= 0.0f) return f3(0, 0, 0);
+    return mul(kd_of(m), 1.0f / kPi);
+  }
+  if (m.type == MPT_BSDF_PHONG) {
+    if (dot(n, wo) <= 0.0f) return f3(0, 0, 0);
+    const float a = fminf(1.0f, fmaxf(0.0f, dot(reflect(wi, n), wo)));
+    const float kda = avg3(kd_of(m)), ksa = avg3(ks_of(m));
+    const float an = a > 0.0f ? exp2f(m.exponent * log2f(a)) : 0.0f;
+    const float fs_s = ksa > 0.0f ? (m.exponent + 2.0f) / (2.0f * kPi * ksa) * an : 0.0f;
+    const float fs_d = kda > 0.0f ? 1.0f / (kPi * kda) : 0.0f;
+    return add(mul(kd_of(m), fs_d), mul(ks_of(m), fs_s));
+  }
+  return f3(0, 0, 0);
+}
+
+struct Sample {
+  float3 wo;   // zero if absorbed
+  float3 fs;   // f / pdf
+};
+
+// BSDF sampling; u = (u0, u1) direction, u3 lobe choice (Phong) or
+// reflect-vs-refract (dielectric).
+__device__ __forceinline__ Sample bsdf_sample(const mpt_material &m, float3 n, float3 wi, float u0, float u1,
+                                              float u3, uint32_t ior_index) {
+  Sample s;
+  s.wo = f3(0, 0, 0);
+  s.fs = f3(0, 0, 0);
+  if (m.type == MPT_BSDF_DIFFUSE) {
+    // DiffuseBSDF::sample / getPd (DiffuseBSDF.cpp:21-38): f/pdf = kd.
+    if (dot(n, wi) > 0.0f) return s;
+    const float3 wo = cos_random(n, u0, u1);
+    if (dot(n, wo) <= 0.0f) return s;
+    s.wo = wo;
+    s.fs = mul(bsdf_eval(m, n, wi, wo), kPi);
+    return s;
+  }
+  if (m.type == MPT_BSDF_PHONG) {
+    // ModifiedPhongBSDF::sample / getPd (ModifiedPhongBSDF.cpp:84-162).
+    const float kda = avg3(kd_of(m)), ksa = avg3(ks_of(m));
+    const float p0 = u3 - kEps;
+    float3 wo;
+    bool spec;
+    if (p0 <= kda) {
+      spec = false;
+      wo = cos_random(n, u0, u1);
+    } else if (p0 <= kda + ksa) {
+      spec = true;
+      const float ca = exp2f(log2f(u0) / (m.exponent + 1.0f));
+      const float sa = sqrtf(fmaxf(0.0f, 1.0f - ca * ca));
+      const float cp = __builtin_amdgcn_cosf(u1), sp = __builtin_amdgcn_sinf(u1);
+      const float3 R = reflect(wi, n);
+      float3 rr = R, uu, vv;
+      if (fabsf(dot(n, rr)) < 0.95f) {
+        uu = normalize(cross(rr, n));
+        vv = cross(uu, rr);
+      } else {
+        rr = normalize(rr);
+        onb(rr, vv, uu);
+      }
+      const float3 w = add(add(mul(uu, sa * cp), mul(rr, ca)), mul(vv, sa * sp));
+      wo = dot(n, w) > 0.0f ? w : R;
+    } else {
+      return s;
+    }
+    const float cosA = dot(n, wo);
+    if (cosA <= 0.0f) return s;
+    float pdf;
+    if (spec) {
+      const float c = fabsf(dot(reflect(wi, n), wo));
+      const float cn = c > 0.0f ? exp2f(m.exponent * log2f(c)) : 0.0f;
+      pdf = fminf(1.0f, (m.exponent + 1.0f) / (2.0f * kPi) * cn);
+    } else {
+      pdf = fabsf(cosA) / kPi;
+    }
+    pdf = pdf / cosA;
+    if (!(pdf > 0.0f)) return s;
+    s.wo = wo;
+    s.fs = mul(bsdf_eval(m, n, wi, wo), 1.0f / pdf);
+    return s;
+  }
+  // DielectricBSDF::sample / getPd / evaluate (DielectricBSDF.cpp:55-158):
+  // pdf 1, f = ks, and a refraction keeps only the path's IOR channel.
+  const float3 N = dot(n, wi) > 0.0f ? mul(n, -1.0f) : n;
+  float3 wt;
+  bool refracted = false;
+  if (refract(wi, n, kAirIor, m.ior, wt)) {
+    const float Fr = fresnel(wi, n, kAirIor, m.ior);
+    const float Fs = m.transparency * (1.0f - Fr);
+    refracted = u3 < Fs;
+  }
+  if (refracted) {
+    s.wo = wt;
+    s.fs = f3(ior_index == 0 ? m.ks[0] : 0.0f, ior_index == 1 ? m.ks[1] : 0.0f, ior_index == 2 ? m.ks[2] : 0.0f);
+  } else {
+    s.wo = reflect(wi, N);
+    s.fs = ks_of(m);
+  }
+  return s;
+}
+
+}  // namespace mpt_dev
+
+#endif

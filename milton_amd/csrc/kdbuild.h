@@ -1,0 +1,56 @@
+// Host SAH kd-tree builder of the path-tracing core.
+//
+// Replaces kdTreeAccel::init / _buildTree / _computeSplitPlaneSAH
+// (accel/kdTreeAccel.cpp:403-472, 533-630, 722-877).  The reference sorts the
+// primitive extrema of every node (O(N log^2 N), single-threaded) and stores
+// 16-byte pointer nodes with per-leaf heap vectors.  This builder bins the
+// clipped primitive bounds per node (O(N log N)), builds subtrees on several
+// threads, and emits the device layout directly:
+//   node   = 2 x u32: word0 = (child << 2) | axis for an interior node (the
+//            right child is child + 1), (count << 2) | 3 for a leaf;
+//            word1 = split position (float bits) or the leaf's first record;
+//   record = 12 x f32 per leaf reference, stored in leaf order so a leaf is a
+//            contiguous run: (v0, id), (v1, kind), (v2, 0) for a triangle,
+//            (centre, id), (r, 0, 0, kind), 0 for a sphere,
+//            (centre, id), (ex, kind), (ez, 0) for a quad.
+#ifndef MPT_KDBUILD_H_
+#define MPT_KDBUILD_H_
+
+#include <stdint.h>
+
+#include <functional>
+#include <vector>
+
+namespace mpt_kd {
+
+enum { KIND_TRIANGLE = 0, KIND_SPHERE = 1, KIND_QUAD = 2 };
+
+struct Box {
+  float lo[3], hi[3];
+};
+
+struct Params {
+  uint32_t leaf_size = 2;
+  uint32_t max_depth = 0;          // 0: 8 + 1.3 log2(N), at most 48
+  float cost_traversal = 1.0f;
+  float cost_intersect = 1.5f;
+  float empty_bias = 0.8f;
+  uint32_t num_threads = 0;        // 0: hardware concurrency
+  uint32_t bins = 32;
+};
+
+struct Output {
+  std::vector<uint32_t> nodes;     // 2 words per node
+  std::vector<float> records;      // 12 floats per reference
+  Box box;
+  uint64_t n_nodes = 0, n_leaves = 0, n_refs = 0, max_depth = 0, max_leaf = 0;
+  double build_ms = 0;
+};
+
+// prims: bounds per primitive; write_record(prim, out12) fills its record.
+void build(const std::vector<Box> &prims, const Params &p,
+           const std::function<void(uint32_t, float *)> &write_record, Output &out);
+
+}  // namespace mpt_kd
+
+#endif

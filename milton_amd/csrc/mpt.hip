@@ -1,0 +1,991 @@
+// MI355X path-tracing core: wavefront kernels and the C ABI (include/milton_pt.h).
+//
+// One render job is processed in batches of whole pixels (all of a pixel's
+// samples are in one batch, so the film needs no atomics and is
+// deterministic).  Per batch:
+//   k_raygen    camera rays of every (pixel, sample) slot
+//   per depth:  k_extend  closest hit of the live ray queue
+//               k_shade   emission, NEE (shadow-ray queue), BSDF sample,
+//                         Russian roulette; survivors are appended to the next
+//                         ray queue with one atomic per wavefront
+//               k_shadow  any-hit test of the shadow queue
+//   k_finalize  the depth-0 spectral-channel rule of PathTracer.cpp:107-114
+//   k_film      per-pixel sum over the pixel's samples
+// Queue counts stay on the device: every kernel reads its count and loops
+// grid-stride, so a batch needs no host synchronisation.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "../../include/milton_pt.h"
+#include "device.h"
+#include "kdbuild.h"
+#include "rng.h"
+
+using namespace mpt_dev;
+
+namespace {
+
+// Path flags.
+constexpr uint32_t F_IOR_MASK = 3u;          // the path's IOR channel (PathTracer.cpp:32)
+constexpr uint32_t F_ANY_SPECULAR = 4u;      // data["specular"] (PathTracer.cpp:46-47)
+constexpr uint32_t F_D0_CONTINUED = 8u;      // the depth-0 vertex recursed
+constexpr uint32_t F_EMIT_AFTER_SPEC = 16u;  // emittedContribution > EPSILON (PathTracer.cpp:97-98)
+
+struct CameraDev {
+  float eye[3], u[3], v[3], n[3];
+  float tan_w, tan_h;
+};
+
+struct JobDev {
+  CameraDev cam;
+  uint32_t width, height, spp, max_depth;
+  uint32_t seed, job_id;
+  uint32_t side;       // floor(sqrt(spp)) strata per axis (core/Viewport.h:29-36)
+  uint32_t pix0;       // first pixel of the batch
+  uint32_t n_paths;    // paths in the batch
+};
+
+__device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
+
+// Appends `want` lanes to a queue with one atomic per wavefront; returns the
+// lane's slot.
+__device__ __forceinline__ uint32_t wave_append(bool want, uint32_t *counter) {
+  const uint64_t mask = __ballot(want);
+  if (mask == 0) return 0;
+  const uint32_t leader = (uint32_t)__ffsll((unsigned long long)mask) - 1;
+  uint32_t base = 0;
+  if (lane_id() == leader) base = atomicAdd(counter, (uint32_t)__popcll(mask));
+  base = __shfl(base, leader);
+  const uint32_t below = __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32),
+                                                   __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
+  return base + below;
+}
+
+// ---------------------------------------------------------------------------
+// Kernels.
+// ---------------------------------------------------------------------------
+__global__ void k_reset(uint32_t *counts, uint32_t n) {
+  for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) counts[i] = 0;
+}
+
+// Camera rays: ThinLensCamera::getWorldRay with aperture 0
+// (cameras/ThinLensCamera.cpp:131-135) at the jittered stratum position of
+// SuperSG + JitteredSG (renderers/generators/SuperSampleGenerator.cpp:48-91,
+// JitteredSampleGenerator.cpp:30-31).
+__global__ __launch_bounds__(256) void k_raygen(JobDev job, float4 *__restrict__ q, float4 *__restrict__ Ls,
+                                                float4 *__restrict__ betas, uint32_t *__restrict__ flags,
+                                                uint32_t *__restrict__ count0) {
+  const uint32_t slot = blockIdx.x * blockDim.x + threadIdx.x;
+  if (slot == 0) *count0 = job.n_paths;
+  if (slot >= job.n_paths) return;
+  const uint32_t pixel = job.pix0 + slot / job.spp, sample = slot % job.spp;
+  const mpt_rng::U4 r = mpt_rng::path_draw(job.seed, job.job_id, pixel, sample, mpt_rng::kCameraDepth, 0);
+  const uint32_t k = sample % (job.side * job.side);
+  const uint32_t sx = k % job.side, sy = k / job.side;
+  const uint32_t px = pixel % job.width, py = pixel / job.width;
+  const float inv_side = 1.0f / (float)job.side;
+  const float x = ((float)px + ((float)sx + mpt_rng::unitf(r.x)) * inv_side) / (float)job.width;
+  const float y = ((float)py + ((float)sy + mpt_rng::unitf(r.y)) * inv_side) / (float)job.height;
+  const float nx = (2.0f * x - 1.0f) * job.cam.tan_w, ny = (1.0f - 2.0f * y) * job.cam.tan_h;
+  const float3 d = normalize(f3(job.cam.u[0] * nx + job.cam.v[0] * ny - job.cam.n[0],
+                                job.cam.u[1] * nx + job.cam.v[1] * ny - job.cam.n[1],
+                                job.cam.u[2] * nx + job.cam.v[2] * ny - job.cam.n[2]));
+  q[2 * slot] = make_float4(job.cam.eye[0], job.cam.eye[1], job.cam.eye[2], __uint_as_float(slot));
+  q[2 * slot + 1] = make_float4(d.x, d.y, d.z, __uint_as_float(kMiss));
+  Ls[slot] = make_float4(0, 0, 0, 0);
+  betas[slot] = make_float4(1, 1, 1, 0);
+  flags[slot] = mpt_rng::third(r.z);
+}
+
+// Closest hit of the live queue (kdTreeAccel::getIntersection).
+__global__ __launch_bounds__(256) void k_extend(Scene sc, const float4 *__restrict__ q,
+                                                const uint32_t *__restrict__ count, float2 *__restrict__ hits) {
+  __shared__ uint2 stack[kStack * kBlock];
+  uint2 *stk = stack + threadIdx.x;
+  const uint32_t n = *count;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const float4 a = q[2 * i], b = q[2 * i + 1];
+    const RaySetup r = setup_ray(xyz(a), xyz(b));
+    float t;
+    uint32_t id;
+    trace<false>(sc, r, INFINITY, __float_as_uint(b.w), kMiss, stk, t, id);
+    hits[i] = make_float2(t, __uint_as_float(id));
+  }
+}
+
+// Any hit of the shadow queue (kdTreeAccel::intersects); unoccluded
+// contributions are added to the path's radiance.
+__global__ __launch_bounds__(256) void k_shadow(Scene sc, const float4 *__restrict__ sq,
+                                                const uint32_t *__restrict__ count, float4 *__restrict__ Ls) {
+  __shared__ uint2 stack[kStack * kBlock];
+  uint2 *stk = stack + threadIdx.x;
+  const uint32_t n = *count;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const float4 a = sq[3 * i], b = sq[3 * i + 1], c = sq[3 * i + 2];
+    const RaySetup r = setup_ray(xyz(a), xyz(b));
+    float t;
+    uint32_t id;
+    if (!trace<true>(sc, r, b.w, __float_as_uint(c.w), kMiss, stk, t, id)) {
+      float4 *L = Ls + __float_as_uint(a.w);
+      atomicAdd(&L->x, c.x);
+      atomicAdd(&L->y, c.y);
+      atomicAdd(&L->z, c.z);
+    }
+  }
+}
+
+struct HitInfo {
+  float3 n;
+  uint32_t mat;
+  float area;   // shape surface area (quads; Sphere::_getSurfaceArea for spheres)
+};
+
+// Shading normal and material of a hit: Mesh::_getGeometricNormal with
+// MeshTriangle::getBaryocentricCoords (shapes/Mesh.cpp:419-433,
+// shapes/MeshTriangle.cpp:344-361), Sphere / Plane normals (shapes/Sphere.cpp:
+// 86-92, shapes/Plane.cpp:59-63).  The normal is not flipped toward the ray.
+__device__ __forceinline__ HitInfo hit_info(const Scene &sc, uint32_t prim, float3 p) {
+  HitInfo h;
+  h.area = 0.0f;
+  if (prim < sc.nt) {
+    const uint4 tri = sc.tris[prim];
+    const float3 v1 = xyz(sc.verts[tri.x]), v2 = xyz(sc.verts[tri.y]), v3 = xyz(sc.verts[tri.z]);
+    const float inv_a = 1.0f / len(cross(sub(v2, v1), sub(v3, v1)));
+    const float pA = len(cross(sub(v2, p), sub(v3, p))) * inv_a;
+    const float pB = len(cross(sub(p, v1), sub(v3, v1))) * inv_a;
+    const float pC = 1.0f - pA - pB;
+    const float3 n = add(add(mul(xyz(sc.vnormals[tri.x]), pA), mul(xyz(sc.vnormals[tri.y]), pB)),
+                         mul(xyz(sc.vnormals[tri.z]), pC));
+    h.n = normalize(n);
+    h.mat = tri.w;
+  } else if (prim < sc.nt + sc.ns) {
+    const float4 s = sc.spheres[prim - sc.nt];
+    h.n = normalize(sub(p, xyz(s)));
+    h.mat = sc.sphere_mat[prim - sc.nt];
+    h.area = kPi;
+  } else {
+    const float4 *q = sc.quads + 4ull * (prim - sc.nt - sc.ns);
+    const float4 n = q[3];
+    h.n = xyz(n);
+    h.mat = __float_as_uint(n.w);
+    h.area = q[0].w;
+  }
+  return h;
+}
+
+// One vertex of PathTracer::_evaluate (renderers/renderers/PathTracer.cpp:
+// 28-116) with DirectIllumination::evaluate (renderers/DirectIllumination.cpp:
+// 46-136).  The recursion becomes a throughput `beta`; the emission terms, NEE
+// and roulette follow the reference rules, each cited below.
+__global__ __launch_bounds__(256) void k_shade(Scene sc, JobDev job, uint32_t depth,
+                                               const float4 *__restrict__ qin, const float2 *__restrict__ hits,
+                                               const uint32_t *__restrict__ count_in, float4 *__restrict__ qout,
+                                               uint32_t *__restrict__ count_out, float4 *__restrict__ sq,
+                                               uint32_t *__restrict__ count_shadow, float4 *__restrict__ Ls,
+                                               float4 *__restrict__ betas, uint32_t *__restrict__ flags) {
+  const uint32_t n = *count_in;
+  const uint32_t stride = gridDim.x * blockDim.x;
+  for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += stride) {
+    const uint32_t i = base + threadIdx.x;
+    const bool active = i < n;
+    bool want_ext = false;
+    float3 p = f3(0, 0, 0), wo = f3(0, 0, 0);
+    uint32_t slot = 0, prim = kMiss;
+    float3 beta = f3(0, 0, 0);
+    uint32_t fl = 0;
+    float4 Lv = make_float4(0, 0, 0, 0);
+    HitInfo hi;
+    hi.n = f3(0, 1, 0);
+    hi.mat = 0;
+    hi.area = 0;
+    bool surface = false;
+    float thit = 0.0f;
+    float3 d = f3(0, 0, 1);
+    uint32_t pixel = 0, sample = 0;
+    if (active) {
+      const float4 a = qin[2 * i], b = qin[2 * i + 1];
+      const float2 h = hits[i];
+      slot = __float_as_uint(a.w);
+      d = xyz(b);
+      pixel = job.pix0 + slot / job.spp;
+      sample = slot % job.spp;
+      const float4 bv = betas[slot];
+      beta = xyz(bv);
+      fl = flags[slot];
+      Lv = Ls[slot];
+      prim = __float_as_uint(h.y);
+      if (prim == kMiss) {
+        // Scene::getBackgroundRadiance (core/Scene.cpp:116-118), every depth.
+        Lv.x += beta.x * sc.env[0];
+        Lv.y += beta.y * sc.env[1];
+        Lv.z += beta.z * sc.env[2];
+      } else {
+        surface = true;
+        thit = h.x;
+        p = add(xyz(a), mul(d, h.x));
+        hi = hit_info(sc, prim, p);
+      }
+    }
+    mpt_material m;
+    if (surface) m = sc.mats[hi.mat];
+    const bool specular = surface && m.type == MPT_BSDF_DIELECTRIC;
+    if (surface) {
+      if (specular) fl |= F_ANY_SPECULAR;
+      // PathTracer.cpp:50-69: emission at depth 0, or after a specular vertex
+      // that sent the ray out on its normal's side (Le * area / t^2).
+      if (emissive(m) && dot(hi.n, d) < 0.0f) {
+        const float3 le = le_of(m);
+        float w = 0.0f;
+        if (depth == 0) {
+          w = 1.0f;
+        } else if (fl & F_EMIT_AFTER_SPEC) {
+          w = hi.area / (thit * thit);
+        }
+        Lv.x += beta.x * le.x * w;
+        Lv.y += beta.y * le.y * w;
+        Lv.z += beta.z * le.z * w;
+      }
+    }
+    // Next-event estimation: one shadow ray per light (DirectIllumination.cpp:
+    // 90-135), skipped at specular vertices (:57) and on the light itself (:93).
+    for (uint32_t li = 0; li < sc.num_lights; ++li) {
+      bool want_sh = false;
+      float3 so = p, sd = f3(0, 0, 1), contrib = f3(0, 0, 0);
+      float sdist = 0.0f;
+      if (surface && !specular) {
+        const uint32_t q = sc.lights[li];
+        const uint32_t lgid = sc.nt + sc.ns + q;
+        if (lgid != prim) {
+          const float4 *lq = sc.quads + 4ull * q;
+          const float4 c0 = lq[0], ex = lq[1], ez = lq[2], ln = lq[3];
+          const mpt_rng::U4 r = mpt_rng::path_draw(job.seed, job.job_id, pixel, sample, depth, 1 + li);
+          const float u = mpt_rng::unitf(r.x), v = mpt_rng::unitf(r.y);
+          // Plane::getPosition (shapes/Plane.cpp:91-93)
+          const float3 lp = add(xyz(c0), add(mul(xyz(ex), u - 0.5f), mul(xyz(ez), v - 0.5f)));
+          float3 w = sub(lp, p);
+          const float dist = len(w);
+          w = mul(w, 1.0f / dist);
+          const float cos_wo = fabsf(dot(hi.n, w));
+          const float cos_wi = fabsf(dot(xyz(ln), w));
+          const float3 fr = bsdf_eval(m, hi.n, d, w);
+          // OrientedEmitter::getLe (emitters/OrientedEmitter.cpp:17-23)
+          const mpt_material lm = sc.mats[__float_as_uint(ln.w)];
+          const bool lit = dot(xyz(ln), w) < 0.0f;
+          if (cos_wo > 0.0f && cos_wi > 0.0f && !is_black(fr) && lit) {
+            const float g = cos_wo * cos_wi / (dist * dist) * c0.w;
+            contrib = mul(mul(beta, mul(fr, le_of(lm))), g);
+            if (!is_black(contrib)) {
+              want_sh = true;
+              sd = w;
+              sdist = dist * (1.0f - 1e-5f);
+            }
+          }
+        }
+      }
+      const uint32_t si = wave_append(want_sh, count_shadow);
+      if (want_sh) {
+        sq[3 * si] = make_float4(so.x, so.y, so.z, __uint_as_float(slot));
+        sq[3 * si + 1] = make_float4(sd.x, sd.y, sd.z, sdist);
+        sq[3 * si + 2] = make_float4(contrib.x, contrib.y, contrib.z, __uint_as_float(prim));
+      }
+    }
+    if (surface) {
+      const mpt_rng::U4 r = mpt_rng::path_draw(job.seed, job.job_id, pixel, sample, depth, 0);
+      const Sample s = bsdf_sample(m, hi.n, d, mpt_rng::unitf(r.x), mpt_rng::unitf(r.y), mpt_rng::unitf(r.w),
+                                   fl & F_IOR_MASK);
+      if (!is_black(s.wo)) {
+        // Russian roulette (PathTracer.cpp:88-92).
+        const float p_cont = depth < 2 ? (is_black(s.fs) ? 0.0f : 1.0f) : fminf(0.95f, max3(s.fs));
+        if (mpt_rng::unitf(r.z) < p_cont && depth + 1 < job.max_depth) {
+          want_ext = true;
+          wo = s.wo;
+          if (depth == 0) fl |= F_D0_CONTINUED;
+          fl = (specular && dot(hi.n, s.wo) > kEps) ? (fl | F_EMIT_AFTER_SPEC) : (fl & ~F_EMIT_AFTER_SPEC);
+          beta = mul(beta, mul(s.fs, 1.0f / p_cont));
+        }
+      }
+    }
+    const uint32_t ei = wave_append(want_ext, count_out);
+    if (want_ext) {
+      qout[2 * ei] = make_float4(p.x, p.y, p.z, __uint_as_float(slot));
+      qout[2 * ei + 1] = make_float4(wo.x, wo.y, wo.z, __uint_as_float(prim));
+      betas[slot] = make_float4(beta.x, beta.y, beta.z, 0.0f);
+    }
+    if (active) {
+      Ls[slot] = Lv;
+      flags[slot] = fl;
+    }
+  }
+}
+
+// PathTracer.cpp:107-114: a path with a specular vertex whose depth-0 vertex
+// recursed keeps only its IOR channel, times the channel count.
+__global__ __launch_bounds__(256) void k_finalize(float4 *__restrict__ Ls, const uint32_t *__restrict__ flags,
+                                                  uint32_t n) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t f = flags[i];
+  if ((f & F_ANY_SPECULAR) && (f & F_D0_CONTINUED)) {
+    float4 L = Ls[i];
+    const uint32_t c = f & F_IOR_MASK;
+    L = make_float4(c == 0 ? 3.0f * L.x : 0.0f, c == 1 ? 3.0f * L.y : 0.0f, c == 2 ? 3.0f * L.z : 0.0f, 0.0f);
+    Ls[i] = L;
+  }
+}
+
+// Film: ProgressiveFilterValue numerator/denominator per pixel
+// (filters/ProgressiveFilterValue.h:20-37) with the pixel-bin weight 1 of
+// RenderOutput::addSample (renderers/RenderOutput.cpp:72-76).
+__global__ __launch_bounds__(256) void k_film(const float4 *__restrict__ Ls, uint32_t pix0, uint32_t npix,
+                                              uint32_t spp, float4 *__restrict__ film) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= npix) return;
+  const float4 *L = Ls + (uint64_t)i * spp;
+  float r = 0, g = 0, b = 0;
+  for (uint32_t s = 0; s < spp; ++s) {
+    const float4 v = L[s];
+    r += v.x;
+    g += v.y;
+    b += v.z;
+  }
+  film[pix0 + i] = make_float4(r, g, b, (float)spp);
+}
+
+__global__ void k_totals(const uint32_t *__restrict__ counts, uint32_t depth, unsigned long long *totals) {
+  if (threadIdx.x != 0) return;
+  unsigned long long ext = 0, sh = 0;
+  for (uint32_t d = 0; d < depth; ++d) {
+    ext += counts[d];
+    sh += counts[depth + 1 + d];
+  }
+  totals[0] += counts[0];
+  totals[1] += ext;
+  totals[2] += sh;
+}
+
+template <bool ANY>
+__global__ __launch_bounds__(256) void k_trace_rays(Scene sc, const mpt_ray *__restrict__ rays,
+                                                    const float *__restrict__ tmax, uint64_t n,
+                                                    mpt_hit *__restrict__ hits, uint8_t *__restrict__ occ) {
+  __shared__ uint2 stack[kStack * kBlock];
+  uint2 *stk = stack + threadIdx.x;
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
+    const mpt_ray ry = rays[i];
+    const RaySetup r = setup_ray(f3(ry.o[0], ry.o[1], ry.o[2]), f3(ry.d[0], ry.d[1], ry.d[2]));
+    float t;
+    uint32_t id;
+    if (ANY) {
+      occ[i] = trace<true>(sc, r, tmax[i], kMiss, kMiss, stk, t, id) ? 1 : 0;
+    } else {
+      trace<false>(sc, r, INFINITY, kMiss, kMiss, stk, t, id);
+      hits[i].t = t;
+      hits[i].prim = id;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Host side.
+// ---------------------------------------------------------------------------
+template <class T>
+struct DBuf {
+  T *p = nullptr;
+  size_t n = 0;
+  DBuf() = default;
+  DBuf(const DBuf &) = delete;
+  DBuf &operator=(const DBuf &) = delete;
+  ~DBuf() { release(); }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+  hipError_t reserve(size_t count) {
+    if (count <= n && p) return hipSuccess;
+    release();
+    const hipError_t e = hipMalloc((void **)&p, std::max<size_t>(count, 1) * sizeof(T));
+    if (e == hipSuccess) n = count;
+    else p = nullptr;
+    return e;
+  }
+  size_t bytes() const { return n * sizeof(T); }
+};
+
+struct Workspace {
+  uint32_t cap = 0, depth = 0, lights = 0;
+  DBuf<float4> q[2];
+  DBuf<float2> hits;
+  DBuf<float4> sq;
+  DBuf<float4> Ls, betas;
+  DBuf<uint32_t> flags;
+  DBuf<uint32_t> counts;
+  DBuf<unsigned long long> totals;
+};
+
+}  // namespace
+
+struct mpt_context {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  int num_cus = 256;
+  std::string err;
+  Workspace ws;
+  uint32_t batch_paths = 1u << 23;   // paths per batch (queue depth)
+};
+
+struct mpt_scene {
+  mpt_context *ctx = nullptr;
+  DBuf<uint2> nodes;
+  DBuf<float4> recs, verts, vnormals, spheres, quads;
+  DBuf<uint4> tris;
+  DBuf<uint32_t> sphere_mat, lights;
+  DBuf<mpt_material> mats;
+  Scene dev{};
+  uint64_t stats[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+};
+
+struct mpt_film {
+  mpt_context *ctx = nullptr;
+  uint32_t width = 0, height = 0;
+  DBuf<float4> data;
+};
+
+namespace {
+
+bool emissive_host(const mpt_material &m) { return m.Le[0] > 0 || m.Le[1] > 0 || m.Le[2] > 0; }
+
+int fail(mpt_context *ctx, int code, const std::string &msg) {
+  if (ctx) ctx->err = msg;
+  return code;
+}
+int hip_fail(mpt_context *ctx, hipError_t e, const char *what) {
+  return fail(ctx, MPT_E_DEVICE, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+#define MPT_HIP(ctx, call)                                  \
+  do {                                                      \
+    const hipError_t e_ = (call);                           \
+    if (e_ != hipSuccess) return hip_fail((ctx), e_, #call); \
+  } while (0)
+
+#define MPT_LAUNCHED(ctx, what)                                  \
+  do {                                                           \
+    const hipError_t e_ = hipGetLastError();                     \
+    if (e_ != hipSuccess) return hip_fail((ctx), e_, (what));    \
+  } while (0)
+
+template <class T>
+int upload(mpt_context *ctx, DBuf<T> &buf, const T *host, size_t count) {
+  MPT_HIP(ctx, buf.reserve(count));
+  if (count) MPT_HIP(ctx, hipMemcpyAsync(buf.p, host, count * sizeof(T), hipMemcpyHostToDevice, ctx->stream));
+  return MPT_OK;
+}
+
+void normalize3(double *v) {
+  const double l = std::sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+  v[0] /= l;
+  v[1] /= l;
+  v[2] /= l;
+}
+
+// ThinLensCamera::setOrientation + _computeMatrix (cameras/ThinLensCamera.cpp:
+// 181-256): n = -look, u = up x n, v = n x u, film scale tan(heightAngle/2).
+CameraDev make_camera(const mpt_camera &c) {
+  double n[3] = {-c.look[0], -c.look[1], -c.look[2]};
+  double u[3] = {c.up[1] * n[2] - c.up[2] * n[1], c.up[2] * n[0] - c.up[0] * n[2], c.up[0] * n[1] - c.up[1] * n[0]};
+  normalize3(n);
+  normalize3(u);
+  const double v[3] = {n[1] * u[2] - n[2] * u[1], n[2] * u[0] - n[0] * u[2], n[0] * u[1] - n[1] * u[0]};
+  CameraDev d;
+  for (int k = 0; k < 3; ++k) {
+    d.eye[k] = c.eye[k];
+    d.u[k] = (float)u[k];
+    d.v[k] = (float)v[k];
+    d.n[k] = (float)n[k];
+  }
+  const double th = std::tan(c.height_angle * M_PI / 360.0);
+  d.tan_h = (float)th;
+  d.tan_w = (float)(c.aspect * th);
+  return d;
+}
+
+int check_job(mpt_context *ctx, const mpt_job *job) {
+  if (!job) return fail(ctx, MPT_E_INVALID, "job is NULL");
+  if (job->width == 0 || job->height == 0 || job->spp == 0 || job->max_depth == 0)
+    return fail(ctx, MPT_E_INVALID, "job needs width, height, spp and max_depth > 0");
+  if (job->max_depth > 64) return fail(ctx, MPT_E_INVALID, "max_depth above 64");
+  if ((uint64_t)job->width * job->height > (1ull << 30)) return fail(ctx, MPT_E_INVALID, "film too large");
+  if (job->spp > (1u << 20)) return fail(ctx, MPT_E_INVALID, "spp above 2^20");
+  const float *l = job->camera.look, *u = job->camera.up;
+  const double ll = l[0] * l[0] + l[1] * l[1] + l[2] * l[2];
+  const double cx = u[1] * l[2] - u[2] * l[1], cy = u[2] * l[0] - u[0] * l[2], cz = u[0] * l[1] - u[1] * l[0];
+  if (!(ll > 0) || !(cx * cx + cy * cy + cz * cz > 0))
+    return fail(ctx, MPT_E_INVALID, "camera look must be non-zero and not collinear with up");
+  if (!(job->camera.height_angle > 0 && job->camera.height_angle < 180) || !(job->camera.aspect > 0))
+    return fail(ctx, MPT_E_INVALID, "camera height_angle must be in (0, 180) and aspect > 0");
+  return MPT_OK;
+}
+
+int ensure_workspace(mpt_context *ctx, uint32_t cap, uint32_t depth, uint32_t lights) {
+  Workspace &w = ctx->ws;
+  const uint64_t ns = (uint64_t)cap * std::max<uint32_t>(lights, 1);
+  if (cap > w.cap || lights > w.lights) {
+    MPT_HIP(ctx, w.q[0].reserve(2ull * cap));
+    MPT_HIP(ctx, w.q[1].reserve(2ull * cap));
+    MPT_HIP(ctx, w.hits.reserve(cap));
+    MPT_HIP(ctx, w.Ls.reserve(cap));
+    MPT_HIP(ctx, w.betas.reserve(cap));
+    MPT_HIP(ctx, w.flags.reserve(cap));
+    MPT_HIP(ctx, w.sq.reserve(3ull * ns));
+    w.cap = std::max(w.cap, cap);
+    w.lights = std::max(w.lights, lights);
+  }
+  if (depth > w.depth || !w.counts.p) {
+    MPT_HIP(ctx, w.counts.reserve(2ull * depth + 2));
+    w.depth = std::max(w.depth, depth);
+  }
+  MPT_HIP(ctx, w.totals.reserve(4));
+  return MPT_OK;
+}
+
+uint32_t grid_for(const mpt_context *ctx, uint64_t n, uint32_t per_cu) {
+  const uint64_t blocks = (n + kBlock - 1) / kBlock;
+  return (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(blocks, (uint64_t)ctx->num_cus * per_cu));
+}
+
+// Renders pixels [pix_begin, pix_end) of a job.  film (device, W*H float4)
+// and/or L_host (per-path radiance) receive the results.
+int render_range(mpt_scene *s, const mpt_job *job, uint32_t pix_begin, uint32_t pix_end, float4 *film,
+                 float *L_host, mpt_render_stats *stats) {
+  mpt_context *ctx = s->ctx;
+  int rc = check_job(ctx, job);
+  if (rc) return rc;
+  const uint32_t spp = job->spp, D = job->max_depth;
+  const uint32_t L = s->dev.num_lights;
+  const uint32_t batch_pix = std::max<uint32_t>(1, ctx->batch_paths / spp);
+  const uint32_t cap = batch_pix * spp;
+  rc = ensure_workspace(ctx, cap, D, L);
+  if (rc) return rc;
+  Workspace &w = ctx->ws;
+  JobDev jd;
+  jd.cam = make_camera(job->camera);
+  jd.width = job->width;
+  jd.height = job->height;
+  jd.spp = spp;
+  jd.max_depth = D;
+  jd.seed = job->seed;
+  jd.job_id = job->job_id;
+  jd.side = std::max<uint32_t>(1, (uint32_t)std::sqrt((double)spp));
+  while ((uint64_t)(jd.side + 1) * (jd.side + 1) <= spp) ++jd.side;
+  while ((uint64_t)jd.side * jd.side > spp) --jd.side;
+  MPT_HIP(ctx, hipMemsetAsync(w.totals.p, 0, 4 * sizeof(unsigned long long), ctx->stream));
+  MPT_HIP(ctx, hipEventRecord(ctx->ev0, ctx->stream));
+  const uint32_t counts_n = 2 * D + 2;
+  uint64_t launches = 0;
+  for (uint32_t p0 = pix_begin; p0 < pix_end; p0 += batch_pix) {
+    const uint32_t np = std::min(batch_pix, pix_end - p0);
+    jd.pix0 = p0;
+    jd.n_paths = np * spp;
+    const uint32_t g = grid_for(ctx, jd.n_paths, 8);
+    k_reset<<<1, 64, 0, ctx->stream>>>(w.counts.p, counts_n);
+    k_raygen<<<(jd.n_paths + 255) / 256, 256, 0, ctx->stream>>>(jd, w.q[0].p, w.Ls.p, w.betas.p, w.flags.p,
+                                                                  w.counts.p);
+    MPT_LAUNCHED(ctx, "k_raygen");
+    for (uint32_t d = 0; d < D; ++d) {
+      float4 *qin = w.q[d & 1].p, *qout = w.q[(d + 1) & 1].p;
+      k_extend<<<g, kBlock, 0, ctx->stream>>>(s->dev, qin, w.counts.p + d, w.hits.p);
+      k_shade<<<g, kBlock, 0, ctx->stream>>>(s->dev, jd, d, qin, w.hits.p, w.counts.p + d, qout,
+                                             w.counts.p + d + 1, w.sq.p, w.counts.p + D + 1 + d, w.Ls.p,
+                                             w.betas.p, w.flags.p);
+      if (L) k_shadow<<<g, kBlock, 0, ctx->stream>>>(s->dev, w.sq.p, w.counts.p + D + 1 + d, w.Ls.p);
+      MPT_LAUNCHED(ctx, "depth kernels");
+      ++launches;
+    }
+    k_finalize<<<(jd.n_paths + 255) / 256, 256, 0, ctx->stream>>>(w.Ls.p, w.flags.p, jd.n_paths);
+    if (film) k_film<<<(np + 255) / 256, 256, 0, ctx->stream>>>(w.Ls.p, p0, np, spp, film);
+    k_totals<<<1, 64, 0, ctx->stream>>>(w.counts.p, D, w.totals.p);
+    MPT_LAUNCHED(ctx, "film kernels");
+    if (L_host) {
+      MPT_HIP(ctx, hipMemcpy2DAsync(L_host + 3ull * (uint64_t)(p0 - pix_begin) * spp, 3 * sizeof(float), w.Ls.p,
+                                    sizeof(float4), 3 * sizeof(float), jd.n_paths, hipMemcpyDeviceToHost,
+                                    ctx->stream));
+    }
+  }
+  MPT_HIP(ctx, hipEventRecord(ctx->ev1, ctx->stream));
+  unsigned long long tot[4] = {0, 0, 0, 0};
+  MPT_HIP(ctx, hipMemcpyAsync(tot, w.totals.p, sizeof(tot), hipMemcpyDeviceToHost, ctx->stream));
+  MPT_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  if (stats) {
+    float ms = 0;
+    MPT_HIP(ctx, hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+    std::memset(stats, 0, sizeof(*stats));
+    stats->paths = (uint64_t)(pix_end - pix_begin) * spp;
+    stats->camera_rays = tot[0];
+    stats->extension_rays = tot[1];
+    stats->shadow_rays = tot[2];
+    stats->render_ms = ms;
+    stats->extend_launches = launches;
+  }
+  return MPT_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int mpt_abi_version(void) { return (int)MPT_ABI_VERSION; }
+
+int mpt_open(int device, mpt_context **out) {
+  if (!out) return MPT_E_INVALID;
+  *out = nullptr;
+  std::unique_ptr<mpt_context> ctx(new (std::nothrow) mpt_context());
+  if (!ctx) return MPT_E_NOMEM;
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || device < 0 || device >= n) return MPT_E_DEVICE;
+  if (hipSetDevice(device) != hipSuccess) return MPT_E_DEVICE;
+  ctx->device = device;
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, device) == hipSuccess && prop.multiProcessorCount > 0)
+    ctx->num_cus = prop.multiProcessorCount;
+  if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) return MPT_E_DEVICE;
+  if (hipEventCreate(&ctx->ev0) != hipSuccess || hipEventCreate(&ctx->ev1) != hipSuccess) return MPT_E_DEVICE;
+  *out = ctx.release();
+  return MPT_OK;
+}
+
+int mpt_close(mpt_context *ctx) {
+  if (!ctx) return MPT_E_INVALID;
+  (void)hipSetDevice(ctx->device);
+  (void)hipStreamSynchronize(ctx->stream);
+
+  if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
+  if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
+  if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+  delete ctx;
+  return MPT_OK;
+}
+
+const char *mpt_last_error(const mpt_context *ctx) { return ctx ? ctx->err.c_str() : "no context"; }
+
+int mpt_sync(mpt_context *ctx) {
+  if (!ctx) return MPT_E_INVALID;
+  MPT_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return MPT_OK;
+}
+
+int mpt_scene_create(mpt_context *ctx, const mpt_scene_desc *d, const mpt_kd_params *kd, mpt_scene **out) {
+  if (!ctx || !d || !out) return fail(ctx, MPT_E_INVALID, "NULL argument");
+  *out = nullptr;
+  if (d->version != MPT_ABI_VERSION) return fail(ctx, MPT_E_INVALID, "scene desc version mismatch");
+  const uint32_t nv = d->num_vertices, nt = d->num_triangles, ns = d->num_spheres, nq = d->num_quads;
+  const uint32_t nm = d->num_materials;
+  if ((uint64_t)nt + ns + nq >= 0x3FFFFFFFull) return fail(ctx, MPT_E_INVALID, "too many primitives");
+  if ((nt && (!d->indices || !d->tri_material || !d->vertices)) || (ns && (!d->spheres || !d->sphere_material)) ||
+      (nq && (!d->quads || !d->quad_material)) || (nm && !d->materials))
+    return fail(ctx, MPT_E_INVALID, "missing scene array");
+  for (uint64_t i = 0; i < 3ull * nt; ++i)
+    if (d->indices[i] >= nv) return fail(ctx, MPT_E_INVALID, "triangle index out of range");
+  auto bad_mat = [&](uint32_t m) { return m >= nm; };
+  for (uint32_t i = 0; i < nt; ++i)
+    if (bad_mat(d->tri_material[i]) || emissive_host(d->materials[d->tri_material[i]]))
+      return fail(ctx, MPT_E_INVALID, "triangle material out of range or emissive (only quads may emit)");
+  for (uint32_t i = 0; i < ns; ++i)
+    if (bad_mat(d->sphere_material[i]) || emissive_host(d->materials[d->sphere_material[i]]))
+      return fail(ctx, MPT_E_INVALID, "sphere material out of range or emissive (only quads may emit)");
+  for (uint32_t i = 0; i < nq; ++i)
+    if (bad_mat(d->quad_material[i])) return fail(ctx, MPT_E_INVALID, "quad material out of range");
+  for (uint32_t i = 0; i < nm; ++i)
+    if (d->materials[i].type > MPT_BSDF_DIELECTRIC) return fail(ctx, MPT_E_INVALID, "unknown BSDF type");
+  MPT_HIP(ctx, hipSetDevice(ctx->device));
+  std::unique_ptr<mpt_scene> s(new (std::nothrow) mpt_scene());
+  if (!s) return fail(ctx, MPT_E_NOMEM, "host allocation");
+  s->ctx = ctx;
+
+  // Host copies in the device layouts.
+  std::vector<float4> verts(nv), vnorm(nv);
+  for (uint32_t i = 0; i < nv; ++i) verts[i] = make_float4(d->vertices[3 * i], d->vertices[3 * i + 1], d->vertices[3 * i + 2], 0);
+  {
+    // Mesh::computeNormals (shapes/Mesh.cpp:335-377) in double.
+    std::vector<double> acc(3ull * nv, 0.0);
+    for (uint32_t t = 0; t < nt; ++t) {
+      const uint32_t *ix = d->indices + 3ull * t;
+      const float *a = d->vertices + 3ull * ix[0], *b = d->vertices + 3ull * ix[1], *c = d->vertices + 3ull * ix[2];
+      const double e1[3] = {(double)b[0] - a[0], (double)b[1] - a[1], (double)b[2] - a[2]};
+      const double e2[3] = {(double)c[0] - a[0], (double)c[1] - a[1], (double)c[2] - a[2]};
+      double n[3] = {e1[1] * e2[2] - e1[2] * e2[1], e1[2] * e2[0] - e1[0] * e2[2], e1[0] * e2[1] - e1[1] * e2[0]};
+      const double l = std::sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+      if (l > 0)
+        for (int k = 0; k < 3; ++k) n[k] /= l;
+      for (int j = 0; j < 3; ++j)
+        for (int k = 0; k < 3; ++k) acc[3ull * ix[j] + k] += n[k];
+    }
+    for (uint32_t i = 0; i < nv; ++i) {
+      double *n = &acc[3ull * i];
+      const double l = std::sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+      const double k = l > 0 ? 1.0 / l : 0.0;
+      vnorm[i] = make_float4((float)(n[0] * k), (float)(n[1] * k), (float)(n[2] * k), 0);
+    }
+  }
+  std::vector<uint4> tris(nt);
+  for (uint32_t t = 0; t < nt; ++t)
+    tris[t] = make_uint4(d->indices[3 * t], d->indices[3 * t + 1], d->indices[3 * t + 2], d->tri_material[t]);
+  std::vector<float4> sph(ns);
+  for (uint32_t i = 0; i < ns; ++i)
+    sph[i] = make_float4(d->spheres[4 * i], d->spheres[4 * i + 1], d->spheres[4 * i + 2], d->spheres[4 * i + 3]);
+  std::vector<float4> quads(4ull * nq);
+  std::vector<uint32_t> lights;
+  for (uint32_t i = 0; i < nq; ++i) {
+    const float *q = d->quads + 9ull * i;
+    const double ex[3] = {q[3], q[4], q[5]}, ez[3] = {q[6], q[7], q[8]};
+    double n[3] = {ez[1] * ex[2] - ez[2] * ex[1], ez[2] * ex[0] - ez[0] * ex[2], ez[0] * ex[1] - ez[1] * ex[0]};
+    const double area = std::sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+    if (!(area > 0)) return fail(ctx, MPT_E_INVALID, "degenerate quad");
+    normalize3(n);
+    quads[4 * i] = make_float4(q[0], q[1], q[2], (float)area);
+    quads[4 * i + 1] = make_float4(q[3], q[4], q[5], 0);
+    quads[4 * i + 2] = make_float4(q[6], q[7], q[8], 0);
+    uint32_t mbits = d->quad_material[i];
+    float mf;
+    std::memcpy(&mf, &mbits, 4);
+    quads[4 * i + 3] = make_float4((float)n[0], (float)n[1], (float)n[2], mf);
+    if (emissive_host(d->materials[d->quad_material[i]])) lights.push_back(i);
+  }
+
+  // kd-tree over all primitives.
+  std::vector<mpt_kd::Box> boxes((size_t)nt + ns + nq);
+  for (uint32_t t = 0; t < nt; ++t) {
+    mpt_kd::Box &b = boxes[t];
+    for (int k = 0; k < 3; ++k) {
+      const float v0 = (&verts[tris[t].x].x)[k], v1 = (&verts[tris[t].y].x)[k], v2 = (&verts[tris[t].z].x)[k];
+      b.lo[k] = std::min(v0, std::min(v1, v2));
+      b.hi[k] = std::max(v0, std::max(v1, v2));
+    }
+  }
+  for (uint32_t i = 0; i < ns; ++i) {
+    mpt_kd::Box &b = boxes[(size_t)nt + i];
+    const float c[3] = {sph[i].x, sph[i].y, sph[i].z};
+    for (int k = 0; k < 3; ++k) {
+      b.lo[k] = std::nextafter(c[k] - sph[i].w, -INFINITY);
+      b.hi[k] = std::nextafter(c[k] + sph[i].w, INFINITY);
+    }
+  }
+  for (uint32_t i = 0; i < nq; ++i) {
+    mpt_kd::Box &b = boxes[(size_t)nt + ns + i];
+    const float4 c = quads[4 * i], ex = quads[4 * i + 1], ez = quads[4 * i + 2];
+    for (int k = 0; k < 3; ++k) {
+      const float cc = (&c.x)[k], a = 0.5f * std::fabs((&ex.x)[k]), e = 0.5f * std::fabs((&ez.x)[k]);
+      b.lo[k] = std::nextafter(cc - a - e, -INFINITY);
+      b.hi[k] = std::nextafter(cc + a + e, INFINITY);
+    }
+  }
+  mpt_kd::Params kp;
+  if (kd) {
+    if (kd->leaf_size) kp.leaf_size = kd->leaf_size;
+    if (kd->max_depth) kp.max_depth = kd->max_depth;
+    if (kd->cost_traversal > 0) kp.cost_traversal = kd->cost_traversal;
+    if (kd->cost_intersect > 0) kp.cost_intersect = kd->cost_intersect;
+    if (kd->empty_bias > 0) kp.empty_bias = kd->empty_bias;
+    kp.num_threads = kd->num_threads;
+    if (kd->bins) kp.bins = kd->bins;
+  }
+  mpt_kd::Output ko;
+  auto write = [&](uint32_t prim, float *r) {
+    uint32_t kind, id = prim;
+    std::memset(r, 0, 12 * sizeof(float));
+    if (prim < nt) {
+      kind = mpt_kd::KIND_TRIANGLE;
+      const float4 a = verts[tris[prim].x], b = verts[tris[prim].y], c = verts[tris[prim].z];
+      r[0] = a.x; r[1] = a.y; r[2] = a.z;
+      r[4] = b.x; r[5] = b.y; r[6] = b.z;
+      r[8] = c.x; r[9] = c.y; r[10] = c.z;
+    } else if (prim < nt + ns) {
+      kind = mpt_kd::KIND_SPHERE;
+      const float4 sp = sph[prim - nt];
+      r[0] = sp.x; r[1] = sp.y; r[2] = sp.z;
+      r[4] = sp.w;
+    } else {
+      kind = mpt_kd::KIND_QUAD;
+      const float4 *q = &quads[4ull * (prim - nt - ns)];
+      r[0] = q[0].x; r[1] = q[0].y; r[2] = q[0].z;
+      r[4] = q[1].x; r[5] = q[1].y; r[6] = q[1].z;
+      r[8] = q[2].x; r[9] = q[2].y; r[10] = q[2].z;
+    }
+    std::memcpy(&r[3], &id, 4);
+    std::memcpy(&r[7], &kind, 4);
+  };
+  try {
+    mpt_kd::build(boxes, kp, write, ko);
+  } catch (const std::bad_alloc &) {
+    return fail(ctx, MPT_E_NOMEM, "kd build: host allocation");
+  }
+  int rc;
+  if ((rc = upload(ctx, s->nodes, reinterpret_cast<const uint2 *>(ko.nodes.data()), ko.nodes.size() / 2))) return rc;
+  if ((rc = upload(ctx, s->recs, reinterpret_cast<const float4 *>(ko.records.data()), ko.records.size() / 4))) return rc;
+  if ((rc = upload(ctx, s->tris, tris.data(), tris.size()))) return rc;
+  if ((rc = upload(ctx, s->verts, verts.data(), verts.size()))) return rc;
+  if ((rc = upload(ctx, s->vnormals, vnorm.data(), vnorm.size()))) return rc;
+  if ((rc = upload(ctx, s->spheres, sph.data(), sph.size()))) return rc;
+  if ((rc = upload(ctx, s->sphere_mat, d->sphere_material, ns))) return rc;
+  if ((rc = upload(ctx, s->quads, quads.data(), quads.size()))) return rc;
+  if ((rc = upload(ctx, s->mats, d->materials, nm))) return rc;
+  if ((rc = upload(ctx, s->lights, lights.data(), lights.size()))) return rc;
+  MPT_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  Scene &sc = s->dev;
+  sc.nodes = s->nodes.p;
+  sc.recs = s->recs.p;
+  for (int k = 0; k < 3; ++k) {
+    sc.box_lo[k] = ko.box.lo[k];
+    sc.box_hi[k] = ko.box.hi[k];
+  }
+  sc.tris = s->tris.p;
+  sc.verts = s->verts.p;
+  sc.vnormals = s->vnormals.p;
+  sc.spheres = s->spheres.p;
+  sc.sphere_mat = s->sphere_mat.p;
+  sc.quads = s->quads.p;
+  sc.mats = s->mats.p;
+  sc.lights = s->lights.p;
+  sc.num_lights = (uint32_t)lights.size();
+  sc.nt = nt;
+  sc.ns = ns;
+  sc.nq = nq;
+  for (int k = 0; k < 3; ++k) sc.env[k] = d->env_radiance[k];
+  s->stats[0] = ko.n_nodes;
+  s->stats[1] = ko.n_leaves;
+  s->stats[2] = ko.n_refs;
+  s->stats[3] = ko.max_depth;
+  s->stats[4] = s->nodes.bytes() + s->recs.bytes() + s->tris.bytes() + s->verts.bytes() + s->vnormals.bytes() +
+                s->spheres.bytes() + s->quads.bytes() + s->mats.bytes();
+  s->stats[5] = (uint64_t)ko.build_ms;
+  s->stats[6] = ko.max_leaf;
+  s->stats[7] = nt;
+  *out = s.release();
+  return MPT_OK;
+}
+
+int mpt_scene_destroy(mpt_scene *s) {
+  if (!s) return MPT_E_INVALID;
+  (void)hipSetDevice(s->ctx->device);
+  (void)hipStreamSynchronize(s->ctx->stream);
+  delete s;
+  return MPT_OK;
+}
+
+int mpt_scene_stats(const mpt_scene *s, uint64_t out[8]) {
+  if (!s || !out) return MPT_E_INVALID;
+  std::memcpy(out, s->stats, sizeof(s->stats));
+  return MPT_OK;
+}
+
+int mpt_trace_closest(mpt_scene *s, const mpt_ray *rays, uint64_t n, mpt_hit *hits) {
+  if (!s || (n && (!rays || !hits))) return fail(s ? s->ctx : nullptr, MPT_E_INVALID, "NULL argument");
+  mpt_context *ctx = s->ctx;
+  if (!n) return MPT_OK;
+  DBuf<mpt_ray> dr;
+  DBuf<mpt_hit> dh;
+  MPT_HIP(ctx, dr.reserve(n));
+  MPT_HIP(ctx, dh.reserve(n));
+  MPT_HIP(ctx, hipMemcpyAsync(dr.p, rays, n * sizeof(mpt_ray), hipMemcpyHostToDevice, ctx->stream));
+  k_trace_rays<false><<<grid_for(ctx, n, 8), kBlock, 0, ctx->stream>>>(s->dev, dr.p, nullptr, n, dh.p, nullptr);
+  MPT_LAUNCHED(ctx, "k_trace_rays");
+  MPT_HIP(ctx, hipMemcpyAsync(hits, dh.p, n * sizeof(mpt_hit), hipMemcpyDeviceToHost, ctx->stream));
+  MPT_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return MPT_OK;
+}
+
+int mpt_trace_any(mpt_scene *s, const mpt_ray *rays, const float *tmax, uint64_t n, uint8_t *occluded) {
+  if (!s || (n && (!rays || !tmax || !occluded))) return fail(s ? s->ctx : nullptr, MPT_E_INVALID, "NULL argument");
+  mpt_context *ctx = s->ctx;
+  if (!n) return MPT_OK;
+  DBuf<mpt_ray> dr;
+  DBuf<float> dt;
+  DBuf<uint8_t> dout;
+  MPT_HIP(ctx, dr.reserve(n));
+  MPT_HIP(ctx, dt.reserve(n));
+  MPT_HIP(ctx, dout.reserve(n));
+  MPT_HIP(ctx, hipMemcpyAsync(dr.p, rays, n * sizeof(mpt_ray), hipMemcpyHostToDevice, ctx->stream));
+  MPT_HIP(ctx, hipMemcpyAsync(dt.p, tmax, n * sizeof(float), hipMemcpyHostToDevice, ctx->stream));
+  k_trace_rays<true><<<grid_for(ctx, n, 8), kBlock, 0, ctx->stream>>>(s->dev, dr.p, dt.p, n, nullptr, dout.p);
+  MPT_LAUNCHED(ctx, "k_trace_rays");
+  MPT_HIP(ctx, hipMemcpyAsync(occluded, dout.p, n, hipMemcpyDeviceToHost, ctx->stream));
+  MPT_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return MPT_OK;
+}
+
+int mpt_film_create(mpt_context *ctx, uint32_t width, uint32_t height, mpt_film **out) {
+  if (!ctx || !out || !width || !height) return fail(ctx, MPT_E_INVALID, "bad film arguments");
+  *out = nullptr;
+  std::unique_ptr<mpt_film> f(new (std::nothrow) mpt_film());
+  if (!f) return fail(ctx, MPT_E_NOMEM, "host allocation");
+  f->ctx = ctx;
+  f->width = width;
+  f->height = height;
+  MPT_HIP(ctx, f->data.reserve((size_t)width * height));
+  MPT_HIP(ctx, hipMemsetAsync(f->data.p, 0, f->data.bytes(), ctx->stream));
+  *out = f.release();
+  return MPT_OK;
+}
+
+int mpt_film_destroy(mpt_film *f) {
+  if (!f) return MPT_E_INVALID;
+  (void)hipStreamSynchronize(f->ctx->stream);
+  delete f;
+  return MPT_OK;
+}
+
+int mpt_render_film(mpt_scene *s, const mpt_job *job, mpt_film *film, mpt_render_stats *stats) {
+  if (!s || !film) return fail(s ? s->ctx : nullptr, MPT_E_INVALID, "NULL argument");
+  if (!job || job->width != film->width || job->height != film->height)
+    return fail(s->ctx, MPT_E_INVALID, "film size does not match the job");
+  return render_range(s, job, 0, job->width * job->height, film->data.p, nullptr, stats);
+}
+
+int mpt_film_download(mpt_film *f, float *rgb) {
+  if (!f || !rgb) return MPT_E_INVALID;
+  mpt_context *ctx = f->ctx;
+  std::vector<float4> host((size_t)f->width * f->height);
+  MPT_HIP(ctx, hipMemcpyAsync(host.data(), f->data.p, f->data.bytes(), hipMemcpyDeviceToHost, ctx->stream));
+  MPT_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  for (size_t i = 0; i < host.size(); ++i) {
+    const float w = host[i].w;
+    const float k = w > 0 ? 1.0f / w : 0.0f;
+    rgb[3 * i] = host[i].x * k;
+    rgb[3 * i + 1] = host[i].y * k;
+    rgb[3 * i + 2] = host[i].z * k;
+  }
+  return MPT_OK;
+}
+
+int mpt_render(mpt_scene *s, const mpt_job *job, float *film_rgb, mpt_render_stats *stats) {
+  if (!s || !film_rgb) return fail(s ? s->ctx : nullptr, MPT_E_INVALID, "NULL argument");
+  int rc = check_job(s->ctx, job);
+  if (rc) return rc;
+  mpt_film *f = nullptr;
+  rc = mpt_film_create(s->ctx, job->width, job->height, &f);
+  if (rc) return rc;
+  rc = mpt_render_film(s, job, f, stats);
+  if (!rc) rc = mpt_film_download(f, film_rgb);
+  mpt_film_destroy(f);
+  return rc;
+}
+
+int mpt_render_paths(mpt_scene *s, const mpt_job *job, uint32_t pix_begin, uint32_t pix_end, float *L_out) {
+  if (!s || !L_out) return fail(s ? s->ctx : nullptr, MPT_E_INVALID, "NULL argument");
+  int rc = check_job(s->ctx, job);
+  if (rc) return rc;
+  if (pix_begin >= pix_end || pix_end > job->width * job->height)
+    return fail(s->ctx, MPT_E_INVALID, "bad pixel range");
+  return render_range(s, job, pix_begin, pix_end, nullptr, L_out, nullptr);
+}
+
+}  // extern "C"

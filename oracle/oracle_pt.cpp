@@ -1,0 +1,1234 @@
+/*
+ * oracle_pt.cpp -- TEST INFRASTRUCTURE ONLY.
+ *
+ * A double-precision CPU restatement of Milton's unidirectional path tracer
+ * and the functions it calls.  It is the checker for the HIP product path:
+ * tests, the smoke check and the CPU-baseline leg of the benchmark load it;
+ * the product library never links or calls it.
+ *
+ * Every function cites the reference file:line it restates (paths relative to
+ * mlt/src/milton/).  Deliberate differences from the reference, each shared
+ * with the GPU path so both compute the same estimator:
+ *   - random numbers come from a counter-based Philox4x32-10 stream keyed by
+ *     (seed, job, pixel, sample, depth, stream) instead of the process-global,
+ *     unsynchronised mt19937 (stats/Random.cpp:29-56), whose per-path stream is
+ *     not reproducible in the reference either;
+ *   - each pixel draws its own jitter in its stratum instead of cycling 133
+ *     shared pre-drawn patterns (renderers/generators/SuperSampleGenerator.cpp:
+ *     23-91); the stratum order (row-major, floor(sqrt(spp))^2 strata) is kept;
+ *   - a maximum path depth caps the recursion (the reference has none);
+ *   - a BSDF sample whose pdf is zero terminates the path instead of
+ *     producing a NaN (ModifiedPhongBSDF.cpp:150-160 can return pdf 0 for a
+ *     back-side hit);
+ *   - the kd node stores its child index instead of a 32-bit-truncated
+ *     pointer (accel/kdTreeAccel.cpp:80-85).
+ * Parity status of each part is listed in the repository design notes.
+ */
+#include "oracle_pt.h"
+
+#include <algorithm>
+#include <atomic>
+#include <cmath>
+#include <cstring>
+#include <limits>
+#include <memory>
+#include <thread>
+#include <vector>
+
+namespace {
+
+const double kEps = 1e-7;                                   // common/arch.h:137
+const double kInf = std::numeric_limits<double>::infinity(); // common/arch.h:136
+const double kAirIor = 1.0003;                              // materials/IndexOfRefraction.cpp:16
+
+struct V3 {
+  double x, y, z;
+  V3() : x(0), y(0), z(0) {}
+  V3(double a, double b, double c) : x(a), y(b), z(c) {}
+  double operator[](int i) const { return i == 0 ? x : (i == 1 ? y : z); }
+  double &operator[](int i) { return i == 0 ? x : (i == 1 ? y : z); }
+};
+inline V3 operator+(V3 a, V3 b) { return V3(a.x + b.x, a.y + b.y, a.z + b.z); }
+inline V3 operator-(V3 a, V3 b) { return V3(a.x - b.x, a.y - b.y, a.z - b.z); }
+inline V3 operator-(V3 a) { return V3(-a.x, -a.y, -a.z); }
+inline V3 operator*(V3 a, double s) { return V3(a.x * s, a.y * s, a.z * s); }
+inline V3 operator*(double s, V3 a) { return a * s; }
+inline V3 operator/(V3 a, double s) { return V3(a.x / s, a.y / s, a.z / s); }
+inline V3 mul(V3 a, V3 b) { return V3(a.x * b.x, a.y * b.y, a.z * b.z); }
+inline double dot(V3 a, V3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+inline V3 cross(V3 a, V3 b) {
+  return V3(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
+}
+inline double len(V3 a) { return std::sqrt(dot(a, a)); }
+inline V3 normalized(V3 a) { return a / len(a); }
+inline bool is_black(V3 a) { return a.x == 0 && a.y == 0 && a.z == 0; }
+inline double max_comp(V3 a) { return std::max(a.x, std::max(a.y, a.z)); }
+
+// core/Ray.h:39-43 -- the ray caches the componentwise reciprocal direction.
+struct Ray {
+  V3 o, d, inv;
+  Ray(V3 o_, V3 d_) : o(o_), d(d_), inv(1.0 / d_.x, 1.0 / d_.y, 1.0 / d_.z) {}
+};
+
+struct Aabb {
+  V3 lo, hi;
+  Aabb() : lo(kInf, kInf, kInf), hi(-kInf, -kInf, -kInf) {}
+  void add(V3 p) {
+    lo = V3(std::min(lo.x, p.x), std::min(lo.y, p.y), std::min(lo.z, p.z));
+    hi = V3(std::max(hi.x, p.x), std::max(hi.y, p.y), std::max(hi.z, p.z));
+  }
+  void add(const Aabb &b) { add(b.lo); add(b.hi); }
+  bool is_point() const { return lo.x == hi.x && lo.y == hi.y && lo.z == hi.z; }
+  // accel/AABB.inl:112-118
+  double area() const {
+    V3 e = hi - lo;
+    return 2 * (e.x * e.y + e.y * e.z + e.z * e.x);
+  }
+  // accel/AABB.inl:120-136
+  void split_areas(int axis, double pos, double &l, double &r) const {
+    V3 e = hi - lo;
+    e[axis] = pos - lo[axis];
+    l = 2 * (e.x * e.y + e.y * e.z + e.z * e.x);
+    e[axis] = hi[axis] - pos;
+    r = 2 * (e.x * e.y + e.y * e.z + e.z * e.x);
+  }
+  // accel/AABB.inl:25-56 (Williams et al. slab test)
+  bool intersects(const Ray &ray, double &tmin, double &tmax) const {
+    const V3 b[2] = {lo, hi};
+    const int xs = ray.inv.x < 0, ys = ray.inv.y < 0, zs = ray.inv.z < 0;
+    tmin = (b[xs].x - ray.o.x) * ray.inv.x;
+    tmax = (b[1 - xs].x - ray.o.x) * ray.inv.x;
+    const double tymin = (b[ys].y - ray.o.y) * ray.inv.y;
+    const double tymax = (b[1 - ys].y - ray.o.y) * ray.inv.y;
+    if (tmin > tymax || tymin > tmax) return false;
+    if (tymin > tmin) tmin = tymin;
+    if (tymax < tmax) tmax = tymax;
+    const double tzmin = (b[zs].z - ray.o.z) * ray.inv.z;
+    const double tzmax = (b[1 - zs].z - ray.o.z) * ray.inv.z;
+    if (tmin > tzmax || tzmin > tmax) return false;
+    if (tzmin > tmin) tmin = tzmin;
+    if (tzmax < tmax) tmax = tzmax;
+    return tmin <= tmax;
+  }
+};
+
+// ---------------------------------------------------------------------------
+// Philox4x32-10 (Salmon, Moraes, Dror, Shaw: "Parallel random numbers: as
+// easy as 1, 2, 3", SC 2011).
+// ---------------------------------------------------------------------------
+inline void philox(const uint32_t in[4], const uint32_t key[2], uint32_t out[4]) {
+  uint32_t c0 = in[0], c1 = in[1], c2 = in[2], c3 = in[3];
+  uint32_t k0 = key[0], k1 = key[1];
+  for (int r = 0; r < 10; ++r) {
+    if (r) { k0 += 0x9E3779B9u; k1 += 0xBB67AE85u; }
+    const uint64_t p0 = (uint64_t)0xD2511F53u * c0;
+    const uint64_t p1 = (uint64_t)0xCD9E8D57u * c2;
+    const uint32_t n0 = (uint32_t)(p1 >> 32) ^ c1 ^ k0;
+    const uint32_t n2 = (uint32_t)(p0 >> 32) ^ c3 ^ k1;
+    c1 = (uint32_t)p1;
+    c3 = (uint32_t)p0;
+    c0 = n0;
+    c2 = n2;
+  }
+  out[0] = c0; out[1] = c1; out[2] = c2; out[3] = c3;
+}
+
+// Stream layout shared with the device path: counter = (pixel, sample,
+// depth << 8 | stream, job), key = (seed, 0x4D505431).
+const uint32_t kKeyHi = 0x4D505431u;
+const uint32_t kCameraDepth = 255;
+struct Uniforms { double u[4]; uint32_t raw[4]; };
+inline Uniforms draw(uint32_t seed, uint32_t job, uint32_t pixel, uint32_t sample,
+                     uint32_t depth, uint32_t stream) {
+  const uint32_t ctr[4] = {pixel, sample, (depth << 8) | stream, job};
+  const uint32_t key[2] = {seed, kKeyHi};
+  Uniforms r;
+  philox(ctr, key, r.raw);
+  for (int i = 0; i < 4; ++i) r.u[i] = (double)(r.raw[i] >> 8) * (1.0 / 16777216.0);
+  return r;
+}
+
+// ---------------------------------------------------------------------------
+// kd-tree: accel/kdTreeAccel.cpp restated.
+// ---------------------------------------------------------------------------
+struct KdParams {
+  unsigned min_prims = 3;     // kdTreeAccel.h:97-104 (BuildParams defaults)
+  unsigned max_depth = 24;
+  double cost_traversal = 1;
+  double empty_bias = 0.9;
+};
+
+struct BNode {
+  int axis = 3;   // 3 == leaf (KD_LEAF_FLAG, kdTreeAccel.cpp:64)
+  float split = 0;
+  std::unique_ptr<BNode[]> kids;
+  std::vector<uint32_t> prims;
+};
+
+struct SplitPlane {
+  // kdTreeAccel.cpp:279-297: SPLIT_MAX < SPLIT_PLANE < SPLIT_MIN on ties
+  enum { MAX = 0, PLANE = 1, MIN = 2 };
+  double pos;
+  int type;
+  bool operator<(const SplitPlane &p) const {
+    if (pos < p.pos) return true;
+    if (pos > p.pos) return false;
+    return type < p.type;
+  }
+};
+
+struct KdNode {
+  uint32_t axis;   // 3 == leaf
+  float split;
+  uint32_t child;  // left child index; the right child is child + 1
+  uint32_t first, count;
+};
+
+class KdTree {
+ public:
+  Aabb box;
+  std::vector<KdNode> nodes;
+  std::vector<uint32_t> lists;
+  std::vector<Aabb> bounds;   // per primitive
+  KdParams params;
+  bool naive = false;
+  uint64_t n_internal = 0, n_leaves = 0, max_depth_seen = 0;
+
+  // kdTreeAccel.cpp:403-472 (init) and SpatialAccel::init (global AABB).
+  void build(std::vector<Aabb> prim_bounds, const KdParams &p, bool use_naive, int nthreads) {
+    bounds = std::move(prim_bounds);
+    params = p;
+    naive = use_naive;
+    box = Aabb();
+    for (const Aabb &b : bounds) box.add(b);
+    nodes.clear();
+    lists.clear();
+    if (naive) return;
+    BNode root;
+    std::vector<uint32_t> all(bounds.size());
+    for (uint32_t i = 0; i < all.size(); ++i) all[i] = i;
+    const int spawn_depth = nthreads > 1 ? std::min(6, (int)std::ceil(std::log2((double)nthreads)) + 1) : 0;
+    build_helper(&root, std::move(all), 3, 0, box, spawn_depth);
+    flatten(root);
+  }
+
+  // kdTreeAccel.cpp:893-1016 (getIntersection).  `isect(prim, ray)` returns
+  // the primitive's t (INFINITY on a miss) and may set a sub-index.
+  template <class F>
+  double closest(const Ray &ray, uint32_t &hit_prim, F &&isect) const {
+    if (naive) return closest_naive(ray, hit_prim, isect);
+    double tmin, tmax;
+    if (!box.intersects(ray, tmin, tmax)) return kInf;
+    tmin -= kEps;
+    tmax += kEps;
+    struct Entry { uint32_t node; double tmax; } stack[64];
+    int sp = 0;
+    uint32_t cur = 0;
+    for (;;) {
+      while (nodes[cur].axis != 3) {
+        const KdNode &n = nodes[cur];
+        const int a = n.axis;
+        const double diff = (double)n.split - ray.o[a];
+        const double d = diff * ray.inv[a];
+        const bool rev = diff <= 0 ? (ray.d[a] <= 0) : (ray.d[a] < 0);
+        cur = n.child + (rev ? 1 : 0);
+        if (d > tmax) continue;
+        const uint32_t far_node = n.child + (rev ? 0 : 1);
+        if (d < tmin) {
+          cur = far_node;
+        } else {
+          stack[sp].node = far_node;
+          stack[sp].tmax = tmax;
+          ++sp;
+          tmax = d + kEps;
+        }
+      }
+      const KdNode &leaf = nodes[cur];
+      double tcell = kInf;
+      uint32_t best = ORC_MISS;
+      for (uint32_t i = leaf.count; i--;) {
+        const uint32_t prim = lists[leaf.first + i];
+        uint32_t sub = prim;
+        const double t = isect(prim, ray, sub);
+        if (t > kEps && t < tcell) {
+          tcell = t;
+          best = sub;
+        }
+      }
+      if (tcell <= tmax) {
+        hit_prim = best;
+        return tcell;
+      }
+      if (sp == 0) return kInf;
+      tmin = tmax - 2 * kEps;
+      --sp;
+      cur = stack[sp].node;
+      tmax = stack[sp].tmax;
+    }
+  }
+
+  // kdTreeAccel.cpp:1018-1094 (intersects).  `any(prim, ray, clip)` is the
+  // primitive's own any-hit test.
+  template <class F>
+  bool any(const Ray &ray, double clip, F &&anyf) const {
+    double tmin, tmax;
+    if (naive) {
+      if (!box.intersects(ray, tmin, tmax)) return false;
+      for (uint32_t i = (uint32_t)bounds.size(); i--;)
+        if (anyf(i, ray, clip)) return true;
+      return false;
+    }
+    if (!box.intersects(ray, tmin, tmax) || clip < tmin) return false;
+    if (clip < tmax) tmax = clip;
+    tmin -= kEps;
+    tmax += kEps;
+    struct Entry { uint32_t node; double tmax; } stack[64];
+    int sp = 0;
+    uint32_t cur = 0;
+    for (;;) {
+      while (nodes[cur].axis != 3) {
+        const KdNode &n = nodes[cur];
+        const int a = n.axis;
+        const double diff = (double)n.split - ray.o[a];
+        const double d = diff * ray.inv[a];
+        const bool rev = diff <= 0 ? (ray.d[a] <= 0) : (ray.d[a] < 0);
+        cur = n.child + (rev ? 1 : 0);
+        if (d > tmax) continue;
+        const uint32_t far_node = n.child + (rev ? 0 : 1);
+        if (d < tmin) {
+          cur = far_node;
+        } else {
+          stack[sp].node = far_node;
+          stack[sp].tmax = tmax;
+          ++sp;
+          tmax = d + kEps;
+        }
+      }
+      const KdNode &leaf = nodes[cur];
+      for (uint32_t i = leaf.count; i--;)
+        if (anyf(lists[leaf.first + i], ray, clip)) return true;
+      if (sp == 0) return false;
+      tmin = tmax;
+      --sp;
+      cur = stack[sp].node;
+      tmax = stack[sp].tmax;
+    }
+  }
+
+  // Prims whose bounds overlap a leaf cell with positive volume but are not
+  // referenced by it (the integrity check disabled at kdTreeAccel.cpp:245-266).
+  uint64_t check_invariant() const {
+    if (naive) return 0;
+    uint64_t bad = 0;
+    check_rec(0, box, bad);
+    return bad;
+  }
+
+ private:
+  template <class F>
+  double closest_naive(const Ray &ray, uint32_t &hit_prim, F &&isect) const {
+    // accel/NaiveSpatialAccel.cpp:16-50: the AABB miss returns 0 (a miss).
+    double tmin, tmax;
+    if (!box.intersects(ray, tmin, tmax)) return 0;
+    double best_t = kInf;
+    for (uint32_t i = (uint32_t)bounds.size(); i--;) {
+      uint32_t sub = i;
+      const double t = isect(i, ray, sub);
+      if (t > kEps && t < best_t) {
+        best_t = t;
+        hit_prim = sub;
+      }
+    }
+    return best_t;
+  }
+
+  void check_rec(uint32_t idx, const Aabb &cell, uint64_t &bad) const {
+    const KdNode &n = nodes[idx];
+    if (n.axis != 3) {
+      Aabb l = cell, r = cell;
+      l.hi[n.axis] = n.split;
+      r.lo[n.axis] = n.split;
+      check_rec(n.child, l, bad);
+      check_rec(n.child + 1, r, bad);
+      return;
+    }
+    std::vector<uint32_t> ref(lists.begin() + n.first, lists.begin() + n.first + n.count);
+    std::sort(ref.begin(), ref.end());
+    for (uint32_t p = 0; p < bounds.size(); ++p) {
+      const Aabb &b = bounds[p];
+      bool overlap = true;
+      for (int a = 0; a < 3; ++a)
+        overlap = overlap && b.lo[a] < cell.hi[a] && b.hi[a] > cell.lo[a];
+      if (overlap && !std::binary_search(ref.begin(), ref.end(), p)) ++bad;
+    }
+  }
+
+  // kdTreeAccel.cpp:533-630 (_buildTree) + 632-660 (_buildTreeHelper).
+  void build_helper(BNode *node, std::vector<uint32_t> prims, int parent_axis,
+                    unsigned depth, Aabb cell, int spawn_depth) {
+    if (prims.empty()) {
+      node->axis = 3;
+      return;
+    }
+    const unsigned n = (unsigned)prims.size();
+    if (n < params.min_prims || depth >= params.max_depth) {
+      node->axis = 3;
+      node->prims = std::move(prims);
+      return;
+    }
+    int axis = 0;
+    double pos = 0;
+    if (!sah_split(prims, parent_axis, cell, axis, pos)) {
+      node->axis = 3;
+      node->prims = std::move(prims);
+      return;
+    }
+    std::vector<uint32_t> left, right;
+    for (uint32_t p : prims) {
+      const Aabb &b = bounds[p];
+      if (b.is_point()) continue;
+      if (b.hi[axis] > pos) right.push_back(p);
+      if (b.lo[axis] <= pos) left.push_back(p);
+    }
+    // kdTreeAccel.cpp:586-597: with SAH, stop only if neither side shrinks.
+    if (left.size() == n && right.size() == n) {
+      node->axis = 3;
+      node->prims = std::move(prims);
+      return;
+    }
+    prims.clear();
+    prims.shrink_to_fit();
+    node->axis = axis;
+    node->split = (float)pos;
+    node->kids.reset(new BNode[2]);
+    Aabb lcell = cell, rcell = cell;
+    lcell.hi[axis] = pos;
+    rcell.lo[axis] = pos;
+    if ((int)depth < spawn_depth) {
+      BNode *rk = &node->kids[1];
+      std::thread t([this, rk, &right, axis, depth, rcell, spawn_depth]() {
+        build_helper(rk, std::move(right), axis, depth + 1, rcell, spawn_depth);
+      });
+      build_helper(&node->kids[0], std::move(left), axis, depth + 1, lcell, spawn_depth);
+      t.join();
+    } else {
+      build_helper(&node->kids[0], std::move(left), axis, depth + 1, lcell, spawn_depth);
+      build_helper(&node->kids[1], std::move(right), axis, depth + 1, rcell, spawn_depth);
+    }
+  }
+
+  // kdTreeAccel.cpp:722-811 (_computeSplitPlaneSAH), 813-853
+  // (_computeSplitPlanesSAH), 855-877 (_computeSplitCostSAH).
+  bool sah_split(const std::vector<uint32_t> &prims, int parent_axis, const Aabb &cell,
+                 int &best_axis, double &best_pos) const {
+    const unsigned n_prims = (unsigned)prims.size();
+    double best_cost = kInf;
+    std::vector<SplitPlane> planes;
+    planes.reserve(2 * prims.size());
+    for (int axis = 0; axis < 3; ++axis) {
+      if (axis == parent_axis || cell.lo[axis] >= cell.hi[axis]) continue;
+      planes.clear();
+      for (uint32_t i = n_prims; i--;) {
+        const Aabb &b = bounds[prims[i]];
+        if (b.is_point()) continue;
+        const double mn = b.lo[axis], mx = b.hi[axis];
+        if (mn == mx) {
+          planes.push_back({mn, SplitPlane::PLANE});
+        } else {
+          planes.push_back({mn, SplitPlane::MIN});
+          planes.push_back({mx, SplitPlane::MAX});
+        }
+      }
+      const unsigned n = (unsigned)planes.size();
+      if (n == 0) continue;
+      std::sort(planes.begin(), planes.end());
+      unsigned index = 0, n_left = 0, n_right = n_prims;
+      while (index < n - 1) {
+        const double cur = planes[index].pos;
+        unsigned cnt[3] = {0, 0, 0};
+        do {
+          ++cnt[planes[index].type];
+        } while (index < n - 1 && planes[++index].pos == cur);
+        const unsigned n_plane = cnt[SplitPlane::PLANE];
+        n_right -= cnt[SplitPlane::MAX] + n_plane;
+        if (cur > cell.lo[axis] && cur < cell.hi[axis]) {
+          double la, ra;
+          cell.split_areas(axis, cur, la, ra);
+          const unsigned nl = n_left + n_plane;
+          double cost = la * nl + ra * n_right;
+          if (nl + n_plane == 0 || n_right == 0) cost *= params.empty_bias;
+          if (cost < best_cost) {
+            best_cost = cost;
+            best_axis = axis;
+            best_pos = cur;
+          }
+        }
+        n_left += cnt[SplitPlane::MIN] + n_plane;
+      }
+    }
+    if (best_cost == kInf) return false;
+    const double leaf_cost = (n_prims - params.cost_traversal) * cell.area();
+    if (best_cost > leaf_cost) return false;
+    return true;
+  }
+
+  void flatten(BNode &root) {
+    nodes.assign(1, KdNode{3, 0, 0, 0, 0});
+    std::vector<std::pair<BNode *, uint32_t>> todo{{&root, 0}};
+    std::vector<std::pair<BNode *, uint32_t>> next;
+    uint64_t depth = 0;
+    while (!todo.empty()) {
+      next.clear();
+      for (auto &e : todo) {
+        BNode *b = e.first;
+        KdNode &k = nodes[e.second];
+        if (b->axis == 3) {
+          k.axis = 3;
+          k.first = (uint32_t)lists.size();
+          k.count = (uint32_t)b->prims.size();
+          lists.insert(lists.end(), b->prims.begin(), b->prims.end());
+          ++n_leaves;
+          max_depth_seen = std::max(max_depth_seen, depth);
+        } else {
+          const uint32_t child = (uint32_t)nodes.size();
+          k.axis = (uint32_t)b->axis;
+          k.split = b->split;
+          k.child = child;
+          nodes.push_back(KdNode{3, 0, 0, 0, 0});
+          nodes.push_back(KdNode{3, 0, 0, 0, 0});
+          next.push_back({&b->kids[0], child});
+          next.push_back({&b->kids[1], child + 1});
+          ++n_internal;
+        }
+      }
+      todo.swap(next);
+      ++depth;
+    }
+  }
+};
+
+// ---------------------------------------------------------------------------
+// Materials (materials/*.cpp) and scene.
+// ---------------------------------------------------------------------------
+enum { MAT_DIFFUSE = 0, MAT_PHONG = 1, MAT_DIELECTRIC = 2 };
+struct Material {
+  int type;
+  V3 kd, ks;
+  double exponent, ior, transparency;
+  V3 Le;
+  bool emitter() const { return Le.x > 0 || Le.y > 0 || Le.z > 0; }
+  bool specular() const { return type == MAT_DIELECTRIC; }   // DielectricBSDF.h:64
+};
+Material load_material(const float *m) {
+  Material r;
+  r.type = (int)m[0];
+  r.kd = V3(m[1], m[2], m[3]);
+  r.ks = V3(m[4], m[5], m[6]);
+  r.exponent = m[7];
+  r.ior = m[8];
+  r.transparency = m[9];
+  r.Le = V3(m[10], m[11], m[12]);
+  return r;
+}
+
+struct Quad {   // shapes/Plane.cpp -- unit plane y=0 mapped by (ex, n, ez) + c
+  V3 c, ex, ez, n;
+  V3 rx, rz;    // rows of the inverse linear map for x and z
+  double area;
+};
+struct Sphere { V3 c; double r; };
+
+}  // namespace
+
+struct orc_scene {
+  std::vector<V3> verts, vnormals;
+  std::vector<uint32_t> idx, tri_mat, sph_mat, quad_mat;
+  std::vector<Sphere> spheres;
+  std::vector<Quad> quads;
+  std::vector<Material> mats;
+  std::vector<uint32_t> lights;   // quad indices with emissive materials
+  V3 env;
+  uint32_t nt = 0, ns = 0, nq = 0;
+  KdTree mesh_tree;   // over triangles (Mesh::init, shapes/Mesh.cpp:211-251)
+  KdTree top_tree;    // over shapes: [mesh?, spheres..., quads...] (ShapeSet.cpp:36-80)
+  std::vector<uint32_t> top_kind;   // 0 mesh, 1 sphere, 2 quad
+  std::vector<uint32_t> top_index;
+
+  // --- primitive tests ---------------------------------------------------
+  // shapes/MeshTriangle.cpp:92-169, non-culling branch (:142-167).
+  double tri_t(uint32_t i, const Ray &ray) const {
+    const V3 &v0 = verts[idx[3 * i]], &v1 = verts[idx[3 * i + 1]], &v2 = verts[idx[3 * i + 2]];
+    const V3 e1 = v1 - v0, e2 = v2 - v0;
+    const V3 p = cross(ray.d, e2);
+    const double det = dot(e1, p);
+    // EQ(det, 0) (common/utilities.h:240) reduces to |det| <= EPSILON.
+    if (std::fabs(det) < kEps || std::fabs(det - 0.0 / det) <= kEps) return kInf;
+    const double inv = 1.0 / det;
+    const V3 tv = ray.o - v0;
+    const double u = dot(tv, p) * inv;
+    if (u < 0.0 || u > 1.0) return kInf;
+    const V3 q = cross(tv, e1);
+    const double v = dot(ray.d, q) * inv;
+    if (v < 0.0 || u + v > 1.0) return kInf;
+    return dot(e2, q) * inv;
+  }
+  // shapes/Sphere.cpp:16-45, in the object space of a unit-diameter sphere.
+  double sphere_t(uint32_t i, const Ray &ray) const {
+    const Sphere &s = spheres[i];
+    const double k = 1.0 / (2 * s.r);
+    const V3 P = (ray.o - s.c) * k, D = ray.d * k;
+    const double A = 2 * dot(D, D);
+    const double B = -2 * dot(P, D);
+    const double C = dot(P, P) - 0.25;
+    double disc = B * B - 2 * A * C;
+    if (disc < 0) return kInf;
+    disc = std::sqrt(disc);
+    const double F = 1 / A;
+    double t = (B - disc) * F;
+    if (t > kEps) return t;
+    t = (B + disc) * F;
+    return t > kEps ? t : kInf;
+  }
+  // shapes/Sphere.cpp:47-73
+  bool sphere_any(uint32_t i, const Ray &ray, double tmax) const {
+    const Sphere &s = spheres[i];
+    const double k = 1.0 / (2 * s.r);
+    const V3 P = (ray.o - s.c) * k, D = ray.d * k;
+    const double A = 2 * dot(D, D);
+    const double B = -2 * dot(P, D);
+    const double C = dot(P, P) - 0.25;
+    double disc = B * B - 2 * A * C;
+    if (disc < 0) return false;
+    disc = std::sqrt(disc);
+    const double F = 1 / A;
+    double t = (B + disc) * F;
+    if (t < tmax && t > kEps) return true;
+    t = (B - disc) * F;
+    return t < tmax && t > kEps;
+  }
+  // shapes/Plane.cpp:25-49 (object-space y = 0 plane, |x|,|z| <= 0.5).
+  bool quad_obj(uint32_t i, const Ray &ray, double &t) const {
+    const Quad &q = quads[i];
+    const V3 rel = ray.o - q.c;
+    const double Py = dot(q.n, rel), Dy = dot(q.n, ray.d);
+    const double Px = dot(q.rx, rel), Dx = dot(q.rx, ray.d);
+    const double Pz = dot(q.rz, rel), Dz = dot(q.rz, ray.d);
+    t = -Py / Dy;
+    return std::fabs(Px + t * Dx) <= 0.5 && std::fabs(Pz + t * Dz) <= 0.5;
+  }
+  double quad_t(uint32_t i, const Ray &ray) const {
+    double t;
+    const bool in = quad_obj(i, ray, t);
+    return (t > kEps && in) ? t : kInf;
+  }
+  bool quad_any(uint32_t i, const Ray &ray, double tmax) const {
+    double t;
+    const bool in = quad_obj(i, ray, t);
+    return t < tmax && t > kEps && in;
+  }
+
+  // Scene::getIntersection (core/Scene.cpp:104-108) -> ShapeSet -> top kd ->
+  // Mesh::getIntersection (shapes/Mesh.cpp:379-395) -> mesh kd.
+  double closest(const Ray &ray, uint32_t &gid) const {
+    gid = ORC_MISS;
+    uint32_t hit = ORC_MISS;
+    const double t = top_tree.closest(ray, hit, [&](uint32_t shape, const Ray &r, uint32_t &sub) -> double {
+      const uint32_t kind = top_kind[shape], k = top_index[shape];
+      if (kind == 0) {
+        uint32_t tri = ORC_MISS;
+        const double tt = mesh_tree.closest(r, tri, [&](uint32_t p, const Ray &rr, uint32_t &s2) -> double {
+          s2 = p;
+          return tri_t(p, rr);
+        });
+        if (tt > kEps && tt < kInf) sub = tri;
+        return tt;
+      }
+      if (kind == 1) { sub = nt + k; return sphere_t(k, r); }
+      sub = nt + ns + k;
+      return quad_t(k, r);
+    });
+    if (t > kEps && t < kInf) gid = hit;
+    return t;
+  }
+
+  // Scene::intersects (core/Scene.cpp:110-114) passes tMax - EPSILON.
+  bool occluded(const Ray &ray, double tmax) const {
+    const double clip = tmax - kEps;
+    return top_tree.any(ray, clip, [&](uint32_t shape, const Ray &r, double c) -> bool {
+      const uint32_t kind = top_kind[shape], k = top_index[shape];
+      if (kind == 0) {
+        // Mesh::intersects (shapes/Mesh.cpp:397-406) -> mesh kd ->
+        // Intersectable::intersects (shapes/Intersectable.cpp:20-25).
+        return mesh_tree.any(r, c, [&](uint32_t p, const Ray &rr, double cc) -> bool {
+          const double t = tri_t(p, rr);
+          return t < cc - kEps && t > kEps;
+        });
+      }
+      if (kind == 1) return sphere_any(k, r, c);
+      return quad_any(k, r, c);
+    });
+  }
+
+  // --- surface points ----------------------------------------------------
+  // Shape::initSurfacePoint (shapes/Shape.cpp:17-27) with the shape's
+  // _getGeometricNormal; normalS = normalG (Material.cpp:183-186, no bump map).
+  V3 normal_at(uint32_t gid, V3 p) const {
+    if (gid < nt) {
+      // Mesh::_getGeometricNormal (shapes/Mesh.cpp:419-433) with
+      // MeshTriangle::getBaryocentricCoords (shapes/MeshTriangle.cpp:344-361).
+      const uint32_t a = idx[3 * gid], b = idx[3 * gid + 1], c = idx[3 * gid + 2];
+      const V3 &v1 = verts[a], &v2 = verts[b], &v3 = verts[c];
+      const double A = 1.0 / len(cross(v2 - v1, v3 - v1));
+      const double pA = len(cross(v2 - p, v3 - p)) * A;
+      const double pB = len(cross(p - v1, v3 - v1)) * A;
+      const double pC = 1 - pA - pB;
+      return normalized(vnormals[a] * pA + vnormals[b] * pB + vnormals[c] * pC);
+    }
+    if (gid < nt + ns) return normalized(p - spheres[gid - nt].c);   // Sphere.cpp:86-92
+    return quads[gid - nt - ns].n;                                    // Plane.cpp:59-63
+  }
+  const Material &material_of(uint32_t gid) const {
+    if (gid < nt) return mats[tri_mat[gid]];
+    if (gid < nt + ns) return mats[sph_mat[gid - nt]];
+    return mats[quad_mat[gid - nt - ns]];
+  }
+  // Shape::getSurfaceArea: Plane.cpp:95-106; Sphere.cpp:138-141 (unit sphere).
+  double shape_area(uint32_t gid) const {
+    if (gid >= nt + ns) return quads[gid - nt - ns].area;
+    if (gid >= nt) return M_PI;
+    return 0;
+  }
+};
+
+namespace {
+
+// ---------------------------------------------------------------------------
+// BSDFs.
+// ---------------------------------------------------------------------------
+// Vector::getOrthonormalBasis (common/math/Vector.inl:527-538).
+inline void onb(V3 n, V3 &U, V3 &V) {
+  n = normalized(n);
+  V3 up(0, 1, 0);
+  if (std::fabs(dot(up, n)) > 0.8) up = V3(1, 0, 0);
+  U = normalized(cross(up, n));
+  V = normalized(cross(n, U));
+}
+// Vector::fromSpherical (common/math/Vector.h:113-117).
+inline V3 from_spherical(double theta, double phi) {
+  return normalized(V3(std::sin(theta) * std::cos(phi), std::cos(theta), std::sin(theta) * std::sin(phi)));
+}
+// Vector3::cosRandom (common/math/Vector.cpp:21-29) + convertHemisphere
+// (common/math/sisd/Vector3.inl:18-22).
+inline V3 cos_random(V3 n, double u1, double u2) {
+  const double theta = std::acos(std::sqrt(u1));
+  const double phi = 2.0 * M_PI * u2;
+  const V3 w = from_spherical(theta, phi);
+  V3 U, V;
+  onb(n, U, V);
+  return U * w.x + normalized(n) * w.y + V * w.z;
+}
+// Vector::reflectVector (common/math/Vector.inl:460-462).
+inline V3 reflect(V3 w, V3 n) { return -2 * n * dot(w, n) + w; }
+// Vector::refractVector (common/math/Vector.inl:468-494).
+inline V3 refract(V3 v, V3 normal, double in, double out) {
+  const V3 w = -v;
+  V3 norm;
+  double n;
+  if (dot(v, normal) < 0) { norm = normal; n = in / out; }
+  else { norm = -normal; n = out / in; }
+  const double d = dot(w, norm);
+  const double det = 1 - n * n * (1 - d * d);
+  if (det < 0) return V3();
+  return -n * w + norm * (n * d - std::sqrt(det));
+}
+// Fresnel::getFr (materials/bsdfs/Fresnel.h:45-90).
+inline double fresnel_cos(double ci, double n1, double n2) {
+  const double snell = (n1 / n2) * std::sin(std::acos(ci));
+  if (std::fabs(snell) >= 1) return 1;
+  const double ct = std::cos(std::asin(snell));
+  const double Rs = (n1 * ci - n2 * ct) / (n1 * ci + n2 * ct);
+  const double Rp = (n1 * ct - n2 * ci) / (n1 * ct + n2 * ci);
+  return (Rs * Rs + Rp * Rp) / 2.0;
+}
+inline double fresnel(V3 wi, V3 N, double n1, double n2) {
+  const double c = dot(N, wi);
+  return c < 0 ? fresnel_cos(-c, n1, n2) : fresnel_cos(c, n2, n1);
+}
+
+struct BsdfSample {
+  V3 wo;        // zero if absorbed
+  V3 fs;        // f / pdf
+  double pdf = 0;
+};
+
+inline double avg(V3 a) { return (a.x + a.y + a.z) / 3.0; }
+
+// f(wi, wo) for each BSDF.
+V3 bsdf_eval(const Material &m, V3 n, V3 wi, V3 wo) {
+  if (m.type == MAT_DIFFUSE) {
+    // DiffuseBSDF::evaluate (materials/bsdfs/DiffuseBSDF.cpp:40-48): one-sided.
+    if (dot(n, wi) >= 0 || dot(n, wo) <= 0) return V3();
+    return m.kd / M_PI;
+  }
+  if (m.type == MAT_PHONG) {
+    // ModifiedPhongBSDF::evaluate (ModifiedPhongBSDF.cpp:164-181).
+    if (dot(n, wo) <= 0) return V3();
+    const V3 r = reflect(wi, n);
+    const double a = std::min(1.0, std::max(0.0, dot(r, wo)));
+    const double kda = avg(m.kd), ksa = avg(m.ks);
+    const double an = std::pow(a, m.exponent);
+    const double fs_s = ksa > 0 ? (m.exponent + 2) / (2 * M_PI * ksa) * an : 0;
+    const double fs_d = kda > 0 ? 1.0 / (M_PI * kda) : 0;
+    return m.kd * fs_d + m.ks * fs_s;
+  }
+  return V3();   // a dielectric is a delta distribution: zero off its events
+}
+
+// pdf of a direction (projected solid angle measure) for diffuse and Phong.
+double bsdf_pdf_diffuse(V3 n, V3 wi, V3 wo) {
+  // DiffuseBSDF::getPd (DiffuseBSDF.cpp:30-38)
+  if (dot(n, wi) > 0) return 0;
+  return dot(n, wo) > 0 ? 1.0 / M_PI : 0;
+}
+
+BsdfSample bsdf_sample(const Material &m, V3 n, V3 wi, const double u[4], uint32_t ior_index) {
+  BsdfSample s;
+  if (m.type == MAT_DIFFUSE) {
+    // DiffuseBSDF::sample (DiffuseBSDF.cpp:21-28)
+    if (dot(n, wi) > 0) return s;
+    s.wo = cos_random(n, u[0], u[1]);
+    s.pdf = bsdf_pdf_diffuse(n, wi, s.wo);
+    if (s.pdf > 0) s.fs = bsdf_eval(m, n, wi, s.wo) / s.pdf;
+    return s;
+  }
+  if (m.type == MAT_PHONG) {
+    // ModifiedPhongBSDF::sample / getPd (ModifiedPhongBSDF.cpp:84-162)
+    const double kda = avg(m.kd), ksa = avg(m.ks);
+    const double p0 = u[3] - kEps;
+    int event;
+    if (p0 <= kda) {
+      event = 1;
+      s.wo = cos_random(n, u[0], u[1]);
+    } else if (p0 <= kda + ksa) {
+      event = 2;
+      const double alpha = std::acos(std::pow(u[0], 1.0 / (m.exponent + 1)));
+      const double phi = 2.0 * M_PI * u[1];
+      const V3 R = reflect(wi, n);
+      V3 r = R, uu, vv;
+      if (std::fabs(dot(n, r)) < .95) {
+        uu = normalized(cross(r, n));
+        vv = cross(uu, r);
+      } else {
+        onb(r, vv, uu);
+        r = normalized(r);
+      }
+      const V3 w = from_spherical(alpha, phi);
+      const V3 wo = uu * w.x + r * w.y + vv * w.z;
+      s.wo = dot(n, wo) > 0 ? wo : R;
+    } else {
+      return s;   // absorbed
+    }
+    const double cosA = dot(n, s.wo);
+    if (cosA <= 0) { s.wo = V3(); return s; }
+    double pdf;
+    if (event == 2) {
+      const V3 r = reflect(wi, n);
+      const double ca = std::fabs(dot(r, s.wo));
+      pdf = std::min(1.0, (m.exponent + 1) / (2 * M_PI) * std::pow(ca, m.exponent));
+    } else {
+      pdf = std::fabs(dot(n, s.wo)) / M_PI;
+    }
+    s.pdf = pdf / cosA;
+    if (!(s.pdf > 0)) { s.wo = V3(); return s; }
+    s.fs = bsdf_eval(m, n, wi, s.wo) / s.pdf;
+    return s;
+  }
+  // DielectricBSDF::sample / getPd / evaluate (DielectricBSDF.cpp:55-158).
+  const V3 N = dot(n, wi) > 0 ? -n : n;
+  const double ior = m.ior;
+  const V3 wt = refract(wi, n, kAirIor, ior);
+  bool refracted = false;
+  if (!is_black(wt)) {
+    const double Fr = fresnel(wi, n, kAirIor, ior);
+    const double Fs = m.transparency * (1 - Fr);
+    if (u[3] < Fs) refracted = true;
+  }
+  s.pdf = 1;
+  if (refracted) {
+    s.wo = wt;
+    V3 sel;
+    sel[ior_index] = 1;   // only the path's IOR channel transmits (:139-150)
+    s.fs = mul(m.ks, sel);
+  } else {
+    s.wo = reflect(wi, N);
+    s.fs = m.ks;
+  }
+  return s;
+}
+
+// Camera ray: ThinLensCamera::getWorldRay with aperture 0 (PinholeCamera.cpp:
+// 22; ThinLensCamera.cpp:131-135, 181-239, 241-256).
+struct Camera {
+  V3 eye, u, v, n;
+  double tan_w, tan_h;
+  explicit Camera(const double *c) {
+    eye = V3(c[0], c[1], c[2]);
+    const V3 look(c[3], c[4], c[5]), up(c[6], c[7], c[8]);
+    n = -look;
+    u = cross(up, n);
+    n = normalized(n);
+    u = normalized(u);
+    v = cross(n, u);
+    const double ha = c[9] * (M_PI / 180);
+    tan_h = std::tan(ha / 2);
+    tan_w = c[10] * tan_h;
+  }
+  V3 dir(double x, double y) const {
+    const double nx = 2 * x - 1, ny = 1 - 2 * y;
+    return normalized(u * (nx * tan_w) + v * (ny * tan_h) - n);
+  }
+};
+
+// Sample position of (pixel, sample): the stratum of SuperSG +
+// JitteredSG (SuperSampleGenerator.cpp:48-91, JitteredSampleGenerator.cpp:
+// 30-31, core/Viewport.h:29-36).
+inline void film_position(const orc_job &job, uint32_t pixel, uint32_t sample, double u0, double u1,
+                          double &x, double &y) {
+  const uint32_t side = (uint32_t)std::sqrt((double)job.spp);
+  const uint32_t k = sample % (side * side);
+  const uint32_t sx = k % side, sy = k / side;
+  const uint32_t px = pixel % job.width, py = pixel / job.width;
+  x = (px + (sx + u0) / side) / job.width;
+  y = (py + (sy + u1) / side) / job.height;
+}
+
+struct PathCounters { uint64_t camera = 0, extension = 0, shadow = 0; };
+
+// PathTracer::_evaluate (renderers/renderers/PathTracer.cpp:28-116) as an
+// iteration, with DirectIllumination::evaluate (renderers/DirectIllumination.
+// cpp:46-136) at each non-specular vertex.
+V3 trace_path(const orc_scene &sc, const orc_job &job, const Camera &cam, uint32_t pixel,
+              uint32_t sample, PathCounters &ctr) {
+  const Uniforms cu = draw(job.seed, job.job_id, pixel, sample, kCameraDepth, 0);
+  double fx, fy;
+  film_position(job, pixel, sample, cu.u[0], cu.u[1], fx, fy);
+  Ray ray(cam.eye, cam.dir(fx, fy));
+  // PathTracer.cpp:32: one IOR channel per camera ray.
+  const uint32_t ior_index = (uint32_t)(((uint64_t)(cu.raw[2] >> 8) * 3) >> 24);
+  ctr.camera++;
+  V3 L, beta(1, 1, 1);
+  bool any_specular = false, d0_continued = false, emit_after_specular = false;
+  for (uint32_t depth = 0; depth < job.max_depth; ++depth) {
+    uint32_t gid;
+    ctr.extension++;
+    const double t = sc.closest(ray, gid);
+    if (!(t > kEps && t < kInf)) {
+      L = L + mul(beta, sc.env);   // Scene::getBackgroundRadiance (Scene.cpp:116-118)
+      break;
+    }
+    const V3 p = ray.o + ray.d * t;
+    const V3 n = sc.normal_at(gid, p);
+    const Material &m = sc.material_of(gid);
+    if (m.specular()) any_specular = true;
+    // PathTracer.cpp:50-69 -- emission at depth 0, or after a specular
+    // bounce that left the surface on the normal's side.
+    if (m.emitter() && dot(n, -ray.d) > 0) {
+      if (depth == 0) L = L + mul(beta, m.Le);
+      else if (emit_after_specular) L = L + mul(beta, m.Le) * (sc.shape_area(gid) / (t * t));
+    }
+    const Uniforms bu = draw(job.seed, job.job_id, pixel, sample, depth, 0);
+    if (!m.specular()) {
+      V3 direct;
+      for (uint32_t li = (uint32_t)sc.lights.size(); li--;) {
+        const uint32_t q = sc.lights[li];
+        const uint32_t lgid = sc.nt + sc.ns + q;
+        if (lgid == gid) continue;
+        const Quad &lq = sc.quads[q];
+        const Uniforms lu = draw(job.seed, job.job_id, pixel, sample, depth, 1 + li);
+        const V3 lp = lq.c + lq.ex * (lu.u[0] - 0.5) + lq.ez * (lu.u[1] - 0.5);   // Plane.cpp:91-93
+        V3 wo = lp - p;
+        const double dist = len(wo);
+        wo = wo / dist;
+        const double cos_wo = std::fabs(dot(n, wo));
+        const double cos_wi = std::fabs(dot(lq.n, wo));
+        const V3 fr = bsdf_eval(m, n, ray.d, wo);
+        if (cos_wo > 0 && cos_wi > 0 && !is_black(fr)) {
+          ctr.shadow++;
+          if (!sc.occluded(Ray(p, wo), dist - kEps)) {
+            const Material &lm = sc.mats[sc.quad_mat[q]];
+            const V3 Le = dot(lq.n, -wo) > 0 ? lm.Le : V3();   // OrientedEmitter.cpp:17-23
+            direct = direct + mul(fr, Le) * (cos_wo * cos_wi / (dist * dist)) * lq.area;
+          }
+        }
+      }
+      L = L + mul(beta, direct);
+    }
+    const BsdfSample bs = bsdf_sample(m, n, ray.d, bu.u, ior_index);
+    if (is_black(bs.wo)) break;
+    // PathTracer.cpp:88-92
+    const double p_cont = depth < 2 ? (is_black(bs.fs) ? 0.0 : 1.0) : std::min(.95, max_comp(bs.fs));
+    if (!(bu.u[2] < p_cont)) break;
+    if (depth + 1 >= job.max_depth) break;
+    if (depth == 0) d0_continued = true;
+    emit_after_specular = m.specular() && dot(n, bs.wo) > kEps;
+    beta = mul(beta, bs.fs) / p_cont;
+    ray = Ray(p, bs.wo);
+  }
+  // PathTracer.cpp:107-114 -- with any specular vertex, keep the path's IOR
+  // channel only, scaled by the channel count.
+  if (any_specular && d0_continued) {
+    V3 r;
+    r[ior_index] = 3 * L[ior_index];
+    L = r;
+  }
+  return L;
+}
+
+template <class F>
+void parallel_for(uint64_t n, int nthreads, uint64_t chunk, F &&f) {
+  if (nthreads <= 1 || n <= chunk) {
+    f(0, n, 0);
+    return;
+  }
+  std::atomic<uint64_t> next(0);
+  std::vector<std::thread> th;
+  for (int t = 0; t < nthreads; ++t) {
+    th.emplace_back([&, t]() {
+      for (;;) {
+        const uint64_t b = next.fetch_add(chunk);
+        if (b >= n) break;
+        f(b, std::min(n, b + chunk), t);
+      }
+    });
+  }
+  for (auto &x : th) x.join();
+}
+
+}  // namespace
+
+extern "C" {
+
+void orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]) {
+  philox(ctr, key, out);
+}
+
+orc_scene *orc_scene_create(uint32_t nv, const float *vertices, uint32_t nt, const uint32_t *indices,
+                            const uint32_t *tri_material, uint32_t ns, const float *spheres,
+                            const uint32_t *sphere_material, uint32_t nq, const float *quads,
+                            const uint32_t *quad_material, uint32_t nm, const float *materials,
+                            const float env_radiance[3], uint32_t kd_min_prims, uint32_t kd_max_depth,
+                            int use_naive, int nthreads) {
+  std::unique_ptr<orc_scene> s(new orc_scene());
+  s->nt = nt;
+  s->ns = ns;
+  s->nq = nq;
+  s->verts.resize(nv);
+  for (uint32_t i = 0; i < nv; ++i) s->verts[i] = V3(vertices[3 * i], vertices[3 * i + 1], vertices[3 * i + 2]);
+  s->idx.assign(indices, indices + 3ull * nt);
+  s->tri_mat.assign(tri_material, tri_material + nt);
+  for (uint32_t i = 0; i < 3ull * nt; ++i)
+    if (s->idx[i] >= nv) return nullptr;
+  for (uint32_t i = 0; i < nm; ++i) s->mats.push_back(load_material(materials + 16ull * i));
+  for (uint32_t i = 0; i < nt; ++i)
+    if (s->tri_mat[i] >= nm || s->mats[s->tri_mat[i]].emitter()) return nullptr;
+  // Mesh::computeNormals (shapes/Mesh.cpp:335-377): sum of unit face normals.
+  s->vnormals.assign(nv, V3());
+  for (uint32_t i = 0; i < nt; ++i) {
+    const uint32_t a = s->idx[3 * i], b = s->idx[3 * i + 1], c = s->idx[3 * i + 2];
+    const V3 fn = normalized(cross(s->verts[b] - s->verts[a], s->verts[c] - s->verts[a]));
+    s->vnormals[a] = s->vnormals[a] + fn;
+    s->vnormals[b] = s->vnormals[b] + fn;
+    s->vnormals[c] = s->vnormals[c] + fn;
+  }
+  for (auto &v : s->vnormals) v = normalized(v);
+  for (uint32_t i = 0; i < ns; ++i) {
+    s->spheres.push_back(Sphere{V3(spheres[4 * i], spheres[4 * i + 1], spheres[4 * i + 2]), spheres[4 * i + 3]});
+    if (sphere_material[i] >= nm || s->mats[sphere_material[i]].emitter()) return nullptr;
+  }
+  s->sph_mat.assign(sphere_material, sphere_material + ns);
+  for (uint32_t i = 0; i < nq; ++i) {
+    const float *q = quads + 9ull * i;
+    Quad Q;
+    Q.c = V3(q[0], q[1], q[2]);
+    Q.ex = V3(q[3], q[4], q[5]);
+    Q.ez = V3(q[6], q[7], q[8]);
+    Q.n = normalized(cross(Q.ez, Q.ex));
+    Q.rx = Q.ex / dot(Q.ex, Q.ex);
+    Q.rz = Q.ez / dot(Q.ez, Q.ez);
+    Q.area = len(cross(Q.ex, Q.ez));
+    s->quads.push_back(Q);
+    if (quad_material[i] >= nm) return nullptr;
+    if (s->mats[quad_material[i]].emitter()) s->lights.push_back(i);
+  }
+  s->quad_mat.assign(quad_material, quad_material + nq);
+  s->env = V3(env_radiance[0], env_radiance[1], env_radiance[2]);
+
+  KdParams kp;
+  if (kd_min_prims) kp.min_prims = kd_min_prims;
+  if (kd_max_depth) kp.max_depth = std::min<unsigned>(kd_max_depth, 60);
+  // MeshTriangle::getAABB (shapes/MeshTriangle.cpp:186-198)
+  std::vector<Aabb> tb(nt);
+  for (uint32_t i = 0; i < nt; ++i)
+    for (int k = 0; k < 3; ++k) tb[i].add(s->verts[s->idx[3 * i + k]]);
+  s->mesh_tree.build(std::move(tb), kp, use_naive != 0, nthreads);
+  std::vector<Aabb> top;
+  if (nt) {
+    top.push_back(s->mesh_tree.box);
+    s->top_kind.push_back(0);
+    s->top_index.push_back(0);
+  }
+  for (uint32_t i = 0; i < ns; ++i) {
+    Aabb b;
+    const Sphere &sp = s->spheres[i];
+    b.add(sp.c - V3(sp.r, sp.r, sp.r));
+    b.add(sp.c + V3(sp.r, sp.r, sp.r));
+    top.push_back(b);
+    s->top_kind.push_back(1);
+    s->top_index.push_back(i);
+  }
+  for (uint32_t i = 0; i < nq; ++i) {
+    const Quad &Q = s->quads[i];
+    Aabb b;
+    for (int a = -1; a <= 1; a += 2)
+      for (int c = -1; c <= 1; c += 2) b.add(Q.c + Q.ex * (0.5 * a) + Q.ez * (0.5 * c));
+    top.push_back(b);
+    s->top_kind.push_back(2);
+    s->top_index.push_back(i);
+  }
+  s->top_tree.build(std::move(top), kp, use_naive != 0, 1);
+  return s.release();
+}
+
+void orc_scene_destroy(orc_scene *s) { delete s; }
+
+void orc_kd_stats(const orc_scene *s, int check, uint64_t out[6]) {
+  const KdTree &k = s->mesh_tree;
+  uint64_t maxleaf = 0;
+  for (const KdNode &n : k.nodes)
+    if (n.axis == 3) maxleaf = std::max<uint64_t>(maxleaf, n.count);
+  out[0] = k.n_internal;
+  out[1] = k.n_leaves;
+  out[2] = k.max_depth_seen;
+  out[3] = k.lists.size();
+  out[4] = maxleaf;
+  out[5] = check ? k.check_invariant() : 0;
+}
+
+void orc_trace_closest(const orc_scene *s, uint64_t n, const double *rays, double *t_out, uint32_t *id_out,
+                       int nthreads) {
+  parallel_for(n, nthreads, 4096, [&](uint64_t b, uint64_t e, int) {
+    for (uint64_t i = b; i < e; ++i) {
+      const double *r = rays + 6 * i;
+      uint32_t gid;
+      const double t = s->closest(Ray(V3(r[0], r[1], r[2]), V3(r[3], r[4], r[5])), gid);
+      const bool hit = t > kEps && t < kInf;
+      t_out[i] = hit ? t : kInf;
+      id_out[i] = hit ? gid : ORC_MISS;
+    }
+  });
+}
+
+void orc_trace_any(const orc_scene *s, uint64_t n, const double *rays, const double *tmax, uint8_t *occ,
+                   int nthreads) {
+  parallel_for(n, nthreads, 4096, [&](uint64_t b, uint64_t e, int) {
+    for (uint64_t i = b; i < e; ++i) {
+      const double *r = rays + 6 * i;
+      occ[i] = s->occluded(Ray(V3(r[0], r[1], r[2]), V3(r[3], r[4], r[5])), tmax[i]) ? 1 : 0;
+    }
+  });
+}
+
+void orc_render(const orc_scene *s, const orc_job *job, double *sum, double *sumsq, uint64_t *counters,
+                int nthreads) {
+  const Camera cam(job->camera);
+  const uint64_t np = job->pix_end - job->pix_begin;
+  const int nt = std::max(1, nthreads);
+  std::vector<PathCounters> pc(nt);
+  parallel_for(np, nt, 64, [&](uint64_t b, uint64_t e, int tid) {
+    for (uint64_t i = b; i < e; ++i) {
+      const uint32_t pixel = job->pix_begin + (uint32_t)i;
+      V3 acc, acc2;
+      for (uint32_t smp = 0; smp < job->spp; ++smp) {
+        const V3 L = trace_path(*s, *job, cam, pixel, smp, pc[tid]);
+        acc = acc + L;
+        acc2 = acc2 + mul(L, L);
+      }
+      for (int c = 0; c < 3; ++c) {
+        sum[3 * i + c] = acc[c];
+        if (sumsq) sumsq[3 * i + c] = acc2[c];
+      }
+    }
+  });
+  if (counters) {
+    PathCounters tot;
+    for (auto &c : pc) {
+      tot.camera += c.camera;
+      tot.extension += c.extension;
+      tot.shadow += c.shadow;
+    }
+    counters[0] = tot.camera;
+    counters[1] = tot.extension;
+    counters[2] = tot.shadow;
+    counters[3] = np * job->spp;
+  }
+}
+
+void orc_render_paths(const orc_scene *s, const orc_job *job, uint64_t n, const uint32_t *pixel,
+                      const uint32_t *sample, double *L_out, int nthreads) {
+  const Camera cam(job->camera);
+  parallel_for(n, std::max(1, nthreads), 1024, [&](uint64_t b, uint64_t e, int) {
+    PathCounters pc;
+    for (uint64_t i = b; i < e; ++i) {
+      const V3 L = trace_path(*s, *job, cam, pixel[i], sample[i], pc);
+      L_out[3 * i] = L.x;
+      L_out[3 * i + 1] = L.y;
+      L_out[3 * i + 2] = L.z;
+    }
+  });
+}
+
+void orc_camera_ray(const orc_job *job, uint32_t pixel, uint32_t sample, double *ray_out) {
+  const Camera cam(job->camera);
+  const Uniforms cu = draw(job->seed, job->job_id, pixel, sample, kCameraDepth, 0);
+  double fx, fy;
+  film_position(*job, pixel, sample, cu.u[0], cu.u[1], fx, fy);
+  const V3 d = cam.dir(fx, fy);
+  ray_out[0] = cam.eye.x; ray_out[1] = cam.eye.y; ray_out[2] = cam.eye.z;
+  ray_out[3] = d.x; ray_out[4] = d.y; ray_out[5] = d.z;
+}
+
+void orc_bsdf_sample(const float *material, const double *n, const double *wi, const double *u,
+                     uint32_t ior_index, double *wo, double *fs, double *pdf, int *specular) {
+  const Material m = load_material(material);
+  const BsdfSample s = bsdf_sample(m, V3(n[0], n[1], n[2]), V3(wi[0], wi[1], wi[2]), u, ior_index);
+  for (int i = 0; i < 3; ++i) { wo[i] = s.wo[i]; fs[i] = s.fs[i]; }
+  *pdf = s.pdf;
+  *specular = m.specular() ? 1 : 0;
+}
+
+void orc_bsdf_eval(const float *material, const double *n, const double *wi, const double *wo,
+                   double *f, double *pdf) {
+  const Material m = load_material(material);
+  const V3 N(n[0], n[1], n[2]), WI(wi[0], wi[1], wi[2]), WO(wo[0], wo[1], wo[2]);
+  const V3 r = bsdf_eval(m, N, WI, WO);
+  for (int i = 0; i < 3; ++i) f[i] = r[i];
+  if (m.type == MAT_DIFFUSE) {
+    *pdf = bsdf_pdf_diffuse(N, WI, WO);
+  } else if (m.type == MAT_PHONG) {
+    // Mixture pdf of the two lobes over projected solid angle.
+    const double kda = avg(m.kd), ksa = avg(m.ks);
+    const double cosA = dot(N, WO);
+    if (cosA <= 0) { *pdf = 0; return; }
+    const double ca = std::fabs(dot(reflect(WI, N), WO));
+    double spec = std::min(1.0, (m.exponent + 1) / (2 * M_PI) * std::pow(ca, m.exponent));
+    *pdf = (kda * (cosA / M_PI) + ksa * spec) / cosA;
+  } else {
+    *pdf = 0;
+  }
+}
+
+double orc_fresnel(const double *wi, const double *n, double n1, double n2) {
+  return fresnel(V3(wi[0], wi[1], wi[2]), V3(n[0], n[1], n[2]), n1, n2);
+}
+
+}  // extern "C"

@@ -1,0 +1,107 @@
+/*
+ * oracle_pt.h -- C interface of the CPU oracle (test infrastructure only).
+ *
+ * The oracle is a double-precision CPU restatement of Milton's unidirectional
+ * path tracer (mlt/src/milton/renderers/renderers/PathTracer.cpp) and of every
+ * function it calls on the hot path: the SAH kd-tree build and traversal
+ * (accel/kdTreeAccel.cpp), the naive linear accel (accel/NaiveSpatialAccel.cpp),
+ * mesh/standalone triangle, plane and sphere intersection, the Lambertian,
+ * modified-Phong and dielectric BSDFs, oriented area emitters, next-event
+ * estimation (renderers/DirectIllumination.cpp), the jittered super-sample
+ * pattern and the pinhole camera.
+ *
+ * It exists to check the HIP product path. Only tests, the smoke check and the
+ * CPU-baseline leg of the benchmark load it; the product never does.
+ *
+ * Scene arrays use the same conventions as include/milton_pt.h:
+ *   vertices  float[3*nv]            triangle indices uint32[3*nt]
+ *   spheres   float[4*ns]  (cx,cy,cz,r)
+ *   quads     float[9*nq]  (center, ex, ez) -- the reference "plane" shape:
+ *             the unit plane y=0, |x|,|z|<=0.5 mapped by ex (object x) and
+ *             ez (object z); its normal is normalize(ez x ex)
+ *   materials float[16*nm] (type, kd[3], ks[3], exponent, ior, transparency,
+ *             Le[3], pad[3]);  type 0 diffuse, 1 modified Phong, 2 dielectric
+ * Global primitive ids: triangles [0,nt), spheres [nt,nt+ns), quads after.
+ */
+#ifndef ORACLE_PT_H_
+#define ORACLE_PT_H_
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ORC_MISS 0xFFFFFFFFu
+
+/* Philox4x32-10 block function (counter, key) -> 4 words. */
+void orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
+
+typedef struct orc_scene orc_scene;
+
+/* Builds the two-level scene (top kd-tree over shapes, a kd-tree over the
+ * mesh triangles) with the reference build parameters.  kd_min_prims /
+ * kd_max_depth of 0 select the reference defaults (3, 24).  use_naive != 0
+ * replaces both kd-trees by NaiveSpatialAccel.  Returns NULL on bad input. */
+orc_scene *orc_scene_create(uint32_t nv, const float *vertices,
+                            uint32_t nt, const uint32_t *indices, const uint32_t *tri_material,
+                            uint32_t ns, const float *spheres, const uint32_t *sphere_material,
+                            uint32_t nq, const float *quads, const uint32_t *quad_material,
+                            uint32_t nm, const float *materials,
+                            const float env_radiance[3],
+                            uint32_t kd_min_prims, uint32_t kd_max_depth,
+                            int use_naive, int nthreads);
+void orc_scene_destroy(orc_scene *s);
+
+/* kd statistics of the mesh tree: [internal, leaves, max_depth, refs,
+ * max_leaf_prims, invariant_violations (prims overlapping a leaf cell but not
+ * referenced by it; checked only when check != 0)] */
+void orc_kd_stats(const orc_scene *s, int check, uint64_t out[6]);
+
+/* Closest hit for n rays (rays = double[6*n]: origin, direction (unit)).
+ * t_out = INFINITY and id_out = ORC_MISS on a miss. */
+void orc_trace_closest(const orc_scene *s, uint64_t n, const double *rays,
+                       double *t_out, uint32_t *id_out, int nthreads);
+
+/* Any hit in (eps, tmax) following Scene::intersects (tmax - EPSILON). */
+void orc_trace_any(const orc_scene *s, uint64_t n, const double *rays,
+                   const double *tmax, uint8_t *occluded, int nthreads);
+
+/* Camera: eye[3], look[3], up[3], heightAngle (degrees), aspect. */
+typedef struct orc_job {
+  double camera[11];
+  uint32_t width, height, spp, max_depth;
+  uint32_t seed, job_id, pix_begin, pix_end;
+} orc_job;
+
+/* Renders pixels [pix_begin, pix_end) of the job.  sum/sumsq are
+ * double[3*(pix_end-pix_begin)] (sum over the spp samples of L and L^2 per
+ * channel); counters (optional, uint64[4]) = camera rays, extension rays
+ * (camera included), shadow rays, paths. */
+void orc_render(const orc_scene *s, const orc_job *job, double *sum, double *sumsq,
+                uint64_t *counters, int nthreads);
+
+/* Per-path radiance for explicit (pixel, sample) pairs: L_out double[3*n]. */
+void orc_render_paths(const orc_scene *s, const orc_job *job, uint64_t n,
+                      const uint32_t *pixel, const uint32_t *sample, double *L_out,
+                      int nthreads);
+
+/* Camera ray of (pixel, sample): ray_out double[6]. */
+void orc_camera_ray(const orc_job *job, uint32_t pixel, uint32_t sample, double *ray_out);
+
+/* BSDF sampling of one material at a hit, for distribution tests:
+ * normal n[3] (unit), incident wi[3]; u[4] uniforms; ior_index in {0,1,2}.
+ * Outputs wo[3] (zero if absorbed), fs[3] = f/pdf, pdf, specular flag. */
+void orc_bsdf_sample(const float *material, const double *n, const double *wi,
+                     const double *u, uint32_t ior_index, double *wo, double *fs,
+                     double *pdf, int *specular);
+/* BSDF evaluate f(wi, wo) and pdf of a reflected direction (projected measure). */
+void orc_bsdf_eval(const float *material, const double *n, const double *wi,
+                   const double *wo, double *f, double *pdf);
+/* Fresnel::getFr(wi, N, n1, n2). */
+double orc_fresnel(const double *wi, const double *n, double n1, double n2);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,190 @@
+"""ctypes binding of the CPU oracle (oracle/liboracle_pt.so).
+
+Test infrastructure only: tests, the smoke check and the benchmark's CPU
+baseline load this module.  The oracle restates the reference's path tracer in
+double precision; see oracle_pt.h for its functions and citations.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle_pt.so")
+MISS = 0xFFFFFFFF
+
+
+def build():
+    """Compiles the oracle with g++ (always recompiles)."""
+    tmp = LIB_PATH + ".tmp"
+    cmd = ["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-pthread", "-Wall",
+           os.path.join(HERE, "oracle_pt.cpp"), "-o", tmp]
+    print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+    os.replace(tmp, LIB_PATH)
+    return LIB_PATH
+
+
+class OrcJob(ctypes.Structure):
+    _fields_ = [("camera", ctypes.c_double * 11), ("width", ctypes.c_uint32), ("height", ctypes.c_uint32),
+                ("spp", ctypes.c_uint32), ("max_depth", ctypes.c_uint32), ("seed", ctypes.c_uint32),
+                ("job_id", ctypes.c_uint32), ("pix_begin", ctypes.c_uint32), ("pix_end", ctypes.c_uint32)]
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = ctypes.CDLL(LIB_PATH)
+        vp, u32, u64, i32 = ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint64, ctypes.c_int
+        L.orc_philox4x32_10.argtypes = [vp, vp, vp]
+        L.orc_scene_create.argtypes = [u32, vp, u32, vp, vp, u32, vp, vp, u32, vp, vp, u32, vp, vp, u32, u32, i32, i32]
+        L.orc_scene_create.restype = vp
+        L.orc_scene_destroy.argtypes = [vp]
+        L.orc_kd_stats.argtypes = [vp, i32, vp]
+        L.orc_trace_closest.argtypes = [vp, u64, vp, vp, vp, i32]
+        L.orc_trace_any.argtypes = [vp, u64, vp, vp, vp, i32]
+        L.orc_render.argtypes = [vp, ctypes.POINTER(OrcJob), vp, vp, vp, i32]
+        L.orc_render_paths.argtypes = [vp, ctypes.POINTER(OrcJob), u64, vp, vp, vp, i32]
+        L.orc_camera_ray.argtypes = [ctypes.POINTER(OrcJob), u32, u32, vp]
+        L.orc_bsdf_sample.argtypes = [vp, vp, vp, vp, u32, vp, vp, vp, vp]
+        L.orc_bsdf_eval.argtypes = [vp, vp, vp, vp, vp, vp]
+        L.orc_fresnel.argtypes = [vp, vp, ctypes.c_double, ctypes.c_double]
+        L.orc_fresnel.restype = ctypes.c_double
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return ctypes.c_void_p(a.ctypes.data) if a is not None and a.size else None
+
+
+def philox(ctr, key):
+    c = np.ascontiguousarray(ctr, np.uint32)
+    k = np.ascontiguousarray(key, np.uint32)
+    out = np.zeros(4, np.uint32)
+    lib().orc_philox4x32_10(_p(c), _p(k), _p(out))
+    return out
+
+
+def nthreads():
+    return max(1, min(16, os.cpu_count() or 1))
+
+
+class OracleScene:
+    """Two-level kd scene of the oracle built from SceneArrays."""
+
+    def __init__(self, arrays, naive=False, kd_min_prims=0, kd_max_depth=0, threads=None):
+        a = arrays
+        self._keep = [a.vertices, a.indices, a.tri_material, a.spheres, a.sphere_material, a.quads,
+                      a.quad_material]
+        self.mats = np.ascontiguousarray(a.materials_f32())
+        self.env = np.ascontiguousarray(a.env, np.float32)
+        self.threads = threads or nthreads()
+        self.h = lib().orc_scene_create(
+            len(a.vertices), _p(a.vertices), len(a.indices), _p(a.indices), _p(a.tri_material),
+            len(a.spheres), _p(a.spheres), _p(a.sphere_material), len(a.quads), _p(a.quads), _p(a.quad_material),
+            len(self.mats), _p(self.mats), _p(self.env), kd_min_prims, kd_max_depth, 1 if naive else 0,
+            self.threads)
+        if not self.h:
+            raise ValueError("oracle rejected the scene")
+
+    def kd_stats(self, check=False):
+        out = np.zeros(6, np.uint64)
+        lib().orc_kd_stats(self.h, 1 if check else 0, _p(out))
+        return dict(zip(["internal", "leaves", "max_depth", "refs", "max_leaf", "violations"], out.tolist()))
+
+    def trace_closest(self, rays):
+        """rays: float64 (n, 6) origin + unit direction -> (t, id)."""
+        rays = np.ascontiguousarray(rays, np.float64).reshape(-1, 6)
+        t = np.zeros(len(rays), np.float64)
+        ids = np.zeros(len(rays), np.uint32)
+        lib().orc_trace_closest(self.h, len(rays), _p(rays), _p(t), _p(ids), self.threads)
+        return t, ids
+
+    def trace_any(self, rays, tmax):
+        rays = np.ascontiguousarray(rays, np.float64).reshape(-1, 6)
+        tmax = np.ascontiguousarray(tmax, np.float64)
+        occ = np.zeros(len(rays), np.uint8)
+        lib().orc_trace_any(self.h, len(rays), _p(rays), _p(tmax), _p(occ), self.threads)
+        return occ
+
+    def render(self, job, pix_begin=0, pix_end=None, threads=None):
+        """Per-pixel sum and sum of squares over the job's samples + counters."""
+        j = make_job(job, pix_begin, pix_end)
+        n = j.pix_end - j.pix_begin
+        s = np.zeros((n, 3), np.float64)
+        s2 = np.zeros((n, 3), np.float64)
+        ctr = np.zeros(4, np.uint64)
+        lib().orc_render(self.h, ctypes.byref(j), _p(s), _p(s2), _p(ctr), threads or self.threads)
+        return s, s2, dict(zip(["camera", "extension", "shadow", "paths"], ctr.tolist()))
+
+    def render_paths(self, job, pixels, samples, threads=None):
+        j = make_job(job, 0, None)
+        px = np.ascontiguousarray(pixels, np.uint32)
+        sm = np.ascontiguousarray(samples, np.uint32)
+        L = np.zeros((len(px), 3), np.float64)
+        lib().orc_render_paths(self.h, ctypes.byref(j), len(px), _p(px), _p(sm), _p(L), threads or self.threads)
+        return L
+
+    def close(self):
+        if self.h:
+            lib().orc_scene_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def make_job(job, pix_begin=0, pix_end=None):
+    """OrcJob from a milton_amd.Job (same camera, film, spp, depth, seed)."""
+    j = OrcJob()
+    c = job.camera
+    vals = list(c.eye) + list(c.look) + list(c.up) + [c.height_angle, c.aspect]
+    for i, v in enumerate(vals):
+        j.camera[i] = v
+    j.width, j.height, j.spp, j.max_depth = job.width, job.height, job.spp, job.max_depth
+    j.seed, j.job_id = job.seed, job.job_id
+    j.pix_begin = pix_begin
+    j.pix_end = job.width * job.height if pix_end is None else pix_end
+    return j
+
+
+def camera_ray(job, pixel, sample):
+    out = np.zeros(6, np.float64)
+    lib().orc_camera_ray(ctypes.byref(make_job(job)), pixel, sample, _p(out))
+    return out
+
+
+def bsdf_sample(material_row, n, wi, u, ior_index=0):
+    m = np.ascontiguousarray(material_row, np.float32)
+    n = np.ascontiguousarray(n, np.float64)
+    wi = np.ascontiguousarray(wi, np.float64)
+    u = np.ascontiguousarray(u, np.float64)
+    wo, fs = np.zeros(3), np.zeros(3)
+    pdf, spec = ctypes.c_double(), ctypes.c_int()
+    lib().orc_bsdf_sample(_p(m), _p(n), _p(wi), _p(u), ior_index, _p(wo), _p(fs), ctypes.byref(pdf),
+                          ctypes.byref(spec))
+    return wo, fs, pdf.value, bool(spec.value)
+
+
+def bsdf_eval(material_row, n, wi, wo):
+    m = np.ascontiguousarray(material_row, np.float32)
+    f = np.zeros(3)
+    pdf = ctypes.c_double()
+    lib().orc_bsdf_eval(_p(m), _p(np.ascontiguousarray(n, np.float64)), _p(np.ascontiguousarray(wi, np.float64)),
+                        _p(np.ascontiguousarray(wo, np.float64)), _p(f), ctypes.byref(pdf))
+    return f, pdf.value
+
+
+def fresnel(wi, n, n1, n2):
+    return lib().orc_fresnel(_p(np.ascontiguousarray(wi, np.float64)), _p(np.ascontiguousarray(n, np.float64)),
+                             n1, n2)
