@@ -1,0 +1,158 @@
+"""Benchmark of the MI355X path-tracing core on the kd-tree stress scene.
+
+Workload (one step = one whole render job): 1,000,000 random triangles in the
+unit cube (lognormal edge lengths, mu = ln 0.005, sigma = 0.5), diffuse albedo
+0.5, constant environment radiance 1, rendered at 1920x1080 with 64 samples per
+pixel and maximum path depth 4, FP32.  The scene and its kd-tree are resident
+in HBM before the timed region; the film stays on the device.
+
+Prints one JSON line: paths/s (whole job), Mrays/s, the roofline of the
+closest-hit kernel (algorithmic bytes per launch over its event-timed average
+duration, against 8 TB/s HBM) and a CPU baseline: the double-precision oracle
+restatement of the reference PathTracer on a bounded random sample of the same
+job's pixels, on the host's cores.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+WIDTH, HEIGHT, SPP, DEPTH = 1920, 1080, 64, 4
+TRIANGLES = 1_000_000
+HBM_PEAK_GBS = 8000.0
+NODE_BYTES, RECORD_BYTES, RAY_BYTES, HIT_BYTES = 8, 48, 32, 8
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(arrays, job, seconds, threads):
+    """Oracle (double precision) on a random pixel sample of the same job."""
+    from oracle import pyoracle as po
+    t0 = time.perf_counter()
+    osc = po.OracleScene(arrays, threads=threads)
+    build_s = time.perf_counter() - t0
+    rng = np.random.default_rng(12345)
+    npix = job.width * job.height
+
+    def run(count):
+        pix = np.sort(rng.choice(npix, count, replace=False)).astype(np.uint32)
+        px = np.repeat(pix, job.spp)
+        sm = np.tile(np.arange(job.spp, dtype=np.uint32), count)
+        t = time.perf_counter()
+        osc.render_paths(job, px, sm, threads=threads)
+        return time.perf_counter() - t, count * job.spp
+
+    dt, paths = run(256)
+    rate = paths / dt
+    count = int(min(npix, max(256, rate * seconds / job.spp)))
+    dt, paths = run(count)
+    osc.close()
+    return {"value": paths / dt, "unit": "paths/s", "cores": threads, "kind": "port",
+            "sample": f"{count} random pixels x {job.spp} spp of the same job ({paths} paths, {dt:.1f} s); "
+                      f"oracle kd build {build_s:.1f} s excluded"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--batch-paths", type=int, default=0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU: this build runs on a single MI355X; --gpus {args.gpus} is not "
+              f"supported (paths shard by pixel range; see the design notes)", file=sys.stderr)
+        sys.exit(2)
+
+    import milton_amd as m
+    ctx = m.Context(0)
+    if args.batch_paths:
+        ctx.set_option(m.MPT_OPT_BATCH_PATHS, args.batch_paths)
+    t0 = time.perf_counter()
+    arrays = m.generate(m.MPT_SCENE_SOUP, TRIANGLES, 1)
+    gen_s = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    scene = m.Scene(ctx, arrays)
+    build_s = time.perf_counter() - t0
+    st = scene.stats()
+    log(f"scene: gen {gen_s:.2f}s build+upload {build_s:.2f}s kd {st}")
+    job = m.make_job(arrays.camera, WIDTH, HEIGHT, SPP, DEPTH, seed=1)
+    film = m.Film(ctx, WIDTH, HEIGHT)
+
+    # Counter pass (untimed): kd nodes and references visited by every
+    # closest-hit ray of the job; the job is deterministic, so the timed runs
+    # trace exactly these rays.
+    ctx.set_option(m.MPT_OPT_COUNTERS, 1)
+    cst = film.render(scene, job)
+    ctx.set_option(m.MPT_OPT_COUNTERS, 0)
+    log(f"counters: {cst}")
+    ext_rays = cst["extension_rays"]
+    alg_bytes = (NODE_BYTES * cst["nodes_visited"] + RECORD_BYTES * cst["prims_tested"]
+                 + (RAY_BYTES + HIT_BYTES) * ext_rays)
+
+    for _ in range(args.warmup):
+        film.render(scene, job)
+    ctx.set_option(m.MPT_OPT_TIMING, 1)
+    ctx.sync()
+    t0 = time.perf_counter()
+    runs = [film.render(scene, job) for _ in range(args.steps)]
+    ctx.sync()
+    elapsed = time.perf_counter() - t0
+    ctx.set_option(m.MPT_OPT_TIMING, 0)
+    for r in runs:
+        log(f"step: {r}")
+    paths = WIDTH * HEIGHT * SPP
+    rays = runs[0]["extension_rays"] + runs[0]["shadow_rays"]
+    assert runs[0]["extension_rays"] == ext_rays
+    ms_step = 1000.0 * elapsed / args.steps
+    extend_ms = float(np.mean([r["extend_ms"] for r in runs]))
+    launches = runs[0]["extend_launches"]
+    achieved = alg_bytes / (extend_ms * 1e-3) / 1e9
+    out = {
+        "metric": "paths/s",
+        "value": paths * args.steps / elapsed,
+        "unit": "paths/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "mrays_per_s": rays * args.steps / elapsed / 1e6,
+        "config": {"workload": "kd-tree stress scene: 1M random triangles, 1920x1080, 64 spp, max depth 4",
+                   "triangles": TRIANGLES, "width": WIDTH, "height": HEIGHT, "spp": SPP, "max_depth": DEPTH,
+                   "rays_per_step": rays, "kd_nodes": st["nodes"], "kd_refs": st["refs"],
+                   "scene_build_s": round(build_s, 3)},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "k_extend", "alg_bytes_per_launch": alg_bytes / launches,
+                     "avg_launch_ms": extend_ms / launches,
+                     "nodes_per_ray": cst["nodes_visited"] / ext_rays,
+                     "refs_per_ray": cst["prims_tested"] / ext_rays},
+    }
+    if not args.no_cpu:
+        threads = min(16, os.cpu_count() or 1)
+        out["cpu_baseline"] = cpu_baseline(arrays, job, args.cpu_seconds, threads)
+    else:
+        out["cpu_baseline"] = None
+    film.close()
+    scene.close()
+    ctx.close()
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

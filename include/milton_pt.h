@@ -142,7 +142,17 @@ typedef struct mpt_render_stats {
   double shade_ms;
   double shadow_ms;
   uint64_t extend_launches;
+  uint64_t nodes_visited;  /* kd nodes fetched by closest-hit rays (MPT_OPT_COUNTERS) */
+  uint64_t prims_tested;   /* leaf references tested by closest-hit rays (MPT_OPT_COUNTERS) */
 } mpt_render_stats;
+
+/* Context options (mpt_set_option).  No reference counterpart: these are the
+ * knobs of the device path (queue depth) and its instrumentation. */
+enum {
+  MPT_OPT_BATCH_PATHS = 1, /* paths per wavefront batch (queue depth), default 2^23 */
+  MPT_OPT_TIMING = 2,      /* 1: time every kernel launch with HIP events into *_ms */
+  MPT_OPT_COUNTERS = 3     /* 1: count kd nodes and references visited by closest-hit rays */
+};
 
 typedef struct mpt_context mpt_context;
 typedef struct mpt_scene mpt_scene;
@@ -196,6 +206,8 @@ int mpt_render_paths(mpt_scene *scene, const mpt_job *job, uint32_t pix_begin, u
 
 /* Synchronises the context's stream. */
 int mpt_sync(mpt_context *ctx);
+/* Sets a context option (MPT_OPT_*). */
+int mpt_set_option(mpt_context *ctx, uint32_t option, uint64_t value);
 
 #ifdef __cplusplus
 }

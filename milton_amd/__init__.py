@@ -20,6 +20,7 @@ MPT_ABI_VERSION = 1
 MPT_MISS = 0xFFFFFFFF
 MPT_BSDF_DIFFUSE, MPT_BSDF_PHONG, MPT_BSDF_DIELECTRIC = 0, 1, 2
 MPT_SCENE_ROOM, MPT_SCENE_SOUP, MPT_SCENE_TERRAIN = 1, 2, 3
+MPT_OPT_BATCH_PATHS, MPT_OPT_TIMING, MPT_OPT_COUNTERS = 1, 2, 3
 ERRORS = {-1: "invalid", -2: "device", -3: "nomem", -4: "budget", -5: "notfound", -6: "cancelled"}
 
 _f32p = ctypes.POINTER(ctypes.c_float)
@@ -69,7 +70,8 @@ class RenderStats(ctypes.Structure):
                 ("extension_rays", ctypes.c_uint64), ("shadow_rays", ctypes.c_uint64),
                 ("render_ms", ctypes.c_double), ("extend_ms", ctypes.c_double),
                 ("shade_ms", ctypes.c_double), ("shadow_ms", ctypes.c_double),
-                ("extend_launches", ctypes.c_uint64)]
+                ("extend_launches", ctypes.c_uint64), ("nodes_visited", ctypes.c_uint64),
+                ("prims_tested", ctypes.c_uint64)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -103,6 +105,7 @@ def lib():
         "mpt_close": ([vp], i32),
         "mpt_last_error": ([vp], ctypes.c_char_p),
         "mpt_sync": ([vp], i32),
+        "mpt_set_option": ([vp, u32, u64], i32),
         "mpt_scene_create": ([vp, ctypes.POINTER(SceneDesc), ctypes.POINTER(KdParams), ctypes.POINTER(vp)], i32),
         "mpt_scene_destroy": ([vp], i32),
         "mpt_scene_stats": ([vp, ctypes.POINTER(ctypes.c_uint64)], i32),
@@ -271,6 +274,12 @@ class Context:
         if rc:
             msg = self.L.mpt_last_error(self.h)
             raise MptError(f"{what} failed ({ERRORS.get(rc, rc)}): {msg.decode() if msg else ''}")
+
+    def set_option(self, option, value):
+        self.check(self.L.mpt_set_option(self.h, option, value), "mpt_set_option")
+
+    def sync(self):
+        self.check(self.L.mpt_sync(self.h), "mpt_sync")
 
     def close(self):
         if self.h:

@@ -14,6 +14,7 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libmilton_pt.so")
 PLUGIN = os.path.join(HERE, "libmilton_pt_plugin.so")
+DRIVER = os.path.join(HERE, "mpt_plugin_driver")
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("MPT_OFFLOAD_ARCH", "gfx950")
@@ -46,6 +47,11 @@ def build(verbose=True):
         _run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-Wall", "-I", os.path.join(ROOT, "include"),
               plugin_src, "-o", tmp, "-L", HERE, "-lmilton_pt", "-Wl,-rpath,$ORIGIN", "-ldl"])
         os.replace(tmp, PLUGIN)
+        tmp = DRIVER + ".tmp"
+        _run(["g++", "-O2", "-std=c++17", "-Wall", "-I", os.path.join(ROOT, "include"),
+              os.path.join(CSRC, "plugin_driver.cpp"), "-o", tmp, "-L", HERE, "-lmilton_pt",
+              "-Wl,-rpath,$ORIGIN", "-ldl"])
+        os.replace(tmp, DRIVER)
     return LIB
 
 

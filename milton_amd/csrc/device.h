@@ -156,9 +156,11 @@ __device__ __forceinline__ float quad_t(const RaySetup &r, float3 c, float3 ex, 
 // [kStack][kBlock] uint2 array.  skip_origin is the primitive the ray leaves
 // (self-intersection guard in place of the reference's FP64 t > EPSILON),
 // skip_target a shadow ray's light.  ANY=true returns at the first hit.
-template <bool ANY>
+// COUNT=true adds the nodes fetched and references tested to n_nodes/n_refs.
+template <bool ANY, bool COUNT = false>
 __device__ __forceinline__ bool trace(const Scene &sc, const RaySetup &r, float tlimit, uint32_t skip_origin,
-                                      uint32_t skip_target, uint2 *stk, float &t_out, uint32_t &id_out) {
+                                      uint32_t skip_target, uint2 *stk, float &t_out, uint32_t &id_out,
+                                      uint32_t *n_nodes = nullptr, uint32_t *n_refs = nullptr) {
   // Root slab test (accel/AABB.inl:25-56).
   float tmin = 0.0f, tmax = tlimit;
   {
@@ -183,6 +185,7 @@ __device__ __forceinline__ bool trace(const Scene &sc, const RaySetup &r, float 
   bool dropped = false;
   for (;;) {
     uint2 nd = sc.nodes[node];
+    if (COUNT) ++*n_nodes;
     while ((nd.x & 3u) != 3u) {
       const uint32_t axis = nd.x & 3u;
       const float split = __uint_as_float(nd.y);
@@ -205,8 +208,10 @@ __device__ __forceinline__ bool trace(const Scene &sc, const RaySetup &r, float 
         tmax = tp;
       }
       nd = sc.nodes[node];
+      if (COUNT) ++*n_nodes;
     }
     const uint32_t count = nd.x >> 2;
+    if (COUNT) *n_refs += count;
     const float4 *rec = sc.recs + 3ull * nd.y;
     for (uint32_t k = 0; k < count; ++k, rec += 3) {
       const float4 a = rec[0], b = rec[1];

@@ -106,19 +106,27 @@ __global__ __launch_bounds__(256) void k_raygen(JobDev job, float4 *__restrict__
   flags[slot] = mpt_rng::third(r.z);
 }
 
-// Closest hit of the live queue (kdTreeAccel::getIntersection).
+// Closest hit of the live queue (kdTreeAccel::getIntersection).  With COUNT
+// the kernel also accumulates the kd nodes and references it visited.
+template <bool COUNT>
 __global__ __launch_bounds__(256) void k_extend(Scene sc, const float4 *__restrict__ q,
-                                                const uint32_t *__restrict__ count, float2 *__restrict__ hits) {
+                                                const uint32_t *__restrict__ count, float2 *__restrict__ hits,
+                                                unsigned long long *__restrict__ counters) {
   __shared__ uint2 stack[kStack * kBlock];
   uint2 *stk = stack + threadIdx.x;
   const uint32_t n = *count;
+  uint32_t n_nodes = 0, n_refs = 0;
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
     const float4 a = q[2 * i], b = q[2 * i + 1];
     const RaySetup r = setup_ray(xyz(a), xyz(b));
     float t;
     uint32_t id;
-    trace<false>(sc, r, INFINITY, __float_as_uint(b.w), kMiss, stk, t, id);
+    trace<false, COUNT>(sc, r, INFINITY, __float_as_uint(b.w), kMiss, stk, t, id, &n_nodes, &n_refs);
     hits[i] = make_float2(t, __uint_as_float(id));
+  }
+  if (COUNT) {
+    atomicAdd(&counters[0], (unsigned long long)n_nodes);
+    atomicAdd(&counters[1], (unsigned long long)n_refs);
   }
 }
 
@@ -441,6 +449,9 @@ struct mpt_context {
   std::string err;
   Workspace ws;
   uint32_t batch_paths = 1u << 23;   // paths per batch (queue depth)
+  bool timing = false;
+  bool counters = false;
+  std::vector<hipEvent_t> events;    // timing pool, pairs per launch
 };
 
 struct mpt_scene {
@@ -554,7 +565,7 @@ int ensure_workspace(mpt_context *ctx, uint32_t cap, uint32_t depth, uint32_t li
     MPT_HIP(ctx, w.counts.reserve(2ull * depth + 2));
     w.depth = std::max(w.depth, depth);
   }
-  MPT_HIP(ctx, w.totals.reserve(4));
+  MPT_HIP(ctx, w.totals.reserve(6));
   return MPT_OK;
 }
 
@@ -588,10 +599,32 @@ int render_range(mpt_scene *s, const mpt_job *job, uint32_t pix_begin, uint32_t 
   jd.side = std::max<uint32_t>(1, (uint32_t)std::sqrt((double)spp));
   while ((uint64_t)(jd.side + 1) * (jd.side + 1) <= spp) ++jd.side;
   while ((uint64_t)jd.side * jd.side > spp) --jd.side;
-  MPT_HIP(ctx, hipMemsetAsync(w.totals.p, 0, 4 * sizeof(unsigned long long), ctx->stream));
+  MPT_HIP(ctx, hipMemsetAsync(w.totals.p, 0, 6 * sizeof(unsigned long long), ctx->stream));
   MPT_HIP(ctx, hipEventRecord(ctx->ev0, ctx->stream));
   const uint32_t counts_n = 2 * D + 2;
   uint64_t launches = 0;
+  // Event pairs of timed launches: kind 0 extend, 1 shade, 2 shadow.
+  std::vector<std::pair<int, size_t>> timed;
+  size_t ev_used = 0;
+  auto tic = [&](int kind) -> int {
+    if (!ctx->timing) return MPT_OK;
+    if (ev_used + 2 > ctx->events.size()) {
+      for (int k = 0; k < 64; ++k) {
+        hipEvent_t e;
+        MPT_HIP(ctx, hipEventCreate(&e));
+        ctx->events.push_back(e);
+      }
+    }
+    timed.push_back({kind, ev_used});
+    MPT_HIP(ctx, hipEventRecord(ctx->events[ev_used], ctx->stream));
+    ev_used += 2;
+    return MPT_OK;
+  };
+  auto toc = [&]() -> int {
+    if (!ctx->timing) return MPT_OK;
+    MPT_HIP(ctx, hipEventRecord(ctx->events[timed.back().second + 1], ctx->stream));
+    return MPT_OK;
+  };
   for (uint32_t p0 = pix_begin; p0 < pix_end; p0 += batch_pix) {
     const uint32_t np = std::min(batch_pix, pix_end - p0);
     jd.pix0 = p0;
@@ -603,11 +636,22 @@ int render_range(mpt_scene *s, const mpt_job *job, uint32_t pix_begin, uint32_t 
     MPT_LAUNCHED(ctx, "k_raygen");
     for (uint32_t d = 0; d < D; ++d) {
       float4 *qin = w.q[d & 1].p, *qout = w.q[(d + 1) & 1].p;
-      k_extend<<<g, kBlock, 0, ctx->stream>>>(s->dev, qin, w.counts.p + d, w.hits.p);
+      if ((rc = tic(0))) return rc;
+      if (ctx->counters)
+        k_extend<true><<<g, kBlock, 0, ctx->stream>>>(s->dev, qin, w.counts.p + d, w.hits.p, w.totals.p + 3);
+      else
+        k_extend<false><<<g, kBlock, 0, ctx->stream>>>(s->dev, qin, w.counts.p + d, w.hits.p, nullptr);
+      if ((rc = toc())) return rc;
+      if ((rc = tic(1))) return rc;
       k_shade<<<g, kBlock, 0, ctx->stream>>>(s->dev, jd, d, qin, w.hits.p, w.counts.p + d, qout,
                                              w.counts.p + d + 1, w.sq.p, w.counts.p + D + 1 + d, w.Ls.p,
                                              w.betas.p, w.flags.p);
-      if (L) k_shadow<<<g, kBlock, 0, ctx->stream>>>(s->dev, w.sq.p, w.counts.p + D + 1 + d, w.Ls.p);
+      if ((rc = toc())) return rc;
+      if (L) {
+        if ((rc = tic(2))) return rc;
+        k_shadow<<<g, kBlock, 0, ctx->stream>>>(s->dev, w.sq.p, w.counts.p + D + 1 + d, w.Ls.p);
+        if ((rc = toc())) return rc;
+      }
       MPT_LAUNCHED(ctx, "depth kernels");
       ++launches;
     }
@@ -622,7 +666,7 @@ int render_range(mpt_scene *s, const mpt_job *job, uint32_t pix_begin, uint32_t 
     }
   }
   MPT_HIP(ctx, hipEventRecord(ctx->ev1, ctx->stream));
-  unsigned long long tot[4] = {0, 0, 0, 0};
+  unsigned long long tot[6] = {0, 0, 0, 0, 0, 0};
   MPT_HIP(ctx, hipMemcpyAsync(tot, w.totals.p, sizeof(tot), hipMemcpyDeviceToHost, ctx->stream));
   MPT_HIP(ctx, hipStreamSynchronize(ctx->stream));
   if (stats) {
@@ -635,6 +679,13 @@ int render_range(mpt_scene *s, const mpt_job *job, uint32_t pix_begin, uint32_t 
     stats->shadow_rays = tot[2];
     stats->render_ms = ms;
     stats->extend_launches = launches;
+    stats->nodes_visited = tot[3];
+    stats->prims_tested = tot[4];
+    for (const auto &e : timed) {
+      float t = 0;
+      MPT_HIP(ctx, hipEventElapsedTime(&t, ctx->events[e.second], ctx->events[e.second + 1]));
+      (e.first == 0 ? stats->extend_ms : e.first == 1 ? stats->shade_ms : stats->shadow_ms) += t;
+    }
   }
   return MPT_OK;
 }
@@ -670,12 +721,31 @@ int mpt_close(mpt_context *ctx) {
 
   if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
   if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
+  for (hipEvent_t e : ctx->events) (void)hipEventDestroy(e);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
   delete ctx;
   return MPT_OK;
 }
 
 const char *mpt_last_error(const mpt_context *ctx) { return ctx ? ctx->err.c_str() : "no context"; }
+
+int mpt_set_option(mpt_context *ctx, uint32_t option, uint64_t value) {
+  if (!ctx) return MPT_E_INVALID;
+  switch (option) {
+    case MPT_OPT_BATCH_PATHS:
+      if (value < 1024 || value > (1ull << 28)) return fail(ctx, MPT_E_INVALID, "batch paths out of [1024, 2^28]");
+      ctx->batch_paths = (uint32_t)value;
+      return MPT_OK;
+    case MPT_OPT_TIMING:
+      ctx->timing = value != 0;
+      return MPT_OK;
+    case MPT_OPT_COUNTERS:
+      ctx->counters = value != 0;
+      return MPT_OK;
+    default:
+      return fail(ctx, MPT_E_INVALID, "unknown option");
+  }
+}
 
 int mpt_sync(mpt_context *ctx) {
   if (!ctx) return MPT_E_INVALID;

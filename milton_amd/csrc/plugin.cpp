@@ -1,0 +1,137 @@
+// Milton "dynamic" renderer plugin: plugin_type / plugin_new / plugin_delete
+// (dynamic/DynamicPlugin.h:39-58) and the PathTracer facade over the C ABI.
+// See include/milton_pt_plugin.hpp.
+#include "milton_pt_plugin.hpp"
+
+#include <cmath>
+#include <cstring>
+#include <new>
+
+extern "C" {
+
+int plugin_type(void) { return MPT_PLUGIN_TYPE_RENDERER; }
+
+void *plugin_new(void) { return new (std::nothrow) milton_amd::PathTracer(); }
+
+void plugin_delete(void *plugin) { delete static_cast<milton_amd::PathTracer *>(plugin); }
+
+}  // extern "C"
+
+namespace milton_amd {
+
+PathTracer::PathTracer() {
+  // Reference defaults: efficientDirect (PathTracer.cpp:23), one direct
+  // sample (DirectIllumination.cpp:33), 4 super samples
+  // (SuperSampleGenerator.cpp:26-28), 480 x 480 output (RenderOutput.cpp:34),
+  // kd build parameters (kdTreeAccel.h:97-104).
+  props_ = {{"efficientDirect", 1}, {"noDirectSamples", 1}, {"noSuperSamples", 4}, {"maxDepth", 8},
+            {"seed", 1}, {"device", 0}, {"kdMinPrimitives", 0}, {"kdMaxDepth", 0}, {"kdCostTraversal", 1},
+            {"kdEmptyBias", 0}, {"kdNoThreads", 0}, {"batchPaths", 0}};
+  camera_.look[2] = 1;
+  camera_.up[1] = 1;
+  camera_.height_angle = 45;   // ThinLensCamera defaults (ThinLensCamera.cpp:47-48)
+  camera_.aspect = 1;
+}
+
+PathTracer::~PathTracer() {
+  if (scene_) mpt_scene_destroy(scene_);
+  if (ctx_) mpt_close(ctx_);
+}
+
+void PathTracer::setValue(const std::string &key, double value) { props_[key] = value; }
+
+double PathTracer::getValue(const std::string &key, double fallback) const {
+  const auto it = props_.find(key);
+  return it == props_.end() ? fallback : it->second;
+}
+
+void PathTracer::setScene(const mpt_scene_desc &d) {
+  verts_.assign(d.vertices, d.vertices + 3ull * d.num_vertices);
+  idx_.assign(d.indices, d.indices + 3ull * d.num_triangles);
+  tri_mat_.assign(d.tri_material, d.tri_material + d.num_triangles);
+  spheres_.assign(d.spheres, d.spheres + 4ull * d.num_spheres);
+  sph_mat_.assign(d.sphere_material, d.sphere_material + d.num_spheres);
+  quads_.assign(d.quads, d.quads + 9ull * d.num_quads);
+  quad_mat_.assign(d.quad_material, d.quad_material + d.num_quads);
+  mats_.assign(d.materials, d.materials + d.num_materials);
+  desc_ = d;
+  desc_.vertices = verts_.data();
+  desc_.indices = idx_.data();
+  desc_.tri_material = tri_mat_.data();
+  desc_.spheres = spheres_.data();
+  desc_.sphere_material = sph_mat_.data();
+  desc_.quads = quads_.data();
+  desc_.quad_material = quad_mat_.data();
+  desc_.materials = mats_.data();
+}
+
+void PathTracer::setCamera(const mpt_camera &camera) { camera_ = camera; }
+
+void PathTracer::setOutput(unsigned width, unsigned height) {
+  width_ = width;
+  height_ = height;
+  props_["width"] = width;
+  props_["height"] = height;
+}
+
+std::string PathTracer::lastError() const {
+  if (!err_.empty()) return err_;
+  return ctx_ ? mpt_last_error(ctx_) : std::string();
+}
+
+int PathTracer::init() {
+  err_.clear();
+  if (getValue("efficientDirect", 1) == 0) {
+    err_ = "efficientDirect = false is not supported by the device path";
+    return MPT_E_INVALID;
+  }
+  if (getValue("noDirectSamples", 1) != 1) {
+    err_ = "noDirectSamples other than 1 is not supported by the device path";
+    return MPT_E_INVALID;
+  }
+  if (desc_.version != MPT_ABI_VERSION) {
+    err_ = "setScene was not called";
+    return MPT_E_INVALID;
+  }
+  if (!ctx_) {
+    const int rc = mpt_open((int)getValue("device", 0), &ctx_);
+    if (rc) {
+      err_ = "mpt_open failed";
+      return rc;
+    }
+  }
+  const double batch = getValue("batchPaths", 0);
+  if (batch > 0) {
+    const int rc = mpt_set_option(ctx_, MPT_OPT_BATCH_PATHS, (uint64_t)batch);
+    if (rc) return rc;
+  }
+  if (scene_) {
+    mpt_scene_destroy(scene_);
+    scene_ = nullptr;
+  }
+  mpt_kd_params kp{};
+  kp.leaf_size = (uint32_t)getValue("kdMinPrimitives", 0);
+  kp.max_depth = (uint32_t)getValue("kdMaxDepth", 0);
+  kp.cost_traversal = (float)getValue("kdCostTraversal", 0);
+  kp.empty_bias = (float)getValue("kdEmptyBias", 0);
+  kp.num_threads = (uint32_t)getValue("kdNoThreads", 0);
+  return mpt_scene_create(ctx_, &desc_, &kp, &scene_);
+}
+
+int PathTracer::render() {
+  if (!scene_) {
+    err_ = "render before init";
+    return MPT_E_INVALID;
+  }
+  mpt_job job{};
+  job.camera = camera_;
+  job.width = (uint32_t)getValue("width", width_);
+  job.height = (uint32_t)getValue("height", height_);
+  job.spp = (uint32_t)getValue("noSuperSamples", 4);
+  job.max_depth = (uint32_t)getValue("maxDepth", 8);
+  job.seed = (uint32_t)getValue("seed", 1);
+  film_.assign(3ull * job.width * job.height, 0.0f);
+  return mpt_render(scene_, &job, film_.data(), &stats_);
+}
+
+}  // namespace milton_amd

@@ -1128,6 +1128,19 @@ void orc_trace_closest(const orc_scene *s, uint64_t n, const double *rays, doubl
   });
 }
 
+void orc_prim_t(const orc_scene *s, uint64_t n, const double *rays, const uint32_t *prim, double *t_out) {
+  for (uint64_t i = 0; i < n; ++i) {
+    const double *r = rays + 6 * i;
+    const Ray ray(V3(r[0], r[1], r[2]), V3(r[3], r[4], r[5]));
+    const uint32_t g = prim[i];
+    double t = kInf;
+    if (g < s->nt) t = s->tri_t(g, ray);
+    else if (g < s->nt + s->ns) t = s->sphere_t(g - s->nt, ray);
+    else if (g < s->nt + s->ns + s->nq) t = s->quad_t(g - s->nt - s->ns, ray);
+    t_out[i] = (t > kEps && t < kInf) ? t : kInf;
+  }
+}
+
 void orc_trace_any(const orc_scene *s, uint64_t n, const double *rays, const double *tmax, uint8_t *occ,
                    int nthreads) {
   parallel_for(n, nthreads, 4096, [&](uint64_t b, uint64_t e, int) {

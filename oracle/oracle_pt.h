@@ -63,6 +63,10 @@ void orc_kd_stats(const orc_scene *s, int check, uint64_t out[6]);
 void orc_trace_closest(const orc_scene *s, uint64_t n, const double *rays,
                        double *t_out, uint32_t *id_out, int nthreads);
 
+/* t of one primitive (global id) for n rays, INFINITY where it misses or
+ * t <= EPSILON: used to classify closest-hit disagreements as exact ties. */
+void orc_prim_t(const orc_scene *s, uint64_t n, const double *rays, const uint32_t *prim, double *t_out);
+
 /* Any hit in (eps, tmax) following Scene::intersects (tmax - EPSILON). */
 void orc_trace_any(const orc_scene *s, uint64_t n, const double *rays,
                    const double *tmax, uint8_t *occluded, int nthreads);

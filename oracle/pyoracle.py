@@ -49,6 +49,7 @@ def lib():
         L.orc_kd_stats.argtypes = [vp, i32, vp]
         L.orc_trace_closest.argtypes = [vp, u64, vp, vp, vp, i32]
         L.orc_trace_any.argtypes = [vp, u64, vp, vp, vp, i32]
+        L.orc_prim_t.argtypes = [vp, u64, vp, vp, vp]
         L.orc_render.argtypes = [vp, ctypes.POINTER(OrcJob), vp, vp, vp, i32]
         L.orc_render_paths.argtypes = [vp, ctypes.POINTER(OrcJob), u64, vp, vp, vp, i32]
         L.orc_camera_ray.argtypes = [ctypes.POINTER(OrcJob), u32, u32, vp]
@@ -106,6 +107,14 @@ class OracleScene:
         ids = np.zeros(len(rays), np.uint32)
         lib().orc_trace_closest(self.h, len(rays), _p(rays), _p(t), _p(ids), self.threads)
         return t, ids
+
+    def prim_t(self, rays, prims):
+        """FP64 t of the given primitive for each ray (INFINITY on a miss)."""
+        rays = np.ascontiguousarray(rays, np.float64).reshape(-1, 6)
+        prims = np.ascontiguousarray(prims, np.uint32)
+        t = np.zeros(len(rays), np.float64)
+        lib().orc_prim_t(self.h, len(rays), _p(rays), _p(prims), _p(t))
+        return t
 
     def trace_any(self, rays, tmax):
         rays = np.ascontiguousarray(rays, np.float64).reshape(-1, 6)
