@@ -161,6 +161,8 @@ typedef struct mpt_film mpt_film;
 /* Opens a context on a HIP device.  (No reference counterpart: the
  * reference has no device.) */
 int mpt_open(int device, mpt_context **ctx);
+/* Closes the context and destroys the scenes and films still open on it;
+ * their handles become invalid. */
 int mpt_close(mpt_context *ctx);
 const char *mpt_last_error(const mpt_context *ctx);
 int mpt_abi_version(void);

@@ -10,6 +10,7 @@ implementation: without the built library every entry point raises.
 """
 import ctypes
 import os
+import weakref
 
 import numpy as np
 
@@ -266,6 +267,7 @@ class Context:
     def __init__(self, device=0):
         self.L = lib()
         self.h = ctypes.c_void_p()
+        self._children = weakref.WeakSet()   # scenes and films, closed before the context
         rc = self.L.mpt_open(device, ctypes.byref(self.h))
         if rc:
             raise MptError(f"mpt_open({device}) failed: {ERRORS.get(rc, rc)}")
@@ -283,6 +285,8 @@ class Context:
 
     def close(self):
         if self.h:
+            for child in list(self._children):
+                child.close()
             self.L.mpt_close(self.h)
             self.h = ctypes.c_void_p()
 
@@ -303,6 +307,7 @@ class Scene:
         kp = kd if kd is not None else KdParams()
         ctx.check(ctx.L.mpt_scene_create(ctx.h, ctypes.byref(d), ctypes.byref(kp), ctypes.byref(self.h)),
                   "mpt_scene_create")
+        ctx._children.add(self)
 
     def stats(self):
         out = (ctypes.c_uint64 * 8)()
@@ -338,9 +343,9 @@ class Scene:
         return L
 
     def close(self):
-        if self.h:
+        if self.h and self.ctx.h:
             self.ctx.L.mpt_scene_destroy(self.h)
-            self.h = ctypes.c_void_p()
+        self.h = ctypes.c_void_p()
 
     def __del__(self):
         try:
@@ -356,6 +361,7 @@ class Film:
         self.ctx, self.width, self.height = ctx, width, height
         self.h = ctypes.c_void_p()
         ctx.check(ctx.L.mpt_film_create(ctx.h, width, height, ctypes.byref(self.h)), "mpt_film_create")
+        ctx._children.add(self)
 
     def render(self, scene, job):
         st = RenderStats()
@@ -369,9 +375,9 @@ class Film:
         return out
 
     def close(self):
-        if self.h:
+        if self.h and self.ctx.h:
             self.ctx.L.mpt_film_destroy(self.h)
-            self.h = ctypes.c_void_p()
+        self.h = ctypes.c_void_p()
 
     def __del__(self):
         try:

@@ -151,6 +151,39 @@ __device__ __forceinline__ float quad_t(const RaySetup &r, float3 c, float3 ex, 
   return (fabsf(u) <= 0.5f && fabsf(v) <= 0.5f) ? t : INFINITY;
 }
 
+// Recomputes the closest hit's t in double precision (one plane or sphere
+// solve per ray), so t matches the reference's FP64 intersect
+// (MeshTriangle.cpp:92-169 computes t in double) even for hits very close to
+// the ray origin, where the FP32 edge-function t loses relative accuracy.
+__device__ __forceinline__ float refine_t(const float4 *rec, const RaySetup &r, float t32) {
+  const float4 a = rec[0], b = rec[1], c = rec[2];
+  const uint32_t kind = __float_as_uint(b.w);
+  const double ox = r.o.x, oy = r.o.y, oz = r.o.z, dx = r.d.x, dy = r.d.y, dz = r.d.z;
+  if (kind == KIND_SPHERE) {
+    const double fx = ox - a.x, fy = oy - a.y, fz = oz - a.z, rad = b.x;
+    const double bb = fx * dx + fy * dy + fz * dz, dd = dx * dx + dy * dy + dz * dz;
+    const double cc = fx * fx + fy * fy + fz * fz - rad * rad;
+    const double disc = bb * bb - dd * cc;
+    if (disc < 0) return t32;
+    const double sq = sqrt(disc);
+    const double t0 = (-bb - sq) / dd, t1 = (-bb + sq) / dd;
+    // the root the FP32 test picked
+    return (float)(fabs(t0 - t32) <= fabs(t1 - t32) ? t0 : t1);
+  }
+  double e1x, e1y, e1z, e2x, e2y, e2z;
+  if (kind == KIND_TRIANGLE) {
+    e1x = (double)b.x - a.x; e1y = (double)b.y - a.y; e1z = (double)b.z - a.z;
+    e2x = (double)c.x - a.x; e2y = (double)c.y - a.y; e2z = (double)c.z - a.z;
+  } else {   // quad: plane through the centre spanned by ex, ez
+    e1x = b.x; e1y = b.y; e1z = b.z;
+    e2x = c.x; e2y = c.y; e2z = c.z;
+  }
+  const double nx = e1y * e2z - e1z * e2y, ny = e1z * e2x - e1x * e2z, nz = e1x * e2y - e1y * e2x;
+  const double dn = dx * nx + dy * ny + dz * nz;
+  if (dn == 0.0) return t32;
+  return (float)((((double)a.x - ox) * nx + ((double)a.y - oy) * ny + ((double)a.z - oz) * nz) / dn);
+}
+
 // kd traversal with a kStack-entry ring stack in LDS and kd-restart on
 // overflow (Foley & Sugerman 2005).  `stk` points at this lane's column of a
 // [kStack][kBlock] uint2 array.  skip_origin is the primitive the ray leaves
@@ -179,6 +212,7 @@ __device__ __forceinline__ bool trace(const Scene &sc, const RaySetup &r, float 
   const float troot = tmax;
   float best = tlimit;
   uint32_t best_id = kMiss;
+  uint32_t best_rec = 0;
   uint32_t node = 0;
   int depth = 0;     // valid entries on the ring
   int top = 0;       // next write slot
@@ -233,6 +267,7 @@ __device__ __forceinline__ bool trace(const Scene &sc, const RaySetup &r, float 
       if (t > kEps && t < best) {
         best = t;
         best_id = id;
+        best_rec = nd.y + k;
         if (ANY) {
           t_out = t;
           id_out = id;
@@ -259,7 +294,7 @@ __device__ __forceinline__ bool trace(const Scene &sc, const RaySetup &r, float 
     if (tmin > best) break;
   }
   if (best_id != kMiss) {
-    t_out = best;
+    t_out = refine_t(sc.recs + 3ull * best_rec, r, best);
     id_out = best_id;
     return true;
   }

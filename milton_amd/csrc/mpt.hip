@@ -452,6 +452,8 @@ struct mpt_context {
   bool timing = false;
   bool counters = false;
   std::vector<hipEvent_t> events;    // timing pool, pairs per launch
+  std::vector<mpt_scene *> scenes;   // live scenes and films, released by mpt_close
+  std::vector<mpt_film *> films;
 };
 
 struct mpt_scene {
@@ -718,6 +720,8 @@ int mpt_close(mpt_context *ctx) {
   if (!ctx) return MPT_E_INVALID;
   (void)hipSetDevice(ctx->device);
   (void)hipStreamSynchronize(ctx->stream);
+  while (!ctx->scenes.empty()) mpt_scene_destroy(ctx->scenes.back());
+  while (!ctx->films.empty()) mpt_film_destroy(ctx->films.back());
 
   if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
   if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
@@ -939,14 +943,17 @@ int mpt_scene_create(mpt_context *ctx, const mpt_scene_desc *d, const mpt_kd_par
   s->stats[5] = (uint64_t)ko.build_ms;
   s->stats[6] = ko.max_leaf;
   s->stats[7] = nt;
+  ctx->scenes.push_back(s.get());
   *out = s.release();
   return MPT_OK;
 }
 
 int mpt_scene_destroy(mpt_scene *s) {
   if (!s) return MPT_E_INVALID;
-  (void)hipSetDevice(s->ctx->device);
-  (void)hipStreamSynchronize(s->ctx->stream);
+  mpt_context *ctx = s->ctx;
+  (void)hipSetDevice(ctx->device);
+  (void)hipStreamSynchronize(ctx->stream);
+  ctx->scenes.erase(std::remove(ctx->scenes.begin(), ctx->scenes.end(), s), ctx->scenes.end());
   delete s;
   return MPT_OK;
 }
@@ -1002,13 +1009,16 @@ int mpt_film_create(mpt_context *ctx, uint32_t width, uint32_t height, mpt_film 
   f->height = height;
   MPT_HIP(ctx, f->data.reserve((size_t)width * height));
   MPT_HIP(ctx, hipMemsetAsync(f->data.p, 0, f->data.bytes(), ctx->stream));
+  ctx->films.push_back(f.get());
   *out = f.release();
   return MPT_OK;
 }
 
 int mpt_film_destroy(mpt_film *f) {
   if (!f) return MPT_E_INVALID;
-  (void)hipStreamSynchronize(f->ctx->stream);
+  mpt_context *ctx = f->ctx;
+  (void)hipStreamSynchronize(ctx->stream);
+  ctx->films.erase(std::remove(ctx->films.begin(), ctx->films.end(), f), ctx->films.end());
   delete f;
   return MPT_OK;
 }

@@ -43,7 +43,8 @@ def classify_hits(oracle_scene, rays, gpu_t, gpu_id, orc_t, orc_id):
         idx = diff[both_hit]
         if len(idx):
             t_alt[both_hit] = oracle_scene.prim_t(r[idx], gpu_id[idx])
-        tie = both_hit & (np.abs(t_alt - orc_t[diff]) <= 1e-6 * np.maximum(orc_t[diff], 1e-3))
+        with np.errstate(invalid="ignore"):
+            tie = both_hit & (np.abs(t_alt - orc_t[diff]) <= 1e-6 * np.maximum(orc_t[diff], 1e-3))
         non_tie = int((~tie).sum())
     hit = same & (orc_id != 0xFFFFFFFF)
     rel = np.abs(gpu_t[hit].astype(np.float64) - orc_t[hit]) / orc_t[hit]
