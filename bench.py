@@ -60,6 +60,22 @@ def cpu_baseline(arrays, job, seconds, threads):
                       f"oracle kd build {build_s:.1f} s excluded"}
 
 
+def copy_calibration(nbytes=256 << 20):
+    """A plain device-to-device copy of known size, the reference dispatch
+    that scales the FETCH_SIZE / WRITE_SIZE counters to bytes."""
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so")
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    assert hip.hipMalloc(ctypes.byref(a), ctypes.c_size_t(nbytes)) == 0
+    assert hip.hipMalloc(ctypes.byref(b), ctypes.c_size_t(nbytes)) == 0
+    assert hip.hipMemset(a, 1, ctypes.c_size_t(nbytes)) == 0
+    assert hip.hipMemcpy(b, a, ctypes.c_size_t(nbytes), 3) == 0   # hipMemcpyDeviceToDevice
+    assert hip.hipDeviceSynchronize() == 0
+    hip.hipFree(a)
+    hip.hipFree(b)
+    log(f"copy calibration: {nbytes} bytes device to device")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -68,6 +84,9 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--batch-paths", type=int, default=0)
+    ap.add_argument("--stack", type=int, default=0, help="traversal short-stack entries (8, 12, 16, 24)")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="also run a 256 MiB device-to-device copy (reference dispatch for HBM counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build runs on a single MI355X; --gpus {args.gpus} is not "
@@ -78,6 +97,10 @@ def main():
     ctx = m.Context(0)
     if args.batch_paths:
         ctx.set_option(m.MPT_OPT_BATCH_PATHS, args.batch_paths)
+    if args.stack:
+        ctx.set_option(m.MPT_OPT_STACK, args.stack)
+    if args.copy_calibration:
+        copy_calibration()
     t0 = time.perf_counter()
     arrays = m.generate(m.MPT_SCENE_SOUP, TRIANGLES, 1)
     gen_s = time.perf_counter() - t0

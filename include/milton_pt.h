@@ -151,7 +151,9 @@ typedef struct mpt_render_stats {
 enum {
   MPT_OPT_BATCH_PATHS = 1, /* paths per wavefront batch (queue depth), default 2^23 */
   MPT_OPT_TIMING = 2,      /* 1: time every kernel launch with HIP events into *_ms */
-  MPT_OPT_COUNTERS = 3     /* 1: count kd nodes and references visited by closest-hit rays */
+  MPT_OPT_COUNTERS = 3,    /* 1: count kd nodes and references visited by closest-hit rays */
+  MPT_OPT_GRAPHS = 4,      /* 1 (default): replay each batch's kernels from a captured hipGraph */
+  MPT_OPT_STACK = 5        /* traversal short-stack entries per ray: 8, 12, 16 (default) or 24 */
 };
 
 typedef struct mpt_context mpt_context;
@@ -205,6 +207,54 @@ int mpt_film_download(mpt_film *film, float *film_rgb);
  * Test hook: the value PathTracer::_evaluate returns for each camera ray. */
 int mpt_render_paths(mpt_scene *scene, const mpt_job *job, uint32_t pix_begin, uint32_t pix_end,
                      float *L_out);
+
+/* ------------------------------------------------------------------------
+ * Batching and serving.  No reference counterpart: the reference renders one
+ * job per Renderer::render call (renderers/PointSampleRenderer.cpp:24-81).
+ * A server keeps scenes resident, admits jobs while their films fit an HBM
+ * budget, and packs the remaining pixels of all admitted jobs that share a
+ * scene into common batches, so small and large jobs run together and new
+ * jobs join between batches.  Every pixel's samples stay in one batch and use
+ * the job's own random streams, so a job's film is bit-identical whether it
+ * runs alone or mixed with others.
+ * ------------------------------------------------------------------------ */
+typedef struct mpt_server mpt_server;
+
+typedef struct mpt_server_config {
+  uint64_t hbm_budget;   /* bytes for the films of admitted jobs (0 = 16 GiB) */
+  uint32_t max_active;   /* jobs admitted at once (0 = 256) */
+  uint32_t max_inflight; /* batches enqueued ahead of completion (0 = 2) */
+} mpt_server_config;
+
+enum { MPT_JOB_QUEUED = 0, MPT_JOB_RUNNING = 1, MPT_JOB_DONE = 2, MPT_JOB_CANCELLED = 3, MPT_JOB_FAILED = 4 };
+
+typedef struct mpt_job_status {
+  uint32_t state;
+  uint32_t error;          /* MPT_E_* of a failed job */
+  uint64_t pixels_done, pixels_total;
+  double submit_ms, start_ms, done_ms; /* server clock: ms since mpt_server_create */
+} mpt_job_status;
+
+int mpt_server_create(mpt_context *ctx, const mpt_server_config *cfg, mpt_server **server);
+int mpt_server_destroy(mpt_server *server);
+/* Queues a job on a resident scene.  MPT_E_BUDGET if its film alone exceeds
+ * the budget; otherwise it waits until admission.  Thread-safe. */
+int mpt_submit(mpt_server *server, mpt_scene *scene, const mpt_job *job, uint64_t *handle);
+int mpt_status(mpt_server *server, uint64_t handle, mpt_job_status *status);
+int mpt_cancel(mpt_server *server, uint64_t handle);
+/* Copies a finished job's film (RGB, W*H*3 floats) and releases the job. */
+int mpt_result(mpt_server *server, uint64_t handle, float *film_rgb);
+/* Blocks until the job leaves the queued/running states; returns 1 if
+ * timeout_ms passed first, 0 when it finished. */
+int mpt_wait(mpt_server *server, uint64_t handle, uint32_t timeout_ms);
+/* Synchronous driving: one scheduling round, or rounds until idle. */
+int mpt_server_step(mpt_server *server, uint32_t *batches_run);
+int mpt_server_drain(mpt_server *server);
+/* Asynchronous driving: a worker thread runs rounds until stopped. */
+int mpt_server_start(mpt_server *server);
+int mpt_server_stop(mpt_server *server);
+/* out: [batches, paths, extension rays, shadow rays, jobs done, peak film bytes] */
+int mpt_server_stats(mpt_server *server, uint64_t out[6]);
 
 /* Synchronises the context's stream. */
 int mpt_sync(mpt_context *ctx);

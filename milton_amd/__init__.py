@@ -21,7 +21,7 @@ MPT_ABI_VERSION = 1
 MPT_MISS = 0xFFFFFFFF
 MPT_BSDF_DIFFUSE, MPT_BSDF_PHONG, MPT_BSDF_DIELECTRIC = 0, 1, 2
 MPT_SCENE_ROOM, MPT_SCENE_SOUP, MPT_SCENE_TERRAIN = 1, 2, 3
-MPT_OPT_BATCH_PATHS, MPT_OPT_TIMING, MPT_OPT_COUNTERS = 1, 2, 3
+MPT_OPT_BATCH_PATHS, MPT_OPT_TIMING, MPT_OPT_COUNTERS, MPT_OPT_GRAPHS, MPT_OPT_STACK = 1, 2, 3, 4, 5
 ERRORS = {-1: "invalid", -2: "device", -3: "nomem", -4: "budget", -5: "notfound", -6: "cancelled"}
 
 _f32p = ctypes.POINTER(ctypes.c_float)
@@ -78,6 +78,21 @@ class RenderStats(ctypes.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class ServerConfig(ctypes.Structure):
+    _fields_ = [("hbm_budget", ctypes.c_uint64), ("max_active", ctypes.c_uint32), ("max_inflight", ctypes.c_uint32)]
+
+
+class JobStatus(ctypes.Structure):
+    _fields_ = [("state", ctypes.c_uint32), ("error", ctypes.c_uint32), ("pixels_done", ctypes.c_uint64),
+                ("pixels_total", ctypes.c_uint64), ("submit_ms", ctypes.c_double), ("start_ms", ctypes.c_double),
+                ("done_ms", ctypes.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+JOB_QUEUED, JOB_RUNNING, JOB_DONE, JOB_CANCELLED, JOB_FAILED = 0, 1, 2, 3, 4
+
 MATERIAL_DTYPE = np.dtype([("type", "<u4"), ("kd", "<f4", 3), ("ks", "<f4", 3), ("exponent", "<f4"),
                            ("ior", "<f4"), ("transparency", "<f4"), ("Le", "<f4", 3), ("pad", "<u4", 3)])
 assert MATERIAL_DTYPE.itemsize == ctypes.sizeof(Material) == 64
@@ -118,6 +133,18 @@ def lib():
         "mpt_render_film": ([vp, ctypes.POINTER(Job), vp, ctypes.POINTER(RenderStats)], i32),
         "mpt_film_download": ([vp, vp], i32),
         "mpt_render_paths": ([vp, ctypes.POINTER(Job), u32, u32, vp], i32),
+        "mpt_server_create": ([vp, ctypes.POINTER(ServerConfig), ctypes.POINTER(vp)], i32),
+        "mpt_server_destroy": ([vp], i32),
+        "mpt_submit": ([vp, vp, ctypes.POINTER(Job), ctypes.POINTER(ctypes.c_uint64)], i32),
+        "mpt_status": ([vp, u64, ctypes.POINTER(JobStatus)], i32),
+        "mpt_cancel": ([vp, u64], i32),
+        "mpt_result": ([vp, u64, vp], i32),
+        "mpt_wait": ([vp, u64, u32], i32),
+        "mpt_server_step": ([vp, ctypes.POINTER(ctypes.c_uint32)], i32),
+        "mpt_server_drain": ([vp], i32),
+        "mpt_server_start": ([vp], i32),
+        "mpt_server_stop": ([vp], i32),
+        "mpt_server_stats": ([vp, ctypes.POINTER(ctypes.c_uint64)], i32),
         "mpt_gen_create": ([u32, u32, u32, ctypes.POINTER(vp)], i32),
         "mpt_gen_describe": ([vp, ctypes.POINTER(SceneDesc), ctypes.POINTER(Camera)], i32),
         "mpt_gen_destroy": ([vp], i32),
@@ -377,6 +404,73 @@ class Film:
     def close(self):
         if self.h and self.ctx.h:
             self.ctx.L.mpt_film_destroy(self.h)
+        self.h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Server:
+    """Continuous-batching render server over resident scenes (mpt_server_*)."""
+
+    def __init__(self, ctx, hbm_budget=0, max_active=0, max_inflight=0):
+        self.ctx = ctx
+        self.h = ctypes.c_void_p()
+        cfg = ServerConfig(hbm_budget, max_active, max_inflight)
+        ctx.check(ctx.L.mpt_server_create(ctx.h, ctypes.byref(cfg), ctypes.byref(self.h)), "mpt_server_create")
+        ctx._children.add(self)
+
+    def submit(self, scene, job):
+        """Returns a handle; raises MptError (budget) if the job can never be admitted."""
+        h = ctypes.c_uint64()
+        self.ctx.check(self.ctx.L.mpt_submit(self.h, scene.h, ctypes.byref(job), ctypes.byref(h)), "mpt_submit")
+        return h.value
+
+    def status(self, handle):
+        st = JobStatus()
+        self.ctx.check(self.ctx.L.mpt_status(self.h, handle, ctypes.byref(st)), "mpt_status")
+        return st.as_dict()
+
+    def cancel(self, handle):
+        self.ctx.check(self.ctx.L.mpt_cancel(self.h, handle), "mpt_cancel")
+
+    def result(self, handle, width, height):
+        out = np.zeros((height, width, 3), np.float32)
+        self.ctx.check(self.ctx.L.mpt_result(self.h, handle, _ptr(out)), "mpt_result")
+        return out
+
+    def wait(self, handle, timeout_ms=600000):
+        rc = self.ctx.L.mpt_wait(self.h, handle, timeout_ms)
+        if rc < 0:
+            self.ctx.check(rc, "mpt_wait")
+        return rc == 0
+
+    def step(self):
+        n = ctypes.c_uint32()
+        self.ctx.check(self.ctx.L.mpt_server_step(self.h, ctypes.byref(n)), "mpt_server_step")
+        return n.value
+
+    def drain(self):
+        self.ctx.check(self.ctx.L.mpt_server_drain(self.h), "mpt_server_drain")
+
+    def start(self):
+        self.ctx.check(self.ctx.L.mpt_server_start(self.h), "mpt_server_start")
+
+    def stop(self):
+        self.ctx.check(self.ctx.L.mpt_server_stop(self.h), "mpt_server_stop")
+
+    def stats(self):
+        out = (ctypes.c_uint64 * 6)()
+        self.ctx.check(self.ctx.L.mpt_server_stats(self.h, out), "mpt_server_stats")
+        return dict(zip(["batches", "paths", "extension_rays", "shadow_rays", "jobs_done", "peak_film_bytes"],
+                        list(out)))
+
+    def close(self):
+        if self.h and self.ctx.h:
+            self.ctx.L.mpt_server_destroy(self.h)
         self.h = ctypes.c_void_p()
 
     def __del__(self):

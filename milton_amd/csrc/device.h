@@ -21,7 +21,7 @@ constexpr uint32_t kMiss = 0xFFFFFFFFu;
 constexpr float kEps = 1e-7f;            // EPSILON, common/arch.h:137
 constexpr float kAirIor = 1.0003f;       // IndexOfRefraction::AIR
 constexpr int kBlock = 256;              // threads per block of every traversal kernel
-constexpr int kStack = 16;               // short-stack entries per lane (LDS ring)
+constexpr int kStack = 16;               // default short-stack entries per lane (LDS ring)
 constexpr float kPi = 3.14159265358979323846f;
 
 enum { KIND_TRIANGLE = 0, KIND_SPHERE = 1, KIND_QUAD = 2 };
@@ -184,13 +184,13 @@ __device__ __forceinline__ float refine_t(const float4 *rec, const RaySetup &r, 
   return (float)((((double)a.x - ox) * nx + ((double)a.y - oy) * ny + ((double)a.z - oz) * nz) / dn);
 }
 
-// kd traversal with a kStack-entry ring stack in LDS and kd-restart on
+// kd traversal with a STACK-entry ring stack in LDS and kd-restart on
 // overflow (Foley & Sugerman 2005).  `stk` points at this lane's column of a
-// [kStack][kBlock] uint2 array.  skip_origin is the primitive the ray leaves
+// [STACK][kBlock] uint2 array.  skip_origin is the primitive the ray leaves
 // (self-intersection guard in place of the reference's FP64 t > EPSILON),
 // skip_target a shadow ray's light.  ANY=true returns at the first hit.
 // COUNT=true adds the nodes fetched and references tested to n_nodes/n_refs.
-template <bool ANY, bool COUNT = false>
+template <bool ANY, bool COUNT = false, int STACK = kStack>
 __device__ __forceinline__ bool trace(const Scene &sc, const RaySetup &r, float tlimit, uint32_t skip_origin,
                                       uint32_t skip_target, uint2 *stk, float &t_out, uint32_t &id_out,
                                       uint32_t *n_nodes = nullptr, uint32_t *n_refs = nullptr) {
@@ -235,8 +235,8 @@ __device__ __forceinline__ bool trace(const Scene &sc, const RaySetup &r, float 
         node = second;
       } else {
         stk[top * kBlock] = make_uint2(second, __float_as_uint(tmax));
-        top = (top + 1) & (kStack - 1);
-        if (depth == kStack) dropped = true;
+        top = top + 1 == STACK ? 0 : top + 1;
+        if (depth == STACK) dropped = true;
         else ++depth;
         node = first;
         tmax = tp;
@@ -286,7 +286,7 @@ __device__ __forceinline__ bool trace(const Scene &sc, const RaySetup &r, float 
       continue;
     }
     --depth;
-    top = (top + kStack - 1) & (kStack - 1);
+    top = top == 0 ? STACK - 1 : top - 1;
     const uint2 e = stk[top * kBlock];
     node = e.x;
     tmin = tmax;

@@ -1,0 +1,189 @@
+// Internal interface of the wavefront engine shared by the C ABI (mpt.hip)
+// and the batching / serving layer (serve.cpp).
+//
+// A batch is a set of path slots filled from one scene and any number of
+// jobs.  Its contents live in a device-side descriptor (job table + chunk
+// table); every kernel of a batch reads its sizes from that descriptor and
+// runs on a grid sized by the batch capacity, so the kernel sequence of a
+// batch is identical for every batch of a (scene, depth) pair and is captured
+// once into a hipGraph and replayed.
+#ifndef MPT_ENGINE_H_
+#define MPT_ENGINE_H_
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <algorithm>
+#include <string>
+#include <vector>
+
+#include "../../include/milton_pt.h"
+#include "device.h"
+
+namespace mpt_engine {
+
+constexpr uint32_t kMaxJobs = 256;      // jobs per batch
+constexpr uint32_t kMaxChunks = 1024;   // pixel ranges per batch
+constexpr uint32_t kJobBits = 20;       // sample index bits in a path's (job, sample) word
+
+struct CameraDev {
+  float eye[3], u[3], v[3], n[3];
+  float tan_w, tan_h;
+};
+
+struct JobDev {
+  CameraDev cam;
+  uint32_t width, height, spp, max_depth;
+  uint32_t seed, job_id, side, pad;
+  float4 *film;   // W*H (sum r, g, b, samples)
+};
+
+struct ChunkDev {
+  uint32_t job;     // index into the batch job table
+  uint32_t pix0;    // first pixel of the chunk in the job's film
+  uint32_t npix;
+  uint32_t slot0;   // first path slot of the chunk in the batch
+  uint32_t pixoff;  // first batch pixel of the chunk
+  uint32_t pad[3];
+};
+
+struct BatchHdr {
+  uint32_t n_chunks, n_paths, n_pixels, max_depth;
+};
+
+struct BatchDesc {
+  BatchHdr hdr;
+  JobDev jobs[kMaxJobs];
+  ChunkDev chunks[kMaxChunks];
+};
+
+template <class T>
+struct DBuf {
+  T *p = nullptr;
+  size_t n = 0;
+  DBuf() = default;
+  DBuf(const DBuf &) = delete;
+  DBuf &operator=(const DBuf &) = delete;
+  ~DBuf() { release(); }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+  hipError_t reserve(size_t count) {
+    if (count <= n && p) return hipSuccess;
+    release();
+    const hipError_t e = hipMalloc((void **)&p, std::max<size_t>(count, 1) * sizeof(T));
+    if (e == hipSuccess) n = count;
+    else p = nullptr;
+    return e;
+  }
+  size_t bytes() const { return n * sizeof(T); }
+};
+
+// Path and queue buffers, sized by the batch capacity.
+struct Workspace {
+  uint32_t cap = 0, depth = 0, lights = 0;
+  uint64_t generation = 0;   // bumped on every reallocation (invalidates graphs)
+  DBuf<float4> q[2];
+  DBuf<float2> hits;
+  DBuf<float4> sq;
+  DBuf<float4> Ls, betas;
+  DBuf<uint32_t> flags;
+  DBuf<uint2> pjs;           // (pixel, job << 20 | sample) per slot
+  DBuf<uint32_t> counts;
+  DBuf<unsigned long long> totals;
+  DBuf<BatchDesc> desc;
+  size_t bytes() const {
+    return q[0].bytes() + q[1].bytes() + hits.bytes() + sq.bytes() + Ls.bytes() + betas.bytes() + flags.bytes() +
+           pjs.bytes() + counts.bytes() + totals.bytes() + desc.bytes();
+  }
+};
+
+// One captured batch sequence.
+struct GraphEntry {
+  const void *scene = nullptr;
+  uint32_t depth = 0, cap = 0, lights = 0;
+  uint64_t generation = 0;
+  hipGraphExec_t exec = nullptr;
+};
+
+// A batch as planned on the host.
+struct BatchPlan {
+  std::vector<JobDev> jobs;
+  std::vector<ChunkDev> chunks;
+  uint32_t n_paths = 0, n_pixels = 0, max_depth = 0;
+};
+
+}  // namespace mpt_engine
+
+struct mpt_server;
+
+struct mpt_context {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  int num_cus = 256;
+  std::string err;
+  mpt_engine::Workspace ws;
+  uint32_t batch_paths = 1u << 23;   // paths per batch (queue depth)
+  bool timing = false;
+  bool counters = false;
+  bool graphs = true;
+  uint32_t stack = 16;               // short-stack entries of the traversal kernels
+  std::vector<hipEvent_t> events;    // timing pool, pairs per launch
+  std::vector<mpt_scene *> scenes;   // live scenes, films and servers, released by mpt_close
+  std::vector<mpt_film *> films;
+  std::vector<mpt_server *> servers;
+  std::vector<mpt_engine::GraphEntry> graphs_cache;
+  mpt_engine::BatchDesc *staging = nullptr;   // pinned ring of descriptors
+  hipEvent_t staging_ev[4] = {nullptr, nullptr, nullptr, nullptr};
+  uint32_t staging_next = 0;
+};
+
+struct mpt_scene {
+  mpt_context *ctx = nullptr;
+  uint64_t uid = 0;   // unique per process, keys the graph cache
+  mpt_engine::DBuf<uint2> nodes;
+  mpt_engine::DBuf<float4> recs, verts, vnormals, spheres, quads;
+  mpt_engine::DBuf<uint4> tris;
+  mpt_engine::DBuf<uint32_t> sphere_mat, lights;
+  mpt_engine::DBuf<mpt_material> mats;
+  mpt_dev::Scene dev{};
+  uint64_t stats[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  uint64_t device_bytes() const {
+    return nodes.bytes() + recs.bytes() + verts.bytes() + vnormals.bytes() + spheres.bytes() + quads.bytes() +
+           tris.bytes() + sphere_mat.bytes() + lights.bytes() + mats.bytes();
+  }
+};
+
+struct mpt_film {
+  mpt_context *ctx = nullptr;
+  uint32_t width = 0, height = 0;
+  mpt_engine::DBuf<float4> data;
+};
+
+namespace mpt_engine {
+
+int fail(mpt_context *ctx, int code, const std::string &msg);
+int check_job(mpt_context *ctx, const mpt_job *job);
+JobDev make_job(const mpt_job &job, float4 *film);
+
+// Makes sure the workspace holds a batch of `cap` paths, `depth` bounces and
+// `lights` shadow rays per vertex.
+int ensure_workspace(mpt_context *ctx, uint32_t cap, uint32_t depth, uint32_t lights);
+
+// Enqueues one batch on the context stream (asynchronous).  With L_host the
+// per-path radiance of the batch is copied to L_host (slot order) and the
+// call synchronises.  Uses a captured graph when the context allows it and
+// no per-launch instrumentation is on.
+int run_batch(mpt_context *ctx, mpt_scene *scene, const BatchPlan &plan, float *L_host);
+
+// Zeroes the device ray totals; read_totals syncs and returns
+// [camera, extension, shadow, nodes, refs].
+int reset_totals(mpt_context *ctx);
+int read_totals(mpt_context *ctx, unsigned long long out[5]);
+
+}  // namespace mpt_engine
+
+#endif

@@ -27,10 +27,12 @@
 
 #include "../../include/milton_pt.h"
 #include "device.h"
+#include "engine.h"
 #include "kdbuild.h"
 #include "rng.h"
 
 using namespace mpt_dev;
+using namespace mpt_engine;
 
 namespace {
 
@@ -39,20 +41,6 @@ constexpr uint32_t F_IOR_MASK = 3u;          // the path's IOR channel (PathTrac
 constexpr uint32_t F_ANY_SPECULAR = 4u;      // data["specular"] (PathTracer.cpp:46-47)
 constexpr uint32_t F_D0_CONTINUED = 8u;      // the depth-0 vertex recursed
 constexpr uint32_t F_EMIT_AFTER_SPEC = 16u;  // emittedContribution > EPSILON (PathTracer.cpp:97-98)
-
-struct CameraDev {
-  float eye[3], u[3], v[3], n[3];
-  float tan_w, tan_h;
-};
-
-struct JobDev {
-  CameraDev cam;
-  uint32_t width, height, spp, max_depth;
-  uint32_t seed, job_id;
-  uint32_t side;       // floor(sqrt(spp)) strata per axis (core/Viewport.h:29-36)
-  uint32_t pix0;       // first pixel of the batch
-  uint32_t n_paths;    // paths in the batch
-};
 
 __device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
 
@@ -77,42 +65,70 @@ __global__ void k_reset(uint32_t *counts, uint32_t n) {
   for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) counts[i] = 0;
 }
 
+__device__ __forceinline__ uint32_t chunk_of_slot(const BatchDesc *bd, uint32_t slot) {
+  uint32_t lo = 0, hi = bd->hdr.n_chunks - 1;
+  while (lo < hi) {
+    const uint32_t mid = (lo + hi + 1) >> 1;
+    if (bd->chunks[mid].slot0 <= slot) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+__device__ __forceinline__ uint32_t chunk_of_pixel(const BatchDesc *bd, uint32_t pix) {
+  uint32_t lo = 0, hi = bd->hdr.n_chunks - 1;
+  while (lo < hi) {
+    const uint32_t mid = (lo + hi + 1) >> 1;
+    if (bd->chunks[mid].pixoff <= pix) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
 // Camera rays: ThinLensCamera::getWorldRay with aperture 0
 // (cameras/ThinLensCamera.cpp:131-135) at the jittered stratum position of
 // SuperSG + JitteredSG (renderers/generators/SuperSampleGenerator.cpp:48-91,
-// JitteredSampleGenerator.cpp:30-31).
-__global__ __launch_bounds__(256) void k_raygen(JobDev job, float4 *__restrict__ q, float4 *__restrict__ Ls,
-                                                float4 *__restrict__ betas, uint32_t *__restrict__ flags,
+// JitteredSampleGenerator.cpp:30-31).  Slot -> (job, pixel, sample) through
+// the batch's chunk table.
+__global__ __launch_bounds__(256) void k_raygen(const BatchDesc *__restrict__ bd, float4 *__restrict__ q,
+                                                float4 *__restrict__ Ls, float4 *__restrict__ betas,
+                                                uint32_t *__restrict__ flags, uint2 *__restrict__ pjs,
                                                 uint32_t *__restrict__ count0) {
   const uint32_t slot = blockIdx.x * blockDim.x + threadIdx.x;
-  if (slot == 0) *count0 = job.n_paths;
-  if (slot >= job.n_paths) return;
-  const uint32_t pixel = job.pix0 + slot / job.spp, sample = slot % job.spp;
+  const uint32_t n = bd->hdr.n_paths;
+  if (slot == 0) *count0 = n;
+  if (slot >= n) return;
+  const ChunkDev ch = bd->chunks[chunk_of_slot(bd, slot)];
+  const JobDev &job = bd->jobs[ch.job];
+  const uint32_t spp = job.spp, side = job.side, width = job.width;
+  const uint32_t local = slot - ch.slot0;
+  const uint32_t pixel = ch.pix0 + local / spp, sample = local % spp;
   const mpt_rng::U4 r = mpt_rng::path_draw(job.seed, job.job_id, pixel, sample, mpt_rng::kCameraDepth, 0);
-  const uint32_t k = sample % (job.side * job.side);
-  const uint32_t sx = k % job.side, sy = k / job.side;
-  const uint32_t px = pixel % job.width, py = pixel / job.width;
-  const float inv_side = 1.0f / (float)job.side;
-  const float x = ((float)px + ((float)sx + mpt_rng::unitf(r.x)) * inv_side) / (float)job.width;
+  const uint32_t k = sample % (side * side);
+  const uint32_t sx = k % side, sy = k / side;
+  const uint32_t px = pixel % width, py = pixel / width;
+  const float inv_side = 1.0f / (float)side;
+  const float x = ((float)px + ((float)sx + mpt_rng::unitf(r.x)) * inv_side) / (float)width;
   const float y = ((float)py + ((float)sy + mpt_rng::unitf(r.y)) * inv_side) / (float)job.height;
-  const float nx = (2.0f * x - 1.0f) * job.cam.tan_w, ny = (1.0f - 2.0f * y) * job.cam.tan_h;
-  const float3 d = normalize(f3(job.cam.u[0] * nx + job.cam.v[0] * ny - job.cam.n[0],
-                                job.cam.u[1] * nx + job.cam.v[1] * ny - job.cam.n[1],
-                                job.cam.u[2] * nx + job.cam.v[2] * ny - job.cam.n[2]));
-  q[2 * slot] = make_float4(job.cam.eye[0], job.cam.eye[1], job.cam.eye[2], __uint_as_float(slot));
+  const CameraDev &cam = job.cam;
+  const float nx = (2.0f * x - 1.0f) * cam.tan_w, ny = (1.0f - 2.0f * y) * cam.tan_h;
+  const float3 d = normalize(f3(cam.u[0] * nx + cam.v[0] * ny - cam.n[0], cam.u[1] * nx + cam.v[1] * ny - cam.n[1],
+                                cam.u[2] * nx + cam.v[2] * ny - cam.n[2]));
+  q[2 * slot] = make_float4(cam.eye[0], cam.eye[1], cam.eye[2], __uint_as_float(slot));
   q[2 * slot + 1] = make_float4(d.x, d.y, d.z, __uint_as_float(kMiss));
   Ls[slot] = make_float4(0, 0, 0, 0);
   betas[slot] = make_float4(1, 1, 1, 0);
   flags[slot] = mpt_rng::third(r.z);
+  pjs[slot] = make_uint2(pixel, (ch.job << kJobBits) | sample);
 }
 
 // Closest hit of the live queue (kdTreeAccel::getIntersection).  With COUNT
 // the kernel also accumulates the kd nodes and references it visited.
-template <bool COUNT>
+template <bool COUNT, int STACK>
 __global__ __launch_bounds__(256) void k_extend(Scene sc, const float4 *__restrict__ q,
                                                 const uint32_t *__restrict__ count, float2 *__restrict__ hits,
                                                 unsigned long long *__restrict__ counters) {
-  __shared__ uint2 stack[kStack * kBlock];
+  __shared__ uint2 stack[STACK * kBlock];
   uint2 *stk = stack + threadIdx.x;
   const uint32_t n = *count;
   uint32_t n_nodes = 0, n_refs = 0;
@@ -121,7 +137,7 @@ __global__ __launch_bounds__(256) void k_extend(Scene sc, const float4 *__restri
     const RaySetup r = setup_ray(xyz(a), xyz(b));
     float t;
     uint32_t id;
-    trace<false, COUNT>(sc, r, INFINITY, __float_as_uint(b.w), kMiss, stk, t, id, &n_nodes, &n_refs);
+    trace<false, COUNT, STACK>(sc, r, INFINITY, __float_as_uint(b.w), kMiss, stk, t, id, &n_nodes, &n_refs);
     hits[i] = make_float2(t, __uint_as_float(id));
   }
   if (COUNT) {
@@ -132,9 +148,10 @@ __global__ __launch_bounds__(256) void k_extend(Scene sc, const float4 *__restri
 
 // Any hit of the shadow queue (kdTreeAccel::intersects); unoccluded
 // contributions are added to the path's radiance.
+template <int STACK>
 __global__ __launch_bounds__(256) void k_shadow(Scene sc, const float4 *__restrict__ sq,
                                                 const uint32_t *__restrict__ count, float4 *__restrict__ Ls) {
-  __shared__ uint2 stack[kStack * kBlock];
+  __shared__ uint2 stack[STACK * kBlock];
   uint2 *stk = stack + threadIdx.x;
   const uint32_t n = *count;
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
@@ -142,7 +159,7 @@ __global__ __launch_bounds__(256) void k_shadow(Scene sc, const float4 *__restri
     const RaySetup r = setup_ray(xyz(a), xyz(b));
     float t;
     uint32_t id;
-    if (!trace<true>(sc, r, b.w, __float_as_uint(c.w), kMiss, stk, t, id)) {
+    if (!trace<true, false, STACK>(sc, r, b.w, __float_as_uint(c.w), kMiss, stk, t, id)) {
       float4 *L = Ls + __float_as_uint(a.w);
       atomicAdd(&L->x, c.x);
       atomicAdd(&L->y, c.y);
@@ -194,12 +211,13 @@ __device__ __forceinline__ HitInfo hit_info(const Scene &sc, uint32_t prim, floa
 // 28-116) with DirectIllumination::evaluate (renderers/DirectIllumination.cpp:
 // 46-136).  The recursion becomes a throughput `beta`; the emission terms, NEE
 // and roulette follow the reference rules, each cited below.
-__global__ __launch_bounds__(256) void k_shade(Scene sc, JobDev job, uint32_t depth,
+__global__ __launch_bounds__(256) void k_shade(Scene sc, const BatchDesc *__restrict__ bd, uint32_t depth,
                                                const float4 *__restrict__ qin, const float2 *__restrict__ hits,
                                                const uint32_t *__restrict__ count_in, float4 *__restrict__ qout,
                                                uint32_t *__restrict__ count_out, float4 *__restrict__ sq,
                                                uint32_t *__restrict__ count_shadow, float4 *__restrict__ Ls,
-                                               float4 *__restrict__ betas, uint32_t *__restrict__ flags) {
+                                               float4 *__restrict__ betas, uint32_t *__restrict__ flags,
+                                               const uint2 *__restrict__ pjs) {
   const uint32_t n = *count_in;
   const uint32_t stride = gridDim.x * blockDim.x;
   for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += stride) {
@@ -218,14 +236,19 @@ __global__ __launch_bounds__(256) void k_shade(Scene sc, JobDev job, uint32_t de
     bool surface = false;
     float thit = 0.0f;
     float3 d = f3(0, 0, 1);
-    uint32_t pixel = 0, sample = 0;
+    uint32_t pixel = 0, sample = 0, seed = 0, job_id = 0, max_depth = 0;
     if (active) {
       const float4 a = qin[2 * i], b = qin[2 * i + 1];
       const float2 h = hits[i];
       slot = __float_as_uint(a.w);
       d = xyz(b);
-      pixel = job.pix0 + slot / job.spp;
-      sample = slot % job.spp;
+      const uint2 pj = pjs[slot];
+      pixel = pj.x;
+      sample = pj.y & ((1u << kJobBits) - 1);
+      const JobDev &job = bd->jobs[pj.y >> kJobBits];
+      seed = job.seed;
+      job_id = job.job_id;
+      max_depth = job.max_depth;
       const float4 bv = betas[slot];
       beta = xyz(bv);
       fl = flags[slot];
@@ -275,7 +298,7 @@ __global__ __launch_bounds__(256) void k_shade(Scene sc, JobDev job, uint32_t de
         if (lgid != prim) {
           const float4 *lq = sc.quads + 4ull * q;
           const float4 c0 = lq[0], ex = lq[1], ez = lq[2], ln = lq[3];
-          const mpt_rng::U4 r = mpt_rng::path_draw(job.seed, job.job_id, pixel, sample, depth, 1 + li);
+          const mpt_rng::U4 r = mpt_rng::path_draw(seed, job_id, pixel, sample, depth, 1 + li);
           const float u = mpt_rng::unitf(r.x), v = mpt_rng::unitf(r.y);
           // Plane::getPosition (shapes/Plane.cpp:91-93)
           const float3 lp = add(xyz(c0), add(mul(xyz(ex), u - 0.5f), mul(xyz(ez), v - 0.5f)));
@@ -307,13 +330,13 @@ __global__ __launch_bounds__(256) void k_shade(Scene sc, JobDev job, uint32_t de
       }
     }
     if (surface) {
-      const mpt_rng::U4 r = mpt_rng::path_draw(job.seed, job.job_id, pixel, sample, depth, 0);
+      const mpt_rng::U4 r = mpt_rng::path_draw(seed, job_id, pixel, sample, depth, 0);
       const Sample s = bsdf_sample(m, hi.n, d, mpt_rng::unitf(r.x), mpt_rng::unitf(r.y), mpt_rng::unitf(r.w),
                                    fl & F_IOR_MASK);
       if (!is_black(s.wo)) {
         // Russian roulette (PathTracer.cpp:88-92).
         const float p_cont = depth < 2 ? (is_black(s.fs) ? 0.0f : 1.0f) : fminf(0.95f, max3(s.fs));
-        if (mpt_rng::unitf(r.z) < p_cont && depth + 1 < job.max_depth) {
+        if (mpt_rng::unitf(r.z) < p_cont && depth + 1 < max_depth) {
           want_ext = true;
           wo = s.wo;
           if (depth == 0) fl |= F_D0_CONTINUED;
@@ -337,10 +360,10 @@ __global__ __launch_bounds__(256) void k_shade(Scene sc, JobDev job, uint32_t de
 
 // PathTracer.cpp:107-114: a path with a specular vertex whose depth-0 vertex
 // recursed keeps only its IOR channel, times the channel count.
-__global__ __launch_bounds__(256) void k_finalize(float4 *__restrict__ Ls, const uint32_t *__restrict__ flags,
-                                                  uint32_t n) {
+__global__ __launch_bounds__(256) void k_finalize(const BatchDesc *__restrict__ bd, float4 *__restrict__ Ls,
+                                                  const uint32_t *__restrict__ flags) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
+  if (i >= bd->hdr.n_paths) return;
   const uint32_t f = flags[i];
   if ((f & F_ANY_SPECULAR) && (f & F_D0_CONTINUED)) {
     float4 L = Ls[i];
@@ -352,12 +375,18 @@ __global__ __launch_bounds__(256) void k_finalize(float4 *__restrict__ Ls, const
 
 // Film: ProgressiveFilterValue numerator/denominator per pixel
 // (filters/ProgressiveFilterValue.h:20-37) with the pixel-bin weight 1 of
-// RenderOutput::addSample (renderers/RenderOutput.cpp:72-76).
-__global__ __launch_bounds__(256) void k_film(const float4 *__restrict__ Ls, uint32_t pix0, uint32_t npix,
-                                              uint32_t spp, float4 *__restrict__ film) {
+// RenderOutput::addSample (renderers/RenderOutput.cpp:72-76).  All samples of
+// a pixel are in one batch and are summed in sample order, so a pixel's value
+// does not depend on what else shares the batch.
+__global__ __launch_bounds__(256) void k_film(const BatchDesc *__restrict__ bd, const float4 *__restrict__ Ls) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= npix) return;
-  const float4 *L = Ls + (uint64_t)i * spp;
+  if (i >= bd->hdr.n_pixels) return;
+  const ChunkDev ch = bd->chunks[chunk_of_pixel(bd, i)];
+  const JobDev &job = bd->jobs[ch.job];
+  float4 *film = job.film;
+  if (!film) return;
+  const uint32_t spp = job.spp;
+  const float4 *L = Ls + ch.slot0 + (uint64_t)(i - ch.pixoff) * spp;
   float r = 0, g = 0, b = 0;
   for (uint32_t s = 0; s < spp; ++s) {
     const float4 v = L[s];
@@ -365,10 +394,11 @@ __global__ __launch_bounds__(256) void k_film(const float4 *__restrict__ Ls, uin
     g += v.y;
     b += v.z;
   }
-  film[pix0 + i] = make_float4(r, g, b, (float)spp);
+  film[ch.pix0 + (i - ch.pixoff)] = make_float4(r, g, b, (float)spp);
 }
 
-__global__ void k_totals(const uint32_t *__restrict__ counts, uint32_t depth, unsigned long long *totals) {
+__global__ void k_totals(const uint32_t *__restrict__ counts, uint32_t depth,
+                         unsigned long long *__restrict__ totals) {
   if (threadIdx.x != 0) return;
   unsigned long long ext = 0, sh = 0;
   for (uint32_t d = 0; d < depth; ++d) {
@@ -401,86 +431,15 @@ __global__ __launch_bounds__(256) void k_trace_rays(Scene sc, const mpt_ray *__r
   }
 }
 
-// ---------------------------------------------------------------------------
-// Host side.
-// ---------------------------------------------------------------------------
-template <class T>
-struct DBuf {
-  T *p = nullptr;
-  size_t n = 0;
-  DBuf() = default;
-  DBuf(const DBuf &) = delete;
-  DBuf &operator=(const DBuf &) = delete;
-  ~DBuf() { release(); }
-  void release() {
-    if (p) (void)hipFree(p);
-    p = nullptr;
-    n = 0;
-  }
-  hipError_t reserve(size_t count) {
-    if (count <= n && p) return hipSuccess;
-    release();
-    const hipError_t e = hipMalloc((void **)&p, std::max<size_t>(count, 1) * sizeof(T));
-    if (e == hipSuccess) n = count;
-    else p = nullptr;
-    return e;
-  }
-  size_t bytes() const { return n * sizeof(T); }
-};
-
-struct Workspace {
-  uint32_t cap = 0, depth = 0, lights = 0;
-  DBuf<float4> q[2];
-  DBuf<float2> hits;
-  DBuf<float4> sq;
-  DBuf<float4> Ls, betas;
-  DBuf<uint32_t> flags;
-  DBuf<uint32_t> counts;
-  DBuf<unsigned long long> totals;
-};
-
 }  // namespace
 
-struct mpt_context {
-  int device = 0;
-  hipStream_t stream = nullptr;
-  hipEvent_t ev0 = nullptr, ev1 = nullptr;
-  int num_cus = 256;
-  std::string err;
-  Workspace ws;
-  uint32_t batch_paths = 1u << 23;   // paths per batch (queue depth)
-  bool timing = false;
-  bool counters = false;
-  std::vector<hipEvent_t> events;    // timing pool, pairs per launch
-  std::vector<mpt_scene *> scenes;   // live scenes and films, released by mpt_close
-  std::vector<mpt_film *> films;
-};
-
-struct mpt_scene {
-  mpt_context *ctx = nullptr;
-  DBuf<uint2> nodes;
-  DBuf<float4> recs, verts, vnormals, spheres, quads;
-  DBuf<uint4> tris;
-  DBuf<uint32_t> sphere_mat, lights;
-  DBuf<mpt_material> mats;
-  Scene dev{};
-  uint64_t stats[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-};
-
-struct mpt_film {
-  mpt_context *ctx = nullptr;
-  uint32_t width = 0, height = 0;
-  DBuf<float4> data;
-};
-
+// ---------------------------------------------------------------------------
+// Host side: the batch engine.
+// ---------------------------------------------------------------------------
 namespace {
 
 bool emissive_host(const mpt_material &m) { return m.Le[0] > 0 || m.Le[1] > 0 || m.Le[2] > 0; }
 
-int fail(mpt_context *ctx, int code, const std::string &msg) {
-  if (ctx) ctx->err = msg;
-  return code;
-}
 int hip_fail(mpt_context *ctx, hipError_t e, const char *what) {
   return fail(ctx, MPT_E_DEVICE, std::string(what) + ": " + hipGetErrorString(e));
 }
@@ -511,25 +470,176 @@ void normalize3(double *v) {
   v[2] /= l;
 }
 
+uint32_t grid_for(const mpt_context *ctx, uint64_t n, uint32_t per_cu) {
+  const uint64_t blocks = (n + kBlock - 1) / kBlock;
+  return (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(blocks, (uint64_t)ctx->num_cus * per_cu));
+}
+
+uint64_t g_scene_uid = 0;
+
+// Per-launch timing of the instrumented (non-graph) path.
+struct Timed {
+  int kind;
+  size_t ev;
+};
+std::vector<Timed> g_timed;   // guarded by the single-threaded use of a context
+
+void drop_graphs(mpt_context *ctx, uint64_t scene_uid) {
+  auto &c = ctx->graphs_cache;
+  for (auto it = c.begin(); it != c.end();) {
+    if (scene_uid == 0 || (uint64_t)(uintptr_t)it->scene == scene_uid) {
+      (void)hipGraphExecDestroy(it->exec);
+      it = c.erase(it);
+    } else {
+      ++it;
+    }
+  }
+}
+
+template <int STACK>
+void launch_extend_t(mpt_context *ctx, uint32_t g, const Scene &sc, const float4 *q, const uint32_t *count,
+                     float2 *hits, unsigned long long *counters) {
+  if (counters)
+    k_extend<true, STACK><<<g, kBlock, 0, ctx->stream>>>(sc, q, count, hits, counters);
+  else
+    k_extend<false, STACK><<<g, kBlock, 0, ctx->stream>>>(sc, q, count, hits, nullptr);
+}
+
+// The short-stack depth is a run-time choice among compiled variants (it
+// trades LDS per wave, hence occupancy, against kd-restarts).
+void launch_extend(mpt_context *ctx, uint32_t g, const Scene &sc, const float4 *q, const uint32_t *count,
+                   float2 *hits, unsigned long long *counters) {
+  switch (ctx->stack) {
+    case 8: launch_extend_t<8>(ctx, g, sc, q, count, hits, counters); break;
+    case 12: launch_extend_t<12>(ctx, g, sc, q, count, hits, counters); break;
+    case 24: launch_extend_t<24>(ctx, g, sc, q, count, hits, counters); break;
+    default: launch_extend_t<16>(ctx, g, sc, q, count, hits, counters); break;
+  }
+}
+
+void launch_shadow(mpt_context *ctx, uint32_t g, const Scene &sc, const float4 *sq, const uint32_t *count,
+                   float4 *Ls) {
+  switch (ctx->stack) {
+    case 8: k_shadow<8><<<g, kBlock, 0, ctx->stream>>>(sc, sq, count, Ls); break;
+    case 12: k_shadow<12><<<g, kBlock, 0, ctx->stream>>>(sc, sq, count, Ls); break;
+    case 24: k_shadow<24><<<g, kBlock, 0, ctx->stream>>>(sc, sq, count, Ls); break;
+    default: k_shadow<16><<<g, kBlock, 0, ctx->stream>>>(sc, sq, count, Ls); break;
+  }
+}
+
+// Enqueues the kernel sequence of one batch (the part a graph captures).
+int enqueue_batch(mpt_context *ctx, mpt_scene *s, uint32_t D, bool instrument) {
+  Workspace &w = ctx->ws;
+  const uint32_t cap = w.cap, L = s->dev.num_lights;
+  const uint32_t g = grid_for(ctx, cap, 8);
+  const uint32_t gcap = (cap + 255) / 256;
+  int rc;
+  auto tic = [&](int kind) -> int {
+    if (!instrument || !ctx->timing) return MPT_OK;
+    size_t used = 0;
+    for (const Timed &t : g_timed) used = std::max(used, t.ev + 2);
+    while (used + 2 > ctx->events.size()) {
+      hipEvent_t e;
+      MPT_HIP(ctx, hipEventCreate(&e));
+      ctx->events.push_back(e);
+    }
+    g_timed.push_back({kind, used});
+    MPT_HIP(ctx, hipEventRecord(ctx->events[used], ctx->stream));
+    return MPT_OK;
+  };
+  auto toc = [&]() -> int {
+    if (!instrument || !ctx->timing) return MPT_OK;
+    MPT_HIP(ctx, hipEventRecord(ctx->events[g_timed.back().ev + 1], ctx->stream));
+    return MPT_OK;
+  };
+  k_reset<<<1, 64, 0, ctx->stream>>>(w.counts.p, 2 * D + 2);
+  k_raygen<<<gcap, 256, 0, ctx->stream>>>(w.desc.p, w.q[0].p, w.Ls.p, w.betas.p, w.flags.p, w.pjs.p, w.counts.p);
+  MPT_LAUNCHED(ctx, "k_raygen");
+  for (uint32_t d = 0; d < D; ++d) {
+    float4 *qin = w.q[d & 1].p, *qout = w.q[(d + 1) & 1].p;
+    if ((rc = tic(0))) return rc;
+    launch_extend(ctx, g, s->dev, qin, w.counts.p + d, w.hits.p, instrument && ctx->counters ? w.totals.p + 3 : nullptr);
+    if ((rc = toc())) return rc;
+    if ((rc = tic(1))) return rc;
+    k_shade<<<g, kBlock, 0, ctx->stream>>>(s->dev, w.desc.p, d, qin, w.hits.p, w.counts.p + d, qout,
+                                           w.counts.p + d + 1, w.sq.p, w.counts.p + D + 1 + d, w.Ls.p, w.betas.p,
+                                           w.flags.p, w.pjs.p);
+    if ((rc = toc())) return rc;
+    if (L) {
+      if ((rc = tic(2))) return rc;
+      launch_shadow(ctx, g, s->dev, w.sq.p, w.counts.p + D + 1 + d, w.Ls.p);
+      if ((rc = toc())) return rc;
+    }
+    MPT_LAUNCHED(ctx, "depth kernels");
+  }
+  k_finalize<<<gcap, 256, 0, ctx->stream>>>(w.desc.p, w.Ls.p, w.flags.p);
+  k_film<<<gcap, 256, 0, ctx->stream>>>(w.desc.p, w.Ls.p);
+  k_totals<<<1, 64, 0, ctx->stream>>>(w.counts.p, D, w.totals.p);
+  MPT_LAUNCHED(ctx, "film kernels");
+  return MPT_OK;
+}
+
+// Copies a batch plan into the next pinned staging descriptor and enqueues
+// its upload.
+int stage_plan(mpt_context *ctx, const BatchPlan &plan) {
+  const uint32_t slot = ctx->staging_next++ & 3u;
+  MPT_HIP(ctx, hipEventSynchronize(ctx->staging_ev[slot]));
+  BatchDesc &bd = ctx->staging[slot];
+  bd.hdr.n_chunks = (uint32_t)plan.chunks.size();
+  bd.hdr.n_paths = plan.n_paths;
+  bd.hdr.n_pixels = plan.n_pixels;
+  bd.hdr.max_depth = plan.max_depth;
+  std::copy(plan.jobs.begin(), plan.jobs.end(), bd.jobs);
+  std::copy(plan.chunks.begin(), plan.chunks.end(), bd.chunks);
+  const size_t head = offsetof(BatchDesc, jobs) + plan.jobs.size() * sizeof(JobDev);
+  MPT_HIP(ctx, hipMemcpyAsync(ctx->ws.desc.p, &bd, head, hipMemcpyHostToDevice, ctx->stream));
+  MPT_HIP(ctx, hipMemcpyAsync(reinterpret_cast<char *>(ctx->ws.desc.p) + offsetof(BatchDesc, chunks), bd.chunks,
+                              plan.chunks.size() * sizeof(ChunkDev), hipMemcpyHostToDevice, ctx->stream));
+  MPT_HIP(ctx, hipEventRecord(ctx->staging_ev[slot], ctx->stream));
+  return MPT_OK;
+}
+
+}  // namespace
+
+namespace mpt_engine {
+
+int fail(mpt_context *ctx, int code, const std::string &msg) {
+  if (ctx) ctx->err = msg;
+  return code;
+}
+
 // ThinLensCamera::setOrientation + _computeMatrix (cameras/ThinLensCamera.cpp:
 // 181-256): n = -look, u = up x n, v = n x u, film scale tan(heightAngle/2).
-CameraDev make_camera(const mpt_camera &c) {
+JobDev make_job(const mpt_job &job, float4 *film) {
+  const mpt_camera &c = job.camera;
   double n[3] = {-c.look[0], -c.look[1], -c.look[2]};
   double u[3] = {c.up[1] * n[2] - c.up[2] * n[1], c.up[2] * n[0] - c.up[0] * n[2], c.up[0] * n[1] - c.up[1] * n[0]};
   normalize3(n);
   normalize3(u);
   const double v[3] = {n[1] * u[2] - n[2] * u[1], n[2] * u[0] - n[0] * u[2], n[0] * u[1] - n[1] * u[0]};
-  CameraDev d;
+  JobDev j;
+  std::memset(&j, 0, sizeof(j));
   for (int k = 0; k < 3; ++k) {
-    d.eye[k] = c.eye[k];
-    d.u[k] = (float)u[k];
-    d.v[k] = (float)v[k];
-    d.n[k] = (float)n[k];
+    j.cam.eye[k] = c.eye[k];
+    j.cam.u[k] = (float)u[k];
+    j.cam.v[k] = (float)v[k];
+    j.cam.n[k] = (float)n[k];
   }
   const double th = std::tan(c.height_angle * M_PI / 360.0);
-  d.tan_h = (float)th;
-  d.tan_w = (float)(c.aspect * th);
-  return d;
+  j.cam.tan_h = (float)th;
+  j.cam.tan_w = (float)(c.aspect * th);
+  j.width = job.width;
+  j.height = job.height;
+  j.spp = job.spp;
+  j.max_depth = job.max_depth;
+  j.seed = job.seed;
+  j.job_id = job.job_id;
+  uint32_t side = std::max<uint32_t>(1, (uint32_t)std::sqrt((double)job.spp));
+  while ((uint64_t)(side + 1) * (side + 1) <= job.spp) ++side;
+  while ((uint64_t)side * side > job.spp) --side;
+  j.side = side;
+  j.film = film;
+  return j;
 }
 
 int check_job(mpt_context *ctx, const mpt_job *job) {
@@ -538,7 +648,7 @@ int check_job(mpt_context *ctx, const mpt_job *job) {
     return fail(ctx, MPT_E_INVALID, "job needs width, height, spp and max_depth > 0");
   if (job->max_depth > 64) return fail(ctx, MPT_E_INVALID, "max_depth above 64");
   if ((uint64_t)job->width * job->height > (1ull << 30)) return fail(ctx, MPT_E_INVALID, "film too large");
-  if (job->spp > (1u << 20)) return fail(ctx, MPT_E_INVALID, "spp above 2^20");
+  if (job->spp >= (1u << kJobBits)) return fail(ctx, MPT_E_INVALID, "spp at or above 2^20");
   const float *l = job->camera.look, *u = job->camera.up;
   const double ll = l[0] * l[0] + l[1] * l[1] + l[2] * l[2];
   const double cx = u[1] * l[2] - u[2] * l[1], cy = u[2] * l[0] - u[0] * l[2], cz = u[0] * l[1] - u[1] * l[0];
@@ -551,126 +661,132 @@ int check_job(mpt_context *ctx, const mpt_job *job) {
 
 int ensure_workspace(mpt_context *ctx, uint32_t cap, uint32_t depth, uint32_t lights) {
   Workspace &w = ctx->ws;
+  cap = std::max(cap, w.cap);
+  depth = std::max(depth, w.depth);
+  lights = std::max(lights, w.lights);
+  if (cap == w.cap && depth == w.depth && lights == w.lights && w.desc.p) return MPT_OK;
+  MPT_HIP(ctx, hipStreamSynchronize(ctx->stream));
   const uint64_t ns = (uint64_t)cap * std::max<uint32_t>(lights, 1);
-  if (cap > w.cap || lights > w.lights) {
-    MPT_HIP(ctx, w.q[0].reserve(2ull * cap));
-    MPT_HIP(ctx, w.q[1].reserve(2ull * cap));
-    MPT_HIP(ctx, w.hits.reserve(cap));
-    MPT_HIP(ctx, w.Ls.reserve(cap));
-    MPT_HIP(ctx, w.betas.reserve(cap));
-    MPT_HIP(ctx, w.flags.reserve(cap));
-    MPT_HIP(ctx, w.sq.reserve(3ull * ns));
-    w.cap = std::max(w.cap, cap);
-    w.lights = std::max(w.lights, lights);
-  }
-  if (depth > w.depth || !w.counts.p) {
-    MPT_HIP(ctx, w.counts.reserve(2ull * depth + 2));
-    w.depth = std::max(w.depth, depth);
-  }
-  MPT_HIP(ctx, w.totals.reserve(6));
+  MPT_HIP(ctx, w.q[0].reserve(2ull * cap));
+  MPT_HIP(ctx, w.q[1].reserve(2ull * cap));
+  MPT_HIP(ctx, w.hits.reserve(cap));
+  MPT_HIP(ctx, w.Ls.reserve(cap));
+  MPT_HIP(ctx, w.betas.reserve(cap));
+  MPT_HIP(ctx, w.flags.reserve(cap));
+  MPT_HIP(ctx, w.pjs.reserve(cap));
+  MPT_HIP(ctx, w.sq.reserve(3ull * ns));
+  MPT_HIP(ctx, w.counts.reserve(2ull * depth + 2));
+  MPT_HIP(ctx, w.totals.reserve(8));
+  MPT_HIP(ctx, w.desc.reserve(1));
+  w.cap = cap;
+  w.depth = depth;
+  w.lights = lights;
+  ++w.generation;
+  drop_graphs(ctx, 0);
   return MPT_OK;
 }
 
-uint32_t grid_for(const mpt_context *ctx, uint64_t n, uint32_t per_cu) {
-  const uint64_t blocks = (n + kBlock - 1) / kBlock;
-  return (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(blocks, (uint64_t)ctx->num_cus * per_cu));
+int reset_totals(mpt_context *ctx) {
+  g_timed.clear();
+  MPT_HIP(ctx, ctx->ws.totals.reserve(8));
+  MPT_HIP(ctx, hipMemsetAsync(ctx->ws.totals.p, 0, 8 * sizeof(unsigned long long), ctx->stream));
+  return MPT_OK;
 }
 
-// Renders pixels [pix_begin, pix_end) of a job.  film (device, W*H float4)
-// and/or L_host (per-path radiance) receive the results.
+int read_totals(mpt_context *ctx, unsigned long long out[5]) {
+  unsigned long long tot[8];
+  MPT_HIP(ctx, hipMemcpyAsync(tot, ctx->ws.totals.p, sizeof(tot), hipMemcpyDeviceToHost, ctx->stream));
+  MPT_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  for (int i = 0; i < 5; ++i) out[i] = tot[i];
+  return MPT_OK;
+}
+
+int run_batch(mpt_context *ctx, mpt_scene *s, const BatchPlan &plan, float *L_host) {
+  if (plan.chunks.empty() || plan.n_paths == 0) return MPT_OK;
+  if (plan.jobs.size() > kMaxJobs || plan.chunks.size() > kMaxChunks)
+    return fail(ctx, MPT_E_INVALID, "batch plan exceeds the job or chunk table");
+  int rc = ensure_workspace(ctx, std::max(plan.n_paths, ctx->batch_paths), plan.max_depth, s->dev.num_lights);
+  if (rc) return rc;
+  if ((rc = stage_plan(ctx, plan))) return rc;
+  const uint32_t D = plan.max_depth;
+  const bool instrument = ctx->timing || ctx->counters;
+  if (ctx->graphs && !instrument) {
+    Workspace &w = ctx->ws;
+    hipGraphExec_t exec = nullptr;
+    const void *key = reinterpret_cast<const void *>((uintptr_t)s->uid);
+    for (const GraphEntry &e : ctx->graphs_cache)
+      if (e.scene == key && e.depth == D && e.cap == w.cap && e.lights == s->dev.num_lights &&
+          e.generation == w.generation) {
+        exec = e.exec;
+        break;
+      }
+    if (!exec) {
+      hipGraph_t graph = nullptr;
+      MPT_HIP(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+      rc = enqueue_batch(ctx, s, D, false);
+      hipGraph_t captured = nullptr;
+      const hipError_t ec = hipStreamEndCapture(ctx->stream, &captured);
+      if (rc) {
+        if (captured) (void)hipGraphDestroy(captured);
+        return rc;
+      }
+      if (ec != hipSuccess) return hip_fail(ctx, ec, "hipStreamEndCapture");
+      graph = captured;
+      const hipError_t ei = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(graph);
+      if (ei != hipSuccess) return hip_fail(ctx, ei, "hipGraphInstantiate");
+      GraphEntry e;
+      e.scene = key;
+      e.depth = D;
+      e.cap = w.cap;
+      e.lights = s->dev.num_lights;
+      e.generation = w.generation;
+      e.exec = exec;
+      ctx->graphs_cache.push_back(e);
+    }
+    MPT_HIP(ctx, hipGraphLaunch(exec, ctx->stream));
+  } else {
+    if ((rc = enqueue_batch(ctx, s, D, true))) return rc;
+  }
+  if (L_host) {
+    MPT_HIP(ctx, hipMemcpy2DAsync(L_host, 3 * sizeof(float), ctx->ws.Ls.p, sizeof(float4), 3 * sizeof(float),
+                                  plan.n_paths, hipMemcpyDeviceToHost, ctx->stream));
+    MPT_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  }
+  return MPT_OK;
+}
+
+}  // namespace mpt_engine
+
+namespace {
+
+// Renders pixels [pix_begin, pix_end) of one job as consecutive batches of
+// whole pixels.
 int render_range(mpt_scene *s, const mpt_job *job, uint32_t pix_begin, uint32_t pix_end, float4 *film,
                  float *L_host, mpt_render_stats *stats) {
   mpt_context *ctx = s->ctx;
   int rc = check_job(ctx, job);
   if (rc) return rc;
-  const uint32_t spp = job->spp, D = job->max_depth;
-  const uint32_t L = s->dev.num_lights;
+  const uint32_t spp = job->spp;
   const uint32_t batch_pix = std::max<uint32_t>(1, ctx->batch_paths / spp);
-  const uint32_t cap = batch_pix * spp;
-  rc = ensure_workspace(ctx, cap, D, L);
-  if (rc) return rc;
-  Workspace &w = ctx->ws;
-  JobDev jd;
-  jd.cam = make_camera(job->camera);
-  jd.width = job->width;
-  jd.height = job->height;
-  jd.spp = spp;
-  jd.max_depth = D;
-  jd.seed = job->seed;
-  jd.job_id = job->job_id;
-  jd.side = std::max<uint32_t>(1, (uint32_t)std::sqrt((double)spp));
-  while ((uint64_t)(jd.side + 1) * (jd.side + 1) <= spp) ++jd.side;
-  while ((uint64_t)jd.side * jd.side > spp) --jd.side;
-  MPT_HIP(ctx, hipMemsetAsync(w.totals.p, 0, 6 * sizeof(unsigned long long), ctx->stream));
+  if ((rc = reset_totals(ctx))) return rc;
   MPT_HIP(ctx, hipEventRecord(ctx->ev0, ctx->stream));
-  const uint32_t counts_n = 2 * D + 2;
-  uint64_t launches = 0;
-  // Event pairs of timed launches: kind 0 extend, 1 shade, 2 shadow.
-  std::vector<std::pair<int, size_t>> timed;
-  size_t ev_used = 0;
-  auto tic = [&](int kind) -> int {
-    if (!ctx->timing) return MPT_OK;
-    if (ev_used + 2 > ctx->events.size()) {
-      for (int k = 0; k < 64; ++k) {
-        hipEvent_t e;
-        MPT_HIP(ctx, hipEventCreate(&e));
-        ctx->events.push_back(e);
-      }
-    }
-    timed.push_back({kind, ev_used});
-    MPT_HIP(ctx, hipEventRecord(ctx->events[ev_used], ctx->stream));
-    ev_used += 2;
-    return MPT_OK;
-  };
-  auto toc = [&]() -> int {
-    if (!ctx->timing) return MPT_OK;
-    MPT_HIP(ctx, hipEventRecord(ctx->events[timed.back().second + 1], ctx->stream));
-    return MPT_OK;
-  };
+  BatchPlan plan;
+  plan.jobs.push_back(make_job(*job, film));
+  plan.max_depth = job->max_depth;
+  uint64_t batches = 0;
   for (uint32_t p0 = pix_begin; p0 < pix_end; p0 += batch_pix) {
     const uint32_t np = std::min(batch_pix, pix_end - p0);
-    jd.pix0 = p0;
-    jd.n_paths = np * spp;
-    const uint32_t g = grid_for(ctx, jd.n_paths, 8);
-    k_reset<<<1, 64, 0, ctx->stream>>>(w.counts.p, counts_n);
-    k_raygen<<<(jd.n_paths + 255) / 256, 256, 0, ctx->stream>>>(jd, w.q[0].p, w.Ls.p, w.betas.p, w.flags.p,
-                                                                  w.counts.p);
-    MPT_LAUNCHED(ctx, "k_raygen");
-    for (uint32_t d = 0; d < D; ++d) {
-      float4 *qin = w.q[d & 1].p, *qout = w.q[(d + 1) & 1].p;
-      if ((rc = tic(0))) return rc;
-      if (ctx->counters)
-        k_extend<true><<<g, kBlock, 0, ctx->stream>>>(s->dev, qin, w.counts.p + d, w.hits.p, w.totals.p + 3);
-      else
-        k_extend<false><<<g, kBlock, 0, ctx->stream>>>(s->dev, qin, w.counts.p + d, w.hits.p, nullptr);
-      if ((rc = toc())) return rc;
-      if ((rc = tic(1))) return rc;
-      k_shade<<<g, kBlock, 0, ctx->stream>>>(s->dev, jd, d, qin, w.hits.p, w.counts.p + d, qout,
-                                             w.counts.p + d + 1, w.sq.p, w.counts.p + D + 1 + d, w.Ls.p,
-                                             w.betas.p, w.flags.p);
-      if ((rc = toc())) return rc;
-      if (L) {
-        if ((rc = tic(2))) return rc;
-        k_shadow<<<g, kBlock, 0, ctx->stream>>>(s->dev, w.sq.p, w.counts.p + D + 1 + d, w.Ls.p);
-        if ((rc = toc())) return rc;
-      }
-      MPT_LAUNCHED(ctx, "depth kernels");
-      ++launches;
-    }
-    k_finalize<<<(jd.n_paths + 255) / 256, 256, 0, ctx->stream>>>(w.Ls.p, w.flags.p, jd.n_paths);
-    if (film) k_film<<<(np + 255) / 256, 256, 0, ctx->stream>>>(w.Ls.p, p0, np, spp, film);
-    k_totals<<<1, 64, 0, ctx->stream>>>(w.counts.p, D, w.totals.p);
-    MPT_LAUNCHED(ctx, "film kernels");
-    if (L_host) {
-      MPT_HIP(ctx, hipMemcpy2DAsync(L_host + 3ull * (uint64_t)(p0 - pix_begin) * spp, 3 * sizeof(float), w.Ls.p,
-                                    sizeof(float4), 3 * sizeof(float), jd.n_paths, hipMemcpyDeviceToHost,
-                                    ctx->stream));
-    }
+    plan.chunks.assign(1, ChunkDev{0, p0, np, 0, 0, {0, 0, 0}});
+    plan.n_paths = np * spp;
+    plan.n_pixels = np;
+    if ((rc = run_batch(ctx, s, plan, L_host ? L_host + 3ull * (uint64_t)(p0 - pix_begin) * spp : nullptr)))
+      return rc;
+    ++batches;
   }
   MPT_HIP(ctx, hipEventRecord(ctx->ev1, ctx->stream));
-  unsigned long long tot[6] = {0, 0, 0, 0, 0, 0};
-  MPT_HIP(ctx, hipMemcpyAsync(tot, w.totals.p, sizeof(tot), hipMemcpyDeviceToHost, ctx->stream));
-  MPT_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  unsigned long long tot[5];
+  if ((rc = read_totals(ctx, tot))) return rc;
   if (stats) {
     float ms = 0;
     MPT_HIP(ctx, hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
@@ -679,14 +795,14 @@ int render_range(mpt_scene *s, const mpt_job *job, uint32_t pix_begin, uint32_t 
     stats->camera_rays = tot[0];
     stats->extension_rays = tot[1];
     stats->shadow_rays = tot[2];
-    stats->render_ms = ms;
-    stats->extend_launches = launches;
     stats->nodes_visited = tot[3];
     stats->prims_tested = tot[4];
-    for (const auto &e : timed) {
-      float t = 0;
-      MPT_HIP(ctx, hipEventElapsedTime(&t, ctx->events[e.second], ctx->events[e.second + 1]));
-      (e.first == 0 ? stats->extend_ms : e.first == 1 ? stats->shade_ms : stats->shadow_ms) += t;
+    stats->render_ms = ms;
+    stats->extend_launches = batches * job->max_depth;
+    for (const Timed &t : g_timed) {
+      float e = 0;
+      MPT_HIP(ctx, hipEventElapsedTime(&e, ctx->events[t.ev], ctx->events[t.ev + 1]));
+      (t.kind == 0 ? stats->extend_ms : t.kind == 1 ? stats->shade_ms : stats->shadow_ms) += e;
     }
   }
   return MPT_OK;
@@ -712,6 +828,10 @@ int mpt_open(int device, mpt_context **out) {
     ctx->num_cus = prop.multiProcessorCount;
   if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) return MPT_E_DEVICE;
   if (hipEventCreate(&ctx->ev0) != hipSuccess || hipEventCreate(&ctx->ev1) != hipSuccess) return MPT_E_DEVICE;
+  if (hipHostMalloc((void **)&ctx->staging, 4 * sizeof(BatchDesc), hipHostMallocDefault) != hipSuccess)
+    return MPT_E_NOMEM;
+  for (hipEvent_t &e : ctx->staging_ev)
+    if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) return MPT_E_DEVICE;
   *out = ctx.release();
   return MPT_OK;
 }
@@ -720,8 +840,13 @@ int mpt_close(mpt_context *ctx) {
   if (!ctx) return MPT_E_INVALID;
   (void)hipSetDevice(ctx->device);
   (void)hipStreamSynchronize(ctx->stream);
+  while (!ctx->servers.empty()) mpt_server_destroy(ctx->servers.back());
   while (!ctx->scenes.empty()) mpt_scene_destroy(ctx->scenes.back());
   while (!ctx->films.empty()) mpt_film_destroy(ctx->films.back());
+  drop_graphs(ctx, 0);
+  for (hipEvent_t e : ctx->staging_ev)
+    if (e) (void)hipEventDestroy(e);
+  if (ctx->staging) (void)hipHostFree(ctx->staging);
 
   if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
   if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
@@ -745,6 +870,15 @@ int mpt_set_option(mpt_context *ctx, uint32_t option, uint64_t value) {
       return MPT_OK;
     case MPT_OPT_COUNTERS:
       ctx->counters = value != 0;
+      return MPT_OK;
+    case MPT_OPT_GRAPHS:
+      ctx->graphs = value != 0;
+      return MPT_OK;
+    case MPT_OPT_STACK:
+      if (value != 8 && value != 12 && value != 16 && value != 24)
+        return fail(ctx, MPT_E_INVALID, "stack depth must be 8, 12, 16 or 24");
+      ctx->stack = (uint32_t)value;
+      drop_graphs(ctx, 0);
       return MPT_OK;
     default:
       return fail(ctx, MPT_E_INVALID, "unknown option");
@@ -938,11 +1072,11 @@ int mpt_scene_create(mpt_context *ctx, const mpt_scene_desc *d, const mpt_kd_par
   s->stats[1] = ko.n_leaves;
   s->stats[2] = ko.n_refs;
   s->stats[3] = ko.max_depth;
-  s->stats[4] = s->nodes.bytes() + s->recs.bytes() + s->tris.bytes() + s->verts.bytes() + s->vnormals.bytes() +
-                s->spheres.bytes() + s->quads.bytes() + s->mats.bytes();
+  s->stats[4] = s->device_bytes();
   s->stats[5] = (uint64_t)ko.build_ms;
   s->stats[6] = ko.max_leaf;
   s->stats[7] = nt;
+  s->uid = ++g_scene_uid;
   ctx->scenes.push_back(s.get());
   *out = s.release();
   return MPT_OK;
@@ -954,6 +1088,7 @@ int mpt_scene_destroy(mpt_scene *s) {
   (void)hipSetDevice(ctx->device);
   (void)hipStreamSynchronize(ctx->stream);
   ctx->scenes.erase(std::remove(ctx->scenes.begin(), ctx->scenes.end(), s), ctx->scenes.end());
+  drop_graphs(ctx, s->uid);
   delete s;
   return MPT_OK;
 }

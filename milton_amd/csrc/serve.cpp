@@ -1,0 +1,447 @@
+// Batching and serving over the wavefront engine (declared in
+// include/milton_pt.h, "Batching and serving").
+//
+// The reference renders one job at a time: PointSampleRenderer::render runs a
+// producer thread and N consumer threads over one film and returns when it is
+// done (renderers/PointSampleRenderer.cpp:24-81).  Here a server keeps
+// scenes resident and runs continuous batching:
+//   admission   queued jobs are admitted in arrival order while their films
+//               fit the HBM budget (and the active-job limit);
+//   packing     a round picks the scene of the oldest admitted job with work
+//               left and fills one batch with whole-pixel chunks of every
+//               admitted job on that scene, oldest first;
+//   overlap     up to max_inflight batches are enqueued before the oldest is
+//               waited for, so planning and result copies overlap the device;
+//   completion  when a job's last batch finishes its film is copied to host
+//               memory and its device film is freed (admitting the next job).
+#include <chrono>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <new>
+#include <thread>
+#include <vector>
+
+#include "engine.h"
+
+using namespace mpt_engine;
+
+namespace {
+
+struct Job {
+  uint64_t handle = 0;
+  mpt_scene *scene = nullptr;
+  mpt_job job{};
+  uint32_t state = MPT_JOB_QUEUED;
+  uint32_t error = 0;
+  uint64_t npix = 0, next_pixel = 0, pixels_done = 0;
+  uint32_t inflight = 0;        // batches in flight that reference the film
+  DBuf<float4> film;
+  std::vector<float> host;      // RGB result
+  double submit_ms = 0, start_ms = 0, done_ms = 0;
+  uint64_t film_bytes() const { return npix * sizeof(float4); }
+};
+
+struct Inflight {
+  hipEvent_t done = nullptr;
+  std::vector<std::pair<Job *, uint32_t>> parts;   // job, pixels in the batch
+  uint64_t paths = 0;
+};
+
+}  // namespace
+
+struct mpt_server {
+  mpt_context *ctx = nullptr;
+  mpt_server_config cfg{};
+  std::mutex mu;
+  std::condition_variable cv;
+  std::map<uint64_t, std::unique_ptr<Job>> jobs;
+  std::deque<Job *> queued;     // arrival order
+  std::vector<Job *> active;    // admission order
+  std::deque<Inflight> inflight;
+  std::vector<hipEvent_t> free_events;
+  uint64_t next_handle = 1;
+  uint64_t film_bytes = 0, peak_film_bytes = 0;
+  uint64_t batches = 0, paths = 0, jobs_done = 0;
+  std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now();
+  std::thread worker;
+  bool running = false, stop = false;
+
+  double now_ms() const {
+    return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  }
+};
+
+namespace {
+
+// Frees what a finished, cancelled or failed job holds on the device once no
+// batch in flight still writes its film.
+void release_film(mpt_server *sv, Job *j) {
+  if (j->inflight == 0 && j->film.p) {
+    j->film.release();
+    sv->film_bytes -= j->film_bytes();
+  }
+}
+
+void admit(mpt_server *sv) {
+  while (!sv->queued.empty() && sv->active.size() < sv->cfg.max_active) {
+    Job *j = sv->queued.front();
+    if (j->state == MPT_JOB_CANCELLED) {
+      sv->queued.pop_front();
+      continue;
+    }
+    if (sv->film_bytes + j->film_bytes() > sv->cfg.hbm_budget) break;
+    if (j->film.reserve(j->npix) != hipSuccess) break;
+    sv->film_bytes += j->film_bytes();
+    sv->peak_film_bytes = std::max(sv->peak_film_bytes, sv->film_bytes);
+    sv->queued.pop_front();
+    j->state = MPT_JOB_RUNNING;
+    j->start_ms = sv->now_ms();
+    sv->active.push_back(j);
+  }
+}
+
+// Waits for the oldest batch in flight and books its pixels.
+int retire_one(mpt_server *sv, std::unique_lock<std::mutex> &lk) {
+  Inflight b = std::move(sv->inflight.front());
+  sv->inflight.pop_front();
+  lk.unlock();
+  const hipError_t e = hipEventSynchronize(b.done);
+  lk.lock();
+  sv->free_events.push_back(b.done);
+  for (auto &part : b.parts) {
+    Job *j = part.first;
+    --j->inflight;
+    if (e != hipSuccess && j->state == MPT_JOB_RUNNING) {
+      j->state = MPT_JOB_FAILED;
+      j->error = (uint32_t)(-MPT_E_DEVICE);
+      j->done_ms = sv->now_ms();
+    }
+    if (j->state != MPT_JOB_RUNNING) {
+      release_film(sv, j);
+      continue;
+    }
+    j->pixels_done += part.second;
+    if (j->pixels_done == j->npix) {
+      // Film (numerator, denominator) -> RGB on the host.
+      std::vector<float4> raw(j->npix);
+      const hipError_t ce = hipMemcpy(raw.data(), j->film.p, j->film_bytes(), hipMemcpyDeviceToHost);
+      if (ce != hipSuccess) {
+        j->state = MPT_JOB_FAILED;
+        j->error = (uint32_t)(-MPT_E_DEVICE);
+      } else {
+        j->host.resize(3 * j->npix);
+        for (uint64_t i = 0; i < j->npix; ++i) {
+          const float w = raw[i].w, k = w > 0 ? 1.0f / w : 0.0f;
+          j->host[3 * i] = raw[i].x * k;
+          j->host[3 * i + 1] = raw[i].y * k;
+          j->host[3 * i + 2] = raw[i].z * k;
+        }
+        j->state = MPT_JOB_DONE;
+        ++sv->jobs_done;
+      }
+      j->done_ms = sv->now_ms();
+      release_film(sv, j);
+    }
+  }
+  sv->cv.notify_all();
+  return e == hipSuccess ? MPT_OK : MPT_E_DEVICE;
+}
+
+// One scheduling round: admit, plan one batch, enqueue it.  Returns the
+// number of batches enqueued (0 when idle).
+int round(mpt_server *sv, std::unique_lock<std::mutex> &lk, uint32_t *ran) {
+  *ran = 0;
+  admit(sv);
+  // Drop jobs that left the running state from the active list.
+  sv->active.erase(std::remove_if(sv->active.begin(), sv->active.end(),
+                                  [](Job *j) { return j->state != MPT_JOB_RUNNING || j->next_pixel == j->npix; }),
+                   sv->active.end());
+  if (sv->active.empty()) {
+    while (!sv->inflight.empty()) retire_one(sv, lk);
+    admit(sv);
+    sv->active.erase(std::remove_if(sv->active.begin(), sv->active.end(),
+                                    [](Job *j) { return j->state != MPT_JOB_RUNNING || j->next_pixel == j->npix; }),
+                     sv->active.end());
+    if (sv->active.empty()) return MPT_OK;
+  }
+  mpt_context *ctx = sv->ctx;
+  mpt_scene *scene = sv->active.front()->scene;
+  const uint32_t cap = ctx->batch_paths;
+  BatchPlan plan;
+  std::vector<std::pair<Job *, uint32_t>> parts;
+  for (Job *j : sv->active) {
+    if (j->scene != scene) continue;
+    if (plan.jobs.size() == kMaxJobs || plan.chunks.size() == kMaxChunks) break;
+    const uint32_t spp = j->job.spp;
+    const uint64_t room = (cap - plan.n_paths) / spp;
+    if (room == 0) {
+      if (plan.n_paths == 0) {
+        // a single pixel wider than the capacity: run it alone
+      } else {
+        break;
+      }
+    }
+    const uint32_t np = (uint32_t)std::min<uint64_t>(std::max<uint64_t>(room, 1), j->npix - j->next_pixel);
+    const uint32_t jidx = (uint32_t)plan.jobs.size();
+    plan.jobs.push_back(make_job(j->job, j->film.p));
+    ChunkDev ch{};
+    ch.job = jidx;
+    ch.pix0 = (uint32_t)j->next_pixel;
+    ch.npix = np;
+    ch.slot0 = plan.n_paths;
+    ch.pixoff = plan.n_pixels;
+    plan.chunks.push_back(ch);
+    plan.n_paths += np * spp;
+    plan.n_pixels += np;
+    plan.max_depth = std::max(plan.max_depth, j->job.max_depth);
+    j->next_pixel += np;
+    ++j->inflight;
+    parts.push_back({j, np});
+    if (plan.n_paths >= cap) break;
+  }
+  if (plan.chunks.empty()) return MPT_OK;
+  while (sv->inflight.size() >= sv->cfg.max_inflight) retire_one(sv, lk);
+  int rc = run_batch(ctx, scene, plan, nullptr);
+  hipEvent_t ev = nullptr;
+  if (!rc) {
+    if (!sv->free_events.empty()) {
+      ev = sv->free_events.back();
+      sv->free_events.pop_back();
+    } else if (hipEventCreateWithFlags(&ev, hipEventDisableTiming) != hipSuccess) {
+      rc = MPT_E_DEVICE;
+    }
+    if (!rc && hipEventRecord(ev, ctx->stream) != hipSuccess) rc = MPT_E_DEVICE;
+  }
+  if (rc) {
+    for (auto &part : parts) {
+      Job *j = part.first;
+      --j->inflight;
+      j->state = MPT_JOB_FAILED;
+      j->error = (uint32_t)(-rc);
+      j->done_ms = sv->now_ms();
+      release_film(sv, j);
+    }
+    if (ev) sv->free_events.push_back(ev);
+    sv->cv.notify_all();
+    return rc;
+  }
+  Inflight b;
+  b.done = ev;
+  b.parts = std::move(parts);
+  b.paths = plan.n_paths;
+  sv->inflight.push_back(std::move(b));
+  ++sv->batches;
+  sv->paths += plan.n_paths;
+  *ran = 1;
+  return MPT_OK;
+}
+
+bool idle(const mpt_server *sv) {
+  if (!sv->inflight.empty()) return false;
+  for (const Job *j : sv->active)
+    if (j->state == MPT_JOB_RUNNING) return false;
+  for (const Job *j : sv->queued)
+    if (j->state == MPT_JOB_QUEUED) return false;
+  return true;
+}
+
+void worker_loop(mpt_server *sv) {
+  (void)hipSetDevice(sv->ctx->device);
+  std::unique_lock<std::mutex> lk(sv->mu);
+  while (!sv->stop) {
+    uint32_t ran = 0;
+    round(sv, lk, &ran);
+    if (!ran) {
+      while (!sv->inflight.empty()) retire_one(sv, lk);
+      if (idle(sv)) sv->cv.wait_for(lk, std::chrono::milliseconds(2));
+    }
+  }
+  while (!sv->inflight.empty()) retire_one(sv, lk);
+}
+
+}  // namespace
+
+extern "C" {
+
+int mpt_server_create(mpt_context *ctx, const mpt_server_config *cfg, mpt_server **out) {
+  if (!ctx || !out) return MPT_E_INVALID;
+  *out = nullptr;
+  std::unique_ptr<mpt_server> sv(new (std::nothrow) mpt_server());
+  if (!sv) return MPT_E_NOMEM;
+  sv->ctx = ctx;
+  if (cfg) sv->cfg = *cfg;
+  if (!sv->cfg.hbm_budget) sv->cfg.hbm_budget = 16ull << 30;
+  if (!sv->cfg.max_active) sv->cfg.max_active = 256;
+  if (!sv->cfg.max_inflight) sv->cfg.max_inflight = 2;
+  sv->cfg.max_inflight = std::min<uint32_t>(sv->cfg.max_inflight, 3);   // the descriptor staging ring holds 4
+  if (reset_totals(ctx)) return MPT_E_DEVICE;
+  ctx->servers.push_back(sv.get());
+  *out = sv.release();
+  return MPT_OK;
+}
+
+int mpt_server_destroy(mpt_server *sv) {
+  if (!sv) return MPT_E_INVALID;
+  mpt_server_stop(sv);
+  {
+    std::unique_lock<std::mutex> lk(sv->mu);
+    while (!sv->inflight.empty()) retire_one(sv, lk);
+  }
+  for (hipEvent_t e : sv->free_events) (void)hipEventDestroy(e);
+  auto &v = sv->ctx->servers;
+  v.erase(std::remove(v.begin(), v.end(), sv), v.end());
+  delete sv;
+  return MPT_OK;
+}
+
+int mpt_submit(mpt_server *sv, mpt_scene *scene, const mpt_job *job, uint64_t *handle) {
+  if (!sv || !scene || !job || !handle) return MPT_E_INVALID;
+  if (scene->ctx != sv->ctx) return fail(sv->ctx, MPT_E_INVALID, "scene belongs to another context");
+  int rc = check_job(sv->ctx, job);
+  if (rc) return rc;
+  std::unique_ptr<Job> j(new (std::nothrow) Job());
+  if (!j) return MPT_E_NOMEM;
+  j->scene = scene;
+  j->job = *job;
+  j->npix = (uint64_t)job->width * job->height;
+  if (j->film_bytes() > sv->cfg.hbm_budget) return fail(sv->ctx, MPT_E_BUDGET, "film exceeds the HBM budget");
+  std::lock_guard<std::mutex> lk(sv->mu);
+  j->handle = sv->next_handle++;
+  j->submit_ms = sv->now_ms();
+  *handle = j->handle;
+  sv->queued.push_back(j.get());
+  sv->jobs[j->handle] = std::move(j);
+  sv->cv.notify_all();
+  return MPT_OK;
+}
+
+int mpt_status(mpt_server *sv, uint64_t handle, mpt_job_status *st) {
+  if (!sv || !st) return MPT_E_INVALID;
+  std::lock_guard<std::mutex> lk(sv->mu);
+  auto it = sv->jobs.find(handle);
+  if (it == sv->jobs.end()) return MPT_E_NOTFOUND;
+  const Job &j = *it->second;
+  st->state = j.state;
+  st->error = j.error;
+  st->pixels_done = j.pixels_done;
+  st->pixels_total = j.npix;
+  st->submit_ms = j.submit_ms;
+  st->start_ms = j.start_ms;
+  st->done_ms = j.done_ms;
+  return MPT_OK;
+}
+
+int mpt_cancel(mpt_server *sv, uint64_t handle) {
+  if (!sv) return MPT_E_INVALID;
+  std::lock_guard<std::mutex> lk(sv->mu);
+  auto it = sv->jobs.find(handle);
+  if (it == sv->jobs.end()) return MPT_E_NOTFOUND;
+  Job *j = it->second.get();
+  if (j->state == MPT_JOB_QUEUED || j->state == MPT_JOB_RUNNING) {
+    j->state = MPT_JOB_CANCELLED;
+    j->done_ms = sv->now_ms();
+    release_film(sv, j);
+    sv->cv.notify_all();
+  }
+  return MPT_OK;
+}
+
+int mpt_result(mpt_server *sv, uint64_t handle, float *rgb) {
+  if (!sv || !rgb) return MPT_E_INVALID;
+  std::lock_guard<std::mutex> lk(sv->mu);
+  auto it = sv->jobs.find(handle);
+  if (it == sv->jobs.end()) return MPT_E_NOTFOUND;
+  Job *j = it->second.get();
+  if (j->state == MPT_JOB_CANCELLED) return MPT_E_CANCELLED;
+  if (j->state == MPT_JOB_FAILED) return -(int)j->error;
+  if (j->state != MPT_JOB_DONE) return fail(sv->ctx, MPT_E_INVALID, "job not finished");
+  std::memcpy(rgb, j->host.data(), j->host.size() * sizeof(float));
+  sv->jobs.erase(it);
+  return MPT_OK;
+}
+
+int mpt_wait(mpt_server *sv, uint64_t handle, uint32_t timeout_ms) {
+  if (!sv) return MPT_E_INVALID;
+  std::unique_lock<std::mutex> lk(sv->mu);
+  const auto deadline = std::chrono::steady_clock::now() + std::chrono::milliseconds(timeout_ms);
+  for (;;) {
+    auto it = sv->jobs.find(handle);
+    if (it == sv->jobs.end()) return MPT_E_NOTFOUND;
+    const uint32_t s = it->second->state;
+    if (s != MPT_JOB_QUEUED && s != MPT_JOB_RUNNING) return 0;
+    if (sv->cv.wait_until(lk, deadline) == std::cv_status::timeout) return 1;
+  }
+}
+
+int mpt_server_step(mpt_server *sv, uint32_t *batches_run) {
+  if (!sv) return MPT_E_INVALID;
+  if (sv->running) return fail(sv->ctx, MPT_E_INVALID, "the worker thread drives this server");
+  std::unique_lock<std::mutex> lk(sv->mu);
+  uint32_t ran = 0;
+  const int rc = round(sv, lk, &ran);
+  if (!ran)
+    while (!sv->inflight.empty()) retire_one(sv, lk);
+  if (batches_run) *batches_run = ran;
+  return rc;
+}
+
+int mpt_server_drain(mpt_server *sv) {
+  if (!sv) return MPT_E_INVALID;
+  if (sv->running) {
+    std::unique_lock<std::mutex> lk(sv->mu);
+    sv->cv.wait(lk, [&] { return idle(sv); });
+    return MPT_OK;
+  }
+  int first_error = MPT_OK;
+  for (;;) {
+    uint32_t ran = 0;
+    const int rc = mpt_server_step(sv, &ran);
+    if (rc && !first_error) first_error = rc;
+    std::lock_guard<std::mutex> lk(sv->mu);
+    if (!ran && idle(sv)) break;
+  }
+  return first_error;
+}
+
+int mpt_server_start(mpt_server *sv) {
+  if (!sv) return MPT_E_INVALID;
+  std::lock_guard<std::mutex> lk(sv->mu);
+  if (sv->running) return MPT_OK;
+  sv->stop = false;
+  sv->running = true;
+  sv->worker = std::thread(worker_loop, sv);
+  return MPT_OK;
+}
+
+int mpt_server_stop(mpt_server *sv) {
+  if (!sv) return MPT_E_INVALID;
+  {
+    std::lock_guard<std::mutex> lk(sv->mu);
+    if (!sv->running) return MPT_OK;
+    sv->stop = true;
+    sv->cv.notify_all();
+  }
+  sv->worker.join();
+  sv->running = false;
+  return MPT_OK;
+}
+
+int mpt_server_stats(mpt_server *sv, uint64_t out[6]) {
+  if (!sv || !out) return MPT_E_INVALID;
+  std::lock_guard<std::mutex> lk(sv->mu);
+  unsigned long long tot[5] = {0, 0, 0, 0, 0};
+  if (read_totals(sv->ctx, tot)) return MPT_E_DEVICE;
+  out[0] = sv->batches;
+  out[1] = sv->paths;
+  out[2] = tot[1];
+  out[3] = tot[2];
+  out[4] = sv->jobs_done;
+  out[5] = sv->peak_film_bytes;
+  return MPT_OK;
+}
+
+}  // extern "C"
