@@ -76,6 +76,109 @@ def copy_calibration(nbytes=256 << 20):
     log(f"copy calibration: {nbytes} bytes device to device")
 
 
+def system_scenes(m, ctx):
+    from milton_amd import workloads
+    scenes, build = {}, {}
+    for kind in workloads.SCENES:
+        arrays = m.generate(kind, workloads.SCENE_COUNTS[kind], 1)
+        t0 = time.perf_counter()
+        scenes[kind] = (arrays, m.Scene(ctx, arrays))
+        build[kind] = round(time.perf_counter() - t0, 2)
+    log(f"scenes built: {build}")
+    return scenes, build
+
+
+def bench_batched(m, ctx, args, scenes=None):
+    """64 mixed jobs: each job alone (plain per-job wavefront) vs one server
+    packing all of them into shared batches (graph-replayed)."""
+    from milton_amd import workloads
+    if scenes is None:
+        scenes, _ = system_scenes(m, ctx)
+    cams = {k: v[0].camera for k, v in scenes.items()}
+    jobs = workloads.batched_jobs(cams, 64, seed=11)
+    total = sum(j.width * j.height * j.spp for _, j in jobs)
+    for kind, (_, sc) in scenes.items():   # warm up graphs and caches per scene
+        j = m.make_job(cams[kind], 64, 64, 4, 8, seed=1)
+        sc.render(j)
+    films = [m.Film(ctx, j.width, j.height) for _, j in jobs]
+    ctx.sync()
+    t0 = time.perf_counter()
+    for (kind, j), f in zip(jobs, films):
+        f.render(scenes[kind][1], j)
+    ctx.sync()
+    t_plain = time.perf_counter() - t0
+    srv = m.Server(ctx)
+    t0 = time.perf_counter()
+    handles = [srv.submit(scenes[kind][1], j) for kind, j in jobs]
+    srv.drain()
+    t_batched = time.perf_counter() - t0
+    st = srv.stats()
+    same = 0
+    for h, (kind, j), f in zip(handles, jobs, films):
+        same += int(np.array_equal(srv.result(h, j.width, j.height), f.download()))
+        f.close()
+    srv.close()
+    log(f"batched: {len(jobs)} jobs, {total} paths, plain {t_plain:.3f}s, batched {t_batched:.3f}s, "
+        f"{st['batches']} batches, identical films {same}/{len(jobs)}")
+    return {"paths": total, "t_plain": t_plain, "t_batched": t_batched, "batches": st["batches"],
+            "identical": same, "jobs": len(jobs), "extension_rays": st["extension_rays"],
+            "shadow_rays": st["shadow_rays"]}
+
+
+def bench_serving(m, ctx, args, scenes, capacity_jobs_s):
+    """Poisson arrivals at 60% of the measured batch capacity; latency is
+    submit -> film ready (host clock of the server)."""
+    from milton_amd import workloads
+    cams = {k: v[0].camera for k, v in scenes.items()}
+    rate = 0.6 * capacity_jobs_s
+    arrivals = workloads.poisson_arrivals(rate, args.serve_seconds, seed=13)
+    rng = np.random.default_rng(13)
+    jobs = [workloads.draw_job(rng, cams, 1000 + i) for i in range(len(arrivals))]
+    srv = m.Server(ctx)
+    srv.start()
+    t0 = time.perf_counter()
+    handles = []
+    for t_arr, (kind, j) in zip(arrivals, jobs):
+        delay = t_arr - (time.perf_counter() - t0)
+        if delay > 0:
+            time.sleep(delay)
+        handles.append(srv.submit(scenes[kind][1], j))
+    for h in handles:
+        srv.wait(h, 600000)
+    t_end = time.perf_counter() - t0
+    lat, paths = [], 0
+    for h, (_, j) in zip(handles, jobs):
+        s = srv.status(h)
+        lat.append(s["done_ms"] - s["submit_ms"])
+        paths += j.width * j.height * j.spp
+    srv.stop()
+    srv.close()
+    lat = np.array(lat)
+    return {"rate_jobs_s": rate, "jobs": len(jobs), "seconds": t_end, "paths_per_s": paths / t_end,
+            "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
+            "max_ms": float(lat.max())}
+
+
+def main_systems(args):
+    import milton_amd as m
+    ctx = m.Context(0)
+    if args.batch_paths:
+        ctx.set_option(m.MPT_OPT_BATCH_PATHS, args.batch_paths)
+    scenes, build = system_scenes(m, ctx)
+    b = bench_batched(m, ctx, args, scenes)
+    out = {"metric": "paths/s", "value": b["paths"] / b["t_batched"], "unit": "paths/s", "n_gpus": 1,
+           "steps": 1, "warmup": 1, "ms_per_step": 1000 * b["t_batched"], "higher_is_better": True,
+           "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+           "config": {"workload": "batched: 64 jobs over room / 1M soup / 2048^2 terrain, sides 128-1024, "
+                                  "spp 4-256, depth 4-8", "scene_build_s": build},
+           "batched": b, "speedup_vs_plain": b["t_plain"] / b["t_batched"],
+           "plain_paths_per_s": b["paths"] / b["t_plain"]}
+    if args.workload == "serving":
+        out["serving"] = bench_serving(m, ctx, args, scenes, b["jobs"] / b["t_batched"])
+    ctx.close()
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -85,6 +188,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--batch-paths", type=int, default=0)
     ap.add_argument("--stack", type=int, default=0, help="traversal short-stack entries (8, 12, 16, 24)")
+    ap.add_argument("--workload", default="soup", choices=["soup", "batched", "serving"],
+                    help="soup: the headline kd-tree stress scene; batched / serving: the systems workloads")
+    ap.add_argument("--serve-seconds", type=float, default=20.0)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a 256 MiB device-to-device copy (reference dispatch for HBM counters)")
     args = ap.parse_args()
@@ -92,6 +198,9 @@ def main():
         print(f"bench.py measures one GPU: this build runs on a single MI355X; --gpus {args.gpus} is not "
               f"supported (paths shard by pixel range; see the design notes)", file=sys.stderr)
         sys.exit(2)
+    if args.workload != "soup":
+        main_systems(args)
+        return
 
     import milton_amd as m
     ctx = m.Context(0)

@@ -153,7 +153,7 @@ enum {
   MPT_OPT_TIMING = 2,      /* 1: time every kernel launch with HIP events into *_ms */
   MPT_OPT_COUNTERS = 3,    /* 1: count kd nodes and references visited by closest-hit rays */
   MPT_OPT_GRAPHS = 4,      /* 1 (default): replay each batch's kernels from a captured hipGraph */
-  MPT_OPT_STACK = 5        /* traversal short-stack entries per ray: 8, 12, 16 (default) or 24 */
+  MPT_OPT_STACK = 5        /* traversal short-stack entries per ray: 8 (default), 12, 16 or 24 */
 };
 
 typedef struct mpt_context mpt_context;

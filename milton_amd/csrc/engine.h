@@ -130,7 +130,7 @@ struct mpt_context {
   bool timing = false;
   bool counters = false;
   bool graphs = true;
-  uint32_t stack = 16;               // short-stack entries of the traversal kernels
+  uint32_t stack = 8;                // short-stack entries of the traversal kernels
   std::vector<hipEvent_t> events;    // timing pool, pairs per launch
   std::vector<mpt_scene *> scenes;   // live scenes, films and servers, released by mpt_close
   std::vector<mpt_film *> films;

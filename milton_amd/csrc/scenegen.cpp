@@ -147,9 +147,14 @@ double value_noise(uint32_t seed, uint32_t octave, double x, double z) {
   const double c = lattice(seed, octave, ix, iz + 1), d = lattice(seed, octave, ix + 1, iz + 1);
   return (a * (1 - tx) + b * tx) * (1 - tz) + (c * (1 - tx) + d * tx) * tz;
 }
+// World size of the terrain: 500 spheres of radius 0.01-0.05 then cover about
+// 10% of it, the dielectric share of the materials.
+const double kTerrainSize = 4.0;
+
 double terrain_height(uint32_t seed, double x, double z) {
-  // 6 octaves, amplitude 0.2, lacunarity 2, gain 0.5, base frequency 4.
-  double h = 0, amp = 0.2, freq = 4.0;
+  // 6 octaves, amplitude 0.2, lacunarity 2, gain 0.5; the first octave has
+  // 4 noise cells across the terrain.
+  double h = 0, amp = 0.2, freq = 4.0 / kTerrainSize;
   for (uint32_t o = 0; o < 6; ++o) {
     h += amp * value_noise(seed, o, x * freq, z * freq);
     amp *= 0.5;
@@ -176,7 +181,7 @@ void build_terrain(Gen &g, uint32_t side, uint32_t seed) {
   g.mats = {lam, phong, glass, sun};
   const uint32_t n = side;
   g.verts.resize(3ull * n * n);
-  const double inv = 1.0 / (n - 1);
+  const double inv = kTerrainSize / (n - 1);
   for (uint32_t j = 0; j < n; ++j)
     for (uint32_t i = 0; i < n; ++i) {
       const double x = i * inv, z = j * inv;
@@ -207,15 +212,15 @@ void build_terrain(Gen &g, uint32_t side, uint32_t seed) {
   for (uint32_t s = 0; s < 500; ++s) {
     uint32_t r[4];
     mpt_rng::philox_words(seed, 0x53504852u, s, 0, r);   // "SPHR"
-    const double x = 0.05 + 0.9 * mpt_rng::unit32(r[0]);
-    const double z = 0.05 + 0.9 * mpt_rng::unit32(r[1]);
+    const double x = kTerrainSize * (0.05 + 0.9 * mpt_rng::unit32(r[0]));
+    const double z = kTerrainSize * (0.05 + 0.9 * mpt_rng::unit32(r[1]));
     const double rad = 0.01 + 0.04 * mpt_rng::unit32(r[2]);
     const float sp[4] = {(float)x, (float)(terrain_height(seed, x, z) + rad), (float)z, (float)rad};
     g.spheres.insert(g.spheres.end(), sp, sp + 4);
     g.sph_mat.push_back(2);
   }
   // Distant area light: a square at distance 100 subtending 0.5 degrees,
-  // facing the terrain centre.
+  // facing the terrain centre (2, 0, 2).
   const double dir[3] = {0.3, 1.0, 0.2};
   const double dl = std::sqrt(dir[0] * dir[0] + dir[1] * dir[1] + dir[2] * dir[2]);
   const double l[3] = {dir[0] / dl, dir[1] / dl, dir[2] / dl};
@@ -227,13 +232,14 @@ void build_terrain(Gen &g, uint32_t side, uint32_t seed) {
   a[0] /= al;
   a[1] /= al;
   const double b[3] = {l[1] * a[2] - l[2] * a[1], l[2] * a[0] - l[0] * a[2], l[0] * a[1] - l[1] * a[0]};
-  const float q[9] = {(float)(0.5 + 100.0 * l[0]), (float)(100.0 * l[1]), (float)(0.5 + 100.0 * l[2]),
+  const double cx = 0.5 * kTerrainSize, cz = 0.5 * kTerrainSize;
+  const float q[9] = {(float)(cx + 100.0 * l[0]), (float)(100.0 * l[1]), (float)(cz + 100.0 * l[2]),
                       (float)(side_len * a[0]), (float)(side_len * a[1]), (float)(side_len * a[2]),
                       (float)(side_len * b[0]), (float)(side_len * b[1]), (float)(side_len * b[2])};
   g.quads.assign(q, q + 9);
   g.quad_mat.push_back(3);
-  const double eye[3] = {0.5, 0.6, -0.3};
-  const double look[3] = {0.0, -0.5, 0.9};
+  const double eye[3] = {2.0, 1.4, -0.6};
+  const double look[3] = {0.0, -0.55, 1.0};
   set_camera(g, eye, look, 45.0f, 1.0f);
 }
 

@@ -93,7 +93,10 @@ def test_terrain_generator_shape():
     n = np.cross(q[6:9], q[3:6])
     assert n[1] < 0   # the distant light faces down
     side = np.linalg.norm(q[3:6])
-    assert abs(2 * np.degrees(np.arctan(side / 2 / np.linalg.norm(q[:3] - [0.5, 0, 0.5]))) - 0.5) < 0.01
+    assert abs(2 * np.degrees(np.arctan(side / 2 / np.linalg.norm(q[:3] - [2.0, 0, 2.0]))) - 0.5) < 0.01
+    # spheres cover about 10% of the 4 x 4 terrain
+    cover = np.pi * (r.astype(np.float64) ** 2).sum() / 16.0
+    assert 0.05 < cover < 0.15
 
 
 def test_generated_rays():

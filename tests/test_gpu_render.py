@@ -112,3 +112,24 @@ def test_ray_counts_match_oracle(ctx):
     _, _, c = osc.render(job)
     assert st["camera_rays"] == c["camera"]
     assert abs(st["extension_rays"] - c["extension"]) <= 0.01 * c["extension"]
+
+
+def test_terrain_paths_same_seed(ctx):
+    """Diffuse + modified Phong terrain, dielectric spheres (hero-wavelength
+    channel rule), distant area light, depth 8."""
+    arrays = m.generate(m.MPT_SCENE_TERRAIN, 256, 1)
+    job = m.make_job(arrays.camera, 64, 64, 16, 8, seed=1)
+    _per_path(ctx, arrays, job, 0, 64 * 64, "terrain-paths", min_frac=0.97)
+
+
+def test_terrain_image_three_sigma(ctx):
+    arrays = m.generate(m.MPT_SCENE_TERRAIN, 256, 1)
+    job_gpu = m.make_job(arrays.camera, 32, 32, 256, 8, seed=1)
+    job_orc = m.make_job(arrays.camera, 32, 32, 256, 8, seed=2)
+    sc = m.Scene(ctx, arrays)
+    L_gpu = sc.render_paths(job_gpu, 0, 32 * 32).astype(np.float64)
+    osc = po.OracleScene(arrays)
+    pix = np.repeat(np.arange(32 * 32, dtype=np.uint32), 256)
+    smp = np.tile(np.arange(256, dtype=np.uint32), 32 * 32)
+    L_orc = osc.render_paths(job_orc, pix, smp)
+    _three_sigma(L_gpu, L_orc, 256, "terrain-3sigma", min_frac=0.97)

@@ -111,3 +111,16 @@ def test_any_hit(ctx):
     # any-hit is consistent with closest-hit
     hits = sc.trace_closest(rays)
     assert np.all(occ[hits["t"] < tmax * (1 - 1e-5)] == 1)
+
+
+def test_terrain_closest(ctx):
+    """Terrain heightfield (shared vertices) with 500 spheres and the distant
+    light quad: triangles, spheres and quads in one tree."""
+    arrays = m.generate(m.MPT_SCENE_TERRAIN, 256, 1)
+    rays = m.generate_rays(m.MPT_SCENE_TERRAIN, 256, 400_000, seed=7, film=(512, 512))
+    # shift the random half onto the 4 x 4 terrain
+    rays["o"][200_000:] *= np.float32(4.0)
+    rays["o"][200_000:, 1] *= np.float32(0.1)
+    hits, orc_t, orc_id = _check_closest(ctx, arrays, rays, "terrain")
+    nt = len(arrays.indices)
+    assert ((orc_id >= nt) & (orc_id != 0xFFFFFFFF)).sum() > 1000   # spheres are hit
