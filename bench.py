@@ -188,6 +188,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--batch-paths", type=int, default=0)
     ap.add_argument("--stack", type=int, default=0, help="traversal short-stack entries (8, 12, 16, 24)")
+    ap.add_argument("--persistent", type=int, default=-1, help="1/0: persistent closest-hit waves on/off")
     ap.add_argument("--workload", default="soup", choices=["soup", "batched", "serving"],
                     help="soup: the headline kd-tree stress scene; batched / serving: the systems workloads")
     ap.add_argument("--serve-seconds", type=float, default=20.0)
@@ -208,6 +209,8 @@ def main():
         ctx.set_option(m.MPT_OPT_BATCH_PATHS, args.batch_paths)
     if args.stack:
         ctx.set_option(m.MPT_OPT_STACK, args.stack)
+    if args.persistent >= 0:
+        ctx.set_option(m.MPT_OPT_PERSISTENT, args.persistent)
     if args.copy_calibration:
         copy_calibration()
     t0 = time.perf_counter()

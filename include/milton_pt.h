@@ -101,6 +101,7 @@ typedef struct mpt_kd_params {
   float empty_bias;       /* kdEmptyBias */
   uint32_t num_threads;   /* kdNoThreads; 0 = all host cores */
   uint32_t bins;          /* SAH bins per axis (0 = default) */
+  uint32_t no_clip;       /* 1: split triangles by their bounds only (0: clip them to the cells) */
 } mpt_kd_params;
 
 /* Replaces ThinLensCamera's keys with aperture 0 (cameras/PinholeCamera.cpp:22,
@@ -153,7 +154,9 @@ enum {
   MPT_OPT_TIMING = 2,      /* 1: time every kernel launch with HIP events into *_ms */
   MPT_OPT_COUNTERS = 3,    /* 1: count kd nodes and references visited by closest-hit rays */
   MPT_OPT_GRAPHS = 4,      /* 1 (default): replay each batch's kernels from a captured hipGraph */
-  MPT_OPT_STACK = 5        /* traversal short-stack entries per ray: 8 (default), 12, 16 or 24 */
+  MPT_OPT_STACK = 5,       /* traversal short-stack entries per ray: 8 (default), 12, 16 or 24 */
+  MPT_OPT_PERSISTENT = 6   /* closest hit: 0 one ray per lane; 1 (default) / 2 persistent waves that
+                              refill finished lanes with new rays, register budget for 7 / 8 waves */
 };
 
 typedef struct mpt_context mpt_context;
@@ -178,6 +181,19 @@ int mpt_scene_create(mpt_context *ctx, const mpt_scene_desc *desc, const mpt_kd_
 int mpt_scene_destroy(mpt_scene *scene);
 /* out: [nodes, leaves, references, max_depth, device_bytes, build_ms, max_leaf, triangles] */
 int mpt_scene_stats(const mpt_scene *scene, uint64_t out[8]);
+
+/* Host-only kd build and trace: the tree mpt_scene_create would upload, built
+ * without a device, and closest hits through it computed on host threads by
+ * the same traversal and intersection code the kernels run.  For inspecting
+ * and tuning trees and for checking the traversal where no device is present.
+ * counters (optional) = [kd nodes fetched, leaf references tested]. */
+typedef struct mpt_kd_tree mpt_kd_tree;
+int mpt_kd_build(const mpt_scene_desc *desc, const mpt_kd_params *kd, mpt_kd_tree **tree);
+/* out: [nodes, leaves, references, max_depth, build_ms, max_leaf] */
+int mpt_kd_stats(const mpt_kd_tree *tree, uint64_t out[6]);
+int mpt_kd_trace(const mpt_kd_tree *tree, const mpt_ray *rays, uint64_t n, mpt_hit *hits, uint64_t counters[2],
+                 uint32_t nthreads);
+int mpt_kd_destroy(mpt_kd_tree *tree);
 
 /* Closest hit for n host rays.  Replaces Scene::getIntersection
  * (core/Scene.cpp:104-108) -> kdTreeAccel::getIntersection

@@ -21,7 +21,7 @@ MPT_ABI_VERSION = 1
 MPT_MISS = 0xFFFFFFFF
 MPT_BSDF_DIFFUSE, MPT_BSDF_PHONG, MPT_BSDF_DIELECTRIC = 0, 1, 2
 MPT_SCENE_ROOM, MPT_SCENE_SOUP, MPT_SCENE_TERRAIN = 1, 2, 3
-MPT_OPT_BATCH_PATHS, MPT_OPT_TIMING, MPT_OPT_COUNTERS, MPT_OPT_GRAPHS, MPT_OPT_STACK = 1, 2, 3, 4, 5
+MPT_OPT_BATCH_PATHS, MPT_OPT_TIMING, MPT_OPT_COUNTERS, MPT_OPT_GRAPHS, MPT_OPT_STACK, MPT_OPT_PERSISTENT = 1, 2, 3, 4, 5, 6
 ERRORS = {-1: "invalid", -2: "device", -3: "nomem", -4: "budget", -5: "notfound", -6: "cancelled"}
 
 _f32p = ctypes.POINTER(ctypes.c_float)
@@ -48,7 +48,8 @@ class KdParams(ctypes.Structure):
     """kdTreeAccel::BuildParams (accel/kdTreeAccel.h:68-116)."""
     _fields_ = [("leaf_size", ctypes.c_uint32), ("max_depth", ctypes.c_uint32),
                 ("cost_traversal", ctypes.c_float), ("cost_intersect", ctypes.c_float),
-                ("empty_bias", ctypes.c_float), ("num_threads", ctypes.c_uint32), ("bins", ctypes.c_uint32)]
+                ("empty_bias", ctypes.c_float), ("num_threads", ctypes.c_uint32), ("bins", ctypes.c_uint32),
+                ("no_clip", ctypes.c_uint32)]
 
 
 class Camera(ctypes.Structure):
@@ -145,6 +146,10 @@ def lib():
         "mpt_server_start": ([vp], i32),
         "mpt_server_stop": ([vp], i32),
         "mpt_server_stats": ([vp, ctypes.POINTER(ctypes.c_uint64)], i32),
+        "mpt_kd_build": ([ctypes.POINTER(SceneDesc), ctypes.POINTER(KdParams), ctypes.POINTER(vp)], i32),
+        "mpt_kd_stats": ([vp, ctypes.POINTER(ctypes.c_uint64)], i32),
+        "mpt_kd_trace": ([vp, vp, u64, vp, ctypes.POINTER(ctypes.c_uint64), u32], i32),
+        "mpt_kd_destroy": ([vp], i32),
         "mpt_gen_create": ([u32, u32, u32, ctypes.POINTER(vp)], i32),
         "mpt_gen_describe": ([vp, ctypes.POINTER(SceneDesc), ctypes.POINTER(Camera)], i32),
         "mpt_gen_destroy": ([vp], i32),
@@ -472,6 +477,45 @@ class Server:
         if self.h and self.ctx.h:
             self.ctx.L.mpt_server_destroy(self.h)
         self.h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class HostKdTree:
+    """The device kd-tree built on the host (mpt_kd_build), with host-thread
+    closest hits through the same traversal code as the kernels."""
+
+    def __init__(self, arrays, kd=None):
+        self.L = lib()
+        self.h = ctypes.c_void_p()
+        d = arrays.desc()
+        kp = kd if kd is not None else KdParams()
+        rc = self.L.mpt_kd_build(ctypes.byref(d), ctypes.byref(kp), ctypes.byref(self.h))
+        if rc:
+            raise MptError(f"mpt_kd_build failed: {ERRORS.get(rc, rc)}")
+
+    def stats(self):
+        out = (ctypes.c_uint64 * 6)()
+        self.L.mpt_kd_stats(self.h, out)
+        return dict(zip(["nodes", "leaves", "refs", "max_depth", "build_ms", "max_leaf"], list(out)))
+
+    def trace_closest(self, rays, threads=0):
+        rays = np.ascontiguousarray(rays, RAY_DTYPE)
+        hits = np.zeros(len(rays), HIT_DTYPE)
+        ctr = (ctypes.c_uint64 * 2)()
+        rc = self.L.mpt_kd_trace(self.h, _ptr(rays), len(rays), _ptr(hits), ctr, threads)
+        if rc:
+            raise MptError(f"mpt_kd_trace failed: {ERRORS.get(rc, rc)}")
+        return hits, {"nodes": ctr[0], "refs": ctr[1]}
+
+    def close(self):
+        if self.h:
+            self.L.mpt_kd_destroy(self.h)
+            self.h = ctypes.c_void_p()
 
     def __del__(self):
         try:

@@ -17,6 +17,21 @@
 
 namespace mpt_dev {
 
+// Traversal and intersection compile for the host too, so the host-side
+// trace entry point runs exactly the code of the kernels.
+#define MPT_HD __host__ __device__ __forceinline__
+
+MPT_HD float mpt_u2f(uint32_t u) {
+  float f;
+  __builtin_memcpy(&f, &u, 4);
+  return f;
+}
+MPT_HD uint32_t mpt_f2u(float f) {
+  uint32_t u;
+  __builtin_memcpy(&u, &f, 4);
+  return u;
+}
+
 constexpr uint32_t kMiss = 0xFFFFFFFFu;
 constexpr float kEps = 1e-7f;            // EPSILON, common/arch.h:137
 constexpr float kAirIor = 1.0003f;       // IndexOfRefraction::AIR
@@ -44,41 +59,42 @@ struct Scene {
   float env[3];
 };
 
-__device__ __forceinline__ float3 f3(float x, float y, float z) { return make_float3(x, y, z); }
-__device__ __forceinline__ float3 add(float3 a, float3 b) { return f3(a.x + b.x, a.y + b.y, a.z + b.z); }
-__device__ __forceinline__ float3 sub(float3 a, float3 b) { return f3(a.x - b.x, a.y - b.y, a.z - b.z); }
-__device__ __forceinline__ float3 mul(float3 a, float s) { return f3(a.x * s, a.y * s, a.z * s); }
-__device__ __forceinline__ float3 mul(float3 a, float3 b) { return f3(a.x * b.x, a.y * b.y, a.z * b.z); }
-__device__ __forceinline__ float dot(float3 a, float3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
-__device__ __forceinline__ float3 cross(float3 a, float3 b) {
+MPT_HD float3 f3(float x, float y, float z) { return make_float3(x, y, z); }
+MPT_HD float3 add(float3 a, float3 b) { return f3(a.x + b.x, a.y + b.y, a.z + b.z); }
+MPT_HD float3 sub(float3 a, float3 b) { return f3(a.x - b.x, a.y - b.y, a.z - b.z); }
+MPT_HD float3 mul(float3 a, float s) { return f3(a.x * s, a.y * s, a.z * s); }
+MPT_HD float3 mul(float3 a, float3 b) { return f3(a.x * b.x, a.y * b.y, a.z * b.z); }
+MPT_HD float dot(float3 a, float3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+MPT_HD float3 cross(float3 a, float3 b) {
   return f3(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
 }
 __device__ __forceinline__ float3 normalize(float3 a) { return mul(a, rsqrtf(dot(a, a))); }
-__device__ __forceinline__ float len(float3 a) { return sqrtf(dot(a, a)); }
-__device__ __forceinline__ float3 xyz(float4 a) { return f3(a.x, a.y, a.z); }
-__device__ __forceinline__ float comp(float3 a, uint32_t i) { return i == 0 ? a.x : (i == 1 ? a.y : a.z); }
-__device__ __forceinline__ bool is_black(float3 a) { return a.x == 0.0f && a.y == 0.0f && a.z == 0.0f; }
-__device__ __forceinline__ float max3(float3 a) { return fmaxf(a.x, fmaxf(a.y, a.z)); }
+MPT_HD float len(float3 a) { return sqrtf(dot(a, a)); }
+MPT_HD float3 xyz(float4 a) { return f3(a.x, a.y, a.z); }
+MPT_HD float comp(float3 a, uint32_t i) { return i == 0 ? a.x : (i == 1 ? a.y : a.z); }
+MPT_HD bool is_black(float3 a) { return a.x == 0.0f && a.y == 0.0f && a.z == 0.0f; }
+MPT_HD float max3(float3 a) { return fmaxf(a.x, fmaxf(a.y, a.z)); }
 
-// Per-ray constants: reciprocal direction for the kd planes and the shear of
-// the watertight triangle test (Woop, Benthin, Wald, JCGT 2013): row vectors
-// mx, my, mz map (v - o) to the ray's sheared frame.
+// Per-ray constants: origin, direction, and the shear of the watertight
+// triangle test (Woop, Benthin, Wald, JCGT 2013): the axis permutation
+// (kx, ky, kz packed in k) and the factors sx, sy, sz.  The reciprocal
+// direction for the kd planes is recomputed per node (one rcp) to keep the
+// per-lane state small.
 struct RaySetup {
-  float3 o, d, inv;
-  float3 mx, my, mz;
-  float dkz;   // |d[kz]|, relates the sheared determinant to Moller-Trumbore's
+  float3 o, d;
+  float sx, sy, sz;
+  uint32_t k;   // kx | ky << 2 | kz << 4
 };
 
-__device__ __forceinline__ float safe_inv(float x) {
+MPT_HD float safe_inv(float x) {
   const float ax = fmaxf(fabsf(x), 1e-20f);
   return 1.0f / copysignf(ax, x);
 }
 
-__device__ __forceinline__ RaySetup setup_ray(float3 o, float3 d) {
+MPT_HD RaySetup setup_ray(float3 o, float3 d) {
   RaySetup r;
   r.o = o;
   r.d = d;
-  r.inv = f3(safe_inv(d.x), safe_inv(d.y), safe_inv(d.z));
   const float ax = fabsf(d.x), ay = fabsf(d.y), az = fabsf(d.z);
   uint32_t kz = (ax >= ay && ax >= az) ? 0 : (ay >= az ? 1 : 2);
   uint32_t kx = kz == 2 ? 0 : kz + 1;
@@ -89,13 +105,10 @@ __device__ __forceinline__ RaySetup setup_ray(float3 o, float3 d) {
     kx = ky;
     ky = t;
   }
-  const float sx = comp(d, kx) / dz, sy = comp(d, ky) / dz, sz = 1.0f / dz;
-  r.mx = f3(kx == 0 ? 1.0f : (kz == 0 ? -sx : 0.0f), kx == 1 ? 1.0f : (kz == 1 ? -sx : 0.0f),
-            kx == 2 ? 1.0f : (kz == 2 ? -sx : 0.0f));
-  r.my = f3(ky == 0 ? 1.0f : (kz == 0 ? -sy : 0.0f), ky == 1 ? 1.0f : (kz == 1 ? -sy : 0.0f),
-            ky == 2 ? 1.0f : (kz == 2 ? -sy : 0.0f));
-  r.mz = f3(kz == 0 ? sz : 0.0f, kz == 1 ? sz : 0.0f, kz == 2 ? sz : 0.0f);
-  r.dkz = fabsf(dz);
+  r.sx = comp(d, kx) / dz;
+  r.sy = comp(d, ky) / dz;
+  r.sz = 1.0f / dz;
+  r.k = kx | (ky << 2) | (kz << 4);
   return r;
 }
 
@@ -103,25 +116,27 @@ __device__ __forceinline__ RaySetup setup_ray(float3 o, float3 d) {
 // rules: edges and vertices count as hits (inclusive u, v, u + v <= 1), and a
 // triangle whose Moller-Trumbore determinant |d . ((v1-v0) x (v2-v0))| is at
 // most EPSILON is rejected (EQ(det, 0), MeshTriangle.cpp:144).  The sheared
-// determinant U + V + W equals that product divided by d[kz].
-__device__ __forceinline__ float tri_t(const RaySetup &r, float3 v0, float3 v1, float3 v2) {
+// determinant U + V + W equals that product divided by d[kz] = 1 / sz.
+MPT_HD float tri_t(const RaySetup &r, float3 v0, float3 v1, float3 v2) {
+  const uint32_t kx = r.k & 3u, ky = (r.k >> 2) & 3u, kz = r.k >> 4;
   const float3 A = sub(v0, r.o), B = sub(v1, r.o), C = sub(v2, r.o);
-  const float Ax = dot(A, r.mx), Ay = dot(A, r.my);
-  const float Bx = dot(B, r.mx), By = dot(B, r.my);
-  const float Cx = dot(C, r.mx), Cy = dot(C, r.my);
+  const float Az = comp(A, kz), Bz = comp(B, kz), Cz = comp(C, kz);
+  const float Ax = comp(A, kx) - r.sx * Az, Ay = comp(A, ky) - r.sy * Az;
+  const float Bx = comp(B, kx) - r.sx * Bz, By = comp(B, ky) - r.sy * Bz;
+  const float Cx = comp(C, kx) - r.sx * Cz, Cy = comp(C, ky) - r.sy * Cz;
   const float U = Cx * By - Cy * Bx;
   const float V = Ax * Cy - Ay * Cx;
   const float W = Bx * Ay - By * Ax;
   if ((U < 0.0f || V < 0.0f || W < 0.0f) && (U > 0.0f || V > 0.0f || W > 0.0f)) return INFINITY;
   const float det = U + V + W;
-  if (fabsf(det) * r.dkz <= kEps) return INFINITY;
-  const float T = U * dot(A, r.mz) + V * dot(B, r.mz) + W * dot(C, r.mz);
+  if (fabsf(det) <= kEps * fabsf(r.sz)) return INFINITY;
+  const float T = (U * Az + V * Bz + W * Cz) * r.sz;
   return T / det;
 }
 
 // Sphere (centre c, radius rad): nearest root above EPSILON; a ray leaving the
 // sphere it starts on skips the root at its own origin.
-__device__ __forceinline__ float sphere_t(const RaySetup &r, float3 c, float rad, bool from_self) {
+MPT_HD float sphere_t(const RaySetup &r, float3 c, float rad, bool from_self) {
   const float3 f = sub(r.o, c);
   const float b = -dot(f, r.d);
   const float3 l = add(f, mul(r.d, b));
@@ -142,7 +157,7 @@ __device__ __forceinline__ float sphere_t(const RaySetup &r, float3 c, float rad
 }
 
 // Quad: the reference unit plane mapped by (ex, ez) around c, two-sided.
-__device__ __forceinline__ float quad_t(const RaySetup &r, float3 c, float3 ex, float3 ez) {
+MPT_HD float quad_t(const RaySetup &r, float3 c, float3 ex, float3 ez) {
   const float3 n = cross(ez, ex);
   const float dn = dot(n, r.d);
   const float t = dot(n, sub(c, r.o)) / dn;
@@ -155,9 +170,9 @@ __device__ __forceinline__ float quad_t(const RaySetup &r, float3 c, float3 ex, 
 // solve per ray), so t matches the reference's FP64 intersect
 // (MeshTriangle.cpp:92-169 computes t in double) even for hits very close to
 // the ray origin, where the FP32 edge-function t loses relative accuracy.
-__device__ __forceinline__ float refine_t(const float4 *rec, const RaySetup &r, float t32) {
+MPT_HD float refine_t(const float4 *rec, const RaySetup &r, float t32) {
   const float4 a = rec[0], b = rec[1], c = rec[2];
-  const uint32_t kind = __float_as_uint(b.w);
+  const uint32_t kind = mpt_f2u(b.w);
   const double ox = r.o.x, oy = r.o.y, oz = r.o.z, dx = r.d.x, dy = r.d.y, dz = r.d.z;
   if (kind == KIND_SPHERE) {
     const double fx = ox - a.x, fy = oy - a.y, fz = oz - a.z, rad = b.x;
@@ -188,43 +203,53 @@ __device__ __forceinline__ float refine_t(const float4 *rec, const RaySetup &r, 
 // overflow (Foley & Sugerman 2005).  `stk` points at this lane's column of a
 // [STACK][kBlock] uint2 array.  skip_origin is the primitive the ray leaves
 // (self-intersection guard in place of the reference's FP64 t > EPSILON),
-// skip_target a shadow ray's light.  ANY=true returns at the first hit.
+// ANY=true finishes at the first hit.
 // COUNT=true adds the nodes fetched and references tested to n_nodes/n_refs.
-template <bool ANY, bool COUNT = false, int STACK = kStack>
-__device__ __forceinline__ bool trace(const Scene &sc, const RaySetup &r, float tlimit, uint32_t skip_origin,
-                                      uint32_t skip_target, uint2 *stk, float &t_out, uint32_t &id_out,
-                                      uint32_t *n_nodes = nullptr, uint32_t *n_refs = nullptr) {
-  // Root slab test (accel/AABB.inl:25-56).
-  float tmin = 0.0f, tmax = tlimit;
-  {
-    const float tx0 = (sc.box_lo[0] - r.o.x) * r.inv.x, tx1 = (sc.box_hi[0] - r.o.x) * r.inv.x;
-    const float ty0 = (sc.box_lo[1] - r.o.y) * r.inv.y, ty1 = (sc.box_hi[1] - r.o.y) * r.inv.y;
-    const float tz0 = (sc.box_lo[2] - r.o.z) * r.inv.z, tz1 = (sc.box_hi[2] - r.o.z) * r.inv.z;
-    tmin = fmaxf(tmin, fmaxf(fminf(tx0, tx1), fmaxf(fminf(ty0, ty1), fminf(tz0, tz1))));
-    tmax = fminf(tmax, fminf(fmaxf(tx0, tx1), fminf(fmaxf(ty0, ty1), fmaxf(tz0, tz1))));
+//
+// The walk is resumable: start() clips the ray to the scene box, each step()
+// descends to the next leaf, tests its references and pops the next cell, and
+// returns true when the ray is finished.  Splitting the walk into leaf steps
+// lets a persistent wave hand a finished lane a new ray while its other lanes
+// are still walking.
+template <bool ANY, int STACK>
+struct Walker {
+  float tmin, tmax, troot, best;
+  uint32_t best_rec, node, skip_origin;   // best_rec == kMiss: no hit yet
+  int depth, top;   // valid ring entries, next write slot
+  bool dropped;
+
+  MPT_HD bool start(const Scene &sc, const RaySetup &r, float tlimit, uint32_t so) {
+    // Root slab test (accel/AABB.inl:25-56).
+    const float ix = safe_inv(r.d.x), iy = safe_inv(r.d.y), iz = safe_inv(r.d.z);
+    const float tx0 = (sc.box_lo[0] - r.o.x) * ix, tx1 = (sc.box_hi[0] - r.o.x) * ix;
+    const float ty0 = (sc.box_lo[1] - r.o.y) * iy, ty1 = (sc.box_hi[1] - r.o.y) * iy;
+    const float tz0 = (sc.box_lo[2] - r.o.z) * iz, tz1 = (sc.box_hi[2] - r.o.z) * iz;
+    tmin = fmaxf(0.0f, fmaxf(fminf(tx0, tx1), fmaxf(fminf(ty0, ty1), fminf(tz0, tz1))));
+    tmax = fminf(tlimit, fminf(fmaxf(tx0, tx1), fminf(fmaxf(ty0, ty1), fmaxf(tz0, tz1))));
     // widen by a few ulps so rays grazing the root box still enter it
     tmin = fmaxf(0.0f, tmin * (1.0f - 1e-6f) - 1e-7f);
     tmax = tmax * (1.0f + 1e-6f) + 1e-7f;
+    troot = tmax;
+    best = tlimit;
+    best_rec = kMiss;
+    node = 0;
+    depth = 0;
+    top = 0;
+    dropped = false;
+    skip_origin = so;
+    return tmin <= tmax;
   }
-  t_out = INFINITY;
-  id_out = kMiss;
-  if (tmin > tmax) return false;
-  const float troot = tmax;
-  float best = tlimit;
-  uint32_t best_id = kMiss;
-  uint32_t best_rec = 0;
-  uint32_t node = 0;
-  int depth = 0;     // valid entries on the ring
-  int top = 0;       // next write slot
-  bool dropped = false;
-  for (;;) {
+
+  template <bool COUNT>
+  MPT_HD bool step(const Scene &sc, const RaySetup &r, uint2 *stk, uint32_t *n_nodes,
+                                       uint32_t *n_refs) {
     uint2 nd = sc.nodes[node];
     if (COUNT) ++*n_nodes;
     while ((nd.x & 3u) != 3u) {
       const uint32_t axis = nd.x & 3u;
-      const float split = __uint_as_float(nd.y);
-      const float oa = comp(r.o, axis), da = comp(r.d, axis), ia = comp(r.inv, axis);
-      const float tp = (split - oa) * ia;
+      const float split = mpt_u2f(nd.y);
+      const float oa = comp(r.o, axis), da = comp(r.d, axis);
+      const float tp = (split - oa) * safe_inv(da);
       const uint32_t child = nd.x >> 2;
       const bool below = (oa < split) || (oa == split && da <= 0.0f);
       const uint32_t first = below ? child : child + 1;
@@ -234,7 +259,7 @@ __device__ __forceinline__ bool trace(const Scene &sc, const RaySetup &r, float 
       } else if (tp < tmin) {
         node = second;
       } else {
-        stk[top * kBlock] = make_uint2(second, __float_as_uint(tmax));
+        stk[top * kBlock] = make_uint2(second, mpt_f2u(tmax));
         top = top + 1 == STACK ? 0 : top + 1;
         if (depth == STACK) dropped = true;
         else ++depth;
@@ -249,9 +274,8 @@ __device__ __forceinline__ bool trace(const Scene &sc, const RaySetup &r, float 
     const float4 *rec = sc.recs + 3ull * nd.y;
     for (uint32_t k = 0; k < count; ++k, rec += 3) {
       const float4 a = rec[0], b = rec[1];
-      const uint32_t id = __float_as_uint(a.w);
-      const uint32_t kind = __float_as_uint(b.w);
-      if (id == skip_target) continue;
+      const uint32_t id = mpt_f2u(a.w);
+      const uint32_t kind = mpt_f2u(b.w);
       float t;
       if (kind == KIND_TRIANGLE) {
         if (id == skip_origin) continue;
@@ -266,39 +290,56 @@ __device__ __forceinline__ bool trace(const Scene &sc, const RaySetup &r, float 
       }
       if (t > kEps && t < best) {
         best = t;
-        best_id = id;
         best_rec = nd.y + k;
-        if (ANY) {
-          t_out = t;
-          id_out = id;
-          return true;
-        }
+        if (ANY) return true;
       }
     }
-    if (best <= tmax) break;
+    if (best <= tmax) return true;
     if (depth == 0) {
-      if (!dropped || tmax >= troot) break;
+      if (!dropped || tmax >= troot) return true;
       // restart from the root past the cells already visited
       dropped = false;
       node = 0;
       tmin = tmax;
       tmax = troot;
-      continue;
+      return false;
     }
     --depth;
     top = top == 0 ? STACK - 1 : top - 1;
     const uint2 e = stk[top * kBlock];
     node = e.x;
     tmin = tmax;
-    tmax = __uint_as_float(e.y);
-    if (tmin > best) break;
+    tmax = mpt_u2f(e.y);
+    return tmin > best;
   }
-  if (best_id != kMiss) {
-    t_out = refine_t(sc.recs + 3ull * best_rec, r, best);
-    id_out = best_id;
+
+  // Result of a finished walk: the closest hit's t is recomputed in double.
+  MPT_HD bool result(const Scene &sc, const RaySetup &r, float &t_out, uint32_t &id_out) const {
+    if (best_rec == kMiss) {
+      t_out = INFINITY;
+      id_out = kMiss;
+      return false;
+    }
+    const float4 *rec = sc.recs + 3ull * best_rec;
+    t_out = ANY ? best : refine_t(rec, r, best);
+    id_out = mpt_f2u(rec[0].w);
     return true;
   }
-  return false;
+};
+
+// kd traversal of one ray to completion (ANY: first hit below tlimit).
+template <bool ANY, bool COUNT = false, int STACK = kStack>
+MPT_HD bool trace(const Scene &sc, const RaySetup &r, float tlimit, uint32_t skip_origin, uint2 *stk,
+                  float &t_out, uint32_t &id_out, uint32_t *n_nodes = nullptr, uint32_t *n_refs = nullptr) {
+  Walker<ANY, STACK> w;
+  if (!w.start(sc, r, tlimit, skip_origin)) {
+    t_out = INFINITY;
+    id_out = kMiss;
+    return false;
+  }
+  while (!w.template step<COUNT>(sc, r, stk, n_nodes, n_refs)) {
+  }
+  return w.result(sc, r, t_out, id_out);
 }
 
 // ---------------------------------------------------------------------------

@@ -91,7 +91,7 @@ struct Workspace {
   DBuf<float4> Ls, betas;
   DBuf<uint32_t> flags;
   DBuf<uint2> pjs;           // (pixel, job << 20 | sample) per slot
-  DBuf<uint32_t> counts;
+  DBuf<uint32_t> counts;     // [0, D] queue sizes, [D+1, 2D] shadow sizes, [2D+1, 3D] ray fetch cursors
   DBuf<unsigned long long> totals;
   DBuf<BatchDesc> desc;
   size_t bytes() const {
@@ -131,6 +131,8 @@ struct mpt_context {
   bool counters = false;
   bool graphs = true;
   uint32_t stack = 8;                // short-stack entries of the traversal kernels
+  uint32_t persistent = 1;           // closest hit: 0 one ray per lane, 1/2 persistent waves with lane
+                                     // refill at 7 / 8 waves per SIMD
   std::vector<hipEvent_t> events;    // timing pool, pairs per launch
   std::vector<mpt_scene *> scenes;   // live scenes, films and servers, released by mpt_close
   std::vector<mpt_film *> films;

@@ -1,5 +1,14 @@
-// Host SAH kd-tree builder (binned SAH, subtree-parallel).  Layout and the
-// reference functions it replaces are described in kdbuild.h.
+// Host SAH kd-tree builder (binned SAH, clipped reference bounds,
+// subtree-parallel).  Layout and the reference functions it replaces are
+// described in kdbuild.h.
+//
+// Each reference carries its own bounds, clipped to the cell it lives in.
+// When a split cuts through a triangle's bounds, the triangle itself is
+// clipped against each child cell (Sutherland-Hodgman in double precision);
+// a child the triangle does not reach gets no reference, and the child's
+// reference bounds are tight.  This is the "perfect split" refinement of SAH
+// kd-trees (Havran; Wald & Havran 2006); the reference builder partitions by
+// unclipped primitive bounds (accel/kdTreeAccel.cpp:571-582).
 #include "kdbuild.h"
 
 #include <algorithm>
@@ -12,52 +21,128 @@
 namespace mpt_kd {
 namespace {
 
+struct Ref {
+  uint32_t prim;
+  Box box;
+};
+
 struct BNode {
   uint32_t axis = 3;   // 3 == leaf
   float split = 0;
   std::unique_ptr<BNode[]> kids;
-  std::vector<uint32_t> refs;
+  std::vector<uint32_t> prims;
 };
 
 inline float area(const float d[3]) { return 2.0f * (d[0] * d[1] + d[1] * d[2] + d[2] * d[0]); }
 
+// Clips triangle v (9 floats) against the cell (widened by a few ulps) and
+// returns the bounds of the remaining polygon, rounded outward.  False if
+// nothing of the triangle is inside.
+bool clip_triangle(const float *v, const Box &cell, Box &out) {
+  double poly[2][12][3];
+  int n = 3, cur = 0;
+  for (int i = 0; i < 3; ++i)
+    for (int k = 0; k < 3; ++k) poly[0][i][k] = v[3 * i + k];
+  for (int axis = 0; axis < 3; ++axis) {
+    for (int side = 0; side < 2; ++side) {
+      const double lim = side == 0 ? (double)cell.lo[axis] - 1e-7 * (1.0 + std::fabs(cell.lo[axis]))
+                                   : (double)cell.hi[axis] + 1e-7 * (1.0 + std::fabs(cell.hi[axis]));
+      const double sgn = side == 0 ? 1.0 : -1.0;   // inside: sgn * (x - lim) >= 0
+      double (*in)[3] = poly[cur];
+      double (*o)[3] = poly[cur ^ 1];
+      int m = 0;
+      for (int i = 0; i < n; ++i) {
+        const double *p = in[i], *q = in[(i + 1) % n];
+        const double dp = sgn * (p[axis] - lim), dq = sgn * (q[axis] - lim);
+        if (dp >= 0) {
+          for (int k = 0; k < 3; ++k) o[m][k] = p[k];
+          ++m;
+        }
+        if ((dp >= 0) != (dq >= 0)) {
+          const double t = dp / (dp - dq);
+          for (int k = 0; k < 3; ++k) o[m][k] = p[k] + t * (q[k] - p[k]);
+          o[m][axis] = lim;
+          ++m;
+        }
+      }
+      n = m;
+      cur ^= 1;
+      if (n == 0) return false;
+    }
+  }
+  double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+  for (int i = 0; i < n; ++i)
+    for (int k = 0; k < 3; ++k) {
+      lo[k] = std::min(lo[k], poly[cur][i][k]);
+      hi[k] = std::max(hi[k], poly[cur][i][k]);
+    }
+  for (int k = 0; k < 3; ++k) {
+    out.lo[k] = std::max(cell.lo[k], std::nextafter((float)lo[k], -INFINITY));
+    out.hi[k] = std::min(cell.hi[k], std::nextafter((float)hi[k], INFINITY));
+    if (out.lo[k] > out.hi[k]) out.lo[k] = out.hi[k] = (out.lo[k] + out.hi[k]) * 0.5f;
+  }
+  return true;
+}
+
 class Builder {
  public:
-  Builder(const std::vector<Box> &prims, const Params &p, uint32_t max_depth, int spawn_depth)
-      : prims_(prims), p_(p), max_depth_(max_depth), spawn_depth_(spawn_depth) {}
+  Builder(const std::vector<Box> &prims, const Params &p, uint32_t max_depth, int spawn_depth,
+          const float *tri_verts, uint32_t num_tris)
+      : prims_(prims), p_(p), max_depth_(max_depth), spawn_depth_(spawn_depth), tri_verts_(tri_verts),
+        num_tris_(num_tris) {}
 
-  void build(BNode *node, std::vector<uint32_t> refs, Box cell, uint32_t depth) {
+  void build(BNode *node, std::vector<Ref> refs, Box cell, uint32_t depth) {
     const size_t n = refs.size();
     if (n <= p_.leaf_size || depth >= max_depth_) {
-      make_leaf(node, std::move(refs));
+      make_leaf(node, refs);
       return;
     }
     uint32_t axis;
     float split;
     if (!find_split(refs, cell, axis, split)) {
-      make_leaf(node, std::move(refs));
+      make_leaf(node, refs);
       return;
     }
-    std::vector<uint32_t> left, right;
-    left.reserve(n / 2 + 16);
-    right.reserve(n / 2 + 16);
-    for (uint32_t r : refs) {
-      const Box &b = prims_[r];
-      // A reference goes to every child whose closed cell its bounds touch.
-      if (b.lo[axis] <= split) left.push_back(r);
-      if (b.hi[axis] >= split) right.push_back(r);
-    }
-    if (left.size() == n && right.size() == n) {
-      make_leaf(node, std::move(refs));
-      return;
-    }
-    std::vector<uint32_t>().swap(refs);
-    node->axis = axis;
-    node->split = split;
-    node->kids.reset(new BNode[2]);
     Box lcell = cell, rcell = cell;
     lcell.hi[axis] = split;
     rcell.lo[axis] = split;
+    std::vector<Ref> left, right;
+    left.reserve(n / 2 + 16);
+    right.reserve(n / 2 + 16);
+    for (const Ref &r : refs) {
+      const Box &b = r.box;
+      if (b.lo[axis] < split && b.hi[axis] > split) {
+        // The split cuts the reference: clip it into each child.
+        Ref lr{r.prim, b}, rr{r.prim, b};
+        bool in_l = true, in_r = true;
+        if (tri_verts_ && r.prim < num_tris_) {
+          in_l = clip_triangle(tri_verts_ + 9ull * r.prim, intersect(b, lcell), lr.box);
+          in_r = clip_triangle(tri_verts_ + 9ull * r.prim, intersect(b, rcell), rr.box);
+          if (!in_l && !in_r) {   // rounding: keep both, unclipped
+            in_l = in_r = true;
+            lr.box = intersect(b, lcell);
+            rr.box = intersect(b, rcell);
+          }
+        } else {
+          lr.box.hi[axis] = split;
+          rr.box.lo[axis] = split;
+        }
+        if (in_l) left.push_back(lr);
+        if (in_r) right.push_back(rr);
+      } else {
+        // A reference goes to every child whose closed cell its bounds touch.
+        if (b.lo[axis] <= split) left.push_back(r);
+        if (b.hi[axis] >= split) right.push_back(r);
+      }
+    }
+    if (left.size() == n && right.size() == n) {
+      make_leaf(node, refs);
+      return;
+    }
+    std::vector<Ref>().swap(refs);
+    node->axis = axis;
+    node->split = split;
+    node->kids.reset(new BNode[2]);
     if ((int)depth < spawn_depth_ && left.size() + right.size() > 4096) {
       BNode *rk = &node->kids[1];
       std::thread t([this, rk, &right, rcell, depth]() { build(rk, std::move(right), rcell, depth + 1); });
@@ -70,14 +155,24 @@ class Builder {
   }
 
  private:
-  void make_leaf(BNode *node, std::vector<uint32_t> refs) {
-    node->axis = 3;
-    node->refs = std::move(refs);
+  static Box intersect(const Box &a, const Box &b) {
+    Box r;
+    for (int k = 0; k < 3; ++k) {
+      r.lo[k] = std::max(a.lo[k], b.lo[k]);
+      r.hi[k] = std::min(a.hi[k], b.hi[k]);
+      if (r.lo[k] > r.hi[k]) r.lo[k] = r.hi[k];
+    }
+    return r;
   }
 
-  // Binned SAH over the reference bounds clipped to the cell.
-  bool find_split(const std::vector<uint32_t> &refs, const Box &cell, uint32_t &best_axis,
-                  float &best_split) const {
+  void make_leaf(BNode *node, const std::vector<Ref> &refs) {
+    node->axis = 3;
+    node->prims.resize(refs.size());
+    for (size_t i = 0; i < refs.size(); ++i) node->prims[i] = refs[i].prim;
+  }
+
+  // Binned SAH over the (cell-clipped) reference bounds.
+  bool find_split(const std::vector<Ref> &refs, const Box &cell, uint32_t &best_axis, float &best_split) const {
     const uint32_t K = std::max<uint32_t>(4, std::min<uint32_t>(p_.bins, 256));
     float d[3];
     for (int a = 0; a < 3; ++a) d[a] = cell.hi[a] - cell.lo[a];
@@ -94,9 +189,8 @@ class Builder {
       std::memset(end, 0, sizeof(uint32_t) * K);
       const float scale = (float)K / ext;
       const float lo0 = cell.lo[a];
-      for (uint32_t r : refs) {
-        const Box &b = prims_[r];
-        const float lo = std::max(b.lo[a], cell.lo[a]), hi = std::min(b.hi[a], cell.hi[a]);
+      for (const Ref &r : refs) {
+        const float lo = std::max(r.box.lo[a], cell.lo[a]), hi = std::min(r.box.hi[a], cell.hi[a]);
         int bl = (int)((lo - lo0) * scale), bh = (int)((hi - lo0) * scale);
         bl = std::min(std::max(bl, 0), (int)K - 1);
         bh = std::min(std::max(bh, 0), (int)K - 1);
@@ -130,6 +224,8 @@ class Builder {
   const Params &p_;
   uint32_t max_depth_;
   int spawn_depth_;
+  const float *tri_verts_;
+  uint32_t num_tris_;
 };
 
 struct Flattener {
@@ -139,13 +235,13 @@ struct Flattener {
     out.max_depth = std::max(out.max_depth, depth);
     if (b.axis == 3) {
       const uint64_t first = out.records.size() / 12;
-      out.nodes[2 * idx] = ((uint32_t)b.refs.size() << 2) | 3u;
+      out.nodes[2 * idx] = ((uint32_t)b.prims.size() << 2) | 3u;
       out.nodes[2 * idx + 1] = (uint32_t)first;
-      out.records.resize(out.records.size() + 12 * b.refs.size());
-      for (size_t i = 0; i < b.refs.size(); ++i) write(b.refs[i], &out.records[12 * (first + i)]);
+      out.records.resize(out.records.size() + 12 * b.prims.size());
+      for (size_t i = 0; i < b.prims.size(); ++i) write(b.prims[i], &out.records[12 * (first + i)]);
       ++out.n_leaves;
-      out.n_refs += b.refs.size();
-      out.max_leaf = std::max<uint64_t>(out.max_leaf, b.refs.size());
+      out.n_refs += b.prims.size();
+      out.max_leaf = std::max<uint64_t>(out.max_leaf, b.prims.size());
       return;
     }
     const uint64_t child = out.nodes.size() / 2;
@@ -162,7 +258,8 @@ struct Flattener {
 }  // namespace
 
 void build(const std::vector<Box> &prims, const Params &p,
-           const std::function<void(uint32_t, float *)> &write_record, Output &out) {
+           const std::function<void(uint32_t, float *)> &write_record, Output &out, const float *tri_verts,
+           uint32_t num_tris) {
   const auto t0 = std::chrono::steady_clock::now();
   out = Output();
   Box box;
@@ -184,10 +281,10 @@ void build(const std::vector<Box> &prims, const Params &p,
   unsigned threads = p.num_threads ? p.num_threads : std::max(1u, std::thread::hardware_concurrency());
   threads = std::min(threads, 64u);
   const int spawn = threads > 1 ? (int)std::ceil(std::log2((double)threads)) + 2 : 0;
-  std::vector<uint32_t> refs(prims.size());
-  for (uint32_t i = 0; i < refs.size(); ++i) refs[i] = i;
+  std::vector<Ref> refs(prims.size());
+  for (uint32_t i = 0; i < refs.size(); ++i) refs[i] = Ref{i, prims[i]};
   BNode root;
-  Builder(prims, p, max_depth, spawn).build(&root, std::move(refs), box, 0);
+  Builder(prims, p, max_depth, spawn, p.clip ? tri_verts : nullptr, num_tris).build(&root, std::move(refs), box, 0);
   out.nodes.assign(2, 0);
   Flattener{write_record, out}.run(root, 0, 0);
   out.n_nodes = out.nodes.size() / 2;

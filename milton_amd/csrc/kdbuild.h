@@ -37,6 +37,7 @@ struct Params {
   float empty_bias = 0.8f;
   uint32_t num_threads = 0;        // 0: hardware concurrency
   uint32_t bins = 32;
+  bool clip = true;                // clip triangles to the cells they are split across
 };
 
 struct Output {
@@ -48,8 +49,11 @@ struct Output {
 };
 
 // prims: bounds per primitive; write_record(prim, out12) fills its record.
+// tri_verts (9 floats per triangle, primitives [0, num_tris)) lets the builder
+// clip triangles to cells; other primitives are split by their bounds.
 void build(const std::vector<Box> &prims, const Params &p,
-           const std::function<void(uint32_t, float *)> &write_record, Output &out);
+           const std::function<void(uint32_t, float *)> &write_record, Output &out,
+           const float *tri_verts = nullptr, uint32_t num_tris = 0);
 
 }  // namespace mpt_kd
 

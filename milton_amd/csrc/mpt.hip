@@ -16,7 +16,9 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
+#include <thread>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -137,8 +139,71 @@ __global__ __launch_bounds__(256) void k_extend(Scene sc, const float4 *__restri
     const RaySetup r = setup_ray(xyz(a), xyz(b));
     float t;
     uint32_t id;
-    trace<false, COUNT, STACK>(sc, r, INFINITY, __float_as_uint(b.w), kMiss, stk, t, id, &n_nodes, &n_refs);
+    trace<false, COUNT, STACK>(sc, r, INFINITY, __float_as_uint(b.w), stk, t, id, &n_nodes, &n_refs);
     hits[i] = make_float2(t, __uint_as_float(id));
+  }
+  if (COUNT) {
+    atomicAdd(&counters[0], (unsigned long long)n_nodes);
+    atomicAdd(&counters[1], (unsigned long long)n_refs);
+  }
+}
+
+// Persistent variant of k_extend: each wave keeps its lanes busy by handing a
+// lane whose ray finished a new ray from the queue (one atomic per wave per
+// refill, taken when at least kRefill lanes are idle or all are), so a wave no
+// longer idles until its slowest ray ends.
+constexpr uint32_t kRefill = 16;
+
+template <bool COUNT, int STACK, int WAVES>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WAVES))) void k_extend_persistent(Scene sc, const float4 *__restrict__ q,
+                                                           const uint32_t *__restrict__ count,
+                                                           float2 *__restrict__ hits,
+                                                           unsigned long long *__restrict__ counters,
+                                                           uint32_t *__restrict__ fetch) {
+  __shared__ uint2 stack[STACK * kBlock];
+  uint2 *stk = stack + threadIdx.x;
+  const uint32_t n = *count;
+  uint32_t n_nodes = 0, n_refs = 0;
+  uint32_t ray = kMiss;
+  bool exhausted = false;
+  RaySetup r;
+  Walker<false, STACK> w;
+  for (;;) {
+    const bool need = ray == kMiss && !exhausted;
+    const uint64_t idle = __ballot(need);
+    const uint64_t busy = __ballot(ray != kMiss);
+    if (idle != 0 && (__popcll(idle) >= kRefill || busy == 0)) {
+      const uint32_t leader = (uint32_t)__ffsll((unsigned long long)idle) - 1;
+      uint32_t base = 0;
+      if (lane_id() == leader) base = atomicAdd(fetch, (uint32_t)__popcll(idle));
+      base = __shfl(base, leader);
+      if (need) {
+        const uint32_t idx = base + __builtin_amdgcn_mbcnt_hi((uint32_t)(idle >> 32),
+                                                              __builtin_amdgcn_mbcnt_lo((uint32_t)idle, 0u));
+        if (idx < n) {
+          const float4 a = q[2 * idx], b = q[2 * idx + 1];
+          r = setup_ray(xyz(a), xyz(b));
+          if (w.start(sc, r, INFINITY, __float_as_uint(b.w))) {
+            ray = idx;
+          } else {
+            hits[idx] = make_float2(INFINITY, __uint_as_float(kMiss));
+          }
+        } else {
+          exhausted = true;
+        }
+      }
+    }
+    if (__ballot(ray != kMiss) == 0) {
+      if (__ballot(!exhausted) == 0) break;
+      continue;
+    }
+    if (ray != kMiss && w.template step<COUNT>(sc, r, stk, &n_nodes, &n_refs)) {
+      float t;
+      uint32_t id;
+      w.result(sc, r, t, id);
+      hits[ray] = make_float2(t, __uint_as_float(id));
+      ray = kMiss;
+    }
   }
   if (COUNT) {
     atomicAdd(&counters[0], (unsigned long long)n_nodes);
@@ -159,7 +224,7 @@ __global__ __launch_bounds__(256) void k_shadow(Scene sc, const float4 *__restri
     const RaySetup r = setup_ray(xyz(a), xyz(b));
     float t;
     uint32_t id;
-    if (!trace<true, false, STACK>(sc, r, b.w, __float_as_uint(c.w), kMiss, stk, t, id)) {
+    if (!trace<true, false, STACK>(sc, r, b.w, __float_as_uint(c.w), stk, t, id)) {
       float4 *L = Ls + __float_as_uint(a.w);
       atomicAdd(&L->x, c.x);
       atomicAdd(&L->y, c.y);
@@ -422,9 +487,9 @@ __global__ __launch_bounds__(256) void k_trace_rays(Scene sc, const mpt_ray *__r
     float t;
     uint32_t id;
     if (ANY) {
-      occ[i] = trace<true>(sc, r, tmax[i], kMiss, kMiss, stk, t, id) ? 1 : 0;
+      occ[i] = trace<true>(sc, r, tmax[i], kMiss, stk, t, id) ? 1 : 0;
     } else {
-      trace<false>(sc, r, INFINITY, kMiss, kMiss, stk, t, id);
+      trace<false>(sc, r, INFINITY, kMiss, stk, t, id);
       hits[i].t = t;
       hits[i].prim = id;
     }
@@ -498,22 +563,33 @@ void drop_graphs(mpt_context *ctx, uint64_t scene_uid) {
 
 template <int STACK>
 void launch_extend_t(mpt_context *ctx, uint32_t g, const Scene &sc, const float4 *q, const uint32_t *count,
-                     float2 *hits, unsigned long long *counters) {
-  if (counters)
+                     float2 *hits, unsigned long long *counters, uint32_t *fetch) {
+  if (ctx->persistent == 2) {
+    if (counters)
+      k_extend_persistent<true, STACK, 8><<<g, kBlock, 0, ctx->stream>>>(sc, q, count, hits, counters, fetch);
+    else
+      k_extend_persistent<false, STACK, 8><<<g, kBlock, 0, ctx->stream>>>(sc, q, count, hits, nullptr, fetch);
+  } else if (ctx->persistent) {
+    if (counters)
+      k_extend_persistent<true, STACK, 7><<<g, kBlock, 0, ctx->stream>>>(sc, q, count, hits, counters, fetch);
+    else
+      k_extend_persistent<false, STACK, 7><<<g, kBlock, 0, ctx->stream>>>(sc, q, count, hits, nullptr, fetch);
+  } else if (counters) {
     k_extend<true, STACK><<<g, kBlock, 0, ctx->stream>>>(sc, q, count, hits, counters);
-  else
+  } else {
     k_extend<false, STACK><<<g, kBlock, 0, ctx->stream>>>(sc, q, count, hits, nullptr);
+  }
 }
 
 // The short-stack depth is a run-time choice among compiled variants (it
 // trades LDS per wave, hence occupancy, against kd-restarts).
 void launch_extend(mpt_context *ctx, uint32_t g, const Scene &sc, const float4 *q, const uint32_t *count,
-                   float2 *hits, unsigned long long *counters) {
+                   float2 *hits, unsigned long long *counters, uint32_t *fetch) {
   switch (ctx->stack) {
-    case 8: launch_extend_t<8>(ctx, g, sc, q, count, hits, counters); break;
-    case 12: launch_extend_t<12>(ctx, g, sc, q, count, hits, counters); break;
-    case 24: launch_extend_t<24>(ctx, g, sc, q, count, hits, counters); break;
-    default: launch_extend_t<16>(ctx, g, sc, q, count, hits, counters); break;
+    case 8: launch_extend_t<8>(ctx, g, sc, q, count, hits, counters, fetch); break;
+    case 12: launch_extend_t<12>(ctx, g, sc, q, count, hits, counters, fetch); break;
+    case 24: launch_extend_t<24>(ctx, g, sc, q, count, hits, counters, fetch); break;
+    default: launch_extend_t<16>(ctx, g, sc, q, count, hits, counters, fetch); break;
   }
 }
 
@@ -552,13 +628,14 @@ int enqueue_batch(mpt_context *ctx, mpt_scene *s, uint32_t D, bool instrument) {
     MPT_HIP(ctx, hipEventRecord(ctx->events[g_timed.back().ev + 1], ctx->stream));
     return MPT_OK;
   };
-  k_reset<<<1, 64, 0, ctx->stream>>>(w.counts.p, 2 * D + 2);
+  k_reset<<<1, 64, 0, ctx->stream>>>(w.counts.p, 3 * D + 2);
   k_raygen<<<gcap, 256, 0, ctx->stream>>>(w.desc.p, w.q[0].p, w.Ls.p, w.betas.p, w.flags.p, w.pjs.p, w.counts.p);
   MPT_LAUNCHED(ctx, "k_raygen");
   for (uint32_t d = 0; d < D; ++d) {
     float4 *qin = w.q[d & 1].p, *qout = w.q[(d + 1) & 1].p;
     if ((rc = tic(0))) return rc;
-    launch_extend(ctx, g, s->dev, qin, w.counts.p + d, w.hits.p, instrument && ctx->counters ? w.totals.p + 3 : nullptr);
+    launch_extend(ctx, g, s->dev, qin, w.counts.p + d, w.hits.p, instrument && ctx->counters ? w.totals.p + 3 : nullptr,
+                  w.counts.p + 2 * D + 1 + d);
     if ((rc = toc())) return rc;
     if ((rc = tic(1))) return rc;
     k_shade<<<g, kBlock, 0, ctx->stream>>>(s->dev, w.desc.p, d, qin, w.hits.p, w.counts.p + d, qout,
@@ -675,7 +752,7 @@ int ensure_workspace(mpt_context *ctx, uint32_t cap, uint32_t depth, uint32_t li
   MPT_HIP(ctx, w.flags.reserve(cap));
   MPT_HIP(ctx, w.pjs.reserve(cap));
   MPT_HIP(ctx, w.sq.reserve(3ull * ns));
-  MPT_HIP(ctx, w.counts.reserve(2ull * depth + 2));
+  MPT_HIP(ctx, w.counts.reserve(3ull * depth + 2));
   MPT_HIP(ctx, w.totals.reserve(8));
   MPT_HIP(ctx, w.desc.reserve(1));
   w.cap = cap;
@@ -810,6 +887,213 @@ int render_range(mpt_scene *s, const mpt_job *job, uint32_t pix_begin, uint32_t 
 
 }  // namespace
 
+namespace {
+
+// Host side of scene creation: validation, the device layouts and the kd
+// build (used by mpt_scene_create and by the host-only kd entry points).
+struct HostScene {
+  std::vector<float4> verts, vnorm, sph, quads;
+  std::vector<uint4> tris;
+  std::vector<uint32_t> lights, sphere_mat;
+  std::vector<mpt_material> mats;
+  mpt_kd::Output ko;
+  uint32_t nt = 0, ns = 0, nq = 0;
+  float env[3] = {0, 0, 0};
+};
+
+int prepare_scene(mpt_context *ctx, const mpt_scene_desc *d, const mpt_kd_params *kd, HostScene &hs) {
+  if (!d) return fail(ctx, MPT_E_INVALID, "NULL argument");
+  if (d->version != MPT_ABI_VERSION) return fail(ctx, MPT_E_INVALID, "scene desc version mismatch");
+  const uint32_t nv = d->num_vertices, nt = d->num_triangles, ns = d->num_spheres, nq = d->num_quads;
+  const uint32_t nm = d->num_materials;
+  if ((uint64_t)nt + ns + nq >= 0x3FFFFFFFull) return fail(ctx, MPT_E_INVALID, "too many primitives");
+  if ((nt && (!d->indices || !d->tri_material || !d->vertices)) || (ns && (!d->spheres || !d->sphere_material)) ||
+      (nq && (!d->quads || !d->quad_material)) || (nm && !d->materials))
+    return fail(ctx, MPT_E_INVALID, "missing scene array");
+  for (uint64_t i = 0; i < 3ull * nt; ++i)
+    if (d->indices[i] >= nv) return fail(ctx, MPT_E_INVALID, "triangle index out of range");
+  auto bad_mat = [&](uint32_t m) { return m >= nm; };
+  for (uint32_t i = 0; i < nt; ++i)
+    if (bad_mat(d->tri_material[i]) || emissive_host(d->materials[d->tri_material[i]]))
+      return fail(ctx, MPT_E_INVALID, "triangle material out of range or emissive (only quads may emit)");
+  for (uint32_t i = 0; i < ns; ++i)
+    if (bad_mat(d->sphere_material[i]) || emissive_host(d->materials[d->sphere_material[i]]))
+      return fail(ctx, MPT_E_INVALID, "sphere material out of range or emissive (only quads may emit)");
+  for (uint32_t i = 0; i < nq; ++i)
+    if (bad_mat(d->quad_material[i])) return fail(ctx, MPT_E_INVALID, "quad material out of range");
+  for (uint32_t i = 0; i < nm; ++i)
+    if (d->materials[i].type > MPT_BSDF_DIELECTRIC) return fail(ctx, MPT_E_INVALID, "unknown BSDF type");
+  // Host copies in the device layouts.
+  std::vector<float4> &verts = hs.verts, &vnorm = hs.vnorm;
+  verts.resize(nv);
+  vnorm.resize(nv);
+  for (uint32_t i = 0; i < nv; ++i) verts[i] = make_float4(d->vertices[3 * i], d->vertices[3 * i + 1], d->vertices[3 * i + 2], 0);
+  {
+    // Mesh::computeNormals (shapes/Mesh.cpp:335-377) in double.
+    std::vector<double> acc(3ull * nv, 0.0);
+    for (uint32_t t = 0; t < nt; ++t) {
+      const uint32_t *ix = d->indices + 3ull * t;
+      const float *a = d->vertices + 3ull * ix[0], *b = d->vertices + 3ull * ix[1], *c = d->vertices + 3ull * ix[2];
+      const double e1[3] = {(double)b[0] - a[0], (double)b[1] - a[1], (double)b[2] - a[2]};
+      const double e2[3] = {(double)c[0] - a[0], (double)c[1] - a[1], (double)c[2] - a[2]};
+      double n[3] = {e1[1] * e2[2] - e1[2] * e2[1], e1[2] * e2[0] - e1[0] * e2[2], e1[0] * e2[1] - e1[1] * e2[0]};
+      const double l = std::sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+      if (l > 0)
+        for (int k = 0; k < 3; ++k) n[k] /= l;
+      for (int j = 0; j < 3; ++j)
+        for (int k = 0; k < 3; ++k) acc[3ull * ix[j] + k] += n[k];
+    }
+    for (uint32_t i = 0; i < nv; ++i) {
+      double *n = &acc[3ull * i];
+      const double l = std::sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+      const double k = l > 0 ? 1.0 / l : 0.0;
+      vnorm[i] = make_float4((float)(n[0] * k), (float)(n[1] * k), (float)(n[2] * k), 0);
+    }
+  }
+  std::vector<uint4> &tris = hs.tris;
+  tris.resize(nt);
+  for (uint32_t t = 0; t < nt; ++t)
+    tris[t] = make_uint4(d->indices[3 * t], d->indices[3 * t + 1], d->indices[3 * t + 2], d->tri_material[t]);
+  std::vector<float4> &sph = hs.sph;
+  sph.resize(ns);
+  for (uint32_t i = 0; i < ns; ++i)
+    sph[i] = make_float4(d->spheres[4 * i], d->spheres[4 * i + 1], d->spheres[4 * i + 2], d->spheres[4 * i + 3]);
+  std::vector<float4> &quads = hs.quads;
+  quads.resize(4ull * nq);
+  std::vector<uint32_t> &lights = hs.lights;
+  for (uint32_t i = 0; i < nq; ++i) {
+    const float *q = d->quads + 9ull * i;
+    const double ex[3] = {q[3], q[4], q[5]}, ez[3] = {q[6], q[7], q[8]};
+    double n[3] = {ez[1] * ex[2] - ez[2] * ex[1], ez[2] * ex[0] - ez[0] * ex[2], ez[0] * ex[1] - ez[1] * ex[0]};
+    const double area = std::sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+    if (!(area > 0)) return fail(ctx, MPT_E_INVALID, "degenerate quad");
+    normalize3(n);
+    quads[4 * i] = make_float4(q[0], q[1], q[2], (float)area);
+    quads[4 * i + 1] = make_float4(q[3], q[4], q[5], 0);
+    quads[4 * i + 2] = make_float4(q[6], q[7], q[8], 0);
+    uint32_t mbits = d->quad_material[i];
+    float mf;
+    std::memcpy(&mf, &mbits, 4);
+    quads[4 * i + 3] = make_float4((float)n[0], (float)n[1], (float)n[2], mf);
+    if (emissive_host(d->materials[d->quad_material[i]])) lights.push_back(i);
+  }
+
+  // kd-tree over all primitives.
+  std::vector<mpt_kd::Box> boxes((size_t)nt + ns + nq);
+  for (uint32_t t = 0; t < nt; ++t) {
+    mpt_kd::Box &b = boxes[t];
+    for (int k = 0; k < 3; ++k) {
+      const float v0 = (&verts[tris[t].x].x)[k], v1 = (&verts[tris[t].y].x)[k], v2 = (&verts[tris[t].z].x)[k];
+      b.lo[k] = std::min(v0, std::min(v1, v2));
+      b.hi[k] = std::max(v0, std::max(v1, v2));
+    }
+  }
+  for (uint32_t i = 0; i < ns; ++i) {
+    mpt_kd::Box &b = boxes[(size_t)nt + i];
+    const float c[3] = {sph[i].x, sph[i].y, sph[i].z};
+    for (int k = 0; k < 3; ++k) {
+      b.lo[k] = std::nextafter(c[k] - sph[i].w, -INFINITY);
+      b.hi[k] = std::nextafter(c[k] + sph[i].w, INFINITY);
+    }
+  }
+  for (uint32_t i = 0; i < nq; ++i) {
+    mpt_kd::Box &b = boxes[(size_t)nt + ns + i];
+    const float4 c = quads[4 * i], ex = quads[4 * i + 1], ez = quads[4 * i + 2];
+    for (int k = 0; k < 3; ++k) {
+      const float cc = (&c.x)[k], a = 0.5f * std::fabs((&ex.x)[k]), e = 0.5f * std::fabs((&ez.x)[k]);
+      b.lo[k] = std::nextafter(cc - a - e, -INFINITY);
+      b.hi[k] = std::nextafter(cc + a + e, INFINITY);
+    }
+  }
+  mpt_kd::Params kp;
+  if (kd) {
+    if (kd->leaf_size) kp.leaf_size = kd->leaf_size;
+    if (kd->max_depth) kp.max_depth = kd->max_depth;
+    if (kd->cost_traversal > 0) kp.cost_traversal = kd->cost_traversal;
+    if (kd->cost_intersect > 0) kp.cost_intersect = kd->cost_intersect;
+    if (kd->empty_bias > 0) kp.empty_bias = kd->empty_bias;
+    kp.num_threads = kd->num_threads;
+    if (kd->bins) kp.bins = kd->bins;
+    kp.clip = kd->no_clip == 0;
+  }
+  std::vector<float> tri_verts(9ull * nt);
+  for (uint32_t t = 0; t < nt; ++t) {
+    const float4 a = verts[tris[t].x], b = verts[tris[t].y], c = verts[tris[t].z];
+    const float v[9] = {a.x, a.y, a.z, b.x, b.y, b.z, c.x, c.y, c.z};
+    std::memcpy(&tri_verts[9ull * t], v, sizeof(v));
+  }
+  mpt_kd::Output &ko = hs.ko;
+  auto write = [&](uint32_t prim, float *r) {
+    uint32_t kind, id = prim;
+    std::memset(r, 0, 12 * sizeof(float));
+    if (prim < nt) {
+      kind = mpt_kd::KIND_TRIANGLE;
+      const float4 a = verts[tris[prim].x], b = verts[tris[prim].y], c = verts[tris[prim].z];
+      r[0] = a.x; r[1] = a.y; r[2] = a.z;
+      r[4] = b.x; r[5] = b.y; r[6] = b.z;
+      r[8] = c.x; r[9] = c.y; r[10] = c.z;
+    } else if (prim < nt + ns) {
+      kind = mpt_kd::KIND_SPHERE;
+      const float4 sp = sph[prim - nt];
+      r[0] = sp.x; r[1] = sp.y; r[2] = sp.z;
+      r[4] = sp.w;
+    } else {
+      kind = mpt_kd::KIND_QUAD;
+      const float4 *q = &quads[4ull * (prim - nt - ns)];
+      r[0] = q[0].x; r[1] = q[0].y; r[2] = q[0].z;
+      r[4] = q[1].x; r[5] = q[1].y; r[6] = q[1].z;
+      r[8] = q[2].x; r[9] = q[2].y; r[10] = q[2].z;
+    }
+    std::memcpy(&r[3], &id, 4);
+    std::memcpy(&r[7], &kind, 4);
+  };
+  try {
+    mpt_kd::build(boxes, kp, write, ko, tri_verts.data(), nt);
+  } catch (const std::bad_alloc &) {
+    return fail(ctx, MPT_E_NOMEM, "kd build: host allocation");
+  }
+  hs.nt = nt;
+  hs.ns = ns;
+  hs.nq = nq;
+  hs.sphere_mat.assign(d->sphere_material, d->sphere_material + ns);
+  hs.mats.assign(d->materials, d->materials + nm);
+  for (int k = 0; k < 3; ++k) hs.env[k] = d->env_radiance[k];
+  return MPT_OK;
+}
+
+// Points a device-layout Scene at host (or device) arrays.
+void bind_scene(Scene &sc, const uint2 *nodes, const float4 *recs, const mpt_kd::Box &box, const uint4 *tris,
+                const float4 *verts, const float4 *vnorm, const float4 *sph, const uint32_t *sph_mat,
+                const float4 *quads, const mpt_material *mats, const uint32_t *lights, uint32_t nl, uint32_t nt,
+                uint32_t ns, uint32_t nq, const float *env) {
+  sc.nodes = nodes;
+  sc.recs = recs;
+  for (int k = 0; k < 3; ++k) {
+    sc.box_lo[k] = box.lo[k];
+    sc.box_hi[k] = box.hi[k];
+  }
+  sc.tris = tris;
+  sc.verts = verts;
+  sc.vnormals = vnorm;
+  sc.spheres = sph;
+  sc.sphere_mat = sph_mat;
+  sc.quads = quads;
+  sc.mats = mats;
+  sc.lights = lights;
+  sc.num_lights = nl;
+  sc.nt = nt;
+  sc.ns = ns;
+  sc.nq = nq;
+  for (int k = 0; k < 3; ++k) sc.env[k] = env[k];
+}
+
+}  // namespace
+
+struct mpt_kd_tree {
+  HostScene hs;
+  Scene sc;
+};
+
 extern "C" {
 
 int mpt_abi_version(void) { return (int)MPT_ABI_VERSION; }
@@ -874,6 +1158,11 @@ int mpt_set_option(mpt_context *ctx, uint32_t option, uint64_t value) {
     case MPT_OPT_GRAPHS:
       ctx->graphs = value != 0;
       return MPT_OK;
+    case MPT_OPT_PERSISTENT:
+      if (value > 2) return fail(ctx, MPT_E_INVALID, "persistent must be 0, 1 or 2");
+      ctx->persistent = (uint32_t)value;
+      drop_graphs(ctx, 0);
+      return MPT_OK;
     case MPT_OPT_STACK:
       if (value != 8 && value != 12 && value != 16 && value != 24)
         return fail(ctx, MPT_E_INVALID, "stack depth must be 8, 12, 16 or 24");
@@ -894,180 +1183,28 @@ int mpt_sync(mpt_context *ctx) {
 int mpt_scene_create(mpt_context *ctx, const mpt_scene_desc *d, const mpt_kd_params *kd, mpt_scene **out) {
   if (!ctx || !d || !out) return fail(ctx, MPT_E_INVALID, "NULL argument");
   *out = nullptr;
-  if (d->version != MPT_ABI_VERSION) return fail(ctx, MPT_E_INVALID, "scene desc version mismatch");
-  const uint32_t nv = d->num_vertices, nt = d->num_triangles, ns = d->num_spheres, nq = d->num_quads;
-  const uint32_t nm = d->num_materials;
-  if ((uint64_t)nt + ns + nq >= 0x3FFFFFFFull) return fail(ctx, MPT_E_INVALID, "too many primitives");
-  if ((nt && (!d->indices || !d->tri_material || !d->vertices)) || (ns && (!d->spheres || !d->sphere_material)) ||
-      (nq && (!d->quads || !d->quad_material)) || (nm && !d->materials))
-    return fail(ctx, MPT_E_INVALID, "missing scene array");
-  for (uint64_t i = 0; i < 3ull * nt; ++i)
-    if (d->indices[i] >= nv) return fail(ctx, MPT_E_INVALID, "triangle index out of range");
-  auto bad_mat = [&](uint32_t m) { return m >= nm; };
-  for (uint32_t i = 0; i < nt; ++i)
-    if (bad_mat(d->tri_material[i]) || emissive_host(d->materials[d->tri_material[i]]))
-      return fail(ctx, MPT_E_INVALID, "triangle material out of range or emissive (only quads may emit)");
-  for (uint32_t i = 0; i < ns; ++i)
-    if (bad_mat(d->sphere_material[i]) || emissive_host(d->materials[d->sphere_material[i]]))
-      return fail(ctx, MPT_E_INVALID, "sphere material out of range or emissive (only quads may emit)");
-  for (uint32_t i = 0; i < nq; ++i)
-    if (bad_mat(d->quad_material[i])) return fail(ctx, MPT_E_INVALID, "quad material out of range");
-  for (uint32_t i = 0; i < nm; ++i)
-    if (d->materials[i].type > MPT_BSDF_DIELECTRIC) return fail(ctx, MPT_E_INVALID, "unknown BSDF type");
+  HostScene hs;
+  int rc = prepare_scene(ctx, d, kd, hs);
+  if (rc) return rc;
   MPT_HIP(ctx, hipSetDevice(ctx->device));
   std::unique_ptr<mpt_scene> s(new (std::nothrow) mpt_scene());
   if (!s) return fail(ctx, MPT_E_NOMEM, "host allocation");
   s->ctx = ctx;
-
-  // Host copies in the device layouts.
-  std::vector<float4> verts(nv), vnorm(nv);
-  for (uint32_t i = 0; i < nv; ++i) verts[i] = make_float4(d->vertices[3 * i], d->vertices[3 * i + 1], d->vertices[3 * i + 2], 0);
-  {
-    // Mesh::computeNormals (shapes/Mesh.cpp:335-377) in double.
-    std::vector<double> acc(3ull * nv, 0.0);
-    for (uint32_t t = 0; t < nt; ++t) {
-      const uint32_t *ix = d->indices + 3ull * t;
-      const float *a = d->vertices + 3ull * ix[0], *b = d->vertices + 3ull * ix[1], *c = d->vertices + 3ull * ix[2];
-      const double e1[3] = {(double)b[0] - a[0], (double)b[1] - a[1], (double)b[2] - a[2]};
-      const double e2[3] = {(double)c[0] - a[0], (double)c[1] - a[1], (double)c[2] - a[2]};
-      double n[3] = {e1[1] * e2[2] - e1[2] * e2[1], e1[2] * e2[0] - e1[0] * e2[2], e1[0] * e2[1] - e1[1] * e2[0]};
-      const double l = std::sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
-      if (l > 0)
-        for (int k = 0; k < 3; ++k) n[k] /= l;
-      for (int j = 0; j < 3; ++j)
-        for (int k = 0; k < 3; ++k) acc[3ull * ix[j] + k] += n[k];
-    }
-    for (uint32_t i = 0; i < nv; ++i) {
-      double *n = &acc[3ull * i];
-      const double l = std::sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
-      const double k = l > 0 ? 1.0 / l : 0.0;
-      vnorm[i] = make_float4((float)(n[0] * k), (float)(n[1] * k), (float)(n[2] * k), 0);
-    }
-  }
-  std::vector<uint4> tris(nt);
-  for (uint32_t t = 0; t < nt; ++t)
-    tris[t] = make_uint4(d->indices[3 * t], d->indices[3 * t + 1], d->indices[3 * t + 2], d->tri_material[t]);
-  std::vector<float4> sph(ns);
-  for (uint32_t i = 0; i < ns; ++i)
-    sph[i] = make_float4(d->spheres[4 * i], d->spheres[4 * i + 1], d->spheres[4 * i + 2], d->spheres[4 * i + 3]);
-  std::vector<float4> quads(4ull * nq);
-  std::vector<uint32_t> lights;
-  for (uint32_t i = 0; i < nq; ++i) {
-    const float *q = d->quads + 9ull * i;
-    const double ex[3] = {q[3], q[4], q[5]}, ez[3] = {q[6], q[7], q[8]};
-    double n[3] = {ez[1] * ex[2] - ez[2] * ex[1], ez[2] * ex[0] - ez[0] * ex[2], ez[0] * ex[1] - ez[1] * ex[0]};
-    const double area = std::sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
-    if (!(area > 0)) return fail(ctx, MPT_E_INVALID, "degenerate quad");
-    normalize3(n);
-    quads[4 * i] = make_float4(q[0], q[1], q[2], (float)area);
-    quads[4 * i + 1] = make_float4(q[3], q[4], q[5], 0);
-    quads[4 * i + 2] = make_float4(q[6], q[7], q[8], 0);
-    uint32_t mbits = d->quad_material[i];
-    float mf;
-    std::memcpy(&mf, &mbits, 4);
-    quads[4 * i + 3] = make_float4((float)n[0], (float)n[1], (float)n[2], mf);
-    if (emissive_host(d->materials[d->quad_material[i]])) lights.push_back(i);
-  }
-
-  // kd-tree over all primitives.
-  std::vector<mpt_kd::Box> boxes((size_t)nt + ns + nq);
-  for (uint32_t t = 0; t < nt; ++t) {
-    mpt_kd::Box &b = boxes[t];
-    for (int k = 0; k < 3; ++k) {
-      const float v0 = (&verts[tris[t].x].x)[k], v1 = (&verts[tris[t].y].x)[k], v2 = (&verts[tris[t].z].x)[k];
-      b.lo[k] = std::min(v0, std::min(v1, v2));
-      b.hi[k] = std::max(v0, std::max(v1, v2));
-    }
-  }
-  for (uint32_t i = 0; i < ns; ++i) {
-    mpt_kd::Box &b = boxes[(size_t)nt + i];
-    const float c[3] = {sph[i].x, sph[i].y, sph[i].z};
-    for (int k = 0; k < 3; ++k) {
-      b.lo[k] = std::nextafter(c[k] - sph[i].w, -INFINITY);
-      b.hi[k] = std::nextafter(c[k] + sph[i].w, INFINITY);
-    }
-  }
-  for (uint32_t i = 0; i < nq; ++i) {
-    mpt_kd::Box &b = boxes[(size_t)nt + ns + i];
-    const float4 c = quads[4 * i], ex = quads[4 * i + 1], ez = quads[4 * i + 2];
-    for (int k = 0; k < 3; ++k) {
-      const float cc = (&c.x)[k], a = 0.5f * std::fabs((&ex.x)[k]), e = 0.5f * std::fabs((&ez.x)[k]);
-      b.lo[k] = std::nextafter(cc - a - e, -INFINITY);
-      b.hi[k] = std::nextafter(cc + a + e, INFINITY);
-    }
-  }
-  mpt_kd::Params kp;
-  if (kd) {
-    if (kd->leaf_size) kp.leaf_size = kd->leaf_size;
-    if (kd->max_depth) kp.max_depth = kd->max_depth;
-    if (kd->cost_traversal > 0) kp.cost_traversal = kd->cost_traversal;
-    if (kd->cost_intersect > 0) kp.cost_intersect = kd->cost_intersect;
-    if (kd->empty_bias > 0) kp.empty_bias = kd->empty_bias;
-    kp.num_threads = kd->num_threads;
-    if (kd->bins) kp.bins = kd->bins;
-  }
-  mpt_kd::Output ko;
-  auto write = [&](uint32_t prim, float *r) {
-    uint32_t kind, id = prim;
-    std::memset(r, 0, 12 * sizeof(float));
-    if (prim < nt) {
-      kind = mpt_kd::KIND_TRIANGLE;
-      const float4 a = verts[tris[prim].x], b = verts[tris[prim].y], c = verts[tris[prim].z];
-      r[0] = a.x; r[1] = a.y; r[2] = a.z;
-      r[4] = b.x; r[5] = b.y; r[6] = b.z;
-      r[8] = c.x; r[9] = c.y; r[10] = c.z;
-    } else if (prim < nt + ns) {
-      kind = mpt_kd::KIND_SPHERE;
-      const float4 sp = sph[prim - nt];
-      r[0] = sp.x; r[1] = sp.y; r[2] = sp.z;
-      r[4] = sp.w;
-    } else {
-      kind = mpt_kd::KIND_QUAD;
-      const float4 *q = &quads[4ull * (prim - nt - ns)];
-      r[0] = q[0].x; r[1] = q[0].y; r[2] = q[0].z;
-      r[4] = q[1].x; r[5] = q[1].y; r[6] = q[1].z;
-      r[8] = q[2].x; r[9] = q[2].y; r[10] = q[2].z;
-    }
-    std::memcpy(&r[3], &id, 4);
-    std::memcpy(&r[7], &kind, 4);
-  };
-  try {
-    mpt_kd::build(boxes, kp, write, ko);
-  } catch (const std::bad_alloc &) {
-    return fail(ctx, MPT_E_NOMEM, "kd build: host allocation");
-  }
-  int rc;
+  const mpt_kd::Output &ko = hs.ko;
+  const uint32_t nt = hs.nt, ns = hs.ns, nq = hs.nq;
   if ((rc = upload(ctx, s->nodes, reinterpret_cast<const uint2 *>(ko.nodes.data()), ko.nodes.size() / 2))) return rc;
   if ((rc = upload(ctx, s->recs, reinterpret_cast<const float4 *>(ko.records.data()), ko.records.size() / 4))) return rc;
-  if ((rc = upload(ctx, s->tris, tris.data(), tris.size()))) return rc;
-  if ((rc = upload(ctx, s->verts, verts.data(), verts.size()))) return rc;
-  if ((rc = upload(ctx, s->vnormals, vnorm.data(), vnorm.size()))) return rc;
-  if ((rc = upload(ctx, s->spheres, sph.data(), sph.size()))) return rc;
-  if ((rc = upload(ctx, s->sphere_mat, d->sphere_material, ns))) return rc;
-  if ((rc = upload(ctx, s->quads, quads.data(), quads.size()))) return rc;
-  if ((rc = upload(ctx, s->mats, d->materials, nm))) return rc;
-  if ((rc = upload(ctx, s->lights, lights.data(), lights.size()))) return rc;
+  if ((rc = upload(ctx, s->tris, hs.tris.data(), hs.tris.size()))) return rc;
+  if ((rc = upload(ctx, s->verts, hs.verts.data(), hs.verts.size()))) return rc;
+  if ((rc = upload(ctx, s->vnormals, hs.vnorm.data(), hs.vnorm.size()))) return rc;
+  if ((rc = upload(ctx, s->spheres, hs.sph.data(), hs.sph.size()))) return rc;
+  if ((rc = upload(ctx, s->sphere_mat, hs.sphere_mat.data(), hs.sphere_mat.size()))) return rc;
+  if ((rc = upload(ctx, s->quads, hs.quads.data(), hs.quads.size()))) return rc;
+  if ((rc = upload(ctx, s->mats, hs.mats.data(), hs.mats.size()))) return rc;
+  if ((rc = upload(ctx, s->lights, hs.lights.data(), hs.lights.size()))) return rc;
   MPT_HIP(ctx, hipStreamSynchronize(ctx->stream));
-  Scene &sc = s->dev;
-  sc.nodes = s->nodes.p;
-  sc.recs = s->recs.p;
-  for (int k = 0; k < 3; ++k) {
-    sc.box_lo[k] = ko.box.lo[k];
-    sc.box_hi[k] = ko.box.hi[k];
-  }
-  sc.tris = s->tris.p;
-  sc.verts = s->verts.p;
-  sc.vnormals = s->vnormals.p;
-  sc.spheres = s->spheres.p;
-  sc.sphere_mat = s->sphere_mat.p;
-  sc.quads = s->quads.p;
-  sc.mats = s->mats.p;
-  sc.lights = s->lights.p;
-  sc.num_lights = (uint32_t)lights.size();
-  sc.nt = nt;
-  sc.ns = ns;
-  sc.nq = nq;
-  for (int k = 0; k < 3; ++k) sc.env[k] = d->env_radiance[k];
+  bind_scene(s->dev, s->nodes.p, s->recs.p, ko.box, s->tris.p, s->verts.p, s->vnormals.p, s->spheres.p,
+             s->sphere_mat.p, s->quads.p, s->mats.p, s->lights.p, (uint32_t)hs.lights.size(), nt, ns, nq, hs.env);
   s->stats[0] = ko.n_nodes;
   s->stats[1] = ko.n_leaves;
   s->stats[2] = ko.n_refs;
@@ -1079,6 +1216,84 @@ int mpt_scene_create(mpt_context *ctx, const mpt_scene_desc *d, const mpt_kd_par
   s->uid = ++g_scene_uid;
   ctx->scenes.push_back(s.get());
   *out = s.release();
+  return MPT_OK;
+}
+
+int mpt_kd_build(const mpt_scene_desc *d, const mpt_kd_params *kd, mpt_kd_tree **out) {
+  if (!d || !out) return MPT_E_INVALID;
+  *out = nullptr;
+  std::unique_ptr<mpt_kd_tree> t(new (std::nothrow) mpt_kd_tree());
+  if (!t) return MPT_E_NOMEM;
+  int rc;
+  try {
+    rc = prepare_scene(nullptr, d, kd, t->hs);
+  } catch (const std::bad_alloc &) {
+    return MPT_E_NOMEM;
+  }
+  if (rc) return rc;
+  HostScene &hs = t->hs;
+  bind_scene(t->sc, reinterpret_cast<const uint2 *>(hs.ko.nodes.data()),
+             reinterpret_cast<const float4 *>(hs.ko.records.data()), hs.ko.box, hs.tris.data(), hs.verts.data(),
+             hs.vnorm.data(), hs.sph.data(), hs.sphere_mat.data(), hs.quads.data(), hs.mats.data(),
+             hs.lights.data(), (uint32_t)hs.lights.size(), hs.nt, hs.ns, hs.nq, hs.env);
+  *out = t.release();
+  return MPT_OK;
+}
+
+int mpt_kd_stats(const mpt_kd_tree *t, uint64_t out[6]) {
+  if (!t || !out) return MPT_E_INVALID;
+  const mpt_kd::Output &ko = t->hs.ko;
+  out[0] = ko.n_nodes;
+  out[1] = ko.n_leaves;
+  out[2] = ko.n_refs;
+  out[3] = ko.max_depth;
+  out[4] = (uint64_t)ko.build_ms;
+  out[5] = ko.max_leaf;
+  return MPT_OK;
+}
+
+int mpt_kd_trace(const mpt_kd_tree *t, const mpt_ray *rays, uint64_t n, mpt_hit *hits, uint64_t counters[2],
+                 uint32_t nthreads) {
+  if (!t || (n && (!rays || !hits))) return MPT_E_INVALID;
+  const unsigned nth = std::max(1u, std::min(nthreads ? nthreads : std::thread::hardware_concurrency(), 64u));
+  std::vector<uint64_t> nodes(nth, 0), refs(nth, 0);
+  std::atomic<uint64_t> next(0);
+  auto work = [&](unsigned tid) {
+    // The LDS ring of a device lane becomes a strided host buffer.
+    std::vector<uint2> stk((size_t)8 * kBlock);
+    for (;;) {
+      const uint64_t b = next.fetch_add(4096);
+      if (b >= n) break;
+      const uint64_t e = std::min<uint64_t>(n, b + 4096);
+      for (uint64_t i = b; i < e; ++i) {
+        const mpt_ray &ry = rays[i];
+        const RaySetup r = setup_ray(make_float3(ry.o[0], ry.o[1], ry.o[2]), make_float3(ry.d[0], ry.d[1], ry.d[2]));
+        float tt;
+        uint32_t id, nn = 0, nr = 0;
+        trace<false, true, 8>(t->sc, r, INFINITY, kMiss, stk.data(), tt, id, &nn, &nr);
+        hits[i].t = tt;
+        hits[i].prim = id;
+        nodes[tid] += nn;
+        refs[tid] += nr;
+      }
+    }
+  };
+  std::vector<std::thread> th;
+  for (unsigned k = 1; k < nth; ++k) th.emplace_back(work, k);
+  work(0);
+  for (auto &x : th) x.join();
+  if (counters) {
+    counters[0] = counters[1] = 0;
+    for (unsigned k = 0; k < nth; ++k) {
+      counters[0] += nodes[k];
+      counters[1] += refs[k];
+    }
+  }
+  return MPT_OK;
+}
+
+int mpt_kd_destroy(mpt_kd_tree *t) {
+  delete t;
   return MPT_OK;
 }
 

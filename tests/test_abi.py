@@ -34,7 +34,7 @@ def test_struct_layouts():
     assert ctypes.sizeof(m.Job) == 44 + 24
     assert ctypes.sizeof(m.Ray) == 24
     assert m.HIT_DTYPE.itemsize == 8
-    assert ctypes.sizeof(m.KdParams) == 28
+    assert ctypes.sizeof(m.KdParams) == 32
     # field order of mpt_scene_desc
     offs = [getattr(m.SceneDesc, f).offset for f, _ in m.SceneDesc._fields_]
     assert offs == sorted(offs)

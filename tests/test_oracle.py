@@ -219,7 +219,7 @@ def test_phong_sampling_and_consistency():
     # hemisphere around the mirror direction
     th = np.linspace(0, math.pi / 2, 20001)
     integrand = (n_exp + 1) / (2 * math.pi) * np.cos(th) ** n_exp * np.sin(th) * 2 * math.pi
-    assert abs(np.trapz(integrand, th) - 1) < 1e-6
+    assert abs(np.trapezoid(integrand, th) - 1) < 1e-6
 
 
 def test_fresnel_textbook():
