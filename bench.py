@@ -76,6 +76,38 @@ def copy_calibration(nbytes=256 << 20):
     log(f"copy calibration: {nbytes} bytes device to device")
 
 
+RMSE_REF_SPP, RMSE_ROWS = 65536, 8
+
+
+def rmse_vs_reference(m, ctx, scene, job):
+    """Per-channel linear RMSE of one job-sized render (job.spp) against a
+    65,536-spp reference of the same camera with an independent seed, over
+    RMSE_ROWS evenly spaced full rows (the reference costs ~1e9 paths)."""
+    rows = [int((k + 0.5) * job.height / RMSE_ROWS) for k in range(RMSE_ROWS)]
+    img = m.Film(ctx, job.width, job.height)
+    img.render(scene, job)
+    ref_job = m.Job.from_buffer_copy(job)
+    ref_job.spp, ref_job.seed = RMSE_REF_SPP, job.seed + 7919
+    ref = m.Film(ctx, job.width, job.height)
+    t0 = time.perf_counter()
+    for r in rows:
+        ref.render_range(scene, ref_job, r * job.width, (r + 1) * job.width)
+    ctx.sync()
+    ref_s = time.perf_counter() - t0
+    a, b = img.download()[rows], ref.download()[rows]
+    img.close()
+    ref.close()
+    rmse = float(np.sqrt(np.mean((a.astype(np.float64) - b) ** 2)))
+    log(f"rmse {rmse:.5f} vs {RMSE_REF_SPP}-spp reference over {len(rows)} rows ({ref_s:.1f} s)")
+    return {"value": rmse, "reference_spp": RMSE_REF_SPP, "pixels": len(rows) * job.width,
+            "rows": rows, "mean_reference": float(b.mean())}
+
+
+def engine_settings(ctx, m):
+    return {"batch_paths": ctx.get_option(m.MPT_OPT_BATCH_PATHS), "stack": ctx.get_option(m.MPT_OPT_STACK),
+            "persistent": ctx.get_option(m.MPT_OPT_PERSISTENT)}
+
+
 def system_scenes(m, ctx):
     from milton_amd import workloads
     scenes, build = {}, {}
@@ -186,9 +218,16 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-rmse", action="store_true", help="skip the RMSE against a 65,536-spp reference")
     ap.add_argument("--batch-paths", type=int, default=0)
     ap.add_argument("--stack", type=int, default=0, help="traversal short-stack entries (8, 12, 16, 24)")
     ap.add_argument("--persistent", type=int, default=-1, help="1/0: persistent closest-hit waves on/off")
+    ap.add_argument("--kd-leaf", type=int, default=0, help="kd leaf size (0: library default)")
+    ap.add_argument("--kd-ci", type=float, default=0.0, help="kd SAH intersection cost (0: default)")
+    ap.add_argument("--kd-eb", type=float, default=0.0, help="kd SAH empty-side factor (0: default)")
+    ap.add_argument("--no-clip", action="store_true", help="kd build without triangle clipping")
+    ap.add_argument("--tune", action="store_true",
+                    help="autotune (kd leaf size and engine settings) on a sample of the job before timing")
     ap.add_argument("--workload", default="soup", choices=["soup", "batched", "serving"],
                     help="soup: the headline kd-tree stress scene; batched / serving: the systems workloads")
     ap.add_argument("--serve-seconds", type=float, default=20.0)
@@ -216,12 +255,19 @@ def main():
     t0 = time.perf_counter()
     arrays = m.generate(m.MPT_SCENE_SOUP, TRIANGLES, 1)
     gen_s = time.perf_counter() - t0
+    job = m.make_job(arrays.camera, WIDTH, HEIGHT, SPP, DEPTH, seed=1)
+    kp = m.KdParams()
+    kp.leaf_size, kp.cost_intersect, kp.empty_bias, kp.no_clip = args.kd_leaf, args.kd_ci, args.kd_eb, args.no_clip
     t0 = time.perf_counter()
-    scene = m.Scene(ctx, arrays)
+    tuned = None
+    if args.tune:
+        scene, tuned = m.Scene.tuned(ctx, arrays, job, kd=kp)
+        log(f"tuned: {tuned}")
+    else:
+        scene = m.Scene(ctx, arrays, kp)
     build_s = time.perf_counter() - t0
     st = scene.stats()
-    log(f"scene: gen {gen_s:.2f}s build+upload {build_s:.2f}s kd {st}")
-    job = m.make_job(arrays.camera, WIDTH, HEIGHT, SPP, DEPTH, seed=1)
+    log(f"scene: gen {gen_s:.2f}s build+upload{' +tune' if tuned else ''} {build_s:.2f}s kd {st}")
     film = m.Film(ctx, WIDTH, HEIGHT)
 
     # Counter pass (untimed): kd nodes and references visited by every
@@ -270,7 +316,8 @@ def main():
         "config": {"workload": "kd-tree stress scene: 1M random triangles, 1920x1080, 64 spp, max depth 4",
                    "triangles": TRIANGLES, "width": WIDTH, "height": HEIGHT, "spp": SPP, "max_depth": DEPTH,
                    "rays_per_step": rays, "kd_nodes": st["nodes"], "kd_refs": st["refs"],
-                   "scene_build_s": round(build_s, 3)},
+                   "scene_build_s": round(build_s, 3), "tuned": tuned,
+                   "engine": engine_settings(ctx, m)},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
                      "kernel": "k_extend", "alg_bytes_per_launch": alg_bytes / launches,
@@ -278,6 +325,8 @@ def main():
                      "nodes_per_ray": cst["nodes_visited"] / ext_rays,
                      "refs_per_ray": cst["prims_tested"] / ext_rays},
     }
+    if not args.no_rmse:
+        out["rmse"] = rmse_vs_reference(m, ctx, scene, job)
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)
         out["cpu_baseline"] = cpu_baseline(arrays, job, args.cpu_seconds, threads)

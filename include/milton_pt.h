@@ -155,7 +155,7 @@ enum {
   MPT_OPT_COUNTERS = 3,    /* 1: count kd nodes and references visited by closest-hit rays */
   MPT_OPT_GRAPHS = 4,      /* 1 (default): replay each batch's kernels from a captured hipGraph */
   MPT_OPT_STACK = 5,       /* traversal short-stack entries per ray: 8 (default), 12, 16 or 24 */
-  MPT_OPT_PERSISTENT = 6   /* closest hit: 0 one ray per lane; 1 (default) / 2 persistent waves that
+  MPT_OPT_PERSISTENT = 6   /* closest hit: 0 (default) one ray per lane; 1 / 2 persistent waves that
                               refill finished lanes with new rays, register budget for 7 / 8 waves */
 };
 
@@ -216,6 +216,11 @@ int mpt_render(mpt_scene *scene, const mpt_job *job, float *film_rgb, mpt_render
 int mpt_film_create(mpt_context *ctx, uint32_t width, uint32_t height, mpt_film **film);
 int mpt_film_destroy(mpt_film *film);
 int mpt_render_film(mpt_scene *scene, const mpt_job *job, mpt_film *film, mpt_render_stats *stats);
+/* Pixels [pix_begin, pix_end) (row-major) of the job only, accumulated into
+ * film: a tile of the job, the unit a job is split into across devices or
+ * over time.  Paths are the same as in mpt_render_film. */
+int mpt_render_film_range(mpt_scene *scene, const mpt_job *job, uint32_t pix_begin, uint32_t pix_end,
+                          mpt_film *film, mpt_render_stats *stats);
 int mpt_film_download(mpt_film *film, float *film_rgb);
 
 /* Per-path radiance of pixels [pix_begin, pix_end) of a job, all samples,
@@ -223,6 +228,29 @@ int mpt_film_download(mpt_film *film, float *film_rgb);
  * Test hook: the value PathTracer::_evaluate returns for each camera ray. */
 int mpt_render_paths(mpt_scene *scene, const mpt_job *job, uint32_t pix_begin, uint32_t pix_end,
                      float *L_out);
+
+/* Autotuning.  No reference counterpart: the reference's only knobs on this
+ * path are kdTreeAccel::BuildParams (accel/kdTreeAccel.h:68-116), fixed per
+ * scene file.  These entry points time a sample of a job (its camera, depth
+ * and seed at fewer samples per pixel over a centred pixel window, about
+ * sample_paths paths) under candidate settings and keep the fastest.  None of
+ * the settings changes a rendered result. */
+typedef struct mpt_tune_result {
+  uint64_t batch_paths;         /* MPT_OPT_BATCH_PATHS chosen */
+  uint32_t stack, persistent;   /* MPT_OPT_STACK, MPT_OPT_PERSISTENT chosen */
+  uint32_t leaf_size;           /* kd leaf size of the scene */
+  uint32_t trials;              /* sample renders timed */
+  double paths_per_s;           /* sampled rate at the chosen settings */
+  double baseline_paths_per_s;  /* sampled rate at the settings on entry */
+} mpt_tune_result;
+
+/* Engine settings (persistent, stack, batch paths) for a resident scene; the
+ * fastest are left set on the scene's context. */
+int mpt_tune(mpt_scene *scene, const mpt_job *job, uint64_t sample_paths, mpt_tune_result *result);
+/* Also the kd leaf size (1..16, hill climb from kd->leaf_size or 2): builds
+ * the scene once per candidate and returns the fastest build in *scene. */
+int mpt_tune_scene(mpt_context *ctx, const mpt_scene_desc *desc, const mpt_kd_params *kd, const mpt_job *job,
+                   uint64_t sample_paths, mpt_scene **scene, mpt_tune_result *result);
 
 /* ------------------------------------------------------------------------
  * Batching and serving.  No reference counterpart: the reference renders one
@@ -276,6 +304,7 @@ int mpt_server_stats(mpt_server *server, uint64_t out[6]);
 int mpt_sync(mpt_context *ctx);
 /* Sets a context option (MPT_OPT_*). */
 int mpt_set_option(mpt_context *ctx, uint32_t option, uint64_t value);
+int mpt_get_option(mpt_context *ctx, uint32_t option, uint64_t *value);
 
 #ifdef __cplusplus
 }

@@ -83,6 +83,15 @@ class ServerConfig(ctypes.Structure):
     _fields_ = [("hbm_budget", ctypes.c_uint64), ("max_active", ctypes.c_uint32), ("max_inflight", ctypes.c_uint32)]
 
 
+class TuneResult(ctypes.Structure):
+    _fields_ = [("batch_paths", ctypes.c_uint64), ("stack", ctypes.c_uint32), ("persistent", ctypes.c_uint32),
+                ("leaf_size", ctypes.c_uint32), ("trials", ctypes.c_uint32), ("paths_per_s", ctypes.c_double),
+                ("baseline_paths_per_s", ctypes.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 class JobStatus(ctypes.Structure):
     _fields_ = [("state", ctypes.c_uint32), ("error", ctypes.c_uint32), ("pixels_done", ctypes.c_uint64),
                 ("pixels_total", ctypes.c_uint64), ("submit_ms", ctypes.c_double), ("start_ms", ctypes.c_double),
@@ -123,6 +132,7 @@ def lib():
         "mpt_last_error": ([vp], ctypes.c_char_p),
         "mpt_sync": ([vp], i32),
         "mpt_set_option": ([vp, u32, u64], i32),
+        "mpt_get_option": ([vp, u32, ctypes.POINTER(ctypes.c_uint64)], i32),
         "mpt_scene_create": ([vp, ctypes.POINTER(SceneDesc), ctypes.POINTER(KdParams), ctypes.POINTER(vp)], i32),
         "mpt_scene_destroy": ([vp], i32),
         "mpt_scene_stats": ([vp, ctypes.POINTER(ctypes.c_uint64)], i32),
@@ -132,6 +142,7 @@ def lib():
         "mpt_film_create": ([vp, u32, u32, ctypes.POINTER(vp)], i32),
         "mpt_film_destroy": ([vp], i32),
         "mpt_render_film": ([vp, ctypes.POINTER(Job), vp, ctypes.POINTER(RenderStats)], i32),
+        "mpt_render_film_range": ([vp, ctypes.POINTER(Job), u32, u32, vp, ctypes.POINTER(RenderStats)], i32),
         "mpt_film_download": ([vp, vp], i32),
         "mpt_render_paths": ([vp, ctypes.POINTER(Job), u32, u32, vp], i32),
         "mpt_server_create": ([vp, ctypes.POINTER(ServerConfig), ctypes.POINTER(vp)], i32),
@@ -146,6 +157,9 @@ def lib():
         "mpt_server_start": ([vp], i32),
         "mpt_server_stop": ([vp], i32),
         "mpt_server_stats": ([vp, ctypes.POINTER(ctypes.c_uint64)], i32),
+        "mpt_tune": ([vp, ctypes.POINTER(Job), u64, ctypes.POINTER(TuneResult)], i32),
+        "mpt_tune_scene": ([vp, ctypes.POINTER(SceneDesc), ctypes.POINTER(KdParams), ctypes.POINTER(Job), u64,
+                            ctypes.POINTER(vp), ctypes.POINTER(TuneResult)], i32),
         "mpt_kd_build": ([ctypes.POINTER(SceneDesc), ctypes.POINTER(KdParams), ctypes.POINTER(vp)], i32),
         "mpt_kd_stats": ([vp, ctypes.POINTER(ctypes.c_uint64)], i32),
         "mpt_kd_trace": ([vp, vp, u64, vp, ctypes.POINTER(ctypes.c_uint64), u32], i32),
@@ -309,6 +323,11 @@ class Context:
             msg = self.L.mpt_last_error(self.h)
             raise MptError(f"{what} failed ({ERRORS.get(rc, rc)}): {msg.decode() if msg else ''}")
 
+    def get_option(self, option):
+        v = ctypes.c_uint64()
+        self.check(self.L.mpt_get_option(self.h, option, ctypes.byref(v)), "mpt_get_option")
+        return v.value
+
     def set_option(self, option, value):
         self.check(self.L.mpt_set_option(self.h, option, value), "mpt_set_option")
 
@@ -332,14 +351,35 @@ class Context:
 class Scene:
     """A device-resident scene with its kd-tree (Scene::init + kdTreeAccel::init)."""
 
-    def __init__(self, ctx, arrays, kd=None):
+    def __init__(self, ctx, arrays, kd=None, _handle=None):
         self.ctx, self.arrays = ctx, arrays
         self.h = ctypes.c_void_p()
+        if _handle is not None:
+            self.h = _handle
+        else:
+            d = arrays.desc()
+            kp = kd if kd is not None else KdParams()
+            ctx.check(ctx.L.mpt_scene_create(ctx.h, ctypes.byref(d), ctypes.byref(kp), ctypes.byref(self.h)),
+                      "mpt_scene_create")
+        ctx._children.add(self)
+
+    @classmethod
+    def tuned(cls, ctx, arrays, job, sample_paths=1 << 24, kd=None):
+        """Builds the scene at the fastest kd leaf size for `job` and leaves the
+        fastest engine settings on ctx (mpt_tune_scene); returns (scene, result)."""
         d = arrays.desc()
         kp = kd if kd is not None else KdParams()
-        ctx.check(ctx.L.mpt_scene_create(ctx.h, ctypes.byref(d), ctypes.byref(kp), ctypes.byref(self.h)),
-                  "mpt_scene_create")
-        ctx._children.add(self)
+        h = ctypes.c_void_p()
+        res = TuneResult()
+        ctx.check(ctx.L.mpt_tune_scene(ctx.h, ctypes.byref(d), ctypes.byref(kp), ctypes.byref(job), sample_paths,
+                                       ctypes.byref(h), ctypes.byref(res)), "mpt_tune_scene")
+        return cls(ctx, arrays, _handle=h), res.as_dict()
+
+    def tune(self, job, sample_paths=1 << 24):
+        """Engine settings for this scene and job (mpt_tune), left on the context."""
+        res = TuneResult()
+        self.ctx.check(self.ctx.L.mpt_tune(self.h, ctypes.byref(job), sample_paths, ctypes.byref(res)), "mpt_tune")
+        return res.as_dict()
 
     def stats(self):
         out = (ctypes.c_uint64 * 8)()
@@ -399,6 +439,13 @@ class Film:
         st = RenderStats()
         self.ctx.check(self.ctx.L.mpt_render_film(scene.h, ctypes.byref(job), self.h, ctypes.byref(st)),
                        "mpt_render_film")
+        return st.as_dict()
+
+    def render_range(self, scene, job, pix_begin, pix_end):
+        """Only pixels [pix_begin, pix_end) of the job (a tile)."""
+        st = RenderStats()
+        self.ctx.check(self.ctx.L.mpt_render_film_range(scene.h, ctypes.byref(job), pix_begin, pix_end, self.h,
+                                                        ctypes.byref(st)), "mpt_render_film_range")
         return st.as_dict()
 
     def download(self):

@@ -75,26 +75,28 @@ MPT_HD float comp(float3 a, uint32_t i) { return i == 0 ? a.x : (i == 1 ? a.y : 
 MPT_HD bool is_black(float3 a) { return a.x == 0.0f && a.y == 0.0f && a.z == 0.0f; }
 MPT_HD float max3(float3 a) { return fmaxf(a.x, fmaxf(a.y, a.z)); }
 
-// Per-ray constants: origin, direction, and the shear of the watertight
-// triangle test (Woop, Benthin, Wald, JCGT 2013): the axis permutation
-// (kx, ky, kz packed in k) and the factors sx, sy, sz.  The reciprocal
-// direction for the kd planes is recomputed per node (one rcp) to keep the
-// per-lane state small.
+// Per-ray constants: origin, direction, its reciprocal for the kd planes,
+// and the shear of the watertight triangle test (Woop, Benthin, Wald, JCGT
+// 2013): the axis permutation (kx, ky, kz packed in k) and the factors sx,
+// sy, sz.
 struct RaySetup {
-  float3 o, d;
+  float3 o, d, inv;
   float sx, sy, sz;
   uint32_t k;   // kx | ky << 2 | kz << 4
 };
 
+// 1/x with |x| clamped to 1e-20 so axis-parallel rays stay finite.  Both
+// zeros map to -1e20: inv < 0 then means d <= 0, which is what the kd walk
+// uses to order the children of a plane the ray starts on.
 MPT_HD float safe_inv(float x) {
-  const float ax = fmaxf(fabsf(x), 1e-20f);
-  return 1.0f / copysignf(ax, x);
+  return x > 0.0f ? 1.0f / fmaxf(x, 1e-20f) : -1.0f / fmaxf(-x, 1e-20f);
 }
 
 MPT_HD RaySetup setup_ray(float3 o, float3 d) {
   RaySetup r;
   r.o = o;
   r.d = d;
+  r.inv = f3(safe_inv(d.x), safe_inv(d.y), safe_inv(d.z));
   const float ax = fabsf(d.x), ay = fabsf(d.y), az = fabsf(d.z);
   uint32_t kz = (ax >= ay && ax >= az) ? 0 : (ay >= az ? 1 : 2);
   uint32_t kx = kz == 2 ? 0 : kz + 1;
@@ -117,7 +119,11 @@ MPT_HD RaySetup setup_ray(float3 o, float3 d) {
 // triangle whose Moller-Trumbore determinant |d . ((v1-v0) x (v2-v0))| is at
 // most EPSILON is rejected (EQ(det, 0), MeshTriangle.cpp:144).  The sheared
 // determinant U + V + W equals that product divided by d[kz] = 1 / sz.
-MPT_HD float tri_t(const RaySetup &r, float3 v0, float3 v1, float3 v2) {
+//
+// tri_td leaves the division to the caller: on a hit it returns the scaled
+// distance T and determinant det > 0 with t = T / det, so a leaf loop can
+// reject candidates beyond its current best without dividing.
+MPT_HD bool tri_td(const RaySetup &r, float3 v0, float3 v1, float3 v2, float &T, float &det) {
   const uint32_t kx = r.k & 3u, ky = (r.k >> 2) & 3u, kz = r.k >> 4;
   const float3 A = sub(v0, r.o), B = sub(v1, r.o), C = sub(v2, r.o);
   const float Az = comp(A, kz), Bz = comp(B, kz), Cz = comp(C, kz);
@@ -127,11 +133,19 @@ MPT_HD float tri_t(const RaySetup &r, float3 v0, float3 v1, float3 v2) {
   const float U = Cx * By - Cy * Bx;
   const float V = Ax * Cy - Ay * Cx;
   const float W = Bx * Ay - By * Ax;
-  if ((U < 0.0f || V < 0.0f || W < 0.0f) && (U > 0.0f || V > 0.0f || W > 0.0f)) return INFINITY;
-  const float det = U + V + W;
-  if (fabsf(det) <= kEps * fabsf(r.sz)) return INFINITY;
-  const float T = (U * Az + V * Bz + W * Cz) * r.sz;
-  return T / det;
+  if ((U < 0.0f || V < 0.0f || W < 0.0f) && (U > 0.0f || V > 0.0f || W > 0.0f)) return false;
+  const float d = U + V + W;
+  if (fabsf(d) <= kEps * fabsf(r.sz)) return false;
+  const float t = (U * Az + V * Bz + W * Cz) * r.sz;
+  // (-T) / (-det) == T / det exactly, so the sign moves onto T
+  T = d < 0.0f ? -t : t;
+  det = fabsf(d);
+  return true;
+}
+
+MPT_HD float tri_t(const RaySetup &r, float3 v0, float3 v1, float3 v2) {
+  float T, det;
+  return tri_td(r, v0, v1, v2, T, det) ? T / det : INFINITY;
 }
 
 // Sphere (centre c, radius rad): nearest root above EPSILON; a ray leaving the
@@ -220,7 +234,7 @@ struct Walker {
 
   MPT_HD bool start(const Scene &sc, const RaySetup &r, float tlimit, uint32_t so) {
     // Root slab test (accel/AABB.inl:25-56).
-    const float ix = safe_inv(r.d.x), iy = safe_inv(r.d.y), iz = safe_inv(r.d.z);
+    const float ix = r.inv.x, iy = r.inv.y, iz = r.inv.z;
     const float tx0 = (sc.box_lo[0] - r.o.x) * ix, tx1 = (sc.box_hi[0] - r.o.x) * ix;
     const float ty0 = (sc.box_lo[1] - r.o.y) * iy, ty1 = (sc.box_hi[1] - r.o.y) * iy;
     const float tz0 = (sc.box_lo[2] - r.o.z) * iz, tz1 = (sc.box_hi[2] - r.o.z) * iz;
@@ -248,10 +262,11 @@ struct Walker {
     while ((nd.x & 3u) != 3u) {
       const uint32_t axis = nd.x & 3u;
       const float split = mpt_u2f(nd.y);
-      const float oa = comp(r.o, axis), da = comp(r.d, axis);
-      const float tp = (split - oa) * safe_inv(da);
+      const float oa = comp(r.o, axis), ia = comp(r.inv, axis);
+      const float tp = (split - oa) * ia;
       const uint32_t child = nd.x >> 2;
-      const bool below = (oa < split) || (oa == split && da <= 0.0f);
+      // ia < 0 <=> d[axis] <= 0 (both zeros map to a negative reciprocal)
+      const bool below = (oa < split) | ((oa == split) & (ia < 0.0f));
       const uint32_t first = below ? child : child + 1;
       const uint32_t second = below ? child + 1 : child;
       if (tp > tmax || tp <= 0.0f) {
@@ -280,7 +295,14 @@ struct Walker {
       if (kind == KIND_TRIANGLE) {
         if (id == skip_origin) continue;
         const float4 c = rec[2];
-        t = tri_t(r, xyz(a), xyz(b), xyz(c));
+        float T, det;
+        // divide only for candidates that can beat the current best; the
+        // 2^-20 margins keep the prefilter conservative, the exact test
+        // below decides
+        if (!tri_td(r, xyz(a), xyz(b), xyz(c), T, det) || T > best * det * (1.0f + 0x1p-20f) ||
+            T < kEps * det * (1.0f - 0x1p-20f))
+          continue;
+        t = T / det;
       } else if (kind == KIND_SPHERE) {
         t = sphere_t(r, xyz(a), b.x, id == skip_origin);
       } else {

@@ -131,7 +131,7 @@ struct mpt_context {
   bool counters = false;
   bool graphs = true;
   uint32_t stack = 8;                // short-stack entries of the traversal kernels
-  uint32_t persistent = 1;           // closest hit: 0 one ray per lane, 1/2 persistent waves with lane
+  uint32_t persistent = 0;           // closest hit: 0 one ray per lane, 1/2 persistent waves with lane
                                      // refill at 7 / 8 waves per SIMD
   std::vector<hipEvent_t> events;    // timing pool, pairs per launch
   std::vector<mpt_scene *> scenes;   // live scenes, films and servers, released by mpt_close
@@ -153,6 +153,7 @@ struct mpt_scene {
   mpt_engine::DBuf<mpt_material> mats;
   mpt_dev::Scene dev{};
   uint64_t stats[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  uint32_t leaf_size = 0;   // kd leaf size the tree was built with
   uint64_t device_bytes() const {
     return nodes.bytes() + recs.bytes() + verts.bytes() + vnormals.bytes() + spheres.bytes() + quads.bytes() +
            tris.bytes() + sphere_mat.bytes() + lights.bytes() + mats.bytes();
@@ -184,6 +185,12 @@ int run_batch(mpt_context *ctx, mpt_scene *scene, const BatchPlan &plan, float *
 // Zeroes the device ray totals; read_totals syncs and returns
 // [camera, extension, shadow, nodes, refs].
 int reset_totals(mpt_context *ctx);
+// Renders pixels [pix_begin, pix_end) of one job into film (accumulating) or
+// per-path radiance into L_host; stats optional.
+int render_range(mpt_scene *s, const mpt_job *job, uint32_t pix_begin, uint32_t pix_end, float4 *film,
+                 float *L_host, mpt_render_stats *stats);
+// Drops captured graphs of one scene uid (0: all).
+void drop_graphs(mpt_context *ctx, uint64_t scene_uid);
 int read_totals(mpt_context *ctx, unsigned long long out[5]);
 
 }  // namespace mpt_engine

@@ -549,17 +549,6 @@ struct Timed {
 };
 std::vector<Timed> g_timed;   // guarded by the single-threaded use of a context
 
-void drop_graphs(mpt_context *ctx, uint64_t scene_uid) {
-  auto &c = ctx->graphs_cache;
-  for (auto it = c.begin(); it != c.end();) {
-    if (scene_uid == 0 || (uint64_t)(uintptr_t)it->scene == scene_uid) {
-      (void)hipGraphExecDestroy(it->exec);
-      it = c.erase(it);
-    } else {
-      ++it;
-    }
-  }
-}
 
 template <int STACK>
 void launch_extend_t(mpt_context *ctx, uint32_t g, const Scene &sc, const float4 *q, const uint32_t *count,
@@ -679,6 +668,18 @@ int stage_plan(mpt_context *ctx, const BatchPlan &plan) {
 }  // namespace
 
 namespace mpt_engine {
+
+void drop_graphs(mpt_context *ctx, uint64_t scene_uid) {
+  auto &c = ctx->graphs_cache;
+  for (auto it = c.begin(); it != c.end();) {
+    if (scene_uid == 0 || (uint64_t)(uintptr_t)it->scene == scene_uid) {
+      (void)hipGraphExecDestroy(it->exec);
+      it = c.erase(it);
+    } else {
+      ++it;
+    }
+  }
+}
 
 int fail(mpt_context *ctx, int code, const std::string &msg) {
   if (ctx) ctx->err = msg;
@@ -833,10 +834,6 @@ int run_batch(mpt_context *ctx, mpt_scene *s, const BatchPlan &plan, float *L_ho
   return MPT_OK;
 }
 
-}  // namespace mpt_engine
-
-namespace {
-
 // Renders pixels [pix_begin, pix_end) of one job as consecutive batches of
 // whole pixels.
 int render_range(mpt_scene *s, const mpt_job *job, uint32_t pix_begin, uint32_t pix_end, float4 *film,
@@ -885,7 +882,7 @@ int render_range(mpt_scene *s, const mpt_job *job, uint32_t pix_begin, uint32_t 
   return MPT_OK;
 }
 
-}  // namespace
+}  // namespace mpt_engine
 
 namespace {
 
@@ -898,6 +895,7 @@ struct HostScene {
   std::vector<mpt_material> mats;
   mpt_kd::Output ko;
   uint32_t nt = 0, ns = 0, nq = 0;
+  uint32_t leaf_size = 0;   // effective kd leaf size
   float env[3] = {0, 0, 0};
 };
 
@@ -1016,6 +1014,7 @@ int prepare_scene(mpt_context *ctx, const mpt_scene_desc *d, const mpt_kd_params
     if (kd->bins) kp.bins = kd->bins;
     kp.clip = kd->no_clip == 0;
   }
+  hs.leaf_size = kp.leaf_size;
   std::vector<float> tri_verts(9ull * nt);
   for (uint32_t t = 0; t < nt; ++t) {
     const float4 a = verts[tris[t].x], b = verts[tris[t].y], c = verts[tris[t].z];
@@ -1174,6 +1173,19 @@ int mpt_set_option(mpt_context *ctx, uint32_t option, uint64_t value) {
   }
 }
 
+int mpt_get_option(mpt_context *ctx, uint32_t option, uint64_t *value) {
+  if (!ctx || !value) return MPT_E_INVALID;
+  switch (option) {
+    case MPT_OPT_BATCH_PATHS: *value = ctx->batch_paths; return MPT_OK;
+    case MPT_OPT_TIMING: *value = ctx->timing; return MPT_OK;
+    case MPT_OPT_COUNTERS: *value = ctx->counters; return MPT_OK;
+    case MPT_OPT_GRAPHS: *value = ctx->graphs; return MPT_OK;
+    case MPT_OPT_STACK: *value = ctx->stack; return MPT_OK;
+    case MPT_OPT_PERSISTENT: *value = ctx->persistent; return MPT_OK;
+    default: return fail(ctx, MPT_E_INVALID, "unknown option");
+  }
+}
+
 int mpt_sync(mpt_context *ctx) {
   if (!ctx) return MPT_E_INVALID;
   MPT_HIP(ctx, hipStreamSynchronize(ctx->stream));
@@ -1213,6 +1225,7 @@ int mpt_scene_create(mpt_context *ctx, const mpt_scene_desc *d, const mpt_kd_par
   s->stats[5] = (uint64_t)ko.build_ms;
   s->stats[6] = ko.max_leaf;
   s->stats[7] = nt;
+  s->leaf_size = hs.leaf_size;
   s->uid = ++g_scene_uid;
   ctx->scenes.push_back(s.get());
   *out = s.release();
@@ -1378,6 +1391,16 @@ int mpt_render_film(mpt_scene *s, const mpt_job *job, mpt_film *film, mpt_render
   if (!job || job->width != film->width || job->height != film->height)
     return fail(s->ctx, MPT_E_INVALID, "film size does not match the job");
   return render_range(s, job, 0, job->width * job->height, film->data.p, nullptr, stats);
+}
+
+int mpt_render_film_range(mpt_scene *s, const mpt_job *job, uint32_t pix_begin, uint32_t pix_end, mpt_film *film,
+                          mpt_render_stats *stats) {
+  if (!s || !film) return fail(s ? s->ctx : nullptr, MPT_E_INVALID, "NULL argument");
+  if (!job || job->width != film->width || job->height != film->height)
+    return fail(s->ctx, MPT_E_INVALID, "film size does not match the job");
+  if (pix_begin >= pix_end || (uint64_t)pix_end > (uint64_t)job->width * job->height)
+    return fail(s->ctx, MPT_E_INVALID, "bad pixel range");
+  return render_range(s, job, pix_begin, pix_end, film->data.p, nullptr, stats);
 }
 
 int mpt_film_download(mpt_film *f, float *rgb) {

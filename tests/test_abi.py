@@ -5,6 +5,7 @@ deterministic and have the specified shapes."""
 import ctypes
 import os
 import re
+import subprocess
 
 import numpy as np
 
@@ -26,6 +27,32 @@ def test_library_exports_every_declared_symbol():
     missing = [n for n in names if not hasattr(L, n)]
     assert not missing, missing
     assert L.mpt_abi_version() == m.MPT_ABI_VERSION
+
+
+_C_STRUCTS = {"Material": "mpt_material", "SceneDesc": "mpt_scene_desc", "KdParams": "mpt_kd_params",
+              "Camera": "mpt_camera", "Job": "mpt_job", "Ray": "mpt_ray", "RenderStats": "mpt_render_stats",
+              "ServerConfig": "mpt_server_config", "JobStatus": "mpt_job_status", "TuneResult": "mpt_tune_result"}
+
+
+def test_struct_layouts_match_header(tmp_path):
+    """Every ctypes mirror has the size and field offsets the C compiler
+    gives the header's struct."""
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "milton_pt.h"', "int main(void) {"]
+    for py, c in _C_STRUCTS.items():
+        lines.append(f'  printf("{py} size %zu\\n", sizeof({c}));')
+        for f, _ in getattr(m, py)._fields_:
+            lines.append(f'  printf("{py} {f} %zu\\n", offsetof({c}, {f}));')
+    lines += ["  return 0;", "}"]
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-std=c99", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
+    got = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split("\n")
+    for line in filter(None, got):
+        py, field, val = line.split()
+        cls = getattr(m, py)
+        want = ctypes.sizeof(cls) if field == "size" else getattr(cls, field).offset
+        assert int(val) == want, (py, field, int(val), want)
 
 
 def test_struct_layouts():
