@@ -94,11 +94,14 @@ typedef struct mpt_scene_desc {
 
 /* Replaces kdTreeAccel::BuildParams (accel/kdTreeAccel.h:68-116). */
 typedef struct mpt_kd_params {
-  uint32_t leaf_size;     /* a node with <= leaf_size references becomes a leaf (kdMinPrimitives) */
+  uint32_t leaf_size;     /* a node with <= leaf_size references becomes a leaf (kdMinPrimitives);
+                             0 = default 4 */
   uint32_t max_depth;     /* kdMaxDepth (at most 48) */
   float cost_traversal;   /* kdCostTraversal */
   float cost_intersect;   /* parsed but unused by the reference (kdCostIntersect) */
-  float empty_bias;       /* kdEmptyBias */
+  float empty_bias;       /* kdEmptyBias: SAH cost factor of a split with an empty side; 0 = default
+                             1.0 (no bonus: on this device a node visit costs more than the
+                             references a bonus split saves) */
   uint32_t num_threads;   /* kdNoThreads; 0 = all host cores */
   uint32_t bins;          /* SAH bins per axis (0 = default) */
   uint32_t no_clip;       /* 1: split triangles by their bounds only (0: clip them to the cells) */
@@ -150,13 +153,15 @@ typedef struct mpt_render_stats {
 /* Context options (mpt_set_option).  No reference counterpart: these are the
  * knobs of the device path (queue depth) and its instrumentation. */
 enum {
-  MPT_OPT_BATCH_PATHS = 1, /* paths per wavefront batch (queue depth), default 2^23 */
+  MPT_OPT_BATCH_PATHS = 1, /* paths per wavefront batch (queue depth), default 2^24 */
   MPT_OPT_TIMING = 2,      /* 1: time every kernel launch with HIP events into *_ms */
   MPT_OPT_COUNTERS = 3,    /* 1: count kd nodes and references visited by closest-hit rays */
   MPT_OPT_GRAPHS = 4,      /* 1 (default): replay each batch's kernels from a captured hipGraph */
   MPT_OPT_STACK = 5,       /* traversal short-stack entries per ray: 8 (default), 12, 16 or 24 */
-  MPT_OPT_PERSISTENT = 6   /* closest hit: 0 (default) one ray per lane; 1 / 2 persistent waves that
-                              refill finished lanes with new rays, register budget for 7 / 8 waves */
+  MPT_OPT_PERSISTENT = 6   /* closest-hit kernel form: 0 (default) one ray per lane, grid-stride over
+                              8 blocks per CU; 1 persistent waves that refill finished lanes from a
+                              per-wave range of the queue; 2 one ray per lane, one grid slot per
+                              queue slot */
 };
 
 typedef struct mpt_context mpt_context;
@@ -247,7 +252,7 @@ typedef struct mpt_tune_result {
 /* Engine settings (persistent, stack, batch paths) for a resident scene; the
  * fastest are left set on the scene's context. */
 int mpt_tune(mpt_scene *scene, const mpt_job *job, uint64_t sample_paths, mpt_tune_result *result);
-/* Also the kd leaf size (1..16, hill climb from kd->leaf_size or 2): builds
+/* Also the kd leaf size (1..16, hill climb from kd->leaf_size or 4): builds
  * the scene once per candidate and returns the fastest build in *scene. */
 int mpt_tune_scene(mpt_context *ctx, const mpt_scene_desc *desc, const mpt_kd_params *kd, const mpt_job *job,
                    uint64_t sample_paths, mpt_scene **scene, mpt_tune_result *result);

@@ -91,7 +91,7 @@ struct Workspace {
   DBuf<float4> Ls, betas;
   DBuf<uint32_t> flags;
   DBuf<uint2> pjs;           // (pixel, job << 20 | sample) per slot
-  DBuf<uint32_t> counts;     // [0, D] queue sizes, [D+1, 2D] shadow sizes, [2D+1, 3D] ray fetch cursors
+  DBuf<uint32_t> counts;     // [0, D] queue sizes, [D+1, 2D] shadow sizes
   DBuf<unsigned long long> totals;
   DBuf<BatchDesc> desc;
   size_t bytes() const {
@@ -126,13 +126,13 @@ struct mpt_context {
   int num_cus = 256;
   std::string err;
   mpt_engine::Workspace ws;
-  uint32_t batch_paths = 1u << 23;   // paths per batch (queue depth)
+  uint32_t batch_paths = 1u << 24;   // paths per batch (queue depth)
   bool timing = false;
   bool counters = false;
   bool graphs = true;
   uint32_t stack = 8;                // short-stack entries of the traversal kernels
-  uint32_t persistent = 0;           // closest hit: 0 one ray per lane, 1/2 persistent waves with lane
-                                     // refill at 7 / 8 waves per SIMD
+  uint32_t persistent = 0;           // closest-hit kernel form: 0 grid-stride, 1 persistent waves with
+                                     // lane refill, 2 one ray per lane over a capacity-sized grid
   std::vector<hipEvent_t> events;    // timing pool, pairs per launch
   std::vector<mpt_scene *> scenes;   // live scenes, films and servers, released by mpt_close
   std::vector<mpt_film *> films;

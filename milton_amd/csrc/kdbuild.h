@@ -30,11 +30,11 @@ struct Box {
 };
 
 struct Params {
-  uint32_t leaf_size = 2;
+  uint32_t leaf_size = 4;
   uint32_t max_depth = 0;          // 0: 8 + 1.3 log2(N), at most 48
   float cost_traversal = 1.0f;
   float cost_intersect = 1.5f;
-  float empty_bias = 0.8f;
+  float empty_bias = 1.0f;
   uint32_t num_threads = 0;        // 0: hardware concurrency
   uint32_t bins = 32;
   bool clip = true;                // clip triangles to the cells they are split across

@@ -148,53 +148,50 @@ __global__ __launch_bounds__(256) void k_extend(Scene sc, const float4 *__restri
   }
 }
 
-// Persistent variant of k_extend: each wave keeps its lanes busy by handing a
-// lane whose ray finished a new ray from the queue (one atomic per wave per
-// refill, taken when at least kRefill lanes are idle or all are), so a wave no
-// longer idles until its slowest ray ends.
+// Persistent variant of k_extend: each wave owns a contiguous range of the
+// queue and keeps its lanes busy by handing a lane whose ray finished the next
+// ray of its range (refill taken when at least kRefill lanes are idle, or all
+// are), so a wave no longer idles until its slowest ray ends.  Ranges are
+// static, so a refill costs no atomic; the imbalance is one ray per lane at
+// the end of a range.
 constexpr uint32_t kRefill = 16;
 
-template <bool COUNT, int STACK, int WAVES>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WAVES))) void k_extend_persistent(Scene sc, const float4 *__restrict__ q,
-                                                           const uint32_t *__restrict__ count,
-                                                           float2 *__restrict__ hits,
-                                                           unsigned long long *__restrict__ counters,
-                                                           uint32_t *__restrict__ fetch) {
+template <bool COUNT, int STACK>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void k_extend_persistent(
+    Scene sc, const float4 *__restrict__ q, const uint32_t *__restrict__ count, float2 *__restrict__ hits,
+    unsigned long long *__restrict__ counters) {
   __shared__ uint2 stack[STACK * kBlock];
   uint2 *stk = stack + threadIdx.x;
   const uint32_t n = *count;
+  const uint32_t waves = gridDim.x * (kBlock / 64);
+  const uint32_t wave = blockIdx.x * (kBlock / 64) + __builtin_amdgcn_readfirstlane(threadIdx.x / 64);
+  const uint32_t per = (n + waves - 1) / waves;
+  uint32_t next = min(n, wave * per);           // wave-uniform
+  const uint32_t end = min(n, next + per);
   uint32_t n_nodes = 0, n_refs = 0;
   uint32_t ray = kMiss;
-  bool exhausted = false;
   RaySetup r;
   Walker<false, STACK> w;
   for (;;) {
-    const bool need = ray == kMiss && !exhausted;
+    const bool need = ray == kMiss;
     const uint64_t idle = __ballot(need);
-    const uint64_t busy = __ballot(ray != kMiss);
-    if (idle != 0 && (__popcll(idle) >= kRefill || busy == 0)) {
-      const uint32_t leader = (uint32_t)__ffsll((unsigned long long)idle) - 1;
-      uint32_t base = 0;
-      if (lane_id() == leader) base = atomicAdd(fetch, (uint32_t)__popcll(idle));
-      base = __shfl(base, leader);
-      if (need) {
-        const uint32_t idx = base + __builtin_amdgcn_mbcnt_hi((uint32_t)(idle >> 32),
-                                                              __builtin_amdgcn_mbcnt_lo((uint32_t)idle, 0u));
-        if (idx < n) {
-          const float4 a = q[2 * idx], b = q[2 * idx + 1];
-          r = setup_ray(xyz(a), xyz(b));
-          if (w.start(sc, r, INFINITY, __float_as_uint(b.w))) {
-            ray = idx;
-          } else {
-            hits[idx] = make_float2(INFINITY, __uint_as_float(kMiss));
-          }
+    const uint32_t n_idle = (uint32_t)__popcll(idle);
+    if (next < end && (n_idle >= kRefill || n_idle == 64)) {
+      const uint32_t idx = next + __builtin_amdgcn_mbcnt_hi((uint32_t)(idle >> 32),
+                                                            __builtin_amdgcn_mbcnt_lo((uint32_t)idle, 0u));
+      next = min(end, next + n_idle);
+      if (need && idx < end) {
+        const float4 a = q[2 * idx], b = q[2 * idx + 1];
+        r = setup_ray(xyz(a), xyz(b));
+        if (w.start(sc, r, INFINITY, __float_as_uint(b.w))) {
+          ray = idx;
         } else {
-          exhausted = true;
+          hits[idx] = make_float2(INFINITY, __uint_as_float(kMiss));
         }
       }
     }
     if (__ballot(ray != kMiss) == 0) {
-      if (__ballot(!exhausted) == 0) break;
+      if (next >= end) break;
       continue;
     }
     if (ray != kMiss && w.template step<COUNT>(sc, r, stk, &n_nodes, &n_refs)) {
@@ -276,7 +273,7 @@ __device__ __forceinline__ HitInfo hit_info(const Scene &sc, uint32_t prim, floa
 // 28-116) with DirectIllumination::evaluate (renderers/DirectIllumination.cpp:
 // 46-136).  The recursion becomes a throughput `beta`; the emission terms, NEE
 // and roulette follow the reference rules, each cited below.
-__global__ __launch_bounds__(256) void k_shade(Scene sc, const BatchDesc *__restrict__ bd, uint32_t depth,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k_shade(Scene sc, const BatchDesc *__restrict__ bd, uint32_t depth,
                                                const float4 *__restrict__ qin, const float2 *__restrict__ hits,
                                                const uint32_t *__restrict__ count_in, float4 *__restrict__ qout,
                                                uint32_t *__restrict__ count_out, float4 *__restrict__ sq,
@@ -552,33 +549,32 @@ std::vector<Timed> g_timed;   // guarded by the single-threaded use of a context
 
 template <int STACK>
 void launch_extend_t(mpt_context *ctx, uint32_t g, const Scene &sc, const float4 *q, const uint32_t *count,
-                     float2 *hits, unsigned long long *counters, uint32_t *fetch) {
-  if (ctx->persistent == 2) {
+                     float2 *hits, unsigned long long *counters) {
+  if (ctx->persistent == 1) {
     if (counters)
-      k_extend_persistent<true, STACK, 8><<<g, kBlock, 0, ctx->stream>>>(sc, q, count, hits, counters, fetch);
+      k_extend_persistent<true, STACK><<<g, kBlock, 0, ctx->stream>>>(sc, q, count, hits, counters);
     else
-      k_extend_persistent<false, STACK, 8><<<g, kBlock, 0, ctx->stream>>>(sc, q, count, hits, nullptr, fetch);
-  } else if (ctx->persistent) {
-    if (counters)
-      k_extend_persistent<true, STACK, 7><<<g, kBlock, 0, ctx->stream>>>(sc, q, count, hits, counters, fetch);
-    else
-      k_extend_persistent<false, STACK, 7><<<g, kBlock, 0, ctx->stream>>>(sc, q, count, hits, nullptr, fetch);
-  } else if (counters) {
-    k_extend<true, STACK><<<g, kBlock, 0, ctx->stream>>>(sc, q, count, hits, counters);
-  } else {
-    k_extend<false, STACK><<<g, kBlock, 0, ctx->stream>>>(sc, q, count, hits, nullptr);
+      k_extend_persistent<false, STACK><<<g, kBlock, 0, ctx->stream>>>(sc, q, count, hits, nullptr);
+    return;
   }
+  // form 0: grid-stride over a fixed grid of 8 blocks per CU; form 2: one
+  // ray per lane over a grid sized by the batch capacity
+  const uint32_t grid = ctx->persistent == 2 ? (uint32_t)((ctx->ws.cap + kBlock - 1) / kBlock) : g;
+  if (counters)
+    k_extend<true, STACK><<<grid, kBlock, 0, ctx->stream>>>(sc, q, count, hits, counters);
+  else
+    k_extend<false, STACK><<<grid, kBlock, 0, ctx->stream>>>(sc, q, count, hits, nullptr);
 }
 
 // The short-stack depth is a run-time choice among compiled variants (it
 // trades LDS per wave, hence occupancy, against kd-restarts).
 void launch_extend(mpt_context *ctx, uint32_t g, const Scene &sc, const float4 *q, const uint32_t *count,
-                   float2 *hits, unsigned long long *counters, uint32_t *fetch) {
+                   float2 *hits, unsigned long long *counters) {
   switch (ctx->stack) {
-    case 8: launch_extend_t<8>(ctx, g, sc, q, count, hits, counters, fetch); break;
-    case 12: launch_extend_t<12>(ctx, g, sc, q, count, hits, counters, fetch); break;
-    case 24: launch_extend_t<24>(ctx, g, sc, q, count, hits, counters, fetch); break;
-    default: launch_extend_t<16>(ctx, g, sc, q, count, hits, counters, fetch); break;
+    case 8: launch_extend_t<8>(ctx, g, sc, q, count, hits, counters); break;
+    case 12: launch_extend_t<12>(ctx, g, sc, q, count, hits, counters); break;
+    case 24: launch_extend_t<24>(ctx, g, sc, q, count, hits, counters); break;
+    default: launch_extend_t<16>(ctx, g, sc, q, count, hits, counters); break;
   }
 }
 
@@ -617,14 +613,13 @@ int enqueue_batch(mpt_context *ctx, mpt_scene *s, uint32_t D, bool instrument) {
     MPT_HIP(ctx, hipEventRecord(ctx->events[g_timed.back().ev + 1], ctx->stream));
     return MPT_OK;
   };
-  k_reset<<<1, 64, 0, ctx->stream>>>(w.counts.p, 3 * D + 2);
+  k_reset<<<1, 64, 0, ctx->stream>>>(w.counts.p, 2 * D + 1);
   k_raygen<<<gcap, 256, 0, ctx->stream>>>(w.desc.p, w.q[0].p, w.Ls.p, w.betas.p, w.flags.p, w.pjs.p, w.counts.p);
   MPT_LAUNCHED(ctx, "k_raygen");
   for (uint32_t d = 0; d < D; ++d) {
     float4 *qin = w.q[d & 1].p, *qout = w.q[(d + 1) & 1].p;
     if ((rc = tic(0))) return rc;
-    launch_extend(ctx, g, s->dev, qin, w.counts.p + d, w.hits.p, instrument && ctx->counters ? w.totals.p + 3 : nullptr,
-                  w.counts.p + 2 * D + 1 + d);
+    launch_extend(ctx, g, s->dev, qin, w.counts.p + d, w.hits.p, instrument && ctx->counters ? w.totals.p + 3 : nullptr);
     if ((rc = toc())) return rc;
     if ((rc = tic(1))) return rc;
     k_shade<<<g, kBlock, 0, ctx->stream>>>(s->dev, w.desc.p, d, qin, w.hits.p, w.counts.p + d, qout,
@@ -753,7 +748,7 @@ int ensure_workspace(mpt_context *ctx, uint32_t cap, uint32_t depth, uint32_t li
   MPT_HIP(ctx, w.flags.reserve(cap));
   MPT_HIP(ctx, w.pjs.reserve(cap));
   MPT_HIP(ctx, w.sq.reserve(3ull * ns));
-  MPT_HIP(ctx, w.counts.reserve(3ull * depth + 2));
+  MPT_HIP(ctx, w.counts.reserve(2ull * depth + 1));
   MPT_HIP(ctx, w.totals.reserve(8));
   MPT_HIP(ctx, w.desc.reserve(1));
   w.cap = cap;

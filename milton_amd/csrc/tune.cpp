@@ -157,7 +157,7 @@ int mpt_tune_scene(mpt_context *ctx, const mpt_scene_desc *desc, const mpt_kd_pa
   if ((rc = mpt_film_create(ctx, job->width, job->height, &film))) return rc;
   mpt_kd_params kp{};
   if (kd) kp = *kd;
-  const uint32_t start = kp.leaf_size ? std::min(16u, std::max(1u, kp.leaf_size)) : 2u;
+  const uint32_t start = kp.leaf_size ? std::min(16u, std::max(1u, kp.leaf_size)) : 4u;
 
   // Hill climb over leaf sizes 1, 2, 4, 8, 16 (and the start value) at the
   // context's engine settings; each candidate is a full tree build.

@@ -91,11 +91,12 @@ def test_film_matches_paths_and_is_deterministic(ctx):
     assert np.array_equal(film1, film2)
     L = sc.render_paths(job, 0, 40 * 30).reshape(30, 40, 9, 3)
     np.testing.assert_allclose(film1, L.mean(axis=2), rtol=1e-5, atol=1e-6)
+    saved = ctx.get_option(m.MPT_OPT_BATCH_PATHS)
     ctx.set_option(m.MPT_OPT_BATCH_PATHS, 1152)   # 128 pixels per batch
     try:
         film3, st3 = sc.render(job)
     finally:
-        ctx.set_option(m.MPT_OPT_BATCH_PATHS, 1 << 23)
+        ctx.set_option(m.MPT_OPT_BATCH_PATHS, saved)
     assert np.array_equal(film1, film3)
     assert st1["paths"] == 40 * 30 * 9 and st1["camera_rays"] == st1["paths"]
     assert st1["extension_rays"] == st3["extension_rays"]

@@ -35,6 +35,7 @@ def test_mixed_batches_match_alone(ctx, scenes):
     (room, sroom), (soup, ssoup) = scenes
     jobs = [(sroom, j) for j in _jobs(room, 10)] + [(ssoup, j) for j in _jobs(soup, 10, seed=12)]
     alone = [sc.render(j)[0] for sc, j in jobs]
+    saved = ctx.get_option(m.MPT_OPT_BATCH_PATHS)
     ctx.set_option(m.MPT_OPT_BATCH_PATHS, 1 << 16)   # force many mixed batches
     try:
         srv = m.Server(ctx)
@@ -51,7 +52,7 @@ def test_mixed_batches_match_alone(ctx, scenes):
         assert st["batches"] < alone_batches
         srv.close()
     finally:
-        ctx.set_option(m.MPT_OPT_BATCH_PATHS, 1 << 23)
+        ctx.set_option(m.MPT_OPT_BATCH_PATHS, saved)
 
 
 def test_budget_admission_and_cancel(ctx, scenes):
