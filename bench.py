@@ -105,7 +105,7 @@ def rmse_vs_reference(m, ctx, scene, job):
 
 def engine_settings(ctx, m):
     return {"batch_paths": ctx.get_option(m.MPT_OPT_BATCH_PATHS), "stack": ctx.get_option(m.MPT_OPT_STACK),
-            "persistent": ctx.get_option(m.MPT_OPT_PERSISTENT)}
+            "persistent": ctx.get_option(m.MPT_OPT_PERSISTENT), "lanes": ctx.get_option(m.MPT_OPT_LANES)}
 
 
 def system_scenes(m, ctx):
@@ -191,18 +191,29 @@ def bench_serving(m, ctx, args, scenes, capacity_jobs_s):
             "max_ms": float(lat.max())}
 
 
+def apply_engine_args(m, ctx, args):
+    if args.batch_paths:
+        ctx.set_option(m.MPT_OPT_BATCH_PATHS, args.batch_paths)
+    if args.stack:
+        ctx.set_option(m.MPT_OPT_STACK, args.stack)
+    if args.persistent >= 0:
+        ctx.set_option(m.MPT_OPT_PERSISTENT, args.persistent)
+    if args.lanes:
+        ctx.set_option(m.MPT_OPT_LANES, args.lanes)
+
+
 def main_systems(args):
     import milton_amd as m
     ctx = m.Context(0)
-    if args.batch_paths:
-        ctx.set_option(m.MPT_OPT_BATCH_PATHS, args.batch_paths)
+    apply_engine_args(m, ctx, args)
     scenes, build = system_scenes(m, ctx)
     b = bench_batched(m, ctx, args, scenes)
     out = {"metric": "paths/s", "value": b["paths"] / b["t_batched"], "unit": "paths/s", "n_gpus": 1,
            "steps": 1, "warmup": 1, "ms_per_step": 1000 * b["t_batched"], "higher_is_better": True,
            "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
            "config": {"workload": "batched: 64 jobs over room / 1M soup / 2048^2 terrain, sides 128-1024, "
-                                  "spp 4-256, depth 4-8", "scene_build_s": build},
+                                  "spp 4-256, depth 4-8", "scene_build_s": build,
+                      "engine": engine_settings(ctx, m)},
            "batched": b, "speedup_vs_plain": b["t_plain"] / b["t_batched"],
            "plain_paths_per_s": b["paths"] / b["t_plain"]}
     if args.workload == "serving":
@@ -221,7 +232,9 @@ def main():
     ap.add_argument("--no-rmse", action="store_true", help="skip the RMSE against a 65,536-spp reference")
     ap.add_argument("--batch-paths", type=int, default=0)
     ap.add_argument("--stack", type=int, default=0, help="traversal short-stack entries (8, 12, 16, 24)")
-    ap.add_argument("--persistent", type=int, default=-1, help="1/0: persistent closest-hit waves on/off")
+    ap.add_argument("--persistent", type=int, default=-1,
+                    help="closest-hit kernel form: 0 grid-stride, 1 persistent waves, 2 one ray per lane")
+    ap.add_argument("--lanes", type=int, default=0, help="engine lanes (streams) batches alternate over")
     ap.add_argument("--kd-leaf", type=int, default=0, help="kd leaf size (0: library default)")
     ap.add_argument("--kd-ci", type=float, default=0.0, help="kd SAH intersection cost (0: default)")
     ap.add_argument("--kd-eb", type=float, default=0.0, help="kd SAH empty-side factor (0: default)")
@@ -244,12 +257,7 @@ def main():
 
     import milton_amd as m
     ctx = m.Context(0)
-    if args.batch_paths:
-        ctx.set_option(m.MPT_OPT_BATCH_PATHS, args.batch_paths)
-    if args.stack:
-        ctx.set_option(m.MPT_OPT_STACK, args.stack)
-    if args.persistent >= 0:
-        ctx.set_option(m.MPT_OPT_PERSISTENT, args.persistent)
+    apply_engine_args(m, ctx, args)
     if args.copy_calibration:
         copy_calibration()
     t0 = time.perf_counter()

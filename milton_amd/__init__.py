@@ -22,6 +22,7 @@ MPT_MISS = 0xFFFFFFFF
 MPT_BSDF_DIFFUSE, MPT_BSDF_PHONG, MPT_BSDF_DIELECTRIC = 0, 1, 2
 MPT_SCENE_ROOM, MPT_SCENE_SOUP, MPT_SCENE_TERRAIN = 1, 2, 3
 MPT_OPT_BATCH_PATHS, MPT_OPT_TIMING, MPT_OPT_COUNTERS, MPT_OPT_GRAPHS, MPT_OPT_STACK, MPT_OPT_PERSISTENT = 1, 2, 3, 4, 5, 6
+MPT_OPT_LANES = 7
 ERRORS = {-1: "invalid", -2: "device", -3: "nomem", -4: "budget", -5: "notfound", -6: "cancelled"}
 
 _f32p = ctypes.POINTER(ctypes.c_float)

@@ -46,9 +46,7 @@ struct Scene {
   const uint2 *nodes;        // kd nodes (kdbuild.h)
   const float4 *recs;        // 3 float4 per leaf reference
   float box_lo[3], box_hi[3];
-  const uint4 *tris;         // (a, b, c, material)
-  const float4 *verts;       // xyz
-  const float4 *vnormals;    // xyz
+  const float4 *tshade;      // 3 per triangle: (n0, material bits), (n1, 0), (n2, 0) vertex normals
   const float4 *spheres;     // (centre, r)
   const uint32_t *sphere_mat;
   const float4 *quads;       // 4 per quad: (c, area), (ex, 0), (ez, 0), (n, material bits)
@@ -123,24 +121,49 @@ MPT_HD RaySetup setup_ray(float3 o, float3 d) {
 // tri_td leaves the division to the caller: on a hit it returns the scaled
 // distance T and determinant det > 0 with t = T / det, so a leaf loop can
 // reject candidates beyond its current best without dividing.
-MPT_HD bool tri_td(const RaySetup &r, float3 v0, float3 v1, float3 v2, float &T, float &det) {
+// Edge functions U, V, W of the sheared triangle (U weights v0, V v1, W v2)
+// and the sheared z of its vertices.
+struct TriEdges {
+  float U, V, W, Az, Bz, Cz;
+};
+
+MPT_HD TriEdges tri_edges(const RaySetup &r, float3 v0, float3 v1, float3 v2) {
   const uint32_t kx = r.k & 3u, ky = (r.k >> 2) & 3u, kz = r.k >> 4;
   const float3 A = sub(v0, r.o), B = sub(v1, r.o), C = sub(v2, r.o);
-  const float Az = comp(A, kz), Bz = comp(B, kz), Cz = comp(C, kz);
-  const float Ax = comp(A, kx) - r.sx * Az, Ay = comp(A, ky) - r.sy * Az;
-  const float Bx = comp(B, kx) - r.sx * Bz, By = comp(B, ky) - r.sy * Bz;
-  const float Cx = comp(C, kx) - r.sx * Cz, Cy = comp(C, ky) - r.sy * Cz;
-  const float U = Cx * By - Cy * Bx;
-  const float V = Ax * Cy - Ay * Cx;
-  const float W = Bx * Ay - By * Ax;
+  TriEdges e;
+  e.Az = comp(A, kz);
+  e.Bz = comp(B, kz);
+  e.Cz = comp(C, kz);
+  const float Ax = comp(A, kx) - r.sx * e.Az, Ay = comp(A, ky) - r.sy * e.Az;
+  const float Bx = comp(B, kx) - r.sx * e.Bz, By = comp(B, ky) - r.sy * e.Bz;
+  const float Cx = comp(C, kx) - r.sx * e.Cz, Cy = comp(C, ky) - r.sy * e.Cz;
+  e.U = Cx * By - Cy * Bx;
+  e.V = Ax * Cy - Ay * Cx;
+  e.W = Bx * Ay - By * Ax;
+  return e;
+}
+
+MPT_HD bool tri_td(const RaySetup &r, float3 v0, float3 v1, float3 v2, float &T, float &det) {
+  const TriEdges e = tri_edges(r, v0, v1, v2);
+  const float U = e.U, V = e.V, W = e.W;
   if ((U < 0.0f || V < 0.0f || W < 0.0f) && (U > 0.0f || V > 0.0f || W > 0.0f)) return false;
   const float d = U + V + W;
   if (fabsf(d) <= kEps * fabsf(r.sz)) return false;
-  const float t = (U * Az + V * Bz + W * Cz) * r.sz;
+  const float t = (U * e.Az + V * e.Bz + W * e.Cz) * r.sz;
   // (-T) / (-det) == T / det exactly, so the sign moves onto T
   T = d < 0.0f ? -t : t;
   det = fabsf(d);
   return true;
+}
+
+// Barycentric weights of v1 and v2 at the ray's hit on the triangle (the
+// weight of v0 is 1 minus both).  They equal the reference's ratios of
+// unsigned sub-triangle areas at the hit point (MeshTriangle.cpp:344-361)
+// for a point on the triangle.
+MPT_HD float2 tri_bary(const RaySetup &r, float3 v0, float3 v1, float3 v2) {
+  const TriEdges e = tri_edges(r, v0, v1, v2);
+  const float inv = 1.0f / (e.U + e.V + e.W);
+  return make_float2(e.V * inv, e.W * inv);
 }
 
 MPT_HD float tri_t(const RaySetup &r, float3 v0, float3 v1, float3 v2) {
@@ -336,15 +359,23 @@ struct Walker {
   }
 
   // Result of a finished walk: the closest hit's t is recomputed in double.
-  MPT_HD bool result(const Scene &sc, const RaySetup &r, float &t_out, uint32_t &id_out) const {
+  // With uv, a triangle hit also reports its barycentric weights of v1, v2
+  // (tri_bary); other hits report (0, 0).
+  MPT_HD bool result(const Scene &sc, const RaySetup &r, float &t_out, uint32_t &id_out,
+                     float2 *uv = nullptr) const {
     if (best_rec == kMiss) {
       t_out = INFINITY;
       id_out = kMiss;
+      if (uv) *uv = make_float2(0.0f, 0.0f);
       return false;
     }
     const float4 *rec = sc.recs + 3ull * best_rec;
     t_out = ANY ? best : refine_t(rec, r, best);
     id_out = mpt_f2u(rec[0].w);
+    if (uv) {
+      const float4 a = rec[0], b = rec[1], c = rec[2];
+      *uv = mpt_f2u(b.w) == KIND_TRIANGLE ? tri_bary(r, xyz(a), xyz(b), xyz(c)) : make_float2(0.0f, 0.0f);
+    }
     return true;
   }
 };
@@ -352,16 +383,18 @@ struct Walker {
 // kd traversal of one ray to completion (ANY: first hit below tlimit).
 template <bool ANY, bool COUNT = false, int STACK = kStack>
 MPT_HD bool trace(const Scene &sc, const RaySetup &r, float tlimit, uint32_t skip_origin, uint2 *stk,
-                  float &t_out, uint32_t &id_out, uint32_t *n_nodes = nullptr, uint32_t *n_refs = nullptr) {
+                  float &t_out, uint32_t &id_out, uint32_t *n_nodes = nullptr, uint32_t *n_refs = nullptr,
+                  float2 *uv = nullptr) {
   Walker<ANY, STACK> w;
   if (!w.start(sc, r, tlimit, skip_origin)) {
     t_out = INFINITY;
     id_out = kMiss;
+    if (uv) *uv = make_float2(0.0f, 0.0f);
     return false;
   }
   while (!w.template step<COUNT>(sc, r, stk, n_nodes, n_refs)) {
   }
-  return w.result(sc, r, t_out, id_out);
+  return w.result(sc, r, t_out, id_out, uv);
 }
 
 // ---------------------------------------------------------------------------

@@ -86,7 +86,7 @@ struct Workspace {
   uint32_t cap = 0, depth = 0, lights = 0;
   uint64_t generation = 0;   // bumped on every reallocation (invalidates graphs)
   DBuf<float4> q[2];
-  DBuf<float2> hits;
+  DBuf<float4> hits;         // (t, prim, barycentric weights of v1 and v2)
   DBuf<float4> sq;
   DBuf<float4> Ls, betas;
   DBuf<uint32_t> flags;
@@ -100,9 +100,26 @@ struct Workspace {
   }
 };
 
+// An engine lane: a stream with its own workspace and descriptor staging
+// ring.  Batches on different lanes are independent (disjoint film pixels,
+// separate path state), so a context alternates its batches over its lanes:
+// the tail of one batch's launches (the last waves of a kernel) overlaps the
+// bulk of another's.
+struct Lane {
+  hipStream_t stream = nullptr;
+  Workspace ws;
+  BatchDesc *staging = nullptr;   // pinned ring of descriptors
+  hipEvent_t staging_ev[4] = {nullptr, nullptr, nullptr, nullptr};
+  uint32_t staging_next = 0;
+  hipEvent_t mark = nullptr;      // fork / join point
+};
+
+constexpr uint32_t kMaxLanes = 4;
+
 // One captured batch sequence.
 struct GraphEntry {
   const void *scene = nullptr;
+  uint32_t lane = 0;
   uint32_t depth = 0, cap = 0, lights = 0;
   uint64_t generation = 0;
   hipGraphExec_t exec = nullptr;
@@ -125,7 +142,10 @@ struct mpt_context {
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   int num_cus = 256;
   std::string err;
-  mpt_engine::Workspace ws;
+  mpt_engine::Lane lanes[mpt_engine::kMaxLanes];
+  uint32_t num_lanes = 2;            // lanes batches alternate over
+  uint32_t next_lane = 0;
+  hipEvent_t mark = nullptr;         // fork point on the context stream
   uint32_t batch_paths = 1u << 24;   // paths per batch (queue depth)
   bool timing = false;
   bool counters = false;
@@ -138,25 +158,21 @@ struct mpt_context {
   std::vector<mpt_film *> films;
   std::vector<mpt_server *> servers;
   std::vector<mpt_engine::GraphEntry> graphs_cache;
-  mpt_engine::BatchDesc *staging = nullptr;   // pinned ring of descriptors
-  hipEvent_t staging_ev[4] = {nullptr, nullptr, nullptr, nullptr};
-  uint32_t staging_next = 0;
 };
 
 struct mpt_scene {
   mpt_context *ctx = nullptr;
   uint64_t uid = 0;   // unique per process, keys the graph cache
   mpt_engine::DBuf<uint2> nodes;
-  mpt_engine::DBuf<float4> recs, verts, vnormals, spheres, quads;
-  mpt_engine::DBuf<uint4> tris;
+  mpt_engine::DBuf<float4> recs, tshade, spheres, quads;
   mpt_engine::DBuf<uint32_t> sphere_mat, lights;
   mpt_engine::DBuf<mpt_material> mats;
   mpt_dev::Scene dev{};
   uint64_t stats[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   uint32_t leaf_size = 0;   // kd leaf size the tree was built with
   uint64_t device_bytes() const {
-    return nodes.bytes() + recs.bytes() + verts.bytes() + vnormals.bytes() + spheres.bytes() + quads.bytes() +
-           tris.bytes() + sphere_mat.bytes() + lights.bytes() + mats.bytes();
+    return nodes.bytes() + recs.bytes() + tshade.bytes() + spheres.bytes() + quads.bytes() + sphere_mat.bytes() +
+           lights.bytes() + mats.bytes();
   }
 };
 
@@ -172,17 +188,28 @@ int fail(mpt_context *ctx, int code, const std::string &msg);
 int check_job(mpt_context *ctx, const mpt_job *job);
 JobDev make_job(const mpt_job &job, float4 *film);
 
-// Makes sure the workspace holds a batch of `cap` paths, `depth` bounces and
-// `lights` shadow rays per vertex.
-int ensure_workspace(mpt_context *ctx, uint32_t cap, uint32_t depth, uint32_t lights);
+// Creates lane k's stream, events and staging ring if it has none yet.
+int ensure_lane(mpt_context *ctx, uint32_t k);
+// Makes sure lane k's workspace holds a batch of `cap` paths, `depth`
+// bounces and `lights` shadow rays per vertex.
+int ensure_workspace(mpt_context *ctx, uint32_t k, uint32_t cap, uint32_t depth, uint32_t lights);
 
-// Enqueues one batch on the context stream (asynchronous).  With L_host the
+// Enqueues one batch on lane `lane` (asynchronous).  With L_host the
 // per-path radiance of the batch is copied to L_host (slot order) and the
 // call synchronises.  Uses a captured graph when the context allows it and
 // no per-launch instrumentation is on.
-int run_batch(mpt_context *ctx, mpt_scene *scene, const BatchPlan &plan, float *L_host);
+int run_batch(mpt_context *ctx, mpt_scene *scene, const BatchPlan &plan, float *L_host, uint32_t lane);
+// The lane the next batch goes to (round robin over num_lanes).
+uint32_t pick_lane(mpt_context *ctx);
+// Orders every lane after the work queued so far on the context stream, and
+// the context stream after the work queued so far on every lane.
+int fork_lanes(mpt_context *ctx);
+int join_lanes(mpt_context *ctx);
+// Orders lane k after the work queued so far on the context stream.
+int lane_follow(mpt_context *ctx, uint32_t k);
 
-// Zeroes the device ray totals; read_totals syncs and returns
+// Zeroes the device ray totals of every lane (on the context stream, before
+// a fork); read_totals syncs and returns their sums:
 // [camera, extension, shadow, nodes, refs].
 int reset_totals(mpt_context *ctx);
 // Renders pixels [pix_begin, pix_end) of one job into film (accumulating) or

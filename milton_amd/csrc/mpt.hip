@@ -128,7 +128,7 @@ __global__ __launch_bounds__(256) void k_raygen(const BatchDesc *__restrict__ bd
 // the kernel also accumulates the kd nodes and references it visited.
 template <bool COUNT, int STACK>
 __global__ __launch_bounds__(256) void k_extend(Scene sc, const float4 *__restrict__ q,
-                                                const uint32_t *__restrict__ count, float2 *__restrict__ hits,
+                                                const uint32_t *__restrict__ count, float4 *__restrict__ hits,
                                                 unsigned long long *__restrict__ counters) {
   __shared__ uint2 stack[STACK * kBlock];
   uint2 *stk = stack + threadIdx.x;
@@ -139,8 +139,9 @@ __global__ __launch_bounds__(256) void k_extend(Scene sc, const float4 *__restri
     const RaySetup r = setup_ray(xyz(a), xyz(b));
     float t;
     uint32_t id;
-    trace<false, COUNT, STACK>(sc, r, INFINITY, __float_as_uint(b.w), stk, t, id, &n_nodes, &n_refs);
-    hits[i] = make_float2(t, __uint_as_float(id));
+    float2 uv;
+    trace<false, COUNT, STACK>(sc, r, INFINITY, __float_as_uint(b.w), stk, t, id, &n_nodes, &n_refs, &uv);
+    hits[i] = make_float4(t, __uint_as_float(id), uv.x, uv.y);
   }
   if (COUNT) {
     atomicAdd(&counters[0], (unsigned long long)n_nodes);
@@ -158,7 +159,7 @@ constexpr uint32_t kRefill = 16;
 
 template <bool COUNT, int STACK>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void k_extend_persistent(
-    Scene sc, const float4 *__restrict__ q, const uint32_t *__restrict__ count, float2 *__restrict__ hits,
+    Scene sc, const float4 *__restrict__ q, const uint32_t *__restrict__ count, float4 *__restrict__ hits,
     unsigned long long *__restrict__ counters) {
   __shared__ uint2 stack[STACK * kBlock];
   uint2 *stk = stack + threadIdx.x;
@@ -186,7 +187,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void k
         if (w.start(sc, r, INFINITY, __float_as_uint(b.w))) {
           ray = idx;
         } else {
-          hits[idx] = make_float2(INFINITY, __uint_as_float(kMiss));
+          hits[idx] = make_float4(INFINITY, __uint_as_float(kMiss), 0.0f, 0.0f);
         }
       }
     }
@@ -197,8 +198,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void k
     if (ray != kMiss && w.template step<COUNT>(sc, r, stk, &n_nodes, &n_refs)) {
       float t;
       uint32_t id;
-      w.result(sc, r, t, id);
-      hits[ray] = make_float2(t, __uint_as_float(id));
+      float2 uv;
+      w.result(sc, r, t, id, &uv);
+      hits[ray] = make_float4(t, __uint_as_float(id), uv.x, uv.y);
       ray = kMiss;
     }
   }
@@ -240,20 +242,17 @@ struct HitInfo {
 // MeshTriangle::getBaryocentricCoords (shapes/Mesh.cpp:419-433,
 // shapes/MeshTriangle.cpp:344-361), Sphere / Plane normals (shapes/Sphere.cpp:
 // 86-92, shapes/Plane.cpp:59-63).  The normal is not flipped toward the ray.
-__device__ __forceinline__ HitInfo hit_info(const Scene &sc, uint32_t prim, float3 p) {
+__device__ __forceinline__ HitInfo hit_info(const Scene &sc, uint32_t prim, float3 p, float2 uv) {
   HitInfo h;
   h.area = 0.0f;
   if (prim < sc.nt) {
-    const uint4 tri = sc.tris[prim];
-    const float3 v1 = xyz(sc.verts[tri.x]), v2 = xyz(sc.verts[tri.y]), v3 = xyz(sc.verts[tri.z]);
-    const float inv_a = 1.0f / len(cross(sub(v2, v1), sub(v3, v1)));
-    const float pA = len(cross(sub(v2, p), sub(v3, p))) * inv_a;
-    const float pB = len(cross(sub(p, v1), sub(v3, v1))) * inv_a;
-    const float pC = 1.0f - pA - pB;
-    const float3 n = add(add(mul(xyz(sc.vnormals[tri.x]), pA), mul(xyz(sc.vnormals[tri.y]), pB)),
-                         mul(xyz(sc.vnormals[tri.z]), pC));
-    h.n = normalize(n);
-    h.mat = tri.w;
+    // Mesh::_getGeometricNormal (shapes/Mesh.cpp:419-433): the vertex normals
+    // weighted by the hit's barycentric coordinates
+    const float4 *ts = sc.tshade + 3ull * prim;
+    const float4 n0 = ts[0], n1 = ts[1], n2 = ts[2];
+    const float w0 = 1.0f - uv.x - uv.y;
+    h.n = normalize(add(add(mul(xyz(n0), w0), mul(xyz(n1), uv.x)), mul(xyz(n2), uv.y)));
+    h.mat = __float_as_uint(n0.w);
   } else if (prim < sc.nt + sc.ns) {
     const float4 s = sc.spheres[prim - sc.nt];
     h.n = normalize(sub(p, xyz(s)));
@@ -274,7 +273,7 @@ __device__ __forceinline__ HitInfo hit_info(const Scene &sc, uint32_t prim, floa
 // 46-136).  The recursion becomes a throughput `beta`; the emission terms, NEE
 // and roulette follow the reference rules, each cited below.
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k_shade(Scene sc, const BatchDesc *__restrict__ bd, uint32_t depth,
-                                               const float4 *__restrict__ qin, const float2 *__restrict__ hits,
+                                               const float4 *__restrict__ qin, const float4 *__restrict__ hits,
                                                const uint32_t *__restrict__ count_in, float4 *__restrict__ qout,
                                                uint32_t *__restrict__ count_out, float4 *__restrict__ sq,
                                                uint32_t *__restrict__ count_shadow, float4 *__restrict__ Ls,
@@ -301,7 +300,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k
     uint32_t pixel = 0, sample = 0, seed = 0, job_id = 0, max_depth = 0;
     if (active) {
       const float4 a = qin[2 * i], b = qin[2 * i + 1];
-      const float2 h = hits[i];
+      const float4 h = hits[i];
       slot = __float_as_uint(a.w);
       d = xyz(b);
       const uint2 pj = pjs[slot];
@@ -325,7 +324,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k
         surface = true;
         thit = h.x;
         p = add(xyz(a), mul(d, h.x));
-        hi = hit_info(sc, prim, p);
+        hi = hit_info(sc, prim, p, make_float2(h.z, h.w));
       }
     }
     mpt_material m;
@@ -548,49 +547,50 @@ std::vector<Timed> g_timed;   // guarded by the single-threaded use of a context
 
 
 template <int STACK>
-void launch_extend_t(mpt_context *ctx, uint32_t g, const Scene &sc, const float4 *q, const uint32_t *count,
-                     float2 *hits, unsigned long long *counters) {
+void launch_extend_t(mpt_context *ctx, hipStream_t st, uint32_t cap, uint32_t g, const Scene &sc, const float4 *q,
+                     const uint32_t *count, float4 *hits, unsigned long long *counters) {
   if (ctx->persistent == 1) {
     if (counters)
-      k_extend_persistent<true, STACK><<<g, kBlock, 0, ctx->stream>>>(sc, q, count, hits, counters);
+      k_extend_persistent<true, STACK><<<g, kBlock, 0, st>>>(sc, q, count, hits, counters);
     else
-      k_extend_persistent<false, STACK><<<g, kBlock, 0, ctx->stream>>>(sc, q, count, hits, nullptr);
+      k_extend_persistent<false, STACK><<<g, kBlock, 0, st>>>(sc, q, count, hits, nullptr);
     return;
   }
   // form 0: grid-stride over a fixed grid of 8 blocks per CU; form 2: one
   // ray per lane over a grid sized by the batch capacity
-  const uint32_t grid = ctx->persistent == 2 ? (uint32_t)((ctx->ws.cap + kBlock - 1) / kBlock) : g;
+  const uint32_t grid = ctx->persistent == 2 ? (uint32_t)((cap + kBlock - 1) / kBlock) : g;
   if (counters)
-    k_extend<true, STACK><<<grid, kBlock, 0, ctx->stream>>>(sc, q, count, hits, counters);
+    k_extend<true, STACK><<<grid, kBlock, 0, st>>>(sc, q, count, hits, counters);
   else
-    k_extend<false, STACK><<<grid, kBlock, 0, ctx->stream>>>(sc, q, count, hits, nullptr);
+    k_extend<false, STACK><<<grid, kBlock, 0, st>>>(sc, q, count, hits, nullptr);
 }
 
 // The short-stack depth is a run-time choice among compiled variants (it
 // trades LDS per wave, hence occupancy, against kd-restarts).
-void launch_extend(mpt_context *ctx, uint32_t g, const Scene &sc, const float4 *q, const uint32_t *count,
-                   float2 *hits, unsigned long long *counters) {
+void launch_extend(mpt_context *ctx, hipStream_t st, uint32_t cap, uint32_t g, const Scene &sc, const float4 *q,
+                   const uint32_t *count, float4 *hits, unsigned long long *counters) {
   switch (ctx->stack) {
-    case 8: launch_extend_t<8>(ctx, g, sc, q, count, hits, counters); break;
-    case 12: launch_extend_t<12>(ctx, g, sc, q, count, hits, counters); break;
-    case 24: launch_extend_t<24>(ctx, g, sc, q, count, hits, counters); break;
-    default: launch_extend_t<16>(ctx, g, sc, q, count, hits, counters); break;
+    case 8: launch_extend_t<8>(ctx, st, cap, g, sc, q, count, hits, counters); break;
+    case 12: launch_extend_t<12>(ctx, st, cap, g, sc, q, count, hits, counters); break;
+    case 24: launch_extend_t<24>(ctx, st, cap, g, sc, q, count, hits, counters); break;
+    default: launch_extend_t<16>(ctx, st, cap, g, sc, q, count, hits, counters); break;
   }
 }
 
-void launch_shadow(mpt_context *ctx, uint32_t g, const Scene &sc, const float4 *sq, const uint32_t *count,
-                   float4 *Ls) {
+void launch_shadow(mpt_context *ctx, hipStream_t st, uint32_t g, const Scene &sc, const float4 *sq,
+                   const uint32_t *count, float4 *Ls) {
   switch (ctx->stack) {
-    case 8: k_shadow<8><<<g, kBlock, 0, ctx->stream>>>(sc, sq, count, Ls); break;
-    case 12: k_shadow<12><<<g, kBlock, 0, ctx->stream>>>(sc, sq, count, Ls); break;
-    case 24: k_shadow<24><<<g, kBlock, 0, ctx->stream>>>(sc, sq, count, Ls); break;
-    default: k_shadow<16><<<g, kBlock, 0, ctx->stream>>>(sc, sq, count, Ls); break;
+    case 8: k_shadow<8><<<g, kBlock, 0, st>>>(sc, sq, count, Ls); break;
+    case 12: k_shadow<12><<<g, kBlock, 0, st>>>(sc, sq, count, Ls); break;
+    case 24: k_shadow<24><<<g, kBlock, 0, st>>>(sc, sq, count, Ls); break;
+    default: k_shadow<16><<<g, kBlock, 0, st>>>(sc, sq, count, Ls); break;
   }
 }
 
 // Enqueues the kernel sequence of one batch (the part a graph captures).
-int enqueue_batch(mpt_context *ctx, mpt_scene *s, uint32_t D, bool instrument) {
-  Workspace &w = ctx->ws;
+int enqueue_batch(mpt_context *ctx, Lane &ln, mpt_scene *s, uint32_t D, bool instrument) {
+  Workspace &w = ln.ws;
+  const hipStream_t st = ln.stream;
   const uint32_t cap = w.cap, L = s->dev.num_lights;
   const uint32_t g = grid_for(ctx, cap, 8);
   const uint32_t gcap = (cap + 255) / 256;
@@ -605,47 +605,48 @@ int enqueue_batch(mpt_context *ctx, mpt_scene *s, uint32_t D, bool instrument) {
       ctx->events.push_back(e);
     }
     g_timed.push_back({kind, used});
-    MPT_HIP(ctx, hipEventRecord(ctx->events[used], ctx->stream));
+    MPT_HIP(ctx, hipEventRecord(ctx->events[used], st));
     return MPT_OK;
   };
   auto toc = [&]() -> int {
     if (!instrument || !ctx->timing) return MPT_OK;
-    MPT_HIP(ctx, hipEventRecord(ctx->events[g_timed.back().ev + 1], ctx->stream));
+    MPT_HIP(ctx, hipEventRecord(ctx->events[g_timed.back().ev + 1], st));
     return MPT_OK;
   };
-  k_reset<<<1, 64, 0, ctx->stream>>>(w.counts.p, 2 * D + 1);
-  k_raygen<<<gcap, 256, 0, ctx->stream>>>(w.desc.p, w.q[0].p, w.Ls.p, w.betas.p, w.flags.p, w.pjs.p, w.counts.p);
+  k_reset<<<1, 64, 0, st>>>(w.counts.p, 2 * D + 1);
+  k_raygen<<<gcap, 256, 0, st>>>(w.desc.p, w.q[0].p, w.Ls.p, w.betas.p, w.flags.p, w.pjs.p, w.counts.p);
   MPT_LAUNCHED(ctx, "k_raygen");
   for (uint32_t d = 0; d < D; ++d) {
     float4 *qin = w.q[d & 1].p, *qout = w.q[(d + 1) & 1].p;
     if ((rc = tic(0))) return rc;
-    launch_extend(ctx, g, s->dev, qin, w.counts.p + d, w.hits.p, instrument && ctx->counters ? w.totals.p + 3 : nullptr);
+    launch_extend(ctx, st, cap, g, s->dev, qin, w.counts.p + d, w.hits.p,
+                  instrument && ctx->counters ? w.totals.p + 3 : nullptr);
     if ((rc = toc())) return rc;
     if ((rc = tic(1))) return rc;
-    k_shade<<<g, kBlock, 0, ctx->stream>>>(s->dev, w.desc.p, d, qin, w.hits.p, w.counts.p + d, qout,
+    k_shade<<<g, kBlock, 0, st>>>(s->dev, w.desc.p, d, qin, w.hits.p, w.counts.p + d, qout,
                                            w.counts.p + d + 1, w.sq.p, w.counts.p + D + 1 + d, w.Ls.p, w.betas.p,
                                            w.flags.p, w.pjs.p);
     if ((rc = toc())) return rc;
     if (L) {
       if ((rc = tic(2))) return rc;
-      launch_shadow(ctx, g, s->dev, w.sq.p, w.counts.p + D + 1 + d, w.Ls.p);
+      launch_shadow(ctx, st, g, s->dev, w.sq.p, w.counts.p + D + 1 + d, w.Ls.p);
       if ((rc = toc())) return rc;
     }
     MPT_LAUNCHED(ctx, "depth kernels");
   }
-  k_finalize<<<gcap, 256, 0, ctx->stream>>>(w.desc.p, w.Ls.p, w.flags.p);
-  k_film<<<gcap, 256, 0, ctx->stream>>>(w.desc.p, w.Ls.p);
-  k_totals<<<1, 64, 0, ctx->stream>>>(w.counts.p, D, w.totals.p);
+  k_finalize<<<gcap, 256, 0, st>>>(w.desc.p, w.Ls.p, w.flags.p);
+  k_film<<<gcap, 256, 0, st>>>(w.desc.p, w.Ls.p);
+  k_totals<<<1, 64, 0, st>>>(w.counts.p, D, w.totals.p);
   MPT_LAUNCHED(ctx, "film kernels");
   return MPT_OK;
 }
 
 // Copies a batch plan into the next pinned staging descriptor and enqueues
 // its upload.
-int stage_plan(mpt_context *ctx, const BatchPlan &plan) {
-  const uint32_t slot = ctx->staging_next++ & 3u;
-  MPT_HIP(ctx, hipEventSynchronize(ctx->staging_ev[slot]));
-  BatchDesc &bd = ctx->staging[slot];
+int stage_plan(mpt_context *ctx, Lane &ln, const BatchPlan &plan) {
+  const uint32_t slot = ln.staging_next++ & 3u;
+  MPT_HIP(ctx, hipEventSynchronize(ln.staging_ev[slot]));
+  BatchDesc &bd = ln.staging[slot];
   bd.hdr.n_chunks = (uint32_t)plan.chunks.size();
   bd.hdr.n_paths = plan.n_paths;
   bd.hdr.n_pixels = plan.n_pixels;
@@ -653,10 +654,10 @@ int stage_plan(mpt_context *ctx, const BatchPlan &plan) {
   std::copy(plan.jobs.begin(), plan.jobs.end(), bd.jobs);
   std::copy(plan.chunks.begin(), plan.chunks.end(), bd.chunks);
   const size_t head = offsetof(BatchDesc, jobs) + plan.jobs.size() * sizeof(JobDev);
-  MPT_HIP(ctx, hipMemcpyAsync(ctx->ws.desc.p, &bd, head, hipMemcpyHostToDevice, ctx->stream));
-  MPT_HIP(ctx, hipMemcpyAsync(reinterpret_cast<char *>(ctx->ws.desc.p) + offsetof(BatchDesc, chunks), bd.chunks,
-                              plan.chunks.size() * sizeof(ChunkDev), hipMemcpyHostToDevice, ctx->stream));
-  MPT_HIP(ctx, hipEventRecord(ctx->staging_ev[slot], ctx->stream));
+  MPT_HIP(ctx, hipMemcpyAsync(ln.ws.desc.p, &bd, head, hipMemcpyHostToDevice, ln.stream));
+  MPT_HIP(ctx, hipMemcpyAsync(reinterpret_cast<char *>(ln.ws.desc.p) + offsetof(BatchDesc, chunks), bd.chunks,
+                              plan.chunks.size() * sizeof(ChunkDev), hipMemcpyHostToDevice, ln.stream));
+  MPT_HIP(ctx, hipEventRecord(ln.staging_ev[slot], ln.stream));
   return MPT_OK;
 }
 
@@ -666,6 +667,10 @@ namespace mpt_engine {
 
 void drop_graphs(mpt_context *ctx, uint64_t scene_uid) {
   auto &c = ctx->graphs_cache;
+  if (c.empty()) return;
+  // a graph may still run on any lane
+  for (Lane &ln : ctx->lanes)
+    if (ln.stream) (void)hipStreamSynchronize(ln.stream);
   for (auto it = c.begin(); it != c.end();) {
     if (scene_uid == 0 || (uint64_t)(uintptr_t)it->scene == scene_uid) {
       (void)hipGraphExecDestroy(it->exec);
@@ -732,13 +737,40 @@ int check_job(mpt_context *ctx, const mpt_job *job) {
   return MPT_OK;
 }
 
-int ensure_workspace(mpt_context *ctx, uint32_t cap, uint32_t depth, uint32_t lights) {
-  Workspace &w = ctx->ws;
+int ensure_lane(mpt_context *ctx, uint32_t k) {
+  Lane &ln = ctx->lanes[k];
+  if (ln.stream) return MPT_OK;
+  MPT_HIP(ctx, hipStreamCreateWithFlags(&ln.stream, hipStreamNonBlocking));
+  MPT_HIP(ctx, hipEventCreateWithFlags(&ln.mark, hipEventDisableTiming));
+  MPT_HIP(ctx, hipHostMalloc((void **)&ln.staging, 4 * sizeof(BatchDesc), hipHostMallocDefault));
+  for (hipEvent_t &e : ln.staging_ev) MPT_HIP(ctx, hipEventCreateWithFlags(&e, hipEventDisableTiming));
+  return MPT_OK;
+}
+
+void release_lane(Lane &ln) {
+  if (ln.stream) (void)hipStreamSynchronize(ln.stream);
+  for (hipEvent_t &e : ln.staging_ev) {
+    if (e) (void)hipEventDestroy(e);
+    e = nullptr;
+  }
+  if (ln.staging) (void)hipHostFree(ln.staging);
+  ln.staging = nullptr;
+  if (ln.mark) (void)hipEventDestroy(ln.mark);
+  ln.mark = nullptr;
+  if (ln.stream) (void)hipStreamDestroy(ln.stream);
+  ln.stream = nullptr;
+}
+
+int ensure_workspace(mpt_context *ctx, uint32_t k, uint32_t cap, uint32_t depth, uint32_t lights) {
+  int rc = ensure_lane(ctx, k);
+  if (rc) return rc;
+  Lane &ln = ctx->lanes[k];
+  Workspace &w = ln.ws;
   cap = std::max(cap, w.cap);
   depth = std::max(depth, w.depth);
   lights = std::max(lights, w.lights);
   if (cap == w.cap && depth == w.depth && lights == w.lights && w.desc.p) return MPT_OK;
-  MPT_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  MPT_HIP(ctx, hipStreamSynchronize(ln.stream));
   const uint64_t ns = (uint64_t)cap * std::max<uint32_t>(lights, 1);
   MPT_HIP(ctx, w.q[0].reserve(2ull * cap));
   MPT_HIP(ctx, w.q[1].reserve(2ull * cap));
@@ -749,8 +781,13 @@ int ensure_workspace(mpt_context *ctx, uint32_t cap, uint32_t depth, uint32_t li
   MPT_HIP(ctx, w.pjs.reserve(cap));
   MPT_HIP(ctx, w.sq.reserve(3ull * ns));
   MPT_HIP(ctx, w.counts.reserve(2ull * depth + 1));
-  MPT_HIP(ctx, w.totals.reserve(8));
   MPT_HIP(ctx, w.desc.reserve(1));
+  if (!w.totals.p) {
+    // totals persist across reallocation: a render's counts are zeroed once
+    // before its first batch
+    MPT_HIP(ctx, w.totals.reserve(8));
+    MPT_HIP(ctx, hipMemsetAsync(w.totals.p, 0, 8 * sizeof(unsigned long long), ln.stream));
+  }
   w.cap = cap;
   w.depth = depth;
   w.lights = lights;
@@ -759,46 +796,96 @@ int ensure_workspace(mpt_context *ctx, uint32_t cap, uint32_t depth, uint32_t li
   return MPT_OK;
 }
 
+uint32_t pick_lane(mpt_context *ctx) {
+  const uint32_t n = std::max(1u, std::min(ctx->num_lanes, kMaxLanes));
+  return ctx->next_lane++ % n;
+}
+
+int fork_lanes(mpt_context *ctx) {
+  MPT_HIP(ctx, hipEventRecord(ctx->mark, ctx->stream));
+  for (uint32_t k = 0; k < kMaxLanes; ++k) {
+    if (k < ctx->num_lanes) {
+      const int rc = ensure_lane(ctx, k);
+      if (rc) return rc;
+    }
+    if (ctx->lanes[k].stream) MPT_HIP(ctx, hipStreamWaitEvent(ctx->lanes[k].stream, ctx->mark, 0));
+  }
+  return MPT_OK;
+}
+
+int lane_follow(mpt_context *ctx, uint32_t k) {
+  const int rc = ensure_lane(ctx, k);
+  if (rc) return rc;
+  MPT_HIP(ctx, hipEventRecord(ctx->mark, ctx->stream));
+  MPT_HIP(ctx, hipStreamWaitEvent(ctx->lanes[k].stream, ctx->mark, 0));
+  return MPT_OK;
+}
+
+int join_lanes(mpt_context *ctx) {
+  for (Lane &ln : ctx->lanes) {
+    if (!ln.stream) continue;
+    MPT_HIP(ctx, hipEventRecord(ln.mark, ln.stream));
+    MPT_HIP(ctx, hipStreamWaitEvent(ctx->stream, ln.mark, 0));
+  }
+  return MPT_OK;
+}
+
 int reset_totals(mpt_context *ctx) {
   g_timed.clear();
-  MPT_HIP(ctx, ctx->ws.totals.reserve(8));
-  MPT_HIP(ctx, hipMemsetAsync(ctx->ws.totals.p, 0, 8 * sizeof(unsigned long long), ctx->stream));
+  for (uint32_t k = 0; k < kMaxLanes; ++k) {
+    Lane &ln = ctx->lanes[k];
+    if (!ln.stream && k >= ctx->num_lanes) continue;
+    int rc = ensure_lane(ctx, k);
+    if (rc) return rc;
+    MPT_HIP(ctx, ln.ws.totals.reserve(8));
+    MPT_HIP(ctx, hipMemsetAsync(ln.ws.totals.p, 0, 8 * sizeof(unsigned long long), ctx->stream));
+  }
   return MPT_OK;
 }
 
 int read_totals(mpt_context *ctx, unsigned long long out[5]) {
-  unsigned long long tot[8];
-  MPT_HIP(ctx, hipMemcpyAsync(tot, ctx->ws.totals.p, sizeof(tot), hipMemcpyDeviceToHost, ctx->stream));
+  unsigned long long tot[kMaxLanes][8];
+  uint32_t used = 0;
+  for (Lane &ln : ctx->lanes) {
+    if (!ln.ws.totals.p) continue;
+    MPT_HIP(ctx, hipMemcpyAsync(tot[used++], ln.ws.totals.p, sizeof(tot[0]), hipMemcpyDeviceToHost, ctx->stream));
+  }
   MPT_HIP(ctx, hipStreamSynchronize(ctx->stream));
-  for (int i = 0; i < 5; ++i) out[i] = tot[i];
+  for (int i = 0; i < 5; ++i) {
+    out[i] = 0;
+    for (uint32_t k = 0; k < used; ++k) out[i] += tot[k][i];
+  }
   return MPT_OK;
 }
 
-int run_batch(mpt_context *ctx, mpt_scene *s, const BatchPlan &plan, float *L_host) {
+int run_batch(mpt_context *ctx, mpt_scene *s, const BatchPlan &plan, float *L_host, uint32_t lane) {
   if (plan.chunks.empty() || plan.n_paths == 0) return MPT_OK;
   if (plan.jobs.size() > kMaxJobs || plan.chunks.size() > kMaxChunks)
     return fail(ctx, MPT_E_INVALID, "batch plan exceeds the job or chunk table");
-  int rc = ensure_workspace(ctx, std::max(plan.n_paths, ctx->batch_paths), plan.max_depth, s->dev.num_lights);
+  if (lane >= kMaxLanes) return fail(ctx, MPT_E_INVALID, "bad lane");
+  int rc = ensure_workspace(ctx, lane, std::max(plan.n_paths, ctx->batch_paths), plan.max_depth,
+                            s->dev.num_lights);
   if (rc) return rc;
-  if ((rc = stage_plan(ctx, plan))) return rc;
+  Lane &ln = ctx->lanes[lane];
+  if ((rc = stage_plan(ctx, ln, plan))) return rc;
   const uint32_t D = plan.max_depth;
   const bool instrument = ctx->timing || ctx->counters;
   if (ctx->graphs && !instrument) {
-    Workspace &w = ctx->ws;
+    Workspace &w = ln.ws;
     hipGraphExec_t exec = nullptr;
     const void *key = reinterpret_cast<const void *>((uintptr_t)s->uid);
     for (const GraphEntry &e : ctx->graphs_cache)
-      if (e.scene == key && e.depth == D && e.cap == w.cap && e.lights == s->dev.num_lights &&
+      if (e.scene == key && e.lane == lane && e.depth == D && e.cap == w.cap && e.lights == s->dev.num_lights &&
           e.generation == w.generation) {
         exec = e.exec;
         break;
       }
     if (!exec) {
       hipGraph_t graph = nullptr;
-      MPT_HIP(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
-      rc = enqueue_batch(ctx, s, D, false);
+      MPT_HIP(ctx, hipStreamBeginCapture(ln.stream, hipStreamCaptureModeThreadLocal));
+      rc = enqueue_batch(ctx, ln, s, D, false);
       hipGraph_t captured = nullptr;
-      const hipError_t ec = hipStreamEndCapture(ctx->stream, &captured);
+      const hipError_t ec = hipStreamEndCapture(ln.stream, &captured);
       if (rc) {
         if (captured) (void)hipGraphDestroy(captured);
         return rc;
@@ -810,6 +897,7 @@ int run_batch(mpt_context *ctx, mpt_scene *s, const BatchPlan &plan, float *L_ho
       if (ei != hipSuccess) return hip_fail(ctx, ei, "hipGraphInstantiate");
       GraphEntry e;
       e.scene = key;
+      e.lane = lane;
       e.depth = D;
       e.cap = w.cap;
       e.lights = s->dev.num_lights;
@@ -817,14 +905,14 @@ int run_batch(mpt_context *ctx, mpt_scene *s, const BatchPlan &plan, float *L_ho
       e.exec = exec;
       ctx->graphs_cache.push_back(e);
     }
-    MPT_HIP(ctx, hipGraphLaunch(exec, ctx->stream));
+    MPT_HIP(ctx, hipGraphLaunch(exec, ln.stream));
   } else {
-    if ((rc = enqueue_batch(ctx, s, D, true))) return rc;
+    if ((rc = enqueue_batch(ctx, ln, s, D, true))) return rc;
   }
   if (L_host) {
-    MPT_HIP(ctx, hipMemcpy2DAsync(L_host, 3 * sizeof(float), ctx->ws.Ls.p, sizeof(float4), 3 * sizeof(float),
-                                  plan.n_paths, hipMemcpyDeviceToHost, ctx->stream));
-    MPT_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    MPT_HIP(ctx, hipMemcpy2DAsync(L_host, 3 * sizeof(float), ln.ws.Ls.p, sizeof(float4), 3 * sizeof(float),
+                                  plan.n_paths, hipMemcpyDeviceToHost, ln.stream));
+    MPT_HIP(ctx, hipStreamSynchronize(ln.stream));
   }
   return MPT_OK;
 }
@@ -840,6 +928,7 @@ int render_range(mpt_scene *s, const mpt_job *job, uint32_t pix_begin, uint32_t 
   const uint32_t batch_pix = std::max<uint32_t>(1, ctx->batch_paths / spp);
   if ((rc = reset_totals(ctx))) return rc;
   MPT_HIP(ctx, hipEventRecord(ctx->ev0, ctx->stream));
+  if ((rc = fork_lanes(ctx))) return rc;
   BatchPlan plan;
   plan.jobs.push_back(make_job(*job, film));
   plan.max_depth = job->max_depth;
@@ -849,10 +938,12 @@ int render_range(mpt_scene *s, const mpt_job *job, uint32_t pix_begin, uint32_t 
     plan.chunks.assign(1, ChunkDev{0, p0, np, 0, 0, {0, 0, 0}});
     plan.n_paths = np * spp;
     plan.n_pixels = np;
-    if ((rc = run_batch(ctx, s, plan, L_host ? L_host + 3ull * (uint64_t)(p0 - pix_begin) * spp : nullptr)))
+    if ((rc = run_batch(ctx, s, plan, L_host ? L_host + 3ull * (uint64_t)(p0 - pix_begin) * spp : nullptr,
+                        pick_lane(ctx))))
       return rc;
     ++batches;
   }
+  if ((rc = join_lanes(ctx))) return rc;
   MPT_HIP(ctx, hipEventRecord(ctx->ev1, ctx->stream));
   unsigned long long tot[5];
   if ((rc = read_totals(ctx, tot))) return rc;
@@ -885,6 +976,7 @@ namespace {
 // build (used by mpt_scene_create and by the host-only kd entry points).
 struct HostScene {
   std::vector<float4> verts, vnorm, sph, quads;
+  std::vector<float4> tshade;   // device shading records, 3 per triangle
   std::vector<uint4> tris;
   std::vector<uint32_t> lights, sphere_mat;
   std::vector<mpt_material> mats;
@@ -947,6 +1039,16 @@ int prepare_scene(mpt_context *ctx, const mpt_scene_desc *d, const mpt_kd_params
   tris.resize(nt);
   for (uint32_t t = 0; t < nt; ++t)
     tris[t] = make_uint4(d->indices[3 * t], d->indices[3 * t + 1], d->indices[3 * t + 2], d->tri_material[t]);
+  hs.tshade.resize(3ull * nt);
+  for (uint32_t t = 0; t < nt; ++t) {
+    const float4 n0 = vnorm[tris[t].x], n1 = vnorm[tris[t].y], n2 = vnorm[tris[t].z];
+    uint32_t mbits = tris[t].w;
+    float mf;
+    std::memcpy(&mf, &mbits, 4);
+    hs.tshade[3ull * t] = make_float4(n0.x, n0.y, n0.z, mf);
+    hs.tshade[3ull * t + 1] = make_float4(n1.x, n1.y, n1.z, 0);
+    hs.tshade[3ull * t + 2] = make_float4(n2.x, n2.y, n2.z, 0);
+  }
   std::vector<float4> &sph = hs.sph;
   sph.resize(ns);
   for (uint32_t i = 0; i < ns; ++i)
@@ -1056,8 +1158,8 @@ int prepare_scene(mpt_context *ctx, const mpt_scene_desc *d, const mpt_kd_params
 }
 
 // Points a device-layout Scene at host (or device) arrays.
-void bind_scene(Scene &sc, const uint2 *nodes, const float4 *recs, const mpt_kd::Box &box, const uint4 *tris,
-                const float4 *verts, const float4 *vnorm, const float4 *sph, const uint32_t *sph_mat,
+void bind_scene(Scene &sc, const uint2 *nodes, const float4 *recs, const mpt_kd::Box &box, const float4 *tshade,
+                const float4 *sph, const uint32_t *sph_mat,
                 const float4 *quads, const mpt_material *mats, const uint32_t *lights, uint32_t nl, uint32_t nt,
                 uint32_t ns, uint32_t nq, const float *env) {
   sc.nodes = nodes;
@@ -1066,9 +1168,7 @@ void bind_scene(Scene &sc, const uint2 *nodes, const float4 *recs, const mpt_kd:
     sc.box_lo[k] = box.lo[k];
     sc.box_hi[k] = box.hi[k];
   }
-  sc.tris = tris;
-  sc.verts = verts;
-  sc.vnormals = vnorm;
+  sc.tshade = tshade;
   sc.spheres = sph;
   sc.sphere_mat = sph_mat;
   sc.quads = quads;
@@ -1106,10 +1206,9 @@ int mpt_open(int device, mpt_context **out) {
     ctx->num_cus = prop.multiProcessorCount;
   if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) return MPT_E_DEVICE;
   if (hipEventCreate(&ctx->ev0) != hipSuccess || hipEventCreate(&ctx->ev1) != hipSuccess) return MPT_E_DEVICE;
-  if (hipHostMalloc((void **)&ctx->staging, 4 * sizeof(BatchDesc), hipHostMallocDefault) != hipSuccess)
-    return MPT_E_NOMEM;
-  for (hipEvent_t &e : ctx->staging_ev)
-    if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) return MPT_E_DEVICE;
+  if (hipEventCreateWithFlags(&ctx->mark, hipEventDisableTiming) != hipSuccess) return MPT_E_DEVICE;
+  for (uint32_t k = 0; k < ctx->num_lanes; ++k)
+    if (ensure_lane(ctx.get(), k)) return MPT_E_DEVICE;
   *out = ctx.release();
   return MPT_OK;
 }
@@ -1122,10 +1221,8 @@ int mpt_close(mpt_context *ctx) {
   while (!ctx->scenes.empty()) mpt_scene_destroy(ctx->scenes.back());
   while (!ctx->films.empty()) mpt_film_destroy(ctx->films.back());
   drop_graphs(ctx, 0);
-  for (hipEvent_t e : ctx->staging_ev)
-    if (e) (void)hipEventDestroy(e);
-  if (ctx->staging) (void)hipHostFree(ctx->staging);
-
+  for (Lane &ln : ctx->lanes) release_lane(ln);
+  if (ctx->mark) (void)hipEventDestroy(ctx->mark);
   if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
   if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
   for (hipEvent_t e : ctx->events) (void)hipEventDestroy(e);
@@ -1157,6 +1254,11 @@ int mpt_set_option(mpt_context *ctx, uint32_t option, uint64_t value) {
       ctx->persistent = (uint32_t)value;
       drop_graphs(ctx, 0);
       return MPT_OK;
+    case MPT_OPT_LANES:
+      if (value < 1 || value > kMaxLanes) return fail(ctx, MPT_E_INVALID, "lanes must be 1..4");
+      ctx->num_lanes = (uint32_t)value;
+      ctx->next_lane = 0;
+      return MPT_OK;
     case MPT_OPT_STACK:
       if (value != 8 && value != 12 && value != 16 && value != 24)
         return fail(ctx, MPT_E_INVALID, "stack depth must be 8, 12, 16 or 24");
@@ -1177,6 +1279,7 @@ int mpt_get_option(mpt_context *ctx, uint32_t option, uint64_t *value) {
     case MPT_OPT_GRAPHS: *value = ctx->graphs; return MPT_OK;
     case MPT_OPT_STACK: *value = ctx->stack; return MPT_OK;
     case MPT_OPT_PERSISTENT: *value = ctx->persistent; return MPT_OK;
+    case MPT_OPT_LANES: *value = ctx->num_lanes; return MPT_OK;
     default: return fail(ctx, MPT_E_INVALID, "unknown option");
   }
 }
@@ -1201,16 +1304,14 @@ int mpt_scene_create(mpt_context *ctx, const mpt_scene_desc *d, const mpt_kd_par
   const uint32_t nt = hs.nt, ns = hs.ns, nq = hs.nq;
   if ((rc = upload(ctx, s->nodes, reinterpret_cast<const uint2 *>(ko.nodes.data()), ko.nodes.size() / 2))) return rc;
   if ((rc = upload(ctx, s->recs, reinterpret_cast<const float4 *>(ko.records.data()), ko.records.size() / 4))) return rc;
-  if ((rc = upload(ctx, s->tris, hs.tris.data(), hs.tris.size()))) return rc;
-  if ((rc = upload(ctx, s->verts, hs.verts.data(), hs.verts.size()))) return rc;
-  if ((rc = upload(ctx, s->vnormals, hs.vnorm.data(), hs.vnorm.size()))) return rc;
+  if ((rc = upload(ctx, s->tshade, hs.tshade.data(), hs.tshade.size()))) return rc;
   if ((rc = upload(ctx, s->spheres, hs.sph.data(), hs.sph.size()))) return rc;
   if ((rc = upload(ctx, s->sphere_mat, hs.sphere_mat.data(), hs.sphere_mat.size()))) return rc;
   if ((rc = upload(ctx, s->quads, hs.quads.data(), hs.quads.size()))) return rc;
   if ((rc = upload(ctx, s->mats, hs.mats.data(), hs.mats.size()))) return rc;
   if ((rc = upload(ctx, s->lights, hs.lights.data(), hs.lights.size()))) return rc;
   MPT_HIP(ctx, hipStreamSynchronize(ctx->stream));
-  bind_scene(s->dev, s->nodes.p, s->recs.p, ko.box, s->tris.p, s->verts.p, s->vnormals.p, s->spheres.p,
+  bind_scene(s->dev, s->nodes.p, s->recs.p, ko.box, s->tshade.p, s->spheres.p,
              s->sphere_mat.p, s->quads.p, s->mats.p, s->lights.p, (uint32_t)hs.lights.size(), nt, ns, nq, hs.env);
   s->stats[0] = ko.n_nodes;
   s->stats[1] = ko.n_leaves;
@@ -1241,8 +1342,8 @@ int mpt_kd_build(const mpt_scene_desc *d, const mpt_kd_params *kd, mpt_kd_tree *
   if (rc) return rc;
   HostScene &hs = t->hs;
   bind_scene(t->sc, reinterpret_cast<const uint2 *>(hs.ko.nodes.data()),
-             reinterpret_cast<const float4 *>(hs.ko.records.data()), hs.ko.box, hs.tris.data(), hs.verts.data(),
-             hs.vnorm.data(), hs.sph.data(), hs.sphere_mat.data(), hs.quads.data(), hs.mats.data(),
+             reinterpret_cast<const float4 *>(hs.ko.records.data()), hs.ko.box, hs.tshade.data(),
+             hs.sph.data(), hs.sphere_mat.data(), hs.quads.data(), hs.mats.data(),
              hs.lights.data(), (uint32_t)hs.lights.size(), hs.nt, hs.ns, hs.nq, hs.env);
   *out = t.release();
   return MPT_OK;

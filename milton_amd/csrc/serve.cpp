@@ -205,7 +205,11 @@ int round(mpt_server *sv, std::unique_lock<std::mutex> &lk, uint32_t *ran) {
   }
   if (plan.chunks.empty()) return MPT_OK;
   while (sv->inflight.size() >= sv->cfg.max_inflight) retire_one(sv, lk);
-  int rc = run_batch(ctx, scene, plan, nullptr);
+  // Batches alternate over the context's lanes; a lane first waits for the
+  // context stream, where new films are cleared.
+  const uint32_t lane = pick_lane(ctx);
+  int rc = lane_follow(ctx, lane);
+  if (!rc) rc = run_batch(ctx, scene, plan, nullptr, lane);
   hipEvent_t ev = nullptr;
   if (!rc) {
     if (!sv->free_events.empty()) {
@@ -214,7 +218,7 @@ int round(mpt_server *sv, std::unique_lock<std::mutex> &lk, uint32_t *ran) {
     } else if (hipEventCreateWithFlags(&ev, hipEventDisableTiming) != hipSuccess) {
       rc = MPT_E_DEVICE;
     }
-    if (!rc && hipEventRecord(ev, ctx->stream) != hipSuccess) rc = MPT_E_DEVICE;
+    if (!rc && hipEventRecord(ev, ctx->lanes[lane].stream) != hipSuccess) rc = MPT_E_DEVICE;
   }
   if (rc) {
     for (auto &part : parts) {
@@ -276,8 +280,11 @@ int mpt_server_create(mpt_context *ctx, const mpt_server_config *cfg, mpt_server
   if (cfg) sv->cfg = *cfg;
   if (!sv->cfg.hbm_budget) sv->cfg.hbm_budget = 16ull << 30;
   if (!sv->cfg.max_active) sv->cfg.max_active = 256;
-  if (!sv->cfg.max_inflight) sv->cfg.max_inflight = 2;
-  sv->cfg.max_inflight = std::min<uint32_t>(sv->cfg.max_inflight, 3);   // the descriptor staging ring holds 4
+  // two batches in flight per lane by default; at most three per lane, since
+  // each lane's descriptor staging ring holds four
+  const uint32_t lanes = std::max(1u, std::min(ctx->num_lanes, kMaxLanes));
+  if (!sv->cfg.max_inflight) sv->cfg.max_inflight = 2 * lanes;
+  sv->cfg.max_inflight = std::min<uint32_t>(sv->cfg.max_inflight, 3 * lanes);
   if (reset_totals(ctx)) return MPT_E_DEVICE;
   ctx->servers.push_back(sv.get());
   *out = sv.release();
