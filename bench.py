@@ -305,8 +305,14 @@ def main():
     assert runs[0]["extension_rays"] == ext_rays
     ms_step = 1000.0 * elapsed / args.steps
     extend_ms = float(np.mean([r["extend_ms"] for r in runs]))
+    extend_wall = float(np.mean([r["extend_wall_ms"] for r in runs]))
     launches = runs[0]["extend_launches"]
-    achieved = alg_bytes / (extend_ms * 1e-3) / 1e9
+    # Launches on different lanes overlap: the throughput of the kernel is its
+    # algorithmic bytes over the time during which at least one launch ran.
+    # The per-launch figure (bytes per launch / average launch duration, the
+    # duration rocprof reports per dispatch) is given beside it.
+    achieved = alg_bytes / (extend_wall * 1e-3) / 1e9
+    per_launch = (alg_bytes / launches) / (extend_ms / launches * 1e-3) / 1e9
     out = {
         "metric": "paths/s",
         "value": paths * args.steps / elapsed,
@@ -329,7 +335,8 @@ def main():
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
                      "kernel": "k_extend", "alg_bytes_per_launch": alg_bytes / launches,
-                     "avg_launch_ms": extend_ms / launches,
+                     "avg_launch_ms": extend_ms / launches, "launches_per_step": launches,
+                     "busy_ms_per_step": extend_wall, "achieved_per_launch": per_launch,
                      "nodes_per_ray": cst["nodes_visited"] / ext_rays,
                      "refs_per_ray": cst["prims_tested"] / ext_rays},
     }

@@ -142,12 +142,16 @@ typedef struct mpt_render_stats {
   uint64_t extension_rays; /* closest-hit rays, camera rays included */
   uint64_t shadow_rays;
   double render_ms;        /* device time of the whole render (HIP events) */
-  double extend_ms;        /* device time in the closest-hit kernel */
+  double extend_ms;        /* sum of closest-hit launch durations (MPT_OPT_TIMING) */
   double shade_ms;
   double shadow_ms;
   uint64_t extend_launches;
   uint64_t nodes_visited;  /* kd nodes fetched by closest-hit rays (MPT_OPT_COUNTERS) */
   uint64_t prims_tested;   /* leaf references tested by closest-hit rays (MPT_OPT_COUNTERS) */
+  double extend_wall_ms;   /* time during which at least one closest-hit launch ran: launches on
+                              different lanes overlap, so this is below extend_ms */
+  double shade_wall_ms;
+  double shadow_wall_ms;
 } mpt_render_stats;
 
 /* Context options (mpt_set_option).  No reference counterpart: these are the
@@ -158,12 +162,12 @@ enum {
   MPT_OPT_COUNTERS = 3,    /* 1: count kd nodes and references visited by closest-hit rays */
   MPT_OPT_GRAPHS = 4,      /* 1 (default): replay each batch's kernels from a captured hipGraph */
   MPT_OPT_STACK = 5,       /* traversal short-stack entries per ray: 8 (default), 12, 16 or 24 */
-  MPT_OPT_PERSISTENT = 6,  /* closest-hit kernel form: 0 (default) one ray per lane, grid-stride over
-                              8 blocks per CU; 1 persistent waves that refill finished lanes from a
-                              per-wave range of the queue; 2 one ray per lane, one grid slot per
-                              queue slot */
+  MPT_OPT_PERSISTENT = 6,  /* closest-hit kernel form: 0 one ray per lane, grid-stride over 8
+                              blocks per CU; 1 persistent waves that refill finished lanes from a
+                              per-wave range of the queue; 2 (default) one ray per lane, one grid
+                              slot per queue slot */
   MPT_OPT_LANES = 7        /* streams (each with its own batch workspace) that consecutive batches
-                              alternate over, 1..4, default 2: one batch's kernel tails overlap
+                              alternate over, 1..4, default 3: one batch's kernel tails overlap
                               another's bulk */
 };
 

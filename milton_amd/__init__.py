@@ -74,7 +74,8 @@ class RenderStats(ctypes.Structure):
                 ("render_ms", ctypes.c_double), ("extend_ms", ctypes.c_double),
                 ("shade_ms", ctypes.c_double), ("shadow_ms", ctypes.c_double),
                 ("extend_launches", ctypes.c_uint64), ("nodes_visited", ctypes.c_uint64),
-                ("prims_tested", ctypes.c_uint64)]
+                ("prims_tested", ctypes.c_uint64), ("extend_wall_ms", ctypes.c_double),
+                ("shade_wall_ms", ctypes.c_double), ("shadow_wall_ms", ctypes.c_double)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

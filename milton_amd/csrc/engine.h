@@ -143,7 +143,7 @@ struct mpt_context {
   int num_cus = 256;
   std::string err;
   mpt_engine::Lane lanes[mpt_engine::kMaxLanes];
-  uint32_t num_lanes = 2;            // lanes batches alternate over
+  uint32_t num_lanes = 3;            // lanes batches alternate over
   uint32_t next_lane = 0;
   hipEvent_t mark = nullptr;         // fork point on the context stream
   uint32_t batch_paths = 1u << 24;   // paths per batch (queue depth)
@@ -151,7 +151,7 @@ struct mpt_context {
   bool counters = false;
   bool graphs = true;
   uint32_t stack = 8;                // short-stack entries of the traversal kernels
-  uint32_t persistent = 0;           // closest-hit kernel form: 0 grid-stride, 1 persistent waves with
+  uint32_t persistent = 2;           // closest-hit kernel form: 0 grid-stride, 1 persistent waves with
                                      // lane refill, 2 one ray per lane over a capacity-sized grid
   std::vector<hipEvent_t> events;    // timing pool, pairs per launch
   std::vector<mpt_scene *> scenes;   // live scenes, films and servers, released by mpt_close

@@ -959,10 +959,31 @@ int render_range(mpt_scene *s, const mpt_job *job, uint32_t pix_begin, uint32_t 
     stats->prims_tested = tot[4];
     stats->render_ms = ms;
     stats->extend_launches = batches * job->max_depth;
+    std::vector<std::pair<float, float>> span[3];   // per kind: [start, end) after ev0
     for (const Timed &t : g_timed) {
-      float e = 0;
+      float e = 0, a = 0, b = 0;
       MPT_HIP(ctx, hipEventElapsedTime(&e, ctx->events[t.ev], ctx->events[t.ev + 1]));
+      MPT_HIP(ctx, hipEventElapsedTime(&a, ctx->ev0, ctx->events[t.ev]));
+      MPT_HIP(ctx, hipEventElapsedTime(&b, ctx->ev0, ctx->events[t.ev + 1]));
       (t.kind == 0 ? stats->extend_ms : t.kind == 1 ? stats->shade_ms : stats->shadow_ms) += e;
+      span[t.kind].push_back({a, b});
+    }
+    double *wall[3] = {&stats->extend_wall_ms, &stats->shade_wall_ms, &stats->shadow_wall_ms};
+    for (int k = 0; k < 3; ++k) {
+      // length of the union of the launch intervals
+      std::sort(span[k].begin(), span[k].end());
+      double total = 0, lo = 0, hi = -1;
+      for (const auto &iv : span[k]) {
+        if (iv.first > hi) {
+          if (hi > lo) total += hi - lo;
+          lo = iv.first;
+          hi = iv.second;
+        } else {
+          hi = std::max(hi, (double)iv.second);
+        }
+      }
+      if (hi > lo) total += hi - lo;
+      *wall[k] = total;
     }
   }
   return MPT_OK;
