@@ -86,10 +86,8 @@ bool clip_triangle(const float *v, const Box &cell, Box &out) {
 
 class Builder {
  public:
-  Builder(const std::vector<Box> &prims, const Params &p, uint32_t max_depth, int spawn_depth,
-          const float *tri_verts, uint32_t num_tris)
-      : prims_(prims), p_(p), max_depth_(max_depth), spawn_depth_(spawn_depth), tri_verts_(tri_verts),
-        num_tris_(num_tris) {}
+  Builder(const Params &p, uint32_t max_depth, int spawn_depth, const float *tri_verts, uint32_t num_tris)
+      : p_(p), max_depth_(max_depth), spawn_depth_(spawn_depth), tri_verts_(tri_verts), num_tris_(num_tris) {}
 
   void build(BNode *node, std::vector<Ref> refs, Box cell, uint32_t depth) {
     const size_t n = refs.size();
@@ -220,7 +218,6 @@ class Builder {
     return found;
   }
 
-  const std::vector<Box> &prims_;
   const Params &p_;
   uint32_t max_depth_;
   int spawn_depth_;
@@ -284,7 +281,7 @@ void build(const std::vector<Box> &prims, const Params &p,
   std::vector<Ref> refs(prims.size());
   for (uint32_t i = 0; i < refs.size(); ++i) refs[i] = Ref{i, prims[i]};
   BNode root;
-  Builder(prims, p, max_depth, spawn, p.clip ? tri_verts : nullptr, num_tris).build(&root, std::move(refs), box, 0);
+  Builder(p, max_depth, spawn, p.clip ? tri_verts : nullptr, num_tris).build(&root, std::move(refs), box, 0);
   out.nodes.assign(2, 0);
   Flattener{write_record, out}.run(root, 0, 0);
   out.n_nodes = out.nodes.size() / 2;

@@ -46,18 +46,32 @@ constexpr uint32_t F_EMIT_AFTER_SPEC = 16u;  // emittedContribution > EPSILON (P
 
 __device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
 
-// Appends `want` lanes to a queue with one atomic per wavefront; returns the
-// lane's slot.
-__device__ __forceinline__ uint32_t wave_append(bool want, uint32_t *counter) {
+// Appends `want` lanes to a queue with one atomic per block: the waves'
+// counts meet in LDS (sh holds WAVES + 1 words) and one lane adds the block
+// total.  Queue counters are hot (every block of a launch appends to the
+// same two or three), so this divides their traffic by the waves per block.
+// Every thread of the block must call it.
+template <int WAVES>
+__device__ __forceinline__ uint32_t block_append(bool want, uint32_t *counter, uint32_t *sh) {
   const uint64_t mask = __ballot(want);
-  if (mask == 0) return 0;
-  const uint32_t leader = (uint32_t)__ffsll((unsigned long long)mask) - 1;
-  uint32_t base = 0;
-  if (lane_id() == leader) base = atomicAdd(counter, (uint32_t)__popcll(mask));
-  base = __shfl(base, leader);
+  const uint32_t wave = threadIdx.x / 64;
+  if (lane_id() == 0) sh[wave] = (uint32_t)__popcll(mask);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t total = 0;
+    for (int w = 0; w < WAVES; ++w) {
+      const uint32_t c = sh[w];
+      sh[w] = total;
+      total += c;
+    }
+    sh[WAVES] = total ? atomicAdd(counter, total) : 0u;
+  }
+  __syncthreads();
   const uint32_t below = __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32),
                                                    __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
-  return base + below;
+  const uint32_t idx = sh[WAVES] + sh[wave] + below;
+  __syncthreads();   // sh is reused by the next append
+  return idx;
 }
 
 // ---------------------------------------------------------------------------
@@ -158,7 +172,7 @@ __global__ __launch_bounds__(256) void k_extend(Scene sc, const float4 *__restri
 constexpr uint32_t kRefill = 16;
 
 template <bool COUNT, int STACK>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void k_extend_persistent(
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(STACK <= 8 ? 8 : (STACK <= 16 ? 4 : 2)))) void k_extend_persistent(
     Scene sc, const float4 *__restrict__ q, const uint32_t *__restrict__ count, float4 *__restrict__ hits,
     unsigned long long *__restrict__ counters) {
   __shared__ uint2 stack[STACK * kBlock];
@@ -272,15 +286,19 @@ __device__ __forceinline__ HitInfo hit_info(const Scene &sc, uint32_t prim, floa
 // 28-116) with DirectIllumination::evaluate (renderers/DirectIllumination.cpp:
 // 46-136).  The recursion becomes a throughput `beta`; the emission terms, NEE
 // and roulette follow the reference rules, each cited below.
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k_shade(Scene sc, const BatchDesc *__restrict__ bd, uint32_t depth,
+constexpr int kShadeBlock = 512;
+
+__global__ __launch_bounds__(kShadeBlock) __attribute__((amdgpu_waves_per_eu(6))) void k_shade(Scene sc, const BatchDesc *__restrict__ bd, uint32_t depth,
                                                const float4 *__restrict__ qin, const float4 *__restrict__ hits,
                                                const uint32_t *__restrict__ count_in, float4 *__restrict__ qout,
                                                uint32_t *__restrict__ count_out, float4 *__restrict__ sq,
                                                uint32_t *__restrict__ count_shadow, float4 *__restrict__ Ls,
                                                float4 *__restrict__ betas, uint32_t *__restrict__ flags,
                                                const uint2 *__restrict__ pjs) {
+  __shared__ uint32_t append_sh[kShadeBlock / 64 + 1];
   const uint32_t n = *count_in;
   const uint32_t stride = gridDim.x * blockDim.x;
+  // block-uniform loop: block_append synchronises the block
   for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += stride) {
     const uint32_t i = base + threadIdx.x;
     const bool active = i < n;
@@ -383,7 +401,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k
           }
         }
       }
-      const uint32_t si = wave_append(want_sh, count_shadow);
+      const uint32_t si = block_append<kShadeBlock / 64>(want_sh, count_shadow, append_sh);
       if (want_sh) {
         sq[3 * si] = make_float4(so.x, so.y, so.z, __uint_as_float(slot));
         sq[3 * si + 1] = make_float4(sd.x, sd.y, sd.z, sdist);
@@ -406,7 +424,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k
         }
       }
     }
-    const uint32_t ei = wave_append(want_ext, count_out);
+    const uint32_t ei = block_append<kShadeBlock / 64>(want_ext, count_out, append_sh);
     if (want_ext) {
       qout[2 * ei] = make_float4(p.x, p.y, p.z, __uint_as_float(slot));
       qout[2 * ei + 1] = make_float4(wo.x, wo.y, wo.z, __uint_as_float(prim));
@@ -623,7 +641,7 @@ int enqueue_batch(mpt_context *ctx, Lane &ln, mpt_scene *s, uint32_t D, bool ins
                   instrument && ctx->counters ? w.totals.p + 3 : nullptr);
     if ((rc = toc())) return rc;
     if ((rc = tic(1))) return rc;
-    k_shade<<<g, kBlock, 0, st>>>(s->dev, w.desc.p, d, qin, w.hits.p, w.counts.p + d, qout,
+    k_shade<<<(g * kBlock + kShadeBlock - 1) / kShadeBlock, kShadeBlock, 0, st>>>(s->dev, w.desc.p, d, qin, w.hits.p, w.counts.p + d, qout,
                                            w.counts.p + d + 1, w.sq.p, w.counts.p + D + 1 + d, w.Ls.p, w.betas.p,
                                            w.flags.p, w.pjs.p);
     if ((rc = toc())) return rc;
