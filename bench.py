@@ -26,7 +26,8 @@ sys.path.insert(0, ROOT)
 WIDTH, HEIGHT, SPP, DEPTH = 1920, 1080, 64, 4
 TRIANGLES = 1_000_000
 HBM_PEAK_GBS = 8000.0
-NODE_BYTES, RECORD_BYTES, RAY_BYTES, HIT_BYTES = 8, 48, 32, 8
+NODE_BYTES, RECORD_BYTES, RAY_BYTES, HIT_BYTES = 8, 48, 32, 16
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "hbm_traffic_soup1M_1920x1080_64spp.md")
 
 
 def log(msg):
@@ -101,6 +102,25 @@ def rmse_vs_reference(m, ctx, scene, job):
     log(f"rmse {rmse:.5f} vs {RMSE_REF_SPP}-spp reference over {len(rows)} rows ({ref_s:.1f} s)")
     return {"value": rmse, "reference_spp": RMSE_REF_SPP, "pixels": len(rows) * job.width,
             "rows": rows, "mean_reference": float(b.mean())}
+
+
+def measured_traffic(engine, kd_leaf):
+    """HBM bytes per k_extend launch from the committed counter summary
+    (FETCH_SIZE + WRITE_SIZE, copy-calibrated), if it was taken with the same
+    batch size, stack, kernel form and kd leaf size; else None."""
+    import re
+    try:
+        text = open(TRAFFIC_PROFILE).read()
+    except OSError:
+        return None
+    m = re.search(r"traffic_bytes_per_launch k_extend batch_paths=(\d+) stack=(\d+) persistent=(\d+) "
+                  r"kd_leaf=(\d+): (\d+)", text)
+    if not m:
+        return None
+    bp, st, pe, leaf, val = (int(x) for x in m.groups())
+    if (bp, st, pe, leaf) != (engine["batch_paths"], engine["stack"], engine["persistent"], kd_leaf):
+        return None
+    return val
 
 
 def engine_settings(ctx, m):
@@ -312,6 +332,9 @@ def main():
     # The per-launch figure (bytes per launch / average launch duration, the
     # duration rocprof reports per dispatch) is given beside it.
     achieved = alg_bytes / (extend_wall * 1e-3) / 1e9
+    engine = engine_settings(ctx, m)
+    default_tree = not (args.tune or args.kd_ci or args.kd_eb or args.no_clip)
+    traffic = measured_traffic(engine, args.kd_leaf or 4) if default_tree else None
     per_launch = (alg_bytes / launches) / (extend_ms / launches * 1e-3) / 1e9
     out = {
         "metric": "paths/s",
@@ -330,10 +353,10 @@ def main():
         "config": {"workload": "kd-tree stress scene: 1M random triangles, 1920x1080, 64 spp, max depth 4",
                    "triangles": TRIANGLES, "width": WIDTH, "height": HEIGHT, "spp": SPP, "max_depth": DEPTH,
                    "rays_per_step": rays, "kd_nodes": st["nodes"], "kd_refs": st["refs"],
-                   "scene_build_s": round(build_s, 3), "tuned": tuned,
-                   "engine": engine_settings(ctx, m)},
+                   "scene_build_s": round(build_s, 3), "tuned": tuned, "engine": engine},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "traffic_source": os.path.relpath(TRAFFIC_PROFILE, ROOT) if traffic else None,
                      "kernel": "k_extend", "alg_bytes_per_launch": alg_bytes / launches,
                      "avg_launch_ms": extend_ms / launches, "launches_per_step": launches,
                      "busy_ms_per_step": extend_wall, "achieved_per_launch": per_launch,
