@@ -141,8 +141,12 @@ def system_scenes(m, ctx):
 
 
 def bench_batched(m, ctx, args, scenes=None):
-    """64 mixed jobs: each job alone (plain per-job wavefront) vs one server
-    packing all of them into shared batches (graph-replayed)."""
+    """64 mixed jobs three ways: one server packing all of them into shared
+    batches (graph-replayed, over the context's lanes); each job alone
+    through the same engine (plain per-job); and each job alone on one
+    stream with kernels launched one by one (a classic wavefront renderer).
+    Every film must come out bit-identical, so the RMSE is equal by
+    construction."""
     from milton_amd import workloads
     if scenes is None:
         scenes, _ = system_scenes(m, ctx)
@@ -153,28 +157,44 @@ def bench_batched(m, ctx, args, scenes=None):
         j = m.make_job(cams[kind], 64, 64, 4, 8, seed=1)
         sc.render(j)
     films = [m.Film(ctx, j.width, j.height) for _, j in jobs]
-    ctx.sync()
-    t0 = time.perf_counter()
-    for (kind, j), f in zip(jobs, films):
-        f.render(scenes[kind][1], j)
-    ctx.sync()
-    t_plain = time.perf_counter() - t0
+
+    def per_job():
+        ctx.sync()
+        t0 = time.perf_counter()
+        for (kind, j), f in zip(jobs, films):
+            f.render(scenes[kind][1], j)
+        ctx.sync()
+        return time.perf_counter() - t0
+
+    saved = {o: ctx.get_option(o) for o in (m.MPT_OPT_LANES, m.MPT_OPT_GRAPHS)}
+    ctx.set_option(m.MPT_OPT_LANES, 1)
+    ctx.set_option(m.MPT_OPT_GRAPHS, 0)
+    per_job()                                # warm-up of the classic form
+    t_classic = per_job()
+    classic = [f.download() for f in films]
+    for o, v in saved.items():
+        ctx.set_option(o, v)
+    per_job()
+    t_plain = per_job()
     srv = m.Server(ctx)
     t0 = time.perf_counter()
     handles = [srv.submit(scenes[kind][1], j) for kind, j in jobs]
     srv.drain()
     t_batched = time.perf_counter() - t0
     st = srv.stats()
-    same = 0
-    for h, (kind, j), f in zip(handles, jobs, films):
-        same += int(np.array_equal(srv.result(h, j.width, j.height), f.download()))
+    same = same_classic = 0
+    for h, (kind, j), f, c in zip(handles, jobs, films, classic):
+        got = srv.result(h, j.width, j.height)
+        same += int(np.array_equal(got, f.download()))
+        same_classic += int(np.array_equal(got, c))
         f.close()
     srv.close()
-    log(f"batched: {len(jobs)} jobs, {total} paths, plain {t_plain:.3f}s, batched {t_batched:.3f}s, "
-        f"{st['batches']} batches, identical films {same}/{len(jobs)}")
-    return {"paths": total, "t_plain": t_plain, "t_batched": t_batched, "batches": st["batches"],
-            "identical": same, "jobs": len(jobs), "extension_rays": st["extension_rays"],
-            "shadow_rays": st["shadow_rays"]}
+    log(f"batched: {len(jobs)} jobs, {total} paths, classic per-job {t_classic:.3f}s, plain per-job {t_plain:.3f}s, "
+        f"batched {t_batched:.3f}s, {st['batches']} batches, identical films {same}/{len(jobs)} "
+        f"(vs classic {same_classic}/{len(jobs)})")
+    return {"paths": total, "t_classic": t_classic, "t_plain": t_plain, "t_batched": t_batched,
+            "batches": st["batches"], "identical": same, "identical_classic": same_classic, "jobs": len(jobs),
+            "extension_rays": st["extension_rays"], "shadow_rays": st["shadow_rays"]}
 
 
 def bench_serving(m, ctx, args, scenes, capacity_jobs_s):
@@ -235,7 +255,8 @@ def main_systems(args):
                                   "spp 4-256, depth 4-8", "scene_build_s": build,
                       "engine": engine_settings(ctx, m)},
            "batched": b, "speedup_vs_plain": b["t_plain"] / b["t_batched"],
-           "plain_paths_per_s": b["paths"] / b["t_plain"]}
+           "speedup_vs_classic": b["t_classic"] / b["t_batched"],
+           "plain_paths_per_s": b["paths"] / b["t_plain"], "classic_paths_per_s": b["paths"] / b["t_classic"]}
     if args.workload == "serving":
         out["serving"] = bench_serving(m, ctx, args, scenes, b["jobs"] / b["t_batched"])
     ctx.close()
