@@ -238,9 +238,10 @@ MPT_HD float refine_t(const float4 *rec, const RaySetup &r, float t32) {
 
 // kd traversal with a STACK-entry ring stack in LDS and kd-restart on
 // overflow (Foley & Sugerman 2005).  `stk` points at this lane's column of a
-// [STACK][kBlock] uint2 array.  skip_origin is the primitive the ray leaves
-// (self-intersection guard in place of the reference's FP64 t > EPSILON),
-// ANY=true finishes at the first hit.
+// [STACK + 1][kBlock] uint2 array (the last row is a dummy store target).
+// skip_origin is the primitive the ray leaves (self-intersection guard in
+// place of the reference's FP64 t > EPSILON), ANY=true finishes at the first
+// hit.
 // COUNT=true adds the nodes fetched and references tested to n_nodes/n_refs.
 //
 // The walk is resumable: start() clips the ray to the scene box, each step()
@@ -292,18 +293,18 @@ struct Walker {
       const bool below = (oa < split) | ((oa == split) & (ia < 0.0f));
       const uint32_t first = below ? child : child + 1;
       const uint32_t second = below ? child + 1 : child;
-      if (tp > tmax || tp <= 0.0f) {
-        node = first;
-      } else if (tp < tmin) {
-        node = second;
-      } else {
-        stk[top * kBlock] = make_uint2(second, mpt_f2u(tmax));
-        top = top + 1 == STACK ? 0 : top + 1;
-        if (depth == STACK) dropped = true;
-        else ++depth;
-        node = first;
-        tmax = tp;
-      }
+      // Branch-free descent: the far cell is pushed only when the ray spans
+      // both children.  The store always happens, into the dummy slot STACK
+      // when there is nothing to push, so it never clobbers a live entry.
+      const bool near_only = tp > tmax || tp <= 0.0f;
+      const bool far_only = !near_only && tp < tmin;
+      const bool both = !near_only && !far_only;
+      stk[(both ? top : STACK) * kBlock] = make_uint2(second, mpt_f2u(tmax));
+      dropped = dropped || (both && depth == STACK);
+      depth = both ? (depth + 1 > STACK ? STACK : depth + 1) : depth;
+      top = both ? (top + 1 == STACK ? 0 : top + 1) : top;
+      tmax = both ? tp : tmax;
+      node = far_only ? second : first;
       nd = sc.nodes[node];
       if (COUNT) ++*n_nodes;
     }
