@@ -312,26 +312,24 @@ struct Walker {
     if (COUNT) *n_refs += count;
     const float4 *rec = sc.recs + 3ull * nd.y;
     for (uint32_t k = 0; k < count; ++k, rec += 3) {
-      const float4 a = rec[0], b = rec[1];
+      // the whole 48-byte record in one go (every kind has three float4s)
+      const float4 a = rec[0], b = rec[1], c = rec[2];
       const uint32_t id = mpt_f2u(a.w);
       const uint32_t kind = mpt_f2u(b.w);
       float t;
       if (kind == KIND_TRIANGLE) {
-        if (id == skip_origin) continue;
-        const float4 c = rec[2];
         float T, det;
         // divide only for candidates that can beat the current best; the
         // 2^-20 margins keep the prefilter conservative, the exact test
         // below decides
-        if (!tri_td(r, xyz(a), xyz(b), xyz(c), T, det) || T > best * det * (1.0f + 0x1p-20f) ||
-            T < kEps * det * (1.0f - 0x1p-20f))
+        if (id == skip_origin || !tri_td(r, xyz(a), xyz(b), xyz(c), T, det) ||
+            T > best * det * (1.0f + 0x1p-20f) || T < kEps * det * (1.0f - 0x1p-20f))
           continue;
         t = T / det;
       } else if (kind == KIND_SPHERE) {
         t = sphere_t(r, xyz(a), b.x, id == skip_origin);
       } else {
         if (id == skip_origin) continue;
-        const float4 c = rec[2];
         t = quad_t(r, xyz(a), xyz(b), xyz(c));
       }
       if (t > kEps && t < best) {
