@@ -91,7 +91,7 @@ struct Workspace {
   DBuf<float4> Ls, betas;
   DBuf<uint32_t> flags;
   DBuf<uint2> pjs;           // (pixel, job << 20 | sample) per slot
-  DBuf<uint32_t> counts;     // [0, D] queue sizes, [D+1, 2D] shadow sizes
+  DBuf<uint32_t> counts;     // [0, D] queue sizes, [D+1, 2D] shadow sizes, [2D+1, 3D] claim cursors
   DBuf<unsigned long long> totals;
   DBuf<BatchDesc> desc;
   size_t bytes() const {

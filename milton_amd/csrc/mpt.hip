@@ -163,26 +163,23 @@ __global__ __launch_bounds__(256) void k_extend(Scene sc, const float4 *__restri
   }
 }
 
-// Persistent variant of k_extend: each wave owns a contiguous range of the
-// queue and keeps its lanes busy by handing a lane whose ray finished the next
-// ray of its range (refill taken when at least kRefill lanes are idle, or all
-// are), so a wave no longer idles until its slowest ray ends.  Ranges are
-// static, so a refill costs no atomic; the imbalance is one ray per lane at
-// the end of a range.
+// Persistent variant of k_extend: each wave keeps its lanes busy by handing a
+// lane whose ray finished the next ray of the wave's claimed chunk (refill
+// taken when at least kRefill lanes are idle, or all are); a wave claims
+// kChunk rays of the queue at a time with one atomic, so the claims balance
+// the load across waves without one atomic per refill.
 constexpr uint32_t kRefill = 16;
+constexpr uint32_t kChunk = 256;
 
 template <bool COUNT, int STACK>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(STACK <= 8 ? 8 : (STACK <= 16 ? 4 : 2)))) void k_extend_persistent(
     Scene sc, const float4 *__restrict__ q, const uint32_t *__restrict__ count, float4 *__restrict__ hits,
-    unsigned long long *__restrict__ counters) {
+    unsigned long long *__restrict__ counters, uint32_t *__restrict__ fetch) {
   __shared__ uint2 stack[(STACK + 1) * kBlock];
   uint2 *stk = stack + threadIdx.x;
   const uint32_t n = *count;
-  const uint32_t waves = gridDim.x * (kBlock / 64);
-  const uint32_t wave = blockIdx.x * (kBlock / 64) + __builtin_amdgcn_readfirstlane(threadIdx.x / 64);
-  const uint32_t per = (n + waves - 1) / waves;
-  uint32_t next = min(n, wave * per);           // wave-uniform
-  const uint32_t end = min(n, next + per);
+  uint32_t next = 0, end = 0;   // wave-uniform: the unassigned part of the claimed chunk
+  bool exhausted = false;       // wave-uniform: the queue has no unclaimed rays left
   uint32_t n_nodes = 0, n_refs = 0;
   uint32_t ray = kMiss;
   RaySetup r;
@@ -191,22 +188,32 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(STACK <= 8 
     const bool need = ray == kMiss;
     const uint64_t idle = __ballot(need);
     const uint32_t n_idle = (uint32_t)__popcll(idle);
-    if (next < end && (n_idle >= kRefill || n_idle == 64)) {
-      const uint32_t idx = next + __builtin_amdgcn_mbcnt_hi((uint32_t)(idle >> 32),
-                                                            __builtin_amdgcn_mbcnt_lo((uint32_t)idle, 0u));
-      next = min(end, next + n_idle);
-      if (need && idx < end) {
-        const float4 a = q[2 * idx], b = q[2 * idx + 1];
-        r = setup_ray(xyz(a), xyz(b));
-        if (w.start(sc, r, INFINITY, __float_as_uint(b.w))) {
-          ray = idx;
-        } else {
-          hits[idx] = make_float4(INFINITY, __uint_as_float(kMiss), 0.0f, 0.0f);
+    if (n_idle >= kRefill || n_idle == 64) {
+      if (next >= end && !exhausted) {
+        uint32_t base = 0;
+        if (lane_id() == 0) base = atomicAdd(fetch, kChunk);
+        base = __builtin_amdgcn_readfirstlane(base);
+        next = min(base, n);
+        end = min(base + kChunk, n);
+        exhausted = base + kChunk >= n;
+      }
+      if (next < end) {
+        const uint32_t idx = next + __builtin_amdgcn_mbcnt_hi((uint32_t)(idle >> 32),
+                                                              __builtin_amdgcn_mbcnt_lo((uint32_t)idle, 0u));
+        next = min(end, next + n_idle);
+        if (need && idx < end) {
+          const float4 a = q[2 * idx], b = q[2 * idx + 1];
+          r = setup_ray(xyz(a), xyz(b));
+          if (w.start(sc, r, INFINITY, __float_as_uint(b.w))) {
+            ray = idx;
+          } else {
+            hits[idx] = make_float4(INFINITY, __uint_as_float(kMiss), 0.0f, 0.0f);
+          }
         }
       }
     }
     if (__ballot(ray != kMiss) == 0) {
-      if (next >= end) break;
+      if (exhausted && next >= end) break;
       continue;
     }
     if (ray != kMiss && w.template step<COUNT>(sc, r, stk, &n_nodes, &n_refs)) {
@@ -566,12 +573,12 @@ std::vector<Timed> g_timed;   // guarded by the single-threaded use of a context
 
 template <int STACK>
 void launch_extend_t(mpt_context *ctx, hipStream_t st, uint32_t cap, uint32_t g, const Scene &sc, const float4 *q,
-                     const uint32_t *count, float4 *hits, unsigned long long *counters) {
+                     const uint32_t *count, float4 *hits, unsigned long long *counters, uint32_t *fetch) {
   if (ctx->persistent == 1) {
     if (counters)
-      k_extend_persistent<true, STACK><<<g, kBlock, 0, st>>>(sc, q, count, hits, counters);
+      k_extend_persistent<true, STACK><<<g, kBlock, 0, st>>>(sc, q, count, hits, counters, fetch);
     else
-      k_extend_persistent<false, STACK><<<g, kBlock, 0, st>>>(sc, q, count, hits, nullptr);
+      k_extend_persistent<false, STACK><<<g, kBlock, 0, st>>>(sc, q, count, hits, nullptr, fetch);
     return;
   }
   // form 0: grid-stride over a fixed grid of 8 blocks per CU; form 2: one
@@ -586,12 +593,12 @@ void launch_extend_t(mpt_context *ctx, hipStream_t st, uint32_t cap, uint32_t g,
 // The short-stack depth is a run-time choice among compiled variants (it
 // trades LDS per wave, hence occupancy, against kd-restarts).
 void launch_extend(mpt_context *ctx, hipStream_t st, uint32_t cap, uint32_t g, const Scene &sc, const float4 *q,
-                   const uint32_t *count, float4 *hits, unsigned long long *counters) {
+                   const uint32_t *count, float4 *hits, unsigned long long *counters, uint32_t *fetch) {
   switch (ctx->stack) {
-    case 8: launch_extend_t<8>(ctx, st, cap, g, sc, q, count, hits, counters); break;
-    case 12: launch_extend_t<12>(ctx, st, cap, g, sc, q, count, hits, counters); break;
-    case 24: launch_extend_t<24>(ctx, st, cap, g, sc, q, count, hits, counters); break;
-    default: launch_extend_t<16>(ctx, st, cap, g, sc, q, count, hits, counters); break;
+    case 8: launch_extend_t<8>(ctx, st, cap, g, sc, q, count, hits, counters, fetch); break;
+    case 12: launch_extend_t<12>(ctx, st, cap, g, sc, q, count, hits, counters, fetch); break;
+    case 24: launch_extend_t<24>(ctx, st, cap, g, sc, q, count, hits, counters, fetch); break;
+    default: launch_extend_t<16>(ctx, st, cap, g, sc, q, count, hits, counters, fetch); break;
   }
 }
 
@@ -631,14 +638,14 @@ int enqueue_batch(mpt_context *ctx, Lane &ln, mpt_scene *s, uint32_t D, bool ins
     MPT_HIP(ctx, hipEventRecord(ctx->events[g_timed.back().ev + 1], st));
     return MPT_OK;
   };
-  k_reset<<<1, 64, 0, st>>>(w.counts.p, 2 * D + 1);
+  k_reset<<<1, 64, 0, st>>>(w.counts.p, 3 * D + 1);
   k_raygen<<<gcap, 256, 0, st>>>(w.desc.p, w.q[0].p, w.Ls.p, w.betas.p, w.flags.p, w.pjs.p, w.counts.p);
   MPT_LAUNCHED(ctx, "k_raygen");
   for (uint32_t d = 0; d < D; ++d) {
     float4 *qin = w.q[d & 1].p, *qout = w.q[(d + 1) & 1].p;
     if ((rc = tic(0))) return rc;
     launch_extend(ctx, st, cap, g, s->dev, qin, w.counts.p + d, w.hits.p,
-                  instrument && ctx->counters ? w.totals.p + 3 : nullptr);
+                  instrument && ctx->counters ? w.totals.p + 3 : nullptr, w.counts.p + 2 * D + 1 + d);
     if ((rc = toc())) return rc;
     if ((rc = tic(1))) return rc;
     k_shade<<<(g * kBlock + kShadeBlock - 1) / kShadeBlock, kShadeBlock, 0, st>>>(s->dev, w.desc.p, d, qin, w.hits.p, w.counts.p + d, qout,
@@ -798,7 +805,7 @@ int ensure_workspace(mpt_context *ctx, uint32_t k, uint32_t cap, uint32_t depth,
   MPT_HIP(ctx, w.flags.reserve(cap));
   MPT_HIP(ctx, w.pjs.reserve(cap));
   MPT_HIP(ctx, w.sq.reserve(3ull * ns));
-  MPT_HIP(ctx, w.counts.reserve(2ull * depth + 1));
+  MPT_HIP(ctx, w.counts.reserve(3ull * depth + 1));
   MPT_HIP(ctx, w.desc.reserve(1));
   if (!w.totals.p) {
     // totals persist across reallocation: a render's counts are zeroed once
