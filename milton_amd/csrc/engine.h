@@ -88,14 +88,14 @@ struct Workspace {
   DBuf<float4> q[2];
   DBuf<float4> hits;         // (t, prim, barycentric weights of v1 and v2)
   DBuf<float4> sq;
-  DBuf<float4> Ls, betas;
+  DBuf<float4> Ls;
   DBuf<uint32_t> flags;
   DBuf<uint2> pjs;           // (pixel, job << 20 | sample) per slot
   DBuf<uint32_t> counts;     // [0, D] queue sizes, [D+1, 2D] shadow sizes, [2D+1, 3D] claim cursors
   DBuf<unsigned long long> totals;
   DBuf<BatchDesc> desc;
   size_t bytes() const {
-    return q[0].bytes() + q[1].bytes() + hits.bytes() + sq.bytes() + Ls.bytes() + betas.bytes() + flags.bytes() +
+    return q[0].bytes() + q[1].bytes() + hits.bytes() + sq.bytes() + Ls.bytes() + flags.bytes() +
            pjs.bytes() + counts.bytes() + totals.bytes() + desc.bytes();
   }
 };

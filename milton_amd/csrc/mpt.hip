@@ -107,9 +107,7 @@ __device__ __forceinline__ uint32_t chunk_of_pixel(const BatchDesc *bd, uint32_t
 // JitteredSampleGenerator.cpp:30-31).  Slot -> (job, pixel, sample) through
 // the batch's chunk table.
 __global__ __launch_bounds__(256) void k_raygen(const BatchDesc *__restrict__ bd, float4 *__restrict__ q,
-                                                float4 *__restrict__ Ls, float4 *__restrict__ betas,
-                                                uint32_t *__restrict__ flags, uint2 *__restrict__ pjs,
-                                                uint32_t *__restrict__ count0) {
+                                                uint2 *__restrict__ pjs, uint32_t *__restrict__ count0) {
   const uint32_t slot = blockIdx.x * blockDim.x + threadIdx.x;
   const uint32_t n = bd->hdr.n_paths;
   if (slot == 0) *count0 = n;
@@ -130,11 +128,10 @@ __global__ __launch_bounds__(256) void k_raygen(const BatchDesc *__restrict__ bd
   const float nx = (2.0f * x - 1.0f) * cam.tan_w, ny = (1.0f - 2.0f * y) * cam.tan_h;
   const float3 d = normalize(f3(cam.u[0] * nx + cam.v[0] * ny - cam.n[0], cam.u[1] * nx + cam.v[1] * ny - cam.n[1],
                                 cam.u[2] * nx + cam.v[2] * ny - cam.n[2]));
-  q[2 * slot] = make_float4(cam.eye[0], cam.eye[1], cam.eye[2], __uint_as_float(slot));
-  q[2 * slot + 1] = make_float4(d.x, d.y, d.z, __uint_as_float(kMiss));
-  Ls[slot] = make_float4(0, 0, 0, 0);
-  betas[slot] = make_float4(1, 1, 1, 0);
-  flags[slot] = mpt_rng::third(r.z);
+  // queue entry: (origin, slot) (direction, origin primitive) (throughput, flags)
+  q[3 * slot] = make_float4(cam.eye[0], cam.eye[1], cam.eye[2], __uint_as_float(slot));
+  q[3 * slot + 1] = make_float4(d.x, d.y, d.z, __uint_as_float(kMiss));
+  q[3 * slot + 2] = make_float4(1.0f, 1.0f, 1.0f, __uint_as_float(mpt_rng::third(r.z)));
   pjs[slot] = make_uint2(pixel, (ch.job << kJobBits) | sample);
 }
 
@@ -149,7 +146,7 @@ __global__ __launch_bounds__(256) void k_extend(Scene sc, const float4 *__restri
   const uint32_t n = *count;
   uint32_t n_nodes = 0, n_refs = 0;
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-    const float4 a = q[2 * i], b = q[2 * i + 1];
+    const float4 a = q[3 * i], b = q[3 * i + 1];
     const RaySetup r = setup_ray(xyz(a), xyz(b));
     float t;
     uint32_t id;
@@ -202,7 +199,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(STACK <= 8 
                                                               __builtin_amdgcn_mbcnt_lo((uint32_t)idle, 0u));
         next = min(end, next + n_idle);
         if (need && idx < end) {
-          const float4 a = q[2 * idx], b = q[2 * idx + 1];
+          const float4 a = q[3 * idx], b = q[3 * idx + 1];
           r = setup_ray(xyz(a), xyz(b));
           if (w.start(sc, r, INFINITY, __float_as_uint(b.w))) {
             ray = idx;
@@ -300,8 +297,7 @@ __global__ __launch_bounds__(kShadeBlock) __attribute__((amdgpu_waves_per_eu(6))
                                                const uint32_t *__restrict__ count_in, float4 *__restrict__ qout,
                                                uint32_t *__restrict__ count_out, float4 *__restrict__ sq,
                                                uint32_t *__restrict__ count_shadow, float4 *__restrict__ Ls,
-                                               float4 *__restrict__ betas, uint32_t *__restrict__ flags,
-                                               const uint2 *__restrict__ pjs) {
+                                               uint32_t *__restrict__ flags, const uint2 *__restrict__ pjs) {
   __shared__ uint32_t append_sh[kShadeBlock / 64 + 1];
   const uint32_t n = *count_in;
   const uint32_t stride = gridDim.x * blockDim.x;
@@ -324,7 +320,8 @@ __global__ __launch_bounds__(kShadeBlock) __attribute__((amdgpu_waves_per_eu(6))
     float3 d = f3(0, 0, 1);
     uint32_t pixel = 0, sample = 0, seed = 0, job_id = 0, max_depth = 0;
     if (active) {
-      const float4 a = qin[2 * i], b = qin[2 * i + 1];
+      // (origin, slot) (direction, origin primitive) (throughput, flags)
+      const float4 a = qin[3 * i], b = qin[3 * i + 1], c = qin[3 * i + 2];
       const float4 h = hits[i];
       slot = __float_as_uint(a.w);
       d = xyz(b);
@@ -335,10 +332,8 @@ __global__ __launch_bounds__(kShadeBlock) __attribute__((amdgpu_waves_per_eu(6))
       seed = job.seed;
       job_id = job.job_id;
       max_depth = job.max_depth;
-      const float4 bv = betas[slot];
-      beta = xyz(bv);
-      fl = flags[slot];
-      Lv = Ls[slot];
+      beta = xyz(c);
+      fl = __float_as_uint(c.w);
       prim = __float_as_uint(h.y);
       if (prim == kMiss) {
         // Scene::getBackgroundRadiance (core/Scene.cpp:116-118), every depth.
@@ -433,13 +428,24 @@ __global__ __launch_bounds__(kShadeBlock) __attribute__((amdgpu_waves_per_eu(6))
     }
     const uint32_t ei = block_append<kShadeBlock / 64>(want_ext, count_out, append_sh);
     if (want_ext) {
-      qout[2 * ei] = make_float4(p.x, p.y, p.z, __uint_as_float(slot));
-      qout[2 * ei + 1] = make_float4(wo.x, wo.y, wo.z, __uint_as_float(prim));
-      betas[slot] = make_float4(beta.x, beta.y, beta.z, 0.0f);
+      qout[3 * ei] = make_float4(p.x, p.y, p.z, __uint_as_float(slot));
+      qout[3 * ei + 1] = make_float4(wo.x, wo.y, wo.z, __uint_as_float(prim));
+      qout[3 * ei + 2] = make_float4(beta.x, beta.y, beta.z, __uint_as_float(fl));
     }
     if (active) {
-      Ls[slot] = Lv;
-      flags[slot] = fl;
+      // A slot's radiance starts with its camera vertex (a store: k_raygen
+      // leaves it unset); later vertices add with atomics that return
+      // nothing.  Shadow contributions of the same depth come after, in
+      // k_shadow, so each slot sees one ordered sequence of additions.
+      if (depth == 0) {
+        Ls[slot] = Lv;
+      } else if (Lv.x != 0.0f || Lv.y != 0.0f || Lv.z != 0.0f) {
+        atomicAdd(&Ls[slot].x, Lv.x);
+        atomicAdd(&Ls[slot].y, Lv.y);
+        atomicAdd(&Ls[slot].z, Lv.z);
+      }
+      // the path's flags as k_finalize needs them, once, when it ends
+      if (!want_ext) flags[slot] = fl;
     }
   }
 }
@@ -602,8 +608,10 @@ void launch_extend(mpt_context *ctx, hipStream_t st, uint32_t cap, uint32_t g, c
   }
 }
 
-void launch_shadow(mpt_context *ctx, hipStream_t st, uint32_t g, const Scene &sc, const float4 *sq,
-                   const uint32_t *count, float4 *Ls) {
+void launch_shadow(mpt_context *ctx, hipStream_t st, uint64_t cap, uint32_t g, const Scene &sc,
+                   const float4 *sq, const uint32_t *count, float4 *Ls) {
+  // form 2: one ray per lane over the queue's capacity, as for closest hits
+  if (ctx->persistent == 2) g = (uint32_t)std::min<uint64_t>((cap + kBlock - 1) / kBlock, 0x7FFFFFFFull);
   switch (ctx->stack) {
     case 8: k_shadow<8><<<g, kBlock, 0, st>>>(sc, sq, count, Ls); break;
     case 12: k_shadow<12><<<g, kBlock, 0, st>>>(sc, sq, count, Ls); break;
@@ -639,7 +647,7 @@ int enqueue_batch(mpt_context *ctx, Lane &ln, mpt_scene *s, uint32_t D, bool ins
     return MPT_OK;
   };
   k_reset<<<1, 64, 0, st>>>(w.counts.p, 3 * D + 1);
-  k_raygen<<<gcap, 256, 0, st>>>(w.desc.p, w.q[0].p, w.Ls.p, w.betas.p, w.flags.p, w.pjs.p, w.counts.p);
+  k_raygen<<<gcap, 256, 0, st>>>(w.desc.p, w.q[0].p, w.pjs.p, w.counts.p);
   MPT_LAUNCHED(ctx, "k_raygen");
   for (uint32_t d = 0; d < D; ++d) {
     float4 *qin = w.q[d & 1].p, *qout = w.q[(d + 1) & 1].p;
@@ -649,12 +657,12 @@ int enqueue_batch(mpt_context *ctx, Lane &ln, mpt_scene *s, uint32_t D, bool ins
     if ((rc = toc())) return rc;
     if ((rc = tic(1))) return rc;
     k_shade<<<(g * kBlock + kShadeBlock - 1) / kShadeBlock, kShadeBlock, 0, st>>>(s->dev, w.desc.p, d, qin, w.hits.p, w.counts.p + d, qout,
-                                           w.counts.p + d + 1, w.sq.p, w.counts.p + D + 1 + d, w.Ls.p, w.betas.p,
+                                           w.counts.p + d + 1, w.sq.p, w.counts.p + D + 1 + d, w.Ls.p,
                                            w.flags.p, w.pjs.p);
     if ((rc = toc())) return rc;
     if (L) {
       if ((rc = tic(2))) return rc;
-      launch_shadow(ctx, st, g, s->dev, w.sq.p, w.counts.p + D + 1 + d, w.Ls.p);
+      launch_shadow(ctx, st, (uint64_t)cap * L, g, s->dev, w.sq.p, w.counts.p + D + 1 + d, w.Ls.p);
       if ((rc = toc())) return rc;
     }
     MPT_LAUNCHED(ctx, "depth kernels");
@@ -797,11 +805,10 @@ int ensure_workspace(mpt_context *ctx, uint32_t k, uint32_t cap, uint32_t depth,
   if (cap == w.cap && depth == w.depth && lights == w.lights && w.desc.p) return MPT_OK;
   MPT_HIP(ctx, hipStreamSynchronize(ln.stream));
   const uint64_t ns = (uint64_t)cap * std::max<uint32_t>(lights, 1);
-  MPT_HIP(ctx, w.q[0].reserve(2ull * cap));
-  MPT_HIP(ctx, w.q[1].reserve(2ull * cap));
+  MPT_HIP(ctx, w.q[0].reserve(3ull * cap));
+  MPT_HIP(ctx, w.q[1].reserve(3ull * cap));
   MPT_HIP(ctx, w.hits.reserve(cap));
   MPT_HIP(ctx, w.Ls.reserve(cap));
-  MPT_HIP(ctx, w.betas.reserve(cap));
   MPT_HIP(ctx, w.flags.reserve(cap));
   MPT_HIP(ctx, w.pjs.reserve(cap));
   MPT_HIP(ctx, w.sq.reserve(3ull * ns));
