@@ -26,7 +26,7 @@ sys.path.insert(0, ROOT)
 WIDTH, HEIGHT, SPP, DEPTH = 1920, 1080, 64, 4
 TRIANGLES = 1_000_000
 HBM_PEAK_GBS = 8000.0
-NODE_BYTES, RECORD_BYTES, RAY_BYTES, HIT_BYTES = 8, 48, 32, 16
+NODE_BYTES, RECORD_BYTES, RAY_BYTES, HIT_BYTES = 8, 48, 32, 8
 TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "hbm_traffic_soup1M_1920x1080_64spp.md")
 
 

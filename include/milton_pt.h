@@ -163,9 +163,9 @@ enum {
   MPT_OPT_GRAPHS = 4,      /* 1 (default): replay each batch's kernels from a captured hipGraph */
   MPT_OPT_STACK = 5,       /* traversal short-stack entries per ray: 8 (default), 12, 16 or 24 */
   MPT_OPT_PERSISTENT = 6,  /* closest-hit kernel form: 0 one ray per lane, grid-stride over 8
-                              blocks per CU; 1 persistent waves that refill finished lanes from a
-                              per-wave range of the queue; 2 (default) one ray per lane, one grid
-                              slot per queue slot */
+                              blocks per CU; 1 (default) persistent waves that refill finished
+                              lanes from 256-ray chunks of the queue each wave claims; 2 one ray
+                              per lane, one grid slot per queue slot (also the shadow-ray form) */
   MPT_OPT_LANES = 7        /* streams (each with its own batch workspace) that consecutive batches
                               alternate over, 1..4, default 3: one batch's kernel tails overlap
                               another's bulk */

@@ -156,21 +156,6 @@ MPT_HD bool tri_td(const RaySetup &r, float3 v0, float3 v1, float3 v2, float &T,
   return true;
 }
 
-// Barycentric weights of v1 and v2 at the ray's hit on the triangle (the
-// weight of v0 is 1 minus both).  They equal the reference's ratios of
-// unsigned sub-triangle areas at the hit point (MeshTriangle.cpp:344-361)
-// for a point on the triangle.
-MPT_HD float2 tri_bary(const RaySetup &r, float3 v0, float3 v1, float3 v2) {
-  const TriEdges e = tri_edges(r, v0, v1, v2);
-  const float inv = 1.0f / (e.U + e.V + e.W);
-  return make_float2(e.V * inv, e.W * inv);
-}
-
-MPT_HD float tri_t(const RaySetup &r, float3 v0, float3 v1, float3 v2) {
-  float T, det;
-  return tri_td(r, v0, v1, v2, T, det) ? T / det : INFINITY;
-}
-
 // Sphere (centre c, radius rad): nearest root above EPSILON; a ray leaving the
 // sphere it starts on skips the root at its own origin.
 MPT_HD float sphere_t(const RaySetup &r, float3 c, float rad, bool from_self) {
@@ -207,10 +192,15 @@ MPT_HD float quad_t(const RaySetup &r, float3 c, float3 ex, float3 ez) {
 // solve per ray), so t matches the reference's FP64 intersect
 // (MeshTriangle.cpp:92-169 computes t in double) even for hits very close to
 // the ray origin, where the FP32 edge-function t loses relative accuracy.
-MPT_HD float refine_t(const float4 *rec, const RaySetup &r, float t32) {
+// With uv, a triangle hit also gets its barycentric weights of v1 and v2 at
+// the refined hit point (the weight of v0 is 1 minus both); for a point on
+// the triangle they equal the reference's ratios of unsigned sub-triangle
+// areas (MeshTriangle.cpp:344-361).  Other kinds report (0, 0).
+MPT_HD float refine_hit(const float4 *rec, float3 o, float3 d, float t32, float2 *uv = nullptr) {
   const float4 a = rec[0], b = rec[1], c = rec[2];
   const uint32_t kind = mpt_f2u(b.w);
-  const double ox = r.o.x, oy = r.o.y, oz = r.o.z, dx = r.d.x, dy = r.d.y, dz = r.d.z;
+  const double ox = o.x, oy = o.y, oz = o.z, dx = d.x, dy = d.y, dz = d.z;
+  if (uv) *uv = make_float2(0.0f, 0.0f);
   if (kind == KIND_SPHERE) {
     const double fx = ox - a.x, fy = oy - a.y, fz = oz - a.z, rad = b.x;
     const double bb = fx * dx + fy * dy + fz * dz, dd = dx * dx + dy * dy + dz * dz;
@@ -233,7 +223,16 @@ MPT_HD float refine_t(const float4 *rec, const RaySetup &r, float t32) {
   const double nx = e1y * e2z - e1z * e2y, ny = e1z * e2x - e1x * e2z, nz = e1x * e2y - e1y * e2x;
   const double dn = dx * nx + dy * ny + dz * nz;
   if (dn == 0.0) return t32;
-  return (float)((((double)a.x - ox) * nx + ((double)a.y - oy) * ny + ((double)a.z - oz) * nz) / dn);
+  const double t = (((double)a.x - ox) * nx + ((double)a.y - oy) * ny + ((double)a.z - oz) * nz) / dn;
+  if (uv && kind == KIND_TRIANGLE) {
+    // p - v0 = b1 e1 + b2 e2  =>  b1 = ((p - v0) x e2).n / n.n, b2 = (e1 x (p - v0)).n / n.n
+    const double wx = ox + t * dx - a.x, wy = oy + t * dy - a.y, wz = oz + t * dz - a.z;
+    const double nn = nx * nx + ny * ny + nz * nz;
+    const double b1 = ((wy * e2z - wz * e2y) * nx + (wz * e2x - wx * e2z) * ny + (wx * e2y - wy * e2x) * nz) / nn;
+    const double b2 = ((e1y * wz - e1z * wy) * nx + (e1z * wx - e1x * wz) * ny + (e1x * wy - e1y * wx) * nz) / nn;
+    *uv = make_float2((float)b1, (float)b2);
+  }
+  return (float)t;
 }
 
 // kd traversal with a STACK-entry ring stack in LDS and kd-restart on
@@ -358,8 +357,14 @@ struct Walker {
   }
 
   // Result of a finished walk: the closest hit's t is recomputed in double.
-  // With uv, a triangle hit also reports its barycentric weights of v1, v2
-  // (tri_bary); other hits report (0, 0).
+  // The closest hit as found: the FP32 t and the leaf record, or kMiss.
+  MPT_HD void raw(float &t32, uint32_t &rec) const {
+    t32 = best_rec == kMiss ? INFINITY : best;
+    rec = best_rec;
+  }
+
+  // Result of a finished walk: the closest hit's t recomputed in double and
+  // the primitive id (refine_hit); with uv also its barycentrics.
   MPT_HD bool result(const Scene &sc, const RaySetup &r, float &t_out, uint32_t &id_out,
                      float2 *uv = nullptr) const {
     if (best_rec == kMiss) {
@@ -369,12 +374,8 @@ struct Walker {
       return false;
     }
     const float4 *rec = sc.recs + 3ull * best_rec;
-    t_out = ANY ? best : refine_t(rec, r, best);
+    t_out = ANY ? best : refine_hit(rec, r.o, r.d, best, uv);
     id_out = mpt_f2u(rec[0].w);
-    if (uv) {
-      const float4 a = rec[0], b = rec[1], c = rec[2];
-      *uv = mpt_f2u(b.w) == KIND_TRIANGLE ? tri_bary(r, xyz(a), xyz(b), xyz(c)) : make_float2(0.0f, 0.0f);
-    }
     return true;
   }
 };

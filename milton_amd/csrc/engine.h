@@ -86,7 +86,7 @@ struct Workspace {
   uint32_t cap = 0, depth = 0, lights = 0;
   uint64_t generation = 0;   // bumped on every reallocation (invalidates graphs)
   DBuf<float4> q[2];
-  DBuf<float4> hits;         // (t, prim, barycentric weights of v1 and v2)
+  DBuf<float2> hits;         // (FP32 t, leaf record of the closest hit or MPT_MISS)
   DBuf<float4> sq;
   DBuf<float4> Ls;
   DBuf<uint32_t> flags;
@@ -151,7 +151,7 @@ struct mpt_context {
   bool counters = false;
   bool graphs = true;
   uint32_t stack = 8;                // short-stack entries of the traversal kernels
-  uint32_t persistent = 2;           // closest-hit kernel form: 0 grid-stride, 1 persistent waves with
+  uint32_t persistent = 1;           // closest-hit kernel form: 0 grid-stride, 1 persistent waves with
                                      // lane refill, 2 one ray per lane over a capacity-sized grid
   std::vector<hipEvent_t> events;    // timing pool, pairs per launch
   std::vector<mpt_scene *> scenes;   // live scenes, films and servers, released by mpt_close

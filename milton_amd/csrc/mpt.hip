@@ -139,7 +139,7 @@ __global__ __launch_bounds__(256) void k_raygen(const BatchDesc *__restrict__ bd
 // the kernel also accumulates the kd nodes and references it visited.
 template <bool COUNT, int STACK>
 __global__ __launch_bounds__(256) void k_extend(Scene sc, const float4 *__restrict__ q,
-                                                const uint32_t *__restrict__ count, float4 *__restrict__ hits,
+                                                const uint32_t *__restrict__ count, float2 *__restrict__ hits,
                                                 unsigned long long *__restrict__ counters) {
   __shared__ uint2 stack[(STACK + 1) * kBlock];
   uint2 *stk = stack + threadIdx.x;
@@ -148,11 +148,15 @@ __global__ __launch_bounds__(256) void k_extend(Scene sc, const float4 *__restri
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
     const float4 a = q[3 * i], b = q[3 * i + 1];
     const RaySetup r = setup_ray(xyz(a), xyz(b));
-    float t;
-    uint32_t id;
-    float2 uv;
-    trace<false, COUNT, STACK>(sc, r, INFINITY, __float_as_uint(b.w), stk, t, id, &n_nodes, &n_refs, &uv);
-    hits[i] = make_float4(t, __uint_as_float(id), uv.x, uv.y);
+    Walker<false, STACK> w;
+    float t = INFINITY;
+    uint32_t rec = kMiss;
+    if (w.start(sc, r, INFINITY, __float_as_uint(b.w))) {
+      while (!w.template step<COUNT>(sc, r, stk, &n_nodes, &n_refs)) {
+      }
+      w.raw(t, rec);
+    }
+    hits[i] = make_float2(t, __uint_as_float(rec));
   }
   if (COUNT) {
     atomicAdd(&counters[0], (unsigned long long)n_nodes);
@@ -170,7 +174,7 @@ constexpr uint32_t kChunk = 256;
 
 template <bool COUNT, int STACK>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(STACK <= 8 ? 8 : (STACK <= 16 ? 4 : 2)))) void k_extend_persistent(
-    Scene sc, const float4 *__restrict__ q, const uint32_t *__restrict__ count, float4 *__restrict__ hits,
+    Scene sc, const float4 *__restrict__ q, const uint32_t *__restrict__ count, float2 *__restrict__ hits,
     unsigned long long *__restrict__ counters, uint32_t *__restrict__ fetch) {
   __shared__ uint2 stack[(STACK + 1) * kBlock];
   uint2 *stk = stack + threadIdx.x;
@@ -204,7 +208,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(STACK <= 8 
           if (w.start(sc, r, INFINITY, __float_as_uint(b.w))) {
             ray = idx;
           } else {
-            hits[idx] = make_float4(INFINITY, __uint_as_float(kMiss), 0.0f, 0.0f);
+            hits[idx] = make_float2(INFINITY, __uint_as_float(kMiss));
           }
         }
       }
@@ -215,10 +219,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(STACK <= 8 
     }
     if (ray != kMiss && w.template step<COUNT>(sc, r, stk, &n_nodes, &n_refs)) {
       float t;
-      uint32_t id;
-      float2 uv;
-      w.result(sc, r, t, id, &uv);
-      hits[ray] = make_float4(t, __uint_as_float(id), uv.x, uv.y);
+      uint32_t rec;
+      w.raw(t, rec);
+      hits[ray] = make_float2(t, __uint_as_float(rec));
       ray = kMiss;
     }
   }
@@ -293,7 +296,7 @@ __device__ __forceinline__ HitInfo hit_info(const Scene &sc, uint32_t prim, floa
 constexpr int kShadeBlock = 512;
 
 __global__ __launch_bounds__(kShadeBlock) __attribute__((amdgpu_waves_per_eu(6))) void k_shade(Scene sc, const BatchDesc *__restrict__ bd, uint32_t depth,
-                                               const float4 *__restrict__ qin, const float4 *__restrict__ hits,
+                                               const float4 *__restrict__ qin, const float2 *__restrict__ hits,
                                                const uint32_t *__restrict__ count_in, float4 *__restrict__ qout,
                                                uint32_t *__restrict__ count_out, float4 *__restrict__ sq,
                                                uint32_t *__restrict__ count_shadow, float4 *__restrict__ Ls,
@@ -322,7 +325,7 @@ __global__ __launch_bounds__(kShadeBlock) __attribute__((amdgpu_waves_per_eu(6))
     if (active) {
       // (origin, slot) (direction, origin primitive) (throughput, flags)
       const float4 a = qin[3 * i], b = qin[3 * i + 1], c = qin[3 * i + 2];
-      const float4 h = hits[i];
+      const float2 h = hits[i];
       slot = __float_as_uint(a.w);
       d = xyz(b);
       const uint2 pj = pjs[slot];
@@ -334,17 +337,22 @@ __global__ __launch_bounds__(kShadeBlock) __attribute__((amdgpu_waves_per_eu(6))
       max_depth = job.max_depth;
       beta = xyz(c);
       fl = __float_as_uint(c.w);
-      prim = __float_as_uint(h.y);
-      if (prim == kMiss) {
+      const uint32_t rec = __float_as_uint(h.y);
+      if (rec == kMiss) {
         // Scene::getBackgroundRadiance (core/Scene.cpp:116-118), every depth.
         Lv.x += beta.x * sc.env[0];
         Lv.y += beta.y * sc.env[1];
         Lv.z += beta.z * sc.env[2];
       } else {
+        // the closest hit's record: primitive id, and t and barycentrics
+        // recomputed in double precision
+        const float4 *rp = sc.recs + 3ull * rec;
+        prim = __float_as_uint(rp[0].w);
+        float2 uv;
         surface = true;
-        thit = h.x;
-        p = add(xyz(a), mul(d, h.x));
-        hi = hit_info(sc, prim, p, make_float2(h.z, h.w));
+        thit = refine_hit(rp, xyz(a), d, h.x, &uv);
+        p = add(xyz(a), mul(d, thit));
+        hi = hit_info(sc, prim, p, uv);
       }
     }
     mpt_material m;
@@ -579,7 +587,7 @@ std::vector<Timed> g_timed;   // guarded by the single-threaded use of a context
 
 template <int STACK>
 void launch_extend_t(mpt_context *ctx, hipStream_t st, uint32_t cap, uint32_t g, const Scene &sc, const float4 *q,
-                     const uint32_t *count, float4 *hits, unsigned long long *counters, uint32_t *fetch) {
+                     const uint32_t *count, float2 *hits, unsigned long long *counters, uint32_t *fetch) {
   if (ctx->persistent == 1) {
     if (counters)
       k_extend_persistent<true, STACK><<<g, kBlock, 0, st>>>(sc, q, count, hits, counters, fetch);
@@ -599,7 +607,7 @@ void launch_extend_t(mpt_context *ctx, hipStream_t st, uint32_t cap, uint32_t g,
 // The short-stack depth is a run-time choice among compiled variants (it
 // trades LDS per wave, hence occupancy, against kd-restarts).
 void launch_extend(mpt_context *ctx, hipStream_t st, uint32_t cap, uint32_t g, const Scene &sc, const float4 *q,
-                   const uint32_t *count, float4 *hits, unsigned long long *counters, uint32_t *fetch) {
+                   const uint32_t *count, float2 *hits, unsigned long long *counters, uint32_t *fetch) {
   switch (ctx->stack) {
     case 8: launch_extend_t<8>(ctx, st, cap, g, sc, q, count, hits, counters, fetch); break;
     case 12: launch_extend_t<12>(ctx, st, cap, g, sc, q, count, hits, counters, fetch); break;
