@@ -15,7 +15,9 @@ import weakref
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "libmilton_pt.so")
+# MPT_LIBRARY names another build of the same library (for example one
+# compiled with different kernel tunables); the in-tree build by default.
+LIB_PATH = os.environ.get("MPT_LIBRARY") or os.path.join(_HERE, "libmilton_pt.so")
 
 MPT_ABI_VERSION = 1
 MPT_MISS = 0xFFFFFFFF

@@ -169,8 +169,14 @@ __global__ __launch_bounds__(256) void k_extend(Scene sc, const float4 *__restri
 // taken when at least kRefill lanes are idle, or all are); a wave claims
 // kChunk rays of the queue at a time with one atomic, so the claims balance
 // the load across waves without one atomic per refill.
-constexpr uint32_t kRefill = 16;
-constexpr uint32_t kChunk = 256;
+#ifndef MPT_REFILL
+#define MPT_REFILL 16
+#endif
+#ifndef MPT_CHUNK
+#define MPT_CHUNK 256
+#endif
+constexpr uint32_t kRefill = MPT_REFILL;   // idle lanes that trigger a refill
+constexpr uint32_t kChunk = MPT_CHUNK;     // rays per claim
 
 template <bool COUNT, int STACK>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(STACK <= 8 ? 8 : (STACK <= 16 ? 4 : 2)))) void k_extend_persistent(
