@@ -157,14 +157,14 @@ typedef struct mpt_render_stats {
 /* Context options (mpt_set_option).  No reference counterpart: these are the
  * knobs of the device path (queue depth) and its instrumentation. */
 enum {
-  MPT_OPT_BATCH_PATHS = 1, /* paths per wavefront batch (queue depth), default 2^24 */
+  MPT_OPT_BATCH_PATHS = 1, /* paths per wavefront batch (queue depth), default 2^25 */
   MPT_OPT_TIMING = 2,      /* 1: time every kernel launch with HIP events into *_ms */
   MPT_OPT_COUNTERS = 3,    /* 1: count kd nodes and references visited by closest-hit rays */
   MPT_OPT_GRAPHS = 4,      /* 1 (default): replay each batch's kernels from a captured hipGraph */
   MPT_OPT_STACK = 5,       /* traversal short-stack entries per ray: 8 (default), 12, 16 or 24 */
   MPT_OPT_PERSISTENT = 6,  /* closest-hit kernel form: 0 one ray per lane, grid-stride over 8
                               blocks per CU; 1 (default) persistent waves that refill finished
-                              lanes from 256-ray chunks of the queue each wave claims; 2 one ray
+                              lanes from 128-ray chunks of the queue each wave claims; 2 one ray
                               per lane, one grid slot per queue slot (also the shadow-ray form) */
   MPT_OPT_LANES = 7        /* streams (each with its own batch workspace) that consecutive batches
                               alternate over, 1..4, default 3: one batch's kernel tails overlap

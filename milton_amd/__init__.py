@@ -368,7 +368,7 @@ class Scene:
         ctx._children.add(self)
 
     @classmethod
-    def tuned(cls, ctx, arrays, job, sample_paths=1 << 24, kd=None):
+    def tuned(cls, ctx, arrays, job, sample_paths=1 << 26, kd=None):
         """Builds the scene at the fastest kd leaf size for `job` and leaves the
         fastest engine settings on ctx (mpt_tune_scene); returns (scene, result)."""
         d = arrays.desc()
@@ -379,7 +379,7 @@ class Scene:
                                        ctypes.byref(h), ctypes.byref(res)), "mpt_tune_scene")
         return cls(ctx, arrays, _handle=h), res.as_dict()
 
-    def tune(self, job, sample_paths=1 << 24):
+    def tune(self, job, sample_paths=1 << 26):
         """Engine settings for this scene and job (mpt_tune), left on the context."""
         res = TuneResult()
         self.ctx.check(self.ctx.L.mpt_tune(self.h, ctypes.byref(job), sample_paths, ctypes.byref(res)), "mpt_tune")

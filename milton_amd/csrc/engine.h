@@ -146,7 +146,7 @@ struct mpt_context {
   uint32_t num_lanes = 3;            // lanes batches alternate over
   uint32_t next_lane = 0;
   hipEvent_t mark = nullptr;         // fork point on the context stream
-  uint32_t batch_paths = 1u << 24;   // paths per batch (queue depth)
+  uint32_t batch_paths = 1u << 25;   // paths per batch (queue depth)
   bool timing = false;
   bool counters = false;
   bool graphs = true;

@@ -168,12 +168,13 @@ __global__ __launch_bounds__(256) void k_extend(Scene sc, const float4 *__restri
 // lane whose ray finished the next ray of the wave's claimed chunk (refill
 // taken when at least kRefill lanes are idle, or all are); a wave claims
 // kChunk rays of the queue at a time with one atomic, so the claims balance
-// the load across waves without one atomic per refill.
+// the load across waves without one atomic per refill.  24 and 128 were the
+// fastest of 8-32 and 64-1024 on the 1M-triangle soup.
 #ifndef MPT_REFILL
-#define MPT_REFILL 16
+#define MPT_REFILL 24
 #endif
 #ifndef MPT_CHUNK
-#define MPT_CHUNK 256
+#define MPT_CHUNK 128
 #endif
 constexpr uint32_t kRefill = MPT_REFILL;   // idle lanes that trigger a refill
 constexpr uint32_t kChunk = MPT_CHUNK;     // rays per claim

@@ -8,7 +8,8 @@
 //   persistent  closest-hit kernel form (one ray per lane, or persistent
 //               waves with lane refill),
 //   stack       short-stack entries of the kd walk (8, 12, 16),
-//   batch paths queue depth of a wavefront batch (2^20 .. 2^24),
+//   batch paths queue depth of a wavefront batch (2^20 .. 2^26; a sample
+//               smaller than a candidate runs as one batch),
 //   leaf size   kd leaf size (1 .. 16, mpt_tune_scene only; rebuilds the tree).
 // The sample is the job's own camera, depth and seed at a reduced number of
 // samples per pixel over a centred window of pixels, about sample_paths paths.
@@ -107,7 +108,7 @@ int tune_engine(mpt_scene *scene, const Sample &smp, mpt_film *film, mpt_tune_re
     s.stack = k;
     if ((rc = trial(s))) return rc;
   }
-  for (uint32_t b = 20; b <= 24; ++b) {
+  for (uint32_t b = 20; b <= 26; ++b) {
     Settings s = best;
     s.batch_paths = 1u << b;
     if ((rc = trial(s))) return rc;

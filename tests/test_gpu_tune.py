@@ -36,7 +36,7 @@ def test_tune_engine_settings_keep_results(ctx):
         print("tune:", res, flush=True)
         assert res["stack"] in (8, 12, 16)
         assert res["persistent"] in (0, 1, 2)
-        assert res["batch_paths"] == saved[m.MPT_OPT_BATCH_PATHS] or res["batch_paths"] in [1 << b for b in range(20, 25)]
+        assert res["batch_paths"] == saved[m.MPT_OPT_BATCH_PATHS] or res["batch_paths"] in [1 << b for b in range(20, 27)]
         assert res["trials"] >= 2
         assert res["paths_per_s"] >= res["baseline_paths_per_s"] > 0
         assert ctx.get_option(m.MPT_OPT_STACK) == res["stack"]
