@@ -34,13 +34,14 @@
 #ifndef MILTON_PT_H_
 #define MILTON_PT_H_
 
+#include <stddef.h>
 #include <stdint.h>
 
 #ifdef __cplusplus
 extern "C" {
 #endif
 
-#define MPT_ABI_VERSION 1u
+#define MPT_ABI_VERSION 2u
 #define MPT_MISS 0xFFFFFFFFu
 
 enum {
@@ -90,7 +91,19 @@ typedef struct mpt_scene_desc {
   uint32_t num_materials;
   const mpt_material *materials;
   float env_radiance[3]; /* constant background radiance (Scene.cpp:116-118) */
+  /* Optional (NULL = all two-sided): per-triangle flags.  MPT_TRI_ONE_SIDED
+   * marks the reference's standalone Triangle, which culls back faces:
+   * a ray with e1 . (d x e2) < EPSILON misses it (shapes/Triangle.cpp:245-270);
+   * mesh triangles are two-sided (shapes/MeshTriangle.cpp:142-167). */
+  const uint32_t *tri_flags;
+  /* Optional (NULL = none): per-vertex normals (3 floats) for the vertices
+   * whose vertex_normal_mask entry is nonzero (a mesh file's normals,
+   * shapes/Mesh.cpp:335-377 computes the others); both or neither. */
+  const float *vertex_normals;
+  const uint8_t *vertex_normal_mask;
 } mpt_scene_desc;
+
+enum { MPT_TRI_ONE_SIDED = 1u };
 
 /* Replaces kdTreeAccel::BuildParams (accel/kdTreeAccel.h:68-116). */
 typedef struct mpt_kd_params {
@@ -124,6 +137,10 @@ typedef struct mpt_job {
   mpt_camera camera;
   uint32_t width, height, spp, max_depth;
   uint32_t seed, job_id;
+  /* light samples per light per non-specular vertex (DirectIllumination's
+   * noDirectSamples, renderers/DirectIllumination.cpp:33,98-133), jittered
+   * over floor(sqrt(n))^2 strata of the light; 0 means 1 */
+  uint32_t direct_samples;
 } mpt_job;
 
 typedef struct mpt_ray {
@@ -240,6 +257,22 @@ int mpt_film_download(mpt_film *film, float *film_rgb);
  * Test hook: the value PathTracer::_evaluate returns for each camera ray. */
 int mpt_render_paths(mpt_scene *scene, const mpt_job *job, uint32_t pix_begin, uint32_t pix_end,
                      float *L_out);
+
+/* Scene files.  Replaces MiltonJSONSceneLoader::parse
+ * (loaders/MiltonJSONSceneLoader.cpp:133-337) for the shapes, materials,
+ * camera, output and renderer keys this path renders: the scene graph is
+ * flattened into an mpt_scene_desc (owned by the mpt_scenefile) and the
+ * camera, film size, samples per pixel, depth cap and seed into an mpt_job.
+ * Unsupported elements fail with a message in err instead of being dropped.
+ * Host only; no context needed.  Relative mesh paths resolve against the
+ * scene file's directory (mpt_scenefile_load) or base_dir (parse). */
+typedef struct mpt_scenefile mpt_scenefile;
+int mpt_scenefile_load(const char *path, mpt_scenefile **scenefile, char *err, size_t err_len);
+int mpt_scenefile_parse(const char *text, const char *base_dir, mpt_scenefile **scenefile, char *err,
+                        size_t err_len);
+const mpt_scene_desc *mpt_scenefile_desc(const mpt_scenefile *scenefile);
+int mpt_scenefile_job(const mpt_scenefile *scenefile, mpt_job *job);
+int mpt_scenefile_destroy(mpt_scenefile *scenefile);
 
 /* Autotuning.  No reference counterpart: the reference's only knobs on this
  * path are kdTreeAccel::BuildParams (accel/kdTreeAccel.h:68-116), fixed per

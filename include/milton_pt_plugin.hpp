@@ -72,6 +72,9 @@ class PathTracer {
   std::vector<float> verts_, spheres_, quads_, film_;
   std::vector<uint32_t> idx_, tri_mat_, sph_mat_, quad_mat_;
   std::vector<mpt_material> mats_;
+  std::vector<uint32_t> tri_flags_;
+  std::vector<float> vnormals_;
+  std::vector<uint8_t> vmask_;
   mpt_scene_desc desc_{};
   mpt_camera camera_{};
   unsigned width_ = 480, height_ = 480;

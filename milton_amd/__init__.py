@@ -19,7 +19,8 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 # compiled with different kernel tunables); the in-tree build by default.
 LIB_PATH = os.environ.get("MPT_LIBRARY") or os.path.join(_HERE, "libmilton_pt.so")
 
-MPT_ABI_VERSION = 1
+MPT_ABI_VERSION = 2
+MPT_TRI_ONE_SIDED = 1
 MPT_MISS = 0xFFFFFFFF
 MPT_BSDF_DIFFUSE, MPT_BSDF_PHONG, MPT_BSDF_DIELECTRIC = 0, 1, 2
 MPT_SCENE_ROOM, MPT_SCENE_SOUP, MPT_SCENE_TERRAIN = 1, 2, 3
@@ -29,6 +30,7 @@ ERRORS = {-1: "invalid", -2: "device", -3: "nomem", -4: "budget", -5: "notfound"
 
 _f32p = ctypes.POINTER(ctypes.c_float)
 _u32p = ctypes.POINTER(ctypes.c_uint32)
+_u8p = ctypes.POINTER(ctypes.c_uint8)
 
 
 class Material(ctypes.Structure):
@@ -44,7 +46,8 @@ class SceneDesc(ctypes.Structure):
                 ("num_spheres", ctypes.c_uint32), ("spheres", _f32p), ("sphere_material", _u32p),
                 ("num_quads", ctypes.c_uint32), ("quads", _f32p), ("quad_material", _u32p),
                 ("num_materials", ctypes.c_uint32), ("materials", ctypes.POINTER(Material)),
-                ("env_radiance", ctypes.c_float * 3)]
+                ("env_radiance", ctypes.c_float * 3), ("tri_flags", _u32p), ("vertex_normals", _f32p),
+                ("vertex_normal_mask", _u8p)]
 
 
 class KdParams(ctypes.Structure):
@@ -63,7 +66,7 @@ class Camera(ctypes.Structure):
 class Job(ctypes.Structure):
     _fields_ = [("camera", Camera), ("width", ctypes.c_uint32), ("height", ctypes.c_uint32),
                 ("spp", ctypes.c_uint32), ("max_depth", ctypes.c_uint32),
-                ("seed", ctypes.c_uint32), ("job_id", ctypes.c_uint32)]
+                ("seed", ctypes.c_uint32), ("job_id", ctypes.c_uint32), ("direct_samples", ctypes.c_uint32)]
 
 
 class Ray(ctypes.Structure):
@@ -128,6 +131,8 @@ def lib():
     if not os.path.exists(LIB_PATH):
         raise MptError(f"{LIB_PATH} is missing: build it with `python -m milton_amd.build`")
     L = ctypes.CDLL(LIB_PATH)
+    if L.mpt_abi_version() != MPT_ABI_VERSION:
+        raise MptError(f"{LIB_PATH} has ABI {L.mpt_abi_version()}, this binding {MPT_ABI_VERSION}: rebuild it")
     vp, u32, u64, i32 = ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint64, ctypes.c_int
     sig = {
         "mpt_abi_version": ([], i32),
@@ -164,6 +169,12 @@ def lib():
         "mpt_tune": ([vp, ctypes.POINTER(Job), u64, ctypes.POINTER(TuneResult)], i32),
         "mpt_tune_scene": ([vp, ctypes.POINTER(SceneDesc), ctypes.POINTER(KdParams), ctypes.POINTER(Job), u64,
                             ctypes.POINTER(vp), ctypes.POINTER(TuneResult)], i32),
+        "mpt_scenefile_load": ([ctypes.c_char_p, ctypes.POINTER(vp), ctypes.c_char_p, ctypes.c_size_t], i32),
+        "mpt_scenefile_parse": ([ctypes.c_char_p, ctypes.c_char_p, ctypes.POINTER(vp), ctypes.c_char_p,
+                                 ctypes.c_size_t], i32),
+        "mpt_scenefile_desc": ([vp], ctypes.POINTER(SceneDesc)),
+        "mpt_scenefile_job": ([vp, ctypes.POINTER(Job)], i32),
+        "mpt_scenefile_destroy": ([vp], i32),
         "mpt_kd_build": ([ctypes.POINTER(SceneDesc), ctypes.POINTER(KdParams), ctypes.POINTER(vp)], i32),
         "mpt_kd_stats": ([vp, ctypes.POINTER(ctypes.c_uint64)], i32),
         "mpt_kd_trace": ([vp, vp, u64, vp, ctypes.POINTER(ctypes.c_uint64), u32], i32),
@@ -192,7 +203,7 @@ class SceneArrays:
     """Primitive and material arrays in the layout of mpt_scene_desc."""
 
     def __init__(self, vertices, indices, tri_material, spheres, sphere_material, quads, quad_material,
-                 materials, env, camera=None):
+                 materials, env, camera=None, tri_flags=None, vertex_normals=None, vertex_normal_mask=None):
         self.vertices = np.ascontiguousarray(vertices, np.float32).reshape(-1, 3)
         self.indices = np.ascontiguousarray(indices, np.uint32).reshape(-1, 3)
         self.tri_material = np.ascontiguousarray(tri_material, np.uint32).reshape(-1)
@@ -203,6 +214,12 @@ class SceneArrays:
         self.materials = np.ascontiguousarray(materials, MATERIAL_DTYPE).reshape(-1)
         self.env = np.ascontiguousarray(env, np.float32).reshape(3)
         self.camera = camera
+        # optional: MPT_TRI_ONE_SIDED per triangle; a mesh file's vertex normals
+        self.tri_flags = None if tri_flags is None else np.ascontiguousarray(tri_flags, np.uint32).reshape(-1)
+        self.vertex_normals = (None if vertex_normals is None
+                               else np.ascontiguousarray(vertex_normals, np.float32).reshape(-1, 3))
+        self.vertex_normal_mask = (None if vertex_normal_mask is None
+                                   else np.ascontiguousarray(vertex_normal_mask, np.uint8).reshape(-1))
 
     @property
     def num_prims(self):
@@ -226,6 +243,11 @@ class SceneArrays:
         d.materials = self.materials.ctypes.data_as(ctypes.POINTER(Material))
         for k in range(3):
             d.env_radiance[k] = float(self.env[k])
+        if self.tri_flags is not None:
+            d.tri_flags = self.tri_flags.ctypes.data_as(_u32p)
+        if self.vertex_normals is not None:
+            d.vertex_normals = self.vertex_normals.ctypes.data_as(_f32p)
+            d.vertex_normal_mask = self.vertex_normal_mask.ctypes.data_as(_u8p)
         return d
 
     def materials_f32(self):
@@ -296,7 +318,7 @@ def camera_dict(cam):
             "height_angle": cam.height_angle, "aspect": cam.aspect}
 
 
-def make_job(camera, width, height, spp, max_depth, seed=1, job_id=0):
+def make_job(camera, width, height, spp, max_depth, seed=1, job_id=0, direct_samples=1):
     j = Job()
     for k in range(3):
         j.camera.eye[k] = camera["eye"][k]
@@ -305,6 +327,7 @@ def make_job(camera, width, height, spp, max_depth, seed=1, job_id=0):
     j.camera.height_angle = camera["height_angle"]
     j.camera.aspect = camera["aspect"]
     j.width, j.height, j.spp, j.max_depth, j.seed, j.job_id = width, height, spp, max_depth, seed, job_id
+    j.direct_samples = direct_samples
     return j
 
 
@@ -573,3 +596,47 @@ class HostKdTree:
             self.close()
         except Exception:
             pass
+
+
+def _copy(ptr, count, dtype, cols=None):
+    if count == 0 or not ptr:
+        return np.zeros((0,) if cols is None else (0, cols), dtype)
+    n = count * (cols or 1)
+    arr = np.ctypeslib.as_array(ctypes.cast(ptr, ctypes.POINTER(np.ctypeslib.as_ctypes_type(dtype))), (n,))
+    arr = np.array(arr, dtype)
+    return arr if cols is None else arr.reshape(-1, cols)
+
+
+def load_scenefile(path=None, text=None, base_dir=None):
+    """A Milton scene file (mpt_scenefile_load / _parse) -> (SceneArrays, Job).
+    The arrays are copies; the job holds the file's camera, film size,
+    samples per pixel (noSuperSamples), depth cap and seed."""
+    L = lib()
+    h = ctypes.c_void_p()
+    err = ctypes.create_string_buffer(1024)
+    if text is not None:
+        rc = L.mpt_scenefile_parse(text.encode(), (base_dir or "").encode(), ctypes.byref(h), err, len(err))
+    else:
+        rc = L.mpt_scenefile_load(str(path).encode(), ctypes.byref(h), err, len(err))
+    if rc:
+        raise MptError(f"scene file: {err.value.decode(errors='replace')}")
+    try:
+        d = L.mpt_scenefile_desc(h).contents
+        nt, nv = d.num_triangles, d.num_vertices
+        mats = np.zeros(d.num_materials, MATERIAL_DTYPE)
+        ctypes.memmove(mats.ctypes.data, d.materials, mats.nbytes)
+        arrays = SceneArrays(
+            _copy(d.vertices, nv, np.float32, 3), _copy(d.indices, nt, np.uint32, 3),
+            _copy(d.tri_material, nt, np.uint32), _copy(d.spheres, d.num_spheres, np.float32, 4),
+            _copy(d.sphere_material, d.num_spheres, np.uint32), _copy(d.quads, d.num_quads, np.float32, 9),
+            _copy(d.quad_material, d.num_quads, np.uint32), mats, list(d.env_radiance),
+            tri_flags=_copy(d.tri_flags, nt, np.uint32) if d.tri_flags else None,
+            vertex_normals=_copy(d.vertex_normals, nv, np.float32, 3) if d.vertex_normals else None,
+            vertex_normal_mask=_copy(d.vertex_normal_mask, nv, np.uint8) if d.vertex_normal_mask else None)
+        job = Job()
+        L.mpt_scenefile_job(h, ctypes.byref(job))
+        arrays.camera = {"eye": list(job.camera.eye), "look": list(job.camera.look), "up": list(job.camera.up),
+                         "height_angle": job.camera.height_angle, "aspect": job.camera.aspect}
+        return arrays, job
+    finally:
+        L.mpt_scenefile_destroy(h)

@@ -28,7 +28,7 @@ COMMON = [
     "-I", os.path.join(ROOT, "include"),
 ]
 
-SOURCES = ["mpt.hip", "kdbuild.cpp", "scenegen.cpp", "serve.cpp", "tune.cpp"]
+SOURCES = ["mpt.hip", "kdbuild.cpp", "scenegen.cpp", "serve.cpp", "tune.cpp", "scenefile.cpp"]
 
 
 def _run(cmd):

@@ -39,7 +39,7 @@ constexpr int kBlock = 256;              // threads per block of every traversal
 constexpr int kStack = 16;               // default short-stack entries per lane (LDS ring)
 constexpr float kPi = 3.14159265358979323846f;
 
-enum { KIND_TRIANGLE = 0, KIND_SPHERE = 1, KIND_QUAD = 2 };
+enum { KIND_TRIANGLE = 0, KIND_SPHERE = 1, KIND_QUAD = 2, KIND_TRIANGLE_ONE_SIDED = 3 };
 
 // Read-only scene as seen by the kernels.
 struct Scene {
@@ -143,12 +143,16 @@ MPT_HD TriEdges tri_edges(const RaySetup &r, float3 v0, float3 v1, float3 v2) {
   return e;
 }
 
-MPT_HD bool tri_td(const RaySetup &r, float3 v0, float3 v1, float3 v2, float &T, float &det) {
+// one_sided: the reference's standalone Triangle, which culls back faces: it
+// rejects e1 . (d x e2) < EPSILON (Triangle.cpp:245-270), and that product
+// is (U + V + W) * |d[kz]| = (U + V + W) / |sz|.
+MPT_HD bool tri_td(const RaySetup &r, float3 v0, float3 v1, float3 v2, float &T, float &det,
+                   bool one_sided = false) {
   const TriEdges e = tri_edges(r, v0, v1, v2);
   const float U = e.U, V = e.V, W = e.W;
   if ((U < 0.0f || V < 0.0f || W < 0.0f) && (U > 0.0f || V > 0.0f || W > 0.0f)) return false;
   const float d = U + V + W;
-  if (fabsf(d) <= kEps * fabsf(r.sz)) return false;
+  if (one_sided ? d < kEps * fabsf(r.sz) : fabsf(d) <= kEps * fabsf(r.sz)) return false;
   const float t = (U * e.Az + V * e.Bz + W * e.Cz) * r.sz;
   // (-T) / (-det) == T / det exactly, so the sign moves onto T
   T = d < 0.0f ? -t : t;
@@ -213,7 +217,8 @@ MPT_HD float refine_hit(const float4 *rec, float3 o, float3 d, float t32, float2
     return (float)(fabs(t0 - t32) <= fabs(t1 - t32) ? t0 : t1);
   }
   double e1x, e1y, e1z, e2x, e2y, e2z;
-  if (kind == KIND_TRIANGLE) {
+  const bool tri = kind == KIND_TRIANGLE || kind == KIND_TRIANGLE_ONE_SIDED;
+  if (tri) {
     e1x = (double)b.x - a.x; e1y = (double)b.y - a.y; e1z = (double)b.z - a.z;
     e2x = (double)c.x - a.x; e2y = (double)c.y - a.y; e2z = (double)c.z - a.z;
   } else {   // quad: plane through the centre spanned by ex, ez
@@ -224,7 +229,7 @@ MPT_HD float refine_hit(const float4 *rec, float3 o, float3 d, float t32, float2
   const double dn = dx * nx + dy * ny + dz * nz;
   if (dn == 0.0) return t32;
   const double t = (((double)a.x - ox) * nx + ((double)a.y - oy) * ny + ((double)a.z - oz) * nz) / dn;
-  if (uv && kind == KIND_TRIANGLE) {
+  if (uv && tri) {
     // p - v0 = b1 e1 + b2 e2  =>  b1 = ((p - v0) x e2).n / n.n, b2 = (e1 x (p - v0)).n / n.n
     const double wx = ox + t * dx - a.x, wy = oy + t * dy - a.y, wz = oz + t * dz - a.z;
     const double nn = nx * nx + ny * ny + nz * nz;
@@ -316,12 +321,12 @@ struct Walker {
       const uint32_t id = mpt_f2u(a.w);
       const uint32_t kind = mpt_f2u(b.w);
       float t;
-      if (kind == KIND_TRIANGLE) {
+      if (kind == KIND_TRIANGLE || kind == KIND_TRIANGLE_ONE_SIDED) {
         float T, det;
         // divide only for candidates that can beat the current best; the
         // 2^-20 margins keep the prefilter conservative, the exact test
         // below decides
-        if (id == skip_origin || !tri_td(r, xyz(a), xyz(b), xyz(c), T, det) ||
+        if (id == skip_origin || !tri_td(r, xyz(a), xyz(b), xyz(c), T, det, kind == KIND_TRIANGLE_ONE_SIDED) ||
             T > best * det * (1.0f + 0x1p-20f) || T < kEps * det * (1.0f - 0x1p-20f))
           continue;
         t = T / det;

@@ -23,6 +23,7 @@
 namespace mpt_engine {
 
 constexpr uint32_t kMaxJobs = 256;      // jobs per batch
+constexpr uint32_t kMaxDirect = 64;     // light samples per light per vertex
 constexpr uint32_t kMaxChunks = 1024;   // pixel ranges per batch
 constexpr uint32_t kJobBits = 20;       // sample index bits in a path's (job, sample) word
 
@@ -34,7 +35,7 @@ struct CameraDev {
 struct JobDev {
   CameraDev cam;
   uint32_t width, height, spp, max_depth;
-  uint32_t seed, job_id, side, pad;
+  uint32_t seed, job_id, side, direct;   // direct: light samples per light (>= 1)
   float4 *film;   // W*H (sum r, g, b, samples)
 };
 
@@ -49,6 +50,7 @@ struct ChunkDev {
 
 struct BatchHdr {
   uint32_t n_chunks, n_paths, n_pixels, max_depth;
+  uint32_t max_direct, pad[3];   // most light samples per light of the batch's jobs
 };
 
 struct BatchDesc {
@@ -130,6 +132,7 @@ struct BatchPlan {
   std::vector<JobDev> jobs;
   std::vector<ChunkDev> chunks;
   uint32_t n_paths = 0, n_pixels = 0, max_depth = 0;
+  uint32_t max_direct = 1;   // most light samples per light of its jobs
 };
 
 }  // namespace mpt_engine

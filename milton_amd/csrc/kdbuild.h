@@ -10,7 +10,8 @@
 //            right child is child + 1), (count << 2) | 3 for a leaf;
 //            word1 = split position (float bits) or the leaf's first record;
 //   record = 12 x f32 per leaf reference, stored in leaf order so a leaf is a
-//            contiguous run: (v0, id), (v1, kind), (v2, 0) for a triangle,
+//            contiguous run: (v0, id), (v1, kind), (v2, 0) for a triangle
+//            (kind 0 two-sided, 3 one-sided),
 //            (centre, id), (r, 0, 0, kind), 0 for a sphere,
 //            (centre, id), (ex, kind), (ez, 0) for a quad.
 #ifndef MPT_KDBUILD_H_
@@ -23,7 +24,7 @@
 
 namespace mpt_kd {
 
-enum { KIND_TRIANGLE = 0, KIND_SPHERE = 1, KIND_QUAD = 2 };
+enum { KIND_TRIANGLE = 0, KIND_SPHERE = 1, KIND_QUAD = 2, KIND_TRIANGLE_ONE_SIDED = 3 };
 
 struct Box {
   float lo[3], hi[3];

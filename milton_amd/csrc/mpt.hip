@@ -329,6 +329,7 @@ __global__ __launch_bounds__(kShadeBlock) __attribute__((amdgpu_waves_per_eu(6))
     float thit = 0.0f;
     float3 d = f3(0, 0, 1);
     uint32_t pixel = 0, sample = 0, seed = 0, job_id = 0, max_depth = 0;
+    uint32_t ndir_job = 1, ndir_side = 1;   // light samples per light, strata per side
     if (active) {
       // (origin, slot) (direction, origin primitive) (throughput, flags)
       const float4 a = qin[3 * i], b = qin[3 * i + 1], c = qin[3 * i + 2];
@@ -342,6 +343,9 @@ __global__ __launch_bounds__(kShadeBlock) __attribute__((amdgpu_waves_per_eu(6))
       seed = job.seed;
       job_id = job.job_id;
       max_depth = job.max_depth;
+      ndir_job = job.direct;
+      ndir_side = 1;
+      while ((ndir_side + 1) * (ndir_side + 1) <= ndir_job) ++ndir_side;
       beta = xyz(c);
       fl = __float_as_uint(c.w);
       const uint32_t rec = __float_as_uint(h.y);
@@ -382,47 +386,56 @@ __global__ __launch_bounds__(kShadeBlock) __attribute__((amdgpu_waves_per_eu(6))
         Lv.z += beta.z * le.z * w;
       }
     }
-    // Next-event estimation: one shadow ray per light (DirectIllumination.cpp:
-    // 90-135), skipped at specular vertices (:57) and on the light itself (:93).
+    // Next-event estimation (DirectIllumination.cpp:90-135): per light, n
+    // shadow rays to points jittered over floor(sqrt(n))^2 strata of the
+    // light, each weighted area / n; skipped at specular vertices (:57) and
+    // on the light itself (:93).  Every lane of the block runs the same
+    // number of appends (the block's largest n), as block_append requires.
+    const uint32_t n_dir = bd->hdr.max_direct;
     for (uint32_t li = 0; li < sc.num_lights; ++li) {
-      bool want_sh = false;
-      float3 so = p, sd = f3(0, 0, 1), contrib = f3(0, 0, 0);
-      float sdist = 0.0f;
-      if (surface && !specular) {
-        const uint32_t q = sc.lights[li];
-        const uint32_t lgid = sc.nt + sc.ns + q;
-        if (lgid != prim) {
-          const float4 *lq = sc.quads + 4ull * q;
-          const float4 c0 = lq[0], ex = lq[1], ez = lq[2], ln = lq[3];
-          const mpt_rng::U4 r = mpt_rng::path_draw(seed, job_id, pixel, sample, depth, 1 + li);
-          const float u = mpt_rng::unitf(r.x), v = mpt_rng::unitf(r.y);
-          // Plane::getPosition (shapes/Plane.cpp:91-93)
-          const float3 lp = add(xyz(c0), add(mul(xyz(ex), u - 0.5f), mul(xyz(ez), v - 0.5f)));
-          float3 w = sub(lp, p);
-          const float dist = len(w);
-          w = mul(w, 1.0f / dist);
-          const float cos_wo = fabsf(dot(hi.n, w));
-          const float cos_wi = fabsf(dot(xyz(ln), w));
-          const float3 fr = bsdf_eval(m, hi.n, d, w);
-          // OrientedEmitter::getLe (emitters/OrientedEmitter.cpp:17-23)
-          const mpt_material lm = sc.mats[__float_as_uint(ln.w)];
-          const bool lit = dot(xyz(ln), w) < 0.0f;
-          if (cos_wo > 0.0f && cos_wi > 0.0f && !is_black(fr) && lit) {
-            const float g = cos_wo * cos_wi / (dist * dist) * c0.w;
-            contrib = mul(mul(beta, mul(fr, le_of(lm))), g);
-            if (!is_black(contrib)) {
-              want_sh = true;
-              sd = w;
-              sdist = dist * (1.0f - 1e-5f);
+      for (uint32_t k = 0; k < n_dir; ++k) {
+        bool want_sh = false;
+        float3 so = p, sd = f3(0, 0, 1), contrib = f3(0, 0, 0);
+        float sdist = 0.0f;
+        if (surface && !specular && k < ndir_job) {
+          const uint32_t q = sc.lights[li];
+          const uint32_t lgid = sc.nt + sc.ns + q;
+          if (lgid != prim) {
+            const float4 *lq = sc.quads + 4ull * q;
+            const float4 c0 = lq[0], ex = lq[1], ez = lq[2], ln = lq[3];
+            const mpt_rng::U4 r = mpt_rng::path_draw(seed, job_id, pixel, sample, depth, 1 + li * ndir_job + k);
+            const uint32_t side = ndir_side, cell = k % (side * side);
+            const float inv = 1.0f / (float)side;
+            const float u = ((float)(cell % side) + mpt_rng::unitf(r.x)) * inv;
+            const float v = ((float)(cell / side) + mpt_rng::unitf(r.y)) * inv;
+            // Plane::getPosition (shapes/Plane.cpp:91-93)
+            const float3 lp = add(xyz(c0), add(mul(xyz(ex), u - 0.5f), mul(xyz(ez), v - 0.5f)));
+            float3 w = sub(lp, p);
+            const float dist = len(w);
+            w = mul(w, 1.0f / dist);
+            const float cos_wo = fabsf(dot(hi.n, w));
+            const float cos_wi = fabsf(dot(xyz(ln), w));
+            const float3 fr = bsdf_eval(m, hi.n, d, w);
+            // OrientedEmitter::getLe (emitters/OrientedEmitter.cpp:17-23)
+            const mpt_material lm = sc.mats[__float_as_uint(ln.w)];
+            const bool lit = dot(xyz(ln), w) < 0.0f;
+            if (cos_wo > 0.0f && cos_wi > 0.0f && !is_black(fr) && lit) {
+              const float g = cos_wo * cos_wi / (dist * dist) * (c0.w / (float)ndir_job);
+              contrib = mul(mul(beta, mul(fr, le_of(lm))), g);
+              if (!is_black(contrib)) {
+                want_sh = true;
+                sd = w;
+                sdist = dist * (1.0f - 1e-5f);
+              }
             }
           }
         }
-      }
-      const uint32_t si = block_append<kShadeBlock / 64>(want_sh, count_shadow, append_sh);
-      if (want_sh) {
-        sq[3 * si] = make_float4(so.x, so.y, so.z, __uint_as_float(slot));
-        sq[3 * si + 1] = make_float4(sd.x, sd.y, sd.z, sdist);
-        sq[3 * si + 2] = make_float4(contrib.x, contrib.y, contrib.z, __uint_as_float(prim));
+        const uint32_t si = block_append<kShadeBlock / 64>(want_sh, count_shadow, append_sh);
+        if (want_sh) {
+          sq[3 * si] = make_float4(so.x, so.y, so.z, __uint_as_float(slot));
+          sq[3 * si + 1] = make_float4(sd.x, sd.y, sd.z, sdist);
+          sq[3 * si + 2] = make_float4(contrib.x, contrib.y, contrib.z, __uint_as_float(prim));
+        }
       }
     }
     if (surface) {
@@ -677,7 +690,8 @@ int enqueue_batch(mpt_context *ctx, Lane &ln, mpt_scene *s, uint32_t D, bool ins
     if ((rc = toc())) return rc;
     if (L) {
       if ((rc = tic(2))) return rc;
-      launch_shadow(ctx, st, (uint64_t)cap * L, g, s->dev, w.sq.p, w.counts.p + D + 1 + d, w.Ls.p);
+      launch_shadow(ctx, st, (uint64_t)cap * std::max(1u, w.lights), g, s->dev, w.sq.p, w.counts.p + D + 1 + d,
+                    w.Ls.p);
       if ((rc = toc())) return rc;
     }
     MPT_LAUNCHED(ctx, "depth kernels");
@@ -699,6 +713,7 @@ int stage_plan(mpt_context *ctx, Lane &ln, const BatchPlan &plan) {
   bd.hdr.n_paths = plan.n_paths;
   bd.hdr.n_pixels = plan.n_pixels;
   bd.hdr.max_depth = plan.max_depth;
+  bd.hdr.max_direct = std::max(1u, plan.max_direct);
   std::copy(plan.jobs.begin(), plan.jobs.end(), bd.jobs);
   std::copy(plan.chunks.begin(), plan.chunks.end(), bd.chunks);
   const size_t head = offsetof(BatchDesc, jobs) + plan.jobs.size() * sizeof(JobDev);
@@ -760,6 +775,7 @@ JobDev make_job(const mpt_job &job, float4 *film) {
   j.max_depth = job.max_depth;
   j.seed = job.seed;
   j.job_id = job.job_id;
+  j.direct = std::max(1u, job.direct_samples);
   uint32_t side = std::max<uint32_t>(1, (uint32_t)std::sqrt((double)job.spp));
   while ((uint64_t)(side + 1) * (side + 1) <= job.spp) ++side;
   while ((uint64_t)side * side > job.spp) --side;
@@ -775,6 +791,7 @@ int check_job(mpt_context *ctx, const mpt_job *job) {
   if (job->max_depth > 64) return fail(ctx, MPT_E_INVALID, "max_depth above 64");
   if ((uint64_t)job->width * job->height > (1ull << 30)) return fail(ctx, MPT_E_INVALID, "film too large");
   if (job->spp >= (1u << kJobBits)) return fail(ctx, MPT_E_INVALID, "spp at or above 2^20");
+  if (job->direct_samples > kMaxDirect) return fail(ctx, MPT_E_INVALID, "direct_samples above 64");
   const float *l = job->camera.look, *u = job->camera.up;
   const double ll = l[0] * l[0] + l[1] * l[1] + l[2] * l[2];
   const double cx = u[1] * l[2] - u[2] * l[1], cy = u[2] * l[0] - u[0] * l[2], cz = u[0] * l[1] - u[1] * l[0];
@@ -910,8 +927,12 @@ int run_batch(mpt_context *ctx, mpt_scene *s, const BatchPlan &plan, float *L_ho
   if (plan.jobs.size() > kMaxJobs || plan.chunks.size() > kMaxChunks)
     return fail(ctx, MPT_E_INVALID, "batch plan exceeds the job or chunk table");
   if (lane >= kMaxLanes) return fail(ctx, MPT_E_INVALID, "bad lane");
+  // shadow rays per vertex: one per light sample; their random streams are
+  // numbered 1 + light * samples + sample below 256
+  const uint32_t shadow_per_vertex = s->dev.num_lights * std::max(1u, plan.max_direct);
+  if (shadow_per_vertex > 254) return fail(ctx, MPT_E_INVALID, "lights x direct_samples above 254");
   int rc = ensure_workspace(ctx, lane, std::max(plan.n_paths, ctx->batch_paths), plan.max_depth,
-                            s->dev.num_lights);
+                            shadow_per_vertex);
   if (rc) return rc;
   Lane &ln = ctx->lanes[lane];
   if ((rc = stage_plan(ctx, ln, plan))) return rc;
@@ -979,6 +1000,7 @@ int render_range(mpt_scene *s, const mpt_job *job, uint32_t pix_begin, uint32_t 
   BatchPlan plan;
   plan.jobs.push_back(make_job(*job, film));
   plan.max_depth = job->max_depth;
+  plan.max_direct = plan.jobs[0].direct;
   uint64_t batches = 0;
   for (uint32_t p0 = pix_begin; p0 < pix_end; p0 += batch_pix) {
     const uint32_t np = std::min(batch_pix, pix_end - p0);
@@ -1065,6 +1087,11 @@ int prepare_scene(mpt_context *ctx, const mpt_scene_desc *d, const mpt_kd_params
     return fail(ctx, MPT_E_INVALID, "missing scene array");
   for (uint64_t i = 0; i < 3ull * nt; ++i)
     if (d->indices[i] >= nv) return fail(ctx, MPT_E_INVALID, "triangle index out of range");
+  if ((d->vertex_normals == nullptr) != (d->vertex_normal_mask == nullptr))
+    return fail(ctx, MPT_E_INVALID, "vertex_normals and vertex_normal_mask go together");
+  if (d->tri_flags)
+    for (uint32_t i = 0; i < nt; ++i)
+      if (d->tri_flags[i] & ~(uint32_t)MPT_TRI_ONE_SIDED) return fail(ctx, MPT_E_INVALID, "unknown triangle flag");
   auto bad_mat = [&](uint32_t m) { return m >= nm; };
   for (uint32_t i = 0; i < nt; ++i)
     if (bad_mat(d->tri_material[i]) || emissive_host(d->materials[d->tri_material[i]]))
@@ -1098,6 +1125,9 @@ int prepare_scene(mpt_context *ctx, const mpt_scene_desc *d, const mpt_kd_params
     }
     for (uint32_t i = 0; i < nv; ++i) {
       double *n = &acc[3ull * i];
+      // a normal the mesh file gave wins over the averaged face normals
+      if (d->vertex_normals && d->vertex_normal_mask[i])
+        for (int k = 0; k < 3; ++k) n[k] = d->vertex_normals[3ull * i + k];
       const double l = std::sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
       const double k = l > 0 ? 1.0 / l : 0.0;
       vnorm[i] = make_float4((float)(n[0] * k), (float)(n[1] * k), (float)(n[2] * k), 0);
@@ -1191,7 +1221,8 @@ int prepare_scene(mpt_context *ctx, const mpt_scene_desc *d, const mpt_kd_params
     uint32_t kind, id = prim;
     std::memset(r, 0, 12 * sizeof(float));
     if (prim < nt) {
-      kind = mpt_kd::KIND_TRIANGLE;
+      kind = d->tri_flags && (d->tri_flags[prim] & MPT_TRI_ONE_SIDED) ? mpt_kd::KIND_TRIANGLE_ONE_SIDED
+                                                                      : mpt_kd::KIND_TRIANGLE;
       const float4 a = verts[tris[prim].x], b = verts[tris[prim].y], c = verts[tris[prim].z];
       r[0] = a.x; r[1] = a.y; r[2] = a.z;
       r[4] = b.x; r[5] = b.y; r[6] = b.z;

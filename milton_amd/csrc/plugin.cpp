@@ -54,6 +54,15 @@ void PathTracer::setScene(const mpt_scene_desc &d) {
   quads_.assign(d.quads, d.quads + 9ull * d.num_quads);
   quad_mat_.assign(d.quad_material, d.quad_material + d.num_quads);
   mats_.assign(d.materials, d.materials + d.num_materials);
+  if (d.tri_flags) tri_flags_.assign(d.tri_flags, d.tri_flags + d.num_triangles);
+  else tri_flags_.clear();
+  if (d.vertex_normals && d.vertex_normal_mask) {
+    vnormals_.assign(d.vertex_normals, d.vertex_normals + 3ull * d.num_vertices);
+    vmask_.assign(d.vertex_normal_mask, d.vertex_normal_mask + d.num_vertices);
+  } else {
+    vnormals_.clear();
+    vmask_.clear();
+  }
   desc_ = d;
   desc_.vertices = verts_.data();
   desc_.indices = idx_.data();
@@ -63,6 +72,9 @@ void PathTracer::setScene(const mpt_scene_desc &d) {
   desc_.quads = quads_.data();
   desc_.quad_material = quad_mat_.data();
   desc_.materials = mats_.data();
+  desc_.tri_flags = tri_flags_.empty() ? nullptr : tri_flags_.data();
+  desc_.vertex_normals = vnormals_.empty() ? nullptr : vnormals_.data();
+  desc_.vertex_normal_mask = vmask_.empty() ? nullptr : vmask_.data();
 }
 
 void PathTracer::setCamera(const mpt_camera &camera) { camera_ = camera; }
@@ -85,8 +97,9 @@ int PathTracer::init() {
     err_ = "efficientDirect = false is not supported by the device path";
     return MPT_E_INVALID;
   }
-  if (getValue("noDirectSamples", 1) != 1) {
-    err_ = "noDirectSamples other than 1 is not supported by the device path";
+  const double nds = getValue("noDirectSamples", 1);
+  if (nds < 1 || nds > 64) {
+    err_ = "noDirectSamples must be in 1..64";
     return MPT_E_INVALID;
   }
   if (desc_.version != MPT_ABI_VERSION) {
@@ -130,6 +143,7 @@ int PathTracer::render() {
   job.spp = (uint32_t)getValue("noSuperSamples", 4);
   job.max_depth = (uint32_t)getValue("maxDepth", 8);
   job.seed = (uint32_t)getValue("seed", 1);
+  job.direct_samples = (uint32_t)getValue("noDirectSamples", 1);
   film_.assign(3ull * job.width * job.height, 0.0f);
   return mpt_render(scene_, &job, film_.data(), &stats_);
 }

@@ -198,6 +198,7 @@ int round(mpt_server *sv, std::unique_lock<std::mutex> &lk, uint32_t *ran) {
     plan.n_paths += np * spp;
     plan.n_pixels += np;
     plan.max_depth = std::max(plan.max_depth, j->job.max_depth);
+    plan.max_direct = std::max(plan.max_direct, std::max(1u, j->job.direct_samples));
     j->next_pixel += np;
     ++j->inflight;
     parts.push_back({j, np});

@@ -543,6 +543,7 @@ struct Sphere { V3 c; double r; };
 struct orc_scene {
   std::vector<V3> verts, vnormals;
   std::vector<uint32_t> idx, tri_mat, sph_mat, quad_mat;
+  std::vector<uint8_t> one_sided;   // standalone Triangle (back faces culled)
   std::vector<Sphere> spheres;
   std::vector<Quad> quads;
   std::vector<Material> mats;
@@ -555,12 +556,23 @@ struct orc_scene {
   std::vector<uint32_t> top_index;
 
   // --- primitive tests ---------------------------------------------------
-  // shapes/MeshTriangle.cpp:92-169, non-culling branch (:142-167).
+  // shapes/MeshTriangle.cpp:92-169, non-culling branch (:142-167); a
+  // standalone Triangle takes the culling branch of shapes/Triangle.cpp:245-270.
   double tri_t(uint32_t i, const Ray &ray) const {
     const V3 &v0 = verts[idx[3 * i]], &v1 = verts[idx[3 * i + 1]], &v2 = verts[idx[3 * i + 2]];
     const V3 e1 = v1 - v0, e2 = v2 - v0;
     const V3 p = cross(ray.d, e2);
     const double det = dot(e1, p);
+    if (one_sided[i]) {
+      if (det < kEps) return kInf;
+      const V3 tv = ray.o - v0;
+      const double u = dot(tv, p);
+      if (u < 0.0 || u > det) return kInf;
+      const V3 q = cross(tv, e1);
+      const double v = dot(ray.d, q);
+      if (v < 0.0 || u + v > det) return kInf;
+      return dot(e2, q) * (1.0 / det);
+    }
     // EQ(det, 0) (common/utilities.h:240) reduces to |det| <= EPSILON.
     if (std::fabs(det) < kEps || std::fabs(det - 0.0 / det) <= kEps) return kInf;
     const double inv = 1.0 / det;
@@ -938,28 +950,39 @@ V3 trace_path(const orc_scene &sc, const orc_job &job, const Camera &cam, uint32
     }
     const Uniforms bu = draw(job.seed, job.job_id, pixel, sample, depth, 0);
     if (!m.specular()) {
+      // DirectIllumination.cpp:90-135: per light, ns samples jittered over
+      // floor(sqrt(ns))^2 strata of the light, weighted area / ns.
+      const uint32_t ns = std::max(1u, job.direct_samples);
+      uint32_t side = 1;
+      while ((side + 1) * (side + 1) <= ns) ++side;
       V3 direct;
       for (uint32_t li = (uint32_t)sc.lights.size(); li--;) {
         const uint32_t q = sc.lights[li];
         const uint32_t lgid = sc.nt + sc.ns + q;
         if (lgid == gid) continue;
         const Quad &lq = sc.quads[q];
-        const Uniforms lu = draw(job.seed, job.job_id, pixel, sample, depth, 1 + li);
-        const V3 lp = lq.c + lq.ex * (lu.u[0] - 0.5) + lq.ez * (lu.u[1] - 0.5);   // Plane.cpp:91-93
-        V3 wo = lp - p;
-        const double dist = len(wo);
-        wo = wo / dist;
-        const double cos_wo = std::fabs(dot(n, wo));
-        const double cos_wi = std::fabs(dot(lq.n, wo));
-        const V3 fr = bsdf_eval(m, n, ray.d, wo);
-        if (cos_wo > 0 && cos_wi > 0 && !is_black(fr)) {
-          ctr.shadow++;
-          if (!sc.occluded(Ray(p, wo), dist - kEps)) {
-            const Material &lm = sc.mats[sc.quad_mat[q]];
-            const V3 Le = dot(lq.n, -wo) > 0 ? lm.Le : V3();   // OrientedEmitter.cpp:17-23
-            direct = direct + mul(fr, Le) * (cos_wo * cos_wi / (dist * dist)) * lq.area;
+        V3 dl;
+        for (uint32_t k = 0; k < ns; ++k) {
+          const Uniforms lu = draw(job.seed, job.job_id, pixel, sample, depth, 1 + li * ns + k);
+          const uint32_t cell = k % (side * side);
+          const double u = ((double)(cell % side) + lu.u[0]) / side, v = ((double)(cell / side) + lu.u[1]) / side;
+          const V3 lp = lq.c + lq.ex * (u - 0.5) + lq.ez * (v - 0.5);   // Plane.cpp:91-93
+          V3 wo = lp - p;
+          const double dist = len(wo);
+          wo = wo / dist;
+          const double cos_wo = std::fabs(dot(n, wo));
+          const double cos_wi = std::fabs(dot(lq.n, wo));
+          const V3 fr = bsdf_eval(m, n, ray.d, wo);
+          if (cos_wo > 0 && cos_wi > 0 && !is_black(fr)) {
+            ctr.shadow++;
+            if (!sc.occluded(Ray(p, wo), dist - kEps)) {
+              const Material &lm = sc.mats[sc.quad_mat[q]];
+              const V3 Le = dot(lq.n, -wo) > 0 ? lm.Le : V3();   // OrientedEmitter.cpp:17-23
+              dl = dl + mul(fr, Le) * (cos_wo * cos_wi / (dist * dist));
+            }
           }
         }
+        direct = direct + dl * (lq.area / ns);
       }
       L = L + mul(beta, direct);
     }
@@ -1012,11 +1035,16 @@ void orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t ou
   philox(ctr, key, out);
 }
 
+// Bumped with every change of an exported signature; pyoracle refuses a
+// library built from older sources.
+int orc_interface_version(void) { return 3; }
+
 orc_scene *orc_scene_create(uint32_t nv, const float *vertices, uint32_t nt, const uint32_t *indices,
                             const uint32_t *tri_material, uint32_t ns, const float *spheres,
                             const uint32_t *sphere_material, uint32_t nq, const float *quads,
                             const uint32_t *quad_material, uint32_t nm, const float *materials,
-                            const float env_radiance[3], uint32_t kd_min_prims, uint32_t kd_max_depth,
+                            const float env_radiance[3], const uint32_t *tri_flags, const float *vertex_normals,
+                            const uint8_t *vertex_normal_mask, uint32_t kd_min_prims, uint32_t kd_max_depth,
                             int use_naive, int nthreads) {
   std::unique_ptr<orc_scene> s(new orc_scene());
   s->nt = nt;
@@ -1026,6 +1054,9 @@ orc_scene *orc_scene_create(uint32_t nv, const float *vertices, uint32_t nt, con
   for (uint32_t i = 0; i < nv; ++i) s->verts[i] = V3(vertices[3 * i], vertices[3 * i + 1], vertices[3 * i + 2]);
   s->idx.assign(indices, indices + 3ull * nt);
   s->tri_mat.assign(tri_material, tri_material + nt);
+  s->one_sided.assign(nt, 0);
+  if (tri_flags)
+    for (uint32_t i = 0; i < nt; ++i) s->one_sided[i] = (tri_flags[i] & 1u) ? 1 : 0;
   for (uint32_t i = 0; i < 3ull * nt; ++i)
     if (s->idx[i] >= nv) return nullptr;
   for (uint32_t i = 0; i < nm; ++i) s->mats.push_back(load_material(materials + 16ull * i));
@@ -1040,6 +1071,11 @@ orc_scene *orc_scene_create(uint32_t nv, const float *vertices, uint32_t nt, con
     s->vnormals[b] = s->vnormals[b] + fn;
     s->vnormals[c] = s->vnormals[c] + fn;
   }
+  // a mesh file's own normals replace the computed ones
+  if (vertex_normals && vertex_normal_mask)
+    for (uint32_t i = 0; i < nv; ++i)
+      if (vertex_normal_mask[i])
+        s->vnormals[i] = V3(vertex_normals[3 * i], vertex_normals[3 * i + 1], vertex_normals[3 * i + 2]);
   for (auto &v : s->vnormals) v = normalized(v);
   for (uint32_t i = 0; i < ns; ++i) {
     s->spheres.push_back(Sphere{V3(spheres[4 * i], spheres[4 * i + 1], spheres[4 * i + 2]), spheres[4 * i + 3]});

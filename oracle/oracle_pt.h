@@ -43,12 +43,16 @@ typedef struct orc_scene orc_scene;
  * mesh triangles) with the reference build parameters.  kd_min_prims /
  * kd_max_depth of 0 select the reference defaults (3, 24).  use_naive != 0
  * replaces both kd-trees by NaiveSpatialAccel.  Returns NULL on bad input. */
+int orc_interface_version(void);   /* 3 */
 orc_scene *orc_scene_create(uint32_t nv, const float *vertices,
                             uint32_t nt, const uint32_t *indices, const uint32_t *tri_material,
                             uint32_t ns, const float *spheres, const uint32_t *sphere_material,
                             uint32_t nq, const float *quads, const uint32_t *quad_material,
                             uint32_t nm, const float *materials,
                             const float env_radiance[3],
+                            const uint32_t *tri_flags,        /* optional: bit 0 = one-sided triangle */
+                            const float *vertex_normals,      /* optional, with the mask */
+                            const uint8_t *vertex_normal_mask,
                             uint32_t kd_min_prims, uint32_t kd_max_depth,
                             int use_naive, int nthreads);
 void orc_scene_destroy(orc_scene *s);
@@ -76,6 +80,7 @@ typedef struct orc_job {
   double camera[11];
   uint32_t width, height, spp, max_depth;
   uint32_t seed, job_id, pix_begin, pix_end;
+  uint32_t direct_samples; /* light samples per light per vertex (0 = 1) */
 } orc_job;
 
 /* Renders pixels [pix_begin, pix_end) of the job.  sum/sumsq are

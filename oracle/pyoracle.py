@@ -15,6 +15,9 @@ LIB_PATH = os.path.join(HERE, "liboracle_pt.so")
 MISS = 0xFFFFFFFF
 
 
+INTERFACE_VERSION = 3
+
+
 def build():
     """Compiles the oracle with g++ (always recompiles)."""
     tmp = LIB_PATH + ".tmp"
@@ -29,7 +32,8 @@ def build():
 class OrcJob(ctypes.Structure):
     _fields_ = [("camera", ctypes.c_double * 11), ("width", ctypes.c_uint32), ("height", ctypes.c_uint32),
                 ("spp", ctypes.c_uint32), ("max_depth", ctypes.c_uint32), ("seed", ctypes.c_uint32),
-                ("job_id", ctypes.c_uint32), ("pix_begin", ctypes.c_uint32), ("pix_end", ctypes.c_uint32)]
+                ("job_id", ctypes.c_uint32), ("pix_begin", ctypes.c_uint32), ("pix_end", ctypes.c_uint32),
+                ("direct_samples", ctypes.c_uint32)]
 
 
 _lib = None
@@ -43,7 +47,10 @@ def lib():
         L = ctypes.CDLL(LIB_PATH)
         vp, u32, u64, i32 = ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint64, ctypes.c_int
         L.orc_philox4x32_10.argtypes = [vp, vp, vp]
-        L.orc_scene_create.argtypes = [u32, vp, u32, vp, vp, u32, vp, vp, u32, vp, vp, u32, vp, vp, u32, u32, i32, i32]
+        if not hasattr(L, "orc_interface_version") or L.orc_interface_version() != INTERFACE_VERSION:
+            raise RuntimeError(f"{LIB_PATH} was built from older sources: rebuild it (python oracle/pyoracle.py)")
+        L.orc_scene_create.argtypes = [u32, vp, u32, vp, vp, u32, vp, vp, u32, vp, vp, u32, vp, vp, vp, vp, vp,
+                                       u32, u32, i32, i32]
         L.orc_scene_create.restype = vp
         L.orc_scene_destroy.argtypes = [vp]
         L.orc_kd_stats.argtypes = [vp, i32, vp]
@@ -87,11 +94,14 @@ class OracleScene:
         self.mats = np.ascontiguousarray(a.materials_f32())
         self.env = np.ascontiguousarray(a.env, np.float32)
         self.threads = threads or nthreads()
+        opt = lambda x: _p(x) if x is not None else None   # noqa: E731
+        flags = getattr(a, "tri_flags", None)
+        vn, vm = getattr(a, "vertex_normals", None), getattr(a, "vertex_normal_mask", None)
         self.h = lib().orc_scene_create(
             len(a.vertices), _p(a.vertices), len(a.indices), _p(a.indices), _p(a.tri_material),
             len(a.spheres), _p(a.spheres), _p(a.sphere_material), len(a.quads), _p(a.quads), _p(a.quad_material),
-            len(self.mats), _p(self.mats), _p(self.env), kd_min_prims, kd_max_depth, 1 if naive else 0,
-            self.threads)
+            len(self.mats), _p(self.mats), _p(self.env), opt(flags), opt(vn), opt(vm),
+            kd_min_prims, kd_max_depth, 1 if naive else 0, self.threads)
         if not self.h:
             raise ValueError("oracle rejected the scene")
 
@@ -162,6 +172,7 @@ def make_job(job, pix_begin=0, pix_end=None):
         j.camera[i] = v
     j.width, j.height, j.spp, j.max_depth = job.width, job.height, job.spp, job.max_depth
     j.seed, j.job_id = job.seed, job.job_id
+    j.direct_samples = getattr(job, "direct_samples", 0)
     j.pix_begin = pix_begin
     j.pix_end = job.width * job.height if pix_end is None else pix_end
     return j

@@ -1,0 +1,262 @@
+"""Milton scene files (mpt_scenefile_*): the tolerant JSON grammar, the scene
+graph flattening (materials, transforms, shapes), the mapping of each element
+to the scene description, one-sided triangles against the oracle, and clear
+errors for what this path does not render.  CPU only."""
+import math
+import os
+
+import numpy as np
+import pytest
+
+import milton_amd as m
+from oracle import pyoracle as po
+from conftest import classify_hits, rays64
+
+REFERENCE_SCENES = "/root/reference/mlt/src/exec/scenes"
+
+# A small Cornell-style room in the Milton scenefile format: one-sided
+# "plane" walls given by their vertices, a transformed unit plane as the
+# light, a sphere, comments, trailing commas and repeated keys.
+ROOM = """
+{
+   /* a test room */
+   "scenefile" : {
+      "version" : 0.1,
+      "renderer" : { "type" : "pathTracer", "noSuperSamples" : 16, "maxDepth" : 5, "seed" : 3, },
+      "camera" : {
+         "type" : "pinhole", "eye" : [ 0.5, 0.5, -1.4 ], "look" : [ 0, 0, 1 ], "up" : [ 0, 1, 0 ],
+         "heightAngle" : 40,
+      },
+      "output" : { "width" : 64, "height" : 48 },
+      "scene" : {
+         "material" : {
+            "emitter" : { "type" : "oriented", "power" : [ 3, 3, 2.5 ] },
+            "bsdf" : { "type" : "diffuse", "kd" : .75 },
+            "transform" : {
+               "translate" : [ 0.5, 0.999, 0.5 ],
+               "rotate" : [ 0, 0, 0, 1, 0, 0, 180 ],
+               "scale" : [ 0.25, 1, 0.25 ],
+               "plane" : { },
+            },
+         },
+         "material" : {
+            "bsdf" : { "type" : "diffuse", "kd" : 0.75 },
+            // floor, ceiling, back wall (front faces towards the inside)
+            "plane" : { "vertices" : [ 0,0,0,  0,0,1,  1,0,1,  1,0,0 ] },
+            "plane" : { "vertices" : [ 0,1,0,  1,1,0,  1,1,1,  0,1,1 ] },
+            "plane" : { "vertices" : [ 0,0,1,  0,1,1,  1,1,1,  1,0,1 ] },
+         },
+         "material" : {
+            "bsdf" : { "type" : "diffuse", "kd" : [ 0.6, 0.1, 0.1 ] },
+            "plane" : { "vertices" : [ 0,0,0,  0,1,0,  0,1,1,  0,0,1 ] },
+         },
+         "material" : {
+            "bsdf" : { "type" : "diffuse", "kd" : [ 0.1, 0.6, 0.1 ] },
+            "plane" : { "vertices" : [ 1,0,0,  1,0,1,  1,1,1,  1,1,0 ] },
+         },
+         "material" : {
+            "bsdf" : { "type" : "dielectric", "ks" : 1, "transparency" : 1 },
+            "medium" : { "ior" : 1.5 },
+            "sphere" : { "position" : [ 0.35, 0.2, 0.6 ], "radius" : 0.2 },
+         },
+         "material" : {
+            "bsdf" : { "type" : "modifiedPhong", "kd" : 0.2, "ks" : 0.6, "n" : 50 },
+            "transform" : {
+               "translate" : [ 0.7, 0.15, 0.35 ],
+               "scale" : [ 0.2, 0.3, 0.2 ],
+               "triangle" : { "vertices" : [ -0.5,0,-0.5,  -0.5,0,0.5,  0.5,0,0.5 ] },
+            },
+         },
+      },
+   },
+}
+"""
+
+
+def _room():
+    return m.load_scenefile(text=ROOM)
+
+
+def test_grammar_and_settings():
+    a, job = _room()
+    assert (job.width, job.height, job.spp, job.max_depth, job.seed) == (64, 48, 16, 5, 3)
+    assert list(job.camera.eye) == pytest.approx([0.5, 0.5, -1.4])
+    assert job.camera.height_angle == pytest.approx(40)
+    assert job.camera.aspect == pytest.approx(1.0)   # the camera default, not W/H
+    # 5 walls x 2 one-sided triangles + 1 standalone triangle; 1 quad light; 1 sphere
+    assert len(a.indices) == 11 and len(a.quads) == 1 and len(a.spheres) == 1
+    assert a.tri_flags is not None and np.all(a.tri_flags == m.MPT_TRI_ONE_SIDED)
+    assert len(a.vertices) == 33   # standalone triangles keep their own vertices
+
+
+def test_light_quad_and_emission():
+    a, _ = _room()
+    q = a.quads[0]
+    c, ex, ez = q[:3], q[3:6], q[6:9]
+    np.testing.assert_allclose(c, [0.5, 0.999, 0.5], atol=1e-6)
+    # rotate 180 degrees about x: x stays, z flips; scale 0.25 in x and z
+    np.testing.assert_allclose(ex, [0.25, 0, 0], atol=1e-6)
+    np.testing.assert_allclose(ez, [0, 0, -0.25], atol=1e-6)
+    mat = a.materials[a.quad_material[0]]
+    area = 0.25 * 0.25
+    np.testing.assert_allclose(mat["Le"], np.array([3, 3, 2.5]) / (math.pi * area), rtol=1e-6)
+
+
+def test_materials_and_sphere():
+    a, _ = _room()
+    s = a.spheres[0]
+    np.testing.assert_allclose(s, [0.35, 0.2, 0.6, 0.2], atol=1e-6)
+    sm = a.materials[a.sphere_material[0]]
+    assert sm["type"] == m.MPT_BSDF_DIELECTRIC and sm["ior"] == pytest.approx(1.5)
+    assert sm["transparency"] == pytest.approx(1) and np.allclose(sm["ks"], 1)
+    red = a.materials[a.tri_material[6]]
+    np.testing.assert_allclose(red["kd"], [0.6, 0.1, 0.1], rtol=1e-6)
+    tri = a.materials[a.tri_material[10]]
+    assert tri["type"] == m.MPT_BSDF_PHONG and tri["exponent"] == pytest.approx(50)
+    np.testing.assert_allclose(tri["kd"], 0.2, rtol=1e-6)
+    np.testing.assert_allclose(tri["ks"], 0.6, rtol=1e-6)
+
+
+def test_transform_order_and_rotation():
+    """Node transform = translate * rotate * scale; children compose under
+    their parents; rotate is right-handed about the given axis and point."""
+    text = """{ "scenefile" : { "scene" : {
+        "transform" : { "translate" : [1, 2, 3],
+          "transform" : { "rotate" : [0, 0, 0,  0, 1, 0,  90], "scale" : [2, 1, 1],
+             "plane" : { } } },
+        "transform" : { "rotate" : [1, 0, 0,  0, 0, 1,  90],
+             "triangle" : { "vertices" : [2,0,0, 2,1,0, 3,0,0] } } } } }"""
+    a, _ = m.load_scenefile(text=text)
+    c, ex, ez = a.quads[0][:3], a.quads[0][3:6], a.quads[0][6:9]
+    np.testing.assert_allclose(c, [1, 2, 3], atol=1e-6)
+    # scale x by 2, then rotate +90 about y: x -> -z, z -> x
+    np.testing.assert_allclose(ex, [0, 0, -2], atol=1e-6)
+    np.testing.assert_allclose(ez, [1, 0, 0], atol=1e-6)
+    # +90 about z through (1, 0, 0): (2,0,0) -> (1,1,0), (2,1,0) -> (0,1,0), (3,0,0) -> (1,2,0)
+    np.testing.assert_allclose(a.vertices[:3], [[1, 1, 0], [0, 1, 0], [1, 2, 0]], atol=1e-6)
+
+
+def test_defaults():
+    a, job = m.load_scenefile(text='{ "scenefile" : { "scene" : { "sphere" : { } } } }')
+    assert (job.width, job.height, job.spp) == (480, 480, 1)
+    assert list(job.camera.eye) == pytest.approx([2, 2, 2])
+    np.testing.assert_allclose(np.array(job.camera.look) / np.linalg.norm(job.camera.look),
+                               -np.ones(3) / math.sqrt(3), atol=1e-6)
+    assert job.camera.height_angle == pytest.approx(45)
+    np.testing.assert_allclose(a.spheres[0], [0, 0, 0, 0.5], atol=1e-7)   # unit sphere, diameter 1
+    assert a.materials[0]["type"] == m.MPT_BSDF_DIFFUSE and np.allclose(a.materials[0]["kd"], 0.5)
+
+
+@pytest.mark.parametrize("snippet, word", [
+    ('"cube" : { }', "cube"),
+    ('"material" : { "emitter" : { "type" : "oriented", "power" : 1 }, "sphere" : { } }', "spheres"),
+    ('"material" : { "emitter" : { "type" : "omni", "power" : 1 }, "plane" : { } }', "omni"),
+    ('"transform" : { "scale" : [1, 2, 1], "sphere" : { } }', "ellipsoid"),
+    ('"material" : { "bsdf" : { "type" : "velvet" }, "plane" : { } }', "velvet"),
+    ('"mesh" : { "path" : "no/such/file.obj" }', "cannot be opened"),
+])
+def test_unsupported_elements_fail_clearly(snippet, word):
+    with pytest.raises(m.MptError) as e:
+        m.load_scenefile(text='{ "scenefile" : { "scene" : { %s } } }' % snippet)
+    assert word in str(e.value)
+
+
+def test_syntax_error_names_the_line():
+    with pytest.raises(m.MptError) as e:
+        m.load_scenefile(text='{ "scenefile" : {\n "scene" : { "plane" : { } \n } ')
+    assert "line" in str(e.value)
+
+
+def test_obj_and_ply_meshes(tmp_path):
+    obj = tmp_path / "quad.obj"
+    obj.write_text("v 0 0 0\nv 1 0 0\nv 1 1 0\nv 0 1 0\nvn 0 0 1\nf 1//1 2//1 3//1 4//1\n")
+    ply = tmp_path / "tri.ply"
+    ply.write_text("ply\nformat ascii 1.0\nelement vertex 3\nproperty float x\nproperty float y\n"
+                   "property float z\nelement face 1\nproperty list uchar int vertex_indices\nend_header\n"
+                   "0 0 0\n2 0 0\n0 4 0\n3 0 1 2\n")
+    scene = tmp_path / "s.js"
+    scene.write_text('{ "scenefile" : { "scene" : { "mesh" : { "path" : "quad.obj" }, '
+                     '"mesh" : { "path" : "tri.ply", "normalize" : true } } } }')
+    a, _ = m.load_scenefile(path=str(scene))
+    assert len(a.indices) == 3                      # quad as a fan of 2, plus 1
+    assert a.tri_flags is not None and np.all(a.tri_flags == 0)   # mesh triangles are two-sided
+    assert a.vertex_normal_mask is not None
+    np.testing.assert_array_equal(a.vertex_normal_mask, [1, 1, 1, 1, 0, 0, 0])
+    np.testing.assert_allclose(a.vertex_normals[:4], [[0, 0, 1]] * 4)
+    # normalize: into [-0.5, 0.5] per axis (a flat axis keeps its coordinate centred)
+    np.testing.assert_allclose(a.vertices[4:], [[-0.5, -0.5, 0], [0.5, -0.5, 0], [-0.5, 0.5, 0]], atol=1e-6)
+
+
+def test_one_sided_triangles_against_oracle():
+    """The device traversal (host build of it) and the oracle agree on the
+    loaded room, whose walls cull their back faces; rays from outside the
+    room pass through the walls' backs."""
+    a, job = _room()
+    rng = np.random.default_rng(5)
+    n = 20000
+    inside = rng.uniform(0.02, 0.98, (n // 2, 3))
+    outside = rng.uniform(-1.5, 2.5, (n // 2, 3))
+    o = np.concatenate([inside, outside]).astype(np.float32)
+    d = rng.normal(size=(n, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    rays = np.zeros(n, m.RAY_DTYPE)
+    rays["o"] = o
+    rays["d"] = d.astype(np.float32)
+    tree = m.HostKdTree(a)
+    hits, _ = tree.trace_closest(rays)
+    osc = po.OracleScene(a)
+    ot, oi = osc.trace_closest(rays64(rays))
+    match, non_tie, rel = classify_hits(osc, rays, hits["t"], hits["prim"], ot, oi)
+    assert non_tie <= 1e-4, (match, non_tie)
+    assert (rel <= 1e-5).mean() >= 0.9999
+    # a ray that meets the floor from below misses it; from above, hits it
+    up = np.zeros(2, m.RAY_DTYPE)
+    up["o"] = [[0.3, -0.5, 0.3], [0.3, 0.5, 0.3]]
+    up["d"] = [[0, 1, 0], [0, -1, 0]]
+    h, _ = tree.trace_closest(up)
+    ot2, oi2 = osc.trace_closest(rays64(up))
+    assert h["prim"][1] in (0, 1) and oi2[1] in (0, 1)           # floor triangles
+    assert h["prim"][0] not in (0, 1) and oi2[0] not in (0, 1)   # culled from below
+
+
+def test_oracle_one_sided_closed_form():
+    """The oracle's culling test follows the reference's (e1 . (d x e2) must
+    reach EPSILON): a triangle is hit from its front at the analytic t and
+    missed from its back; the same triangle two-sided is hit from both."""
+    verts = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0]], np.float32)   # front normal +z
+    mats = np.zeros(1, m.MATERIAL_DTYPE)
+    mats["kd"] = 0.5
+    rays = np.array([[0.2, 0.2, 2, 0, 0, -1], [0.2, 0.2, -2, 0, 0, 1]], np.float64)
+    # e1 = (1,0,0), e2 = (0,1,0): det = e1 . (d x e2) = -d_z, so the ray going -z hits
+    for flags, expect in (([1], [2.0, np.inf]), ([0], [2.0, 2.0])):
+        a = m.SceneArrays(verts, [[0, 1, 2]], [0], np.zeros((0, 4)), [], np.zeros((0, 9)), [], mats, [0, 0, 0],
+                          tri_flags=flags)
+        t, _ = po.OracleScene(a).trace_closest(rays)
+        np.testing.assert_allclose(t, expect)
+
+
+@pytest.mark.skipif(not os.path.isdir(REFERENCE_SCENES), reason="reference scene files not present")
+def test_reference_scene_files():
+    """Every scene file Milton ships either loads or is refused with a
+    message naming what is not supported; the Cornell box loads."""
+    loaded, refused = [], []
+    for name in sorted(os.listdir(REFERENCE_SCENES)):
+        if not name.endswith(".js"):
+            continue
+        try:
+            a, job = m.load_scenefile(path=os.path.join(REFERENCE_SCENES, name))
+            loaded.append((name, a.num_prims))
+        except m.MptError as e:
+            msg = str(e)
+            assert any(w in msg for w in ("not supported", "cannot be opened", "syntax", "needs", "no supported")), msg
+            refused.append((name, msg))
+    print("loaded:", loaded)
+    print("refused:", refused)
+    names = [n for n, _ in loaded]
+    assert "cornellBox.js" in names
+    a, job = m.load_scenefile(path=os.path.join(REFERENCE_SCENES, "cornellBox.js"))
+    assert len(a.quads) == 1 and len(a.indices) == 2 * 15   # light; 15 vertex planes of two triangles
+    assert job.direct_samples == 4 and job.spp == 1024
+    le = a.materials[a.quad_material[0]]["Le"]
+    np.testing.assert_allclose(le, np.array([2e6, 2e6, 1.6e6]) / (math.pi * 140 * 125), rtol=1e-5)
+    assert list(job.camera.eye) == pytest.approx([278, 273, -650])
