@@ -301,8 +301,11 @@ __device__ __forceinline__ HitInfo hit_info(const Scene &sc, uint32_t prim, floa
 // 46-136).  The recursion becomes a throughput `beta`; the emission terms, NEE
 // and roulette follow the reference rules, each cited below.
 constexpr int kShadeBlock = 512;
+#ifndef MPT_SHADE_WAVES
+#define MPT_SHADE_WAVES 6
+#endif
 
-__global__ __launch_bounds__(kShadeBlock) __attribute__((amdgpu_waves_per_eu(6))) void k_shade(Scene sc, const BatchDesc *__restrict__ bd, uint32_t depth,
+__global__ __launch_bounds__(kShadeBlock) __attribute__((amdgpu_waves_per_eu(MPT_SHADE_WAVES))) void k_shade(Scene sc, const BatchDesc *__restrict__ bd, uint32_t depth,
                                                const float4 *__restrict__ qin, const float2 *__restrict__ hits,
                                                const uint32_t *__restrict__ count_in, float4 *__restrict__ qout,
                                                uint32_t *__restrict__ count_out, float4 *__restrict__ sq,

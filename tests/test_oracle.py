@@ -319,6 +319,55 @@ def test_estimator_direct_light_closed_form():
         assert abs(L[k, 0] - expected) <= 1e-4 * expected
 
 
+def _form_factor_rect(p, h, x0, x1, z0, z1):
+    """Configuration factor from a floor point p (facing +y) to the rectangle
+    [x0, x1] x [z0, z1] at height h above it, facing down: the corner formula
+    for a rectangle with a corner above the point (Howell's catalogue, B-1
+    parallel case) combined over the four corners with signs."""
+    def corner(x, z):
+        X, Z = abs(x) / h, abs(z) / h
+        f = (X / math.sqrt(1 + X * X) * math.atan(Z / math.sqrt(1 + X * X))
+             + Z / math.sqrt(1 + Z * Z) * math.atan(X / math.sqrt(1 + Z * Z))) / (2 * math.pi)
+        return math.copysign(1, x) * math.copysign(1, z) * f
+    px, pz = p[0], p[2]
+    return (corner(x1 - px, z1 - pz) - corner(x0 - px, z1 - pz) - corner(x1 - px, z0 - pz)
+            + corner(x0 - px, z0 - pz))
+
+
+def test_estimator_direct_samples_area_light_closed_form():
+    """NEE from a large square light with 1 and 4 light samples per vertex
+    (DirectIllumination.cpp:98-133, noDirectSamples): the mean radiance of a
+    floor point is kd * Le * F, F the point's configuration factor to the
+    light (closed form), and 4 stratified samples cut the variance well
+    below a quarter of one sample's."""
+    side, h, Le, kd = 1.0, 0.5, 3.0, 0.5
+    arrays = _floor_scene([0, 0, 0], light_size=side, light_h=h, Le=Le, kd=kd)
+    # a low camera (below the light) looking at the floor under the light
+    arrays.camera = {"eye": [0.0, 0.3, -2.0], "look": [0.0, -0.3, 2.0], "up": [0.0, 1.0, 0.0],
+                     "height_angle": 20.0, "aspect": 1.0}
+    osc = po.OracleScene(arrays)
+    var = {}
+    for n in (1, 4):
+        job = m.make_job(arrays.camera, 8, 8, 4096, 1, seed=11, direct_samples=n)
+        for pixel in (27, 36):
+            pix = np.full(4096, pixel, np.uint32)
+            smp = np.arange(4096, dtype=np.uint32)
+            L = osc.render_paths(job, pix, smp)[:, 0]
+            expect = []
+            for k in range(4096):
+                ray = po.camera_ray(job, pixel, k)
+                o, d = ray[:3], ray[3:]
+                pt = o + (-o[1] / d[1]) * d
+                expect.append(kd * Le * _form_factor_rect(pt, h, -side / 2, side / 2, -side / 2, side / 2))
+            expect = np.array(expect)
+            sigma = L.std() / math.sqrt(len(L))
+            assert abs(L.mean() - expect.mean()) <= 4 * sigma + 1e-3 * expect.mean(), (n, pixel)
+            # the variance the light sampling adds on top of the pixel footprint's
+            var[(n, pixel)] = np.var(L - expect)
+    for pixel in (27, 36):
+        assert var[(4, pixel)] < var[(1, pixel)] / 4
+
+
 def test_three_sigma_oracle_self_consistency(room):
     """Two independent seeds of the oracle agree within 3 sigma per pixel on
     >= 98% of pixels (the statistical test used for the device is sound)."""
