@@ -123,6 +123,7 @@ struct GraphEntry {
   const void *scene = nullptr;
   uint32_t lane = 0;
   uint32_t depth = 0, cap = 0, lights = 0;
+  bool multi = false;   // several light samples per light
   uint64_t generation = 0;
   hipGraphExec_t exec = nullptr;
 };

@@ -305,6 +305,9 @@ constexpr int kShadeBlock = 512;
 #define MPT_SHADE_WAVES 6
 #endif
 
+// kMulti: some job of the batch draws several light samples per light; the
+// common single-sample form keeps the NEE loop's counters out of registers.
+template <bool kMulti>
 __global__ __launch_bounds__(kShadeBlock) __attribute__((amdgpu_waves_per_eu(MPT_SHADE_WAVES))) void k_shade(Scene sc, const BatchDesc *__restrict__ bd, uint32_t depth,
                                                const float4 *__restrict__ qin, const float2 *__restrict__ hits,
                                                const uint32_t *__restrict__ count_in, float4 *__restrict__ qout,
@@ -332,7 +335,7 @@ __global__ __launch_bounds__(kShadeBlock) __attribute__((amdgpu_waves_per_eu(MPT
     float thit = 0.0f;
     float3 d = f3(0, 0, 1);
     uint32_t pixel = 0, sample = 0, seed = 0, job_id = 0, max_depth = 0;
-    uint32_t ndir_job = 1, ndir_side = 1;   // light samples per light, strata per side
+    uint32_t ndir_job = 1;   // light samples per light
     if (active) {
       // (origin, slot) (direction, origin primitive) (throughput, flags)
       const float4 a = qin[3 * i], b = qin[3 * i + 1], c = qin[3 * i + 2];
@@ -346,9 +349,7 @@ __global__ __launch_bounds__(kShadeBlock) __attribute__((amdgpu_waves_per_eu(MPT
       seed = job.seed;
       job_id = job.job_id;
       max_depth = job.max_depth;
-      ndir_job = job.direct;
-      ndir_side = 1;
-      while ((ndir_side + 1) * (ndir_side + 1) <= ndir_job) ++ndir_side;
+      if (kMulti) ndir_job = job.direct;
       beta = xyz(c);
       fl = __float_as_uint(c.w);
       const uint32_t rec = __float_as_uint(h.y);
@@ -389,12 +390,25 @@ __global__ __launch_bounds__(kShadeBlock) __attribute__((amdgpu_waves_per_eu(MPT
         Lv.z += beta.z * le.z * w;
       }
     }
+    if (active) {
+      // A slot's radiance starts with its camera vertex (a store: k_raygen
+      // leaves it unset); later vertices add with atomics that return
+      // nothing.  Shadow contributions of the same depth come after, in
+      // k_shadow, so each slot sees one ordered sequence of additions.
+      if (depth == 0) {
+        Ls[slot] = Lv;
+      } else if (Lv.x != 0.0f || Lv.y != 0.0f || Lv.z != 0.0f) {
+        atomicAdd(&Ls[slot].x, Lv.x);
+        atomicAdd(&Ls[slot].y, Lv.y);
+        atomicAdd(&Ls[slot].z, Lv.z);
+      }
+    }
     // Next-event estimation (DirectIllumination.cpp:90-135): per light, n
     // shadow rays to points jittered over floor(sqrt(n))^2 strata of the
     // light, each weighted area / n; skipped at specular vertices (:57) and
     // on the light itself (:93).  Every lane of the block runs the same
     // number of appends (the block's largest n), as block_append requires.
-    const uint32_t n_dir = bd->hdr.max_direct;
+    const uint32_t n_dir = kMulti ? bd->hdr.max_direct : 1u;
     for (uint32_t li = 0; li < sc.num_lights; ++li) {
       for (uint32_t k = 0; k < n_dir; ++k) {
         bool want_sh = false;
@@ -407,7 +421,11 @@ __global__ __launch_bounds__(kShadeBlock) __attribute__((amdgpu_waves_per_eu(MPT
             const float4 *lq = sc.quads + 4ull * q;
             const float4 c0 = lq[0], ex = lq[1], ez = lq[2], ln = lq[3];
             const mpt_rng::U4 r = mpt_rng::path_draw(seed, job_id, pixel, sample, depth, 1 + li * ndir_job + k);
-            const uint32_t side = ndir_side, cell = k % (side * side);
+            // floor(sqrt(n)) strata per side
+            uint32_t side = 1;
+            if (kMulti)
+              while ((side + 1) * (side + 1) <= ndir_job) ++side;
+            const uint32_t cell = k % (side * side);
             const float inv = 1.0f / (float)side;
             const float u = ((float)(cell % side) + mpt_rng::unitf(r.x)) * inv;
             const float v = ((float)(cell / side) + mpt_rng::unitf(r.y)) * inv;
@@ -463,21 +481,8 @@ __global__ __launch_bounds__(kShadeBlock) __attribute__((amdgpu_waves_per_eu(MPT
       qout[3 * ei + 1] = make_float4(wo.x, wo.y, wo.z, __uint_as_float(prim));
       qout[3 * ei + 2] = make_float4(beta.x, beta.y, beta.z, __uint_as_float(fl));
     }
-    if (active) {
-      // A slot's radiance starts with its camera vertex (a store: k_raygen
-      // leaves it unset); later vertices add with atomics that return
-      // nothing.  Shadow contributions of the same depth come after, in
-      // k_shadow, so each slot sees one ordered sequence of additions.
-      if (depth == 0) {
-        Ls[slot] = Lv;
-      } else if (Lv.x != 0.0f || Lv.y != 0.0f || Lv.z != 0.0f) {
-        atomicAdd(&Ls[slot].x, Lv.x);
-        atomicAdd(&Ls[slot].y, Lv.y);
-        atomicAdd(&Ls[slot].z, Lv.z);
-      }
-      // the path's flags as k_finalize needs them, once, when it ends
-      if (!want_ext) flags[slot] = fl;
-    }
+    // the path's flags as k_finalize needs them, once, when it ends
+    if (active && !want_ext) flags[slot] = fl;
   }
 }
 
@@ -652,7 +657,7 @@ void launch_shadow(mpt_context *ctx, hipStream_t st, uint64_t cap, uint32_t g, c
 }
 
 // Enqueues the kernel sequence of one batch (the part a graph captures).
-int enqueue_batch(mpt_context *ctx, Lane &ln, mpt_scene *s, uint32_t D, bool instrument) {
+int enqueue_batch(mpt_context *ctx, Lane &ln, mpt_scene *s, uint32_t D, bool multi, bool instrument) {
   Workspace &w = ln.ws;
   const hipStream_t st = ln.stream;
   const uint32_t cap = w.cap, L = s->dev.num_lights;
@@ -687,9 +692,15 @@ int enqueue_batch(mpt_context *ctx, Lane &ln, mpt_scene *s, uint32_t D, bool ins
                   instrument && ctx->counters ? w.totals.p + 3 : nullptr, w.counts.p + 2 * D + 1 + d);
     if ((rc = toc())) return rc;
     if ((rc = tic(1))) return rc;
-    k_shade<<<(g * kBlock + kShadeBlock - 1) / kShadeBlock, kShadeBlock, 0, st>>>(s->dev, w.desc.p, d, qin, w.hits.p, w.counts.p + d, qout,
-                                           w.counts.p + d + 1, w.sq.p, w.counts.p + D + 1 + d, w.Ls.p,
-                                           w.flags.p, w.pjs.p);
+    const uint32_t sgrid = (g * kBlock + kShadeBlock - 1) / kShadeBlock;
+    if (multi)
+      k_shade<true><<<sgrid, kShadeBlock, 0, st>>>(s->dev, w.desc.p, d, qin, w.hits.p, w.counts.p + d, qout,
+                                                  w.counts.p + d + 1, w.sq.p, w.counts.p + D + 1 + d, w.Ls.p,
+                                                  w.flags.p, w.pjs.p);
+    else
+      k_shade<false><<<sgrid, kShadeBlock, 0, st>>>(s->dev, w.desc.p, d, qin, w.hits.p, w.counts.p + d, qout,
+                                                   w.counts.p + d + 1, w.sq.p, w.counts.p + D + 1 + d, w.Ls.p,
+                                                   w.flags.p, w.pjs.p);
     if ((rc = toc())) return rc;
     if (L) {
       if ((rc = tic(2))) return rc;
@@ -940,6 +951,7 @@ int run_batch(mpt_context *ctx, mpt_scene *s, const BatchPlan &plan, float *L_ho
   Lane &ln = ctx->lanes[lane];
   if ((rc = stage_plan(ctx, ln, plan))) return rc;
   const uint32_t D = plan.max_depth;
+  const bool multi = plan.max_direct > 1;
   const bool instrument = ctx->timing || ctx->counters;
   if (ctx->graphs && !instrument) {
     Workspace &w = ln.ws;
@@ -947,14 +959,14 @@ int run_batch(mpt_context *ctx, mpt_scene *s, const BatchPlan &plan, float *L_ho
     const void *key = reinterpret_cast<const void *>((uintptr_t)s->uid);
     for (const GraphEntry &e : ctx->graphs_cache)
       if (e.scene == key && e.lane == lane && e.depth == D && e.cap == w.cap && e.lights == s->dev.num_lights &&
-          e.generation == w.generation) {
+          e.multi == multi && e.generation == w.generation) {
         exec = e.exec;
         break;
       }
     if (!exec) {
       hipGraph_t graph = nullptr;
       MPT_HIP(ctx, hipStreamBeginCapture(ln.stream, hipStreamCaptureModeThreadLocal));
-      rc = enqueue_batch(ctx, ln, s, D, false);
+      rc = enqueue_batch(ctx, ln, s, D, multi, false);
       hipGraph_t captured = nullptr;
       const hipError_t ec = hipStreamEndCapture(ln.stream, &captured);
       if (rc) {
@@ -972,13 +984,14 @@ int run_batch(mpt_context *ctx, mpt_scene *s, const BatchPlan &plan, float *L_ho
       e.depth = D;
       e.cap = w.cap;
       e.lights = s->dev.num_lights;
+      e.multi = multi;
       e.generation = w.generation;
       e.exec = exec;
       ctx->graphs_cache.push_back(e);
     }
     MPT_HIP(ctx, hipGraphLaunch(exec, ln.stream));
   } else {
-    if ((rc = enqueue_batch(ctx, ln, s, D, true))) return rc;
+    if ((rc = enqueue_batch(ctx, ln, s, D, multi, true))) return rc;
   }
   if (L_host) {
     MPT_HIP(ctx, hipMemcpy2DAsync(L_host, 3 * sizeof(float), ln.ws.Ls.p, sizeof(float4), 3 * sizeof(float),
