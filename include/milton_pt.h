@@ -51,7 +51,8 @@ enum {
   MPT_E_NOMEM = -3,       /* device or host allocation failed */
   MPT_E_BUDGET = -4,      /* request exceeds the admission budget */
   MPT_E_NOTFOUND = -5,    /* unknown job handle */
-  MPT_E_CANCELLED = -6    /* job was cancelled */
+  MPT_E_CANCELLED = -6,   /* job was cancelled */
+  MPT_E_IO = -7           /* image file could not be written, or its format is unsupported */
 };
 
 enum { MPT_BSDF_DIFFUSE = 0, MPT_BSDF_PHONG = 1, MPT_BSDF_DIELECTRIC = 2 };
@@ -251,6 +252,50 @@ int mpt_render_film(mpt_scene *scene, const mpt_job *job, mpt_film *film, mpt_re
 int mpt_render_film_range(mpt_scene *scene, const mpt_job *job, uint32_t pix_begin, uint32_t pix_end,
                           mpt_film *film, mpt_render_stats *stats);
 int mpt_film_download(mpt_film *film, float *film_rgb);
+/* The film's accumulators as stored: numerator RGB and denominator per pixel
+ * (W*H*4 floats, row-major), the ProgressiveFilterValue pair
+ * (filters/ProgressiveFilterValue.h:20-37).  Upload replaces them: films
+ * rendered as tiles or on several devices are merged by summing these. */
+int mpt_film_download_raw(mpt_film *film, float *rgbw);
+int mpt_film_upload_raw(mpt_film *film, const float *rgbw);
+
+/* Tone maps.  Replaces ToneMap::create (renderers/ToneMap.cpp:17-31):
+ * "reinhard" and "default" select Reinhard, any other name the gamma-only
+ * map. */
+enum {
+  MPT_TONEMAP_REINHARD = 0, /* ReinhardToneMap::map (renderers/tonemaps/ReinhardToneMap.cpp:25-98) */
+  MPT_TONEMAP_GAMMA = 1     /* NullToneMap::map (renderers/tonemaps/NullToneMap.cpp:21-46) */
+};
+uint32_t mpt_tonemap_from_name(const char *name);
+
+/* The film's finalized 8-bit image, computed on the device: per pixel value
+ * = numerator / denominator (0 where the denominator is 0), then the tone
+ * map over the whole film.  rgba: W*H*4 bytes, row-major from the top row,
+ * bytes r, g, b, a = 255.  Replaces RenderOutput::getFinalizedOutput
+ * (renderers/RenderOutput.cpp:129-231) for point-sample renderers. */
+int mpt_film_tonemap(mpt_film *film, uint32_t tonemap, uint8_t *rgba);
+
+/* Writes the film to a file chosen by the path's extension (case-insensitive),
+ * as FileRenderOutput::finalize -> _save (renderers/outputs/FileRenderOutput.cpp:
+ * 45-76) -> Image::save (common/image/Image.cpp:103-129) does:
+ *   .png  8-bit RGB PNG of the tone-mapped image
+ *   .ppm  binary PPM (P6) of the tone-mapped image
+ *   .hdr / .rgbe  Radiance RGBE, run-length encoded (HDRUtils::saveHDR,
+ *         common/image/HDRUtils.cpp:114-148)
+ *   .pfm  little-endian RGB PFM (HDRUtils::savePFM, :181-184, is
+ *         unimplemented in the reference; here it is written)
+ * HDR formats hold the tone-mapped bytes / 255, as the reference's
+ * RgbaImage output does, unless MPT_SAVE_RADIANCE asks for the linear film
+ * values.  Other extensions (jpg, gif, tiff, exr) fail with MPT_E_IO. */
+enum { MPT_SAVE_RADIANCE = 1u };
+int mpt_film_save(mpt_film *film, const char *path, uint32_t tonemap, uint32_t flags);
+
+/* The writers behind mpt_film_save, on host images (no device needed):
+ * rgba is W*H*4 bytes (alpha ignored), rgb W*H*3 floats, both row-major
+ * from the top row.  An LDR image written to an HDR format becomes
+ * bytes / 255; an HDR image can only go to .hdr, .rgbe or .pfm. */
+int mpt_image_write_ldr(const char *path, uint32_t width, uint32_t height, const uint8_t *rgba);
+int mpt_image_write_hdr(const char *path, uint32_t width, uint32_t height, const float *rgb);
 
 /* Per-path radiance of pixels [pix_begin, pix_end) of a job, all samples,
  * ordered (pixel, sample): L_out holds (pix_end-pix_begin)*spp*3 floats.
@@ -272,6 +317,9 @@ int mpt_scenefile_parse(const char *text, const char *base_dir, mpt_scenefile **
                         size_t err_len);
 const mpt_scene_desc *mpt_scenefile_desc(const mpt_scenefile *scenefile);
 int mpt_scenefile_job(const mpt_scenefile *scenefile, mpt_job *job);
+/* The output's tone map (MPT_TONEMAP_*): "tonemap" of the output object,
+ * read as ToneMap::create does (renderers/RenderOutput.cpp:52-56). */
+uint32_t mpt_scenefile_tonemap(const mpt_scenefile *scenefile);
 int mpt_scenefile_destroy(mpt_scenefile *scenefile);
 
 /* Autotuning.  No reference counterpart: the reference's only knobs on this

@@ -26,6 +26,9 @@ MPT_BSDF_DIFFUSE, MPT_BSDF_PHONG, MPT_BSDF_DIELECTRIC = 0, 1, 2
 MPT_SCENE_ROOM, MPT_SCENE_SOUP, MPT_SCENE_TERRAIN = 1, 2, 3
 MPT_OPT_BATCH_PATHS, MPT_OPT_TIMING, MPT_OPT_COUNTERS, MPT_OPT_GRAPHS, MPT_OPT_STACK, MPT_OPT_PERSISTENT = 1, 2, 3, 4, 5, 6
 MPT_OPT_LANES = 7
+MPT_TONEMAP_REINHARD, MPT_TONEMAP_GAMMA = 0, 1
+MPT_SAVE_RADIANCE = 1
+MPT_E_IO = -7
 ERRORS = {-1: "invalid", -2: "device", -3: "nomem", -4: "budget", -5: "notfound", -6: "cancelled"}
 
 _f32p = ctypes.POINTER(ctypes.c_float)
@@ -153,6 +156,13 @@ def lib():
         "mpt_render_film": ([vp, ctypes.POINTER(Job), vp, ctypes.POINTER(RenderStats)], i32),
         "mpt_render_film_range": ([vp, ctypes.POINTER(Job), u32, u32, vp, ctypes.POINTER(RenderStats)], i32),
         "mpt_film_download": ([vp, vp], i32),
+        "mpt_film_download_raw": ([vp, vp], i32),
+        "mpt_film_upload_raw": ([vp, vp], i32),
+        "mpt_tonemap_from_name": ([ctypes.c_char_p], u32),
+        "mpt_film_tonemap": ([vp, u32, vp], i32),
+        "mpt_film_save": ([vp, ctypes.c_char_p, u32, u32], i32),
+        "mpt_image_write_ldr": ([ctypes.c_char_p, u32, u32, vp], i32),
+        "mpt_image_write_hdr": ([ctypes.c_char_p, u32, u32, vp], i32),
         "mpt_render_paths": ([vp, ctypes.POINTER(Job), u32, u32, vp], i32),
         "mpt_server_create": ([vp, ctypes.POINTER(ServerConfig), ctypes.POINTER(vp)], i32),
         "mpt_server_destroy": ([vp], i32),
@@ -174,6 +184,7 @@ def lib():
                                  ctypes.c_size_t], i32),
         "mpt_scenefile_desc": ([vp], ctypes.POINTER(SceneDesc)),
         "mpt_scenefile_job": ([vp, ctypes.POINTER(Job)], i32),
+        "mpt_scenefile_tonemap": ([vp], u32),
         "mpt_scenefile_destroy": ([vp], i32),
         "mpt_kd_build": ([ctypes.POINTER(SceneDesc), ctypes.POINTER(KdParams), ctypes.POINTER(vp)], i32),
         "mpt_kd_stats": ([vp, ctypes.POINTER(ctypes.c_uint64)], i32),
@@ -480,6 +491,36 @@ class Film:
         self.ctx.check(self.ctx.L.mpt_film_download(self.h, _ptr(out)), "mpt_film_download")
         return out
 
+    def download_raw(self):
+        """(H, W, 4) float32: numerator RGB and denominator per pixel."""
+        out = np.zeros((self.height, self.width, 4), np.float32)
+        self.ctx.check(self.ctx.L.mpt_film_download_raw(self.h, _ptr(out)), "mpt_film_download_raw")
+        return out
+
+    def upload_raw(self, rgbw):
+        a = np.ascontiguousarray(rgbw, np.float32)
+        if a.shape != (self.height, self.width, 4):
+            raise ValueError(f"expected shape {(self.height, self.width, 4)}, got {a.shape}")
+        self.ctx.check(self.ctx.L.mpt_film_upload_raw(self.h, _ptr(a)), "mpt_film_upload_raw")
+
+    def tonemap(self, tonemap=MPT_TONEMAP_REINHARD):
+        """(H, W, 4) uint8: the finalized image (r, g, b, 255); tonemap is
+        MPT_TONEMAP_* or a reference tone-map name ("reinhard", "default",
+        anything else = gamma only)."""
+        if isinstance(tonemap, str):
+            tonemap = tonemap_from_name(tonemap)
+        out = np.zeros((self.height, self.width, 4), np.uint8)
+        self.ctx.check(self.ctx.L.mpt_film_tonemap(self.h, tonemap, _ptr(out)), "mpt_film_tonemap")
+        return out
+
+    def save(self, path, tonemap=MPT_TONEMAP_REINHARD, radiance=False):
+        """Writes .png / .ppm (tone-mapped) or .hdr / .rgbe / .pfm (tone-mapped
+        bytes / 255, or the linear film with radiance=True)."""
+        if isinstance(tonemap, str):
+            tonemap = tonemap_from_name(tonemap)
+        self.ctx.check(self.ctx.L.mpt_film_save(self.h, os.fsencode(path), tonemap,
+                                                MPT_SAVE_RADIANCE if radiance else 0), "mpt_film_save")
+
     def close(self):
         if self.h and self.ctx.h:
             self.ctx.L.mpt_film_destroy(self.h)
@@ -490,6 +531,28 @@ class Film:
             self.close()
         except Exception:
             pass
+
+
+def tonemap_from_name(name):
+    """ToneMap::create's choice: "reinhard" and "default" -> Reinhard, else gamma."""
+    return lib().mpt_tonemap_from_name(name.encode())
+
+
+def write_image(path, image):
+    """Writes a host image: (H, W, 4|3) uint8 to .png/.ppm/.hdr/.rgbe/.pfm, or
+    (H, W, 3) float32 radiance to .hdr/.rgbe/.pfm."""
+    a = np.asarray(image)
+    h, w = a.shape[:2]
+    if a.dtype == np.uint8:
+        if a.shape[2] == 3:
+            a = np.concatenate([a, np.full((h, w, 1), 255, np.uint8)], axis=2)
+        a = np.ascontiguousarray(a)
+        rc = lib().mpt_image_write_ldr(os.fsencode(path), w, h, _ptr(a))
+    else:
+        a = np.ascontiguousarray(a[..., :3], np.float32)
+        rc = lib().mpt_image_write_hdr(os.fsencode(path), w, h, _ptr(a))
+    if rc:
+        raise MptError(f"cannot write {path} (error {rc})")
 
 
 class Server:
@@ -610,7 +673,8 @@ def _copy(ptr, count, dtype, cols=None):
 def load_scenefile(path=None, text=None, base_dir=None):
     """A Milton scene file (mpt_scenefile_load / _parse) -> (SceneArrays, Job).
     The arrays are copies; the job holds the file's camera, film size,
-    samples per pixel (noSuperSamples), depth cap and seed."""
+    samples per pixel (noSuperSamples), depth cap and seed; arrays.tonemap
+    the output's tone map (MPT_TONEMAP_*)."""
     L = lib()
     h = ctypes.c_void_p()
     err = ctypes.create_string_buffer(1024)
@@ -637,6 +701,7 @@ def load_scenefile(path=None, text=None, base_dir=None):
         L.mpt_scenefile_job(h, ctypes.byref(job))
         arrays.camera = {"eye": list(job.camera.eye), "look": list(job.camera.look), "up": list(job.camera.up),
                          "height_angle": job.camera.height_angle, "aspect": job.camera.aspect}
+        arrays.tonemap = L.mpt_scenefile_tonemap(h)   # the output's MPT_TONEMAP_*
         return arrays, job
     finally:
         L.mpt_scenefile_destroy(h)

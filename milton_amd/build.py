@@ -28,7 +28,7 @@ COMMON = [
     "-I", os.path.join(ROOT, "include"),
 ]
 
-SOURCES = ["mpt.hip", "kdbuild.cpp", "scenegen.cpp", "serve.cpp", "tune.cpp", "scenefile.cpp"]
+SOURCES = ["mpt.hip", "kdbuild.cpp", "scenegen.cpp", "serve.cpp", "tune.cpp", "scenefile.cpp", "filmio.cpp"]
 
 
 def _run(cmd):
@@ -39,7 +39,7 @@ def _run(cmd):
 def build(verbose=True):
     srcs = [os.path.join(CSRC, s) for s in SOURCES if os.path.exists(os.path.join(CSRC, s))]
     tmp = LIB + ".tmp"
-    _run([HIPCC, "-x", "hip", *srcs, *COMMON, "-pthread", "-o", tmp])
+    _run([HIPCC, "-x", "hip", *srcs, *COMMON, "-pthread", "-o", tmp, "-lz"])
     os.replace(tmp, LIB)
     plugin_src = os.path.join(CSRC, "plugin.cpp")
     if os.path.exists(plugin_src):

@@ -184,6 +184,8 @@ struct mpt_film {
   mpt_context *ctx = nullptr;
   uint32_t width = 0, height = 0;
   mpt_engine::DBuf<float4> data;
+  mpt_engine::DBuf<uchar4> ldr;     // tone-mapped image
+  mpt_engine::DBuf<double2> tm;     // tone-map partials, then (scale, white)
 };
 
 namespace mpt_engine {

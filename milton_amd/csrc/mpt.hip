@@ -16,6 +16,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cctype>
 #include <atomic>
 #include <chrono>
 #include <thread>
@@ -523,6 +524,113 @@ __global__ __launch_bounds__(256) void k_film(const BatchDesc *__restrict__ bd, 
     b += v.z;
   }
   film[ch.pix0 + (i - ch.pixoff)] = make_float4(r, g, b, (float)spp);
+}
+
+// ---------------------------------------------------------------------------
+// Tone maps on the device, in double precision like the reference
+// (renderers/tonemaps/ReinhardToneMap.cpp:25-98, NullToneMap.cpp:21-46).
+// Reinhard needs two image-wide figures, the log-average luminance (key) and
+// the largest luminance (white): k_tm_reduce folds a fixed grid of partials,
+// k_tm_fold reduces them in a fixed order (the same bits every run), and
+// k_tm_map maps each pixel.  HBM-bound: 16 B read per pixel per pass, 4 B
+// written.
+// ---------------------------------------------------------------------------
+constexpr int kTmBlock = 256, kTmGrid = 512;
+
+// utils/ColorUtils.cpp:115-133: the sRGB (D65) primaries matrices.
+__constant__ double kRgbToXyz[9] = {0.41245322014416, 0.35757958129350, 0.18042258997054,
+                                    0.21267112134122, 0.71515920531078, 0.07216877677613,
+                                    0.01933381646199, 0.11919354020663, 0.95022692228971};
+__constant__ double kXyzToRgb[9] = {3.240479, -1.537150, -0.498535, -0.969256, 1.875992,
+                                    0.041556, 0.055648,  -0.204043, 1.057311};
+
+// numerator / denominator, black where nothing was accumulated
+// (filters/ProgressiveFilterValue.h:27-32)
+__device__ __forceinline__ void film_value(float4 f, double &r, double &g, double &b) {
+  const double w = f.w;
+  r = w == 0 ? 0.0 : f.x / w;
+  g = w == 0 ? 0.0 : f.y / w;
+  b = w == 0 ? 0.0 : f.z / w;
+}
+
+// (unsigned char) CLAMP(v, 0, 255): at or above 255 -> 255, at or below 0
+// or NaN -> 0, else truncation.
+__device__ __forceinline__ uint8_t tm_byte(double v) {
+  if (v >= 255.0) return 255;
+  if (!(v > 0.0)) return 0;
+  return (uint8_t)(int)v;
+}
+
+__global__ __launch_bounds__(kTmBlock) void k_tm_reduce(const float4 *__restrict__ film, uint32_t n,
+                                                       double2 *__restrict__ partials) {
+  __shared__ double2 sh[kTmBlock];
+  double sum = 0.0, top = 0.0;
+  for (uint32_t i = blockIdx.x * kTmBlock + threadIdx.x; i < n; i += kTmGrid * kTmBlock) {
+    double r, g, b;
+    film_value(film[i], r, g, b);
+    const double Y = kRgbToXyz[3] * r + kRgbToXyz[4] * g + kRgbToXyz[5] * b;
+    sum += log(1e-7 + Y);
+    top = fmax(top, Y);
+  }
+  sh[threadIdx.x] = make_double2(sum, top);
+  __syncthreads();
+  for (int w = kTmBlock / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) {
+      sh[threadIdx.x].x += sh[threadIdx.x + w].x;
+      sh[threadIdx.x].y = fmax(sh[threadIdx.x].y, sh[threadIdx.x + w].y);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) partials[blockIdx.x] = sh[0];
+}
+
+// One block: folds the kTmGrid partials and writes (scale, white) after them.
+__global__ __launch_bounds__(kTmBlock) void k_tm_fold(double2 *__restrict__ partials, uint32_t n) {
+  __shared__ double2 sh[kTmBlock];
+  double sum = 0.0, top = 0.0;
+  for (int i = threadIdx.x; i < kTmGrid; i += kTmBlock) {
+    sum += partials[i].x;
+    top = fmax(top, partials[i].y);
+  }
+  sh[threadIdx.x] = make_double2(sum, top);
+  __syncthreads();
+  for (int w = kTmBlock / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) {
+      sh[threadIdx.x].x += sh[threadIdx.x + w].x;
+      sh[threadIdx.x].y = fmax(sh[threadIdx.x].y, sh[threadIdx.x + w].y);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    const double key = exp(sh[0].x / (double)n);
+    const double top_y = sh[0].y;
+    partials[kTmGrid] = make_double2(key <= 0 ? 0.0 : 0.18 / key, top_y <= 0 ? 0.0 : 1.0 / (top_y * top_y));
+  }
+}
+
+__global__ __launch_bounds__(kTmBlock) void k_tm_map(const float4 *__restrict__ film, uint32_t n, uint32_t mode,
+                                                    const double2 *__restrict__ params, uchar4 *__restrict__ out) {
+  const uint32_t i = blockIdx.x * kTmBlock + threadIdx.x;
+  if (i >= n) return;
+  double r, g, b;
+  film_value(film[i], r, g, b);
+  const double correction = 1.0 / 2.22;
+  if (mode == MPT_TONEMAP_REINHARD) {
+    const double scale = params[kTmGrid].x, white = params[kTmGrid].y;
+    const double X = kRgbToXyz[0] * r + kRgbToXyz[1] * g + kRgbToXyz[2] * b;
+    const double Y = kRgbToXyz[3] * r + kRgbToXyz[4] * g + kRgbToXyz[5] * b;
+    const double Z = kRgbToXyz[6] * r + kRgbToXyz[7] * g + kRgbToXyz[8] * b;
+    // chromaticity kept, luminance compressed: L (1 + L white) / (1 + L)
+    const double cx = X / (X + Y + Z), cy = Y / (X + Y + Z);
+    const double lum = scale * Y;
+    const double Yd = lum * (1.0 + lum * white) / (1.0 + lum);
+    const double Xd = Yd * cx / cy, Zd = Yd * (1 - cx - cy) / cy;
+    r = kXyzToRgb[0] * Xd + kXyzToRgb[1] * Yd + kXyzToRgb[2] * Zd;
+    g = kXyzToRgb[3] * Xd + kXyzToRgb[4] * Yd + kXyzToRgb[5] * Zd;
+    b = kXyzToRgb[6] * Xd + kXyzToRgb[7] * Yd + kXyzToRgb[8] * Zd;
+  }
+  out[i] = make_uchar4(tm_byte(255 * pow(r, correction)), tm_byte(255 * pow(g, correction)),
+                       tm_byte(255 * pow(b, correction)), 255);
 }
 
 __global__ void k_totals(const uint32_t *__restrict__ counts, uint32_t depth,
@@ -1627,6 +1735,67 @@ int mpt_film_download(mpt_film *f, float *rgb) {
     rgb[3 * i + 1] = host[i].y * k;
     rgb[3 * i + 2] = host[i].z * k;
   }
+  return MPT_OK;
+}
+
+int mpt_film_download_raw(mpt_film *f, float *rgbw) {
+  if (!f || !rgbw) return MPT_E_INVALID;
+  mpt_context *ctx = f->ctx;
+  MPT_HIP(ctx, hipMemcpyAsync(rgbw, f->data.p, f->data.bytes(), hipMemcpyDeviceToHost, ctx->stream));
+  MPT_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return MPT_OK;
+}
+
+int mpt_film_upload_raw(mpt_film *f, const float *rgbw) {
+  if (!f || !rgbw) return MPT_E_INVALID;
+  mpt_context *ctx = f->ctx;
+  MPT_HIP(ctx, hipMemcpyAsync(f->data.p, rgbw, f->data.bytes(), hipMemcpyHostToDevice, ctx->stream));
+  MPT_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return MPT_OK;
+}
+
+int mpt_film_tonemap(mpt_film *f, uint32_t tonemap, uint8_t *rgba) {
+  if (!f || !rgba) return MPT_E_INVALID;
+  mpt_context *ctx = f->ctx;
+  if (tonemap != MPT_TONEMAP_REINHARD && tonemap != MPT_TONEMAP_GAMMA)
+    return fail(ctx, MPT_E_INVALID, "unknown tone map");
+  const uint32_t n = f->width * f->height;
+  MPT_HIP(ctx, f->ldr.reserve(n));
+  MPT_HIP(ctx, f->tm.reserve(kTmGrid + 1));
+  hipStream_t st = ctx->stream;
+  if (tonemap == MPT_TONEMAP_REINHARD) {
+    k_tm_reduce<<<kTmGrid, kTmBlock, 0, st>>>(f->data.p, n, f->tm.p);
+    k_tm_fold<<<1, kTmBlock, 0, st>>>(f->tm.p, n);
+  }
+  k_tm_map<<<(n + kTmBlock - 1) / kTmBlock, kTmBlock, 0, st>>>(f->data.p, n, tonemap, f->tm.p, f->ldr.p);
+  MPT_LAUNCHED(ctx, "tone map");
+  MPT_HIP(ctx, hipMemcpyAsync(rgba, f->ldr.p, (size_t)n * 4, hipMemcpyDeviceToHost, st));
+  MPT_HIP(ctx, hipStreamSynchronize(st));
+  return MPT_OK;
+}
+
+int mpt_film_save(mpt_film *f, const char *path, uint32_t tonemap, uint32_t flags) {
+  if (!f || !path) return MPT_E_INVALID;
+  mpt_context *ctx = f->ctx;
+  std::string ext(path);
+  const size_t dot = ext.rfind('.');
+  ext = dot == std::string::npos ? std::string() : ext.substr(dot + 1);
+  for (char &c : ext) c = (char)std::tolower((unsigned char)c);
+  const bool hdr = ext == "hdr" || ext == "rgbe" || ext == "pfm";
+  if (!hdr && ext != "png" && ext != "ppm")
+    return fail(ctx, MPT_E_IO, "unsupported image format '" + ext + "' (png, ppm, hdr, rgbe, pfm)");
+  const size_t n = (size_t)f->width * f->height;
+  int rc;
+  if (hdr && (flags & MPT_SAVE_RADIANCE)) {
+    std::vector<float> rgb(3 * n);
+    if ((rc = mpt_film_download(f, rgb.data()))) return rc;
+    rc = mpt_image_write_hdr(path, f->width, f->height, rgb.data());
+  } else {
+    std::vector<uint8_t> rgba(4 * n);
+    if ((rc = mpt_film_tonemap(f, tonemap, rgba.data()))) return rc;
+    rc = mpt_image_write_ldr(path, f->width, f->height, rgba.data());
+  }
+  if (rc) return fail(ctx, rc, std::string("cannot write ") + path);
   return MPT_OK;
 }
 

@@ -24,7 +24,8 @@
 //   background         emitter with a constant "power" only
 //   camera             pinhole / thinlens (aperture ignored), eye, look or
 //                      focus, up, heightAngle, aspectRatio (:842-899)
-//   output             width, height, size, dimensions (default 480 x 480)
+//   output             width, height, size, dimensions (default 480 x 480),
+//                      tonemap (:783; "default" = Reinhard)
 //   renderer           noSuperSamples (spp), noDirectSamples, seed, maxDepth
 //                      (a cap the reference does not have; default 8)
 // Anything else that would change the image (cube, cone, cylinder, blob,
@@ -276,6 +277,7 @@ struct Settings {
   double eye[3] = {2, 2, 2}, look[3] = {-1, -1, -1}, up[3] = {0, 1, 0};
   double height_angle = 45, aspect = 1;
   uint32_t width = 480, height = 480, spp = 1, max_depth = 8, seed = 1, direct = 1;
+  uint32_t tonemap = MPT_TONEMAP_REINHARD;   // RenderOutput::init's "default"
 };
 
 class Loader {
@@ -371,6 +373,10 @@ class Loader {
     }
     if (num(v.find("width"), x)) set_.width = (uint32_t)x;
     if (num(v.find("height"), x)) set_.height = (uint32_t)x;
+    if (const JVal *tm = v.find("tonemap")) {
+      if (tm->kind != JVal::STR) return bad("output.tonemap must be a string");
+      set_.tonemap = mpt_tonemap_from_name(tm->str.c_str());
+    }
     return true;
   }
 
@@ -913,6 +919,8 @@ int mpt_scenefile_load(const char *path, mpt_scenefile **out, char *err, size_t 
 }
 
 const mpt_scene_desc *mpt_scenefile_desc(const mpt_scenefile *sf) { return sf ? &sf->desc : nullptr; }
+
+uint32_t mpt_scenefile_tonemap(const mpt_scenefile *sf) { return sf ? sf->set.tonemap : MPT_TONEMAP_REINHARD; }
 
 int mpt_scenefile_job(const mpt_scenefile *sf, mpt_job *job) {
   if (!sf || !job) return MPT_E_INVALID;

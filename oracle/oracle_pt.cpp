@@ -1037,7 +1037,7 @@ void orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t ou
 
 // Bumped with every change of an exported signature; pyoracle refuses a
 // library built from older sources.
-int orc_interface_version(void) { return 3; }
+int orc_interface_version(void) { return 4; }
 
 orc_scene *orc_scene_create(uint32_t nv, const float *vertices, uint32_t nt, const uint32_t *indices,
                             const uint32_t *tri_material, uint32_t ns, const float *spheres,
@@ -1281,3 +1281,85 @@ double orc_fresnel(const double *wi, const double *n, double n1, double n2) {
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Tone maps: RenderOutput::getFinalizedOutput (renderers/RenderOutput.cpp:
+// 129-231, the non-MLT branch: value = numerator / denominator, black where
+// the denominator is 0, ProgressiveFilterValue.h:27-32) followed by the
+// selected map.  Input: W*H (numerator RGB, denominator) float quadruples.
+// ---------------------------------------------------------------------------
+
+namespace {
+
+// utils/ColorUtils.cpp:115-133: the sRGB (D65) primaries matrices.
+const double kRgbToXyz[9] = {0.41245322014416, 0.35757958129350, 0.18042258997054,
+                             0.21267112134122, 0.71515920531078, 0.07216877677613,
+                             0.01933381646199, 0.11919354020663, 0.95022692228971};
+const double kXyzToRgb[9] = {3.240479, -1.537150, -0.498535, -0.969256, 1.875992,
+                             0.041556, 0.055648,  -0.204043, 1.057311};
+
+void mat3(const double *m, double a, double b, double c, double &x, double &y, double &z) {
+  x = m[0] * a + m[1] * b + m[2] * c;
+  y = m[3] * a + m[4] * b + m[5] * c;
+  z = m[6] * a + m[7] * b + m[8] * c;
+}
+
+// (unsigned char) CLAMP(v, 0, 255) (common/utilities.h:195-200): values at or
+// above 255 give 255, at or below 0 give 0, the rest truncate.  A NaN passes
+// the clamp; x86-64's truncating conversion of it yields byte 0.
+uint8_t to_byte(double v) {
+  if (v >= 255.0) return 255;
+  if (v <= 0.0 || v != v) return 0;
+  return (uint8_t)(int)v;
+}
+
+}  // namespace
+
+extern "C" void orc_tonemap(const float *rgbw, uint64_t n, int mode, uint8_t *rgba) {
+  std::vector<double> v(3 * n);
+  for (uint64_t i = 0; i < n; ++i) {
+    const double w = rgbw[4 * i + 3];
+    for (int c = 0; c < 3; ++c) v[3 * i + c] = w == 0 ? 0.0 : (double)rgbw[4 * i + c] / w;
+  }
+  const double correction = 1.0 / 2.22;
+  if (mode != 0) {
+    // NullToneMap::map (renderers/tonemaps/NullToneMap.cpp:21-46)
+    for (uint64_t i = 0; i < n; ++i) {
+      for (int c = 0; c < 3; ++c) rgba[4 * i + c] = to_byte(255 * std::pow(v[3 * i + c], correction));
+      rgba[4 * i + 3] = 255;
+    }
+    return;
+  }
+  // ReinhardToneMap::map (renderers/tonemaps/ReinhardToneMap.cpp:25-98):
+  // to XYZ, chromaticity x = X/(X+Y+Z), y = Y/(X+Y+Z); key = exp(mean of
+  // log(1e-7 + Y)) over the image (visited last pixel first); white = 1 /
+  // max(Y)^2.
+  std::vector<double> cx(n), cy(n), Y(n);
+  double key = 0, white = 0;
+  for (uint64_t i = n; i--;) {
+    double X, Yv, Z;
+    mat3(kRgbToXyz, v[3 * i], v[3 * i + 1], v[3 * i + 2], X, Yv, Z);
+    cx[i] = X / (X + Yv + Z);
+    cy[i] = Yv / (X + Yv + Z);
+    Y[i] = Yv;
+    key += std::log(1e-7 + Yv);
+    white = std::max(white, Yv);
+  }
+  key = std::exp(key / (double)n);
+  white = white <= 0 ? 0 : 1.0 / (white * white);
+  // scaled luminance L = (0.18 / key) Y, compressed to L (1 + L white) / (1 + L),
+  // back through xyY -> XYZ -> RGB, then gamma 2.22
+  const double scale = key <= 0 ? 0 : 0.18 / key;
+  for (uint64_t i = n; i--;) {
+    const double lum = scale * Y[i];
+    const double Yd = lum * (1.0 + lum * white) / (1.0 + lum);
+    const double X = Yd * cx[i] / cy[i];
+    const double Z = Yd * (1 - cx[i] - cy[i]) / cy[i];
+    double r, g, b;
+    mat3(kXyzToRgb, X, Yd, Z, r, g, b);
+    rgba[4 * i] = to_byte(255 * std::pow(r, correction));
+    rgba[4 * i + 1] = to_byte(255 * std::pow(g, correction));
+    rgba[4 * i + 2] = to_byte(255 * std::pow(b, correction));
+    rgba[4 * i + 3] = 255;
+  }
+}

@@ -110,6 +110,12 @@ void orc_bsdf_eval(const float *material, const double *n, const double *wi,
 /* Fresnel::getFr(wi, N, n1, n2). */
 double orc_fresnel(const double *wi, const double *n, double n1, double n2);
 
+/* Finalized 8-bit image of a film: rgbw holds n (numerator RGB, denominator)
+ * quadruples; mode 0 Reinhard (renderers/tonemaps/ReinhardToneMap.cpp:25-98),
+ * otherwise gamma only (NullToneMap.cpp:21-46); rgba gets n * 4 bytes
+ * (r, g, b, 255). */
+void orc_tonemap(const float *rgbw, uint64_t n, int mode, uint8_t *rgba);
+
 #ifdef __cplusplus
 }
 #endif

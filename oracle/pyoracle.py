@@ -15,7 +15,7 @@ LIB_PATH = os.path.join(HERE, "liboracle_pt.so")
 MISS = 0xFFFFFFFF
 
 
-INTERFACE_VERSION = 3
+INTERFACE_VERSION = 4
 
 
 def build():
@@ -64,6 +64,7 @@ def lib():
         L.orc_bsdf_eval.argtypes = [vp, vp, vp, vp, vp, vp]
         L.orc_fresnel.argtypes = [vp, vp, ctypes.c_double, ctypes.c_double]
         L.orc_fresnel.restype = ctypes.c_double
+        L.orc_tonemap.argtypes = [vp, u64, i32, vp]
         _lib = L
     return _lib
 
@@ -208,3 +209,13 @@ def bsdf_eval(material_row, n, wi, wo):
 def fresnel(wi, n, n1, n2):
     return lib().orc_fresnel(_p(np.ascontiguousarray(wi, np.float64)), _p(np.ascontiguousarray(n, np.float64)),
                              n1, n2)
+
+
+def tonemap(rgbw, mode=0):
+    """Finalized 8-bit image of a film: rgbw (..., 4) float32 numerator RGB and
+    denominator; mode 0 Reinhard, 1 gamma only.  Returns (..., 4) uint8."""
+    a = np.ascontiguousarray(rgbw, np.float32)
+    n = a.size // 4
+    out = np.zeros(a.shape[:-1] + (4,), np.uint8)
+    lib().orc_tonemap(_p(a), n, int(mode), _p(out))
+    return out

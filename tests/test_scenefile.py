@@ -145,6 +145,15 @@ def test_defaults():
     assert job.camera.height_angle == pytest.approx(45)
     np.testing.assert_allclose(a.spheres[0], [0, 0, 0, 0.5], atol=1e-7)   # unit sphere, diameter 1
     assert a.materials[0]["type"] == m.MPT_BSDF_DIFFUSE and np.allclose(a.materials[0]["kd"], 0.5)
+    assert a.tonemap == m.MPT_TONEMAP_REINHARD
+
+
+@pytest.mark.parametrize("name, want", [("reinhard", m.MPT_TONEMAP_REINHARD), ("default", m.MPT_TONEMAP_REINHARD),
+                                        ("null", m.MPT_TONEMAP_GAMMA)])
+def test_output_tonemap(name, want):
+    a, _ = m.load_scenefile(text='{ "scenefile" : { "output" : { "width" : 8, "height" : 8, "tonemap" : "%s" },'
+                                 ' "scene" : { "sphere" : { } } } }' % name)
+    assert a.tonemap == want
 
 
 @pytest.mark.parametrize("snippet, word", [
