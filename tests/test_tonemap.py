@@ -11,6 +11,7 @@ decoding their files with independent readers written here from the
 published formats (zlib for PNG's deflate stream).
 """
 import math
+import re
 import struct
 import zlib
 
@@ -133,11 +134,16 @@ def read_png(path):
 
 
 def read_ppm(path):
+    """Header tokens separated by whitespace; exactly one whitespace byte
+    after maxval, then the raster (which may itself start with a byte that
+    reads as whitespace)."""
     data = open(path, "rb").read()
-    magic, w, h, mx, rest = data.split(maxsplit=4)
-    assert magic == b"P6" and mx == b"255"
-    w, h = int(w), int(h)
-    return np.frombuffer(rest[:3 * w * h], np.uint8).reshape(h, w, 3)
+    hdr = re.match(rb"P6\s+(\d+)\s+(\d+)\s+(\d+)\s", data)
+    assert hdr and hdr.group(3) == b"255"
+    w, h = int(hdr.group(1)), int(hdr.group(2))
+    rest = data[hdr.end():]
+    assert len(rest) == 3 * w * h
+    return np.frombuffer(rest, np.uint8).reshape(h, w, 3)
 
 
 def read_pfm(path):
@@ -205,6 +211,7 @@ def rgbe_within_step(dec, ref):
 def test_png_and_ppm_round_trip(tmp_path):
     rng = np.random.default_rng(5)
     img = rng.integers(0, 256, (13, 21, 4), dtype=np.uint8)
+    img[0, 0, :3] = (10, 32, 9)   # raster starting with bytes that read as whitespace
     m.write_image(tmp_path / "a.png", img)
     m.write_image(tmp_path / "a.PPM", img)
     assert (read_png(tmp_path / "a.png") == img[..., :3]).all()
