@@ -41,7 +41,7 @@
 extern "C" {
 #endif
 
-#define MPT_ABI_VERSION 2u
+#define MPT_ABI_VERSION 3u
 #define MPT_MISS 0xFFFFFFFFu
 
 enum {
@@ -74,6 +74,31 @@ typedef struct mpt_material {
   uint32_t pad[3];
 } mpt_material;
 
+/* Reconstruction kernels of KernelFilter::create (filters/KernelFilter.cpp:
+ * 18-45): box, triangle, gaussian (sigma), mitchell (B, C), lanczosSinc (tau). */
+enum {
+  MPT_FILTER_BOX = 0,
+  MPT_FILTER_TRIANGLE = 1,
+  MPT_FILTER_GAUSSIAN = 2,
+  MPT_FILTER_MITCHELL = 3,
+  MPT_FILTER_LANCZOS = 4
+};
+
+/* A lat-long environment image for the background: EnvironmentMap
+ * (materials/emitters/EnvironmentMap.cpp:24-95) looked up through its
+ * material's kernel filter (KernelFilter::apply, filters/KernelFilter.cpp:
+ * 124-160).  A ray that leaves the scene in direction d sees the image at
+ * the UV of -d (Scene::getBackgroundRadiance, core/Scene.cpp:116-118). */
+typedef struct mpt_env_map {
+  uint32_t width, height; /* texels; both at least 1 */
+  const float *rgb;       /* width * height * 3 floats, row-major from the top row (v = 0) */
+  float repeat_u, repeat_v; /* "repeatU" / "repeatV" (EnvironmentMap.cpp:49-50); 0 = default 1 */
+  uint32_t filter;        /* MPT_FILTER_*; the material's "filter", default triangle (Material.cpp:53) */
+  uint32_t support;       /* "support", default 2 when 0; the kernel is (ceil(support) | 1)^2 taps, at most 15^2 */
+  float params[2];        /* gaussian: sigma (0 = 1); mitchell: B, C (both 0 = 1/3, 1/3);
+                             lanczosSinc: tau (0 = 3) */
+} mpt_env_map;
+
 /* Replaces the scene graph the loader flattens (loaders/SceneGraph.cpp:23-135)
  * and Scene::init (core/Scene.cpp:31-71). */
 typedef struct mpt_scene_desc {
@@ -102,6 +127,9 @@ typedef struct mpt_scene_desc {
    * shapes/Mesh.cpp:335-377 computes the others); both or neither. */
   const float *vertex_normals;
   const uint8_t *vertex_normal_mask;
+  /* Optional (NULL = the constant env_radiance): an environment image in
+   * place of the constant background. */
+  const mpt_env_map *env_map;
 } mpt_scene_desc;
 
 enum { MPT_TRI_ONE_SIDED = 1u };
@@ -296,6 +324,18 @@ int mpt_film_save(mpt_film *film, const char *path, uint32_t tonemap, uint32_t f
  * bytes / 255; an HDR image can only go to .hdr, .rgbe or .pfm. */
 int mpt_image_write_ldr(const char *path, uint32_t width, uint32_t height, const uint8_t *rgba);
 int mpt_image_write_hdr(const char *path, uint32_t width, uint32_t height, const float *rgb);
+
+/* Reads an image as linear RGB floats (row-major from the top row), as
+ * Image::load (common/image/Image.cpp:67-101) does for the formats it reads
+ * without Qt or OpenEXR: .hdr / .rgbe (Radiance RGBE, flat or run-length
+ * encoded scanlines, "-Y H +X W"; HDRUtils::loadHDR, HDRUtils.cpp:36-70,
+ * decoding byte * 2^(e-136)), .pfm (unimplemented in the reference,
+ * HDRUtils.cpp:108-112), and binary .ppm (bytes / 255, as 8-bit images read
+ * through RgbaImage).  Sets *width and *height; fills rgb when it holds at
+ * least width * height * 3 floats (capacity, in floats), else returns
+ * MPT_E_INVALID after setting the size, so a first call with rgb = NULL
+ * asks for it.  MPT_E_IO if the file cannot be read or parsed. */
+int mpt_image_read(const char *path, uint32_t *width, uint32_t *height, float *rgb, size_t capacity);
 
 /* Per-path radiance of pixels [pix_begin, pix_end) of a job, all samples,
  * ordered (pixel, sample): L_out holds (pix_end-pix_begin)*spp*3 floats.

@@ -19,7 +19,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 # compiled with different kernel tunables); the in-tree build by default.
 LIB_PATH = os.environ.get("MPT_LIBRARY") or os.path.join(_HERE, "libmilton_pt.so")
 
-MPT_ABI_VERSION = 2
+MPT_ABI_VERSION = 3
 MPT_TRI_ONE_SIDED = 1
 MPT_MISS = 0xFFFFFFFF
 MPT_BSDF_DIFFUSE, MPT_BSDF_PHONG, MPT_BSDF_DIELECTRIC = 0, 1, 2
@@ -27,6 +27,7 @@ MPT_SCENE_ROOM, MPT_SCENE_SOUP, MPT_SCENE_TERRAIN = 1, 2, 3
 MPT_OPT_BATCH_PATHS, MPT_OPT_TIMING, MPT_OPT_COUNTERS, MPT_OPT_GRAPHS, MPT_OPT_STACK, MPT_OPT_PERSISTENT = 1, 2, 3, 4, 5, 6
 MPT_OPT_LANES = 7
 MPT_TONEMAP_REINHARD, MPT_TONEMAP_GAMMA = 0, 1
+MPT_FILTER_BOX, MPT_FILTER_TRIANGLE, MPT_FILTER_GAUSSIAN, MPT_FILTER_MITCHELL, MPT_FILTER_LANCZOS = 0, 1, 2, 3, 4
 MPT_SAVE_RADIANCE = 1
 MPT_E_IO = -7
 ERRORS = {-1: "invalid", -2: "device", -3: "nomem", -4: "budget", -5: "notfound", -6: "cancelled"}
@@ -42,6 +43,12 @@ class Material(ctypes.Structure):
                 ("Le", ctypes.c_float * 3), ("pad", ctypes.c_uint32 * 3)]
 
 
+class EnvMapDesc(ctypes.Structure):
+    _fields_ = [("width", ctypes.c_uint32), ("height", ctypes.c_uint32), ("rgb", _f32p),
+                ("repeat_u", ctypes.c_float), ("repeat_v", ctypes.c_float), ("filter", ctypes.c_uint32),
+                ("support", ctypes.c_uint32), ("params", ctypes.c_float * 2)]
+
+
 class SceneDesc(ctypes.Structure):
     _fields_ = [("version", ctypes.c_uint32),
                 ("num_vertices", ctypes.c_uint32), ("vertices", _f32p),
@@ -50,7 +57,7 @@ class SceneDesc(ctypes.Structure):
                 ("num_quads", ctypes.c_uint32), ("quads", _f32p), ("quad_material", _u32p),
                 ("num_materials", ctypes.c_uint32), ("materials", ctypes.POINTER(Material)),
                 ("env_radiance", ctypes.c_float * 3), ("tri_flags", _u32p), ("vertex_normals", _f32p),
-                ("vertex_normal_mask", _u8p)]
+                ("vertex_normal_mask", _u8p), ("env_map", ctypes.POINTER(EnvMapDesc))]
 
 
 class KdParams(ctypes.Structure):
@@ -163,6 +170,7 @@ def lib():
         "mpt_film_save": ([vp, ctypes.c_char_p, u32, u32], i32),
         "mpt_image_write_ldr": ([ctypes.c_char_p, u32, u32, vp], i32),
         "mpt_image_write_hdr": ([ctypes.c_char_p, u32, u32, vp], i32),
+        "mpt_image_read": ([ctypes.c_char_p, ctypes.POINTER(u32), ctypes.POINTER(u32), vp, ctypes.c_size_t], i32),
         "mpt_render_paths": ([vp, ctypes.POINTER(Job), u32, u32, vp], i32),
         "mpt_server_create": ([vp, ctypes.POINTER(ServerConfig), ctypes.POINTER(vp)], i32),
         "mpt_server_destroy": ([vp], i32),
@@ -210,11 +218,33 @@ def _ptr(a):
 # ---------------------------------------------------------------------------
 # Scenes as numpy arrays (shared by the device path and the oracle).
 # ---------------------------------------------------------------------------
+class EnvMap:
+    """A lat-long environment image for the background (mpt_env_map): image
+    (H, W, 3) float32 from the top row, repeats, and the kernel filter its
+    lookups go through (MPT_FILTER_*, support, params; zeros = defaults)."""
+
+    def __init__(self, image, repeat_u=1.0, repeat_v=1.0, filter=MPT_FILTER_TRIANGLE, support=2, params=(0.0, 0.0)):
+        self.image = np.ascontiguousarray(image, np.float32).reshape(np.shape(image)[0], -1, 3)
+        self.repeat_u, self.repeat_v = float(repeat_u), float(repeat_v)
+        self.filter, self.support = int(filter), int(support)
+        self.params = (float(params[0]), float(params[1]))
+
+    def desc(self):
+        e = EnvMapDesc()
+        e.height, e.width = self.image.shape[:2]
+        e.rgb = self.image.ctypes.data_as(_f32p)
+        e.repeat_u, e.repeat_v = self.repeat_u, self.repeat_v
+        e.filter, e.support = self.filter, self.support
+        e.params[0], e.params[1] = self.params
+        return e
+
+
 class SceneArrays:
     """Primitive and material arrays in the layout of mpt_scene_desc."""
 
     def __init__(self, vertices, indices, tri_material, spheres, sphere_material, quads, quad_material,
-                 materials, env, camera=None, tri_flags=None, vertex_normals=None, vertex_normal_mask=None):
+                 materials, env, camera=None, tri_flags=None, vertex_normals=None, vertex_normal_mask=None,
+                 env_map=None):
         self.vertices = np.ascontiguousarray(vertices, np.float32).reshape(-1, 3)
         self.indices = np.ascontiguousarray(indices, np.uint32).reshape(-1, 3)
         self.tri_material = np.ascontiguousarray(tri_material, np.uint32).reshape(-1)
@@ -231,6 +261,7 @@ class SceneArrays:
                                else np.ascontiguousarray(vertex_normals, np.float32).reshape(-1, 3))
         self.vertex_normal_mask = (None if vertex_normal_mask is None
                                    else np.ascontiguousarray(vertex_normal_mask, np.uint8).reshape(-1))
+        self.env_map = env_map   # optional EnvMap in place of the constant env
 
     @property
     def num_prims(self):
@@ -259,6 +290,9 @@ class SceneArrays:
         if self.vertex_normals is not None:
             d.vertex_normals = self.vertex_normals.ctypes.data_as(_f32p)
             d.vertex_normal_mask = self.vertex_normal_mask.ctypes.data_as(_u8p)
+        if self.env_map is not None:
+            d._env = self.env_map.desc()   # kept alive with the descriptor
+            d.env_map = ctypes.pointer(d._env)
         return d
 
     def materials_f32(self):
@@ -555,6 +589,20 @@ def write_image(path, image):
         raise MptError(f"cannot write {path} (error {rc})")
 
 
+def read_image(path):
+    """(H, W, 3) float32 linear RGB of a .hdr / .rgbe / .pfm / .ppm file."""
+    L = lib()
+    w, h = ctypes.c_uint32(), ctypes.c_uint32()
+    rc = L.mpt_image_read(os.fsencode(path), ctypes.byref(w), ctypes.byref(h), None, 0)
+    if rc != -1:
+        raise MptError(f"cannot read {path} (error {rc})")
+    out = np.zeros((h.value, w.value, 3), np.float32)
+    rc = L.mpt_image_read(os.fsencode(path), ctypes.byref(w), ctypes.byref(h), _ptr(out), out.size)
+    if rc:
+        raise MptError(f"cannot read {path} (error {rc})")
+    return out
+
+
 class Server:
     """Continuous-batching render server over resident scenes (mpt_server_*)."""
 
@@ -697,6 +745,11 @@ def load_scenefile(path=None, text=None, base_dir=None):
             tri_flags=_copy(d.tri_flags, nt, np.uint32) if d.tri_flags else None,
             vertex_normals=_copy(d.vertex_normals, nv, np.float32, 3) if d.vertex_normals else None,
             vertex_normal_mask=_copy(d.vertex_normal_mask, nv, np.uint8) if d.vertex_normal_mask else None)
+        if d.env_map:
+            e = d.env_map.contents
+            img = _copy(e.rgb, e.width * e.height, np.float32, 3).reshape(e.height, e.width, 3)
+            arrays.env_map = EnvMap(img, e.repeat_u or 1.0, e.repeat_v or 1.0, e.filter, e.support or 2,
+                                    (e.params[0], e.params[1]))
         job = Job()
         L.mpt_scenefile_job(h, ctypes.byref(job))
         arrays.camera = {"eye": list(job.camera.eye), "look": list(job.camera.look), "up": list(job.camera.up),

@@ -55,6 +55,12 @@ struct Scene {
   uint32_t num_lights;
   uint32_t nt, ns, nq;
   float env[3];
+  // environment image (env_w 0: the constant env): texels (r, g, b, 0) from
+  // the top row, the filter kernel (env_m x env_m weights, then their sum)
+  const float4 *env_tex;
+  const double *env_kernel;
+  uint32_t env_w, env_h, env_m;
+  float env_rep_u, env_rep_v;
 };
 
 MPT_HD float3 f3(float x, float y, float z) { return make_float3(x, y, z); }

@@ -171,12 +171,14 @@ struct mpt_scene {
   mpt_engine::DBuf<float4> recs, tshade, spheres, quads;
   mpt_engine::DBuf<uint32_t> sphere_mat, lights;
   mpt_engine::DBuf<mpt_material> mats;
+  mpt_engine::DBuf<float4> env_tex;
+  mpt_engine::DBuf<double> env_kernel;
   mpt_dev::Scene dev{};
   uint64_t stats[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   uint32_t leaf_size = 0;   // kd leaf size the tree was built with
   uint64_t device_bytes() const {
     return nodes.bytes() + recs.bytes() + tshade.bytes() + spheres.bytes() + quads.bytes() + sphere_mat.bytes() +
-           lights.bytes() + mats.bytes();
+           lights.bytes() + mats.bytes() + env_tex.bytes() + env_kernel.bytes();
   }
 };
 

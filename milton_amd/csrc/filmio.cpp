@@ -1,7 +1,9 @@
-// Image file writers behind mpt_film_save and mpt_image_write_*: the file
-// side of FileRenderOutput::_save (renderers/outputs/FileRenderOutput.cpp:
-// 57-76) -> Image::save (common/image/Image.cpp:103-129), which picks the
-// format from the file name's extension.  Host code only.
+// Image files: the writers behind mpt_film_save and mpt_image_write_* (the
+// file side of FileRenderOutput::_save, renderers/outputs/FileRenderOutput.cpp:
+// 57-76 -> Image::save, common/image/Image.cpp:103-129, which picks the
+// format from the file name's extension) and the reader behind
+// mpt_image_read (Image::load, Image.cpp:67-101) for environment images.
+// Host code only.
 //
 // Formats (all rows top to bottom unless the format says otherwise):
 //   PNG   8-bit RGB, one IDAT of zlib-compressed scanlines with filter 0
@@ -17,6 +19,7 @@
 #include <cctype>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -191,9 +194,195 @@ bool write_pfm(const char *path, uint32_t w, uint32_t h, const float *rgb) {
   return f.close();
 }
 
+// ---------------------------------------------------------------------------
+// Readers.
+// ---------------------------------------------------------------------------
+
+struct In {
+  FILE *f;
+  explicit In(const char *path) : f(std::fopen(path, "rb")) {}
+  ~In() {
+    if (f) std::fclose(f);
+  }
+  bool line(std::string &s) {   // one line without its '\n'; false at EOF
+    s.clear();
+    int c;
+    while ((c = std::fgetc(f)) != EOF && c != '\n') s.push_back((char)c);
+    return c != EOF || !s.empty();
+  }
+  bool get(void *p, size_t n) { return std::fread(p, 1, n, f) == n; }
+};
+
+// RGBE_ReadHeader as the reference's copy of the codec reads it
+// (common/image/rgbe.cpp:159-216): lines up to the first blank one, one of
+// them "FORMAT=32-bit_rle_rgbe", then "-Y height +X width".
+bool read_rgbe(const char *path, uint32_t &w, uint32_t &h, std::vector<float> &rgb) {
+  In in(path);
+  if (!in.f) return false;
+  std::string s;
+  bool format = false;
+  for (;;) {
+    if (!in.line(s)) return false;
+    if (s.empty()) break;
+    if (s == "FORMAT=32-bit_rle_rgbe") format = true;
+  }
+  if (!format || !in.line(s)) return false;
+  int hh = 0, ww = 0;
+  if (std::sscanf(s.c_str(), "-Y %d +X %d", &hh, &ww) != 2 || hh <= 0 || ww <= 0) return false;
+  w = (uint32_t)ww;
+  h = (uint32_t)hh;
+  rgb.assign(3 * (size_t)w * h, 0.0f);
+  std::vector<uint8_t> px(4 * (size_t)w);
+  for (uint32_t y = 0; y < h; ++y) {
+    bool flat = w < 8 || w > 0x7fff;
+    size_t have = 0;   // pixels of this scanline already in px
+    if (!flat) {
+      uint8_t b[4];
+      if (!in.get(b, 4)) return false;
+      if (b[0] != 2 || b[1] != 2 || (b[2] & 0x80)) {
+        // not run-length encoded: these four bytes are the first pixel
+        std::memcpy(px.data(), b, 4);
+        have = 1;
+        flat = true;
+      } else {
+        if ((((uint32_t)b[2] << 8) | b[3]) != w) return false;
+        for (int c = 0; c < 4; ++c) {
+          uint32_t x = 0;
+          while (x < w) {
+            uint8_t hd[2];
+            if (!in.get(hd, 2)) return false;
+            if (hd[0] > 128) {
+              const uint32_t n = hd[0] - 128u;
+              if (n > w - x) return false;
+              for (uint32_t i = 0; i < n; ++i) px[4 * (size_t)(x + i) + c] = hd[1];
+              x += n;
+            } else {
+              const uint32_t n = hd[0];
+              if (n == 0 || n > w - x) return false;
+              px[4 * (size_t)x + c] = hd[1];
+              for (uint32_t i = 1; i < n; ++i) {
+                uint8_t v;
+                if (!in.get(&v, 1)) return false;
+                px[4 * (size_t)(x + i) + c] = v;
+              }
+              x += n;
+            }
+          }
+        }
+      }
+    }
+    if (flat && !in.get(px.data() + 4 * have, 4 * (w - have))) return false;
+    float *out = rgb.data() + 3 * (size_t)y * w;
+    for (uint32_t x = 0; x < w; ++x) {
+      const uint8_t *q = &px[4 * (size_t)x];
+      // rgbe2float: byte * 2^(e - 136), zero when e = 0
+      const float f = q[3] ? (float)std::ldexp(1.0, (int)q[3] - 136) : 0.0f;
+      for (int c = 0; c < 3; ++c) out[3 * x + c] = q[c] * f;
+    }
+  }
+  return true;
+}
+
+// Header tokens of PFM / PPM: whitespace-separated, '#' comments (PPM).
+bool token(In &in, std::string &t) {
+  t.clear();
+  int c;
+  while ((c = std::fgetc(in.f)) != EOF) {
+    if (c == '#' && t.empty()) {
+      while ((c = std::fgetc(in.f)) != EOF && c != '\n') {
+      }
+      continue;
+    }
+    if (std::isspace(c)) {
+      if (!t.empty()) return true;   // consumes exactly one whitespace after the token
+      continue;
+    }
+    t.push_back((char)c);
+  }
+  return !t.empty();
+}
+
+bool read_pfm(const char *path, uint32_t &w, uint32_t &h, std::vector<float> &rgb) {
+  In in(path);
+  if (!in.f) return false;
+  std::string magic, sw, sh, ss;
+  if (!token(in, magic) || (magic != "PF" && magic != "Pf") || !token(in, sw) || !token(in, sh) || !token(in, ss))
+    return false;
+  const long ww = std::atol(sw.c_str()), hh = std::atol(sh.c_str());
+  const double scale = std::atof(ss.c_str());
+  if (ww <= 0 || hh <= 0 || scale == 0) return false;
+  w = (uint32_t)ww;
+  h = (uint32_t)hh;
+  const int ch = magic == "PF" ? 3 : 1;
+  const bool big = scale > 0;
+  std::vector<uint8_t> row(4 * (size_t)ch * w);
+  rgb.assign(3 * (size_t)w * h, 0.0f);
+  for (uint32_t y = h; y--;) {   // bottom row first
+    if (!in.get(row.data(), row.size())) return false;
+    for (size_t i = 0; i < (size_t)ch * w; ++i) {
+      uint8_t b[4];
+      std::memcpy(b, &row[4 * i], 4);
+      if (big) {
+        std::swap(b[0], b[3]);
+        std::swap(b[1], b[2]);
+      }
+      float v;
+      std::memcpy(&v, b, 4);
+      const size_t x = i / ch;
+      if (ch == 3) rgb[3 * ((size_t)y * w + x) + i % 3] = v;
+      else
+        for (int c = 0; c < 3; ++c) rgb[3 * ((size_t)y * w + x) + c] = v;
+    }
+  }
+  return true;
+}
+
+bool read_ppm(const char *path, uint32_t &w, uint32_t &h, std::vector<float> &rgb) {
+  In in(path);
+  if (!in.f) return false;
+  std::string magic, sw, sh, sm;
+  if (!token(in, magic) || magic != "P6" || !token(in, sw) || !token(in, sh) || !token(in, sm)) return false;
+  const long ww = std::atol(sw.c_str()), hh = std::atol(sh.c_str()), mx = std::atol(sm.c_str());
+  if (ww <= 0 || hh <= 0 || mx != 255) return false;
+  w = (uint32_t)ww;
+  h = (uint32_t)hh;
+  std::vector<uint8_t> px(3 * (size_t)w * h);
+  if (!in.get(px.data(), px.size())) return false;
+  rgb.resize(px.size());
+  // Rgba32 -> Vector3: byte / 255 (common/image/Rgba.inl:32-36)
+  for (size_t i = 0; i < px.size(); ++i) rgb[i] = (float)(px[i] / 255.0);
+  return true;
+}
+
 }  // namespace
 
 extern "C" {
+
+int mpt_image_read(const char *path, uint32_t *width, uint32_t *height, float *rgb, size_t capacity) {
+  if (!path || !width || !height) return MPT_E_INVALID;
+  uint32_t w = 0, h = 0;
+  std::vector<float> img;
+  bool ok;
+  switch (format_of(path)) {
+    case F_RGBE:
+      ok = read_rgbe(path, w, h, img);
+      break;
+    case F_PFM:
+      ok = read_pfm(path, w, h, img);
+      break;
+    case F_PPM:
+      ok = read_ppm(path, w, h, img);
+      break;
+    default:
+      ok = false;
+  }
+  if (!ok) return MPT_E_IO;
+  *width = w;
+  *height = h;
+  if (!rgb || capacity < img.size()) return MPT_E_INVALID;
+  std::memcpy(rgb, img.data(), img.size() * sizeof(float));
+  return MPT_OK;
+}
 
 uint32_t mpt_tonemap_from_name(const char *name) {
   if (!name) return MPT_TONEMAP_REINHARD;

@@ -31,6 +31,7 @@
 #include "../../include/milton_pt.h"
 #include "device.h"
 #include "engine.h"
+#include "envmap.h"
 #include "kdbuild.h"
 #include "rng.h"
 
@@ -301,6 +302,53 @@ __device__ __forceinline__ HitInfo hit_info(const Scene &sc, uint32_t prim, floa
 // 28-116) with DirectIllumination::evaluate (renderers/DirectIllumination.cpp:
 // 46-136).  The recursion becomes a throughput `beta`; the emission terms, NEE
 // and roulette follow the reference rules, each cited below.
+// Scene::getBackgroundRadiance(d) = EnvironmentMap::getLe(-d)
+// (core/Scene.cpp:116-118; materials/emitters/EnvironmentMap.cpp:44-85) in
+// double precision: spherical coordinates of w = -d, the lat-long UV, the
+// repeat, the texel position clamped into the image, then the kernel
+// filter over the clamped neighbourhood (filters/KernelFilter.cpp:124-160),
+// normalised by the weights' sum.
+// Out of line, with the scene's fields by value: the miss path is rare and
+// its FP64 trigonometry would otherwise raise the shading kernel's register
+// pressure everywhere.
+__device__ __noinline__ float3 env_radiance_at(const float4 *__restrict__ tex, const double *__restrict__ kern,
+                                               uint32_t env_w, uint32_t env_h, uint32_t env_m, float rep_u,
+                                               float rep_v, float3 dir) {
+  constexpr double kPiD = 3.14159265358979323846;
+  const double wx = -(double)dir.x, wy = -(double)dir.y, wz = -(double)dir.z;
+  const double rho = sqrt(wx * wx + wz * wz);
+  const double mag = sqrt(wx * wx + wy * wy + wz * wz);
+  const double theta = rho < fabs(wy) ? acos(rho / mag) * (wy >= 0 ? 1.0 : -1.0) : asin(wy / mag);
+  const double phi = (wx == 0 && wz == 0) ? 0.0 : atan2(wx, wz);
+  const double u = phi / (-2 * kPiD) + 0.5, v = theta / -kPiD + 0.5;
+  const double s = u * rep_u, t = v * rep_v;
+  const double W = env_w, H = env_h;
+  double x = (s - floor(s)) * W, y = (t - floor(t)) * H;
+  x = x >= W - 1 ? W - 1 : (x <= 0 ? 0.0 : x);
+  y = y >= H - 1 ? H - 1 : (y <= 0 ? 0.0 : y);
+  const int xi = (int)floor(x), yi = (int)floor(y), half = (int)(env_m >> 1);
+  const int wmax = (int)env_w - 1, hmax = (int)env_h - 1;
+  double r = 0, g = 0, b = 0;
+  const double *k = kern;
+  for (int i = -half; i <= half; ++i) {
+    const int row = min(max(i + yi, 0), hmax);
+    for (int j = -half; j <= half; ++j) {
+      const int col = min(max(j + xi, 0), wmax);
+      const float4 tx = tex[(size_t)row * env_w + col];
+      const double wgt = *k++;
+      r += tx.x * wgt;
+      g += tx.y * wgt;
+      b += tx.z * wgt;
+    }
+  }
+  const double sum = *k;
+  return f3((float)(r / sum), (float)(g / sum), (float)(b / sum));
+}
+
+__device__ __forceinline__ float3 env_radiance(const Scene &sc, float3 dir) {
+  return env_radiance_at(sc.env_tex, sc.env_kernel, sc.env_w, sc.env_h, sc.env_m, sc.env_rep_u, sc.env_rep_v, dir);
+}
+
 constexpr int kShadeBlock = 512;
 #ifndef MPT_SHADE_WAVES
 #define MPT_SHADE_WAVES 6
@@ -308,7 +356,9 @@ constexpr int kShadeBlock = 512;
 
 // kMulti: some job of the batch draws several light samples per light; the
 // common single-sample form keeps the NEE loop's counters out of registers.
-template <bool kMulti>
+// kEnv: the scene's background is an environment image (a constant
+// otherwise).
+template <bool kMulti, bool kEnv>
 __global__ __launch_bounds__(kShadeBlock) __attribute__((amdgpu_waves_per_eu(MPT_SHADE_WAVES))) void k_shade(Scene sc, const BatchDesc *__restrict__ bd, uint32_t depth,
                                                const float4 *__restrict__ qin, const float2 *__restrict__ hits,
                                                const uint32_t *__restrict__ count_in, float4 *__restrict__ qout,
@@ -356,9 +406,10 @@ __global__ __launch_bounds__(kShadeBlock) __attribute__((amdgpu_waves_per_eu(MPT
       const uint32_t rec = __float_as_uint(h.y);
       if (rec == kMiss) {
         // Scene::getBackgroundRadiance (core/Scene.cpp:116-118), every depth.
-        Lv.x += beta.x * sc.env[0];
-        Lv.y += beta.y * sc.env[1];
-        Lv.z += beta.z * sc.env[2];
+        const float3 le = kEnv ? env_radiance(sc, d) : f3(sc.env[0], sc.env[1], sc.env[2]);
+        Lv.x += beta.x * le.x;
+        Lv.y += beta.y * le.y;
+        Lv.z += beta.z * le.z;
       } else {
         // the closest hit's record: primitive id, and t and barycentrics
         // recomputed in double precision
@@ -801,14 +852,16 @@ int enqueue_batch(mpt_context *ctx, Lane &ln, mpt_scene *s, uint32_t D, bool mul
     if ((rc = toc())) return rc;
     if ((rc = tic(1))) return rc;
     const uint32_t sgrid = (g * kBlock + kShadeBlock - 1) / kShadeBlock;
+    auto shade = [&](auto kern) {
+      kern<<<sgrid, kShadeBlock, 0, st>>>(s->dev, w.desc.p, d, qin, w.hits.p, w.counts.p + d, qout,
+                                          w.counts.p + d + 1, w.sq.p, w.counts.p + D + 1 + d, w.Ls.p, w.flags.p,
+                                          w.pjs.p);
+    };
+    const bool env = s->dev.env_w != 0;
     if (multi)
-      k_shade<true><<<sgrid, kShadeBlock, 0, st>>>(s->dev, w.desc.p, d, qin, w.hits.p, w.counts.p + d, qout,
-                                                  w.counts.p + d + 1, w.sq.p, w.counts.p + D + 1 + d, w.Ls.p,
-                                                  w.flags.p, w.pjs.p);
+      env ? shade(k_shade<true, true>) : shade(k_shade<true, false>);
     else
-      k_shade<false><<<sgrid, kShadeBlock, 0, st>>>(s->dev, w.desc.p, d, qin, w.hits.p, w.counts.p + d, qout,
-                                                   w.counts.p + d + 1, w.sq.p, w.counts.p + D + 1 + d, w.Ls.p,
-                                                   w.flags.p, w.pjs.p);
+      env ? shade(k_shade<false, true>) : shade(k_shade<false, false>);
     if ((rc = toc())) return rc;
     if (L) {
       if ((rc = tic(2))) return rc;
@@ -1198,6 +1251,11 @@ struct HostScene {
   uint32_t nt = 0, ns = 0, nq = 0;
   uint32_t leaf_size = 0;   // effective kd leaf size
   float env[3] = {0, 0, 0};
+  // environment image, device layout; kernel weights then their sum
+  std::vector<float4> env_tex;
+  std::vector<double> env_kernel;
+  uint32_t env_w = 0, env_h = 0, env_m = 0;
+  float env_rep_u = 1, env_rep_v = 1;
 };
 
 int prepare_scene(mpt_context *ctx, const mpt_scene_desc *d, const mpt_kd_params *kd, HostScene &hs) {
@@ -1380,6 +1438,24 @@ int prepare_scene(mpt_context *ctx, const mpt_scene_desc *d, const mpt_kd_params
   return MPT_OK;
 }
 
+// The environment image of a scene description, in the device layout.
+int prepare_env(mpt_context *ctx, const mpt_scene_desc *d, HostScene &hs) {
+  if (!d->env_map) return MPT_OK;
+  const mpt_env_map &e = *d->env_map;
+  if (!e.width || !e.height || !e.rgb) return fail(ctx, MPT_E_INVALID, "environment map without texels");
+  if ((uint64_t)e.width * e.height > (1ull << 28)) return fail(ctx, MPT_E_INVALID, "environment map too large");
+  std::string err;
+  if (!mpt_env::kernel(e, hs.env_kernel, hs.env_m, err)) return fail(ctx, MPT_E_INVALID, err);
+  const size_t n = (size_t)e.width * e.height;
+  hs.env_tex.resize(n);
+  for (size_t i = 0; i < n; ++i) hs.env_tex[i] = make_float4(e.rgb[3 * i], e.rgb[3 * i + 1], e.rgb[3 * i + 2], 0);
+  hs.env_w = e.width;
+  hs.env_h = e.height;
+  hs.env_rep_u = e.repeat_u != 0 ? e.repeat_u : 1.0f;
+  hs.env_rep_v = e.repeat_v != 0 ? e.repeat_v : 1.0f;
+  return MPT_OK;
+}
+
 // Points a device-layout Scene at host (or device) arrays.
 void bind_scene(Scene &sc, const uint2 *nodes, const float4 *recs, const mpt_kd::Box &box, const float4 *tshade,
                 const float4 *sph, const uint32_t *sph_mat,
@@ -1408,7 +1484,7 @@ void bind_scene(Scene &sc, const uint2 *nodes, const float4 *recs, const mpt_kd:
 
 struct mpt_kd_tree {
   HostScene hs;
-  Scene sc;
+  Scene sc{};
 };
 
 extern "C" {
@@ -1518,6 +1594,7 @@ int mpt_scene_create(mpt_context *ctx, const mpt_scene_desc *d, const mpt_kd_par
   *out = nullptr;
   HostScene hs;
   int rc = prepare_scene(ctx, d, kd, hs);
+  if (!rc) rc = prepare_env(ctx, d, hs);
   if (rc) return rc;
   MPT_HIP(ctx, hipSetDevice(ctx->device));
   std::unique_ptr<mpt_scene> s(new (std::nothrow) mpt_scene());
@@ -1533,9 +1610,18 @@ int mpt_scene_create(mpt_context *ctx, const mpt_scene_desc *d, const mpt_kd_par
   if ((rc = upload(ctx, s->quads, hs.quads.data(), hs.quads.size()))) return rc;
   if ((rc = upload(ctx, s->mats, hs.mats.data(), hs.mats.size()))) return rc;
   if ((rc = upload(ctx, s->lights, hs.lights.data(), hs.lights.size()))) return rc;
+  if ((rc = upload(ctx, s->env_tex, hs.env_tex.data(), hs.env_tex.size()))) return rc;
+  if ((rc = upload(ctx, s->env_kernel, hs.env_kernel.data(), hs.env_kernel.size()))) return rc;
   MPT_HIP(ctx, hipStreamSynchronize(ctx->stream));
   bind_scene(s->dev, s->nodes.p, s->recs.p, ko.box, s->tshade.p, s->spheres.p,
              s->sphere_mat.p, s->quads.p, s->mats.p, s->lights.p, (uint32_t)hs.lights.size(), nt, ns, nq, hs.env);
+  s->dev.env_tex = s->env_tex.p;
+  s->dev.env_kernel = s->env_kernel.p;
+  s->dev.env_w = hs.env_w;
+  s->dev.env_h = hs.env_h;
+  s->dev.env_m = hs.env_m;
+  s->dev.env_rep_u = hs.env_rep_u;
+  s->dev.env_rep_v = hs.env_rep_v;
   s->stats[0] = ko.n_nodes;
   s->stats[1] = ko.n_leaves;
   s->stats[2] = ko.n_refs;

@@ -21,7 +21,8 @@
 //   sphere             unit sphere of radius 0.5 under position / radius
 //                      (:569-583) and the transform (similarities only)
 //   mesh               OBJ or ASCII PLY file, optional "normalize" (:484-525)
-//   background         emitter with a constant "power" only
+//   background         emitter with a constant "power", or an "environment"
+//                      image (.hdr / .rgbe / .pfm / .ppm) with its filter
 //   camera             pinhole / thinlens (aperture ignored), eye, look or
 //                      focus, up, heightAngle, aspectRatio (:842-899)
 //   output             width, height, size, dimensions (default 480 x 480),
@@ -29,8 +30,9 @@
 //   renderer           noSuperSamples (spp), noDirectSamples, seed, maxDepth
 //                      (a cap the reference does not have; default 8)
 // Anything else that would change the image (cube, cone, cylinder, blob,
-// point, dynamic plugins, emitting triangles or spheres, environment images,
-// non-uniform sphere scales, sheared planes, per-channel IOR) is an error,
+// point, dynamic plugins, emitting triangles or spheres, images Qt or
+// OpenEXR would read, non-uniform sphere scales, sheared planes, per-channel
+// IOR) is an error,
 // not silently dropped.
 #include <algorithm>
 #include <cctype>
@@ -46,6 +48,7 @@
 #include <vector>
 
 #include "../../include/milton_pt.h"
+#include "envmap.h"
 
 namespace {
 
@@ -271,6 +274,9 @@ struct Flat {
   std::vector<int> mat_emits;            // per material: has an emitter with power
   std::vector<double> mat_power;         // 3 per material
   float env[3] = {0, 0, 0};
+  bool has_env_map = false;              // background is an environment image
+  std::vector<float> env_rgb;
+  mpt_env_map env_map{};
 };
 
 struct Settings {
@@ -444,12 +450,66 @@ class Loader {
     const JVal *e = v.find("emitter");
     if (!e || e->kind != JVal::OBJ) return bad("background needs one emitter");
     const JVal *type = e->find("type");
-    if (type && type->kind == JVal::STR && type->str == "environment" && e->find("path"))
-      return bad("environment-map images are not supported; give the background emitter a constant power");
+    if (type && type->kind == JVal::STR && type->str == "environment") return environment(*e);
     double p[3];
     if (!spectrum(e->find("power"), p)) return bad("background emitter needs a power");
     // a constant environment emits its power as radiance (1x1 image semantics)
     for (int i = 0; i < 3; ++i) flat_.env[i] = (float)p[i];
+    return true;
+  }
+
+  // An "environment" background emitter (MiltonJSONSceneLoader.cpp:1025-1048):
+  // the image at "path" (relative paths against the scene file's directory),
+  // "repeatU" / "repeatV", and the material's kernel filter: "filter"
+  // (Material::init default "triangle", materials/Material.cpp:52-56) with
+  // "support", "sigma", "B", "C", "tau" (filters/KernelFilter.cpp:18-45).
+  bool environment(const JVal &e) {
+    const JVal *path = e.find("path");
+    if (!path || path->kind != JVal::STR)
+      return bad("environment map must contain a 'path' to an environment texture");
+    const std::string &rel = path->str;
+    const std::string file = (!rel.empty() && rel[0] == '/') || base_.empty() ? rel : base_ + "/" + rel;
+    uint32_t w = 0, h = 0;
+    if (mpt_image_read(file.c_str(), &w, &h, nullptr, 0) != MPT_E_INVALID)
+      return bad("environment image '" + file + "' cannot be read (hdr, rgbe, pfm or ppm)");
+    flat_.env_rgb.assign(3 * (size_t)w * h, 0.0f);
+    if (mpt_image_read(file.c_str(), &w, &h, flat_.env_rgb.data(), flat_.env_rgb.size()) != MPT_OK)
+      return bad("environment image '" + file + "' cannot be read");
+    mpt_env_map &em = flat_.env_map;
+    em = mpt_env_map{};
+    em.width = w;
+    em.height = h;
+    em.filter = MPT_FILTER_TRIANGLE;
+    double x;
+    if (num(e.find("repeatU"), x)) em.repeat_u = (float)x;
+    if (num(e.find("repeatV"), x)) em.repeat_v = (float)x;
+    if (const JVal *f = e.find("filter")) {
+      if (f->kind != JVal::STR) return bad("filter must be a string");
+      const std::string &n = f->str;
+      if (n == "box") em.filter = MPT_FILTER_BOX;
+      else if (n == "triangle") em.filter = MPT_FILTER_TRIANGLE;
+      else if (n == "gaussian") em.filter = MPT_FILTER_GAUSSIAN;
+      else if (n == "mitchell") em.filter = MPT_FILTER_MITCHELL;
+      else if (n == "lanczosSinc") em.filter = MPT_FILTER_LANCZOS;
+      else return bad("encountered unexpected filter: " + n);
+    }
+    if (num(e.find("support"), x)) {
+      if (!(x >= 1)) return bad("filter support must be at least 1");
+      em.support = (uint32_t)x;
+    }
+    if (em.filter == MPT_FILTER_GAUSSIAN && num(e.find("sigma"), x)) em.params[0] = (float)x;
+    if (em.filter == MPT_FILTER_MITCHELL) {
+      em.params[0] = 1.0f / 3.0f;
+      em.params[1] = 1.0f / 3.0f;
+      if (num(e.find("B"), x)) em.params[0] = (float)x;
+      if (num(e.find("C"), x)) em.params[1] = (float)x;
+    }
+    if (em.filter == MPT_FILTER_LANCZOS && num(e.find("tau"), x)) em.params[0] = (float)x;
+    std::vector<double> k;
+    uint32_t m;
+    std::string err;
+    if (!mpt_env::kernel(em, k, m, err)) return bad(err);
+    flat_.has_env_map = true;
     return true;
   }
 
@@ -859,6 +919,10 @@ int finish(Loader &ld, mpt_scenefile **out, char *err, size_t err_len) {
   d.num_materials = (uint32_t)f.mats.size();
   d.materials = f.mats.data();
   for (int i = 0; i < 3; ++i) d.env_radiance[i] = f.env[i];
+  if (f.has_env_map) {
+    f.env_map.rgb = f.env_rgb.data();
+    d.env_map = &f.env_map;
+  }
   d.tri_flags = f.tri_flags.data();
   d.vertex_normals = f.any_normals ? f.normals.data() : nullptr;
   d.vertex_normal_mask = f.any_normals ? f.vert_has_normal.data() : nullptr;

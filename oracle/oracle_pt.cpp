@@ -549,7 +549,47 @@ struct orc_scene {
   std::vector<Material> mats;
   std::vector<uint32_t> lights;   // quad indices with emissive materials
   V3 env;
+  // Environment image (EnvironmentMap, materials/emitters/EnvironmentMap.cpp):
+  // texels in double, row-major from the top row; the filter kernel as
+  // KernelFilter::_computeKernel tabulates it (m x m, row-major).
+  uint32_t env_w = 0, env_h = 0, env_m = 0;
+  std::vector<V3> env_img;
+  std::vector<double> env_kernel;
+  double env_rep_u = 1, env_rep_v = 1;
   uint32_t nt = 0, ns = 0, nq = 0;
+
+  // Scene::getBackgroundRadiance(d) = background->getLe(-d) (core/Scene.cpp:116-118).
+  V3 background(const V3 &d) const {
+    if (!env_w) return env;
+    // EnvironmentMap::getLe (EnvironmentMap.cpp:44-61) with getSphericalCoords
+    // (:63-73) and getUV (:79-85) of w = -d
+    const V3 w = -d;
+    const double rho = std::sqrt(w.x * w.x + w.z * w.z);
+    const double mag = std::sqrt(w.x * w.x + w.y * w.y + w.z * w.z);
+    const double theta = rho < std::fabs(w.y) ? std::acos(rho / mag) * (w.y >= 0 ? 1 : -1) : std::asin(w.y / mag);
+    const double phi = (w.x == 0 && w.z == 0) ? 0 : std::atan2(w.x, w.z);
+    const double u = phi / (-2 * M_PI) + 0.5, v = theta / -M_PI + 0.5;
+    const double ss = u * env_rep_u, tt = v * env_rep_v;
+    auto clampd = [](double x, double lo, double hi) { return x >= hi ? hi : (x <= lo ? lo : x); };
+    const double x = clampd((ss - std::floor(ss)) * env_w, 0, env_w - 1.0);
+    const double y = clampd((tt - std::floor(tt)) * env_h, 0, env_h - 1.0);
+    // KernelFilter::apply (filters/KernelFilter.cpp:124-160)
+    const int xi = (int)std::floor(x), yi = (int)std::floor(y), half = (int)(env_m >> 1);
+    V3 rgb;
+    double sum = 0;
+    size_t k = 0;
+    for (int i = -half; i <= half; ++i) {
+      const int row = std::min(std::max(i + yi, 0), (int)env_h - 1);
+      for (int j = -half; j <= half; ++j) {
+        const int col = std::min(std::max(j + xi, 0), (int)env_w - 1);
+        const V3 &t = env_img[(size_t)row * env_w + col];
+        const double wgt = env_kernel[k++];
+        sum += wgt;
+        rgb = rgb + t * wgt;
+      }
+    }
+    return rgb / sum;
+  }
   KdTree mesh_tree;   // over triangles (Mesh::init, shapes/Mesh.cpp:211-251)
   KdTree top_tree;    // over shapes: [mesh?, spheres..., quads...] (ShapeSet.cpp:36-80)
   std::vector<uint32_t> top_kind;   // 0 mesh, 1 sphere, 2 quad
@@ -935,7 +975,7 @@ V3 trace_path(const orc_scene &sc, const orc_job &job, const Camera &cam, uint32
     ctr.extension++;
     const double t = sc.closest(ray, gid);
     if (!(t > kEps && t < kInf)) {
-      L = L + mul(beta, sc.env);   // Scene::getBackgroundRadiance (Scene.cpp:116-118)
+      L = L + mul(beta, sc.background(ray.d));   // Scene::getBackgroundRadiance (Scene.cpp:116-118)
       break;
     }
     const V3 p = ray.o + ray.d * t;
@@ -1037,7 +1077,7 @@ void orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t ou
 
 // Bumped with every change of an exported signature; pyoracle refuses a
 // library built from older sources.
-int orc_interface_version(void) { return 4; }
+int orc_interface_version(void) { return 5; }
 
 orc_scene *orc_scene_create(uint32_t nv, const float *vertices, uint32_t nt, const uint32_t *indices,
                             const uint32_t *tri_material, uint32_t ns, const float *spheres,
@@ -1362,4 +1402,74 @@ extern "C" void orc_tonemap(const float *rgbw, uint64_t n, int mode, uint8_t *rg
     rgba[4 * i + 2] = to_byte(255 * std::pow(b, correction));
     rgba[4 * i + 3] = 255;
   }
+}
+
+// ---------------------------------------------------------------------------
+// Environment images.
+// ---------------------------------------------------------------------------
+namespace {
+
+// Filter2D::evaluate of the five KernelFilter types
+// (filters/{Box,Triangle,Gaussian,Mitchell,LanczosSinc}Filter.cpp).
+double orc_filter(int type, double support, const double *p, double x, double y) {
+  if (type == 0) return 2.0 * std::sqrt(x * x + y * y) < support;   // BoxFilter.cpp:14-16
+  if (type == 1) {                                                    // TriangleFilter.cpp:14-18
+    const double radius = support / 2.0;
+    return std::max(0.0, 1.0 - x / radius) * std::max(0.0, 1.0 - y / radius);
+  }
+  if (type == 2) {   // GaussianFilter.cpp:22-31
+    const double coeff = (-0.5) / (p[0] * p[0]);
+    return std::max(0.0, (1.0 / (std::sqrt(2.0 * M_PI) * p[0])) * std::exp(coeff * (x * x + y * y)));
+  }
+  if (type == 3) {   // MitchellFilter.cpp:27-43
+    auto f = [&](double t) {
+      t = std::fabs(2.0 * t / support);
+      const double B = p[0], C = p[1];
+      if (t > 1.0) return (((-B / 6.0 - C) * t + (B + 5.0 * C)) * t + (-2.0 * B - 8.0 * C)) * t + (4.0 / 3.0 * B + 4.0 * C);
+      return t * t * ((2.0 - 1.5 * B - C) * t + (-3.0 + 2.0 * B + C)) + (1.0 - B / 3.0);
+    };
+    return f(x) * f(y);
+  }
+  auto f = [&](double t) {   // LanczosSincFilter.cpp:26-45
+    t = std::fabs(t / support);
+    if (t < kEps) return 1.0;
+    if (t > 1) return 0.0;
+    t *= M_PI;
+    return std::sin(t * p[0]) / (t * p[0]) * (std::sin(t) / t);
+  };
+  return f(x) * f(y);
+}
+
+}  // namespace
+
+extern "C" int orc_scene_set_env_map(orc_scene *s, uint32_t w, uint32_t h, const float *rgb, float repeat_u,
+                                     float repeat_v, int filter, uint32_t support, const float *params) {
+  if (!s || !w || !h || !rgb || filter < 0 || filter > 4) return -1;
+  support = support ? support : 2;
+  // KernelFilter(support): m_width = ceil(support) | 1 (filters/KernelFilter.h:26-29)
+  const uint32_t m = ((uint32_t)std::ceil((double)support)) | 1u;
+  double p[2] = {params ? params[0] : 0.0, params ? params[1] : 0.0};
+  if (filter == 2 && p[0] == 0) p[0] = 1.0;
+  if (filter == 3 && p[0] == 0 && p[1] == 0) p[0] = p[1] = 1.0 / 3.0;
+  if (filter == 4 && p[0] == 0) p[0] = 3.0;
+  // KernelFilter::_computeKernel (filters/KernelFilter.cpp:162-176)
+  const double half = m / 2.0;
+  s->env_kernel.assign((size_t)m * m, 0.0);
+  for (uint32_t i = m; i--;)
+    for (uint32_t j = m; j--;) s->env_kernel[(size_t)i * m + j] = orc_filter(filter, support, p, i - half, j - half);
+  s->env_m = m;
+  s->env_w = w;
+  s->env_h = h;
+  s->env_img.resize((size_t)w * h);
+  for (size_t i = 0; i < (size_t)w * h; ++i) s->env_img[i] = V3(rgb[3 * i], rgb[3 * i + 1], rgb[3 * i + 2]);
+  s->env_rep_u = repeat_u ? repeat_u : 1.0;
+  s->env_rep_v = repeat_v ? repeat_v : 1.0;
+  return 0;
+}
+
+extern "C" void orc_background(const orc_scene *s, const double *d, double *rgb) {
+  const V3 L = s->background(V3(d[0], d[1], d[2]));
+  rgb[0] = L.x;
+  rgb[1] = L.y;
+  rgb[2] = L.z;
 }

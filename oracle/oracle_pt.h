@@ -110,6 +110,18 @@ void orc_bsdf_eval(const float *material, const double *n, const double *wi,
 /* Fresnel::getFr(wi, N, n1, n2). */
 double orc_fresnel(const double *wi, const double *n, double n1, double n2);
 
+/* Replaces the scene's constant background by a lat-long environment image
+ * (materials/emitters/EnvironmentMap.cpp:24-95) looked up through a kernel
+ * filter (filters/KernelFilter.cpp:124-176): rgb holds w*h*3 floats from
+ * the top row; filter 0 box, 1 triangle, 2 gaussian (params[0] = sigma),
+ * 3 mitchell (B, C), 4 lanczosSinc (tau); zero params and repeats take the
+ * reference's defaults, support 0 means 2.  Returns 0, or -1 on bad
+ * arguments. */
+int orc_scene_set_env_map(orc_scene *s, uint32_t w, uint32_t h, const float *rgb, float repeat_u, float repeat_v,
+                          int filter, uint32_t support, const float *params);
+/* Background radiance seen by a ray leaving the scene in direction d. */
+void orc_background(const orc_scene *s, const double *d, double *rgb);
+
 /* Finalized 8-bit image of a film: rgbw holds n (numerator RGB, denominator)
  * quadruples; mode 0 Reinhard (renderers/tonemaps/ReinhardToneMap.cpp:25-98),
  * otherwise gamma only (NullToneMap.cpp:21-46); rgba gets n * 4 bytes

@@ -15,7 +15,7 @@ LIB_PATH = os.path.join(HERE, "liboracle_pt.so")
 MISS = 0xFFFFFFFF
 
 
-INTERFACE_VERSION = 4
+INTERFACE_VERSION = 5
 
 
 def build():
@@ -65,6 +65,8 @@ def lib():
         L.orc_fresnel.argtypes = [vp, vp, ctypes.c_double, ctypes.c_double]
         L.orc_fresnel.restype = ctypes.c_double
         L.orc_tonemap.argtypes = [vp, u64, i32, vp]
+        L.orc_scene_set_env_map.argtypes = [vp, u32, u32, vp, ctypes.c_float, ctypes.c_float, i32, u32, vp]
+        L.orc_background.argtypes = [vp, vp, vp]
         _lib = L
     return _lib
 
@@ -105,6 +107,22 @@ class OracleScene:
             kd_min_prims, kd_max_depth, 1 if naive else 0, self.threads)
         if not self.h:
             raise ValueError("oracle rejected the scene")
+        em = getattr(a, "env_map", None)
+        if em is not None:
+            img = np.ascontiguousarray(em.image, np.float32)
+            self._keep.append(img)
+            prm = np.ascontiguousarray(em.params, np.float32)
+            if lib().orc_scene_set_env_map(self.h, img.shape[1], img.shape[0], _p(img), em.repeat_u, em.repeat_v,
+                                           em.filter, em.support, _p(prm)):
+                raise ValueError("oracle rejected the environment map")
+
+    def background(self, dirs):
+        """Background radiance for ray directions (n, 3) -> (n, 3) float64."""
+        d = np.ascontiguousarray(dirs, np.float64).reshape(-1, 3)
+        out = np.zeros_like(d)
+        for i in range(len(d)):
+            lib().orc_background(self.h, _p(d[i]), _p(out[i]))
+        return out
 
     def kd_stats(self, check=False):
         out = np.zeros(6, np.uint64)

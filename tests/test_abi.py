@@ -31,7 +31,8 @@ def test_library_exports_every_declared_symbol():
 
 _C_STRUCTS = {"Material": "mpt_material", "SceneDesc": "mpt_scene_desc", "KdParams": "mpt_kd_params",
               "Camera": "mpt_camera", "Job": "mpt_job", "Ray": "mpt_ray", "RenderStats": "mpt_render_stats",
-              "ServerConfig": "mpt_server_config", "JobStatus": "mpt_job_status", "TuneResult": "mpt_tune_result"}
+              "ServerConfig": "mpt_server_config", "JobStatus": "mpt_job_status", "TuneResult": "mpt_tune_result",
+              "EnvMapDesc": "mpt_env_map"}
 
 
 def test_struct_layouts_match_header(tmp_path):

@@ -257,7 +257,8 @@ def test_reference_scene_files():
             loaded.append((name, a.num_prims))
         except m.MptError as e:
             msg = str(e)
-            assert any(w in msg for w in ("not supported", "cannot be opened", "syntax", "needs", "no supported")), msg
+            assert any(w in msg for w in ("not supported", "cannot be opened", "cannot be read", "syntax", "needs",
+                                          "no supported")), msg
             refused.append((name, msg))
     print("loaded:", loaded)
     print("refused:", refused)
