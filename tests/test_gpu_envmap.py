@@ -69,7 +69,7 @@ def test_scenefile_environment(ctx, tmp_path):
     m.write_image(tmp_path / "sky.hdr", _sky(16, 32))
     text = """{ "scenefile" : {
       "renderer" : { "noSuperSamples" : 4, "maxDepth" : 4, "seed" : 9 },
-      "camera" : { "eye" : [0, 0.3, -2.5], "look" : [0, -0.1, 1], "heightAngle" : 50 },
+      "camera" : { "type" : "pinhole", "eye" : [0, 0.3, -2.5], "look" : [0, -0.1, 1], "heightAngle" : 50 },
       "output" : { "width" : 32, "height" : 32 },
       "scene" : {
         "background" : { "emitter" : { "type" : "environment", "path" : "sky.hdr", "filter" : "box" } },
