@@ -157,6 +157,13 @@ typedef struct mpt_camera {
   float up[3];
   float height_angle; /* degrees, "heightAngle" */
   float aspect;       /* "aspectRatio" */
+  /* Thin lens (ThinLensCamera::init / getWorldRay, cameras/ThinLensCamera.cpp:
+   * 62-129, 131-172); aperture 0 is the pinhole (PinholeCamera.cpp:21-25). */
+  float aperture;       /* lens diameter in metres (the file's "aperture" is in mm) */
+  float focal_length;   /* metres: fstop * aperture */
+  float focal_distance; /* metres; with autofocus, the value used when the focus ray misses */
+  uint32_t autofocus;   /* nonzero: focal distance = hit distance of the ray through focal_point */
+  float focal_point[2]; /* [0,1]^2 film position of the focus ray ("focalPoint", default 0.5, 0.5) */
 } mpt_camera;
 
 /* One render job: the film ("width"/"height", RenderOutput.cpp:34), samples

@@ -70,7 +70,9 @@ class KdParams(ctypes.Structure):
 
 class Camera(ctypes.Structure):
     _fields_ = [("eye", ctypes.c_float * 3), ("look", ctypes.c_float * 3), ("up", ctypes.c_float * 3),
-                ("height_angle", ctypes.c_float), ("aspect", ctypes.c_float)]
+                ("height_angle", ctypes.c_float), ("aspect", ctypes.c_float), ("aperture", ctypes.c_float),
+                ("focal_length", ctypes.c_float), ("focal_distance", ctypes.c_float), ("autofocus", ctypes.c_uint32),
+                ("focal_point", ctypes.c_float * 2)]
 
 
 class Job(ctypes.Structure):
@@ -371,6 +373,14 @@ def make_job(camera, width, height, spp, max_depth, seed=1, job_id=0, direct_sam
         j.camera.up[k] = camera["up"][k]
     j.camera.height_angle = camera["height_angle"]
     j.camera.aspect = camera["aspect"]
+    # thin lens (optional): aperture and focal length in metres, focal distance,
+    # autofocus through focal_point
+    j.camera.aperture = camera.get("aperture", 0.0)
+    j.camera.focal_length = camera.get("focal_length", 0.0)
+    j.camera.focal_distance = camera.get("focal_distance", 1.0)
+    j.camera.autofocus = 1 if camera.get("autofocus", False) else 0
+    fp = camera.get("focal_point", (0.5, 0.5))
+    j.camera.focal_point[0], j.camera.focal_point[1] = fp[0], fp[1]
     j.width, j.height, j.spp, j.max_depth, j.seed, j.job_id = width, height, spp, max_depth, seed, job_id
     j.direct_samples = direct_samples
     return j
@@ -752,8 +762,11 @@ def load_scenefile(path=None, text=None, base_dir=None):
                                     (e.params[0], e.params[1]))
         job = Job()
         L.mpt_scenefile_job(h, ctypes.byref(job))
-        arrays.camera = {"eye": list(job.camera.eye), "look": list(job.camera.look), "up": list(job.camera.up),
-                         "height_angle": job.camera.height_angle, "aspect": job.camera.aspect}
+        c = job.camera
+        arrays.camera = {"eye": list(c.eye), "look": list(c.look), "up": list(c.up), "height_angle": c.height_angle,
+                         "aspect": c.aspect, "aperture": c.aperture, "focal_length": c.focal_length,
+                         "focal_distance": c.focal_distance, "autofocus": bool(c.autofocus),
+                         "focal_point": list(c.focal_point)}
         arrays.tonemap = L.mpt_scenefile_tonemap(h)   # the output's MPT_TONEMAP_*
         return arrays, job
     finally:

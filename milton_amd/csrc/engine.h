@@ -30,6 +30,7 @@ constexpr uint32_t kJobBits = 20;       // sample index bits in a path's (job, s
 struct CameraDev {
   float eye[3], u[3], v[3], n[3];
   float tan_w, tan_h;
+  float aperture, focal_length, focal_distance;   // thin lens; aperture 0 = pinhole
 };
 
 struct JobDev {
@@ -195,6 +196,9 @@ namespace mpt_engine {
 int fail(mpt_context *ctx, int code, const std::string &msg);
 int check_job(mpt_context *ctx, const mpt_job *job);
 JobDev make_job(const mpt_job &job, float4 *film);
+// A thin-lens job with autofocus gets its focal distance from the scene: the
+// hit distance of the ray through its focal point.
+int resolve_focus(mpt_scene *s, mpt_job &job);
 
 // Creates lane k's stream, events and staging ring if it has none yet.
 int ensure_lane(mpt_context *ctx, uint32_t k);

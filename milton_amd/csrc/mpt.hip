@@ -128,10 +128,30 @@ __global__ __launch_bounds__(256) void k_raygen(const BatchDesc *__restrict__ bd
   const float y = ((float)py + ((float)sy + mpt_rng::unitf(r.y)) * inv_side) / (float)job.height;
   const CameraDev &cam = job.cam;
   const float nx = (2.0f * x - 1.0f) * cam.tan_w, ny = (1.0f - 2.0f * y) * cam.tan_h;
-  const float3 d = normalize(f3(cam.u[0] * nx + cam.v[0] * ny - cam.n[0], cam.u[1] * nx + cam.v[1] * ny - cam.n[1],
-                                cam.u[2] * nx + cam.v[2] * ny - cam.n[2]));
+  const float3 U = f3(cam.u[0], cam.u[1], cam.u[2]), V = f3(cam.v[0], cam.v[1], cam.v[2]);
+  const float3 N = f3(cam.n[0], cam.n[1], cam.n[2]);
+  float3 d = normalize(sub(add(mul(U, nx), mul(V, ny)), N));
+  float3 o = f3(cam.eye[0], cam.eye[1], cam.eye[2]);
+  if (cam.aperture > 0.0f) {
+    // ThinLensCamera::getWorldRay (cameras/ThinLensCamera.cpp:131-172): the
+    // ray meets the focal plane at t; the origin moves within the circle of
+    // confusion of radius aperture (l - D) / l, l = f t / (t - f), at
+    // r = radius sin(e1), e1 ~ U(0, pi/2), angle e2 ~ U(0, 2 pi), and the ray
+    // aims at the focal-plane point again.
+    const float t = -cam.focal_distance / dot(d, N);
+    if (t != cam.focal_length) {
+      const float3 focus = add(o, mul(d, t));
+      const float l_image = cam.focal_length * t / (t - cam.focal_length);
+      const float cof = cam.aperture * (l_image - cam.focal_distance) / l_image;
+      const mpt_rng::U4 lr = mpt_rng::path_draw(job.seed, job.job_id, pixel, sample, mpt_rng::kCameraDepth, 1);
+      const float e1 = mpt_rng::unitf(lr.x) * (0.5f * kPi), e2 = mpt_rng::unitf(lr.y) * (2.0f * kPi);
+      const float r = cof * sinf(e1);
+      o = add(o, add(mul(U, r * cosf(e2)), mul(V, r * sinf(e2))));
+      d = normalize(sub(focus, o));
+    }
+  }
   // queue entry: (origin, slot) (direction, origin primitive) (throughput, flags)
-  q[3 * slot] = make_float4(cam.eye[0], cam.eye[1], cam.eye[2], __uint_as_float(slot));
+  q[3 * slot] = make_float4(o.x, o.y, o.z, __uint_as_float(slot));
   q[3 * slot + 1] = make_float4(d.x, d.y, d.z, __uint_as_float(kMiss));
   q[3 * slot + 2] = make_float4(1.0f, 1.0f, 1.0f, __uint_as_float(mpt_rng::third(r.z)));
   pjs[slot] = make_uint2(pixel, (ch.job << kJobBits) | sample);
@@ -944,6 +964,9 @@ JobDev make_job(const mpt_job &job, float4 *film) {
   const double th = std::tan(c.height_angle * M_PI / 360.0);
   j.cam.tan_h = (float)th;
   j.cam.tan_w = (float)(c.aspect * th);
+  j.cam.aperture = c.aperture > 0 ? c.aperture : 0.0f;
+  j.cam.focal_length = c.focal_length;
+  j.cam.focal_distance = c.focal_distance;
   j.width = job.width;
   j.height = job.height;
   j.spp = job.spp;
@@ -974,6 +997,42 @@ int check_job(mpt_context *ctx, const mpt_job *job) {
     return fail(ctx, MPT_E_INVALID, "camera look must be non-zero and not collinear with up");
   if (!(job->camera.height_angle > 0 && job->camera.height_angle < 180) || !(job->camera.aspect > 0))
     return fail(ctx, MPT_E_INVALID, "camera height_angle must be in (0, 180) and aspect > 0");
+  const mpt_camera &c = job->camera;
+  if (!(c.aperture >= 0) || !std::isfinite(c.aperture)) return fail(ctx, MPT_E_INVALID, "camera aperture must be >= 0");
+  if (c.aperture > 0 && (!(c.focal_length > 0) || !std::isfinite(c.focal_length) || !(c.focal_distance > 0) ||
+                         !std::isfinite(c.focal_distance)))
+    return fail(ctx, MPT_E_INVALID, "a thin-lens camera needs focal_length > 0 and focal_distance > 0");
+  return MPT_OK;
+}
+
+int resolve_focus(mpt_scene *s, mpt_job &job) {
+  mpt_camera &c = job.camera;
+  if (!(c.aperture > 0) || !c.autofocus) return MPT_OK;
+  // ThinLensCamera::init (cameras/ThinLensCamera.cpp:97-118): the film point
+  // (x - 1/2, 1/2 - y, -1) through filmToWorld, i.e. half the extent
+  // getWorldRay gives the same film position; the hit distance if the ray
+  // hits, else the given focal distance.
+  double n[3] = {-c.look[0], -c.look[1], -c.look[2]};
+  double u[3] = {c.up[1] * n[2] - c.up[2] * n[1], c.up[2] * n[0] - c.up[0] * n[2], c.up[0] * n[1] - c.up[1] * n[0]};
+  normalize3(n);
+  normalize3(u);
+  const double v[3] = {n[1] * u[2] - n[2] * u[1], n[2] * u[0] - n[0] * u[2], n[0] * u[1] - n[1] * u[0]};
+  const double th = std::tan(c.height_angle * M_PI / 360.0), tw = c.aspect * th;
+  const double a = std::min(std::max((double)c.focal_point[0], 0.0), 1.0) - 0.5;
+  const double b = 0.5 - std::min(std::max((double)c.focal_point[1], 0.0), 1.0);
+  double d[3];
+  for (int k = 0; k < 3; ++k) d[k] = a * tw * u[k] + b * th * v[k] - n[k];
+  normalize3(d);
+  mpt_ray ray;
+  for (int k = 0; k < 3; ++k) {
+    ray.o[k] = c.eye[k];
+    ray.d[k] = (float)d[k];
+  }
+  mpt_hit hit;
+  const int rc = mpt_trace_closest(s, &ray, 1, &hit);
+  if (rc) return rc;
+  if (hit.prim != MPT_MISS && std::isfinite(hit.t) && hit.t > 0) c.focal_distance = hit.t;
+  c.autofocus = 0;
   return MPT_OK;
 }
 
@@ -1169,6 +1228,9 @@ int render_range(mpt_scene *s, const mpt_job *job, uint32_t pix_begin, uint32_t 
   mpt_context *ctx = s->ctx;
   int rc = check_job(ctx, job);
   if (rc) return rc;
+  mpt_job focused = *job;
+  if ((rc = resolve_focus(s, focused))) return rc;
+  job = &focused;
   const uint32_t spp = job->spp;
   const uint32_t batch_pix = std::max<uint32_t>(1, ctx->batch_paths / spp);
   if ((rc = reset_totals(ctx))) return rc;

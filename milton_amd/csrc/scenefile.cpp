@@ -23,8 +23,9 @@
 //   mesh               OBJ or ASCII PLY file, optional "normalize" (:484-525)
 //   background         emitter with a constant "power", or an "environment"
 //                      image (.hdr / .rgbe / .pfm / .ppm) with its filter
-//   camera             pinhole / thinlens (aperture ignored), eye, look or
-//                      focus, up, heightAngle, aspectRatio (:842-899)
+//   camera             pinhole / thinlens (aperture, fstop, focalDistance,
+//                      focalPoint), eye, look or focus, up, heightAngle,
+//                      aspectRatio (:842-899)
 //   output             width, height, size, dimensions (default 480 x 480),
 //                      tonemap (:783; "default" = Reinhard)
 //   renderer           noSuperSamples (spp), noDirectSamples, seed, maxDepth
@@ -282,6 +283,9 @@ struct Flat {
 struct Settings {
   double eye[3] = {2, 2, 2}, look[3] = {-1, -1, -1}, up[3] = {0, 1, 0};
   double height_angle = 45, aspect = 1;
+  // thin lens (metres); aperture 0 = pinhole
+  double aperture = 0, focal_length = 0, focal_distance = 1, focal_point[2] = {0.5, 0.5};
+  bool autofocus = false;
   uint32_t width = 480, height = 480, spp = 1, max_depth = 8, seed = 1, direct = 1;
   uint32_t tonemap = MPT_TONEMAP_REINHARD;   // RenderOutput::init's "default"
 };
@@ -367,6 +371,26 @@ class Loader {
     vec(v.find("up"), set_.up, 3);
     num(v.find("heightAngle"), set_.height_angle);
     num(v.find("aspectRatio"), set_.aspect);
+    if (type->str == "thinlens") {
+      // ThinLensCamera::init (cameras/ThinLensCamera.cpp:88-118): aperture in
+      // mm (default 35), fstop (default 14), focal length = fstop * aperture;
+      // autofocus through focalPoint (default the film centre) unless only a
+      // focalDistance (default 1) is given.
+      double mm = 35, fstop = 14;
+      num(v.find("aperture"), mm);
+      num(v.find("fstop"), fstop);
+      if (!(mm >= 0) || !(fstop > 0)) return bad("camera aperture must be >= 0 and fstop > 0");
+      set_.aperture = mm / 1000.0;
+      set_.focal_length = fstop * set_.aperture;
+      const JVal *fp = v.find("focalPoint");
+      const JVal *fd = v.find("focalDistance");
+      set_.autofocus = fp || !fd;
+      if (fp && !vec(fp, set_.focal_point, 2)) return bad("focalPoint expects 2 numbers");
+      if (fd && (!num(fd, set_.focal_distance) || !(set_.focal_distance > 0)))
+        return bad("focalDistance must be a positive number");
+    } else {
+      set_.aperture = 0;   // PinholeCamera::init (cameras/PinholeCamera.cpp:21-25)
+    }
     return true;
   }
 
@@ -997,6 +1021,12 @@ int mpt_scenefile_job(const mpt_scenefile *sf, mpt_job *job) {
   }
   job->camera.height_angle = (float)s.height_angle;
   job->camera.aspect = (float)s.aspect;
+  job->camera.aperture = (float)s.aperture;
+  job->camera.focal_length = (float)s.focal_length;
+  job->camera.focal_distance = (float)s.focal_distance;
+  job->camera.autofocus = s.autofocus ? 1u : 0u;
+  job->camera.focal_point[0] = (float)s.focal_point[0];
+  job->camera.focal_point[1] = (float)s.focal_point[1];
   job->width = s.width;
   job->height = s.height;
   job->spp = s.spp;

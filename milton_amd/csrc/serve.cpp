@@ -95,6 +95,15 @@ void admit(mpt_server *sv) {
     }
     if (sv->film_bytes + j->film_bytes() > sv->cfg.hbm_budget) break;
     if (j->film.reserve(j->npix) != hipSuccess) break;
+    if (resolve_focus(j->scene, j->job) != MPT_OK) {
+      // the job's autofocus ray could not be traced: it fails on its own
+      sv->queued.pop_front();
+      j->film.release();
+      j->state = MPT_JOB_FAILED;
+      j->error = (uint32_t)-MPT_E_DEVICE;
+      j->done_ms = sv->now_ms();
+      continue;
+    }
     sv->film_bytes += j->film_bytes();
     sv->peak_film_bytes = std::max(sv->peak_film_bytes, sv->film_bytes);
     sv->queued.pop_front();

@@ -923,7 +923,10 @@ BsdfSample bsdf_sample(const Material &m, V3 n, V3 wi, const double u[4], uint32
 struct Camera {
   V3 eye, u, v, n;
   double tan_w, tan_h;
-  explicit Camera(const double *c) {
+  double aperture, focal_length, focal_distance;   // ThinLensCamera (m)
+  // sc: resolves autofocus (ThinLensCamera::init, cameras/ThinLensCamera.cpp:
+  // 97-118) when given
+  explicit Camera(const double *c, const orc_scene *sc = nullptr) {
     eye = V3(c[0], c[1], c[2]);
     const V3 look(c[3], c[4], c[5]), up(c[6], c[7], c[8]);
     n = -look;
@@ -934,10 +937,39 @@ struct Camera {
     const double ha = c[9] * (M_PI / 180);
     tan_h = std::tan(ha / 2);
     tan_w = c[10] * tan_h;
+    aperture = c[11] > 0 ? c[11] : 0;
+    focal_length = c[12];
+    focal_distance = c[13];
+    if (aperture > 0 && c[16] != 0 && sc) {
+      // filmToWorld of (x - 1/2, 1/2 - y, -1): half the extent of getWorldRay
+      const double a = std::min(std::max(c[14], 0.0), 1.0) - 0.5, b = 0.5 - std::min(std::max(c[15], 0.0), 1.0);
+      uint32_t gid;
+      const double t = sc->closest(Ray(eye, normalized(u * (a * tan_w) + v * (b * tan_h) - n)), gid);
+      if (t > kEps && t < kInf) focal_distance = t;
+    }
   }
   V3 dir(double x, double y) const {
     const double nx = 2 * x - 1, ny = 1 - 2 * y;
     return normalized(u * (nx * tan_w) + v * (ny * tan_h) - n);
+  }
+  // ThinLensCamera::getWorldRay (cameras/ThinLensCamera.cpp:131-172): with
+  // an aperture, the origin moves by r = cof sin(e1) at angle e2 within the
+  // lens plane, e1 = pi/2 l0, e2 = 2 pi l1, cof = aperture (l - D) / l with
+  // l = f t / (t - f) and t the distance to the focal plane along the ray,
+  // and the ray is re-aimed at the point where the pinhole ray meets the
+  // focal plane.
+  Ray primary(double x, double y, double l0, double l1) const {
+    Ray ray(eye, dir(x, y));
+    if (!(aperture > 0)) return ray;
+    const double t = -focal_distance / dot(ray.d, n);
+    if (t == focal_length) return ray;
+    const V3 focus = ray.o + ray.d * t;
+    const double l_image = focal_length * t / (t - focal_length);
+    const double cof = aperture * (l_image - focal_distance) / l_image;
+    const double e1 = l0 * (M_PI / 2), e2 = l1 * (2 * M_PI);
+    const double r = cof * std::sin(e1);
+    const V3 o = ray.o + u * (r * std::cos(e2)) + v * (r * std::sin(e2));
+    return Ray(o, normalized(focus - o));
   }
 };
 
@@ -964,7 +996,9 @@ V3 trace_path(const orc_scene &sc, const orc_job &job, const Camera &cam, uint32
   const Uniforms cu = draw(job.seed, job.job_id, pixel, sample, kCameraDepth, 0);
   double fx, fy;
   film_position(job, pixel, sample, cu.u[0], cu.u[1], fx, fy);
-  Ray ray(cam.eye, cam.dir(fx, fy));
+  // the lens draws its own stream
+  const Uniforms lu = cam.aperture > 0 ? draw(job.seed, job.job_id, pixel, sample, kCameraDepth, 1) : cu;
+  Ray ray = cam.primary(fx, fy, lu.u[0], lu.u[1]);
   // PathTracer.cpp:32: one IOR channel per camera ray.
   const uint32_t ior_index = (uint32_t)(((uint64_t)(cu.raw[2] >> 8) * 3) >> 24);
   ctr.camera++;
@@ -1077,7 +1111,7 @@ void orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t ou
 
 // Bumped with every change of an exported signature; pyoracle refuses a
 // library built from older sources.
-int orc_interface_version(void) { return 5; }
+int orc_interface_version(void) { return 6; }
 
 orc_scene *orc_scene_create(uint32_t nv, const float *vertices, uint32_t nt, const uint32_t *indices,
                             const uint32_t *tri_material, uint32_t ns, const float *spheres,
@@ -1229,7 +1263,7 @@ void orc_trace_any(const orc_scene *s, uint64_t n, const double *rays, const dou
 
 void orc_render(const orc_scene *s, const orc_job *job, double *sum, double *sumsq, uint64_t *counters,
                 int nthreads) {
-  const Camera cam(job->camera);
+  const Camera cam(job->camera, s);
   const uint64_t np = job->pix_end - job->pix_begin;
   const int nt = std::max(1, nthreads);
   std::vector<PathCounters> pc(nt);
@@ -1264,7 +1298,7 @@ void orc_render(const orc_scene *s, const orc_job *job, double *sum, double *sum
 
 void orc_render_paths(const orc_scene *s, const orc_job *job, uint64_t n, const uint32_t *pixel,
                       const uint32_t *sample, double *L_out, int nthreads) {
-  const Camera cam(job->camera);
+  const Camera cam(job->camera, s);
   parallel_for(n, std::max(1, nthreads), 1024, [&](uint64_t b, uint64_t e, int) {
     PathCounters pc;
     for (uint64_t i = b; i < e; ++i) {
@@ -1281,9 +1315,10 @@ void orc_camera_ray(const orc_job *job, uint32_t pixel, uint32_t sample, double 
   const Uniforms cu = draw(job->seed, job->job_id, pixel, sample, kCameraDepth, 0);
   double fx, fy;
   film_position(*job, pixel, sample, cu.u[0], cu.u[1], fx, fy);
-  const V3 d = cam.dir(fx, fy);
-  ray_out[0] = cam.eye.x; ray_out[1] = cam.eye.y; ray_out[2] = cam.eye.z;
-  ray_out[3] = d.x; ray_out[4] = d.y; ray_out[5] = d.z;
+  const Uniforms lu = draw(job->seed, job->job_id, pixel, sample, kCameraDepth, 1);
+  const Ray r = cam.primary(fx, fy, lu.u[0], lu.u[1]);
+  ray_out[0] = r.o.x; ray_out[1] = r.o.y; ray_out[2] = r.o.z;
+  ray_out[3] = r.d.x; ray_out[4] = r.d.y; ray_out[5] = r.d.z;
 }
 
 void orc_bsdf_sample(const float *material, const double *n, const double *wi, const double *u,

@@ -75,9 +75,13 @@ void orc_prim_t(const orc_scene *s, uint64_t n, const double *rays, const uint32
 void orc_trace_any(const orc_scene *s, uint64_t n, const double *rays,
                    const double *tmax, uint8_t *occluded, int nthreads);
 
-/* Camera: eye[3], look[3], up[3], heightAngle (degrees), aspect. */
+/* Camera: eye[3], look[3], up[3], heightAngle (degrees), aspect, then the
+ * thin lens: aperture (m, 0 = pinhole), focal length (m), focal distance (m),
+ * focal point x, y ([0,1]^2), autofocus (nonzero: the focal distance is the
+ * hit distance of the ray through the focal point, cameras/ThinLensCamera.cpp:
+ * 97-118, when the scene is known; the given distance otherwise). */
 typedef struct orc_job {
-  double camera[11];
+  double camera[17];
   uint32_t width, height, spp, max_depth;
   uint32_t seed, job_id, pix_begin, pix_end;
   uint32_t direct_samples; /* light samples per light per vertex (0 = 1) */
@@ -95,7 +99,8 @@ void orc_render_paths(const orc_scene *s, const orc_job *job, uint64_t n,
                       const uint32_t *pixel, const uint32_t *sample, double *L_out,
                       int nthreads);
 
-/* Camera ray of (pixel, sample): ray_out double[6]. */
+/* Camera ray of (pixel, sample): ray_out double[6].  No scene: a thin lens
+ * focuses at the given focal distance. */
 void orc_camera_ray(const orc_job *job, uint32_t pixel, uint32_t sample, double *ray_out);
 
 /* BSDF sampling of one material at a hit, for distribution tests:

@@ -15,7 +15,7 @@ LIB_PATH = os.path.join(HERE, "liboracle_pt.so")
 MISS = 0xFFFFFFFF
 
 
-INTERFACE_VERSION = 5
+INTERFACE_VERSION = 6
 
 
 def build():
@@ -30,7 +30,7 @@ def build():
 
 
 class OrcJob(ctypes.Structure):
-    _fields_ = [("camera", ctypes.c_double * 11), ("width", ctypes.c_uint32), ("height", ctypes.c_uint32),
+    _fields_ = [("camera", ctypes.c_double * 17), ("width", ctypes.c_uint32), ("height", ctypes.c_uint32),
                 ("spp", ctypes.c_uint32), ("max_depth", ctypes.c_uint32), ("seed", ctypes.c_uint32),
                 ("job_id", ctypes.c_uint32), ("pix_begin", ctypes.c_uint32), ("pix_end", ctypes.c_uint32),
                 ("direct_samples", ctypes.c_uint32)]
@@ -186,7 +186,8 @@ def make_job(job, pix_begin=0, pix_end=None):
     """OrcJob from a milton_amd.Job (same camera, film, spp, depth, seed)."""
     j = OrcJob()
     c = job.camera
-    vals = list(c.eye) + list(c.look) + list(c.up) + [c.height_angle, c.aspect]
+    vals = (list(c.eye) + list(c.look) + list(c.up) + [c.height_angle, c.aspect, c.aperture, c.focal_length,
+                                                        c.focal_distance] + list(c.focal_point) + [c.autofocus])
     for i, v in enumerate(vals):
         j.camera[i] = v
     j.width, j.height, j.spp, j.max_depth = job.width, job.height, job.spp, job.max_depth
