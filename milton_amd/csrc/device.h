@@ -264,7 +264,6 @@ struct Walker {
   float tmin, tmax, troot, best;
   uint32_t best_rec, node, skip_origin;   // best_rec == kMiss: no hit yet
   int depth, top;   // valid ring entries, next write slot
-  bool dropped;
 
   MPT_HD bool start(const Scene &sc, const RaySetup &r, float tlimit, uint32_t so) {
     // Root slab test (accel/AABB.inl:25-56).
@@ -283,7 +282,6 @@ struct Walker {
     node = 0;
     depth = 0;
     top = 0;
-    dropped = false;
     skip_origin = so;
     return tmin <= tmax;
   }
@@ -310,9 +308,13 @@ struct Walker {
       const bool far_only = !near_only && tp < tmin;
       const bool both = !near_only && !far_only;
       stk[(both ? top : STACK) * kBlock] = make_uint2(second, mpt_f2u(tmax));
-      dropped = dropped || (both && depth == STACK);
-      depth = both ? (depth + 1 > STACK ? STACK : depth + 1) : depth;
-      top = both ? (top + 1 == STACK ? 0 : top + 1) : top;
+      const int inc = both ? 1 : 0;
+      depth = min(depth + inc, STACK);
+      if ((STACK & (STACK - 1)) == 0) {
+        top = (top + inc) & (STACK - 1);
+      } else {
+        top = top + inc == STACK ? 0 : top + inc;
+      }
       tmax = both ? tp : tmax;
       node = far_only ? second : first;
       nd = sc.nodes[node];
@@ -350,9 +352,11 @@ struct Walker {
     }
     if (best <= tmax) return true;
     if (depth == 0) {
-      if (!dropped || tmax >= troot) return true;
-      // restart from the root past the cells already visited
-      dropped = false;
+      // An empty stack ends the walk at the root cell's exit: the oldest
+      // entry holds troot itself unless the ring overwrote it, and with no
+      // entry lost tmax is back at troot here.  Otherwise restart from the
+      // root past the cells already visited.
+      if (tmax >= troot) return true;
       node = 0;
       tmin = tmax;
       tmax = troot;
