@@ -177,6 +177,17 @@ typedef struct mpt_job {
    * noDirectSamples, renderers/DirectIllumination.cpp:33,98-133), jittered
    * over floor(sqrt(n))^2 strata of the light; 0 means 1 */
   uint32_t direct_samples;
+  /* Film reconstruction (ReconstructionRenderOutput::addSample,
+   * renderers/outputs/ReconstructionRenderOutput.cpp:30-83): 0 bins each
+   * sample into its own pixel with weight 1 (RenderOutput.cpp:72-76); else
+   * 1 + MPT_FILTER_* splats it onto the 3 x 3 pixels around its own with
+   * weight filter(x - X, y - Y) > 0, for integer pixel corners (x, y) and
+   * the continuous position (X, Y) in pixels; the film holds numerator and
+   * denominator.  A filtered render adds into the film (mpt_render_film
+   * clears it first, mpt_render_film_range does not: tiles then merge). */
+  uint32_t filter;
+  uint32_t filter_support; /* the filter's "support", 0 = 2 */
+  float filter_params[2];  /* gaussian sigma (0 = 1); mitchell B, C (both 0 = 1/3); lanczosSinc tau (0 = 3) */
 } mpt_job;
 
 typedef struct mpt_ray {

@@ -78,7 +78,8 @@ class Camera(ctypes.Structure):
 class Job(ctypes.Structure):
     _fields_ = [("camera", Camera), ("width", ctypes.c_uint32), ("height", ctypes.c_uint32),
                 ("spp", ctypes.c_uint32), ("max_depth", ctypes.c_uint32),
-                ("seed", ctypes.c_uint32), ("job_id", ctypes.c_uint32), ("direct_samples", ctypes.c_uint32)]
+                ("seed", ctypes.c_uint32), ("job_id", ctypes.c_uint32), ("direct_samples", ctypes.c_uint32),
+                ("filter", ctypes.c_uint32), ("filter_support", ctypes.c_uint32), ("filter_params", ctypes.c_float * 2)]
 
 
 class Ray(ctypes.Structure):
@@ -365,7 +366,8 @@ def camera_dict(cam):
             "height_angle": cam.height_angle, "aspect": cam.aspect}
 
 
-def make_job(camera, width, height, spp, max_depth, seed=1, job_id=0, direct_samples=1):
+def make_job(camera, width, height, spp, max_depth, seed=1, job_id=0, direct_samples=1, film_filter=None):
+    """film_filter: None (pixel bin) or (MPT_FILTER_*, support, (p0, p1))."""
     j = Job()
     for k in range(3):
         j.camera.eye[k] = camera["eye"][k]
@@ -383,6 +385,13 @@ def make_job(camera, width, height, spp, max_depth, seed=1, job_id=0, direct_sam
     j.camera.focal_point[0], j.camera.focal_point[1] = fp[0], fp[1]
     j.width, j.height, j.spp, j.max_depth, j.seed, j.job_id = width, height, spp, max_depth, seed, job_id
     j.direct_samples = direct_samples
+    if film_filter is not None:
+        ff = list(film_filter)
+        kind = ff[0]
+        support = ff[1] if len(ff) > 1 else 2
+        params = ff[2] if len(ff) > 2 else (0.0, 0.0)
+        j.filter, j.filter_support = 1 + kind, support
+        j.filter_params[0], j.filter_params[1] = params
     return j
 
 

@@ -37,7 +37,10 @@ struct JobDev {
   CameraDev cam;
   uint32_t width, height, spp, max_depth;
   uint32_t seed, job_id, side, direct;   // direct: light samples per light (>= 1)
-  float4 *film;   // W*H (sum r, g, b, samples)
+  float4 *film;   // W*H (sum r, g, b, samples) or, filtered, (numerator, denominator)
+  uint32_t filter;                       // 0: pixel bin; else 1 + MPT_FILTER_*
+  uint32_t filter_support;
+  float filter_p0, filter_p1;
 };
 
 struct ChunkDev {
@@ -46,12 +49,16 @@ struct ChunkDev {
   uint32_t npix;
   uint32_t slot0;   // first path slot of the chunk in the batch
   uint32_t pixoff;  // first batch pixel of the chunk
-  uint32_t pad[3];
+  // filtered jobs: the film pixels the chunk's samples reach (whole rows
+  // from trow0) are targets [tgt0, tgt0 + ntgt) of the batch
+  uint32_t tgt0, trow0, ntgt;
 };
 
 struct BatchHdr {
   uint32_t n_chunks, n_paths, n_pixels, max_depth;
-  uint32_t max_direct, pad[3];   // most light samples per light of the batch's jobs
+  uint32_t max_direct;   // most light samples per light of the batch's jobs
+  uint32_t n_targets;    // film pixels of filtered chunks (ChunkDev::tgt0)
+  uint32_t pad[2];
 };
 
 struct BatchDesc {
@@ -94,12 +101,13 @@ struct Workspace {
   DBuf<float4> Ls;
   DBuf<uint32_t> flags;
   DBuf<uint2> pjs;           // (pixel, job << 20 | sample) per slot
+  DBuf<float2> pos;          // film position of each camera sample, in pixels (filtered jobs)
   DBuf<uint32_t> counts;     // [0, D] queue sizes, [D+1, 2D] shadow sizes, [2D+1, 3D] claim cursors
   DBuf<unsigned long long> totals;
   DBuf<BatchDesc> desc;
   size_t bytes() const {
     return q[0].bytes() + q[1].bytes() + hits.bytes() + sq.bytes() + Ls.bytes() + flags.bytes() +
-           pjs.bytes() + counts.bytes() + totals.bytes() + desc.bytes();
+           pjs.bytes() + pos.bytes() + counts.bytes() + totals.bytes() + desc.bytes();
   }
 };
 
@@ -124,7 +132,8 @@ struct GraphEntry {
   const void *scene = nullptr;
   uint32_t lane = 0;
   uint32_t depth = 0, cap = 0, lights = 0;
-  bool multi = false;   // several light samples per light
+  bool multi = false;      // several light samples per light
+  bool filtered = false;   // the batch splats filtered samples
   uint64_t generation = 0;
   hipGraphExec_t exec = nullptr;
 };
@@ -135,6 +144,7 @@ struct BatchPlan {
   std::vector<ChunkDev> chunks;
   uint32_t n_paths = 0, n_pixels = 0, max_depth = 0;
   uint32_t max_direct = 1;   // most light samples per light of its jobs
+  bool filtered = false;     // some job reconstructs its film through a filter
 };
 
 }  // namespace mpt_engine

@@ -8,51 +8,11 @@
 #include <vector>
 
 #include "../../include/milton_pt.h"
+#include "filters.h"
 
 namespace mpt_env {
 
 constexpr uint32_t kMaxKernelWidth = 15;
-
-// One filter's weight at the sample offset (x, y), for the formulas of
-// filters/{Box,Triangle,Gaussian,Mitchell,LanczosSinc}Filter.cpp.
-inline double filter_weight(uint32_t filter, double support, const double p[2], double x, double y) {
-  switch (filter) {
-    case MPT_FILTER_BOX:   // inside the disc of diameter support
-      return 2.0 * std::sqrt(x * x + y * y) < support ? 1.0 : 0.0;
-    case MPT_FILTER_TRIANGLE: {   // signed offsets, as the reference evaluates them
-      const double r = support / 2.0;
-      return std::max(0.0, 1.0 - x / r) * std::max(0.0, 1.0 - y / r);
-    }
-    case MPT_FILTER_GAUSSIAN: {
-      const double sigma = p[0];
-      const double e = (-0.5) / (sigma * sigma) * (x * x + y * y);
-      return std::max(0.0, 1.0 / (std::sqrt(2.0 * M_PI) * sigma) * std::exp(e));
-    }
-    case MPT_FILTER_MITCHELL: {
-      const double B = p[0], C = p[1];
-      auto m1 = [&](double v) {
-        v = std::fabs(2.0 * v / support);
-        if (v > 1.0)
-          return (((-B / 6.0 - C) * v + (B + 5.0 * C)) * v + (-2.0 * B - 8.0 * C)) * v + (4.0 / 3.0 * B + 4.0 * C);
-        return v * v * ((2.0 - 1.5 * B - C) * v + (-3.0 + 2.0 * B + C)) + (1.0 - B / 3.0);
-      };
-      return m1(x) * m1(y);
-    }
-    case MPT_FILTER_LANCZOS: {
-      const double tau = p[0];
-      auto l1 = [&](double v) {
-        v = std::fabs(v / support);
-        if (v < 1e-7) return 1.0;
-        if (v > 1) return 0.0;
-        v *= M_PI;
-        return std::sin(v * tau) / (v * tau) * (std::sin(v) / v);
-      };
-      return l1(x) * l1(y);
-    }
-    default:
-      return 0.0;
-  }
-}
 
 // The kernel KernelFilter::_computeKernel (filters/KernelFilter.cpp:162-176)
 // tabulates: m = ceil(support) | 1 taps per side, weight [i][j] at the offsets
@@ -67,32 +27,27 @@ inline bool kernel(const mpt_env_map &e, std::vector<double> &k, uint32_t &m, st
     err = "environment filter support too large (kernel at most 15 x 15)";
     return false;
   }
-  double p[2] = {e.params[0], e.params[1]};
   switch (e.filter) {
     case MPT_FILTER_BOX:
     case MPT_FILTER_TRIANGLE:
+    case MPT_FILTER_MITCHELL:
+    case MPT_FILTER_LANCZOS:
       break;
     case MPT_FILTER_GAUSSIAN:
-      if (p[0] == 0) p[0] = 1.0;
-      if (!(p[0] > 0)) {
+      if (e.params[0] < 0) {
         err = "gaussian filter sigma must be positive";
         return false;
       }
-      break;
-    case MPT_FILTER_MITCHELL:
-      if (p[0] == 0 && p[1] == 0) p[0] = p[1] = 1.0 / 3.0;
-      break;
-    case MPT_FILTER_LANCZOS:
-      if (p[0] == 0) p[0] = 3.0;
       break;
     default:
       err = "unknown environment filter";
       return false;
   }
+  const mpt_filter::Params fp = mpt_filter::resolve(e.filter, support, e.params[0], e.params[1]);
   const double half = m / 2.0;
   k.assign((size_t)m * m + 1, 0.0);
   for (uint32_t i = 0; i < m; ++i)
-    for (uint32_t j = 0; j < m; ++j) k[(size_t)i * m + j] = filter_weight(e.filter, support, p, i - half, j - half);
+    for (uint32_t j = 0; j < m; ++j) k[(size_t)i * m + j] = mpt_filter::weight(fp, i - half, j - half);
   double sum = 0;
   for (size_t t = 0; t < (size_t)m * m; ++t) sum += k[t];
   if (!(sum != 0) || !std::isfinite(sum)) {

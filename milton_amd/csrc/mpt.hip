@@ -32,6 +32,7 @@
 #include "device.h"
 #include "engine.h"
 #include "envmap.h"
+#include "filters.h"
 #include "kdbuild.h"
 #include "rng.h"
 
@@ -109,7 +110,8 @@ __device__ __forceinline__ uint32_t chunk_of_pixel(const BatchDesc *bd, uint32_t
 // JitteredSampleGenerator.cpp:30-31).  Slot -> (job, pixel, sample) through
 // the batch's chunk table.
 __global__ __launch_bounds__(256) void k_raygen(const BatchDesc *__restrict__ bd, float4 *__restrict__ q,
-                                                uint2 *__restrict__ pjs, uint32_t *__restrict__ count0) {
+                                                uint2 *__restrict__ pjs, float2 *__restrict__ pos,
+                                                uint32_t *__restrict__ count0) {
   const uint32_t slot = blockIdx.x * blockDim.x + threadIdx.x;
   const uint32_t n = bd->hdr.n_paths;
   if (slot == 0) *count0 = n;
@@ -126,6 +128,8 @@ __global__ __launch_bounds__(256) void k_raygen(const BatchDesc *__restrict__ bd
   const float inv_side = 1.0f / (float)side;
   const float x = ((float)px + ((float)sx + mpt_rng::unitf(r.x)) * inv_side) / (float)width;
   const float y = ((float)py + ((float)sy + mpt_rng::unitf(r.y)) * inv_side) / (float)job.height;
+  // a filtered film needs the sample's position (in pixels) when it splats
+  if (job.filter && pos) pos[slot] = make_float2(x * (float)width, y * (float)job.height);
   const CameraDev &cam = job.cam;
   const float nx = (2.0f * x - 1.0f) * cam.tan_w, ny = (1.0f - 2.0f * y) * cam.tan_h;
   const float3 U = f3(cam.u[0], cam.u[1], cam.u[2]), V = f3(cam.v[0], cam.v[1], cam.v[2]);
@@ -584,7 +588,7 @@ __global__ __launch_bounds__(256) void k_film(const BatchDesc *__restrict__ bd, 
   const ChunkDev ch = bd->chunks[chunk_of_pixel(bd, i)];
   const JobDev &job = bd->jobs[ch.job];
   float4 *film = job.film;
-  if (!film) return;
+  if (!film || job.filter) return;
   const uint32_t spp = job.spp;
   const float4 *L = Ls + ch.slot0 + (uint64_t)(i - ch.pixoff) * spp;
   float r = 0, g = 0, b = 0;
@@ -702,6 +706,72 @@ __global__ __launch_bounds__(kTmBlock) void k_tm_map(const float4 *__restrict__ 
   }
   out[i] = make_uchar4(tm_byte(255 * pow(r, correction)), tm_byte(255 * pow(g, correction)),
                        tm_byte(255 * pow(b, correction)), 255);
+}
+
+__device__ __forceinline__ uint32_t chunk_of_target(const BatchDesc *bd, uint32_t t) {
+  uint32_t lo = 0, hi = bd->hdr.n_chunks - 1;
+  while (lo < hi) {
+    const uint32_t mid = (lo + hi + 1) >> 1;
+    if (bd->chunks[mid].tgt0 <= t) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+// Filtered films: ReconstructionRenderOutput::addSample
+// (renderers/outputs/ReconstructionRenderOutput.cpp:30-83) as a gather.  A
+// sample splats onto the 3 x 3 pixels around the pixel it falls in (the
+// reference's output filters keep that width whatever their support) with
+// weight filter(x - X, y - Y) for the integer corner (x, y) of the target
+// and the sample position (X, Y), when that weight is positive.  One thread
+// per target pixel of a chunk sums its neighbours' samples in a fixed order
+// (rows, columns, samples) in double and adds numerator and denominator to
+// the film; a pixel near a chunk's edge also receives the neighbouring
+// chunk's part (two additions, the same sum in either order).
+__global__ __launch_bounds__(256) void k_film_filtered(const BatchDesc *__restrict__ bd,
+                                                       const float4 *__restrict__ Ls,
+                                                       const float2 *__restrict__ pos) {
+  const uint32_t n = bd->hdr.n_targets;
+  for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < n; t += gridDim.x * blockDim.x) {
+    const ChunkDev ch = bd->chunks[chunk_of_target(bd, t)];
+    const JobDev &job = bd->jobs[ch.job];
+    if (!job.film || !job.filter || t - ch.tgt0 >= ch.ntgt) continue;
+    const uint32_t W = job.width, H = job.height, spp = job.spp;
+    const uint32_t P = ch.trow0 * W + (t - ch.tgt0);
+    const int px = (int)(P % W), py = (int)(P / W);
+    const mpt_filter::Params fp = mpt_filter::resolve(job.filter - 1, job.filter_support, job.filter_p0,
+                                                      job.filter_p1);
+    double r = 0, g = 0, b = 0, wsum = 0;
+    for (int dy = -1; dy <= 1; ++dy) {
+      const int qy = py + dy;
+      if (qy < 0 || qy >= (int)H) continue;
+      for (int dx = -1; dx <= 1; ++dx) {
+        const int qx = px + dx;
+        if (qx < 0 || qx >= (int)W) continue;
+        const uint32_t Q = (uint32_t)qy * W + (uint32_t)qx;
+        if (Q < ch.pix0 || Q - ch.pix0 >= ch.npix) continue;
+        const uint64_t s0 = ch.slot0 + (uint64_t)(Q - ch.pix0) * spp;
+        for (uint32_t s = 0; s < spp; ++s) {
+          const float2 c = pos[s0 + s];
+          const double w = mpt_filter::weight(fp, (double)px - (double)c.x, (double)py - (double)c.y);
+          if (w > 0) {
+            const float4 L = Ls[s0 + s];
+            r += w * L.x;
+            g += w * L.y;
+            b += w * L.z;
+            wsum += w;
+          }
+        }
+      }
+    }
+    if (wsum > 0) {
+      float4 *f = job.film + P;
+      atomicAdd(&f->x, (float)r);
+      atomicAdd(&f->y, (float)g);
+      atomicAdd(&f->z, (float)b);
+      atomicAdd(&f->w, (float)wsum);
+    }
+  }
 }
 
 __global__ void k_totals(const uint32_t *__restrict__ counts, uint32_t depth,
@@ -836,7 +906,8 @@ void launch_shadow(mpt_context *ctx, hipStream_t st, uint64_t cap, uint32_t g, c
 }
 
 // Enqueues the kernel sequence of one batch (the part a graph captures).
-int enqueue_batch(mpt_context *ctx, Lane &ln, mpt_scene *s, uint32_t D, bool multi, bool instrument) {
+int enqueue_batch(mpt_context *ctx, Lane &ln, mpt_scene *s, uint32_t D, bool multi, bool filtered,
+                  bool instrument) {
   Workspace &w = ln.ws;
   const hipStream_t st = ln.stream;
   const uint32_t cap = w.cap, L = s->dev.num_lights;
@@ -862,7 +933,7 @@ int enqueue_batch(mpt_context *ctx, Lane &ln, mpt_scene *s, uint32_t D, bool mul
     return MPT_OK;
   };
   k_reset<<<1, 64, 0, st>>>(w.counts.p, 3 * D + 1);
-  k_raygen<<<gcap, 256, 0, st>>>(w.desc.p, w.q[0].p, w.pjs.p, w.counts.p);
+  k_raygen<<<gcap, 256, 0, st>>>(w.desc.p, w.q[0].p, w.pjs.p, filtered ? w.pos.p : nullptr, w.counts.p);
   MPT_LAUNCHED(ctx, "k_raygen");
   for (uint32_t d = 0; d < D; ++d) {
     float4 *qin = w.q[d & 1].p, *qout = w.q[(d + 1) & 1].p;
@@ -893,6 +964,7 @@ int enqueue_batch(mpt_context *ctx, Lane &ln, mpt_scene *s, uint32_t D, bool mul
   }
   k_finalize<<<gcap, 256, 0, st>>>(w.desc.p, w.Ls.p, w.flags.p);
   k_film<<<gcap, 256, 0, st>>>(w.desc.p, w.Ls.p);
+  if (filtered) k_film_filtered<<<gcap, 256, 0, st>>>(w.desc.p, w.Ls.p, w.pos.p);
   k_totals<<<1, 64, 0, st>>>(w.counts.p, D, w.totals.p);
   MPT_LAUNCHED(ctx, "film kernels");
   return MPT_OK;
@@ -911,6 +983,25 @@ int stage_plan(mpt_context *ctx, Lane &ln, const BatchPlan &plan) {
   bd.hdr.max_direct = std::max(1u, plan.max_direct);
   std::copy(plan.jobs.begin(), plan.jobs.end(), bd.jobs);
   std::copy(plan.chunks.begin(), plan.chunks.end(), bd.chunks);
+  // the film pixels each filtered chunk's samples reach: whole rows from one
+  // above its first pixel to one below its last
+  uint32_t targets = 0;
+  for (size_t c = 0; c < plan.chunks.size(); ++c) {
+    ChunkDev &ch = bd.chunks[c];
+    const JobDev &j = plan.jobs[ch.job];
+    ch.tgt0 = targets;
+    ch.trow0 = 0;
+    ch.ntgt = 0;
+    if (!j.filter || !ch.npix) continue;
+    const uint32_t r0 = ch.pix0 / j.width, r1 = (ch.pix0 + ch.npix - 1) / j.width;
+    ch.trow0 = r0 > 0 ? r0 - 1 : 0;
+    const uint32_t last = std::min(r1 + 1, j.height - 1);
+    const uint64_t n = (uint64_t)(last - ch.trow0 + 1) * j.width;
+    if ((uint64_t)targets + n > 0xFFFFFFFFull) return fail(ctx, MPT_E_INVALID, "filtered batch too large");
+    ch.ntgt = (uint32_t)n;
+    targets += ch.ntgt;
+  }
+  bd.hdr.n_targets = targets;
   const size_t head = offsetof(BatchDesc, jobs) + plan.jobs.size() * sizeof(JobDev);
   MPT_HIP(ctx, hipMemcpyAsync(ln.ws.desc.p, &bd, head, hipMemcpyHostToDevice, ln.stream));
   MPT_HIP(ctx, hipMemcpyAsync(reinterpret_cast<char *>(ln.ws.desc.p) + offsetof(BatchDesc, chunks), bd.chunks,
@@ -979,6 +1070,10 @@ JobDev make_job(const mpt_job &job, float4 *film) {
   while ((uint64_t)side * side > job.spp) --side;
   j.side = side;
   j.film = film;
+  j.filter = job.filter;
+  j.filter_support = job.filter_support;
+  j.filter_p0 = job.filter_params[0];
+  j.filter_p1 = job.filter_params[1];
   return j;
 }
 
@@ -1002,6 +1097,11 @@ int check_job(mpt_context *ctx, const mpt_job *job) {
   if (c.aperture > 0 && (!(c.focal_length > 0) || !std::isfinite(c.focal_length) || !(c.focal_distance > 0) ||
                          !std::isfinite(c.focal_distance)))
     return fail(ctx, MPT_E_INVALID, "a thin-lens camera needs focal_length > 0 and focal_distance > 0");
+  if (job->filter > 1 + MPT_FILTER_LANCZOS) return fail(ctx, MPT_E_INVALID, "unknown film filter");
+  if (job->filter && (job->filter_support > 64 || !std::isfinite(job->filter_params[0]) ||
+                      !std::isfinite(job->filter_params[1]) ||
+                      (job->filter == 1 + MPT_FILTER_GAUSSIAN && job->filter_params[0] < 0)))
+    return fail(ctx, MPT_E_INVALID, "bad film filter parameters");
   return MPT_OK;
 }
 
@@ -1172,6 +1272,16 @@ int run_batch(mpt_context *ctx, mpt_scene *s, const BatchPlan &plan, float *L_ho
   if ((rc = stage_plan(ctx, ln, plan))) return rc;
   const uint32_t D = plan.max_depth;
   const bool multi = plan.max_direct > 1;
+  bool filtered = false;
+  for (const JobDev &j : plan.jobs) filtered = filtered || j.filter != 0;
+  if (filtered && ln.ws.pos.n < ln.ws.cap) {
+    // the sample positions of filtered films; a new buffer invalidates the
+    // graphs that captured the old one
+    MPT_HIP(ctx, hipStreamSynchronize(ln.stream));
+    MPT_HIP(ctx, ln.ws.pos.reserve(ln.ws.cap));
+    ++ln.ws.generation;
+    drop_graphs(ctx, 0);
+  }
   const bool instrument = ctx->timing || ctx->counters;
   if (ctx->graphs && !instrument) {
     Workspace &w = ln.ws;
@@ -1179,14 +1289,14 @@ int run_batch(mpt_context *ctx, mpt_scene *s, const BatchPlan &plan, float *L_ho
     const void *key = reinterpret_cast<const void *>((uintptr_t)s->uid);
     for (const GraphEntry &e : ctx->graphs_cache)
       if (e.scene == key && e.lane == lane && e.depth == D && e.cap == w.cap && e.lights == s->dev.num_lights &&
-          e.multi == multi && e.generation == w.generation) {
+          e.multi == multi && e.filtered == filtered && e.generation == w.generation) {
         exec = e.exec;
         break;
       }
     if (!exec) {
       hipGraph_t graph = nullptr;
       MPT_HIP(ctx, hipStreamBeginCapture(ln.stream, hipStreamCaptureModeThreadLocal));
-      rc = enqueue_batch(ctx, ln, s, D, multi, false);
+      rc = enqueue_batch(ctx, ln, s, D, multi, filtered, false);
       hipGraph_t captured = nullptr;
       const hipError_t ec = hipStreamEndCapture(ln.stream, &captured);
       if (rc) {
@@ -1205,13 +1315,14 @@ int run_batch(mpt_context *ctx, mpt_scene *s, const BatchPlan &plan, float *L_ho
       e.cap = w.cap;
       e.lights = s->dev.num_lights;
       e.multi = multi;
+      e.filtered = filtered;
       e.generation = w.generation;
       e.exec = exec;
       ctx->graphs_cache.push_back(e);
     }
     MPT_HIP(ctx, hipGraphLaunch(exec, ln.stream));
   } else {
-    if ((rc = enqueue_batch(ctx, ln, s, D, multi, true))) return rc;
+    if ((rc = enqueue_batch(ctx, ln, s, D, multi, filtered, true))) return rc;
   }
   if (L_host) {
     MPT_HIP(ctx, hipMemcpy2DAsync(L_host, 3 * sizeof(float), ln.ws.Ls.p, sizeof(float4), 3 * sizeof(float),
@@ -1243,7 +1354,7 @@ int render_range(mpt_scene *s, const mpt_job *job, uint32_t pix_begin, uint32_t 
   uint64_t batches = 0;
   for (uint32_t p0 = pix_begin; p0 < pix_end; p0 += batch_pix) {
     const uint32_t np = std::min(batch_pix, pix_end - p0);
-    plan.chunks.assign(1, ChunkDev{0, p0, np, 0, 0, {0, 0, 0}});
+    plan.chunks.assign(1, ChunkDev{0, p0, np, 0, 0, 0, 0, 0});
     plan.n_paths = np * spp;
     plan.n_pixels = np;
     if ((rc = run_batch(ctx, s, plan, L_host ? L_host + 3ull * (uint64_t)(p0 - pix_begin) * spp : nullptr,
@@ -1857,6 +1968,8 @@ int mpt_render_film(mpt_scene *s, const mpt_job *job, mpt_film *film, mpt_render
   if (!s || !film) return fail(s ? s->ctx : nullptr, MPT_E_INVALID, "NULL argument");
   if (!job || job->width != film->width || job->height != film->height)
     return fail(s->ctx, MPT_E_INVALID, "film size does not match the job");
+  // a filtered render adds into the film
+  if (job->filter) MPT_HIP(s->ctx, hipMemsetAsync(film->data.p, 0, film->data.bytes(), s->ctx->stream));
   return render_range(s, job, 0, job->width * job->height, film->data.p, nullptr, stats);
 }
 

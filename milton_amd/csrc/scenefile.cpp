@@ -27,7 +27,8 @@
 //                      focalPoint), eye, look or focus, up, heightAngle,
 //                      aspectRatio (:842-899)
 //   output             width, height, size, dimensions (default 480 x 480),
-//                      tonemap (:783; "default" = Reinhard)
+//                      tonemap (:783; "default" = Reinhard), type and filter
+//                      (box, triangle, gaussian, mitchell, lanczosSinc)
 //   renderer           noSuperSamples (spp), noDirectSamples, seed, maxDepth
 //                      (a cap the reference does not have; default 8)
 // Anything else that would change the image (cube, cone, cylinder, blob,
@@ -288,6 +289,8 @@ struct Settings {
   bool autofocus = false;
   uint32_t width = 480, height = 480, spp = 1, max_depth = 8, seed = 1, direct = 1;
   uint32_t tonemap = MPT_TONEMAP_REINHARD;   // RenderOutput::init's "default"
+  uint32_t filter = 0, filter_support = 0;    // mpt_job.filter: 0 pixel bin, else 1 + MPT_FILTER_*
+  float filter_params[2] = {0, 0};
 };
 
 class Loader {
@@ -406,6 +409,41 @@ class Loader {
     if (const JVal *tm = v.find("tonemap")) {
       if (tm->kind != JVal::STR) return bad("output.tonemap must be a string");
       set_.tonemap = mpt_tonemap_from_name(tm->str.c_str());
+    }
+    // the reconstruction filter (MiltonJSONSceneLoader.cpp:771-821 and
+    // _parse_filter, :1389-1440); a "naive" output ignores it
+    std::string type = "reconstruction";
+    if (const JVal *t = v.find("type")) {
+      if (t->kind != JVal::STR) return bad("output.type must be a string");
+      type = t->str;
+    }
+    if (type != "naive" && type != "default" && type != "reconstruction")
+      return bad("output type '" + type + "' is not supported");
+    const JVal *f = v.find("filter");
+    if (f && type != "naive") {
+      if (f->kind != JVal::OBJ) return bad("output.filter must be an object");
+      const JVal *ft = f->find("type");
+      if (!ft || ft->kind != JVal::STR) return bad("unable to infer filter type");
+      const std::string &n = ft->str;
+      uint32_t k;
+      if (n == "box") k = MPT_FILTER_BOX;
+      else if (n == "triangle") k = MPT_FILTER_TRIANGLE;
+      else if (n == "gaussian") k = MPT_FILTER_GAUSSIAN;
+      else if (n == "mitchell") k = MPT_FILTER_MITCHELL;
+      else if (n == "lanczosSinc") k = MPT_FILTER_LANCZOS;
+      else return bad("output filter '" + n + "' is not supported");
+      set_.filter = 1 + k;
+      if (num(f->find("support"), x)) {
+        if (!(x >= 1 && x <= 64)) return bad("filter support must be in [1, 64]");
+        set_.filter_support = (uint32_t)x;
+      }
+      if (k == MPT_FILTER_GAUSSIAN && num(f->find("sigma"), x)) set_.filter_params[0] = (float)x;
+      if (k == MPT_FILTER_MITCHELL) {
+        set_.filter_params[0] = set_.filter_params[1] = 1.0f / 3.0f;
+        if (num(f->find("B"), x)) set_.filter_params[0] = (float)x;
+        if (num(f->find("C"), x)) set_.filter_params[1] = (float)x;
+      }
+      if (k == MPT_FILTER_LANCZOS && num(f->find("tau"), x)) set_.filter_params[0] = (float)x;
     }
     return true;
   }
@@ -1033,6 +1071,10 @@ int mpt_scenefile_job(const mpt_scenefile *sf, mpt_job *job) {
   job->max_depth = s.max_depth;
   job->seed = s.seed;
   job->direct_samples = s.direct;
+  job->filter = s.filter;
+  job->filter_support = s.filter_support;
+  job->filter_params[0] = s.filter_params[0];
+  job->filter_params[1] = s.filter_params[1];
   return MPT_OK;
 }
 

@@ -95,6 +95,8 @@ void admit(mpt_server *sv) {
     }
     if (sv->film_bytes + j->film_bytes() > sv->cfg.hbm_budget) break;
     if (j->film.reserve(j->npix) != hipSuccess) break;
+    // a filtered film is accumulated into, so it starts at zero
+    if (j->job.filter && hipMemset(j->film.p, 0, j->film.bytes()) != hipSuccess) break;
     if (resolve_focus(j->scene, j->job) != MPT_OK) {
       // the job's autofocus ray could not be traced: it fails on its own
       sv->queued.pop_front();

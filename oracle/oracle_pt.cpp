@@ -1111,7 +1111,7 @@ void orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t ou
 
 // Bumped with every change of an exported signature; pyoracle refuses a
 // library built from older sources.
-int orc_interface_version(void) { return 6; }
+int orc_interface_version(void) { return 7; }
 
 orc_scene *orc_scene_create(uint32_t nv, const float *vertices, uint32_t nt, const uint32_t *indices,
                             const uint32_t *tri_material, uint32_t ns, const float *spheres,
@@ -1308,6 +1308,58 @@ void orc_render_paths(const orc_scene *s, const orc_job *job, uint64_t n, const 
       L_out[3 * i + 2] = L.z;
     }
   });
+}
+
+namespace {
+double orc_filter(int type, double support, const double *p, double x, double y);
+}
+
+void orc_render_film(const orc_scene *s, const orc_job *job, double *rgbw_out, int nthreads) {
+  const uint32_t W = job->width, H = job->height, spp = job->spp;
+  const uint64_t n = (uint64_t)W * H * spp;
+  std::vector<double> L(3 * n);
+  const Camera cam(job->camera, s);
+  parallel_for(n, std::max(1, nthreads), 1024, [&](uint64_t b, uint64_t e, int) {
+    PathCounters pc;
+    for (uint64_t i = b; i < e; ++i) {
+      const V3 v = trace_path(*s, *job, cam, (uint32_t)(i / spp), (uint32_t)(i % spp), pc);
+      L[3 * i] = v.x;
+      L[3 * i + 1] = v.y;
+      L[3 * i + 2] = v.z;
+    }
+  });
+  std::fill(rgbw_out, rgbw_out + 4ull * W * H, 0.0);
+  const int type = (int)job->filter - 1;
+  const double support = job->filter_support ? job->filter_support : 2.0;
+  double p[2] = {job->filter_params[0], job->filter_params[1]};
+  if (type == 2 && p[0] == 0) p[0] = 1.0;
+  if (type == 3 && p[0] == 0 && p[1] == 0) p[0] = p[1] = 1.0 / 3.0;
+  if (type == 4 && p[0] == 0) p[0] = 3.0;
+  for (uint64_t i = 0; i < n; ++i) {
+    const uint32_t pixel = (uint32_t)(i / spp), sample = (uint32_t)(i % spp);
+    if (job->filter == 0) {   // RenderOutput::addSample: its own pixel, weight 1
+      for (int c = 0; c < 3; ++c) rgbw_out[4ull * pixel + c] += L[3 * i + c];
+      rgbw_out[4ull * pixel + 3] += 1;
+      continue;
+    }
+    const Uniforms cu = draw(job->seed, job->job_id, pixel, sample, kCameraDepth, 0);
+    double fx, fy;
+    film_position(*job, pixel, sample, cu.u[0], cu.u[1], fx, fy);
+    // Viewport::getBin (core/Viewport.h:78-81), then the 3 x 3 neighbourhood
+    // (the output filter's width: KernelFilter() keeps support 2's width 3)
+    const int col = (int)std::min(std::max(std::floor(fx * W), 0.0), W - 1.0);
+    const int row = (int)std::min(std::max(std::floor(fy * H), 0.0), H - 1.0);
+    const double cx = fx * W, cy = fy * H;
+    for (int y = std::max(row - 1, 0); y <= std::min(row + 1, (int)H - 1); ++y)
+      for (int x = std::max(col - 1, 0); x <= std::min(col + 1, (int)W - 1); ++x) {
+        const double w = orc_filter(type, support, p, x - cx, y - cy);
+        if (w > 0) {
+          double *o = rgbw_out + 4ull * ((uint64_t)y * W + x);
+          for (int c = 0; c < 3; ++c) o[c] += w * L[3 * i + c];
+          o[3] += w;
+        }
+      }
+  }
 }
 
 void orc_camera_ray(const orc_job *job, uint32_t pixel, uint32_t sample, double *ray_out) {

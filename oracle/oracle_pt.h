@@ -85,6 +85,9 @@ typedef struct orc_job {
   uint32_t width, height, spp, max_depth;
   uint32_t seed, job_id, pix_begin, pix_end;
   uint32_t direct_samples; /* light samples per light per vertex (0 = 1) */
+  uint32_t filter;         /* film: 0 pixel bin, else 1 + filter type (0 box ... 4 lanczosSinc) */
+  uint32_t filter_support; /* 0 = 2 */
+  double filter_params[2]; /* gaussian sigma; mitchell B, C; lanczosSinc tau (0 = defaults) */
 } orc_job;
 
 /* Renders pixels [pix_begin, pix_end) of the job.  sum/sumsq are
@@ -98,6 +101,12 @@ void orc_render(const orc_scene *s, const orc_job *job, double *sum, double *sum
 void orc_render_paths(const orc_scene *s, const orc_job *job, uint64_t n,
                       const uint32_t *pixel, const uint32_t *sample, double *L_out,
                       int nthreads);
+
+/* The job's film as ReconstructionRenderOutput builds it
+ * (renderers/outputs/ReconstructionRenderOutput.cpp:30-83, RenderOutput.cpp:
+ * 72-101): rgbw_out double[4*W*H] = numerator RGB and denominator per pixel,
+ * every sample splatted (filter) or binned (no filter). */
+void orc_render_film(const orc_scene *s, const orc_job *job, double *rgbw_out, int nthreads);
 
 /* Camera ray of (pixel, sample): ray_out double[6].  No scene: a thin lens
  * focuses at the given focal distance. */

@@ -15,7 +15,7 @@ LIB_PATH = os.path.join(HERE, "liboracle_pt.so")
 MISS = 0xFFFFFFFF
 
 
-INTERFACE_VERSION = 6
+INTERFACE_VERSION = 7
 
 
 def build():
@@ -33,7 +33,8 @@ class OrcJob(ctypes.Structure):
     _fields_ = [("camera", ctypes.c_double * 17), ("width", ctypes.c_uint32), ("height", ctypes.c_uint32),
                 ("spp", ctypes.c_uint32), ("max_depth", ctypes.c_uint32), ("seed", ctypes.c_uint32),
                 ("job_id", ctypes.c_uint32), ("pix_begin", ctypes.c_uint32), ("pix_end", ctypes.c_uint32),
-                ("direct_samples", ctypes.c_uint32)]
+                ("direct_samples", ctypes.c_uint32), ("filter", ctypes.c_uint32),
+                ("filter_support", ctypes.c_uint32), ("filter_params", ctypes.c_double * 2)]
 
 
 _lib = None
@@ -67,6 +68,7 @@ def lib():
         L.orc_tonemap.argtypes = [vp, u64, i32, vp]
         L.orc_scene_set_env_map.argtypes = [vp, u32, u32, vp, ctypes.c_float, ctypes.c_float, i32, u32, vp]
         L.orc_background.argtypes = [vp, vp, vp]
+        L.orc_render_film.argtypes = [vp, ctypes.POINTER(OrcJob), vp, i32]
         _lib = L
     return _lib
 
@@ -115,6 +117,13 @@ class OracleScene:
             if lib().orc_scene_set_env_map(self.h, img.shape[1], img.shape[0], _p(img), em.repeat_u, em.repeat_v,
                                            em.filter, em.support, _p(prm)):
                 raise ValueError("oracle rejected the environment map")
+
+    def render_film(self, job, threads=None):
+        """(H, W, 4) float64: numerator RGB and denominator of the job's film."""
+        j = make_job(job)
+        out = np.zeros((job.height, job.width, 4), np.float64)
+        lib().orc_render_film(self.h, ctypes.byref(j), _p(out), threads or self.threads)
+        return out
 
     def background(self, dirs):
         """Background radiance for ray directions (n, 3) -> (n, 3) float64."""
@@ -193,6 +202,9 @@ def make_job(job, pix_begin=0, pix_end=None):
     j.width, j.height, j.spp, j.max_depth = job.width, job.height, job.spp, job.max_depth
     j.seed, j.job_id = job.seed, job.job_id
     j.direct_samples = getattr(job, "direct_samples", 0)
+    j.filter, j.filter_support = getattr(job, "filter", 0), getattr(job, "filter_support", 0)
+    fp = getattr(job, "filter_params", (0.0, 0.0))
+    j.filter_params[0], j.filter_params[1] = fp[0], fp[1]
     j.pix_begin = pix_begin
     j.pix_end = job.width * job.height if pix_end is None else pix_end
     return j
