@@ -62,7 +62,7 @@ def test_tiles_and_small_batches_add_up(ctx):
     assert np.allclose(tiles.download_raw(), ref, rtol=1e-6, atol=1e-6)
     saved = ctx.get_option(m.MPT_OPT_BATCH_PATHS)
     try:
-        ctx.set_option(m.MPT_OPT_BATCH_PATHS, 4 * 70)   # 70 pixels: batches end mid-row
+        ctx.set_option(m.MPT_OPT_BATCH_PATHS, 1024)   # 256 pixels: batches end mid-row
         small = m.Film(ctx, W, H)
         small.render(sc, job)
         assert np.allclose(small.download_raw(), ref, rtol=1e-6, atol=1e-6)
