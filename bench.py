@@ -58,7 +58,20 @@ def cpu_baseline(arrays, job, seconds, threads):
     osc.close()
     return {"value": paths / dt, "unit": "paths/s", "cores": threads, "kind": "port",
             "sample": f"{count} random pixels x {job.spp} spp of the same job ({paths} paths, {dt:.1f} s); "
-                      f"oracle kd build {build_s:.1f} s excluded"}
+                      f"oracle kd build {build_s:.1f} s excluded",
+            "cpu": cpu_model()}
+
+
+def cpu_model():
+    """The host CPU's model name (the first "model name" of /proc/cpuinfo)."""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
 
 
 def copy_calibration(nbytes=256 << 20):
