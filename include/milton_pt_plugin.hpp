@@ -75,6 +75,8 @@ class PathTracer {
   std::vector<uint32_t> tri_flags_;
   std::vector<float> vnormals_;
   std::vector<uint8_t> vmask_;
+  std::vector<float> env_rgb_;   // the environment image, when the scene has one
+  mpt_env_map env_{};
   mpt_scene_desc desc_{};
   mpt_camera camera_{};
   unsigned width_ = 480, height_ = 480;

@@ -75,6 +75,15 @@ void PathTracer::setScene(const mpt_scene_desc &d) {
   desc_.tri_flags = tri_flags_.empty() ? nullptr : tri_flags_.data();
   desc_.vertex_normals = vnormals_.empty() ? nullptr : vnormals_.data();
   desc_.vertex_normal_mask = vmask_.empty() ? nullptr : vmask_.data();
+  if (d.env_map && d.env_map->rgb) {
+    env_ = *d.env_map;
+    env_rgb_.assign(env_.rgb, env_.rgb + 3ull * env_.width * env_.height);
+    env_.rgb = env_rgb_.data();
+    desc_.env_map = &env_;
+  } else {
+    env_rgb_.clear();
+    desc_.env_map = nullptr;
+  }
 }
 
 void PathTracer::setCamera(const mpt_camera &camera) { camera_ = camera; }
