@@ -68,10 +68,14 @@ typedef struct mpt_material {
   float kd[3];
   float ks[3];
   float exponent;
-  float ior;
+  float ior;          /* index of refraction; the red channel when ior_g / ior_b are set */
   float transparency;
   float Le[3];
-  uint32_t pad[3];
+  /* Per-channel IOR of a dispersive dielectric (DielectricBSDF's "ior"
+   * spectrum, materials/bsdfs/DielectricBSDF.cpp:49, read at the path's
+   * channel, :65-87); 0 = ior. */
+  float ior_g, ior_b;
+  uint32_t pad;
 } mpt_material;
 
 /* Reconstruction kernels of KernelFilter::create (filters/KernelFilter.cpp:

@@ -40,7 +40,8 @@ _u8p = ctypes.POINTER(ctypes.c_uint8)
 class Material(ctypes.Structure):
     _fields_ = [("type", ctypes.c_uint32), ("kd", ctypes.c_float * 3), ("ks", ctypes.c_float * 3),
                 ("exponent", ctypes.c_float), ("ior", ctypes.c_float), ("transparency", ctypes.c_float),
-                ("Le", ctypes.c_float * 3), ("pad", ctypes.c_uint32 * 3)]
+                ("Le", ctypes.c_float * 3), ("ior_g", ctypes.c_float), ("ior_b", ctypes.c_float),
+                ("pad", ctypes.c_uint32)]
 
 
 class EnvMapDesc(ctypes.Structure):
@@ -124,7 +125,8 @@ class JobStatus(ctypes.Structure):
 JOB_QUEUED, JOB_RUNNING, JOB_DONE, JOB_CANCELLED, JOB_FAILED = 0, 1, 2, 3, 4
 
 MATERIAL_DTYPE = np.dtype([("type", "<u4"), ("kd", "<f4", 3), ("ks", "<f4", 3), ("exponent", "<f4"),
-                           ("ior", "<f4"), ("transparency", "<f4"), ("Le", "<f4", 3), ("pad", "<u4", 3)])
+                           ("ior", "<f4"), ("transparency", "<f4"), ("Le", "<f4", 3), ("ior_g", "<f4"),
+                           ("ior_b", "<f4"), ("pad", "<u4")])
 assert MATERIAL_DTYPE.itemsize == ctypes.sizeof(Material) == 64
 RAY_DTYPE = np.dtype([("o", "<f4", 3), ("d", "<f4", 3)])
 HIT_DTYPE = np.dtype([("t", "<f4"), ("prim", "<u4")])
@@ -309,6 +311,8 @@ class SceneArrays:
         out[:, 8] = m["ior"]
         out[:, 9] = m["transparency"]
         out[:, 10:13] = m["Le"]
+        out[:, 13] = m["ior_g"]
+        out[:, 14] = m["ior_b"]
         return out
 
 

@@ -565,10 +565,12 @@ __device__ __forceinline__ Sample bsdf_sample(const mpt_material &m, float3 n, f
   // DielectricBSDF::sample / getPd / evaluate (DielectricBSDF.cpp:55-158):
   // pdf 1, f = ks, and a refraction keeps only the path's IOR channel.
   const float3 N = dot(n, wi) > 0.0f ? mul(n, -1.0f) : n;
+  // the path's channel of a dispersive IOR (DielectricBSDF.cpp:65-87)
+  const float ior = ior_index == 1 && m.ior_g > 0.0f ? m.ior_g : (ior_index == 2 && m.ior_b > 0.0f ? m.ior_b : m.ior);
   float3 wt;
   bool refracted = false;
-  if (refract(wi, n, kAirIor, m.ior, wt)) {
-    const float Fr = fresnel(wi, n, kAirIor, m.ior);
+  if (refract(wi, n, kAirIor, ior, wt)) {
+    const float Fr = fresnel(wi, n, kAirIor, ior);
     const float Fs = m.transparency * (1.0f - Fr);
     refracted = u3 < Fs;
   }

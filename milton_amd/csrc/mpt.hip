@@ -1456,8 +1456,11 @@ int prepare_scene(mpt_context *ctx, const mpt_scene_desc *d, const mpt_kd_params
       return fail(ctx, MPT_E_INVALID, "sphere material out of range or emissive (only quads may emit)");
   for (uint32_t i = 0; i < nq; ++i)
     if (bad_mat(d->quad_material[i])) return fail(ctx, MPT_E_INVALID, "quad material out of range");
-  for (uint32_t i = 0; i < nm; ++i)
-    if (d->materials[i].type > MPT_BSDF_DIELECTRIC) return fail(ctx, MPT_E_INVALID, "unknown BSDF type");
+  for (uint32_t i = 0; i < nm; ++i) {
+    const mpt_material &mt = d->materials[i];
+    if (mt.type > MPT_BSDF_DIELECTRIC) return fail(ctx, MPT_E_INVALID, "unknown BSDF type");
+    if (!(mt.ior_g >= 0) || !(mt.ior_b >= 0)) return fail(ctx, MPT_E_INVALID, "negative per-channel IOR");
+  }
   // Host copies in the device layouts.
   std::vector<float4> &verts = hs.verts, &vnorm = hs.vnorm;
   verts.resize(nv);

@@ -33,8 +33,7 @@
 //                      (a cap the reference does not have; default 8)
 // Anything else that would change the image (cube, cone, cylinder, blob,
 // point, dynamic plugins, emitting triangles or spheres, images Qt or
-// OpenEXR would read, non-uniform sphere scales, sheared planes, per-channel
-// IOR) is an error,
+// OpenEXR would read, non-uniform sphere scales, sheared planes) is an error,
 // not silently dropped.
 #include <algorithm>
 #include <cctype>
@@ -633,8 +632,11 @@ class Loader {
         }
       }
       if (have_ior) {
-        if (ior[0] != ior[1] || ior[1] != ior[2]) return bad("per-channel IOR is not supported");
         m.ior = (float)ior[0];
+        if (ior[0] != ior[1] || ior[1] != ior[2]) {   // a dispersive IOR spectrum
+          m.ior_g = (float)ior[1];
+          m.ior_b = (float)ior[2];
+        }
       }
       int emits = 0;
       double power[3] = {0, 0, 0};

@@ -515,6 +515,7 @@ struct Material {
   int type;
   V3 kd, ks;
   double exponent, ior, transparency;
+  double ior_gb[2];   // per-channel IOR of green and blue (0 = ior)
   V3 Le;
   bool emitter() const { return Le.x > 0 || Le.y > 0 || Le.z > 0; }
   bool specular() const { return type == MAT_DIELECTRIC; }   // DielectricBSDF.h:64
@@ -528,6 +529,8 @@ Material load_material(const float *m) {
   r.ior = m[8];
   r.transparency = m[9];
   r.Le = V3(m[10], m[11], m[12]);
+  r.ior_gb[0] = m[13];
+  r.ior_gb[1] = m[14];
   return r;
 }
 
@@ -897,7 +900,8 @@ BsdfSample bsdf_sample(const Material &m, V3 n, V3 wi, const double u[4], uint32
   }
   // DielectricBSDF::sample / getPd / evaluate (DielectricBSDF.cpp:55-158).
   const V3 N = dot(n, wi) > 0 ? -n : n;
-  const double ior = m.ior;
+  // m_ior[index] with index = the path's channel (DielectricBSDF.cpp:65-87)
+  const double ior = ior_index > 0 && m.ior_gb[ior_index - 1] > 0 ? m.ior_gb[ior_index - 1] : m.ior;
   const V3 wt = refract(wi, n, kAirIor, ior);
   bool refracted = false;
   if (!is_black(wt)) {

@@ -20,7 +20,8 @@
  *             the unit plane y=0, |x|,|z|<=0.5 mapped by ex (object x) and
  *             ez (object z); its normal is normalize(ez x ex)
  *   materials float[16*nm] (type, kd[3], ks[3], exponent, ior, transparency,
- *             Le[3], pad[3]);  type 0 diffuse, 1 modified Phong, 2 dielectric
+ *             Le[3], ior_g, ior_b, pad);  type 0 diffuse, 1 modified Phong,
+ *             2 dielectric; ior_g / ior_b 0 = ior
  * Global primitive ids: triangles [0,nt), spheres [nt,nt+ns), quads after.
  */
 #ifndef ORACLE_PT_H_

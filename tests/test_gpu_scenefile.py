@@ -54,3 +54,12 @@ def test_direct_samples_paths_same_seed(ctx, direct):
     f1.close()
     # the same vertices draw `direct` times the candidate shadow rays
     assert st["shadow_rays"] > 2 * st1["shadow_rays"]
+
+
+def test_dispersive_glass_paths_same_seed(ctx):
+    """A glass sphere whose IOR differs per channel: a refracting path keeps
+    one channel, refracted with that channel's index."""
+    arrays, job = m.load_scenefile(text=ROOM.replace('"medium" : { "ior" : 1.5 }',
+                                                     '"medium" : { "ior" : [1.4, 1.55, 1.8] }'))
+    assert arrays.materials[arrays.sphere_material[0]]["ior_b"] > 0
+    _per_path(ctx, arrays, job, 0, job.width * job.height, "scenefile-room-dispersive", min_frac=0.98)

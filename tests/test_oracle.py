@@ -256,6 +256,21 @@ def test_dielectric_snell_and_channel():
     np.testing.assert_array_equal(fs, [1, 1, 1])
 
 
+def test_dielectric_dispersive_ior():
+    """A dispersive IOR spectrum: each channel refracts with its own index
+    (m_ior[index] at the path's channel, DielectricBSDF.cpp:65-87)."""
+    mat = _mat(2, kd=0.0, ks=1.0, ior=1.4)
+    mat[13], mat[14] = 1.55, 1.8   # ior_g, ior_b
+    n = np.array([0.0, 1.0, 0.0])
+    th = 0.6
+    wi = np.array([math.sin(th), -math.cos(th), 0.0])
+    for ch, eta in ((0, 1.4), (1, 1.55), (2, 1.8)):
+        wo, fs, _, _ = po.bsdf_sample(mat, n, wi, [0.1, 0.2, 0.3, 0.0], ior_index=ch)
+        sin_t = math.hypot(wo[0], wo[2])
+        assert abs(1.0003 * math.sin(th) - float(np.float32(eta)) * sin_t) < 1e-12, ch
+        assert fs[ch] == 1 and fs.sum() == 1
+
+
 # ---------------------------------------------------------------------------
 # Estimator closed forms
 # ---------------------------------------------------------------------------

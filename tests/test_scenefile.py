@@ -108,6 +108,13 @@ def test_materials_and_sphere():
     np.testing.assert_allclose(s, [0.35, 0.2, 0.6, 0.2], atol=1e-6)
     sm = a.materials[a.sphere_material[0]]
     assert sm["type"] == m.MPT_BSDF_DIELECTRIC and sm["ior"] == pytest.approx(1.5)
+    assert sm["ior_g"] == 0 and sm["ior_b"] == 0
+
+
+def test_dispersive_ior():
+    a, _ = m.load_scenefile(text=ROOM.replace('"medium" : { "ior" : 1.5 }', '"medium" : { "ior" : [1.4, 1.55, 1.8] }'))
+    sm = a.materials[a.sphere_material[0]]
+    assert sm["ior"] == pytest.approx(1.4) and sm["ior_g"] == pytest.approx(1.55) and sm["ior_b"] == pytest.approx(1.8)
     assert sm["transparency"] == pytest.approx(1) and np.allclose(sm["ks"], 1)
     red = a.materials[a.tri_material[6]]
     np.testing.assert_allclose(red["kd"], [0.6, 0.1, 0.1], rtol=1e-6)
