@@ -17,6 +17,7 @@
 
 #include <algorithm>
 #include <cctype>
+#include <new>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -27,6 +28,9 @@
 #include "../../include/milton_pt.h"
 
 namespace {
+
+// Images larger than this many pixels are refused when read.
+constexpr uint64_t kMaxReadPixels = 1ull << 28;
 
 enum Format { F_NONE, F_PNG, F_PPM, F_RGBE, F_PFM };
 
@@ -229,6 +233,7 @@ bool read_rgbe(const char *path, uint32_t &w, uint32_t &h, std::vector<float> &r
   if (!format || !in.line(s)) return false;
   int hh = 0, ww = 0;
   if (std::sscanf(s.c_str(), "-Y %d +X %d", &hh, &ww) != 2 || hh <= 0 || ww <= 0) return false;
+  if ((uint64_t)hh * (uint64_t)ww > kMaxReadPixels) return false;
   w = (uint32_t)ww;
   h = (uint32_t)hh;
   rgb.assign(3 * (size_t)w * h, 0.0f);
@@ -310,7 +315,7 @@ bool read_pfm(const char *path, uint32_t &w, uint32_t &h, std::vector<float> &rg
     return false;
   const long ww = std::atol(sw.c_str()), hh = std::atol(sh.c_str());
   const double scale = std::atof(ss.c_str());
-  if (ww <= 0 || hh <= 0 || scale == 0) return false;
+  if (ww <= 0 || hh <= 0 || scale == 0 || (uint64_t)ww * (uint64_t)hh > kMaxReadPixels) return false;
   w = (uint32_t)ww;
   h = (uint32_t)hh;
   const int ch = magic == "PF" ? 3 : 1;
@@ -343,7 +348,7 @@ bool read_ppm(const char *path, uint32_t &w, uint32_t &h, std::vector<float> &rg
   std::string magic, sw, sh, sm;
   if (!token(in, magic) || magic != "P6" || !token(in, sw) || !token(in, sh) || !token(in, sm)) return false;
   const long ww = std::atol(sw.c_str()), hh = std::atol(sh.c_str()), mx = std::atol(sm.c_str());
-  if (ww <= 0 || hh <= 0 || mx != 255) return false;
+  if (ww <= 0 || hh <= 0 || mx != 255 || (uint64_t)ww * (uint64_t)hh > kMaxReadPixels) return false;
   w = (uint32_t)ww;
   h = (uint32_t)hh;
   std::vector<uint8_t> px(3 * (size_t)w * h);
@@ -363,18 +368,22 @@ int mpt_image_read(const char *path, uint32_t *width, uint32_t *height, float *r
   uint32_t w = 0, h = 0;
   std::vector<float> img;
   bool ok;
-  switch (format_of(path)) {
-    case F_RGBE:
-      ok = read_rgbe(path, w, h, img);
-      break;
-    case F_PFM:
-      ok = read_pfm(path, w, h, img);
-      break;
-    case F_PPM:
-      ok = read_ppm(path, w, h, img);
-      break;
-    default:
-      ok = false;
+  try {
+    switch (format_of(path)) {
+      case F_RGBE:
+        ok = read_rgbe(path, w, h, img);
+        break;
+      case F_PFM:
+        ok = read_pfm(path, w, h, img);
+        break;
+      case F_PPM:
+        ok = read_ppm(path, w, h, img);
+        break;
+      default:
+        ok = false;
+    }
+  } catch (const std::bad_alloc &) {
+    return MPT_E_NOMEM;
   }
   if (!ok) return MPT_E_IO;
   *width = w;
@@ -392,33 +401,41 @@ uint32_t mpt_tonemap_from_name(const char *name) {
 
 int mpt_image_write_ldr(const char *path, uint32_t w, uint32_t h, const uint8_t *rgba) {
   if (!path || !rgba || !w || !h) return MPT_E_INVALID;
-  switch (format_of(path)) {
-    case F_PNG:
-      return write_png(path, w, h, rgba) ? MPT_OK : MPT_E_IO;
-    case F_PPM:
-      return write_ppm(path, w, h, rgba) ? MPT_OK : MPT_E_IO;
-    case F_RGBE:
-    case F_PFM: {
-      // Rgba32 -> RgbaHDR: byte / 255 (common/image/Rgba.inl:32-36)
-      std::vector<float> rgb(3 * (size_t)w * h);
-      for (size_t i = 0; i < (size_t)w * h; ++i)
-        for (int c = 0; c < 3; ++c) rgb[3 * i + c] = (float)(rgba[4 * i + c] / 255.0);
-      return mpt_image_write_hdr(path, w, h, rgb.data());
+  try {
+    switch (format_of(path)) {
+      case F_PNG:
+        return write_png(path, w, h, rgba) ? MPT_OK : MPT_E_IO;
+      case F_PPM:
+        return write_ppm(path, w, h, rgba) ? MPT_OK : MPT_E_IO;
+      case F_RGBE:
+      case F_PFM: {
+        // Rgba32 -> RgbaHDR: byte / 255 (common/image/Rgba.inl:32-36)
+        std::vector<float> rgb(3 * (size_t)w * h);
+        for (size_t i = 0; i < (size_t)w * h; ++i)
+          for (int c = 0; c < 3; ++c) rgb[3 * i + c] = (float)(rgba[4 * i + c] / 255.0);
+        return mpt_image_write_hdr(path, w, h, rgb.data());
+      }
+      default:
+        return MPT_E_IO;
     }
-    default:
-      return MPT_E_IO;
+  } catch (const std::bad_alloc &) {
+    return MPT_E_NOMEM;
   }
 }
 
 int mpt_image_write_hdr(const char *path, uint32_t w, uint32_t h, const float *rgb) {
   if (!path || !rgb || !w || !h) return MPT_E_INVALID;
-  switch (format_of(path)) {
-    case F_RGBE:
-      return write_rgbe(path, w, h, rgb) ? MPT_OK : MPT_E_IO;
-    case F_PFM:
-      return write_pfm(path, w, h, rgb) ? MPT_OK : MPT_E_IO;
-    default:
-      return MPT_E_IO;
+  try {
+    switch (format_of(path)) {
+      case F_RGBE:
+        return write_rgbe(path, w, h, rgb) ? MPT_OK : MPT_E_IO;
+      case F_PFM:
+        return write_pfm(path, w, h, rgb) ? MPT_OK : MPT_E_IO;
+      default:
+        return MPT_E_IO;
+    }
+  } catch (const std::bad_alloc &) {
+    return MPT_E_NOMEM;
   }
 }
 

@@ -1989,7 +1989,12 @@ int mpt_render_film_range(mpt_scene *s, const mpt_job *job, uint32_t pix_begin, 
 int mpt_film_download(mpt_film *f, float *rgb) {
   if (!f || !rgb) return MPT_E_INVALID;
   mpt_context *ctx = f->ctx;
-  std::vector<float4> host((size_t)f->width * f->height);
+  std::vector<float4> host;
+  try {
+    host.resize((size_t)f->width * f->height);
+  } catch (const std::bad_alloc &) {
+    return fail(ctx, MPT_E_NOMEM, "host allocation");
+  }
   MPT_HIP(ctx, hipMemcpyAsync(host.data(), f->data.p, f->data.bytes(), hipMemcpyDeviceToHost, ctx->stream));
   MPT_HIP(ctx, hipStreamSynchronize(ctx->stream));
   for (size_t i = 0; i < host.size(); ++i) {
@@ -2050,14 +2055,18 @@ int mpt_film_save(mpt_film *f, const char *path, uint32_t tonemap, uint32_t flag
     return fail(ctx, MPT_E_IO, "unsupported image format '" + ext + "' (png, ppm, hdr, rgbe, pfm)");
   const size_t n = (size_t)f->width * f->height;
   int rc;
-  if (hdr && (flags & MPT_SAVE_RADIANCE)) {
-    std::vector<float> rgb(3 * n);
-    if ((rc = mpt_film_download(f, rgb.data()))) return rc;
-    rc = mpt_image_write_hdr(path, f->width, f->height, rgb.data());
-  } else {
-    std::vector<uint8_t> rgba(4 * n);
-    if ((rc = mpt_film_tonemap(f, tonemap, rgba.data()))) return rc;
-    rc = mpt_image_write_ldr(path, f->width, f->height, rgba.data());
+  try {
+    if (hdr && (flags & MPT_SAVE_RADIANCE)) {
+      std::vector<float> rgb(3 * n);
+      if ((rc = mpt_film_download(f, rgb.data()))) return rc;
+      rc = mpt_image_write_hdr(path, f->width, f->height, rgb.data());
+    } else {
+      std::vector<uint8_t> rgba(4 * n);
+      if ((rc = mpt_film_tonemap(f, tonemap, rgba.data()))) return rc;
+      rc = mpt_image_write_ldr(path, f->width, f->height, rgba.data());
+    }
+  } catch (const std::bad_alloc &) {
+    return fail(ctx, MPT_E_NOMEM, "host allocation");
   }
   if (rc) return fail(ctx, rc, std::string("cannot write ") + path);
   return MPT_OK;
