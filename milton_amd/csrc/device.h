@@ -246,12 +246,13 @@ MPT_HD float refine_hit(const float4 *rec, float3 o, float3 d, float t32, float2
   return (float)t;
 }
 
-// kd traversal with a STACK-entry ring stack in LDS and kd-restart on
-// overflow (Foley & Sugerman 2005).  `stk` points at this lane's column of a
-// [STACK + 1][kBlock] uint2 array (the last row is a dummy store target).
-// skip_origin is the primitive the ray leaves (self-intersection guard in
-// place of the reference's FP64 t > EPSILON), ANY=true finishes at the first
-// hit.
+// kd traversal with a ring stack in LDS and kd-restart on overflow (Foley &
+// Sugerman 2005).  `stk` points at this lane's column of a [STACK][kBlock]
+// uint2 array: a ring of STACK rows that holds at most STACK - 1 entries, so
+// the row after the newest entry is always free and every descent step can
+// store into it whether or not it pushes.  skip_origin is the primitive the
+// ray leaves (self-intersection guard in place of the reference's FP64
+// t > EPSILON), ANY=true finishes at the first hit.
 // COUNT=true adds the nodes fetched and references tested to n_nodes/n_refs.
 //
 // The walk is resumable: start() clips the ray to the scene box, each step()
@@ -259,11 +260,18 @@ MPT_HD float refine_hit(const float4 *rec, float3 o, float3 d, float t32, float2
 // returns true when the ray is finished.  Splitting the walk into leaf steps
 // lets a persistent wave hand a finished lane a new ray while its other lanes
 // are still walking.
+//
+// Node references and ring positions are byte offsets, so the descent does
+// no index arithmetic: a child is (word0 & ~3) + 8 * side, a ring row is
+// row * kRow.
 template <bool ANY, int STACK>
 struct Walker {
+  static constexpr uint32_t kRow = kBlock * sizeof(uint2);           // bytes per ring row
+  static constexpr uint32_t kCap = (STACK - 1) * kRow;               // entries held, in bytes
+  static constexpr bool kPow2 = (STACK & (STACK - 1)) == 0;
   float tmin, tmax, troot, best;
-  uint32_t best_rec, node, skip_origin;   // best_rec == kMiss: no hit yet
-  int depth, top;   // valid ring entries, next write slot
+  uint32_t best_rec, node, skip_origin;   // best_rec == kMiss: no hit yet; node: byte offset
+  uint32_t depth, top;   // entries held and next write row, both as row * kRow
 
   MPT_HD bool start(const Scene &sc, const RaySetup &r, float tlimit, uint32_t so) {
     // Root slab test (accel/AABB.inl:25-56).
@@ -289,35 +297,40 @@ struct Walker {
   template <bool COUNT>
   MPT_HD bool step(const Scene &sc, const RaySetup &r, uint2 *stk, uint32_t *n_nodes,
                                        uint32_t *n_refs) {
-    uint2 nd = sc.nodes[node];
+    const char *nodes = reinterpret_cast<const char *>(sc.nodes);
+    char *ring = reinterpret_cast<char *>(stk);
+    uint2 nd = *reinterpret_cast<const uint2 *>(nodes + node);
     if (COUNT) ++*n_nodes;
     while ((nd.x & 3u) != 3u) {
       const uint32_t axis = nd.x & 3u;
       const float split = mpt_u2f(nd.y);
       const float oa = comp(r.o, axis), ia = comp(r.inv, axis);
       const float tp = (split - oa) * ia;
-      const uint32_t child = nd.x >> 2;
-      // ia < 0 <=> d[axis] <= 0 (both zeros map to a negative reciprocal)
-      const bool below = (oa < split) | ((oa == split) & (ia < 0.0f));
-      const uint32_t first = below ? child : child + 1;
-      const uint32_t second = below ? child + 1 : child;
-      // Branch-free descent: the far cell is pushed only when the ray spans
-      // both children.  The store always happens, into the dummy slot STACK
-      // when there is nothing to push, so it never clobbers a live entry.
-      const bool near_only = tp > tmax || tp <= 0.0f;
-      const bool far_only = !near_only && tp < tmin;
-      const bool both = !near_only && !far_only;
-      stk[(both ? top : STACK) * kBlock] = make_uint2(second, mpt_f2u(tmax));
-      const int inc = both ? 1 : 0;
-      depth = min(depth + inc, STACK);
-      if ((STACK & (STACK - 1)) == 0) {
-        top = (top + inc) & (STACK - 1);
+      // Near child first: the left one when the ray runs up the axis (ia > 0),
+      // the right one otherwise (a zero direction has ia = -1e20).  Whatever
+      // side the origin is on, the cell's segment [tmin, tmax] then lies
+      // wholly in the far child when tp < tmin (this covers a plane behind
+      // the origin), wholly in the near child when tp > tmax, and crosses
+      // from near to far at tp otherwise.
+      const uint32_t first = (nd.x & ~3u) | ((mpt_f2u(ia) >> 31) << 3);
+      const uint32_t second = first ^ 8u;
+      const bool far_only = tp < tmin;
+      const bool near_only = tp > tmax;
+      const bool both = !far_only && !near_only;
+      // Branch-free push: the far cell always goes into the free row; only
+      // when the ray spans both children does the ring advance over it.
+      *reinterpret_cast<uint2 *>(ring + top) = make_uint2(second, mpt_f2u(tmax));
+      const uint32_t inc = both ? kRow : 0u;
+      const uint32_t held = depth + inc;
+      depth = held < kCap ? held : kCap;
+      if (kPow2) {
+        top = (top + inc) & (STACK * kRow - 1);
       } else {
-        top = top + inc == STACK ? 0 : top + inc;
+        top = top + inc == STACK * kRow ? 0u : top + inc;
       }
       tmax = both ? tp : tmax;
       node = far_only ? second : first;
-      nd = sc.nodes[node];
+      nd = *reinterpret_cast<const uint2 *>(nodes + node);
       if (COUNT) ++*n_nodes;
     }
     const uint32_t count = nd.x >> 2;
@@ -362,9 +375,13 @@ struct Walker {
       tmax = troot;
       return false;
     }
-    --depth;
-    top = top == 0 ? STACK - 1 : top - 1;
-    const uint2 e = stk[top * kBlock];
+    depth -= kRow;
+    if (kPow2) {
+      top = (top - kRow) & (STACK * kRow - 1);
+    } else {
+      top = top == 0 ? (STACK - 1) * kRow : top - kRow;
+    }
+    const uint2 e = *reinterpret_cast<const uint2 *>(ring + top);
     node = e.x;
     tmin = tmax;
     tmax = mpt_u2f(e.y);

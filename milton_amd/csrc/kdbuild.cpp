@@ -241,11 +241,12 @@ struct Flattener {
       out.max_leaf = std::max<uint64_t>(out.max_leaf, b.prims.size());
       return;
     }
-    const uint64_t child = out.nodes.size() / 2;
+    const uint64_t child = out.nodes.size() / 2;   // even: node 1 pads the root
+    if (child + 2 > (1ull << 29)) out.too_large = true;
     out.nodes.resize(out.nodes.size() + 4);
     uint32_t bits;
     std::memcpy(&bits, &b.split, 4);
-    out.nodes[2 * idx] = ((uint32_t)child << 2) | b.axis;
+    out.nodes[2 * idx] = ((uint32_t)child << 3) | b.axis;
     out.nodes[2 * idx + 1] = bits;
     run(b.kids[0], child, depth + 1);
     run(b.kids[1], child + 1, depth + 1);
@@ -282,9 +283,9 @@ void build(const std::vector<Box> &prims, const Params &p,
   for (uint32_t i = 0; i < refs.size(); ++i) refs[i] = Ref{i, prims[i]};
   BNode root;
   Builder(p, max_depth, spawn, p.clip ? tri_verts : nullptr, num_tris).build(&root, std::move(refs), box, 0);
-  out.nodes.assign(2, 0);
+  out.nodes.assign(4, 0);
   Flattener{write_record, out}.run(root, 0, 0);
-  out.n_nodes = out.nodes.size() / 2;
+  out.n_nodes = out.nodes.size() / 2 - 1;
   out.build_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
 }
 

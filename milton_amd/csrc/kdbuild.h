@@ -6,9 +6,13 @@
 // 16-byte pointer nodes with per-leaf heap vectors.  This builder bins the
 // clipped primitive bounds per node (O(N log N)), builds subtrees on several
 // threads, and emits the device layout directly:
-//   node   = 2 x u32: word0 = (child << 2) | axis for an interior node (the
-//            right child is child + 1), (count << 2) | 3 for a leaf;
+//   node   = 2 x u32: word0 = (child << 3) | axis for an interior node, i.e.
+//            the byte offset of its left child in the node array plus the
+//            axis (the right child follows at + 8, and every child pair
+//            starts at an even node, so the offsets of the two differ only in
+//            bit 3); (count << 2) | 3 for a leaf;
 //            word1 = split position (float bits) or the leaf's first record;
+//            node 0 is the root, node 1 unused (it aligns the pairs);
 //   record = 12 x f32 per leaf reference, stored in leaf order so a leaf is a
 //            contiguous run: (v0, id), (v1, kind), (v2, 0) for a triangle
 //            (kind 0 two-sided, 3 one-sided),
@@ -45,7 +49,8 @@ struct Output {
   std::vector<uint32_t> nodes;     // 2 words per node
   std::vector<float> records;      // 12 floats per reference
   Box box;
-  uint64_t n_nodes = 0, n_leaves = 0, n_refs = 0, max_depth = 0, max_leaf = 0;
+  uint64_t n_nodes = 0, n_leaves = 0, n_refs = 0, max_depth = 0, max_leaf = 0;   // n_nodes: root and children
+  bool too_large = false;          // node offsets past 2^32 bytes: the tree is unusable
   double build_ms = 0;
 };
 

@@ -167,7 +167,7 @@ template <bool COUNT, int STACK>
 __global__ __launch_bounds__(256) void k_extend(Scene sc, const float4 *__restrict__ q,
                                                 const uint32_t *__restrict__ count, float2 *__restrict__ hits,
                                                 unsigned long long *__restrict__ counters) {
-  __shared__ uint2 stack[(STACK + 1) * kBlock];
+  __shared__ uint2 stack[STACK * kBlock];
   uint2 *stk = stack + threadIdx.x;
   const uint32_t n = *count;
   uint32_t n_nodes = 0, n_refs = 0;
@@ -209,7 +209,7 @@ template <bool COUNT, int STACK>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(STACK <= 8 ? 8 : (STACK <= 16 ? 4 : 2)))) void k_extend_persistent(
     Scene sc, const float4 *__restrict__ q, const uint32_t *__restrict__ count, float2 *__restrict__ hits,
     unsigned long long *__restrict__ counters, uint32_t *__restrict__ fetch) {
-  __shared__ uint2 stack[(STACK + 1) * kBlock];
+  __shared__ uint2 stack[STACK * kBlock];
   uint2 *stk = stack + threadIdx.x;
   const uint32_t n = *count;
   uint32_t next = 0, end = 0;   // wave-uniform: the unassigned part of the claimed chunk
@@ -269,7 +269,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(STACK <= 8 
 template <int STACK>
 __global__ __launch_bounds__(256) void k_shadow(Scene sc, const float4 *__restrict__ sq,
                                                 const uint32_t *__restrict__ count, float4 *__restrict__ Ls) {
-  __shared__ uint2 stack[(STACK + 1) * kBlock];
+  __shared__ uint2 stack[STACK * kBlock];
   uint2 *stk = stack + threadIdx.x;
   const uint32_t n = *count;
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
@@ -791,7 +791,7 @@ template <bool ANY>
 __global__ __launch_bounds__(256) void k_trace_rays(Scene sc, const mpt_ray *__restrict__ rays,
                                                     const float *__restrict__ tmax, uint64_t n,
                                                     mpt_hit *__restrict__ hits, uint8_t *__restrict__ occ) {
-  __shared__ uint2 stack[(kStack + 1) * kBlock];
+  __shared__ uint2 stack[kStack * kBlock];
   uint2 *stk = stack + threadIdx.x;
   for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
     const mpt_ray ry = rays[i];
@@ -1605,6 +1605,7 @@ int prepare_scene(mpt_context *ctx, const mpt_scene_desc *d, const mpt_kd_params
   } catch (const std::bad_alloc &) {
     return fail(ctx, MPT_E_NOMEM, "kd build: host allocation");
   }
+  if (ko.too_large) return fail(ctx, MPT_E_INVALID, "kd tree above 2^29 nodes");
   hs.nt = nt;
   hs.ns = ns;
   hs.nq = nq;
@@ -1854,7 +1855,7 @@ int mpt_kd_trace(const mpt_kd_tree *t, const mpt_ray *rays, uint64_t n, mpt_hit 
   std::atomic<uint64_t> next(0);
   auto work = [&](unsigned tid) {
     // The LDS ring of a device lane becomes a strided host buffer.
-    std::vector<uint2> stk((size_t)(8 + 1) * kBlock);
+    std::vector<uint2> stk((size_t)8 * kBlock);
     for (;;) {
       const uint64_t b = next.fetch_add(4096);
       if (b >= n) break;
