@@ -39,7 +39,7 @@ constexpr int kBlock = 256;              // threads per block of every traversal
 constexpr int kStack = 16;               // default short-stack entries per lane (LDS ring)
 constexpr float kPi = 3.14159265358979323846f;
 
-enum { KIND_TRIANGLE = 0, KIND_SPHERE = 1, KIND_QUAD = 2, KIND_TRIANGLE_ONE_SIDED = 3 };
+enum { KIND_TRIANGLE = 0, KIND_TRIANGLE_ONE_SIDED = 1, KIND_SPHERE = 2, KIND_QUAD = 3 };   // triangles: kind < 2
 
 // Read-only scene as seen by the kernels.
 struct Scene {
@@ -76,6 +76,17 @@ __device__ __forceinline__ float3 normalize(float3 a) { return mul(a, rsqrtf(dot
 MPT_HD float len(float3 a) { return sqrtf(dot(a, a)); }
 MPT_HD float3 xyz(float4 a) { return f3(a.x, a.y, a.z); }
 MPT_HD float comp(float3 a, uint32_t i) { return i == 0 ? a.x : (i == 1 ? a.y : a.z); }
+
+// a / b through the hardware reciprocal (within 2 ulp): for the FP32 hit
+// distances of the traversal, which only rank candidates and are recomputed
+// in double for the hit that wins.
+MPT_HD float div_approx(float a, float b) {
+#ifdef __HIP_DEVICE_COMPILE__
+  return a * __builtin_amdgcn_rcpf(b);
+#else
+  return a * (1.0f / b);
+#endif
+}
 MPT_HD bool is_black(float3 a) { return a.x == 0.0f && a.y == 0.0f && a.z == 0.0f; }
 MPT_HD float max3(float3 a) { return fmaxf(a.x, fmaxf(a.y, a.z)); }
 
@@ -152,18 +163,20 @@ MPT_HD TriEdges tri_edges(const RaySetup &r, float3 v0, float3 v1, float3 v2) {
 // one_sided: the reference's standalone Triangle, which culls back faces: it
 // rejects e1 . (d x e2) < EPSILON (Triangle.cpp:245-270), and that product
 // is (U + V + W) * |d[kz]| = (U + V + W) / |sz|.
+// Everything is computed unconditionally and the tests are combined without
+// branches: a leaf loop over triangles then runs straight-line code.
 MPT_HD bool tri_td(const RaySetup &r, float3 v0, float3 v1, float3 v2, float &T, float &det,
                    bool one_sided = false) {
   const TriEdges e = tri_edges(r, v0, v1, v2);
   const float U = e.U, V = e.V, W = e.W;
-  if ((U < 0.0f || V < 0.0f || W < 0.0f) && (U > 0.0f || V > 0.0f || W > 0.0f)) return false;
+  const bool mixed = (fminf(U, fminf(V, W)) < 0.0f) & (fmaxf(U, fmaxf(V, W)) > 0.0f);
   const float d = U + V + W;
-  if (one_sided ? d < kEps * fabsf(r.sz) : fabsf(d) <= kEps * fabsf(r.sz)) return false;
+  const bool flat = one_sided ? d < kEps * fabsf(r.sz) : fabsf(d) <= kEps * fabsf(r.sz);
   const float t = (U * e.Az + V * e.Bz + W * e.Cz) * r.sz;
   // (-T) / (-det) == T / det exactly, so the sign moves onto T
   T = d < 0.0f ? -t : t;
   det = fabsf(d);
-  return true;
+  return !(mixed | flat);
 }
 
 // Sphere (centre c, radius rad): nearest root above EPSILON; a ray leaving the
@@ -223,7 +236,7 @@ MPT_HD float refine_hit(const float4 *rec, float3 o, float3 d, float t32, float2
     return (float)(fabs(t0 - t32) <= fabs(t1 - t32) ? t0 : t1);
   }
   double e1x, e1y, e1z, e2x, e2y, e2z;
-  const bool tri = kind == KIND_TRIANGLE || kind == KIND_TRIANGLE_ONE_SIDED;
+  const bool tri = kind < KIND_SPHERE;
   if (tri) {
     e1x = (double)b.x - a.x; e1y = (double)b.y - a.y; e1z = (double)b.z - a.z;
     e2x = (double)c.x - a.x; e2y = (double)c.y - a.y; e2z = (double)c.z - a.z;
@@ -341,20 +354,19 @@ struct Walker {
       const float4 a = rec[0], b = rec[1], c = rec[2];
       const uint32_t id = mpt_f2u(a.w);
       const uint32_t kind = mpt_f2u(b.w);
-      float t;
-      if (kind == KIND_TRIANGLE || kind == KIND_TRIANGLE_ONE_SIDED) {
+      float t = INFINITY;
+      if (kind < KIND_SPHERE) {
         float T, det;
         // divide only for candidates that can beat the current best; the
         // 2^-20 margins keep the prefilter conservative, the exact test
         // below decides
-        if (id == skip_origin || !tri_td(r, xyz(a), xyz(b), xyz(c), T, det, kind == KIND_TRIANGLE_ONE_SIDED) ||
-            T > best * det * (1.0f + 0x1p-20f) || T < kEps * det * (1.0f - 0x1p-20f))
-          continue;
-        t = T / det;
+        const bool hit = tri_td(r, xyz(a), xyz(b), xyz(c), T, det, kind == KIND_TRIANGLE_ONE_SIDED);
+        if (hit & (id != skip_origin) & (T <= best * det * (1.0f + 0x1p-20f)) &
+            (T >= kEps * det * (1.0f - 0x1p-20f)))
+          t = div_approx(T, det);
       } else if (kind == KIND_SPHERE) {
         t = sphere_t(r, xyz(a), b.x, id == skip_origin);
-      } else {
-        if (id == skip_origin) continue;
+      } else if (id != skip_origin) {
         t = quad_t(r, xyz(a), xyz(b), xyz(c));
       }
       if (t > kEps && t < best) {

@@ -15,7 +15,7 @@
 //            node 0 is the root, node 1 unused (it aligns the pairs);
 //   record = 12 x f32 per leaf reference, stored in leaf order so a leaf is a
 //            contiguous run: (v0, id), (v1, kind), (v2, 0) for a triangle
-//            (kind 0 two-sided, 3 one-sided),
+//            (kind 0 two-sided, 1 one-sided),
 //            (centre, id), (r, 0, 0, kind), 0 for a sphere,
 //            (centre, id), (ex, kind), (ez, 0) for a quad.
 #ifndef MPT_KDBUILD_H_
@@ -28,7 +28,7 @@
 
 namespace mpt_kd {
 
-enum { KIND_TRIANGLE = 0, KIND_SPHERE = 1, KIND_QUAD = 2, KIND_TRIANGLE_ONE_SIDED = 3 };
+enum { KIND_TRIANGLE = 0, KIND_TRIANGLE_ONE_SIDED = 1, KIND_SPHERE = 2, KIND_QUAD = 3 };   // triangles: kind < 2
 
 struct Box {
   float lo[3], hi[3];
