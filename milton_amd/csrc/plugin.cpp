@@ -45,36 +45,35 @@ double PathTracer::getValue(const std::string &key, double fallback) const {
   return it == props_.end() ? fallback : it->second;
 }
 
-void PathTracer::setScene(const mpt_scene_desc &d) {
-  verts_.assign(d.vertices, d.vertices + 3ull * d.num_vertices);
-  idx_.assign(d.indices, d.indices + 3ull * d.num_triangles);
-  tri_mat_.assign(d.tri_material, d.tri_material + d.num_triangles);
-  spheres_.assign(d.spheres, d.spheres + 4ull * d.num_spheres);
-  sph_mat_.assign(d.sphere_material, d.sphere_material + d.num_spheres);
-  quads_.assign(d.quads, d.quads + 9ull * d.num_quads);
-  quad_mat_.assign(d.quad_material, d.quad_material + d.num_quads);
-  mats_.assign(d.materials, d.materials + d.num_materials);
-  if (d.tri_flags) tri_flags_.assign(d.tri_flags, d.tri_flags + d.num_triangles);
-  else tri_flags_.clear();
-  if (d.vertex_normals && d.vertex_normal_mask) {
-    vnormals_.assign(d.vertex_normals, d.vertex_normals + 3ull * d.num_vertices);
-    vmask_.assign(d.vertex_normal_mask, d.vertex_normal_mask + d.num_vertices);
-  } else {
-    vnormals_.clear();
-    vmask_.clear();
+namespace {
+
+// Copies n elements of src into v; a missing array stays missing (nullptr),
+// so mpt_scene_create reports it instead of the copy reading through it.
+template <class T>
+const T *keep(std::vector<T> &v, const T *src, uint64_t n) {
+  if (!src) {
+    v.clear();
+    return nullptr;
   }
+  v.assign(src, src + n);
+  return v.data();
+}
+
+}  // namespace
+
+void PathTracer::setScene(const mpt_scene_desc &d) {
   desc_ = d;
-  desc_.vertices = verts_.data();
-  desc_.indices = idx_.data();
-  desc_.tri_material = tri_mat_.data();
-  desc_.spheres = spheres_.data();
-  desc_.sphere_material = sph_mat_.data();
-  desc_.quads = quads_.data();
-  desc_.quad_material = quad_mat_.data();
-  desc_.materials = mats_.data();
-  desc_.tri_flags = tri_flags_.empty() ? nullptr : tri_flags_.data();
-  desc_.vertex_normals = vnormals_.empty() ? nullptr : vnormals_.data();
-  desc_.vertex_normal_mask = vmask_.empty() ? nullptr : vmask_.data();
+  desc_.vertices = keep(verts_, d.vertices, 3ull * d.num_vertices);
+  desc_.indices = keep(idx_, d.indices, 3ull * d.num_triangles);
+  desc_.tri_material = keep(tri_mat_, d.tri_material, d.num_triangles);
+  desc_.spheres = keep(spheres_, d.spheres, 4ull * d.num_spheres);
+  desc_.sphere_material = keep(sph_mat_, d.sphere_material, d.num_spheres);
+  desc_.quads = keep(quads_, d.quads, 9ull * d.num_quads);
+  desc_.quad_material = keep(quad_mat_, d.quad_material, d.num_quads);
+  desc_.materials = keep(mats_, d.materials, d.num_materials);
+  desc_.tri_flags = keep(tri_flags_, d.tri_flags, d.num_triangles);
+  desc_.vertex_normals = keep(vnormals_, d.vertex_normals, 3ull * d.num_vertices);
+  desc_.vertex_normal_mask = keep(vmask_, d.vertex_normal_mask, d.num_vertices);
   if (d.env_map && d.env_map->rgb) {
     env_ = *d.env_map;
     env_rgb_.assign(env_.rgb, env_.rgb + 3ull * env_.width * env_.height);
@@ -82,7 +81,8 @@ void PathTracer::setScene(const mpt_scene_desc &d) {
     desc_.env_map = &env_;
   } else {
     env_rgb_.clear();
-    desc_.env_map = nullptr;
+    desc_.env_map = d.env_map ? &env_ : nullptr;
+    if (d.env_map) env_ = *d.env_map;   // no texels: mpt_scene_create rejects it
   }
 }
 

@@ -83,9 +83,18 @@ int main(int argc, char **argv) {
   const bool same = std::memcmp(film.data(), ref.data(), 4 * ref.size()) == 0;
   std::printf("plugin render: mean %.6f, paths %llu, identical to C ABI: %s\n", mean,
               (unsigned long long)pt->getStats().paths, same ? "yes" : "no");
+  // a scene description with a missing array is rejected at init
+  milton_amd::PathTracer *bad = static_cast<milton_amd::PathTracer *>(pnew());
+  mpt_scene_desc broken = desc;
+  broken.indices = nullptr;
+  bad->setScene(broken);
+  bad->setCamera(cam);
+  const bool rejected = broken.num_triangles > 0 && bad->init() == MPT_E_INVALID;
+  std::printf("missing array rejected: %s (%s)\n", rejected ? "yes" : "no", bad->lastError().c_str());
+  pdel(bad);
   mpt_scene_destroy(scene);
   mpt_close(ctx);
   pdel(pt);
   mpt_gen_destroy(gen);
-  return same && std::isfinite(mean) && mean > 0 ? 0 : 1;
+  return same && rejected && std::isfinite(mean) && mean > 0 ? 0 : 1;
 }

@@ -14,6 +14,7 @@
 //               waited for, so planning and result copies overlap the device;
 //   completion  when a job's last batch finishes its film is copied to host
 //               memory and its device film is freed (admitting the next job).
+#include <atomic>
 #include <chrono>
 #include <condition_variable>
 #include <cstring>
@@ -68,7 +69,8 @@ struct mpt_server {
   uint64_t batches = 0, paths = 0, jobs_done = 0;
   std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now();
   std::thread worker;
-  bool running = false, stop = false;
+  std::atomic<bool> running{false};   // read without the mutex by step / drain
+  bool stop = false;
 
   double now_ms() const {
     return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
@@ -95,8 +97,9 @@ void admit(mpt_server *sv) {
     }
     if (sv->film_bytes + j->film_bytes() > sv->cfg.hbm_budget) break;
     if (j->film.reserve(j->npix) != hipSuccess) break;
-    // a filtered film is accumulated into, so it starts at zero
-    if (j->job.filter && hipMemset(j->film.p, 0, j->film.bytes()) != hipSuccess) break;
+    // a filtered film is accumulated into, so it starts at zero; the clear
+    // goes on the context stream, which every lane waits for before a batch
+    if (j->job.filter && hipMemsetAsync(j->film.p, 0, j->film.bytes(), sv->ctx->stream) != hipSuccess) break;
     if (resolve_focus(j->scene, j->job) != MPT_OK) {
       // the job's autofocus ray could not be traced: it fails on its own
       sv->queued.pop_front();

@@ -1,5 +1,6 @@
 """The Milton renderer plugin renders on the device, and its film is
-bit-identical to the same job through the C ABI."""
+bit-identical to the same job through the C ABI; a scene description with a
+missing array is rejected at init instead of being read through."""
 import os
 import subprocess
 
@@ -16,3 +17,4 @@ def test_plugin_render_matches_c_abi():
     print(out.stdout, out.stderr)
     assert out.returncode == 0
     assert "identical to C ABI: yes" in out.stdout
+    assert "missing array rejected: yes" in out.stdout
