@@ -229,7 +229,7 @@ enum {
   MPT_OPT_TIMING = 2,      /* 1: time every kernel launch with HIP events into *_ms */
   MPT_OPT_COUNTERS = 3,    /* 1: count kd nodes and references visited by closest-hit rays */
   MPT_OPT_GRAPHS = 4,      /* 1 (default): replay each batch's kernels from a captured hipGraph */
-  MPT_OPT_STACK = 5,       /* traversal short-stack entries per ray: 8 (default), 12, 16 or 24 */
+  MPT_OPT_STACK = 5,       /* traversal short-stack rows per ray: 8 (default), 12, 16 or 24; a ring of n rows holds n - 1 entries */
   MPT_OPT_PERSISTENT = 6,  /* closest-hit kernel form: 0 one ray per lane, grid-stride over 8
                               blocks per CU; 1 (default) persistent waves that refill finished
                               lanes from 128-ray chunks of the queue each wave claims; 2 one ray
