@@ -17,6 +17,8 @@
 //   plane              with "vertices": two one-sided triangles (v1 v2 v3),
 //                      (v1 v3 v4) (:612-627); otherwise the transformed unit
 //                      plane y = 0, |x|, |z| <= 0.5 as a quad
+//   cube               the transformed unit cube as six quads with outward
+//                      normals (shapes/Cube.cpp:17-52; not emitting)
 //   triangle           one-sided triangle with its face normal (:526-567)
 //   sphere             unit sphere of radius 0.5 under position / radius
 //                      (:569-583) and the transform (similarities only)
@@ -31,9 +33,9 @@
 //                      (box, triangle, gaussian, mitchell, lanczosSinc)
 //   renderer           noSuperSamples (spp), noDirectSamples, seed, maxDepth
 //                      (a cap the reference does not have; default 8)
-// Anything else that would change the image (cube, cone, cylinder, blob,
-// point, dynamic plugins, emitting triangles or spheres, images Qt or
-// OpenEXR would read, non-uniform sphere scales, sheared planes) is an error,
+// Anything else that would change the image (cone, cylinder, blob, point,
+// dynamic plugins, emitting triangles, spheres or cubes, images Qt or
+// OpenEXR would read, non-uniform sphere scales, sheared planes or cubes) is an error,
 // not silently dropped.
 #include <algorithm>
 #include <cctype>
@@ -493,9 +495,9 @@ class Loader {
         const JVal *type = c.find("type");
         if (!type || type->kind != JVal::STR) return bad("shape needs a type");
         if (!shape(type->str, c, M, mat)) return false;
-      } else if (k == "plane" || k == "sphere" || k == "triangle" || k == "mesh") {
+      } else if (k == "plane" || k == "sphere" || k == "triangle" || k == "mesh" || k == "cube") {
         if (!shape(k, c, M, mat)) return false;
-      } else if (k == "cube" || k == "cone" || k == "cylinder" || k == "blob" || k == "point" ||
+      } else if (k == "cone" || k == "cylinder" || k == "blob" || k == "point" ||
                  k == "meta" || k == "dynamic") {
         return bad("shape '" + k + "' is not supported by this path tracer");
       } else if (k == "background" && root) {
@@ -692,6 +694,31 @@ class Loader {
     return true;
   }
 
+  // The unit plane through o spanned by x and z (|u|, |v| <= 0.5) under M, as
+  // a quad whose normal z x x is M's normal transform of the object's
+  // (Transformable: the inverse transpose), so a mirroring transform
+  // (negative determinant) swaps the spanning vectors.
+  bool quad(const Mat &M, const double o[3], const double x[3], const double z[3], uint32_t m, const char *what) {
+    double c[3], ex[3], ez[3];
+    M.point(o, c);
+    M.vector(x, ex);
+    M.vector(z, ez);
+    const double dotxz = ex[0] * ez[0] + ex[1] * ez[1] + ex[2] * ez[2];
+    const double lx = std::sqrt(ex[0] * ex[0] + ex[1] * ex[1] + ex[2] * ex[2]);
+    const double lz = std::sqrt(ez[0] * ez[0] + ez[1] * ez[1] + ez[2] * ez[2]);
+    if (!(lx > 0 && lz > 0) || std::fabs(dotxz) > 1e-9 * lx * lz)
+      return bad(std::string("a sheared ") + what + " is not supported (its transform must keep the face axes orthogonal)");
+    const double det = M.m[0][0] * (M.m[1][1] * M.m[2][2] - M.m[1][2] * M.m[2][1]) -
+                       M.m[0][1] * (M.m[1][0] * M.m[2][2] - M.m[1][2] * M.m[2][0]) +
+                       M.m[0][2] * (M.m[1][0] * M.m[2][1] - M.m[1][1] * M.m[2][0]);
+    const double *qx = det < 0 ? ez : ex, *qz = det < 0 ? ex : ez;
+    for (int i = 0; i < 3; ++i) flat_.quads.push_back((float)c[i]);
+    for (int i = 0; i < 3; ++i) flat_.quads.push_back((float)qx[i]);
+    for (int i = 0; i < 3; ++i) flat_.quads.push_back((float)qz[i]);
+    flat_.quad_mat.push_back(m);
+    return true;
+  }
+
   bool shape(const std::string &type, const JVal &v, const Mat &M, int mat_in) {
     if (v.kind != JVal::OBJ) return bad(type + " must be an object");
     // a material nested in the shape object applies to the shape
@@ -709,19 +736,24 @@ class Loader {
         return one_sided(w[0], w[1], w[2], m) && one_sided(w[0], w[2], w[3], m);
       }
       const double o[3] = {0, 0, 0}, x[3] = {1, 0, 0}, z[3] = {0, 0, 1};
-      double c[3], ex[3], ez[3];
-      M.point(o, c);
-      M.vector(x, ex);
-      M.vector(z, ez);
-      const double dotxz = ex[0] * ez[0] + ex[1] * ez[1] + ex[2] * ez[2];
-      const double lx = std::sqrt(ex[0] * ex[0] + ex[1] * ex[1] + ex[2] * ex[2]);
-      const double lz = std::sqrt(ez[0] * ez[0] + ez[1] * ez[1] + ez[2] * ez[2]);
-      if (!(lx > 0 && lz > 0) || std::fabs(dotxz) > 1e-9 * lx * lz)
-        return bad("a sheared plane is not supported (its transform must keep x and z orthogonal)");
-      for (int i = 0; i < 3; ++i) flat_.quads.push_back((float)c[i]);
-      for (int i = 0; i < 3; ++i) flat_.quads.push_back((float)ex[i]);
-      for (int i = 0; i < 3; ++i) flat_.quads.push_back((float)ez[i]);
-      flat_.quad_mat.push_back(m);
+      return quad(M, o, x, z, m, "plane");
+    }
+    if (type == "cube") {
+      // Cube::getIntersection (shapes/Cube.cpp:17-52): the unit cube
+      // [-0.5, 0.5]^3 hit on its six faces p[u] = +-0.5 with |p[v]|, |p[w]|
+      // <= 0.5 from either side, i.e. six unit planes with outward normals.
+      if (flat_.mat_emits[m]) return bad("emitting cubes are not supported (only planes may emit)");
+      for (int u = 0; u < 3; ++u) {
+        const int a = (u + 1) % 3, b = (u + 2) % 3;
+        for (int sgn = -1; sgn <= 1; sgn += 2) {
+          double o[3] = {0, 0, 0}, x[3] = {0, 0, 0}, z[3] = {0, 0, 0};
+          o[u] = 0.5 * sgn;
+          // the quad's normal is z x x: e_u for (x, z) = (e_b, e_a), -e_u swapped
+          x[sgn > 0 ? b : a] = 1;
+          z[sgn > 0 ? a : b] = 1;
+          if (!quad(M, o, x, z, m, "cube")) return false;
+        }
+      }
       return true;
     }
     if (type == "triangle") {

@@ -2,8 +2,8 @@
 tests/test_scenefile.py (one-sided walls, a transformed light plane, a glass
 sphere, a Phong triangle) loaded by mpt_scenefile_*, rendered on the GPU and
 compared with the oracle on the same arrays, per path (same seeds) and per
-pixel (independent seeds); and several light samples per vertex
-(DirectIllumination's noDirectSamples)."""
+pixel (independent seeds); several light samples per vertex
+(DirectIllumination's noDirectSamples); and a rotated, scaled cube."""
 import numpy as np
 import pytest
 
@@ -63,3 +63,19 @@ def test_dispersive_glass_paths_same_seed(ctx):
                                                      '"medium" : { "ior" : [1.4, 1.55, 1.8] }'))
     assert arrays.materials[arrays.sphere_material[0]]["ior_b"] > 0
     _per_path(ctx, arrays, job, 0, job.width * job.height, "scenefile-room-dispersive", min_frac=0.98)
+
+
+def test_cube_paths_same_seed(ctx):
+    """The room with the Phong triangle replaced by a rotated, non-uniformly
+    scaled Phong cube (six quads)."""
+    tri = '"triangle" : { "vertices" : [ -0.5,0,-0.5,  -0.5,0,0.5,  0.5,0,0.5 ] },'
+    assert tri in ROOM
+    # lifted off the floor so that no face is coplanar with a wall
+    text = ROOM.replace('"translate" : [ 0.7, 0.15, 0.35 ],\n               "scale" : [ 0.2, 0.3, 0.2 ],',
+                        '"translate" : [ 0.7, 0.17, 0.35 ], "rotate" : [ 0, 0, 0, 0, 1, 0, 25 ],'
+                        ' "scale" : [ 0.2, 0.3, 0.15 ],')
+    assert text != ROOM
+    text = text.replace(tri, '"cube" : { },')
+    arrays, job = m.load_scenefile(text=text)
+    assert len(arrays.quads) == 7
+    _per_path(ctx, arrays, job, 0, job.width * job.height, "scenefile-room-cube", min_frac=0.98)

@@ -164,7 +164,9 @@ def test_output_tonemap(name, want):
 
 
 @pytest.mark.parametrize("snippet, word", [
-    ('"cube" : { }', "cube"),
+    ('"cone" : { }', "cone"),
+    ('"material" : { "emitter" : { "type" : "oriented", "power" : 1 }, "cube" : { } }', "cubes"),
+    ('"transform" : { "arbitraryTransform" : [1,0.5,0,0, 0,1,0,0, 0,0,1,0, 0,0,0,1], "cube" : { } }', "sheared"),
     ('"material" : { "emitter" : { "type" : "oriented", "power" : 1 }, "sphere" : { } }', "spheres"),
     ('"material" : { "emitter" : { "type" : "omni", "power" : 1 }, "plane" : { } }', "omni"),
     ('"transform" : { "scale" : [1, 2, 1], "sphere" : { } }', "ellipsoid"),
@@ -277,3 +279,55 @@ def test_reference_scene_files():
     le = a.materials[a.quad_material[0]]["Le"]
     np.testing.assert_allclose(le, np.array([2e6, 2e6, 1.6e6]) / (math.pi * 140 * 125), rtol=1e-5)
     assert list(job.camera.eye) == pytest.approx([278, 273, -650])
+
+
+def _quad_normals(a):
+    q = a.quads.reshape(-1, 9).astype(np.float64)
+    n = np.cross(q[:, 6:9], q[:, 3:6])
+    return q, n / np.linalg.norm(n, axis=1, keepdims=True)
+
+
+@pytest.mark.parametrize("scale", [[2, 1, 0.5], [-1, 1, 1], [1, -0.5, 3]])
+def test_cube_faces(scale):
+    """Cube::getIntersection (shapes/Cube.cpp:17-52): six faces of the unit
+    cube under the node's transform, each a quad whose normal points out of
+    the cube (also under a mirroring scale), with face areas the products of
+    the two scales along it."""
+    text = ('{ "scenefile" : { "scene" : { "transform" : { "translate" : [1, 2, 3], "rotate" : [0, 0, 0, 0, 1, 0, 30],'
+            ' "scale" : %s, "cube" : { } } } } }' % scale)
+    a, _ = m.load_scenefile(text=text)
+    assert a.quads.reshape(-1, 9).shape == (6, 9) and len(a.indices) == 0 and len(a.spheres) == 0
+    q, n = _quad_normals(a)
+    centre = np.array([1, 2, 3], float)
+    assert (np.einsum("ij,ij->i", n, q[:, :3] - centre) > 0).all()
+    sx, sy, sz = np.abs(scale)
+    areas = np.linalg.norm(np.cross(q[:, 6:9], q[:, 3:6]), axis=1)
+    assert sorted(np.round(areas, 6)) == sorted(np.round([sy * sz] * 2 + [sx * sz] * 2 + [sx * sy] * 2, 6))
+
+
+def test_mirrored_plane_normal():
+    """A plane under a scale that flips y keeps the normal the inverse
+    transpose gives it: (0, -1, 0)."""
+    a, _ = m.load_scenefile(text='{ "scenefile" : { "scene" : { "transform" : { "scale" : [1, -1, 1],'
+                                 ' "plane" : { } } } } }')
+    _, n = _quad_normals(a)
+    assert np.allclose(n[0], [0, -1, 0])
+
+
+def test_cube_hits_closed_form():
+    """Axis-aligned cube [0, 2] x [1, 2] x [-1, 1]: rays from outside along
+    each axis hit the near face at the distance to it, a ray from inside hits
+    the far face, a ray passing beside the cube misses."""
+    a, _ = m.load_scenefile(text='{ "scenefile" : { "scene" : { "transform" : { "translate" : [1, 1.5, 0],'
+                                 ' "scale" : [2, 1, 2], "cube" : { } } } } }')
+    osc = po.OracleScene(a)
+    rays = np.array([[-3, 1.5, 0, 1, 0, 0],     # x = 0 face at t = 3
+                     [1, 5, 0.5, 0, -1, 0],     # y = 2 face at t = 3
+                     [0.5, 1.2, 4, 0, 0, -1],   # z = 1 face at t = 3
+                     [1, 1.5, 0, 0, 0, 1],      # inside: z = 1 face at t = 1
+                     [1, 1.5, 0, 0.8, 0, 0.6],  # inside: x = 2 face at t = 1.25 (z = 1 at 5/3)
+                     [-3, 3, 0, 1, 0, 0]],      # above the cube: miss
+                    float)
+    t, prim = osc.trace_closest(rays)
+    assert np.allclose(t[:5], [3, 3, 3, 1, 1.25], rtol=1e-12)
+    assert (prim[:5] != 0xFFFFFFFF).all() and prim[5] == 0xFFFFFFFF and np.isinf(t[5])
