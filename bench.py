@@ -285,7 +285,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-rmse", action="store_true", help="skip the RMSE against a 65,536-spp reference")
     ap.add_argument("--batch-paths", type=int, default=0)
-    ap.add_argument("--stack", type=int, default=0, help="traversal short-stack entries (8, 12, 16, 24)")
+    ap.add_argument("--stack", type=int, default=0, help="traversal short-stack rows (8, 12, 16, 24)")
     ap.add_argument("--persistent", type=int, default=-1,
                     help="closest-hit kernel form: 0 grid-stride, 1 persistent waves, 2 one ray per lane")
     ap.add_argument("--lanes", type=int, default=0, help="engine lanes (streams) batches alternate over")
