@@ -41,7 +41,7 @@
 extern "C" {
 #endif
 
-#define MPT_ABI_VERSION 3u
+#define MPT_ABI_VERSION 4u
 #define MPT_MISS 0xFFFFFFFFu
 
 enum {
@@ -134,6 +134,12 @@ typedef struct mpt_scene_desc {
   /* Optional (NULL = the constant env_radiance): an environment image in
    * place of the constant background. */
   const mpt_env_map *env_map;
+  /* Optional point lights: the reference's "point" shape with an "omni"
+   * emitter (shapes/PointShape.cpp, emitters/OmniEmitter.cpp).  6 floats per
+   * light: position x, y, z and power r, g, b; radiance P / (4 pi) in every
+   * direction, one light sample per vertex (DirectIllumination.cpp:96-98). */
+  uint32_t num_points;
+  const float *points;
 } mpt_scene_desc;
 
 enum { MPT_TRI_ONE_SIDED = 1u };

@@ -76,6 +76,7 @@ class PathTracer {
   std::vector<float> vnormals_;
   std::vector<uint8_t> vmask_;
   std::vector<float> env_rgb_;   // the environment image, when the scene has one
+  std::vector<float> points_;    // point lights
   mpt_env_map env_{};
   mpt_scene_desc desc_{};
   mpt_camera camera_{};

@@ -19,7 +19,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 # compiled with different kernel tunables); the in-tree build by default.
 LIB_PATH = os.environ.get("MPT_LIBRARY") or os.path.join(_HERE, "libmilton_pt.so")
 
-MPT_ABI_VERSION = 3
+MPT_ABI_VERSION = 4
 MPT_TRI_ONE_SIDED = 1
 MPT_MISS = 0xFFFFFFFF
 MPT_BSDF_DIFFUSE, MPT_BSDF_PHONG, MPT_BSDF_DIELECTRIC = 0, 1, 2
@@ -58,7 +58,8 @@ class SceneDesc(ctypes.Structure):
                 ("num_quads", ctypes.c_uint32), ("quads", _f32p), ("quad_material", _u32p),
                 ("num_materials", ctypes.c_uint32), ("materials", ctypes.POINTER(Material)),
                 ("env_radiance", ctypes.c_float * 3), ("tri_flags", _u32p), ("vertex_normals", _f32p),
-                ("vertex_normal_mask", _u8p), ("env_map", ctypes.POINTER(EnvMapDesc))]
+                ("vertex_normal_mask", _u8p), ("env_map", ctypes.POINTER(EnvMapDesc)),
+                ("num_points", ctypes.c_uint32), ("points", _f32p)]
 
 
 class KdParams(ctypes.Structure):
@@ -249,7 +250,7 @@ class SceneArrays:
 
     def __init__(self, vertices, indices, tri_material, spheres, sphere_material, quads, quad_material,
                  materials, env, camera=None, tri_flags=None, vertex_normals=None, vertex_normal_mask=None,
-                 env_map=None):
+                 env_map=None, points=None):
         self.vertices = np.ascontiguousarray(vertices, np.float32).reshape(-1, 3)
         self.indices = np.ascontiguousarray(indices, np.uint32).reshape(-1, 3)
         self.tri_material = np.ascontiguousarray(tri_material, np.uint32).reshape(-1)
@@ -267,6 +268,9 @@ class SceneArrays:
         self.vertex_normal_mask = (None if vertex_normal_mask is None
                                    else np.ascontiguousarray(vertex_normal_mask, np.uint8).reshape(-1))
         self.env_map = env_map   # optional EnvMap in place of the constant env
+        # optional point lights: (x, y, z, power r, g, b) per light
+        self.points = (np.zeros((0, 6), np.float32) if points is None
+                       else np.ascontiguousarray(points, np.float32).reshape(-1, 6))
 
     @property
     def num_prims(self):
@@ -298,6 +302,8 @@ class SceneArrays:
         if self.env_map is not None:
             d._env = self.env_map.desc()   # kept alive with the descriptor
             d.env_map = ctypes.pointer(d._env)
+        d.num_points = len(self.points)
+        d.points = self.points.ctypes.data_as(_f32p)
         return d
 
     def materials_f32(self):
@@ -773,6 +779,8 @@ def load_scenefile(path=None, text=None, base_dir=None):
             img = _copy(e.rgb, e.width * e.height, np.float32, 3).reshape(e.height, e.width, 3)
             arrays.env_map = EnvMap(img, e.repeat_u or 1.0, e.repeat_v or 1.0, e.filter, e.support or 2,
                                     (e.params[0], e.params[1]))
+        if d.num_points:
+            arrays.points = _copy(d.points, d.num_points, np.float32, 6)
         job = Job()
         L.mpt_scenefile_job(h, ctypes.byref(job))
         c = job.camera

@@ -61,6 +61,9 @@ struct Scene {
   const double *env_kernel;
   uint32_t env_w, env_h, env_m;
   float env_rep_u, env_rep_v;
+  // point lights: (position, 0), (radiance P / (4 pi), 0) per light
+  const float4 *points;
+  uint32_t num_points;
 };
 
 MPT_HD float3 f3(float x, float y, float z) { return make_float3(x, y, z); }

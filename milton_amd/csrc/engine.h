@@ -182,7 +182,7 @@ struct mpt_scene {
   mpt_engine::DBuf<float4> recs, tshade, spheres, quads;
   mpt_engine::DBuf<uint32_t> sphere_mat, lights;
   mpt_engine::DBuf<mpt_material> mats;
-  mpt_engine::DBuf<float4> env_tex;
+  mpt_engine::DBuf<float4> env_tex, points;
   mpt_engine::DBuf<double> env_kernel;
   mpt_dev::Scene dev{};
   uint64_t stats[8] = {0, 0, 0, 0, 0, 0, 0, 0};

@@ -535,6 +535,36 @@ __global__ __launch_bounds__(kShadeBlock) __attribute__((amdgpu_waves_per_eu(MPT
         }
       }
     }
+    // Point lights (DirectIllumination.cpp:96-121 with isPoint): one sample,
+    // no area weight, and a point has no normal, so its cosine is 1
+    // (SurfacePoint's normal dot, core/SurfacePoint.h:143).
+    for (uint32_t li = 0; li < sc.num_points; ++li) {
+      bool want_sh = false;
+      float3 sd = f3(0, 0, 1), contrib = f3(0, 0, 0);
+      float sdist = 0.0f;
+      if (surface && !specular) {
+        const float4 lp = sc.points[2 * li], le = sc.points[2 * li + 1];
+        float3 w = sub(xyz(lp), p);
+        const float dist = len(w);
+        w = mul(w, 1.0f / dist);
+        const float cos_wo = fabsf(dot(hi.n, w));
+        const float3 fr = bsdf_eval(m, hi.n, d, w);
+        if (dist > 0.0f && cos_wo > 0.0f && !is_black(fr)) {
+          contrib = mul(mul(beta, mul(fr, xyz(le))), cos_wo / (dist * dist));
+          if (!is_black(contrib)) {
+            want_sh = true;
+            sd = w;
+            sdist = dist * (1.0f - 1e-5f);
+          }
+        }
+      }
+      const uint32_t si = block_append<kShadeBlock / 64>(want_sh, count_shadow, append_sh);
+      if (want_sh) {
+        sq[3 * si] = make_float4(p.x, p.y, p.z, __uint_as_float(slot));
+        sq[3 * si + 1] = make_float4(sd.x, sd.y, sd.z, sdist);
+        sq[3 * si + 2] = make_float4(contrib.x, contrib.y, contrib.z, __uint_as_float(prim));
+      }
+    }
     if (surface) {
       const mpt_rng::U4 r = mpt_rng::path_draw(seed, job_id, pixel, sample, depth, 0);
       const Sample s = bsdf_sample(m, hi.n, d, mpt_rng::unitf(r.x), mpt_rng::unitf(r.y), mpt_rng::unitf(r.w),
@@ -910,7 +940,7 @@ int enqueue_batch(mpt_context *ctx, Lane &ln, mpt_scene *s, uint32_t D, bool mul
                   bool instrument) {
   Workspace &w = ln.ws;
   const hipStream_t st = ln.stream;
-  const uint32_t cap = w.cap, L = s->dev.num_lights;
+  const uint32_t cap = w.cap, L = s->dev.num_lights + s->dev.num_points;
   const uint32_t g = grid_for(ctx, cap, 8);
   const uint32_t gcap = (cap + 255) / 256;
   int rc;
@@ -1263,7 +1293,7 @@ int run_batch(mpt_context *ctx, mpt_scene *s, const BatchPlan &plan, float *L_ho
   if (lane >= kMaxLanes) return fail(ctx, MPT_E_INVALID, "bad lane");
   // shadow rays per vertex: one per light sample; their random streams are
   // numbered 1 + light * samples + sample below 256
-  const uint32_t shadow_per_vertex = s->dev.num_lights * std::max(1u, plan.max_direct);
+  const uint32_t shadow_per_vertex = s->dev.num_lights * std::max(1u, plan.max_direct) + s->dev.num_points;
   if (shadow_per_vertex > 254) return fail(ctx, MPT_E_INVALID, "lights x direct_samples above 254");
   int rc = ensure_workspace(ctx, lane, std::max(plan.n_paths, ctx->batch_paths), plan.max_depth,
                             shadow_per_vertex);
@@ -1429,6 +1459,7 @@ struct HostScene {
   std::vector<double> env_kernel;
   uint32_t env_w = 0, env_h = 0, env_m = 0;
   float env_rep_u = 1, env_rep_v = 1;
+  std::vector<float4> points;   // 2 per point light, the device layout
 };
 
 int prepare_scene(mpt_context *ctx, const mpt_scene_desc *d, const mpt_kd_params *kd, HostScene &hs) {
@@ -1456,6 +1487,17 @@ int prepare_scene(mpt_context *ctx, const mpt_scene_desc *d, const mpt_kd_params
       return fail(ctx, MPT_E_INVALID, "sphere material out of range or emissive (only quads may emit)");
   for (uint32_t i = 0; i < nq; ++i)
     if (bad_mat(d->quad_material[i])) return fail(ctx, MPT_E_INVALID, "quad material out of range");
+  if (d->num_points && !d->points) return fail(ctx, MPT_E_INVALID, "missing scene array");
+  for (uint32_t i = 0; i < d->num_points; ++i) {
+    const float *pt = d->points + 6ull * i;
+    for (int k = 0; k < 6; ++k)
+      if (!std::isfinite(pt[k]) || (k >= 3 && pt[k] < 0)) return fail(ctx, MPT_E_INVALID, "bad point light");
+    // OmniEmitter::getLe (emitters/OmniEmitter.cpp:15-17) with the radiant
+    // exitance of a zero-area shape equal to its power (Emitter.cpp:44-46)
+    const double k4pi = 1.0 / (4.0 * M_PI);
+    hs.points.push_back(make_float4(pt[0], pt[1], pt[2], 0.0f));
+    hs.points.push_back(make_float4((float)(pt[3] * k4pi), (float)(pt[4] * k4pi), (float)(pt[5] * k4pi), 0.0f));
+  }
   for (uint32_t i = 0; i < nm; ++i) {
     const mpt_material &mt = d->materials[i];
     if (mt.type > MPT_BSDF_DIELECTRIC) return fail(ctx, MPT_E_INVALID, "unknown BSDF type");
@@ -1789,6 +1831,7 @@ int mpt_scene_create(mpt_context *ctx, const mpt_scene_desc *d, const mpt_kd_par
   if ((rc = upload(ctx, s->lights, hs.lights.data(), hs.lights.size()))) return rc;
   if ((rc = upload(ctx, s->env_tex, hs.env_tex.data(), hs.env_tex.size()))) return rc;
   if ((rc = upload(ctx, s->env_kernel, hs.env_kernel.data(), hs.env_kernel.size()))) return rc;
+  if ((rc = upload(ctx, s->points, hs.points.data(), hs.points.size()))) return rc;
   MPT_HIP(ctx, hipStreamSynchronize(ctx->stream));
   bind_scene(s->dev, s->nodes.p, s->recs.p, ko.box, s->tshade.p, s->spheres.p,
              s->sphere_mat.p, s->quads.p, s->mats.p, s->lights.p, (uint32_t)hs.lights.size(), nt, ns, nq, hs.env);
@@ -1799,6 +1842,8 @@ int mpt_scene_create(mpt_context *ctx, const mpt_scene_desc *d, const mpt_kd_par
   s->dev.env_m = hs.env_m;
   s->dev.env_rep_u = hs.env_rep_u;
   s->dev.env_rep_v = hs.env_rep_v;
+  s->dev.points = s->points.p;
+  s->dev.num_points = (uint32_t)(hs.points.size() / 2);
   s->stats[0] = ko.n_nodes;
   s->stats[1] = ko.n_leaves;
   s->stats[2] = ko.n_refs;

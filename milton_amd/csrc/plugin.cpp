@@ -74,6 +74,7 @@ void PathTracer::setScene(const mpt_scene_desc &d) {
   desc_.tri_flags = keep(tri_flags_, d.tri_flags, d.num_triangles);
   desc_.vertex_normals = keep(vnormals_, d.vertex_normals, 3ull * d.num_vertices);
   desc_.vertex_normal_mask = keep(vmask_, d.vertex_normal_mask, d.num_vertices);
+  desc_.points = keep(points_, d.points, 6ull * d.num_points);
   if (d.env_map && d.env_map->rgb) {
     env_ = *d.env_map;
     env_rgb_.assign(env_.rgb, env_.rgb + 3ull * env_.width * env_.height);

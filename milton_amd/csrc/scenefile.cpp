@@ -33,7 +33,9 @@
 //                      (box, triangle, gaussian, mitchell, lanczosSinc)
 //   renderer           noSuperSamples (spp), noDirectSamples, seed, maxDepth
 //                      (a cap the reference does not have; default 8)
-// Anything else that would change the image (cone, cylinder, blob, point,
+//   point              with an "omni" emitter: a point light (PointShape,
+//                      OmniEmitter)
+// Anything else that would change the image (cone, cylinder, blob,
 // dynamic plugins, emitting triangles, spheres or cubes, images Qt or
 // OpenEXR would read, non-uniform sphere scales, sheared planes or cubes) is an error,
 // not silently dropped.
@@ -272,9 +274,10 @@ struct Flat {
   bool any_normals = false;              // some mesh brought its own normals
   std::vector<uint8_t> vert_has_normal;  // per vertex: normal given by the file
   std::vector<float> spheres, quads;
+  std::vector<float> points;             // point lights: position, power
   std::vector<uint32_t> sphere_mat, quad_mat;
   std::vector<mpt_material> mats;
-  std::vector<int> mat_emits;            // per material: has an emitter with power
+  std::vector<int> mat_emits;            // per material: 1 oriented emitter, 2 omni emitter (with power)
   std::vector<double> mat_power;         // 3 per material
   float env[3] = {0, 0, 0};
   bool has_env_map = false;              // background is an environment image
@@ -495,9 +498,9 @@ class Loader {
         const JVal *type = c.find("type");
         if (!type || type->kind != JVal::STR) return bad("shape needs a type");
         if (!shape(type->str, c, M, mat)) return false;
-      } else if (k == "plane" || k == "sphere" || k == "triangle" || k == "mesh" || k == "cube") {
+      } else if (k == "plane" || k == "sphere" || k == "triangle" || k == "mesh" || k == "cube" || k == "point") {
         if (!shape(k, c, M, mat)) return false;
-      } else if (k == "cone" || k == "cylinder" || k == "blob" || k == "point" ||
+      } else if (k == "cone" || k == "cylinder" || k == "blob" ||
                  k == "meta" || k == "dynamic") {
         return bad("shape '" + k + "' is not supported by this path tracer");
       } else if (k == "background" && root) {
@@ -647,10 +650,11 @@ class Loader {
         if (em->kind != JVal::OBJ) return bad("emitter must be an object");
         const JVal *type = em->find("type");
         if (!type || type->kind != JVal::STR) return bad("emitter needs a type");
-        if (type->str == "oriented") {
-          if (spectrum(em->find("power"), power) && (power[0] > 0 || power[1] > 0 || power[2] > 0)) emits = 1;
+        if (type->str == "oriented" || type->str == "omni") {
+          if (spectrum(em->find("power"), power) && (power[0] > 0 || power[1] > 0 || power[2] > 0))
+            emits = type->str == "omni" ? 2 : 1;
         } else if (type->str != "null") {
-          return bad("emitter type '" + type->str + "' is not supported (oriented or null)");
+          return bad("emitter type '" + type->str + "' is not supported (oriented, omni or null)");
         }
       }
       flat_.mat_emits.push_back(emits);
@@ -699,6 +703,7 @@ class Loader {
   // (Transformable: the inverse transpose), so a mirroring transform
   // (negative determinant) swaps the spanning vectors.
   bool quad(const Mat &M, const double o[3], const double x[3], const double z[3], uint32_t m, const char *what) {
+    if (flat_.mat_emits[m] == 2) return bad("an omni emitter needs a point shape");
     double c[3], ex[3], ez[3];
     M.point(o, c);
     M.vector(x, ex);
@@ -737,6 +742,19 @@ class Loader {
       }
       const double o[3] = {0, 0, 0}, x[3] = {1, 0, 0}, z[3] = {0, 0, 1};
       return quad(M, o, x, z, m, "plane");
+    }
+    if (type == "point") {
+      // PointShape (shapes/PointShape.cpp): the transformed origin; it is
+      // never hit, so only an omni emitter on it changes the image
+      if (flat_.mat_emits[m] == 1) return bad("oriented emitters on points are not supported (omni only)");
+      if (flat_.mat_emits[m] == 2) {
+        const double o[3] = {0, 0, 0};
+        double w[3];
+        M.point(o, w);
+        for (int i = 0; i < 3; ++i) flat_.points.push_back((float)w[i]);
+        for (int i = 0; i < 3; ++i) flat_.points.push_back((float)flat_.mat_power[3 * m + i]);
+      }
+      return true;
     }
     if (type == "cube") {
       // Cube::getIntersection (shapes/Cube.cpp:17-52): the unit cube
@@ -1022,6 +1040,8 @@ int finish(Loader &ld, mpt_scenefile **out, char *err, size_t err_len) {
   d.tri_flags = f.tri_flags.data();
   d.vertex_normals = f.any_normals ? f.normals.data() : nullptr;
   d.vertex_normal_mask = f.any_normals ? f.vert_has_normal.data() : nullptr;
+  d.num_points = (uint32_t)(f.points.size() / 6);
+  d.points = f.points.data();
   *out = sf.release();
   (void)err;
   (void)err_len;

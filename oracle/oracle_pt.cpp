@@ -551,6 +551,10 @@ struct orc_scene {
   std::vector<Quad> quads;
   std::vector<Material> mats;
   std::vector<uint32_t> lights;   // quad indices with emissive materials
+  struct Point {
+    V3 p, Le;
+  };
+  std::vector<Point> points;      // point lights: position, P / (4 pi)
   V3 env;
   // Environment image (EnvironmentMap, materials/emitters/EnvironmentMap.cpp):
   // texels in double, row-major from the top row; the filter kernel as
@@ -1062,6 +1066,22 @@ V3 trace_path(const orc_scene &sc, const orc_job &job, const Camera &cam, uint32
         }
         direct = direct + dl * (lq.area / ns);
       }
+      // Point lights (DirectIllumination.cpp:96-121, isPoint): one sample, no
+      // area weight; a PointShape has no normal, so SurfacePoint's normal
+      // dot is 1 (core/SurfacePoint.h:143) and so is cos_wi.
+      for (uint32_t li = (uint32_t)sc.points.size(); li--;) {
+        const orc_scene::Point &pl = sc.points[li];
+        V3 wo = pl.p - p;
+        const double dist = len(wo);
+        if (!(dist > 0)) continue;
+        wo = wo / dist;
+        const double cos_wo = std::fabs(dot(n, wo));
+        const V3 fr = bsdf_eval(m, n, ray.d, wo);
+        if (cos_wo > 0 && !is_black(fr)) {
+          ctr.shadow++;
+          if (!sc.occluded(Ray(p, wo), dist - kEps)) direct = direct + mul(fr, pl.Le) * (cos_wo / (dist * dist));
+        }
+      }
       L = L + mul(beta, direct);
     }
     const BsdfSample bs = bsdf_sample(m, n, ray.d, bu.u, ior_index);
@@ -1115,7 +1135,7 @@ void orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t ou
 
 // Bumped with every change of an exported signature; pyoracle refuses a
 // library built from older sources.
-int orc_interface_version(void) { return 7; }
+int orc_interface_version(void) { return 8; }
 
 orc_scene *orc_scene_create(uint32_t nv, const float *vertices, uint32_t nt, const uint32_t *indices,
                             const uint32_t *tri_material, uint32_t ns, const float *spheres,
@@ -1532,6 +1552,19 @@ double orc_filter(int type, double support, const double *p, double x, double y)
 }
 
 }  // namespace
+
+// Point lights: 6 floats each, position and power; radiance P / (4 pi)
+// (emitters/OmniEmitter.cpp:15-17; Emitter.cpp:44-46 for a zero area).
+extern "C" void orc_scene_set_points(orc_scene *s, uint32_t n, const float *pts) {
+  s->points.clear();
+  for (uint32_t i = 0; i < n; ++i) {
+    const float *q = pts + 6ull * i;
+    orc_scene::Point pl;
+    pl.p = V3(q[0], q[1], q[2]);
+    pl.Le = V3(q[3], q[4], q[5]) / (4.0 * M_PI);
+    s->points.push_back(pl);
+  }
+}
 
 extern "C" int orc_scene_set_env_map(orc_scene *s, uint32_t w, uint32_t h, const float *rgb, float repeat_u,
                                      float repeat_v, int filter, uint32_t support, const float *params) {

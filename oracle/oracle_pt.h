@@ -44,7 +44,7 @@ typedef struct orc_scene orc_scene;
  * mesh triangles) with the reference build parameters.  kd_min_prims /
  * kd_max_depth of 0 select the reference defaults (3, 24).  use_naive != 0
  * replaces both kd-trees by NaiveSpatialAccel.  Returns NULL on bad input. */
-int orc_interface_version(void);   /* 3 */
+int orc_interface_version(void);   /* 8 */
 orc_scene *orc_scene_create(uint32_t nv, const float *vertices,
                             uint32_t nt, const uint32_t *indices, const uint32_t *tri_material,
                             uint32_t ns, const float *spheres, const uint32_t *sphere_material,
@@ -132,6 +132,7 @@ double orc_fresnel(const double *wi, const double *n, double n1, double n2);
  * 3 mitchell (B, C), 4 lanczosSinc (tau); zero params and repeats take the
  * reference's defaults, support 0 means 2.  Returns 0, or -1 on bad
  * arguments. */
+void orc_scene_set_points(orc_scene *s, uint32_t n, const float *points);
 int orc_scene_set_env_map(orc_scene *s, uint32_t w, uint32_t h, const float *rgb, float repeat_u, float repeat_v,
                           int filter, uint32_t support, const float *params);
 /* Background radiance seen by a ray leaving the scene in direction d. */

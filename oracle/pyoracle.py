@@ -15,7 +15,7 @@ LIB_PATH = os.path.join(HERE, "liboracle_pt.so")
 MISS = 0xFFFFFFFF
 
 
-INTERFACE_VERSION = 7
+INTERFACE_VERSION = 8
 
 
 def build():
@@ -68,6 +68,7 @@ def lib():
         L.orc_tonemap.argtypes = [vp, u64, i32, vp]
         L.orc_scene_set_env_map.argtypes = [vp, u32, u32, vp, ctypes.c_float, ctypes.c_float, i32, u32, vp]
         L.orc_background.argtypes = [vp, vp, vp]
+        L.orc_scene_set_points.argtypes = [vp, u32, vp]
         L.orc_render_film.argtypes = [vp, ctypes.POINTER(OrcJob), vp, i32]
         _lib = L
     return _lib
@@ -109,6 +110,11 @@ class OracleScene:
             kd_min_prims, kd_max_depth, 1 if naive else 0, self.threads)
         if not self.h:
             raise ValueError("oracle rejected the scene")
+        pts = getattr(a, "points", None)
+        if pts is not None and len(pts):
+            pts = np.ascontiguousarray(pts, np.float32).reshape(-1, 6)
+            self._keep.append(pts)
+            lib().orc_scene_set_points(self.h, len(pts), _p(pts))
         em = getattr(a, "env_map", None)
         if em is not None:
             img = np.ascontiguousarray(em.image, np.float32)
