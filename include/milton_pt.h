@@ -331,6 +331,17 @@ uint32_t mpt_tonemap_from_name(const char *name);
  * (renderers/RenderOutput.cpp:129-231) for point-sample renderers. */
 int mpt_film_tonemap(mpt_film *film, uint32_t tonemap, uint8_t *rgba);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, serving loops or harnesses that
+ * forbid hipMalloc): the same bytes as mpt_film_tonemap, computed with a
+ * fixed scratch array the library holds on each device for the life of the
+ * process (16 MiB: the reduction partials and the 8-bit staging image)
+ * instead of buffers allocated with the film.  Allocates nothing.  Films
+ * above MPT_NOALLOC_MAX_PIXELS pixels fail with MPT_E_INVALID.  All calls on
+ * one device share that array: calls that overlap on one device race. */
+#define MPT_NOALLOC_MAX_PIXELS 4194304u
+int mpt_film_tonemap_noalloc(mpt_film *film, uint32_t tonemap, uint8_t *rgba);
+
 /* Writes the film to a file chosen by the path's extension (case-insensitive),
  * as FileRenderOutput::finalize -> _save (renderers/outputs/FileRenderOutput.cpp:
  * 45-76) -> Image::save (common/image/Image.cpp:103-129) does:

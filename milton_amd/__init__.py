@@ -173,6 +173,7 @@ def lib():
         "mpt_film_upload_raw": ([vp, vp], i32),
         "mpt_tonemap_from_name": ([ctypes.c_char_p], u32),
         "mpt_film_tonemap": ([vp, u32, vp], i32),
+        "mpt_film_tonemap_noalloc": ([vp, u32, vp], i32),
         "mpt_film_save": ([vp, ctypes.c_char_p, u32, u32], i32),
         "mpt_image_write_ldr": ([ctypes.c_char_p, u32, u32, vp], i32),
         "mpt_image_write_hdr": ([ctypes.c_char_p, u32, u32, vp], i32),

@@ -24,6 +24,7 @@
 #include <cstdio>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <new>
 #include <string>
 #include <vector>
@@ -738,6 +739,16 @@ __global__ __launch_bounds__(kTmBlock) void k_tm_map(const float4 *__restrict__ 
                        tm_byte(255 * pow(b, correction)), 255);
 }
 
+// Scratch memory of mpt_film_tonemap_noalloc, one copy per device for the
+// life of the process: the tone map's partials (kTmGrid + 1 double2) and the
+// 8-bit staging image of films of up to kNoAllocMaxPixels pixels.  Host code
+// never passes `scratch` itself to a kernel: it looks the device address up
+// once per device (scratch_setup) and passes that.
+constexpr uint32_t kNoAllocMaxPixels = 1u << 22;
+constexpr size_t kScratchPartialBytes = ((kTmGrid + 1) * sizeof(double2) + 255) / 256 * 256;
+constexpr size_t kScratchBytes = kScratchPartialBytes + (size_t)kNoAllocMaxPixels * sizeof(uchar4);
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
 __device__ __forceinline__ uint32_t chunk_of_target(const BatchDesc *bd, uint32_t t) {
   uint32_t lo = 0, hi = bd->hdr.n_chunks - 1;
   while (lo < hi) {
@@ -883,6 +894,39 @@ uint32_t grid_for(const mpt_context *ctx, uint64_t n, uint32_t per_cu) {
 }
 
 uint64_t g_scene_uid = 0;
+
+// Device address of `scratch` per device, looked up once per device with that
+// device current (the array's name is only a host-side stand-in).
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[kMaxDevices] = {};
+hipError_t g_scratch_err[kMaxDevices] = {};
+bool g_scratch_done[kMaxDevices] = {};
+
+// Called by mpt_open with `device` current.  A failed lookup is kept and
+// reported by the entry point that needs the scratch array; nothing else
+// depends on it.
+void scratch_setup(int device) {
+  if (device < 0 || device >= kMaxDevices) return;
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (g_scratch_done[device]) return;
+  void *ptr = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+  g_scratch_err[device] = e;
+  g_scratch_ptr[device] = e == hipSuccess ? ptr : nullptr;
+  g_scratch_done[device] = true;
+}
+
+// The device address of this device's scratch array, or null (err set).
+void *scratch_ptr(int device, hipError_t *err) {
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (device < 0 || device >= kMaxDevices || !g_scratch_done[device]) {
+    *err = hipErrorNotInitialized;
+    return nullptr;
+  }
+  *err = g_scratch_err[device];
+  return g_scratch_ptr[device];
+}
 
 // Per-launch timing of the instrumented (non-graph) path.
 struct Timed {
@@ -1725,6 +1769,7 @@ int mpt_open(int device, mpt_context **out) {
   if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) return MPT_E_DEVICE;
   if (hipEventCreate(&ctx->ev0) != hipSuccess || hipEventCreate(&ctx->ev1) != hipSuccess) return MPT_E_DEVICE;
   if (hipEventCreateWithFlags(&ctx->mark, hipEventDisableTiming) != hipSuccess) return MPT_E_DEVICE;
+  scratch_setup(device);
   for (uint32_t k = 0; k < ctx->num_lanes; ++k)
     if (ensure_lane(ctx.get(), k)) return MPT_E_DEVICE;
   *out = ctx.release();
@@ -2085,6 +2130,36 @@ int mpt_film_tonemap(mpt_film *f, uint32_t tonemap, uint8_t *rgba) {
   k_tm_map<<<(n + kTmBlock - 1) / kTmBlock, kTmBlock, 0, st>>>(f->data.p, n, tonemap, f->tm.p, f->ldr.p);
   MPT_LAUNCHED(ctx, "tone map");
   MPT_HIP(ctx, hipMemcpyAsync(rgba, f->ldr.p, (size_t)n * 4, hipMemcpyDeviceToHost, st));
+  MPT_HIP(ctx, hipStreamSynchronize(st));
+  return MPT_OK;
+}
+
+int mpt_film_tonemap_noalloc(mpt_film *f, uint32_t tonemap, uint8_t *rgba) {
+  if (!f || !rgba) return MPT_E_INVALID;
+  mpt_context *ctx = f->ctx;
+  if (tonemap != MPT_TONEMAP_REINHARD && tonemap != MPT_TONEMAP_GAMMA)
+    return fail(ctx, MPT_E_INVALID, "unknown tone map");
+  const uint64_t n64 = (uint64_t)f->width * f->height;
+  if (n64 > kNoAllocMaxPixels)
+    return fail(ctx, MPT_E_INVALID,
+                "film of " + std::to_string(n64) + " pixels exceeds the allocation-free tone map's scratch limit of " +
+                    std::to_string(kNoAllocMaxPixels) + " pixels");
+  hipError_t e = hipSuccess;
+  char *base = static_cast<char *>(scratch_ptr(ctx->device, &e));
+  if (!base) return hip_fail(ctx, e, "scratch array address (hipGetSymbolAddress)");
+  double2 *partials = reinterpret_cast<double2 *>(base);
+  uchar4 *ldr = reinterpret_cast<uchar4 *>(base + kScratchPartialBytes);
+  const uint32_t n = (uint32_t)n64;
+  hipStream_t st = ctx->stream;
+  // every partial is written by k_tm_reduce before k_tm_fold reads it, so the
+  // array needs no clearing between calls
+  if (tonemap == MPT_TONEMAP_REINHARD) {
+    k_tm_reduce<<<kTmGrid, kTmBlock, 0, st>>>(f->data.p, n, partials);
+    k_tm_fold<<<1, kTmBlock, 0, st>>>(partials, n);
+  }
+  k_tm_map<<<(n + kTmBlock - 1) / kTmBlock, kTmBlock, 0, st>>>(f->data.p, n, tonemap, partials, ldr);
+  MPT_LAUNCHED(ctx, "tone map");
+  MPT_HIP(ctx, hipMemcpyAsync(rgba, ldr, (size_t)n * 4, hipMemcpyDeviceToHost, st));
   MPT_HIP(ctx, hipStreamSynchronize(st));
   return MPT_OK;
 }
