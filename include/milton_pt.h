@@ -472,6 +472,15 @@ int mpt_server_stop(mpt_server *server);
 /* out: [batches, paths, extension rays, shadow rays, jobs done, peak film bytes] */
 int mpt_server_stats(mpt_server *server, uint64_t out[6]);
 
+/* Paths per batch the engine plans for a job with direct_samples light
+ * samples per area light (0 = 1) on a scene with num_lights area lights and
+ * num_points point lights, given the context's MPT_OPT_BATCH_PATHS: each
+ * path of a batch reserves one shadow ray (32 bytes) per light sample, and
+ * the batch is lowered so that a batch's shadow rays stay within 2^28 (but
+ * not below 1024 paths).  Host only; no reference counterpart. */
+uint64_t mpt_batch_paths_for(uint64_t batch_paths, uint32_t num_lights, uint32_t num_points,
+                             uint32_t direct_samples);
+
 /* Synchronises the context's stream. */
 int mpt_sync(mpt_context *ctx);
 /* Sets a context option (MPT_OPT_*). */

@@ -152,6 +152,7 @@ def lib():
     vp, u32, u64, i32 = ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint64, ctypes.c_int
     sig = {
         "mpt_abi_version": ([], i32),
+        "mpt_batch_paths_for": ([u64, u32, u32, u32], u64),
         "mpt_open": ([i32, ctypes.POINTER(vp)], i32),
         "mpt_close": ([vp], i32),
         "mpt_last_error": ([vp], ctypes.c_char_p),

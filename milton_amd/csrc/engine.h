@@ -26,6 +26,10 @@ constexpr uint32_t kMaxJobs = 256;      // jobs per batch
 constexpr uint32_t kMaxDirect = 64;     // light samples per light per vertex
 constexpr uint32_t kMaxChunks = 1024;   // pixel ranges per batch
 constexpr uint32_t kJobBits = 20;       // sample index bits in a path's (job, sample) word
+// Shadow rays a batch may hold (32 B each, 8 GiB per lane): a batch of a job
+// with many light samples per vertex gets fewer paths.
+constexpr uint64_t kShadowRayBudget = 1ull << 28;
+constexpr uint32_t kMinBatchPaths = 1024;
 
 struct CameraDev {
   float eye[3], u[3], v[3], n[3];
@@ -93,20 +97,26 @@ struct DBuf {
 
 // Path and queue buffers, sized by the batch capacity.
 struct Workspace {
-  uint32_t cap = 0, depth = 0, lights = 0;
+  uint32_t cap = 0, depth = 0;
+  uint64_t shadow_entries = 0;   // shadow rays the shadow buffer holds
   uint64_t generation = 0;   // bumped on every reallocation (invalidates graphs)
   DBuf<float4> q[2];
   DBuf<float2> hits;         // (FP32 t, leaf record of the closest hit or MPT_MISS)
-  DBuf<float4> sq;
+  // NEE: one group per shading vertex with light samples, (origin, slot)
+  // (origin primitive, ray count, 0, 0); its rays sit at sr[slot * spv ...],
+  // (direction, distance limit) (contribution, 0) each, spv = shadow rays per
+  // vertex of the batch
+  DBuf<float4> sg;
+  DBuf<float4> sr;
   DBuf<float4> Ls;
   DBuf<uint32_t> flags;
   DBuf<uint2> pjs;           // (pixel, job << 20 | sample) per slot
   DBuf<float2> pos;          // film position of each camera sample, in pixels (filtered jobs)
-  DBuf<uint32_t> counts;     // [0, D] queue sizes, [D+1, 2D] shadow sizes, [2D+1, 3D] claim cursors
+  DBuf<uint32_t> counts;     // [0, D] queue sizes, [D+1, 2D] shadow group counts, [2D+1, 3D] claim cursors
   DBuf<unsigned long long> totals;
   DBuf<BatchDesc> desc;
   size_t bytes() const {
-    return q[0].bytes() + q[1].bytes() + hits.bytes() + sq.bytes() + Ls.bytes() + flags.bytes() +
+    return q[0].bytes() + q[1].bytes() + hits.bytes() + sg.bytes() + sr.bytes() + Ls.bytes() + flags.bytes() +
            pjs.bytes() + pos.bytes() + counts.bytes() + totals.bytes() + desc.bytes();
   }
 };
@@ -173,6 +183,7 @@ struct mpt_context {
   std::vector<mpt_film *> films;
   std::vector<mpt_server *> servers;
   std::vector<mpt_engine::GraphEntry> graphs_cache;
+  std::vector<std::pair<const void *, uint32_t>> grid_cache;   // resident blocks per kernel
 };
 
 struct mpt_scene {
@@ -213,8 +224,12 @@ int resolve_focus(mpt_scene *s, mpt_job &job);
 // Creates lane k's stream, events and staging ring if it has none yet.
 int ensure_lane(mpt_context *ctx, uint32_t k);
 // Makes sure lane k's workspace holds a batch of `cap` paths, `depth`
-// bounces and `lights` shadow rays per vertex.
-int ensure_workspace(mpt_context *ctx, uint32_t k, uint32_t cap, uint32_t depth, uint32_t lights);
+// bounces and `spv` shadow rays per vertex.
+int ensure_workspace(mpt_context *ctx, uint32_t k, uint32_t cap, uint32_t depth, uint32_t spv);
+// Paths per batch for a scene and the largest light-sample count of the
+// batch's jobs: the context's batch size, lowered so that the shadow rays
+// of one batch (paths x shadow rays per vertex) fit kShadowRayBudget.
+uint32_t batch_capacity(const mpt_context *ctx, const mpt_scene *s, uint32_t max_direct);
 
 // Enqueues one batch on lane `lane` (asynchronous).  With L_host the
 // per-path radiance of the batch is copied to L_host (slot order) and the

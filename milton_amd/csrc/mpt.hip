@@ -78,6 +78,47 @@ __device__ __forceinline__ uint32_t block_append(bool want, uint32_t *counter, u
   return idx;
 }
 
+// Two appends with one exchange: lanes with wa get consecutive indices of
+// counter ca and lanes with wb of counter cb, with one atomic per counter per
+// block.  sh holds 2 * (WAVES + 1) words per call; callers alternate between
+// two such regions (`parity`), so a call needs no barrier at its end: the
+// next call that writes the same region comes after another call's first
+// barrier, which every thread reaches only after reading this call's result.
+// Every thread of the block must call it.
+template <int WAVES>
+__device__ __forceinline__ void block_append2(bool wa, bool wb, uint32_t *ca, uint32_t *cb, uint32_t *sh_pair,
+                                              uint32_t parity, uint32_t &ia, uint32_t &ib) {
+  uint32_t *sh = sh_pair + parity * 2 * (WAVES + 1);
+  const uint64_t ma = __ballot(wa), mb = __ballot(wb);
+  const uint32_t wave = threadIdx.x / 64;
+  if (lane_id() == 0) {
+    sh[wave] = (uint32_t)__popcll(ma);
+    sh[WAVES + 1 + wave] = (uint32_t)__popcll(mb);
+  }
+  __syncthreads();
+  if (threadIdx.x < 2) {   // thread 0 books counter a, thread 1 counter b
+    uint32_t *row = sh + threadIdx.x * (WAVES + 1);
+    uint32_t total = 0;
+    for (int w = 0; w < WAVES; ++w) {
+      const uint32_t c = row[w];
+      row[w] = total;
+      total += c;
+    }
+    row[WAVES] = total ? atomicAdd(threadIdx.x ? cb : ca, total) : 0u;
+  }
+  __syncthreads();
+  const uint32_t below_a = __builtin_amdgcn_mbcnt_hi((uint32_t)(ma >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)ma, 0u));
+  const uint32_t below_b = __builtin_amdgcn_mbcnt_hi((uint32_t)(mb >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mb, 0u));
+  ia = sh[WAVES] + sh[wave] + below_a;
+  ib = sh[2 * WAVES + 1] + sh[WAVES + 1 + wave] + below_b;
+}
+
+// Sum of v over the wave (for statistics counters).
+__device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+  return v;
+}
+
 // ---------------------------------------------------------------------------
 // Kernels.
 // ---------------------------------------------------------------------------
@@ -265,26 +306,48 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(STACK <= 8 
   }
 }
 
-// Any hit of the shadow queue (kdTreeAccel::intersects); unoccluded
-// contributions are added to the path's radiance.
+// Any hit of the shadow rays (kdTreeAccel::intersects).  One lane per group:
+// the light samples of one vertex are a run of the shadow buffer
+// (sr[slot * spv ...], spv = shadow rays per vertex of the batch), traced in
+// (light, sample) order.  Their unoccluded contributions are summed in that
+// order and added to the path's radiance once, as DirectIllumination sums
+// per light in a fixed loop (renderers/DirectIllumination.cpp:90-135), so a
+// path's value does not depend on how the rays were scheduled.  Each slot
+// has at most one group per launch, so its radiance sees one addition here.
 template <int STACK>
-__global__ __launch_bounds__(256) void k_shadow(Scene sc, const float4 *__restrict__ sq,
-                                                const uint32_t *__restrict__ count, float4 *__restrict__ Ls) {
+__global__ __launch_bounds__(256) void k_shadow(Scene sc, const BatchDesc *__restrict__ bd,
+                                                const float4 *__restrict__ sg, const float4 *__restrict__ sr,
+                                                const uint32_t *__restrict__ count, float4 *__restrict__ Ls,
+                                                unsigned long long *__restrict__ total_rays) {
   __shared__ uint2 stack[STACK * kBlock];
   uint2 *stk = stack + threadIdx.x;
   const uint32_t n = *count;
+  const uint32_t spv = sc.num_lights * bd->hdr.max_direct + sc.num_points;
+  uint32_t rays = 0;
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-    const float4 a = sq[3 * i], b = sq[3 * i + 1], c = sq[3 * i + 2];
-    const RaySetup r = setup_ray(xyz(a), xyz(b));
-    float t;
-    uint32_t id;
-    if (!trace<true, false, STACK>(sc, r, b.w, __float_as_uint(c.w), stk, t, id)) {
-      float4 *L = Ls + __float_as_uint(a.w);
-      atomicAdd(&L->x, c.x);
-      atomicAdd(&L->y, c.y);
-      atomicAdd(&L->z, c.z);
+    // group: (origin, slot) (origin primitive, ray count, 0, 0)
+    const float4 g0 = sg[2ull * i], g1 = sg[2ull * i + 1];
+    const uint32_t slot = __float_as_uint(g0.w), prim = __float_as_uint(g1.x), cnt = __float_as_uint(g1.y);
+    // ray: (direction, distance limit) (contribution, 0)
+    const float4 *run = sr + 2ull * ((uint64_t)slot * spv);
+    float3 sum = f3(0, 0, 0);
+    for (uint32_t j = 0; j < cnt; ++j) {
+      const float4 a = run[2 * j], c = run[2 * j + 1];
+      const RaySetup r = setup_ray(xyz(g0), xyz(a));
+      float t;
+      uint32_t id;
+      if (!trace<true, false, STACK>(sc, r, a.w, prim, stk, t, id)) sum = add(sum, xyz(c));
+    }
+    rays += cnt;
+    if (!is_black(sum)) {
+      float4 *L = Ls + slot;
+      atomicAdd(&L->x, sum.x);
+      atomicAdd(&L->y, sum.y);
+      atomicAdd(&L->z, sum.z);
     }
   }
+  rays = wave_sum(rays);
+  if (lane_id() == 0 && rays) atomicAdd(total_rays, (unsigned long long)rays);
 }
 
 struct HitInfo {
@@ -374,27 +437,40 @@ __device__ __forceinline__ float3 env_radiance(const Scene &sc, float3 dir) {
   return env_radiance_at(sc.env_tex, sc.env_kernel, sc.env_w, sc.env_h, sc.env_m, sc.env_rep_u, sc.env_rep_v, dir);
 }
 
-constexpr int kShadeBlock = 512;
+#ifndef MPT_SHADE_BLOCK
+#define MPT_SHADE_BLOCK 256
+#endif
+constexpr int kShadeBlock = MPT_SHADE_BLOCK;   // threads per shading block
+// Waves per SIMD the shading kernel is compiled for (its register budget):
+// the most at which the single-sample form does not spill; the form with
+// several light samples per light gets one fewer.
 #ifndef MPT_SHADE_WAVES
-#define MPT_SHADE_WAVES 6
+#define MPT_SHADE_WAVES 5
 #endif
 
 // kMulti: some job of the batch draws several light samples per light; the
 // common single-sample form keeps the NEE loop's counters out of registers.
 // kEnv: the scene's background is an environment image (a constant
 // otherwise).
+//
+// The grid holds as many blocks as fit on the device at once (every block
+// strides over the queue the same number of times, give or take one), and
+// each pass appends its shadow groups and extension rays with one exchange.
 template <bool kMulti, bool kEnv>
-__global__ __launch_bounds__(kShadeBlock) __attribute__((amdgpu_waves_per_eu(MPT_SHADE_WAVES))) void k_shade(Scene sc, const BatchDesc *__restrict__ bd, uint32_t depth,
-                                               const float4 *__restrict__ qin, const float2 *__restrict__ hits,
-                                               const uint32_t *__restrict__ count_in, float4 *__restrict__ qout,
-                                               uint32_t *__restrict__ count_out, float4 *__restrict__ sq,
-                                               uint32_t *__restrict__ count_shadow, float4 *__restrict__ Ls,
-                                               uint32_t *__restrict__ flags, const uint2 *__restrict__ pjs) {
-  __shared__ uint32_t append_sh[kShadeBlock / 64 + 1];
+__global__ __launch_bounds__(kShadeBlock) __attribute__((amdgpu_waves_per_eu(kMulti ? MPT_SHADE_WAVES - 1 : MPT_SHADE_WAVES))) void k_shade(
+    Scene sc, const BatchDesc *__restrict__ bd, uint32_t depth, const float4 *__restrict__ qin,
+    const float2 *__restrict__ hits, const uint32_t *__restrict__ count_in, float4 *__restrict__ qout,
+    uint32_t *__restrict__ count_out, float4 *__restrict__ sg, float4 *__restrict__ sr,
+    uint32_t *__restrict__ count_groups, float4 *__restrict__ Ls, uint32_t *__restrict__ flags,
+    const uint2 *__restrict__ pjs) {
+  __shared__ uint32_t append_sh[4 * (kShadeBlock / 64 + 1)];
   const uint32_t n = *count_in;
   const uint32_t stride = gridDim.x * blockDim.x;
-  // block-uniform loop: block_append synchronises the block
-  for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += stride) {
+  // shadow rays per vertex: the stride of a slot's run in the shadow buffer
+  const uint32_t spv = sc.num_lights * bd->hdr.max_direct + sc.num_points;
+  uint32_t parity = 0;
+  // block-uniform loop: block_append2 synchronises the block
+  for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += stride, parity ^= 1u) {
     const uint32_t i = base + threadIdx.x;
     const bool active = i < n;
     bool want_ext = false;
@@ -414,7 +490,7 @@ __global__ __launch_bounds__(kShadeBlock) __attribute__((amdgpu_waves_per_eu(MPT
     uint32_t ndir_job = 1;   // light samples per light
     if (active) {
       // (origin, slot) (direction, origin primitive) (throughput, flags)
-      const float4 a = qin[3 * i], b = qin[3 * i + 1], c = qin[3 * i + 2];
+      const float4 a = qin[3ull * i], b = qin[3ull * i + 1], c = qin[3ull * i + 2];
       const float2 h = hits[i];
       slot = __float_as_uint(a.w);
       d = xyz(b);
@@ -483,67 +559,52 @@ __global__ __launch_bounds__(kShadeBlock) __attribute__((amdgpu_waves_per_eu(MPT
     // Next-event estimation (DirectIllumination.cpp:90-135): per light, n
     // shadow rays to points jittered over floor(sqrt(n))^2 strata of the
     // light, each weighted area / n; skipped at specular vertices (:57) and
-    // on the light itself (:93).  Every lane of the block runs the same
-    // number of appends (the block's largest n), as block_append requires.
-    const uint32_t n_dir = kMulti ? bd->hdr.max_direct : 1u;
-    for (uint32_t li = 0; li < sc.num_lights; ++li) {
-      for (uint32_t k = 0; k < n_dir; ++k) {
-        bool want_sh = false;
-        float3 so = p, sd = f3(0, 0, 1), contrib = f3(0, 0, 0);
-        float sdist = 0.0f;
-        if (surface && !specular && k < ndir_job) {
-          const uint32_t q = sc.lights[li];
-          const uint32_t lgid = sc.nt + sc.ns + q;
-          if (lgid != prim) {
-            const float4 *lq = sc.quads + 4ull * q;
-            const float4 c0 = lq[0], ex = lq[1], ez = lq[2], ln = lq[3];
-            const mpt_rng::U4 r = mpt_rng::path_draw(seed, job_id, pixel, sample, depth, 1 + li * ndir_job + k);
-            // floor(sqrt(n)) strata per side
-            uint32_t side = 1;
-            if (kMulti)
-              while ((side + 1) * (side + 1) <= ndir_job) ++side;
-            const uint32_t cell = k % (side * side);
-            const float inv = 1.0f / (float)side;
-            const float u = ((float)(cell % side) + mpt_rng::unitf(r.x)) * inv;
-            const float v = ((float)(cell / side) + mpt_rng::unitf(r.y)) * inv;
-            // Plane::getPosition (shapes/Plane.cpp:91-93)
-            const float3 lp = add(xyz(c0), add(mul(xyz(ex), u - 0.5f), mul(xyz(ez), v - 0.5f)));
-            float3 w = sub(lp, p);
-            const float dist = len(w);
-            w = mul(w, 1.0f / dist);
-            const float cos_wo = fabsf(dot(hi.n, w));
-            const float cos_wi = fabsf(dot(xyz(ln), w));
-            const float3 fr = bsdf_eval(m, hi.n, d, w);
-            // OrientedEmitter::getLe (emitters/OrientedEmitter.cpp:17-23)
-            const mpt_material lm = sc.mats[__float_as_uint(ln.w)];
-            const bool lit = dot(xyz(ln), w) < 0.0f;
-            if (cos_wo > 0.0f && cos_wi > 0.0f && !is_black(fr) && lit) {
-              const float g = cos_wo * cos_wi / (dist * dist) * (c0.w / (float)ndir_job);
-              contrib = mul(mul(beta, mul(fr, le_of(lm))), g);
-              if (!is_black(contrib)) {
-                want_sh = true;
-                sd = w;
-                sdist = dist * (1.0f - 1e-5f);
-              }
+    // on the light itself (:93).  The rays go to the slot's own run of the
+    // shadow buffer in (light, sample) order; one group entry per vertex
+    // points k_shadow at them.
+    uint32_t n_sh = 0;
+    if (surface && !specular && spv) {
+      float4 *run = sr + 2ull * ((uint64_t)slot * spv);
+      uint32_t side = 1;   // floor(sqrt(n)) strata per side
+      if (kMulti)
+        while ((side + 1) * (side + 1) <= ndir_job) ++side;
+      const float inv = 1.0f / (float)side;
+      for (uint32_t li = 0; li < sc.num_lights; ++li) {
+        const uint32_t q = sc.lights[li];
+        if (sc.nt + sc.ns + q == prim) continue;
+        const float4 *lq = sc.quads + 4ull * q;
+        const float4 c0 = lq[0], ex = lq[1], ez = lq[2], ln = lq[3];
+        // OrientedEmitter::getLe (emitters/OrientedEmitter.cpp:17-23)
+        const float3 lle = le_of(sc.mats[__float_as_uint(ln.w)]);
+        for (uint32_t k = 0; k < ndir_job; ++k) {
+          const mpt_rng::U4 r = mpt_rng::path_draw(seed, job_id, pixel, sample, depth, 1 + li * ndir_job + k);
+          const uint32_t cell = kMulti ? k % (side * side) : 0u;
+          const float u = ((float)(cell % side) + mpt_rng::unitf(r.x)) * inv;
+          const float v = ((float)(cell / side) + mpt_rng::unitf(r.y)) * inv;
+          // Plane::getPosition (shapes/Plane.cpp:91-93)
+          const float3 lp = add(xyz(c0), add(mul(xyz(ex), u - 0.5f), mul(xyz(ez), v - 0.5f)));
+          float3 w = sub(lp, p);
+          const float dist = len(w);
+          w = mul(w, 1.0f / dist);
+          const float cos_wo = fabsf(dot(hi.n, w));
+          const float cos_wi = fabsf(dot(xyz(ln), w));
+          const float3 fr = bsdf_eval(m, hi.n, d, w);
+          const bool lit = dot(xyz(ln), w) < 0.0f;
+          if (cos_wo > 0.0f && cos_wi > 0.0f && !is_black(fr) && lit) {
+            const float g = cos_wo * cos_wi / (dist * dist) * (c0.w / (float)ndir_job);
+            const float3 contrib = mul(mul(beta, mul(fr, lle)), g);
+            if (!is_black(contrib)) {
+              run[2 * n_sh] = make_float4(w.x, w.y, w.z, dist * (1.0f - 1e-5f));
+              run[2 * n_sh + 1] = make_float4(contrib.x, contrib.y, contrib.z, 0.0f);
+              ++n_sh;
             }
           }
         }
-        const uint32_t si = block_append<kShadeBlock / 64>(want_sh, count_shadow, append_sh);
-        if (want_sh) {
-          sq[3 * si] = make_float4(so.x, so.y, so.z, __uint_as_float(slot));
-          sq[3 * si + 1] = make_float4(sd.x, sd.y, sd.z, sdist);
-          sq[3 * si + 2] = make_float4(contrib.x, contrib.y, contrib.z, __uint_as_float(prim));
-        }
       }
-    }
-    // Point lights (DirectIllumination.cpp:96-121 with isPoint): one sample,
-    // no area weight, and a point has no normal, so its cosine is 1
-    // (SurfacePoint's normal dot, core/SurfacePoint.h:143).
-    for (uint32_t li = 0; li < sc.num_points; ++li) {
-      bool want_sh = false;
-      float3 sd = f3(0, 0, 1), contrib = f3(0, 0, 0);
-      float sdist = 0.0f;
-      if (surface && !specular) {
+      // Point lights (DirectIllumination.cpp:96-121 with isPoint): one
+      // sample, no area weight, and a point has no normal, so its cosine is
+      // 1 (SurfacePoint's normal dot, core/SurfacePoint.h:143).
+      for (uint32_t li = 0; li < sc.num_points; ++li) {
         const float4 lp = sc.points[2 * li], le = sc.points[2 * li + 1];
         float3 w = sub(xyz(lp), p);
         const float dist = len(w);
@@ -551,19 +612,13 @@ __global__ __launch_bounds__(kShadeBlock) __attribute__((amdgpu_waves_per_eu(MPT
         const float cos_wo = fabsf(dot(hi.n, w));
         const float3 fr = bsdf_eval(m, hi.n, d, w);
         if (dist > 0.0f && cos_wo > 0.0f && !is_black(fr)) {
-          contrib = mul(mul(beta, mul(fr, xyz(le))), cos_wo / (dist * dist));
+          const float3 contrib = mul(mul(beta, mul(fr, xyz(le))), cos_wo / (dist * dist));
           if (!is_black(contrib)) {
-            want_sh = true;
-            sd = w;
-            sdist = dist * (1.0f - 1e-5f);
+            run[2 * n_sh] = make_float4(w.x, w.y, w.z, dist * (1.0f - 1e-5f));
+            run[2 * n_sh + 1] = make_float4(contrib.x, contrib.y, contrib.z, 0.0f);
+            ++n_sh;
           }
         }
-      }
-      const uint32_t si = block_append<kShadeBlock / 64>(want_sh, count_shadow, append_sh);
-      if (want_sh) {
-        sq[3 * si] = make_float4(p.x, p.y, p.z, __uint_as_float(slot));
-        sq[3 * si + 1] = make_float4(sd.x, sd.y, sd.z, sdist);
-        sq[3 * si + 2] = make_float4(contrib.x, contrib.y, contrib.z, __uint_as_float(prim));
       }
     }
     if (surface) {
@@ -582,11 +637,16 @@ __global__ __launch_bounds__(kShadeBlock) __attribute__((amdgpu_waves_per_eu(MPT
         }
       }
     }
-    const uint32_t ei = block_append<kShadeBlock / 64>(want_ext, count_out, append_sh);
+    uint32_t gi, ei;
+    block_append2<kShadeBlock / 64>(n_sh != 0, want_ext, count_groups, count_out, append_sh, parity, gi, ei);
+    if (n_sh) {
+      sg[2ull * gi] = make_float4(p.x, p.y, p.z, __uint_as_float(slot));
+      sg[2ull * gi + 1] = make_float4(__uint_as_float(prim), __uint_as_float(n_sh), 0.0f, 0.0f);
+    }
     if (want_ext) {
-      qout[3 * ei] = make_float4(p.x, p.y, p.z, __uint_as_float(slot));
-      qout[3 * ei + 1] = make_float4(wo.x, wo.y, wo.z, __uint_as_float(prim));
-      qout[3 * ei + 2] = make_float4(beta.x, beta.y, beta.z, __uint_as_float(fl));
+      qout[3ull * ei] = make_float4(p.x, p.y, p.z, __uint_as_float(slot));
+      qout[3ull * ei + 1] = make_float4(wo.x, wo.y, wo.z, __uint_as_float(prim));
+      qout[3ull * ei + 2] = make_float4(beta.x, beta.y, beta.z, __uint_as_float(fl));
     }
     // the path's flags as k_finalize needs them, once, when it ends
     if (active && !want_ext) flags[slot] = fl;
@@ -623,7 +683,23 @@ __global__ __launch_bounds__(256) void k_film(const BatchDesc *__restrict__ bd, 
   const uint32_t spp = job.spp;
   const float4 *L = Ls + ch.slot0 + (uint64_t)(i - ch.pixoff) * spp;
   float r = 0, g = 0, b = 0;
-  for (uint32_t s = 0; s < spp; ++s) {
+  // Neighbouring lanes' runs are spp * 16 bytes apart, so one load
+  // instruction touches a cache line per lane.  Eight loads back to back use
+  // the whole 128-byte line while it is still cached; the sum stays in
+  // sample order.
+  uint32_t s = 0;
+  for (; s + 8 <= spp; s += 8) {
+    float4 v[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) v[k] = L[s + k];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      r += v[k].x;
+      g += v[k].y;
+      b += v[k].z;
+    }
+  }
+  for (; s < spp; ++s) {
     const float4 v = L[s];
     r += v.x;
     g += v.y;
@@ -815,17 +891,14 @@ __global__ __launch_bounds__(256) void k_film_filtered(const BatchDesc *__restri
   }
 }
 
+// Camera and extension rays of a batch (shadow rays are counted by k_shadow).
 __global__ void k_totals(const uint32_t *__restrict__ counts, uint32_t depth,
                          unsigned long long *__restrict__ totals) {
   if (threadIdx.x != 0) return;
-  unsigned long long ext = 0, sh = 0;
-  for (uint32_t d = 0; d < depth; ++d) {
-    ext += counts[d];
-    sh += counts[depth + 1 + d];
-  }
+  unsigned long long ext = 0;
+  for (uint32_t d = 0; d < depth; ++d) ext += counts[d];
   totals[0] += counts[0];
   totals[1] += ext;
-  totals[2] += sh;
 }
 
 template <bool ANY>
@@ -967,16 +1040,32 @@ void launch_extend(mpt_context *ctx, hipStream_t st, uint32_t cap, uint32_t g, c
   }
 }
 
-void launch_shadow(mpt_context *ctx, hipStream_t st, uint64_t cap, uint32_t g, const Scene &sc,
-                   const float4 *sq, const uint32_t *count, float4 *Ls) {
-  // form 2: one ray per lane over the queue's capacity, as for closest hits
-  if (ctx->persistent == 2) g = (uint32_t)std::min<uint64_t>((cap + kBlock - 1) / kBlock, 0x7FFFFFFFull);
+void launch_shadow(mpt_context *ctx, hipStream_t st, uint32_t cap, uint32_t g, const Scene &sc, const BatchDesc *bd,
+                   const float4 *sg, const float4 *sr, const uint32_t *count, float4 *Ls,
+                   unsigned long long *total_rays) {
+  // form 2: one group per lane over the group queue's capacity (one group per
+  // path slot at most), as for closest hits
+  if (ctx->persistent == 2) g = (uint32_t)(((uint64_t)cap + kBlock - 1) / kBlock);
   switch (ctx->stack) {
-    case 8: k_shadow<8><<<g, kBlock, 0, st>>>(sc, sq, count, Ls); break;
-    case 12: k_shadow<12><<<g, kBlock, 0, st>>>(sc, sq, count, Ls); break;
-    case 24: k_shadow<24><<<g, kBlock, 0, st>>>(sc, sq, count, Ls); break;
-    default: k_shadow<16><<<g, kBlock, 0, st>>>(sc, sq, count, Ls); break;
+    case 8: k_shadow<8><<<g, kBlock, 0, st>>>(sc, bd, sg, sr, count, Ls, total_rays); break;
+    case 12: k_shadow<12><<<g, kBlock, 0, st>>>(sc, bd, sg, sr, count, Ls, total_rays); break;
+    case 24: k_shadow<24><<<g, kBlock, 0, st>>>(sc, bd, sg, sr, count, Ls, total_rays); break;
+    default: k_shadow<16><<<g, kBlock, 0, st>>>(sc, bd, sg, sr, count, Ls, total_rays); break;
   }
+}
+
+// Blocks of a kernel resident on the whole device at once (at least one per
+// CU), cached per kernel.
+template <class K>
+uint32_t resident_grid(mpt_context *ctx, K kernel, int block) {
+  const void *key = reinterpret_cast<const void *>(kernel);
+  for (const auto &e : ctx->grid_cache)
+    if (e.first == key) return e.second;
+  int per_cu = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, block, 0) != hipSuccess || per_cu < 1) per_cu = 1;
+  const uint32_t g = (uint32_t)ctx->num_cus * (uint32_t)per_cu;
+  ctx->grid_cache.push_back({key, g});
+  return g;
 }
 
 // Enqueues the kernel sequence of one batch (the part a graph captures).
@@ -1016,11 +1105,11 @@ int enqueue_batch(mpt_context *ctx, Lane &ln, mpt_scene *s, uint32_t D, bool mul
                   instrument && ctx->counters ? w.totals.p + 3 : nullptr, w.counts.p + 2 * D + 1 + d);
     if ((rc = toc())) return rc;
     if ((rc = tic(1))) return rc;
-    const uint32_t sgrid = (g * kBlock + kShadeBlock - 1) / kShadeBlock;
     auto shade = [&](auto kern) {
+      const uint32_t sgrid = resident_grid(ctx, kern, kShadeBlock);
       kern<<<sgrid, kShadeBlock, 0, st>>>(s->dev, w.desc.p, d, qin, w.hits.p, w.counts.p + d, qout,
-                                          w.counts.p + d + 1, w.sq.p, w.counts.p + D + 1 + d, w.Ls.p, w.flags.p,
-                                          w.pjs.p);
+                                          w.counts.p + d + 1, w.sg.p, w.sr.p, w.counts.p + D + 1 + d, w.Ls.p,
+                                          w.flags.p, w.pjs.p);
     };
     const bool env = s->dev.env_w != 0;
     if (multi)
@@ -1030,8 +1119,8 @@ int enqueue_batch(mpt_context *ctx, Lane &ln, mpt_scene *s, uint32_t D, bool mul
     if ((rc = toc())) return rc;
     if (L) {
       if ((rc = tic(2))) return rc;
-      launch_shadow(ctx, st, (uint64_t)cap * std::max(1u, w.lights), g, s->dev, w.sq.p, w.counts.p + D + 1 + d,
-                    w.Ls.p);
+      launch_shadow(ctx, st, cap, g, s->dev, w.desc.p, w.sg.p, w.sr.p, w.counts.p + D + 1 + d, w.Ls.p,
+                    w.totals.p + 2);
       if ((rc = toc())) return rc;
     }
     MPT_LAUNCHED(ctx, "depth kernels");
@@ -1210,6 +1299,10 @@ int resolve_focus(mpt_scene *s, mpt_job &job) {
   return MPT_OK;
 }
 
+uint32_t batch_capacity(const mpt_context *ctx, const mpt_scene *s, uint32_t max_direct) {
+  return (uint32_t)mpt_batch_paths_for(ctx->batch_paths, s->dev.num_lights, s->dev.num_points, max_direct);
+}
+
 int ensure_lane(mpt_context *ctx, uint32_t k) {
   Lane &ln = ctx->lanes[k];
   if (ln.stream) return MPT_OK;
@@ -1234,24 +1327,25 @@ void release_lane(Lane &ln) {
   ln.stream = nullptr;
 }
 
-int ensure_workspace(mpt_context *ctx, uint32_t k, uint32_t cap, uint32_t depth, uint32_t lights) {
+int ensure_workspace(mpt_context *ctx, uint32_t k, uint32_t cap, uint32_t depth, uint32_t spv) {
   int rc = ensure_lane(ctx, k);
   if (rc) return rc;
   Lane &ln = ctx->lanes[k];
   Workspace &w = ln.ws;
+  // the shadow buffer holds spv rays for each of this batch's slots
+  const uint64_t shadow = std::max<uint64_t>(w.shadow_entries, (uint64_t)cap * spv);
   cap = std::max(cap, w.cap);
   depth = std::max(depth, w.depth);
-  lights = std::max(lights, w.lights);
-  if (cap == w.cap && depth == w.depth && lights == w.lights && w.desc.p) return MPT_OK;
+  if (cap == w.cap && depth == w.depth && shadow == w.shadow_entries && w.desc.p) return MPT_OK;
   MPT_HIP(ctx, hipStreamSynchronize(ln.stream));
-  const uint64_t ns = (uint64_t)cap * std::max<uint32_t>(lights, 1);
   MPT_HIP(ctx, w.q[0].reserve(3ull * cap));
   MPT_HIP(ctx, w.q[1].reserve(3ull * cap));
   MPT_HIP(ctx, w.hits.reserve(cap));
   MPT_HIP(ctx, w.Ls.reserve(cap));
   MPT_HIP(ctx, w.flags.reserve(cap));
   MPT_HIP(ctx, w.pjs.reserve(cap));
-  MPT_HIP(ctx, w.sq.reserve(3ull * ns));
+  MPT_HIP(ctx, w.sg.reserve(2ull * cap));
+  MPT_HIP(ctx, w.sr.reserve(2ull * std::max<uint64_t>(shadow, 1)));
   MPT_HIP(ctx, w.counts.reserve(3ull * depth + 1));
   MPT_HIP(ctx, w.desc.reserve(1));
   if (!w.totals.p) {
@@ -1262,7 +1356,7 @@ int ensure_workspace(mpt_context *ctx, uint32_t k, uint32_t cap, uint32_t depth,
   }
   w.cap = cap;
   w.depth = depth;
-  w.lights = lights;
+  w.shadow_entries = shadow;
   ++w.generation;
   drop_graphs(ctx, 0);
   return MPT_OK;
@@ -1339,8 +1433,8 @@ int run_batch(mpt_context *ctx, mpt_scene *s, const BatchPlan &plan, float *L_ho
   // numbered 1 + light * samples + sample below 256
   const uint32_t shadow_per_vertex = s->dev.num_lights * std::max(1u, plan.max_direct) + s->dev.num_points;
   if (shadow_per_vertex > 254) return fail(ctx, MPT_E_INVALID, "lights x direct_samples above 254");
-  int rc = ensure_workspace(ctx, lane, std::max(plan.n_paths, ctx->batch_paths), plan.max_depth,
-                            shadow_per_vertex);
+  int rc = ensure_workspace(ctx, lane, std::max(plan.n_paths, batch_capacity(ctx, s, plan.max_direct)),
+                            plan.max_depth, shadow_per_vertex);
   if (rc) return rc;
   Lane &ln = ctx->lanes[lane];
   if ((rc = stage_plan(ctx, ln, plan))) return rc;
@@ -1417,7 +1511,7 @@ int render_range(mpt_scene *s, const mpt_job *job, uint32_t pix_begin, uint32_t 
   if ((rc = resolve_focus(s, focused))) return rc;
   job = &focused;
   const uint32_t spp = job->spp;
-  const uint32_t batch_pix = std::max<uint32_t>(1, ctx->batch_paths / spp);
+  const uint32_t batch_pix = std::max<uint32_t>(1, batch_capacity(ctx, s, std::max(1u, job->direct_samples)) / spp);
   if ((rc = reset_totals(ctx))) return rc;
   MPT_HIP(ctx, hipEventRecord(ctx->ev0, ctx->stream));
   if ((rc = fork_lanes(ctx))) return rc;
@@ -1753,6 +1847,14 @@ struct mpt_kd_tree {
 extern "C" {
 
 int mpt_abi_version(void) { return (int)MPT_ABI_VERSION; }
+
+uint64_t mpt_batch_paths_for(uint64_t batch_paths, uint32_t num_lights, uint32_t num_points,
+                             uint32_t direct_samples) {
+  const uint64_t spv = (uint64_t)num_lights * std::max(1u, direct_samples) + num_points;
+  if (spv <= 1) return batch_paths;
+  const uint64_t fit = std::max<uint64_t>(kMinBatchPaths, kShadowRayBudget / spv);
+  return std::min<uint64_t>(batch_paths, fit);
+}
 
 int mpt_open(int device, mpt_context **out) {
   if (!out) return MPT_E_INVALID;

@@ -184,14 +184,15 @@ int round(mpt_server *sv, std::unique_lock<std::mutex> &lk, uint32_t *ran) {
   }
   mpt_context *ctx = sv->ctx;
   mpt_scene *scene = sv->active.front()->scene;
-  const uint32_t cap = ctx->batch_paths;
   BatchPlan plan;
   std::vector<std::pair<Job *, uint32_t>> parts;
   for (Job *j : sv->active) {
     if (j->scene != scene) continue;
     if (plan.jobs.size() == kMaxJobs || plan.chunks.size() == kMaxChunks) break;
     const uint32_t spp = j->job.spp;
-    const uint64_t room = (cap - plan.n_paths) / spp;
+    // a job with more light samples per vertex lowers the batch's capacity
+    const uint32_t cap = batch_capacity(ctx, scene, std::max(plan.max_direct, std::max(1u, j->job.direct_samples)));
+    const uint64_t room = plan.n_paths >= cap ? 0 : (cap - plan.n_paths) / spp;
     if (room == 0) {
       if (plan.n_paths == 0) {
         // a single pixel wider than the capacity: run it alone

@@ -151,3 +151,26 @@ def test_plugin_loads_like_milton():
     bad = subprocess.run([drv, os.path.join(ROOT, "milton_amd", "missing.so"), "--load-only"],
                          capture_output=True, text=True, timeout=60)
     assert bad.returncode == 1 and "failed to load" in bad.stderr
+
+
+def test_batch_capacity_keeps_shadow_rays_in_budget():
+    """A batch reserves one 32-byte shadow ray per light sample of each path;
+    the engine lowers the batch so that paths x shadow rays per vertex stays
+    within 2^28 (32-bit indices and 8 GiB per lane), never below 1024
+    paths, and never above the context's batch size."""
+    f = m.lib().mpt_batch_paths_for
+    assert f(1 << 25, 0, 0, 1) == 1 << 25          # no lights
+    assert f(1 << 25, 1, 0, 1) == 1 << 25          # one shadow ray per vertex
+    assert f(1 << 25, 0, 1, 64) == 1 << 25         # a point light draws one sample
+    assert f(1 << 25, 1, 0, 0) == 1 << 25          # 0 light samples means 1
+    for lights, points, direct in [(1, 0, 4), (1, 0, 64), (3, 1, 64), (2, 2, 64), (3, 2, 84)]:
+        spv = lights * direct + points
+        for bp in (1 << 20, 1 << 25, 1 << 28):
+            cap = f(bp, lights, points, direct)
+            assert 1024 <= cap <= bp
+            assert cap * spv <= 1 << 28 or cap == 1024
+            assert cap * spv < 1 << 32
+            if bp * spv <= 1 << 28:
+                assert cap == bp
+    # the largest shadow-ray count per vertex the engine accepts (254)
+    assert f(1 << 28, 254, 0, 1) * 254 <= 1 << 28
