@@ -8,9 +8,13 @@ in HBM before the timed region; the film stays on the device.
 
 Prints one JSON line: paths/s (whole job), Mrays/s, the roofline of the
 closest-hit kernel (algorithmic bytes per launch over its event-timed average
-duration, against 8 TB/s HBM) and a CPU baseline: the double-precision oracle
-restatement of the reference PathTracer on a bounded random sample of the same
-job's pixels, on the host's cores.
+duration when launches do not overlap, against 8 TB/s HBM), a CPU baseline
+(the double-precision oracle restatement of the reference PathTracer on a
+bounded random sample of the same job's pixels at 65,536 spp, on all of the
+process's host cores, which also gives the RMSE against an FP64 reference),
+and the systems workloads: `batched` (64 jobs over the room, the soup and the
+2048^2 terrain through one server, against the same jobs one at a time) and
+`serving` (Poisson arrivals, p50 / p99 latency).
 """
 import argparse
 import json
@@ -34,32 +38,49 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(arrays, job, seconds, threads):
-    """Oracle (double precision) on a random pixel sample of the same job."""
+def cpu_baseline(arrays, job, film_rgb, seconds, threads, nproc, max_pixels=256, ref_spp=65536):
+    """The oracle (the FP64 restatement of the reference PathTracer) on host
+    threads over a bounded sample of the same job: random pixels of its film
+    at ref_spp samples each, i.e. a 65,536-spp double-precision reference of
+    those pixels.  Pixels are added in groups of 16 until max_pixels or
+    `seconds` of render time.  Returns the timing (paths/s; paths/s does not
+    depend on spp) and the RMSE of the device image (film_rgb, the job's own
+    spp) against that reference at the sampled pixels."""
     from oracle import pyoracle as po
     t0 = time.perf_counter()
     osc = po.OracleScene(arrays, threads=threads)
     build_s = time.perf_counter() - t0
     rng = np.random.default_rng(12345)
     npix = job.width * job.height
-
-    def run(count):
-        pix = np.sort(rng.choice(npix, count, replace=False)).astype(np.uint32)
-        px = np.repeat(pix, job.spp)
-        sm = np.tile(np.arange(job.spp, dtype=np.uint32), count)
+    order = rng.choice(npix, max_pixels, replace=False).astype(np.uint32)
+    ref_job = job.__class__.from_buffer_copy(job)
+    ref_job.spp, ref_job.seed = ref_spp, job.seed + 7919
+    samples = np.arange(ref_spp, dtype=np.uint32)
+    means, done, render_s = [], 0, 0.0
+    while done < max_pixels and (done < 64 or render_s < seconds):
+        group = order[done:done + 16]
+        px = np.repeat(group, ref_spp)
+        sm = np.tile(samples, len(group))
         t = time.perf_counter()
-        osc.render_paths(job, px, sm, threads=threads)
-        return time.perf_counter() - t, count * job.spp
-
-    dt, paths = run(256)
-    rate = paths / dt
-    count = int(min(npix, max(256, rate * seconds / job.spp)))
-    dt, paths = run(count)
+        L = osc.render_paths(ref_job, px, sm, threads=threads)
+        render_s += time.perf_counter() - t
+        means.append(L.reshape(len(group), ref_spp, 3).mean(axis=1))
+        done += len(group)
     osc.close()
-    return {"value": paths / dt, "unit": "paths/s", "cores": threads, "kind": "port",
-            "sample": f"{count} random pixels x {job.spp} spp of the same job ({paths} paths, {dt:.1f} s); "
-                      f"oracle kd build {build_s:.1f} s excluded",
-            "cpu": cpu_model()}
+    ref = np.concatenate(means)
+    pix = order[:done]
+    img = film_rgb.reshape(-1, 3)[pix].astype(np.float64)
+    rmse = float(np.sqrt(np.mean((img - ref) ** 2)))
+    paths = done * ref_spp
+    log(f"cpu baseline: {paths} paths in {render_s:.1f}s on {threads} threads; fp64 rmse {rmse:.5f}")
+    base = {"value": paths / render_s, "unit": "paths/s", "cores": threads, "kind": "port",
+            "sample": f"{done} random pixels x {ref_spp} spp of the same camera, scene and depth "
+                      f"({paths} paths, {render_s:.1f} s on {threads} threads; oracle kd build {build_s:.1f} s "
+                      f"excluded)",
+            "nproc": nproc, "threads": threads, "cpu": cpu_model()}
+    rm = {"value": rmse, "reference": "fp64 oracle", "reference_spp": ref_spp, "pixels": int(done),
+          "mean_reference": float(ref.mean())}
+    return base, rm
 
 
 def cpu_model():
@@ -141,10 +162,14 @@ def engine_settings(ctx, m):
             "persistent": ctx.get_option(m.MPT_OPT_PERSISTENT), "lanes": ctx.get_option(m.MPT_OPT_LANES)}
 
 
-def system_scenes(m, ctx):
+def system_scenes(m, ctx, reuse=None):
+    """The three resident scenes of the batched and serving workloads;
+    `reuse` maps a scene kind to an (arrays, Scene) pair already built."""
     from milton_amd import workloads
-    scenes, build = {}, {}
+    scenes, build = dict(reuse or {}), {}
     for kind in workloads.SCENES:
+        if kind in scenes:
+            continue
         arrays = m.generate(kind, workloads.SCENE_COUNTS[kind], 1)
         t0 = time.perf_counter()
         scenes[kind] = (arrays, m.Scene(ctx, arrays))
@@ -153,29 +178,28 @@ def system_scenes(m, ctx):
     return scenes, build
 
 
-def bench_batched(m, ctx, args, scenes=None):
+def bench_batched(m, ctx, scenes):
     """64 mixed jobs three ways: one server packing all of them into shared
     batches (graph-replayed, over the context's lanes); each job alone
-    through the same engine (plain per-job); and each job alone on one
-    stream with kernels launched one by one (a classic wavefront renderer).
-    Every film must come out bit-identical, so the RMSE is equal by
-    construction."""
+    through the same engine (per-job); and each job alone on one stream with
+    kernels launched one by one (a classic wavefront renderer).  Every film
+    must come out bit-identical, so the RMSE is equal by construction."""
     from milton_amd import workloads
-    if scenes is None:
-        scenes, _ = system_scenes(m, ctx)
     cams = {k: v[0].camera for k, v in scenes.items()}
     jobs = workloads.batched_jobs(cams, 64, seed=11)
     total = sum(j.width * j.height * j.spp for _, j in jobs)
     for kind, (_, sc) in scenes.items():   # warm up graphs and caches per scene
-        j = m.make_job(cams[kind], 64, 64, 4, 8, seed=1)
-        sc.render(j)
+        sc.render(m.make_job(cams[kind], 64, 64, 4, 8, seed=1))
     films = [m.Film(ctx, j.width, j.height) for _, j in jobs]
+    job_s = np.zeros(len(jobs))
 
     def per_job():
         ctx.sync()
         t0 = time.perf_counter()
-        for (kind, j), f in zip(jobs, films):
+        for k, ((kind, j), f) in enumerate(zip(jobs, films)):
+            t1 = time.perf_counter()
             f.render(scenes[kind][1], j)
+            job_s[k] = time.perf_counter() - t1
         ctx.sync()
         return time.perf_counter() - t0
 
@@ -190,6 +214,7 @@ def bench_batched(m, ctx, args, scenes=None):
     per_job()
     t_plain = per_job()
     srv = m.Server(ctx)
+    ctx.sync()
     t0 = time.perf_counter()
     handles = [srv.submit(scenes[kind][1], j) for kind, j in jobs]
     srv.drain()
@@ -202,21 +227,29 @@ def bench_batched(m, ctx, args, scenes=None):
         same_classic += int(np.array_equal(got, c))
         f.close()
     srv.close()
-    log(f"batched: {len(jobs)} jobs, {total} paths, classic per-job {t_classic:.3f}s, plain per-job {t_plain:.3f}s, "
+    # where the per-job time goes: jobs under 4 M paths against the rest
+    paths = np.array([j.width * j.height * j.spp for _, j in jobs], np.float64)
+    small = paths < 4e6
+    log(f"batched: {len(jobs)} jobs, {total} paths, classic per-job {t_classic:.3f}s, per-job {t_plain:.3f}s, "
         f"batched {t_batched:.3f}s, {st['batches']} batches, identical films {same}/{len(jobs)} "
-        f"(vs classic {same_classic}/{len(jobs)})")
-    return {"paths": total, "t_classic": t_classic, "t_plain": t_plain, "t_batched": t_batched,
-            "batches": st["batches"], "identical": same, "identical_classic": same_classic, "jobs": len(jobs),
-            "extension_rays": st["extension_rays"], "shadow_rays": st["shadow_rays"]}
+        f"(vs classic {same_classic}/{len(jobs)}); small jobs {int(small.sum())} = "
+        f"{job_s[small].sum() / job_s.sum():.3f} of per-job time")
+    return {"jobs": len(jobs), "paths": total, "seconds": t_batched, "paths_per_s": total / t_batched,
+            "per_job_seconds": t_plain, "per_job_paths_per_s": total / t_plain,
+            "classic_seconds": t_classic, "classic_paths_per_s": total / t_classic,
+            "speedup_vs_per_job": t_plain / t_batched, "speedup_vs_classic": t_classic / t_batched,
+            "identical_films": same, "identical_films_classic": same_classic, "batches": st["batches"],
+            "extension_rays": st["extension_rays"], "shadow_rays": st["shadow_rays"],
+            "small_jobs": int(small.sum()), "small_job_share_of_per_job_time": float(job_s[small].sum() / job_s.sum())}
 
 
-def bench_serving(m, ctx, args, scenes, capacity_jobs_s):
-    """Poisson arrivals at 60% of the measured batch capacity; latency is
-    submit -> film ready (host clock of the server)."""
+def bench_serving(m, ctx, scenes, rate, seconds):
+    """Poisson arrivals of jobs drawn as in the batched mix at `rate` jobs/s
+    for `seconds`; latency is submit -> film ready (host clock of the
+    server)."""
     from milton_amd import workloads
     cams = {k: v[0].camera for k, v in scenes.items()}
-    rate = 0.6 * capacity_jobs_s
-    arrivals = workloads.poisson_arrivals(rate, args.serve_seconds, seed=13)
+    arrivals = workloads.poisson_arrivals(rate, seconds, seed=13)
     rng = np.random.default_rng(13)
     jobs = [workloads.draw_job(rng, cams, 1000 + i) for i in range(len(arrivals))]
     srv = m.Server(ctx)
@@ -239,9 +272,26 @@ def bench_serving(m, ctx, args, scenes, capacity_jobs_s):
     srv.stop()
     srv.close()
     lat = np.array(lat)
-    return {"rate_jobs_s": rate, "jobs": len(jobs), "seconds": t_end, "paths_per_s": paths / t_end,
-            "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
-            "max_ms": float(lat.max())}
+    out = {"rate_jobs_s": rate, "jobs": len(jobs), "seconds": t_end, "paths": paths, "paths_per_s": paths / t_end,
+           "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
+           "max_ms": float(lat.max())}
+    log(f"serving: {out}")
+    return out
+
+
+def systems(m, ctx, args, reuse=None):
+    """The batched and serving workloads on the three resident scenes."""
+    scenes, build = system_scenes(m, ctx, reuse)
+    out = {"scene_build_s": build}
+    out["batched"] = bench_batched(m, ctx, scenes)
+    if args.serve_seconds > 0:
+        # Poisson arrivals at 60% of the job rate the batched run sustained
+        out["serving"] = bench_serving(m, ctx, scenes, 0.6 * out["batched"]["jobs"] / out["batched"]["seconds"],
+                                       args.serve_seconds)
+    for kind, (_, sc) in scenes.items():
+        if not reuse or kind not in reuse:
+            sc.close()
+    return out
 
 
 def apply_engine_args(m, ctx, args):
@@ -256,34 +306,46 @@ def apply_engine_args(m, ctx, args):
 
 
 def main_systems(args):
+    """--workload batched / serving: only the systems workloads."""
     import milton_amd as m
     ctx = m.Context(0)
     apply_engine_args(m, ctx, args)
-    scenes, build = system_scenes(m, ctx)
-    b = bench_batched(m, ctx, args, scenes)
-    out = {"metric": "paths/s", "value": b["paths"] / b["t_batched"], "unit": "paths/s", "n_gpus": 1,
-           "steps": 1, "warmup": 1, "ms_per_step": 1000 * b["t_batched"], "higher_is_better": True,
+    if args.workload == "batched":
+        args.serve_seconds = 0
+    sy = systems(m, ctx, args)
+    b = sy["batched"]
+    out = {"metric": "paths/s", "value": b["paths_per_s"], "unit": "paths/s", "n_gpus": 1,
+           "steps": 1, "warmup": 1, "ms_per_step": 1000 * b["seconds"], "higher_is_better": True,
            "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
            "config": {"workload": "batched: 64 jobs over room / 1M soup / 2048^2 terrain, sides 128-1024, "
-                                  "spp 4-256, depth 4-8", "scene_build_s": build,
-                      "engine": engine_settings(ctx, m)},
-           "batched": b, "speedup_vs_plain": b["t_plain"] / b["t_batched"],
-           "speedup_vs_classic": b["t_classic"] / b["t_batched"],
-           "plain_paths_per_s": b["paths"] / b["t_plain"], "classic_paths_per_s": b["paths"] / b["t_classic"]}
-    if args.workload == "serving":
-        out["serving"] = bench_serving(m, ctx, args, scenes, b["jobs"] / b["t_batched"])
+                                  "spp 4-256, depth 4-8", "scene_build_s": sy["scene_build_s"],
+                      "engine": engine_settings(ctx, m)}}
+    out.update({k: v for k, v in sy.items() if k != "scene_build_s"})
     ctx.close()
     print(json.dumps(out), flush=True)
+
+
+def timed_render(m, ctx, film, scene, job, lanes):
+    """One render with per-launch HIP event timing on `lanes` lanes."""
+    saved = ctx.get_option(m.MPT_OPT_LANES)
+    ctx.set_option(m.MPT_OPT_LANES, lanes)
+    ctx.set_option(m.MPT_OPT_TIMING, 1)
+    try:
+        return film.render(scene, job)
+    finally:
+        ctx.set_option(m.MPT_OPT_TIMING, 0)
+        ctx.set_option(m.MPT_OPT_LANES, saved)
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
-    ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--cpu-seconds", type=float, default=15.0)
-    ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--no-rmse", action="store_true", help="skip the RMSE against a 65,536-spp reference")
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--cpu-seconds", type=float, default=20.0)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline (and the FP64 RMSE it yields)")
+    ap.add_argument("--no-rmse", action="store_true", help="skip the RMSE against a 65,536-spp device reference")
+    ap.add_argument("--no-systems", action="store_true", help="skip the batched and serving workloads")
     ap.add_argument("--batch-paths", type=int, default=0)
     ap.add_argument("--stack", type=int, default=0, help="traversal short-stack rows (8, 12, 16, 24)")
     ap.add_argument("--persistent", type=int, default=-1,
@@ -296,8 +358,9 @@ def main():
     ap.add_argument("--tune", action="store_true",
                     help="autotune (kd leaf size and engine settings) on a sample of the job before timing")
     ap.add_argument("--workload", default="soup", choices=["soup", "batched", "serving"],
-                    help="soup: the headline kd-tree stress scene; batched / serving: the systems workloads")
-    ap.add_argument("--serve-seconds", type=float, default=20.0)
+                    help="soup: the headline kd-tree stress scene (plus the systems workloads unless "
+                         "--no-systems); batched / serving: only the systems workloads")
+    ap.add_argument("--serve-seconds", type=float, default=10.0)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a 256 MiB device-to-device copy (reference dispatch for HBM counters)")
     args = ap.parse_args()
@@ -343,33 +406,39 @@ def main():
     alg_bytes = (NODE_BYTES * cst["nodes_visited"] + RECORD_BYTES * cst["prims_tested"]
                  + (RAY_BYTES + HIT_BYTES) * ext_rays)
 
+    # Timed steps: the production configuration (graph-replayed batches over
+    # the context's lanes, no per-launch instrumentation).
     for _ in range(args.warmup):
         film.render(scene, job)
-    ctx.set_option(m.MPT_OPT_TIMING, 1)
     ctx.sync()
     t0 = time.perf_counter()
     runs = [film.render(scene, job) for _ in range(args.steps)]
     ctx.sync()
     elapsed = time.perf_counter() - t0
-    ctx.set_option(m.MPT_OPT_TIMING, 0)
-    for r in runs:
+    for r in runs[:3]:
         log(f"step: {r}")
     paths = WIDTH * HEIGHT * SPP
     rays = runs[0]["extension_rays"] + runs[0]["shadow_rays"]
     assert runs[0]["extension_rays"] == ext_rays
     ms_step = 1000.0 * elapsed / args.steps
-    extend_ms = float(np.mean([r["extend_ms"] for r in runs]))
-    extend_wall = float(np.mean([r["extend_wall_ms"] for r in runs]))
-    launches = runs[0]["extend_launches"]
-    # Algorithmic bytes per launch over the average launch duration (the
-    # duration rocprof reports per dispatch).  Launches on different lanes
-    # overlap, so each one shares the GPU; the kernel's throughput over the
-    # time during which at least one launch ran is given beside it.
-    achieved_busy = alg_bytes / (extend_wall * 1e-3) / 1e9
+    film_rgb = film.download()
+
+    # Roofline of the closest-hit kernel: algorithmic bytes of one launch over
+    # its duration measured alone (one lane, so no other batch's kernels
+    # share the device), HIP events on the lane's stream.  Beside it, all
+    # launches' bytes over the time during which at least one ran in the
+    # production configuration.
+    excl = [timed_render(m, ctx, film, scene, job, 1) for _ in range(2)][-1]
+    busy = timed_render(m, ctx, film, scene, job, ctx.get_option(m.MPT_OPT_LANES))
+    log(f"timed one lane: {excl}")
+    log(f"timed all lanes: {busy}")
+    launches = excl["extend_launches"]
+    avg_launch_ms = excl["extend_ms"] / launches
+    achieved = (alg_bytes / launches) / (avg_launch_ms * 1e-3) / 1e9
+    achieved_busy = alg_bytes / (busy["extend_wall_ms"] * 1e-3) / 1e9
     engine = engine_settings(ctx, m)
     default_tree = not (args.tune or args.kd_ci or args.kd_eb or args.no_clip)
     traffic = measured_traffic(engine, args.kd_leaf or 4) if default_tree else None
-    achieved = (alg_bytes / launches) / (extend_ms / launches * 1e-3) / 1e9
     out = {
         "metric": "paths/s",
         "value": paths * args.steps / elapsed,
@@ -392,19 +461,25 @@ def main():
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "traffic_source": os.path.relpath(TRAFFIC_PROFILE, ROOT) if traffic else None,
                      "kernel": "k_extend", "alg_bytes_per_launch": alg_bytes / launches,
-                     "avg_launch_ms": extend_ms / launches, "launches_per_step": launches,
-                     "busy_ms_per_step": extend_wall, "achieved_busy": achieved_busy,
+                     "avg_launch_ms": avg_launch_ms, "launch_timing": "one lane (launches do not overlap)",
+                     "launches_per_step": launches, "busy_ms_per_step": busy["extend_wall_ms"],
+                     "achieved_busy": achieved_busy, "frac_busy": achieved_busy / HBM_PEAK_GBS,
                      "nodes_per_ray": cst["nodes_visited"] / ext_rays,
-                     "refs_per_ray": cst["prims_tested"] / ext_rays},
+                     "refs_per_ray": cst["prims_tested"] / ext_rays,
+                     "shade_ms_per_step_one_lane": excl["shade_ms"]},
     }
     if not args.no_rmse:
         out["rmse"] = rmse_vs_reference(m, ctx, scene, job)
     if not args.no_cpu:
-        threads = min(16, os.cpu_count() or 1)
-        out["cpu_baseline"] = cpu_baseline(arrays, job, args.cpu_seconds, threads)
+        nproc = os.cpu_count() or 1
+        threads = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else nproc
+        out["cpu_baseline"], out["rmse_fp64"] = cpu_baseline(arrays, job, film_rgb, args.cpu_seconds, threads,
+                                                             nproc)
     else:
         out["cpu_baseline"] = None
     film.close()
+    if not args.no_systems:
+        out.update(systems(m, ctx, args, reuse={m.MPT_SCENE_SOUP: (arrays, scene)}))
     scene.close()
     ctx.close()
     print(json.dumps(out), flush=True)
