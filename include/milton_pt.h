@@ -285,6 +285,13 @@ int mpt_kd_destroy(mpt_kd_tree *tree);
  * (core/Scene.cpp:104-108) -> kdTreeAccel::getIntersection
  * (accel/kdTreeAccel.cpp:893-1016).  Hits with t <= 1e-7 are ignored. */
 int mpt_trace_closest(mpt_scene *scene, const mpt_ray *rays, uint64_t n, mpt_hit *hits);
+/* The same closest hits traced by the render's own closest-hit kernel (the
+ * context's kernel form and short-stack depth) with its work counters:
+ * counters = [kd nodes fetched, leaf references tested] summed over the rays,
+ * the per-ray counts behind the algorithmic bytes of the closest-hit kernel;
+ * mpt_kd_trace counts the same walk on the host.  At most 2^31 rays. */
+int mpt_trace_closest_stats(mpt_scene *scene, const mpt_ray *rays, uint64_t n, mpt_hit *hits,
+                            uint64_t counters[2]);
 /* Any hit with 1e-7 < t < tmax[i].  Replaces Scene::intersects
  * (core/Scene.cpp:110-114) -> kdTreeAccel::intersects (:1018-1094). */
 int mpt_trace_any(mpt_scene *scene, const mpt_ray *rays, const float *tmax, uint64_t n,

@@ -164,6 +164,7 @@ def lib():
         "mpt_scene_stats": ([vp, ctypes.POINTER(ctypes.c_uint64)], i32),
         "mpt_trace_closest": ([vp, vp, u64, vp], i32),
         "mpt_trace_any": ([vp, vp, vp, u64, vp], i32),
+        "mpt_trace_closest_stats": ([vp, vp, u64, vp, ctypes.POINTER(ctypes.c_uint64)], i32),
         "mpt_render": ([vp, ctypes.POINTER(Job), vp, ctypes.POINTER(RenderStats)], i32),
         "mpt_film_create": ([vp, u32, u32, ctypes.POINTER(vp)], i32),
         "mpt_film_destroy": ([vp], i32),
@@ -496,6 +497,17 @@ class Scene:
         self.ctx.check(self.ctx.L.mpt_trace_closest(self.h, _ptr(rays), len(rays), _ptr(hits)),
                        "mpt_trace_closest")
         return hits
+
+    def trace_closest_stats(self, rays):
+        """Closest hits through the render's closest-hit kernel with its work
+        counters: (hits, {"nodes": kd nodes fetched, "refs": leaf references
+        tested})."""
+        rays = np.ascontiguousarray(rays, RAY_DTYPE)
+        hits = np.zeros(len(rays), HIT_DTYPE)
+        ctr = (ctypes.c_uint64 * 2)()
+        self.ctx.check(self.ctx.L.mpt_trace_closest_stats(self.h, _ptr(rays), len(rays), _ptr(hits), ctr),
+                       "mpt_trace_closest_stats")
+        return hits, {"nodes": ctr[0], "refs": ctr[1]}
 
     def trace_any(self, rays, tmax):
         rays = np.ascontiguousarray(rays, RAY_DTYPE)

@@ -901,6 +901,39 @@ __global__ void k_totals(const uint32_t *__restrict__ counts, uint32_t depth,
   totals[1] += ext;
 }
 
+// Host rays as closest-hit queue entries (origin, -) (direction, no origin
+// primitive), so the render's own closest-hit kernels can trace them.
+__global__ __launch_bounds__(256) void k_rays_to_queue(const mpt_ray *__restrict__ rays, uint32_t n,
+                                                       float4 *__restrict__ q) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const mpt_ray r = rays[i];
+  q[3ull * i] = make_float4(r.o[0], r.o[1], r.o[2], __uint_as_float(i));
+  q[3ull * i + 1] = make_float4(r.d[0], r.d[1], r.d[2], __uint_as_float(kMiss));
+}
+
+// Raw closest hits (FP32 t, leaf record) as (t recomputed in double,
+// primitive id), as the shading kernel resolves them.
+__global__ __launch_bounds__(256) void k_resolve_hits(Scene sc, const mpt_ray *__restrict__ rays,
+                                                      const float2 *__restrict__ raw, uint32_t n,
+                                                      mpt_hit *__restrict__ out) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float2 h = raw[i];
+  const uint32_t rec = __float_as_uint(h.y);
+  mpt_hit o;
+  if (rec == kMiss) {
+    o.t = INFINITY;
+    o.prim = kMiss;
+  } else {
+    const float4 *rp = sc.recs + 3ull * rec;
+    const mpt_ray r = rays[i];
+    o.t = refine_hit(rp, f3(r.o[0], r.o[1], r.o[2]), f3(r.d[0], r.d[1], r.d[2]), h.x);
+    o.prim = __float_as_uint(rp[0].w);
+  }
+  out[i] = o;
+}
+
 template <bool ANY>
 __global__ __launch_bounds__(256) void k_trace_rays(Scene sc, const mpt_ray *__restrict__ rays,
                                                     const float *__restrict__ tmax, uint64_t n,
@@ -2114,6 +2147,43 @@ int mpt_trace_closest(mpt_scene *s, const mpt_ray *rays, uint64_t n, mpt_hit *hi
   MPT_LAUNCHED(ctx, "k_trace_rays");
   MPT_HIP(ctx, hipMemcpyAsync(hits, dh.p, n * sizeof(mpt_hit), hipMemcpyDeviceToHost, ctx->stream));
   MPT_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return MPT_OK;
+}
+
+int mpt_trace_closest_stats(mpt_scene *s, const mpt_ray *rays, uint64_t n, mpt_hit *hits, uint64_t counters[2]) {
+  if (!s || !counters || (n && (!rays || !hits))) return fail(s ? s->ctx : nullptr, MPT_E_INVALID, "NULL argument");
+  mpt_context *ctx = s->ctx;
+  counters[0] = counters[1] = 0;
+  if (!n) return MPT_OK;
+  if (n > (1ull << 31)) return fail(ctx, MPT_E_INVALID, "at most 2^31 rays per call");
+  const uint32_t n32 = (uint32_t)n;
+  DBuf<mpt_ray> dr;
+  DBuf<float4> q;
+  DBuf<float2> raw;
+  DBuf<mpt_hit> dh;
+  DBuf<uint32_t> cnt;   // [queue size, claim cursor]
+  DBuf<unsigned long long> ctr;
+  MPT_HIP(ctx, dr.reserve(n));
+  MPT_HIP(ctx, q.reserve(3ull * n));
+  MPT_HIP(ctx, raw.reserve(n));
+  MPT_HIP(ctx, dh.reserve(n));
+  MPT_HIP(ctx, cnt.reserve(2));
+  MPT_HIP(ctx, ctr.reserve(2));
+  const uint32_t init[2] = {n32, 0u};
+  MPT_HIP(ctx, hipMemcpyAsync(dr.p, rays, n * sizeof(mpt_ray), hipMemcpyHostToDevice, ctx->stream));
+  MPT_HIP(ctx, hipMemcpyAsync(cnt.p, init, sizeof(init), hipMemcpyHostToDevice, ctx->stream));
+  MPT_HIP(ctx, hipMemsetAsync(ctr.p, 0, 2 * sizeof(unsigned long long), ctx->stream));
+  const uint32_t blocks = (n32 + 255) / 256;
+  k_rays_to_queue<<<blocks, 256, 0, ctx->stream>>>(dr.p, n32, q.p);
+  launch_extend(ctx, ctx->stream, n32, grid_for(ctx, n, 8), s->dev, q.p, cnt.p, raw.p, ctr.p, cnt.p + 1);
+  k_resolve_hits<<<blocks, 256, 0, ctx->stream>>>(s->dev, dr.p, raw.p, n32, dh.p);
+  MPT_LAUNCHED(ctx, "k_trace closest with counters");
+  unsigned long long c[2];
+  MPT_HIP(ctx, hipMemcpyAsync(hits, dh.p, n * sizeof(mpt_hit), hipMemcpyDeviceToHost, ctx->stream));
+  MPT_HIP(ctx, hipMemcpyAsync(c, ctr.p, sizeof(c), hipMemcpyDeviceToHost, ctx->stream));
+  MPT_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  counters[0] = c[0];
+  counters[1] = c[1];
   return MPT_OK;
 }
 

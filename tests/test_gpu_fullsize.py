@@ -56,4 +56,4 @@ def test_terrain_full_job(ctx):
     arrays = m.generate(m.MPT_SCENE_TERRAIN, 2048, 1)
     job = m.make_job(dict(arrays.camera), 1024, 1024, 1024, 8, seed=1)
     runs = [(r * 1024 + c, r * 1024 + c + 12) for r, c in ((100, 40), (600, 700))]
-    _full_job(ctx, arrays, job, runs, 0.97, "terrain 2048^2 1024x1024x1024")
+    _full_job(ctx, arrays, job, runs, 0.999, "terrain 2048^2 1024x1024x1024")

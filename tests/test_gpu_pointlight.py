@@ -28,4 +28,4 @@ def test_room_with_point_light_paths(ctx):
     assert text != ROOM
     arrays, job = m.load_scenefile(text=text)
     assert len(arrays.points) == 1
-    _per_path(ctx, arrays, job, 0, job.width * job.height, "scenefile room + point light", min_frac=0.98)
+    _per_path(ctx, arrays, job, 0, job.width * job.height, "scenefile room + point light", min_frac=0.999)

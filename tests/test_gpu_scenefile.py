@@ -19,7 +19,7 @@ def test_scenefile_room_paths_same_seed(ctx):
     arrays, job = m.load_scenefile(text=ROOM)
     print("scenefile room:", len(arrays.indices), "one-sided triangles,", len(arrays.quads), "quad,",
           len(arrays.spheres), "sphere", flush=True)
-    _per_path(ctx, arrays, job, 0, job.width * job.height, "scenefile-room-paths", min_frac=0.98)
+    _per_path(ctx, arrays, job, 0, job.width * job.height, "scenefile-room-paths", min_frac=0.999)
 
 
 def test_scenefile_room_image_three_sigma(ctx):
@@ -62,7 +62,7 @@ def test_dispersive_glass_paths_same_seed(ctx):
     arrays, job = m.load_scenefile(text=ROOM.replace('"medium" : { "ior" : 1.5 }',
                                                      '"medium" : { "ior" : [1.4, 1.55, 1.8] }'))
     assert arrays.materials[arrays.sphere_material[0]]["ior_b"] > 0
-    _per_path(ctx, arrays, job, 0, job.width * job.height, "scenefile-room-dispersive", min_frac=0.98)
+    _per_path(ctx, arrays, job, 0, job.width * job.height, "scenefile-room-dispersive", min_frac=0.999)
 
 
 def test_cube_paths_same_seed(ctx):

@@ -12,7 +12,7 @@ import pytest
 
 import milton_amd as m
 from oracle import pyoracle as po
-from conftest import classify_hits, rays64
+from conftest import classify_hits, explain_mismatches, rays64
 
 pytestmark = pytest.mark.gpu
 
@@ -26,11 +26,15 @@ def _check_closest(ctx, arrays, rays, label):
     orc_t, orc_id = osc.trace_closest(rays64(rays))
     match, non_tie, rel = classify_hits(osc, rays, hits["t"], hits["prim"], orc_t, orc_id)
     frac_t = (rel <= T_REL).mean() if len(rel) else 1.0
-    print(f"[{label}] rays={len(rays)} id_match={match:.6f} non_tie={non_tie:.2e} "
-          f"t_within_1e-5={frac_t:.6f} t_rel_max={rel.max() if len(rel) else 0:.2e} "
+    counts, left = explain_mismatches(arrays, osc, rays, hits["t"], hits["prim"], orc_t, orc_id)
+    print(f"[{label}] rays={len(rays)} id_match={match:.7f} non_tie={non_tie:.2e} mismatches={counts} "
+          f"unexplained={len(left)} t_within_1e-5={frac_t:.7f} t_rel_max={rel.max() if len(rel) else 0:.2e} "
           f"hit_frac={(orc_id != 0xFFFFFFFF).mean():.3f}", flush=True)
-    assert non_tie <= 1e-4
-    assert frac_t >= 0.99999
+    for i in left[:10]:
+        print(f"  unexplained ray {i}: o={rays['o'][i]} d={rays['d'][i]} gpu=({hits['t'][i]}, {hits['prim'][i]}) "
+              f"oracle=({orc_t[i]}, {orc_id[i]})", flush=True)
+    assert not left, f"{len(left)} disagreements outside the allowed windows"
+    assert frac_t == 1.0
     sc.close()
     osc.close()
     return hits, orc_t, orc_id
@@ -124,3 +128,36 @@ def test_terrain_closest(ctx):
     hits, orc_t, orc_id = _check_closest(ctx, arrays, rays, "terrain")
     nt = len(arrays.indices)
     assert ((orc_id >= nt) & (orc_id != 0xFFFFFFFF)).sum() > 1000   # spheres are hit
+
+
+def test_device_counters_match_host_replay(ctx):
+    """The closest-hit kernel's work counters, on which the algorithmic bytes
+    of its roofline rest (8 B per kd node fetched, 48 B per leaf reference
+    tested), equal a host replay of the same walk over the same tree
+    (mpt_kd_trace) to 0.1%, on 2,000,000 seeded rays of the 1M-triangle
+    scene."""
+    arrays = m.generate(m.MPT_SCENE_SOUP, 1_000_000, 1)
+    rays = m.generate_rays(m.MPT_SCENE_SOUP, 1_000_000, 2_000_000, seed=9)
+    saved = ctx.get_option(m.MPT_OPT_STACK)
+    ctx.set_option(m.MPT_OPT_STACK, 8)   # the host replay's short stack
+    try:
+        sc = m.Scene(ctx, arrays)
+        hits, dev = sc.trace_closest_stats(rays)
+    finally:
+        ctx.set_option(m.MPT_OPT_STACK, saved)
+    hk = m.HostKdTree(arrays)
+    hh, host = hk.trace_closest(rays)
+    st, hst = sc.stats(), hk.stats()
+    assert (st["nodes"], st["refs"]) == (hst["nodes"], hst["refs"])   # the same tree
+    same = (hits["prim"] == hh["prim"]).mean()
+    print(f"[counters] device nodes={dev['nodes']} refs={dev['refs']} host nodes={host['nodes']} "
+          f"refs={host['refs']} per ray {dev['nodes'] / len(rays):.3f} / {dev['refs'] / len(rays):.3f} "
+          f"same prim {same:.7f}", flush=True)
+    assert abs(dev["nodes"] - host["nodes"]) <= 1e-3 * host["nodes"]
+    assert abs(dev["refs"] - host["refs"]) <= 1e-3 * host["refs"]
+    assert same >= 0.9999
+    # the counted kernel returns the same hits as the plain query
+    plain = sc.trace_closest(rays)
+    assert (plain["prim"] == hits["prim"]).mean() >= 0.99999
+    sc.close()
+    hk.close()
