@@ -41,7 +41,7 @@
 extern "C" {
 #endif
 
-#define MPT_ABI_VERSION 4u
+#define MPT_ABI_VERSION 5u
 #define MPT_MISS 0xFFFFFFFFu
 
 enum {
@@ -198,6 +198,12 @@ typedef struct mpt_job {
   uint32_t filter;
   uint32_t filter_support; /* the filter's "support", 0 = 2 */
   float filter_params[2];  /* gaussian sigma (0 = 1); mitchell B, C (both 0 = 1/3); lanczosSinc tau (0 = 3) */
+  /* 1: the renderer's efficientDirect = false (renderers/renderers/
+   * PathTracer.cpp:23, 50-52, 70-73): emitted radiance is counted at every
+   * vertex and next-event estimation is off.  0 (the reference default,
+   * efficientDirect = true): emission only at depth 0 or after a reflective
+   * specular bounce, next-event estimation at every non-specular vertex. */
+  uint32_t direct_off;
 } mpt_job;
 
 typedef struct mpt_ray {

@@ -24,13 +24,21 @@
 
 #include "milton_pt.h"
 
-extern "C" {
 /* MILTON_PLUGIN_TYPE_RENDERER (dynamic/DynamicPlugin.h:39-50). */
 enum { MPT_PLUGIN_TYPE_RENDERER = 0 };
+
+/* The plugin library's entry points.  A Milton-side adapter that defines its
+ * own plugin_type / plugin_new / plugin_delete (with Milton's
+ * MILTON_PLUGIN_TYPE return type) defines MILTON_PT_PLUGIN_NO_ENTRY_POINTS
+ * before including this header and constructs milton_amd::PathTracer
+ * directly. */
+#ifndef MILTON_PT_PLUGIN_NO_ENTRY_POINTS
+extern "C" {
 int plugin_type(void);
 void *plugin_new(void);
 void plugin_delete(void *plugin);
 }
+#endif
 
 namespace milton_amd {
 
@@ -42,8 +50,8 @@ class PathTracer {
   /* PropertyMap keys read on the path (numbers only):
    *   renderer  "noSuperSamples" (spp, SuperSampleGenerator.cpp:26-28),
    *             "maxDepth" (the cap), "seed" (-seed=, MiltonApp.cpp:329),
-   *             "efficientDirect" (must be 1, PathTracer.cpp:23),
-   *             "noDirectSamples" (must be 1, DirectIllumination.cpp:33),
+   *             "efficientDirect" (PathTracer.cpp:23),
+   *             "noDirectSamples" (1..64, DirectIllumination.cpp:33),
    *             "device" (HIP device), "batchPaths" (queue depth)
    *   output    "width" / "height" (RenderOutput.cpp:34)
    *   kd-tree   "kdMinPrimitives" (leaf size), "kdMaxDepth", "kdCostTraversal",

@@ -45,6 +45,8 @@ struct JobDev {
   uint32_t filter;                       // 0: pixel bin; else 1 + MPT_FILTER_*
   uint32_t filter_support;
   float filter_p0, filter_p1;
+  uint32_t direct_off;   // efficientDirect = false: emission at every vertex, no NEE
+  uint32_t pad[3];
 };
 
 struct ChunkDev {

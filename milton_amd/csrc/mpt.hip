@@ -488,6 +488,7 @@ __global__ __launch_bounds__(kShadeBlock) __attribute__((amdgpu_waves_per_eu(kMu
     float3 d = f3(0, 0, 1);
     uint32_t pixel = 0, sample = 0, seed = 0, job_id = 0, max_depth = 0;
     uint32_t ndir_job = 1;   // light samples per light
+    bool direct_off = false; // efficientDirect = false
     if (active) {
       // (origin, slot) (direction, origin primitive) (throughput, flags)
       const float4 a = qin[3ull * i], b = qin[3ull * i + 1], c = qin[3ull * i + 2];
@@ -502,6 +503,7 @@ __global__ __launch_bounds__(kShadeBlock) __attribute__((amdgpu_waves_per_eu(kMu
       job_id = job.job_id;
       max_depth = job.max_depth;
       if (kMulti) ndir_job = job.direct;
+      direct_off = job.direct_off != 0;
       beta = xyz(c);
       fl = __float_as_uint(c.w);
       const uint32_t rec = __float_as_uint(h.y);
@@ -529,11 +531,12 @@ __global__ __launch_bounds__(kShadeBlock) __attribute__((amdgpu_waves_per_eu(kMu
     if (surface) {
       if (specular) fl |= F_ANY_SPECULAR;
       // PathTracer.cpp:50-69: emission at depth 0, or after a specular vertex
-      // that sent the ray out on its normal's side (Le * area / t^2).
+      // that sent the ray out on its normal's side (Le * area / t^2); with
+      // efficientDirect = false at every vertex (:51).
       if (emissive(m) && dot(hi.n, d) < 0.0f) {
         const float3 le = le_of(m);
         float w = 0.0f;
-        if (depth == 0) {
+        if (depth == 0 || direct_off) {
           w = 1.0f;
         } else if (fl & F_EMIT_AFTER_SPEC) {
           w = hi.area / (thit * thit);
@@ -563,7 +566,7 @@ __global__ __launch_bounds__(kShadeBlock) __attribute__((amdgpu_waves_per_eu(kMu
     // shadow buffer in (light, sample) order; one group entry per vertex
     // points k_shadow at them.
     uint32_t n_sh = 0;
-    if (surface && !specular && spv) {
+    if (surface && !specular && spv && !direct_off) {
       float4 *run = sr + 2ull * ((uint64_t)slot * spv);
       uint32_t side = 1;   // floor(sqrt(n)) strata per side
       if (kMulti)
@@ -1270,6 +1273,7 @@ JobDev make_job(const mpt_job &job, float4 *film) {
   j.filter_support = job.filter_support;
   j.filter_p0 = job.filter_params[0];
   j.filter_p1 = job.filter_params[1];
+  j.direct_off = job.direct_off ? 1u : 0u;
   return j;
 }
 
@@ -1281,6 +1285,7 @@ int check_job(mpt_context *ctx, const mpt_job *job) {
   if ((uint64_t)job->width * job->height > (1ull << 30)) return fail(ctx, MPT_E_INVALID, "film too large");
   if (job->spp >= (1u << kJobBits)) return fail(ctx, MPT_E_INVALID, "spp at or above 2^20");
   if (job->direct_samples > kMaxDirect) return fail(ctx, MPT_E_INVALID, "direct_samples above 64");
+  if (job->direct_off > 1) return fail(ctx, MPT_E_INVALID, "direct_off must be 0 or 1");
   const float *l = job->camera.look, *u = job->camera.up;
   const double ll = l[0] * l[0] + l[1] * l[1] + l[2] * l[2];
   const double cx = u[1] * l[2] - u[2] * l[1], cy = u[2] * l[0] - u[0] * l[2], cz = u[0] * l[1] - u[1] * l[0];

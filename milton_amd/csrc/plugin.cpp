@@ -103,10 +103,6 @@ std::string PathTracer::lastError() const {
 
 int PathTracer::init() {
   err_.clear();
-  if (getValue("efficientDirect", 1) == 0) {
-    err_ = "efficientDirect = false is not supported by the device path";
-    return MPT_E_INVALID;
-  }
   const double nds = getValue("noDirectSamples", 1);
   if (nds < 1 || nds > 64) {
     err_ = "noDirectSamples must be in 1..64";
@@ -154,6 +150,7 @@ int PathTracer::render() {
   job.max_depth = (uint32_t)getValue("maxDepth", 8);
   job.seed = (uint32_t)getValue("seed", 1);
   job.direct_samples = (uint32_t)getValue("noDirectSamples", 1);
+  job.direct_off = getValue("efficientDirect", 1) == 0 ? 1u : 0u;   // PathTracer.cpp:23
   film_.assign(3ull * job.width * job.height, 0.0f);
   return mpt_render(scene_, &job, film_.data(), &stats_);
 }

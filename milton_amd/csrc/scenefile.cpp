@@ -31,7 +31,8 @@
 //   output             width, height, size, dimensions (default 480 x 480),
 //                      tonemap (:783; "default" = Reinhard), type and filter
 //                      (box, triangle, gaussian, mitchell, lanczosSinc)
-//   renderer           noSuperSamples (spp), noDirectSamples, seed, maxDepth
+//   renderer           noSuperSamples (spp), noDirectSamples, seed, maxDepth,
+//                      efficientDirect
 //                      (a cap the reference does not have; default 8)
 //   point              with an "omni" emitter: a point light (PointShape,
 //                      OmniEmitter)
@@ -292,6 +293,7 @@ struct Settings {
   double aperture = 0, focal_length = 0, focal_distance = 1, focal_point[2] = {0.5, 0.5};
   bool autofocus = false;
   uint32_t width = 480, height = 480, spp = 1, max_depth = 8, seed = 1, direct = 1;
+  uint32_t direct_off = 0;   // the renderer's efficientDirect = false
   uint32_t tonemap = MPT_TONEMAP_REINHARD;   // RenderOutput::init's "default"
   uint32_t filter = 0, filter_support = 0;    // mpt_job.filter: 0 pixel bin, else 1 + MPT_FILTER_*
   float filter_params[2] = {0, 0};
@@ -360,6 +362,13 @@ class Loader {
     if (num(v.find("maxDepth"), x)) set_.max_depth = (uint32_t)std::max(1.0, x);
     if (num(v.find("seed"), x)) set_.seed = (uint32_t)x;
     if (num(v.find("noDirectSamples"), x)) set_.direct = (uint32_t)std::max(1.0, x);
+    // PathTracer::init (renderers/renderers/PathTracer.cpp:22-26)
+    const JVal *ed = v.find("efficientDirect");
+    if (ed) {
+      if (ed->kind == JVal::BOOL) set_.direct_off = ed->b ? 0u : 1u;
+      else if (ed->kind == JVal::NUM) set_.direct_off = ed->num == 0 ? 1u : 0u;
+      else return bad("renderer.efficientDirect must be a boolean");
+    }
     return true;
   }
 
@@ -1125,6 +1134,7 @@ int mpt_scenefile_job(const mpt_scenefile *sf, mpt_job *job) {
   job->max_depth = s.max_depth;
   job->seed = s.seed;
   job->direct_samples = s.direct;
+  job->direct_off = s.direct_off;
   job->filter = s.filter;
   job->filter_support = s.filter_support;
   job->filter_params[0] = s.filter_params[0];

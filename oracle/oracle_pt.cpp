@@ -1025,13 +1025,15 @@ V3 trace_path(const orc_scene &sc, const orc_job &job, const Camera &cam, uint32
     const Material &m = sc.material_of(gid);
     if (m.specular()) any_specular = true;
     // PathTracer.cpp:50-69 -- emission at depth 0, or after a specular
-    // bounce that left the surface on the normal's side.
+    // bounce that left the surface on the normal's side; at every vertex
+    // when efficientDirect is false (:51).
     if (m.emitter() && dot(n, -ray.d) > 0) {
-      if (depth == 0) L = L + mul(beta, m.Le);
+      if (depth == 0 || job.direct_off) L = L + mul(beta, m.Le);
       else if (emit_after_specular) L = L + mul(beta, m.Le) * (sc.shape_area(gid) / (t * t));
     }
     const Uniforms bu = draw(job.seed, job.job_id, pixel, sample, depth, 0);
-    if (!m.specular()) {
+    // PathTracer.cpp:72-73: next-event estimation only with efficientDirect
+    if (!m.specular() && !job.direct_off) {
       // DirectIllumination.cpp:90-135: per light, ns samples jittered over
       // floor(sqrt(ns))^2 strata of the light, weighted area / ns.
       const uint32_t ns = std::max(1u, job.direct_samples);
@@ -1135,7 +1137,7 @@ void orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t ou
 
 // Bumped with every change of an exported signature; pyoracle refuses a
 // library built from older sources.
-int orc_interface_version(void) { return 8; }
+int orc_interface_version(void) { return 9; }
 
 orc_scene *orc_scene_create(uint32_t nv, const float *vertices, uint32_t nt, const uint32_t *indices,
                             const uint32_t *tri_material, uint32_t ns, const float *spheres,

@@ -89,6 +89,7 @@ typedef struct orc_job {
   uint32_t filter;         /* film: 0 pixel bin, else 1 + filter type (0 box ... 4 lanczosSinc) */
   uint32_t filter_support; /* 0 = 2 */
   double filter_params[2]; /* gaussian sigma; mitchell B, C; lanczosSinc tau (0 = defaults) */
+  uint32_t direct_off;     /* 1: efficientDirect = false (PathTracer.cpp:50-52, 70-73) */
 } orc_job;
 
 /* Renders pixels [pix_begin, pix_end) of the job.  sum/sumsq are
