@@ -268,7 +268,10 @@ MPT_HD float refine_hit(const float4 *rec, float3 o, float3 d, float t32, float2
 // the row after the newest entry is always free and every descent step can
 // store into it whether or not it pushes.  skip_origin is the primitive the
 // ray leaves (self-intersection guard in place of the reference's FP64
-// t > EPSILON), ANY=true finishes at the first hit.
+// t > EPSILON); skip_target the triangle or quad a shadow ray aims at (its
+// only crossing of that primitive is the end point, which the reference's
+// FP64 shadow window t < tMax - 2 EPSILON leaves out).  ANY=true finishes at
+// the first hit.
 // COUNT=true adds the nodes fetched and references tested to n_nodes/n_refs.
 //
 // The walk is resumable: start() clips the ray to the scene box, each step()
@@ -287,9 +290,10 @@ struct Walker {
   static constexpr bool kPow2 = (STACK & (STACK - 1)) == 0;
   float tmin, tmax, troot, best;
   uint32_t best_rec, node, skip_origin;   // best_rec == kMiss: no hit yet; node: byte offset
+  uint32_t skip_target;
   uint32_t depth, top;   // entries held and next write row, both as row * kRow
 
-  MPT_HD bool start(const Scene &sc, const RaySetup &r, float tlimit, uint32_t so) {
+  MPT_HD bool start(const Scene &sc, const RaySetup &r, float tlimit, uint32_t so, uint32_t st = kMiss) {
     // Root slab test (accel/AABB.inl:25-56).
     const float ix = r.inv.x, iy = r.inv.y, iz = r.inv.z;
     const float tx0 = (sc.box_lo[0] - r.o.x) * ix, tx1 = (sc.box_hi[0] - r.o.x) * ix;
@@ -307,6 +311,7 @@ struct Walker {
     depth = 0;
     top = 0;
     skip_origin = so;
+    skip_target = st;
     return tmin <= tmax;
   }
 
@@ -364,12 +369,12 @@ struct Walker {
         // 2^-20 margins keep the prefilter conservative, the exact test
         // below decides
         const bool hit = tri_td(r, xyz(a), xyz(b), xyz(c), T, det, kind == KIND_TRIANGLE_ONE_SIDED);
-        if (hit & (id != skip_origin) & (T <= best * det * (1.0f + 0x1p-20f)) &
+        if (hit & (id != skip_origin) & (id != skip_target) & (T <= best * det * (1.0f + 0x1p-20f)) &
             (T >= kEps * det * (1.0f - 0x1p-20f)))
           t = div_approx(T, det);
       } else if (kind == KIND_SPHERE) {
         t = sphere_t(r, xyz(a), b.x, id == skip_origin);
-      } else if (id != skip_origin) {
+      } else if (id != skip_origin && id != skip_target) {
         t = quad_t(r, xyz(a), xyz(b), xyz(c));
       }
       if (t > kEps && t < best) {
@@ -431,9 +436,9 @@ struct Walker {
 template <bool ANY, bool COUNT = false, int STACK = kStack>
 MPT_HD bool trace(const Scene &sc, const RaySetup &r, float tlimit, uint32_t skip_origin, uint2 *stk,
                   float &t_out, uint32_t &id_out, uint32_t *n_nodes = nullptr, uint32_t *n_refs = nullptr,
-                  float2 *uv = nullptr) {
+                  float2 *uv = nullptr, uint32_t skip_target = kMiss) {
   Walker<ANY, STACK> w;
-  if (!w.start(sc, r, tlimit, skip_origin)) {
+  if (!w.start(sc, r, tlimit, skip_origin, skip_target)) {
     t_out = INFINITY;
     id_out = kMiss;
     if (uv) *uv = make_float2(0.0f, 0.0f);

@@ -181,6 +181,7 @@ struct mpt_context {
   uint32_t persistent = 1;           // closest-hit kernel form: 0 grid-stride, 1 persistent waves with
                                      // lane refill, 2 one ray per lane over a capacity-sized grid
   std::vector<hipEvent_t> events;    // timing pool, pairs per launch
+  std::vector<std::pair<int, size_t>> timed;   // instrumented launches: (kind, first event) since the render began
   std::vector<mpt_scene *> scenes;   // live scenes, films and servers, released by mpt_close
   std::vector<mpt_film *> films;
   std::vector<mpt_server *> servers;

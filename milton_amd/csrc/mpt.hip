@@ -306,7 +306,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(STACK <= 8 
   }
 }
 
-// Any hit of the shadow rays (kdTreeAccel::intersects).  One lane per group:
+// Any hit of the shadow rays (kdTreeAccel::intersects) over t < the ray's
+// limit, skipping the primitive it leaves and the light primitive it aims
+// at.  One lane per group:
 // the light samples of one vertex are a run of the shadow buffer
 // (sr[slot * spv ...], spv = shadow rays per vertex of the batch), traced in
 // (light, sample) order.  Their unoccluded contributions are summed in that
@@ -328,7 +330,7 @@ __global__ __launch_bounds__(256) void k_shadow(Scene sc, const BatchDesc *__res
     // group: (origin, slot) (origin primitive, ray count, 0, 0)
     const float4 g0 = sg[2ull * i], g1 = sg[2ull * i + 1];
     const uint32_t slot = __float_as_uint(g0.w), prim = __float_as_uint(g1.x), cnt = __float_as_uint(g1.y);
-    // ray: (direction, distance limit) (contribution, 0)
+    // ray: (direction, distance limit) (contribution, light primitive or kMiss)
     const float4 *run = sr + 2ull * ((uint64_t)slot * spv);
     float3 sum = f3(0, 0, 0);
     for (uint32_t j = 0; j < cnt; ++j) {
@@ -336,7 +338,8 @@ __global__ __launch_bounds__(256) void k_shadow(Scene sc, const BatchDesc *__res
       const RaySetup r = setup_ray(xyz(g0), xyz(a));
       float t;
       uint32_t id;
-      if (!trace<true, false, STACK>(sc, r, a.w, prim, stk, t, id)) sum = add(sum, xyz(c));
+      if (!trace<true, false, STACK>(sc, r, a.w, prim, stk, t, id, nullptr, nullptr, nullptr, __float_as_uint(c.w)))
+        sum = add(sum, xyz(c));
     }
     rays += cnt;
     if (!is_black(sum)) {
@@ -597,8 +600,10 @@ __global__ __launch_bounds__(kShadeBlock) __attribute__((amdgpu_waves_per_eu(kMu
             const float g = cos_wo * cos_wi / (dist * dist) * (c0.w / (float)ndir_job);
             const float3 contrib = mul(mul(beta, mul(fr, lle)), g);
             if (!is_black(contrib)) {
-              run[2 * n_sh] = make_float4(w.x, w.y, w.z, dist * (1.0f - 1e-5f));
-              run[2 * n_sh + 1] = make_float4(contrib.x, contrib.y, contrib.z, 0.0f);
+              // Scene::intersects(ray, t - EPSILON) over the kd window
+              // t < tMax - EPSILON (DirectIllumination.cpp:123, Scene.cpp:113)
+              run[2 * n_sh] = make_float4(w.x, w.y, w.z, dist - 2.0f * kEps);
+              run[2 * n_sh + 1] = make_float4(contrib.x, contrib.y, contrib.z, __uint_as_float(sc.nt + sc.ns + q));
               ++n_sh;
             }
           }
@@ -617,8 +622,8 @@ __global__ __launch_bounds__(kShadeBlock) __attribute__((amdgpu_waves_per_eu(kMu
         if (dist > 0.0f && cos_wo > 0.0f && !is_black(fr)) {
           const float3 contrib = mul(mul(beta, mul(fr, xyz(le))), cos_wo / (dist * dist));
           if (!is_black(contrib)) {
-            run[2 * n_sh] = make_float4(w.x, w.y, w.z, dist * (1.0f - 1e-5f));
-            run[2 * n_sh + 1] = make_float4(contrib.x, contrib.y, contrib.z, 0.0f);
+            run[2 * n_sh] = make_float4(w.x, w.y, w.z, dist - 2.0f * kEps);
+            run[2 * n_sh + 1] = make_float4(contrib.x, contrib.y, contrib.z, __uint_as_float(kMiss));
             ++n_sh;
           }
         }
@@ -1037,12 +1042,6 @@ void *scratch_ptr(int device, hipError_t *err) {
   return g_scratch_ptr[device];
 }
 
-// Per-launch timing of the instrumented (non-graph) path.
-struct Timed {
-  int kind;
-  size_t ev;
-};
-std::vector<Timed> g_timed;   // guarded by the single-threaded use of a context
 
 
 template <int STACK>
@@ -1116,19 +1115,19 @@ int enqueue_batch(mpt_context *ctx, Lane &ln, mpt_scene *s, uint32_t D, bool mul
   auto tic = [&](int kind) -> int {
     if (!instrument || !ctx->timing) return MPT_OK;
     size_t used = 0;
-    for (const Timed &t : g_timed) used = std::max(used, t.ev + 2);
+    for (const auto &t : ctx->timed) used = std::max(used, t.second + 2);
     while (used + 2 > ctx->events.size()) {
       hipEvent_t e;
       MPT_HIP(ctx, hipEventCreate(&e));
       ctx->events.push_back(e);
     }
-    g_timed.push_back({kind, used});
+    ctx->timed.push_back({kind, used});
     MPT_HIP(ctx, hipEventRecord(ctx->events[used], st));
     return MPT_OK;
   };
   auto toc = [&]() -> int {
     if (!instrument || !ctx->timing) return MPT_OK;
-    MPT_HIP(ctx, hipEventRecord(ctx->events[g_timed.back().ev + 1], st));
+    MPT_HIP(ctx, hipEventRecord(ctx->events[ctx->timed.back().second + 1], st));
     return MPT_OK;
   };
   k_reset<<<1, 64, 0, st>>>(w.counts.p, 3 * D + 1);
@@ -1435,7 +1434,7 @@ int join_lanes(mpt_context *ctx) {
 }
 
 int reset_totals(mpt_context *ctx) {
-  g_timed.clear();
+  ctx->timed.clear();
   for (uint32_t k = 0; k < kMaxLanes; ++k) {
     Lane &ln = ctx->lanes[k];
     if (!ln.stream && k >= ctx->num_lanes) continue;
@@ -1585,13 +1584,13 @@ int render_range(mpt_scene *s, const mpt_job *job, uint32_t pix_begin, uint32_t 
     stats->render_ms = ms;
     stats->extend_launches = batches * job->max_depth;
     std::vector<std::pair<float, float>> span[3];   // per kind: [start, end) after ev0
-    for (const Timed &t : g_timed) {
+    for (const auto &t : ctx->timed) {
       float e = 0, a = 0, b = 0;
-      MPT_HIP(ctx, hipEventElapsedTime(&e, ctx->events[t.ev], ctx->events[t.ev + 1]));
-      MPT_HIP(ctx, hipEventElapsedTime(&a, ctx->ev0, ctx->events[t.ev]));
-      MPT_HIP(ctx, hipEventElapsedTime(&b, ctx->ev0, ctx->events[t.ev + 1]));
-      (t.kind == 0 ? stats->extend_ms : t.kind == 1 ? stats->shade_ms : stats->shadow_ms) += e;
-      span[t.kind].push_back({a, b});
+      MPT_HIP(ctx, hipEventElapsedTime(&e, ctx->events[t.second], ctx->events[t.second + 1]));
+      MPT_HIP(ctx, hipEventElapsedTime(&a, ctx->ev0, ctx->events[t.second]));
+      MPT_HIP(ctx, hipEventElapsedTime(&b, ctx->ev0, ctx->events[t.second + 1]));
+      (t.first == 0 ? stats->extend_ms : t.first == 1 ? stats->shade_ms : stats->shadow_ms) += e;
+      span[t.first].push_back({a, b});
     }
     double *wall[3] = {&stats->extend_wall_ms, &stats->shade_wall_ms, &stats->shadow_wall_ms};
     for (int k = 0; k < 3; ++k) {

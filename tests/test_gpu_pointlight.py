@@ -29,3 +29,14 @@ def test_room_with_point_light_paths(ctx):
     arrays, job = m.load_scenefile(text=text)
     assert len(arrays.points) == 1
     _per_path(ctx, arrays, job, 0, job.width * job.height, "scenefile room + point light", min_frac=0.999)
+
+
+def test_occluder_next_to_point_light(ctx):
+    """An occluder 5e-6 below the point light shadows the floor on the
+    device too (shadow rays end 2 EPSILON before the light, as in the
+    reference), path by path equal to the oracle."""
+    from test_pointlight import occluded_near_light
+    a = occluded_near_light()
+    job = m.make_job(a.camera, 32, 32, 2, 3, seed=8)
+    L_gpu, L_orc = _per_path(ctx, a, job, 0, 32 * 32, "occluder next to a point light", min_frac=1.0)
+    assert (L_orc == 0).all() and (L_gpu == 0).all()

@@ -78,3 +78,24 @@ def test_scenefile_point_light():
         with pytest.raises(m.MptError) as e:
             m.load_scenefile(text=base % bad)
         assert word in str(e.value)
+
+
+def occluded_near_light():
+    """The floor with a small quad 5e-6 below the point light, facing down:
+    it shadows the whole floor in the reference's FP64 window
+    t < dist - 2 EPSILON (DirectIllumination.cpp:123, Scene.cpp:113)."""
+    a = _floor()
+    gap = 5e-6
+    a.quads = np.concatenate([a.quads, np.float32([[LIGHT[0], LIGHT[1] - gap, LIGHT[2], 0.01, 0, 0, 0, 0, -0.01]])])
+    a.quad_material = np.concatenate([a.quad_material, np.uint32([0])])
+    assert np.float32(LIGHT[1] - gap) < np.float32(LIGHT[1])
+    return a
+
+
+def test_occluder_next_to_point_light():
+    a = occluded_near_light()
+    job = m.make_job(a.camera, 16, 16, 2, 3, seed=8)
+    pix = np.repeat(np.arange(256, dtype=np.uint32), 2)
+    smp = np.tile(np.arange(2, dtype=np.uint32), 256)
+    L = po.OracleScene(a).render_paths(job, pix, smp)
+    assert (L == 0).all()
