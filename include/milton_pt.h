@@ -26,8 +26,11 @@
  *  - a sphere is (cx, cy, cz, r) (shapes/Sphere.cpp:16-45);
  *  - a quad is the reference "plane" shape (shapes/Plane.cpp:25-106): the
  *    unit square y=0, |x|,|z|<=0.5 mapped to center + x*ex + z*ez; its normal
- *    is normalize(ez x ex) and its area |ex x ez|.  Quads with an emissive
- *    material are the area lights sampled by next-event estimation;
+ *    is normalize(ez x ex) and its area |ex x ez|;
+ *  - quads, spheres and (mesh) triangles with an emissive material are the
+ *    area lights sampled by next-event estimation, in that order (quads,
+ *    spheres, then emitting shapes by first triangle); standalone one-sided
+ *    triangles may not emit;
  *  - global primitive ids: triangles [0,nt), spheres [nt, nt+ns), quads
  *    [nt+ns, nt+ns+nq); MPT_MISS marks no hit.
  */
@@ -140,6 +143,21 @@ typedef struct mpt_scene_desc {
    * direction, one light sample per vertex (DirectIllumination.cpp:96-98). */
   uint32_t num_points;
   const float *points;
+  /* Optional (NULL = triangles that share an emitting material form one
+   * shape): the reference shape (Mesh) each triangle belongs to.  The
+   * triangles of one emitting shape are one light, sampled as
+   * Mesh::getPosition does (shapes/Mesh.cpp:442-450: a triangle picked by
+   * the sample's u, a point at barycentrics (r1, v, r2) / (r1 + v + r2) with
+   * two more uniforms) and weighted by the shape's world-space area
+   * (Mesh::_getSurfaceArea, :452-466). */
+  const uint32_t *tri_shape;
+  /* Optional (NULL = 2r times the identity): per sphere, the linear part of
+   * its object-to-world transform (9 floats, column-major: the images of the
+   * object axes).  An emitting sphere places its light samples through it,
+   * as Sphere::getPosition does (shapes/Sphere.cpp:117-124: a point at
+   * object-space distance 1 from the centre near the +z axis), with area pi
+   * (Sphere::_getSurfaceArea ignores the scale, :138-141). */
+  const float *sphere_frames;
 } mpt_scene_desc;
 
 enum { MPT_TRI_ONE_SIDED = 1u };

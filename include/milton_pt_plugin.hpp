@@ -85,6 +85,8 @@ class PathTracer {
   std::vector<uint8_t> vmask_;
   std::vector<float> env_rgb_;   // the environment image, when the scene has one
   std::vector<float> points_;    // point lights
+  std::vector<uint32_t> tri_shape_;
+  std::vector<float> sphere_frames_;
   mpt_env_map env_{};
   mpt_scene_desc desc_{};
   mpt_camera camera_{};

@@ -59,7 +59,8 @@ class SceneDesc(ctypes.Structure):
                 ("num_materials", ctypes.c_uint32), ("materials", ctypes.POINTER(Material)),
                 ("env_radiance", ctypes.c_float * 3), ("tri_flags", _u32p), ("vertex_normals", _f32p),
                 ("vertex_normal_mask", _u8p), ("env_map", ctypes.POINTER(EnvMapDesc)),
-                ("num_points", ctypes.c_uint32), ("points", _f32p)]
+                ("num_points", ctypes.c_uint32), ("points", _f32p), ("tri_shape", _u32p),
+                ("sphere_frames", _f32p)]
 
 
 class KdParams(ctypes.Structure):
@@ -254,7 +255,7 @@ class SceneArrays:
 
     def __init__(self, vertices, indices, tri_material, spheres, sphere_material, quads, quad_material,
                  materials, env, camera=None, tri_flags=None, vertex_normals=None, vertex_normal_mask=None,
-                 env_map=None, points=None):
+                 env_map=None, points=None, tri_shape=None, sphere_frames=None):
         self.vertices = np.ascontiguousarray(vertices, np.float32).reshape(-1, 3)
         self.indices = np.ascontiguousarray(indices, np.uint32).reshape(-1, 3)
         self.tri_material = np.ascontiguousarray(tri_material, np.uint32).reshape(-1)
@@ -275,6 +276,11 @@ class SceneArrays:
         # optional point lights: (x, y, z, power r, g, b) per light
         self.points = (np.zeros((0, 6), np.float32) if points is None
                        else np.ascontiguousarray(points, np.float32).reshape(-1, 6))
+        # optional: the emitting shape each triangle belongs to; per sphere the
+        # linear part of its transform (columns: images of the object axes)
+        self.tri_shape = None if tri_shape is None else np.ascontiguousarray(tri_shape, np.uint32).reshape(-1)
+        self.sphere_frames = (None if sphere_frames is None
+                              else np.ascontiguousarray(sphere_frames, np.float32).reshape(-1, 9))
 
     @property
     def num_prims(self):
@@ -308,6 +314,10 @@ class SceneArrays:
             d.env_map = ctypes.pointer(d._env)
         d.num_points = len(self.points)
         d.points = self.points.ctypes.data_as(_f32p)
+        if getattr(self, "tri_shape", None) is not None:
+            d.tri_shape = self.tri_shape.ctypes.data_as(_u32p)
+        if getattr(self, "sphere_frames", None) is not None:
+            d.sphere_frames = self.sphere_frames.ctypes.data_as(_f32p)
         return d
 
     def materials_f32(self):
@@ -796,6 +806,10 @@ def load_scenefile(path=None, text=None, base_dir=None):
                                     (e.params[0], e.params[1]))
         if d.num_points:
             arrays.points = _copy(d.points, d.num_points, np.float32, 6)
+        if d.tri_shape:
+            arrays.tri_shape = _copy(d.tri_shape, nt, np.uint32)
+        if d.sphere_frames:
+            arrays.sphere_frames = _copy(d.sphere_frames, d.num_spheres, np.float32, 9)
         job = Job()
         L.mpt_scenefile_job(h, ctypes.byref(job))
         c = job.camera

@@ -11,8 +11,9 @@
 // Its two converters follow the mapping of Milton's flattened scene to the
 // C ABI's scene description:
 //   flatten()      Scene -> mpt_scene_desc: Mesh triangles (world space, the
-//                  mesh's own normals when it has them), standalone Triangles
-//                  (one-sided), Spheres, Planes (quads), PointShapes with omni
+//                  mesh's own normals when it has them; an emitting mesh is
+//                  one light), standalone Triangles (one-sided), Spheres (with
+//                  their transforms), Planes (quads), PointShapes with omni
 //                  emitters (point lights), and the materials' BSDF and
 //                  emitter keys;
 //   to_camera()    Pinhole / ThinLensCamera -> mpt_camera.
@@ -33,9 +34,10 @@ using namespace milton;
 
 // Milton's scene flattened into the arrays an mpt_scene_desc points at.
 struct Flat {
-  std::vector<float> vertices, vertex_normals, spheres, quads, points;
+  std::vector<float> vertices, vertex_normals, spheres, quads, points, sphere_frames;
   std::vector<uint8_t> normal_mask;
-  std::vector<uint32_t> indices, tri_material, tri_flags, sphere_material, quad_material;
+  std::vector<uint32_t> indices, tri_material, tri_flags, tri_shape, sphere_material, quad_material;
+  uint32_t shapes = 0;
   std::vector<mpt_material> materials;
   std::map<Material *, uint32_t> slot;   // Milton material -> index
   bool any_normals = false;
@@ -50,6 +52,8 @@ struct Flat {
     d.indices = indices.data();
     d.tri_material = tri_material.data();
     d.tri_flags = tri_flags.data();
+    d.tri_shape = tri_shape.data();
+    d.sphere_frames = sphere_frames.data();
     d.num_spheres = (uint32_t)(spheres.size() / 4);
     d.spheres = spheres.data();
     d.sphere_material = sphere_material.data();
@@ -137,8 +141,9 @@ double norm(const Vector3 &a) { return std::sqrt(a[0] * a[0] + a[1] * a[1] + a[2
 void add_shape(Flat &f, Shape *s);
 
 void add_mesh(Flat &f, Mesh *mesh) {
-  if (emitter_type(mesh->getMaterial()) != "null") {
-    f.error = "emitting meshes are not rendered by the device path";
+  const std::string et = emitter_type(mesh->getMaterial());
+  if (et != "null" && et != "oriented") {
+    f.error = "mesh emitters must be oriented";
     return;
   }
   const Matrix4x4 &M = mesh->getTransToWorld();
@@ -147,7 +152,19 @@ void add_mesh(Flat &f, Mesh *mesh) {
   const MeshTriangle *t = mesh->getTriangles(nt);
   const Normal *n = mesh->getNormals(nn);
   const uint32_t base = (uint32_t)(f.vertices.size() / 3);
-  const uint32_t mat = material_of(f, mesh->getMaterial(), 0);
+  // an emitting mesh's radiance spreads its power over its world-space
+  // area (Mesh::_getSurfaceArea, shapes/Mesh.cpp:452-466)
+  double area = 0;
+  if (et == "oriented")
+    for (unsigned i = 0; i < nt; ++i) {
+      const Point3 a = M * Point3(v[t[i].A][0], v[t[i].A][1], v[t[i].A][2]);
+      const Point3 b = M * Point3(v[t[i].B][0], v[t[i].B][1], v[t[i].B][2]);
+      const Point3 c = M * Point3(v[t[i].C][0], v[t[i].C][1], v[t[i].C][2]);
+      const Vector3 e1(b[0] - a[0], b[1] - a[1], b[2] - a[2]), e2(c[0] - a[0], c[1] - a[1], c[2] - a[2]);
+      area += 0.5 * norm(cross(e1, e2));
+    }
+  const uint32_t mat = material_of(f, mesh->getMaterial(), area);
+  const uint32_t shape = f.shapes++;
   for (unsigned i = 0; i < nv; ++i) {
     push3(f.vertices, M * Point3(v[i][0], v[i][1], v[i][2]));
     // a mesh file's normals (Mesh::computeNormals fills the others,
@@ -170,6 +187,7 @@ void add_mesh(Flat &f, Mesh *mesh) {
     f.indices.push_back(base + t[i].C);
     f.tri_material.push_back(mat);
     f.tri_flags.push_back(0);
+    f.tri_shape.push_back(shape);
   }
 }
 
@@ -194,16 +212,22 @@ void add_shape(Flat &f, Shape *s) {
     }
     f.tri_material.push_back(material_of(f, tri->getMaterial(), 0));
     f.tri_flags.push_back(MPT_TRI_ONE_SIDED);
+    f.tri_shape.push_back(f.shapes++);
   } else if (Sphere *sph = dynamic_cast<Sphere *>(s)) {
-    // the unit sphere of radius 0.5 (shapes/Sphere.cpp:16-45) under its transform
-    if (emitter_type(sph->getMaterial()) != "null") {
-      f.error = "emitting spheres are not rendered by the device path";
+    // the unit sphere of radius 0.5 (shapes/Sphere.cpp:16-45) under its
+    // transform, whose linear part also places an emitting sphere's light
+    // samples (Sphere::getPosition, :117-124); its area is pi whatever the
+    // scale (Sphere::_getSurfaceArea, :138-141)
+    const std::string et = emitter_type(sph->getMaterial());
+    if (et != "null" && et != "oriented") {
+      f.error = "sphere emitters must be oriented";
       return;
     }
     const Matrix4x4 &M = sph->getTransToWorld();
     push3(f.spheres, M * Point3(0, 0, 0));
     f.spheres.push_back((float)(0.5 * norm(M * Vector3(1, 0, 0))));
-    f.sphere_material.push_back(material_of(f, sph->getMaterial(), 0));
+    for (int k = 0; k < 3; ++k) push3(f.sphere_frames, M * Vector3(k == 0, k == 1, k == 2));
+    f.sphere_material.push_back(material_of(f, sph->getMaterial(), et == "oriented" ? M_PI : 0));
   } else if (Plane *pl = dynamic_cast<Plane *>(s)) {
     // the unit square y = 0 (shapes/Plane.cpp:25-106): centre and the images
     // of its x and z axes; normal ez x ex

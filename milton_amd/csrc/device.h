@@ -41,18 +41,33 @@ constexpr float kPi = 3.14159265358979323846f;
 
 enum { KIND_TRIANGLE = 0, KIND_TRIANGLE_ONE_SIDED = 1, KIND_SPHERE = 2, KIND_QUAD = 3 };   // triangles: kind < 2
 
+// An area light of next-event estimation (DirectIllumination.cpp:90-135 over
+// Scene::getLights).  kind: LIGHT_QUAD (index = quad), LIGHT_SPHERE (index =
+// sphere; frame = the linear part of its object-to-world transform, columns),
+// LIGHT_MESH (triangles light_tris[first, first + count)).  area: the
+// shape's getSurfaceArea; mat: its emitting material.
+enum { LIGHT_QUAD = 0, LIGHT_SPHERE = 1, LIGHT_MESH = 2 };
+struct LightDev {
+  uint32_t kind, index, first, count;
+  float area;
+  uint32_t mat, pad[2];
+  float4 frame[3];
+};
+
 // Read-only scene as seen by the kernels.
 struct Scene {
   const uint2 *nodes;        // kd nodes (kdbuild.h)
   const float4 *recs;        // 3 float4 per leaf reference
   float box_lo[3], box_hi[3];
-  const float4 *tshade;      // 3 per triangle: (n0, material bits), (n1, 0), (n2, 0) vertex normals
+  const float4 *tshade;      // 3 per triangle: vertex normals (n0, material bits), (n1, area of the emitting
+                             // shape or 0), (n2, its light index + 1 or 0)
   const float4 *spheres;     // (centre, r)
   const uint32_t *sphere_mat;
   const float4 *quads;       // 4 per quad: (c, area), (ex, 0), (ez, 0), (n, material bits)
   const mpt_material *mats;
-  const uint32_t *lights;    // quad indices of emissive quads
+  const struct LightDev *lights;   // area lights: emitting quads, spheres, meshes
   uint32_t num_lights;
+  const float4 *light_tris;  // 3 per triangle of an emitting mesh: (v0, triangle id) (v1, 0) (v2, 0)
   uint32_t nt, ns, nq;
   float env[3];
   // environment image (env_w 0: the constant env): texels (r, g, b, 0) from

@@ -194,7 +194,9 @@ struct mpt_scene {
   uint64_t uid = 0;   // unique per process, keys the graph cache
   mpt_engine::DBuf<uint2> nodes;
   mpt_engine::DBuf<float4> recs, tshade, spheres, quads;
-  mpt_engine::DBuf<uint32_t> sphere_mat, lights;
+  mpt_engine::DBuf<uint32_t> sphere_mat;
+  mpt_engine::DBuf<mpt_dev::LightDev> lights;
+  mpt_engine::DBuf<float4> light_tris;
   mpt_engine::DBuf<mpt_material> mats;
   mpt_engine::DBuf<float4> env_tex, points;
   mpt_engine::DBuf<double> env_kernel;
@@ -203,7 +205,7 @@ struct mpt_scene {
   uint32_t leaf_size = 0;   // kd leaf size the tree was built with
   uint64_t device_bytes() const {
     return nodes.bytes() + recs.bytes() + tshade.bytes() + spheres.bytes() + quads.bytes() + sphere_mat.bytes() +
-           lights.bytes() + mats.bytes() + env_tex.bytes() + env_kernel.bytes();
+           lights.bytes() + light_tris.bytes() + mats.bytes() + env_tex.bytes() + env_kernel.bytes();
   }
 };
 

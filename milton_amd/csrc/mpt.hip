@@ -356,7 +356,8 @@ __global__ __launch_bounds__(256) void k_shadow(Scene sc, const BatchDesc *__res
 struct HitInfo {
   float3 n;
   uint32_t mat;
-  float area;   // shape surface area (quads; Sphere::_getSurfaceArea for spheres)
+  float area;       // shape surface area (quads, emitting meshes; Sphere::_getSurfaceArea for spheres)
+  uint32_t light;   // the emitting mesh the hit triangle belongs to (light index), else kMiss
 };
 
 // Shading normal and material of a hit: Mesh::_getGeometricNormal with
@@ -366,6 +367,7 @@ struct HitInfo {
 __device__ __forceinline__ HitInfo hit_info(const Scene &sc, uint32_t prim, float3 p, float2 uv) {
   HitInfo h;
   h.area = 0.0f;
+  h.light = kMiss;
   if (prim < sc.nt) {
     // Mesh::_getGeometricNormal (shapes/Mesh.cpp:419-433): the vertex normals
     // weighted by the hit's barycentric coordinates
@@ -374,6 +376,8 @@ __device__ __forceinline__ HitInfo hit_info(const Scene &sc, uint32_t prim, floa
     const float w0 = 1.0f - uv.x - uv.y;
     h.n = normalize(add(add(mul(xyz(n0), w0), mul(xyz(n1), uv.x)), mul(xyz(n2), uv.y)));
     h.mat = __float_as_uint(n0.w);
+    h.area = n1.w;                          // the emitting shape's area (0 otherwise)
+    h.light = __float_as_uint(n2.w) - 1u;   // light index + 1, 0 for a triangle that does not emit
   } else if (prim < sc.nt + sc.ns) {
     const float4 s = sc.spheres[prim - sc.nt];
     h.n = normalize(sub(p, xyz(s)));
@@ -576,34 +580,72 @@ __global__ __launch_bounds__(kShadeBlock) __attribute__((amdgpu_waves_per_eu(kMu
         while ((side + 1) * (side + 1) <= ndir_job) ++side;
       const float inv = 1.0f / (float)side;
       for (uint32_t li = 0; li < sc.num_lights; ++li) {
-        const uint32_t q = sc.lights[li];
-        if (sc.nt + sc.ns + q == prim) continue;
-        const float4 *lq = sc.quads + 4ull * q;
-        const float4 c0 = lq[0], ex = lq[1], ez = lq[2], ln = lq[3];
+        const LightDev &ld = sc.lights[li];
+        // not the light the vertex lies on (DirectIllumination.cpp:93)
+        if ((ld.kind == LIGHT_QUAD && prim == sc.nt + sc.ns + ld.index) ||
+            (ld.kind == LIGHT_SPHERE && prim == sc.nt + ld.index) || (ld.kind == LIGHT_MESH && hi.light == li))
+          continue;
         // OrientedEmitter::getLe (emitters/OrientedEmitter.cpp:17-23)
-        const float3 lle = le_of(sc.mats[__float_as_uint(ln.w)]);
+        const float3 lle = le_of(sc.mats[ld.mat]);
         for (uint32_t k = 0; k < ndir_job; ++k) {
           const mpt_rng::U4 r = mpt_rng::path_draw(seed, job_id, pixel, sample, depth, 1 + li * ndir_job + k);
           const uint32_t cell = kMulti ? k % (side * side) : 0u;
           const float u = ((float)(cell % side) + mpt_rng::unitf(r.x)) * inv;
           const float v = ((float)(cell / side) + mpt_rng::unitf(r.y)) * inv;
-          // Plane::getPosition (shapes/Plane.cpp:91-93)
-          const float3 lp = add(xyz(c0), add(mul(xyz(ex), u - 0.5f), mul(xyz(ez), v - 0.5f)));
+          // Shape::getPoint (shapes/Shape.cpp:52-64): the light point, its
+          // normal, and the primitive the shadow ray may skip (its only
+          // crossing of a planar light is the end point)
+          float3 lp, ln;
+          uint32_t target = kMiss;
+          if (ld.kind == LIGHT_QUAD) {
+            // Plane::getPosition (shapes/Plane.cpp:91-93)
+            const float4 *lq = sc.quads + 4ull * ld.index;
+            const float4 c0 = lq[0], ex = lq[1], ez = lq[2], qn = lq[3];
+            lp = add(xyz(c0), add(mul(xyz(ex), u - 0.5f), mul(xyz(ez), v - 0.5f)));
+            ln = xyz(qn);
+            target = sc.nt + sc.ns + ld.index;
+          } else if (ld.kind == LIGHT_SPHERE) {
+            // Sphere::getPosition (shapes/Sphere.cpp:117-124): an object-space
+            // point at distance 1 from the centre near +z, mapped by the
+            // sphere's transform; the normal is that point's direction
+            // (Sphere.cpp:86-92, a similarity transform)
+            const float x = fminf(0.5f, fmaxf(-0.5f, tanf((1.0f - u) * (0.5f / kPi))));
+            const float y = fminf(0.5f, fmaxf(-0.5f, 0.5f * sinf((0.5f - v) * (1.0f / kPi))));
+            const float z = sqrtf(1.0f - x * x - y * y);
+            const float3 q = add(add(mul(xyz(ld.frame[0]), x), mul(xyz(ld.frame[1]), y)), mul(xyz(ld.frame[2]), z));
+            lp = add(xyz(sc.spheres[ld.index]), q);
+            ln = normalize(q);
+          } else {
+            // Mesh::getPosition (shapes/Mesh.cpp:442-450): the triangle picked
+            // by u, barycentrics (r1, v, r2) / (r1 + v + r2)
+            // (MeshTriangle.cpp:299-333); the normal interpolates the vertex
+            // normals there (Mesh::_getGeometricNormal, :419-433)
+            const uint32_t ti = ld.first + min((uint32_t)floor((double)u * ld.count), ld.count - 1u);
+            const float4 *lt = sc.light_tris + 3ull * ti;
+            const float4 t0 = lt[0], t1 = lt[1], t2 = lt[2];
+            const float r1 = mpt_rng::unitf(r.z), r2 = mpt_rng::unitf(r.w);
+            const float ks = 1.0f / (r1 + v + r2);
+            const float b0 = r1 * ks, b1 = v * ks, b2 = r2 * ks;
+            lp = add(add(mul(xyz(t0), b0), mul(xyz(t1), b1)), mul(xyz(t2), b2));
+            target = __float_as_uint(t0.w);
+            const float4 *ts = sc.tshade + 3ull * target;
+            ln = normalize(add(add(mul(xyz(ts[0]), b0), mul(xyz(ts[1]), b1)), mul(xyz(ts[2]), b2)));
+          }
           float3 w = sub(lp, p);
           const float dist = len(w);
           w = mul(w, 1.0f / dist);
           const float cos_wo = fabsf(dot(hi.n, w));
-          const float cos_wi = fabsf(dot(xyz(ln), w));
+          const float cos_wi = fabsf(dot(ln, w));
           const float3 fr = bsdf_eval(m, hi.n, d, w);
-          const bool lit = dot(xyz(ln), w) < 0.0f;
+          const bool lit = dot(ln, w) < 0.0f;
           if (cos_wo > 0.0f && cos_wi > 0.0f && !is_black(fr) && lit) {
-            const float g = cos_wo * cos_wi / (dist * dist) * (c0.w / (float)ndir_job);
+            const float g = cos_wo * cos_wi / (dist * dist) * (ld.area / (float)ndir_job);
             const float3 contrib = mul(mul(beta, mul(fr, lle)), g);
             if (!is_black(contrib)) {
               // Scene::intersects(ray, t - EPSILON) over the kd window
               // t < tMax - EPSILON (DirectIllumination.cpp:123, Scene.cpp:113)
               run[2 * n_sh] = make_float4(w.x, w.y, w.z, dist - 2.0f * kEps);
-              run[2 * n_sh + 1] = make_float4(contrib.x, contrib.y, contrib.z, __uint_as_float(sc.nt + sc.ns + q));
+              run[2 * n_sh + 1] = make_float4(contrib.x, contrib.y, contrib.z, __uint_as_float(target));
               ++n_sh;
             }
           }
@@ -1623,7 +1665,9 @@ struct HostScene {
   std::vector<float4> verts, vnorm, sph, quads;
   std::vector<float4> tshade;   // device shading records, 3 per triangle
   std::vector<uint4> tris;
-  std::vector<uint32_t> lights, sphere_mat;
+  std::vector<uint32_t> sphere_mat;
+  std::vector<LightDev> lights;
+  std::vector<float4> light_tris;   // 3 per triangle of an emitting mesh
   std::vector<mpt_material> mats;
   mpt_kd::Output ko;
   uint32_t nt = 0, ns = 0, nq = 0;
@@ -1654,12 +1698,18 @@ int prepare_scene(mpt_context *ctx, const mpt_scene_desc *d, const mpt_kd_params
     for (uint32_t i = 0; i < nt; ++i)
       if (d->tri_flags[i] & ~(uint32_t)MPT_TRI_ONE_SIDED) return fail(ctx, MPT_E_INVALID, "unknown triangle flag");
   auto bad_mat = [&](uint32_t m) { return m >= nm; };
-  for (uint32_t i = 0; i < nt; ++i)
-    if (bad_mat(d->tri_material[i]) || emissive_host(d->materials[d->tri_material[i]]))
-      return fail(ctx, MPT_E_INVALID, "triangle material out of range or emissive (only quads may emit)");
+  for (uint32_t i = 0; i < nt; ++i) {
+    if (bad_mat(d->tri_material[i])) return fail(ctx, MPT_E_INVALID, "triangle material out of range");
+    // a standalone Triangle's light samples depend on its texture
+    // coordinates (Triangle::getPosition, shapes/Triangle.cpp:330-353)
+    if (emissive_host(d->materials[d->tri_material[i]]) && d->tri_flags && (d->tri_flags[i] & MPT_TRI_ONE_SIDED))
+      return fail(ctx, MPT_E_INVALID, "a one-sided (standalone) triangle may not emit");
+  }
   for (uint32_t i = 0; i < ns; ++i)
-    if (bad_mat(d->sphere_material[i]) || emissive_host(d->materials[d->sphere_material[i]]))
-      return fail(ctx, MPT_E_INVALID, "sphere material out of range or emissive (only quads may emit)");
+    if (bad_mat(d->sphere_material[i])) return fail(ctx, MPT_E_INVALID, "sphere material out of range");
+  if (d->sphere_frames)
+    for (uint64_t i = 0; i < 9ull * ns; ++i)
+      if (!std::isfinite(d->sphere_frames[i])) return fail(ctx, MPT_E_INVALID, "bad sphere frame");
   for (uint32_t i = 0; i < nq; ++i)
     if (bad_mat(d->quad_material[i])) return fail(ctx, MPT_E_INVALID, "quad material out of range");
   if (d->num_points && !d->points) return fail(ctx, MPT_E_INVALID, "missing scene array");
@@ -1728,7 +1778,6 @@ int prepare_scene(mpt_context *ctx, const mpt_scene_desc *d, const mpt_kd_params
     sph[i] = make_float4(d->spheres[4 * i], d->spheres[4 * i + 1], d->spheres[4 * i + 2], d->spheres[4 * i + 3]);
   std::vector<float4> &quads = hs.quads;
   quads.resize(4ull * nq);
-  std::vector<uint32_t> &lights = hs.lights;
   for (uint32_t i = 0; i < nq; ++i) {
     const float *q = d->quads + 9ull * i;
     const double ex[3] = {q[3], q[4], q[5]}, ez[3] = {q[6], q[7], q[8]};
@@ -1743,7 +1792,87 @@ int prepare_scene(mpt_context *ctx, const mpt_scene_desc *d, const mpt_kd_params
     float mf;
     std::memcpy(&mf, &mbits, 4);
     quads[4 * i + 3] = make_float4((float)n[0], (float)n[1], (float)n[2], mf);
-    if (emissive_host(d->materials[d->quad_material[i]])) lights.push_back(i);
+  }
+  // Area lights (Scene::getLights): emitting quads, then spheres, then
+  // emitting meshes in the order of their first triangle.
+  std::vector<LightDev> &lights = hs.lights;
+  auto light = [](uint32_t kind, uint32_t index, uint32_t mat, double area) {
+    LightDev l;
+    std::memset(&l, 0, sizeof(l));
+    l.kind = kind;
+    l.index = index;
+    l.mat = mat;
+    l.area = (float)area;
+    return l;
+  };
+  for (uint32_t i = 0; i < nq; ++i)
+    if (emissive_host(d->materials[d->quad_material[i]]))
+      lights.push_back(light(LIGHT_QUAD, i, d->quad_material[i], quads[4 * i].w));
+  for (uint32_t i = 0; i < ns; ++i) {
+    if (!emissive_host(d->materials[d->sphere_material[i]])) continue;
+    // Sphere::_getSurfaceArea: 4 pi 0.25 whatever the scale (shapes/Sphere.cpp:138-141)
+    LightDev l = light(LIGHT_SPHERE, i, d->sphere_material[i], M_PI);
+    const float r2 = 2.0f * sph[i].w;
+    for (int k = 0; k < 3; ++k) {
+      const float *f = d->sphere_frames ? d->sphere_frames + 9ull * i + 3 * k : nullptr;
+      l.frame[k] = f ? make_float4(f[0], f[1], f[2], 0) : make_float4(k == 0 ? r2 : 0, k == 1 ? r2 : 0, k == 2 ? r2 : 0, 0);
+    }
+    lights.push_back(l);
+  }
+  {
+    // emitting triangles grouped by shape (tri_shape, else material); a
+    // shape's world area is the sum of its triangles' (Mesh::_getSurfaceArea)
+    std::vector<uint32_t> group_of(nt, kMiss);
+    std::vector<uint32_t> order;               // groups by first triangle
+    std::vector<std::vector<uint32_t>> members;
+    std::vector<uint64_t> keys;
+    for (uint32_t t = 0; t < nt; ++t) {
+      if (!emissive_host(d->materials[d->tri_material[t]])) continue;
+      const uint64_t key = d->tri_shape ? ((uint64_t)1 << 32) | d->tri_shape[t] : d->tri_material[t];
+      uint32_t g = kMiss;
+      for (uint32_t k = 0; k < keys.size(); ++k)
+        if (keys[k] == key) g = k;
+      if (g == kMiss) {
+        g = (uint32_t)keys.size();
+        keys.push_back(key);
+        members.emplace_back();
+      }
+      if (d->tri_material[t] != d->tri_material[members[g].empty() ? t : members[g][0]])
+        return fail(ctx, MPT_E_INVALID, "triangles of one emitting shape have different materials");
+      members[g].push_back(t);
+      group_of[t] = g;
+    }
+    for (uint32_t g = 0; g < members.size(); ++g) {
+      double area = 0;
+      for (uint32_t t : members[g]) {
+        const float4 a = verts[tris[t].x], b = verts[tris[t].y], c = verts[tris[t].z];
+        const double e1[3] = {(double)b.x - a.x, (double)b.y - a.y, (double)b.z - a.z};
+        const double e2[3] = {(double)c.x - a.x, (double)c.y - a.y, (double)c.z - a.z};
+        const double cx = e1[1] * e2[2] - e1[2] * e2[1], cy = e1[2] * e2[0] - e1[0] * e2[2],
+                     cz = e1[0] * e2[1] - e1[1] * e2[0];
+        area += 0.5 * std::sqrt(cx * cx + cy * cy + cz * cz);
+      }
+      const uint32_t li = (uint32_t)lights.size();
+      LightDev l = light(LIGHT_MESH, 0, d->tri_material[members[g][0]], area);
+      l.first = (uint32_t)(hs.light_tris.size() / 3);
+      l.count = (uint32_t)members[g].size();
+      for (uint32_t t : members[g]) {
+        uint32_t tb = t;
+        float tf;
+        std::memcpy(&tf, &tb, 4);
+        const float4 a = verts[tris[t].x], b = verts[tris[t].y], c = verts[tris[t].z];
+        hs.light_tris.push_back(make_float4(a.x, a.y, a.z, tf));
+        hs.light_tris.push_back(make_float4(b.x, b.y, b.z, 0));
+        hs.light_tris.push_back(make_float4(c.x, c.y, c.z, 0));
+        // the shading record carries the shape's area and light index + 1
+        uint32_t lb = li + 1;
+        float lf;
+        std::memcpy(&lf, &lb, 4);
+        hs.tshade[3ull * t + 1].w = (float)area;
+        hs.tshade[3ull * t + 2].w = lf;
+      }
+      lights.push_back(l);
+    }
   }
 
   // kd-tree over all primitives.
@@ -1853,7 +1982,8 @@ int prepare_env(mpt_context *ctx, const mpt_scene_desc *d, HostScene &hs) {
 // Points a device-layout Scene at host (or device) arrays.
 void bind_scene(Scene &sc, const uint2 *nodes, const float4 *recs, const mpt_kd::Box &box, const float4 *tshade,
                 const float4 *sph, const uint32_t *sph_mat,
-                const float4 *quads, const mpt_material *mats, const uint32_t *lights, uint32_t nl, uint32_t nt,
+                const float4 *quads, const mpt_material *mats, const LightDev *lights, uint32_t nl,
+                const float4 *light_tris, uint32_t nt,
                 uint32_t ns, uint32_t nq, const float *env) {
   sc.nodes = nodes;
   sc.recs = recs;
@@ -1868,6 +1998,7 @@ void bind_scene(Scene &sc, const uint2 *nodes, const float4 *recs, const mpt_kd:
   sc.mats = mats;
   sc.lights = lights;
   sc.num_lights = nl;
+  sc.light_tris = light_tris;
   sc.nt = nt;
   sc.ns = ns;
   sc.nq = nq;
@@ -2013,12 +2144,14 @@ int mpt_scene_create(mpt_context *ctx, const mpt_scene_desc *d, const mpt_kd_par
   if ((rc = upload(ctx, s->quads, hs.quads.data(), hs.quads.size()))) return rc;
   if ((rc = upload(ctx, s->mats, hs.mats.data(), hs.mats.size()))) return rc;
   if ((rc = upload(ctx, s->lights, hs.lights.data(), hs.lights.size()))) return rc;
+  if ((rc = upload(ctx, s->light_tris, hs.light_tris.data(), hs.light_tris.size()))) return rc;
   if ((rc = upload(ctx, s->env_tex, hs.env_tex.data(), hs.env_tex.size()))) return rc;
   if ((rc = upload(ctx, s->env_kernel, hs.env_kernel.data(), hs.env_kernel.size()))) return rc;
   if ((rc = upload(ctx, s->points, hs.points.data(), hs.points.size()))) return rc;
   MPT_HIP(ctx, hipStreamSynchronize(ctx->stream));
   bind_scene(s->dev, s->nodes.p, s->recs.p, ko.box, s->tshade.p, s->spheres.p,
-             s->sphere_mat.p, s->quads.p, s->mats.p, s->lights.p, (uint32_t)hs.lights.size(), nt, ns, nq, hs.env);
+             s->sphere_mat.p, s->quads.p, s->mats.p, s->lights.p, (uint32_t)hs.lights.size(), s->light_tris.p, nt, ns,
+             nq, hs.env);
   s->dev.env_tex = s->env_tex.p;
   s->dev.env_kernel = s->env_kernel.p;
   s->dev.env_w = hs.env_w;
@@ -2059,7 +2192,7 @@ int mpt_kd_build(const mpt_scene_desc *d, const mpt_kd_params *kd, mpt_kd_tree *
   bind_scene(t->sc, reinterpret_cast<const uint2 *>(hs.ko.nodes.data()),
              reinterpret_cast<const float4 *>(hs.ko.records.data()), hs.ko.box, hs.tshade.data(),
              hs.sph.data(), hs.sphere_mat.data(), hs.quads.data(), hs.mats.data(),
-             hs.lights.data(), (uint32_t)hs.lights.size(), hs.nt, hs.ns, hs.nq, hs.env);
+             hs.lights.data(), (uint32_t)hs.lights.size(), hs.light_tris.data(), hs.nt, hs.ns, hs.nq, hs.env);
   *out = t.release();
   return MPT_OK;
 }

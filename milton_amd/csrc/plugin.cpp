@@ -75,6 +75,8 @@ void PathTracer::setScene(const mpt_scene_desc &d) {
   desc_.vertex_normals = keep(vnormals_, d.vertex_normals, 3ull * d.num_vertices);
   desc_.vertex_normal_mask = keep(vmask_, d.vertex_normal_mask, d.num_vertices);
   desc_.points = keep(points_, d.points, 6ull * d.num_points);
+  desc_.tri_shape = keep(tri_shape_, d.tri_shape, d.num_triangles);
+  desc_.sphere_frames = keep(sphere_frames_, d.sphere_frames, 9ull * d.num_spheres);
   if (d.env_map && d.env_map->rgb) {
     env_ = *d.env_map;
     env_rgb_.assign(env_.rgb, env_.rgb + 3ull * env_.width * env_.height);

@@ -272,6 +272,9 @@ Mat rotate(const double p[3], const double a[3], double deg) {
 struct Flat {
   std::vector<float> verts, normals;     // normals: one per vertex when has_normals
   std::vector<uint32_t> idx, tri_mat, tri_flags;
+  std::vector<uint32_t> tri_shape;       // per triangle: the mesh or triangle shape it came from
+  uint32_t shapes = 0;
+  std::vector<float> sphere_frames;      // per sphere: images of the object axes (9 floats)
   bool any_normals = false;              // some mesh brought its own normals
   std::vector<uint8_t> vert_has_normal;  // per vertex: normal given by the file
   std::vector<float> spheres, quads;
@@ -696,7 +699,9 @@ class Loader {
   // One triangle with its own three vertices: the standalone Triangle's
   // normal is its face normal (Triangle.cpp:305-311).
   bool one_sided(const double a[3], const double b[3], const double c[3], uint32_t mat) {
-    if (flat_.mat_emits[mat]) return bad("emitting triangles are not supported (only planes may emit)");
+    // a standalone Triangle samples its light by texture coordinates
+    // (Triangle::getPosition, shapes/Triangle.cpp:330-353)
+    if (flat_.mat_emits[mat]) return bad("emitting triangles are not supported (planes, spheres and meshes may emit)");
     const uint32_t base = (uint32_t)(flat_.verts.size() / 3);
     add_vertex(a);
     add_vertex(b);
@@ -704,6 +709,7 @@ class Loader {
     flat_.idx.insert(flat_.idx.end(), {base, base + 1, base + 2});
     flat_.tri_mat.push_back(mat);
     flat_.tri_flags.push_back(MPT_TRI_ONE_SIDED);
+    flat_.tri_shape.push_back(flat_.shapes++);
     return true;
   }
 
@@ -769,7 +775,7 @@ class Loader {
       // Cube::getIntersection (shapes/Cube.cpp:17-52): the unit cube
       // [-0.5, 0.5]^3 hit on its six faces p[u] = +-0.5 with |p[v]|, |p[w]|
       // <= 0.5 from either side, i.e. six unit planes with outward normals.
-      if (flat_.mat_emits[m]) return bad("emitting cubes are not supported (only planes may emit)");
+      if (flat_.mat_emits[m]) return bad("emitting cubes are not supported (planes, spheres and meshes may emit)");
       for (int u = 0; u < 3; ++u) {
         const int a = (u + 1) % 3, b = (u + 2) % 3;
         for (int sgn = -1; sgn <= 1; sgn += 2) {
@@ -791,7 +797,7 @@ class Loader {
       return one_sided(w[0], w[1], w[2], m);
     }
     if (type == "sphere") {
-      if (flat_.mat_emits[m]) return bad("emitting spheres are not supported (only planes may emit)");
+      if (flat_.mat_emits[m] == 2) return bad("an omni emitter needs a point shape");
       Mat T = M;
       double pos[3], r;
       if (vec(v.find("position"), pos, 3)) T = T * translate(pos);
@@ -818,10 +824,14 @@ class Loader {
       for (int i = 0; i < 3; ++i) flat_.spheres.push_back((float)c[i]);
       flat_.spheres.push_back((float)(0.5 * len[0]));
       flat_.sphere_mat.push_back(m);
+      // the transform's linear part places an emitting sphere's light
+      // samples (Sphere::getPosition, shapes/Sphere.cpp:117-124)
+      for (int k = 0; k < 3; ++k)
+        for (int i = 0; i < 3; ++i) flat_.sphere_frames.push_back((float)ax[k][i]);
       return true;
     }
     if (type == "mesh") {
-      if (flat_.mat_emits[m]) return bad("emitting meshes are not supported (only planes may emit)");
+      if (flat_.mat_emits[m] == 2) return bad("an omni emitter needs a point shape");
       const JVal *path = v.find("path");
       if (!path || path->kind != JVal::STR) return bad("mesh needs a path");
       const JVal *norm = v.find("normalize");
@@ -979,10 +989,12 @@ class Loader {
         }
       }
     }
+    const uint32_t shape = flat_.shapes++;
     for (size_t f = 0; f + 2 < F.size(); f += 3) {
       flat_.idx.insert(flat_.idx.end(), {base + F[f], base + F[f + 1], base + F[f + 2]});
       flat_.tri_mat.push_back(mat);
       flat_.tri_flags.push_back(0);
+      flat_.tri_shape.push_back(shape);
     }
     return true;
   }
@@ -1024,6 +1036,41 @@ int finish(Loader &ld, mpt_scenefile **out, char *err, size_t err_len) {
     f.quad_mat[q] = (uint32_t)mats.size();
     mats.push_back(em);
   }
+  // emitting spheres: area pi whatever the scale (Sphere::_getSurfaceArea,
+  // shapes/Sphere.cpp:138-141)
+  for (size_t i = 0; i < f.sphere_mat.size(); ++i) {
+    const uint32_t m = f.sphere_mat[i];
+    if (!f.mat_emits[m]) continue;
+    mpt_material em = f.mats[m];
+    for (int k = 0; k < 3; ++k) em.Le[k] = (float)(f.mat_power[3 * m + k] / (M_PI * M_PI));
+    f.sphere_mat[i] = (uint32_t)mats.size();
+    mats.push_back(em);
+  }
+  // emitting meshes: their world-space area (Mesh::_getSurfaceArea,
+  // shapes/Mesh.cpp:452-466), one material per mesh
+  {
+    std::vector<double> area(f.shapes, 0.0);
+    for (size_t t = 0; t < f.tri_mat.size(); ++t) {
+      if (!f.mat_emits[f.tri_mat[t]]) continue;
+      const float *a = &f.verts[3 * f.idx[3 * t]], *b = &f.verts[3 * f.idx[3 * t + 1]], *c = &f.verts[3 * f.idx[3 * t + 2]];
+      const double e1[3] = {(double)b[0] - a[0], (double)b[1] - a[1], (double)b[2] - a[2]};
+      const double e2[3] = {(double)c[0] - a[0], (double)c[1] - a[1], (double)c[2] - a[2]};
+      const double cx = e1[1] * e2[2] - e1[2] * e2[1], cy = e1[2] * e2[0] - e1[0] * e2[2], cz = e1[0] * e2[1] - e1[1] * e2[0];
+      area[f.tri_shape[t]] += 0.5 * std::sqrt(cx * cx + cy * cy + cz * cz);
+    }
+    std::vector<uint32_t> shape_mat(f.shapes, ~0u);
+    for (size_t t = 0; t < f.tri_mat.size(); ++t) {
+      const uint32_t m = f.tri_mat[t], sh = f.tri_shape[t];
+      if (!f.mat_emits[m]) continue;
+      if (shape_mat[sh] == ~0u) {
+        mpt_material em = f.mats[m];
+        for (int k = 0; k < 3; ++k) em.Le[k] = (float)(area[sh] > 0 ? f.mat_power[3 * m + k] / (M_PI * area[sh]) : 0);
+        shape_mat[sh] = (uint32_t)mats.size();
+        mats.push_back(em);
+      }
+      f.tri_mat[t] = shape_mat[sh];
+    }
+  }
   f.mats = std::move(mats);
   mpt_scene_desc &d = sf->desc;
   std::memset(&d, 0, sizeof(d));
@@ -1047,6 +1094,8 @@ int finish(Loader &ld, mpt_scenefile **out, char *err, size_t err_len) {
     d.env_map = &f.env_map;
   }
   d.tri_flags = f.tri_flags.data();
+  d.tri_shape = f.tri_shape.data();
+  d.sphere_frames = f.sphere_frames.data();
   d.vertex_normals = f.any_normals ? f.normals.data() : nullptr;
   d.vertex_normal_mask = f.any_normals ? f.vert_has_normal.data() : nullptr;
   d.num_points = (uint32_t)(f.points.size() / 6);

@@ -550,7 +550,20 @@ struct orc_scene {
   std::vector<Sphere> spheres;
   std::vector<Quad> quads;
   std::vector<Material> mats;
-  std::vector<uint32_t> lights;   // quad indices with emissive materials
+  // Area lights (Scene::getLights): emitting quads, spheres, then meshes by
+  // first triangle.  A sphere samples through the linear part of its
+  // transform (frame: images of the object axes); a mesh is its triangles.
+  struct Light {
+    int kind;           // 0 quad, 1 sphere, 2 mesh
+    uint32_t index;     // quad or sphere index
+    uint32_t mat;
+    double area;        // Shape::getSurfaceArea
+    V3 frame[3];
+    std::vector<uint32_t> tris;
+  };
+  std::vector<Light> lights;
+  std::vector<uint32_t> tri_light;    // per triangle: its emitting mesh's light index, or ORC_MISS
+  std::vector<double> tri_area;       // per triangle: its emitting mesh's area, or 0
   struct Point {
     V3 p, Le;
   };
@@ -755,7 +768,56 @@ struct orc_scene {
   double shape_area(uint32_t gid) const {
     if (gid >= nt + ns) return quads[gid - nt - ns].area;
     if (gid >= nt) return M_PI;
-    return 0;
+    return tri_area[gid];
+  }
+
+  // The light list from the scene's emitters; tri_shape (optional) names
+  // the shape of each triangle (else triangles sharing an emitting material
+  // form one shape), sphere_frames (optional) the spheres' transforms (else
+  // 2r times the identity).
+  void build_lights(const uint32_t *tri_shape, const float *sphere_frames) {
+    lights.clear();
+    tri_light.assign(nt, ORC_MISS);
+    tri_area.assign(nt, 0.0);
+    for (uint32_t q = 0; q < nq; ++q)
+      if (mats[quad_mat[q]].emitter()) lights.push_back(Light{0, q, quad_mat[q], quads[q].area, {}, {}});
+    for (uint32_t i = 0; i < ns; ++i) {
+      if (!mats[sph_mat[i]].emitter()) continue;
+      // Sphere::_getSurfaceArea ignores the transform (shapes/Sphere.cpp:138-141)
+      Light l{1, i, sph_mat[i], M_PI, {}, {}};
+      for (int k = 0; k < 3; ++k)
+        l.frame[k] = sphere_frames ? V3(sphere_frames[9 * i + 3 * k], sphere_frames[9 * i + 3 * k + 1],
+                                        sphere_frames[9 * i + 3 * k + 2])
+                                   : V3(k == 0, k == 1, k == 2) * (2.0 * spheres[i].r);
+      lights.push_back(l);
+    }
+    std::vector<uint64_t> keys;
+    std::vector<size_t> slot;
+    for (uint32_t t = 0; t < nt; ++t) {
+      if (!mats[tri_mat[t]].emitter()) continue;
+      const uint64_t key = tri_shape ? ((uint64_t)1 << 32) | tri_shape[t] : tri_mat[t];
+      size_t g = keys.size();
+      for (size_t k = 0; k < keys.size(); ++k)
+        if (keys[k] == key) g = k;
+      if (g == keys.size()) {
+        keys.push_back(key);
+        slot.push_back(lights.size());
+        lights.push_back(Light{2, 0, tri_mat[t], 0.0, {}, {}});
+      }
+      lights[slot[g]].tris.push_back(t);
+    }
+    for (size_t g = 0; g < slot.size(); ++g) {
+      Light &l = lights[slot[g]];
+      // Mesh::_getSurfaceArea (shapes/Mesh.cpp:452-466), in world space
+      for (uint32_t t : l.tris) {
+        const V3 &a = verts[idx[3 * t]], &b = verts[idx[3 * t + 1]], &c = verts[idx[3 * t + 2]];
+        l.area += 0.5 * len(cross(b - a, c - a));
+      }
+      for (uint32_t t : l.tris) {
+        tri_light[t] = (uint32_t)slot[g];
+        tri_area[t] = l.area;
+      }
+    }
   }
 };
 
@@ -1041,32 +1103,58 @@ V3 trace_path(const orc_scene &sc, const orc_job &job, const Camera &cam, uint32
       while ((side + 1) * (side + 1) <= ns) ++side;
       V3 direct;
       for (uint32_t li = (uint32_t)sc.lights.size(); li--;) {
-        const uint32_t q = sc.lights[li];
-        const uint32_t lgid = sc.nt + sc.ns + q;
-        if (lgid == gid) continue;
-        const Quad &lq = sc.quads[q];
+        const orc_scene::Light &L = sc.lights[li];
+        // not the light the vertex lies on (DirectIllumination.cpp:93)
+        if ((L.kind == 0 && gid == sc.nt + sc.ns + L.index) || (L.kind == 1 && gid == sc.nt + L.index) ||
+            (L.kind == 2 && gid < sc.nt && sc.tri_light[gid] == li))
+          continue;
         V3 dl;
         for (uint32_t k = 0; k < ns; ++k) {
           const Uniforms lu = draw(job.seed, job.job_id, pixel, sample, depth, 1 + li * ns + k);
           const uint32_t cell = k % (side * side);
           const double u = ((double)(cell % side) + lu.u[0]) / side, v = ((double)(cell / side) + lu.u[1]) / side;
-          const V3 lp = lq.c + lq.ex * (u - 0.5) + lq.ez * (v - 0.5);   // Plane.cpp:91-93
+          // Shape::getPoint (shapes/Shape.cpp:52-64): position and normal
+          V3 lp, ln;
+          if (L.kind == 0) {
+            const Quad &lq = sc.quads[L.index];
+            lp = lq.c + lq.ex * (u - 0.5) + lq.ez * (v - 0.5);   // Plane.cpp:91-93
+            ln = lq.n;
+          } else if (L.kind == 1) {
+            // Sphere::getPosition (shapes/Sphere.cpp:117-124) through the
+            // transform; the normal is the point's object-space direction
+            // carried to world space (Sphere.cpp:86-92)
+            const double x = std::min(0.5, std::max(-0.5, std::tan((1.0 - u) / (2 * M_PI))));
+            const double y = std::min(0.5, std::max(-0.5, std::sin((0.5 - v) / M_PI) * 0.5));
+            const double z = std::sqrt(1 - x * x - y * y);
+            const V3 q = L.frame[0] * x + L.frame[1] * y + L.frame[2] * z;
+            lp = sc.spheres[L.index].c + q;
+            ln = normalized(q);
+          } else {
+            // Mesh::getPosition (shapes/Mesh.cpp:442-450, MESH_GET_INDEX :26)
+            // with MeshTriangle::getPosition (MeshTriangle.cpp:299-333)
+            const uint32_t n_tri = (uint32_t)L.tris.size();
+            const uint32_t t = L.tris[std::min((uint32_t)std::floor(u * n_tri), n_tri - 1)];
+            const double r1 = lu.u[2], r2 = lu.u[3], sum = r1 + v + r2;
+            lp = sc.verts[sc.idx[3 * t]] * (r1 / sum) + sc.verts[sc.idx[3 * t + 1]] * (v / sum) +
+                 sc.verts[sc.idx[3 * t + 2]] * (r2 / sum);
+            ln = sc.normal_at(t, lp);   // Mesh::_getGeometricNormal at that point
+          }
           V3 wo = lp - p;
           const double dist = len(wo);
           wo = wo / dist;
           const double cos_wo = std::fabs(dot(n, wo));
-          const double cos_wi = std::fabs(dot(lq.n, wo));
+          const double cos_wi = std::fabs(dot(ln, wo));
           const V3 fr = bsdf_eval(m, n, ray.d, wo);
           if (cos_wo > 0 && cos_wi > 0 && !is_black(fr)) {
             ctr.shadow++;
             if (!sc.occluded(Ray(p, wo), dist - kEps)) {
-              const Material &lm = sc.mats[sc.quad_mat[q]];
-              const V3 Le = dot(lq.n, -wo) > 0 ? lm.Le : V3();   // OrientedEmitter.cpp:17-23
+              const Material &lm = sc.mats[L.mat];
+              const V3 Le = dot(ln, -wo) > 0 ? lm.Le : V3();   // OrientedEmitter.cpp:17-23
               dl = dl + mul(fr, Le) * (cos_wo * cos_wi / (dist * dist));
             }
           }
         }
-        direct = direct + dl * (lq.area / ns);
+        direct = direct + dl * (L.area / ns);
       }
       // Point lights (DirectIllumination.cpp:96-121, isPoint): one sample, no
       // area weight; a PointShape has no normal, so SurfacePoint's normal
@@ -1137,7 +1225,7 @@ void orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t ou
 
 // Bumped with every change of an exported signature; pyoracle refuses a
 // library built from older sources.
-int orc_interface_version(void) { return 9; }
+int orc_interface_version(void) { return 10; }
 
 orc_scene *orc_scene_create(uint32_t nv, const float *vertices, uint32_t nt, const uint32_t *indices,
                             const uint32_t *tri_material, uint32_t ns, const float *spheres,
@@ -1161,7 +1249,7 @@ orc_scene *orc_scene_create(uint32_t nv, const float *vertices, uint32_t nt, con
     if (s->idx[i] >= nv) return nullptr;
   for (uint32_t i = 0; i < nm; ++i) s->mats.push_back(load_material(materials + 16ull * i));
   for (uint32_t i = 0; i < nt; ++i)
-    if (s->tri_mat[i] >= nm || s->mats[s->tri_mat[i]].emitter()) return nullptr;
+    if (s->tri_mat[i] >= nm || (s->mats[s->tri_mat[i]].emitter() && s->one_sided[i])) return nullptr;
   // Mesh::computeNormals (shapes/Mesh.cpp:335-377): sum of unit face normals.
   s->vnormals.assign(nv, V3());
   for (uint32_t i = 0; i < nt; ++i) {
@@ -1179,7 +1267,7 @@ orc_scene *orc_scene_create(uint32_t nv, const float *vertices, uint32_t nt, con
   for (auto &v : s->vnormals) v = normalized(v);
   for (uint32_t i = 0; i < ns; ++i) {
     s->spheres.push_back(Sphere{V3(spheres[4 * i], spheres[4 * i + 1], spheres[4 * i + 2]), spheres[4 * i + 3]});
-    if (sphere_material[i] >= nm || s->mats[sphere_material[i]].emitter()) return nullptr;
+    if (sphere_material[i] >= nm) return nullptr;
   }
   s->sph_mat.assign(sphere_material, sphere_material + ns);
   for (uint32_t i = 0; i < nq; ++i) {
@@ -1194,9 +1282,9 @@ orc_scene *orc_scene_create(uint32_t nv, const float *vertices, uint32_t nt, con
     Q.area = len(cross(Q.ex, Q.ez));
     s->quads.push_back(Q);
     if (quad_material[i] >= nm) return nullptr;
-    if (s->mats[quad_material[i]].emitter()) s->lights.push_back(i);
   }
   s->quad_mat.assign(quad_material, quad_material + nq);
+  s->build_lights(nullptr, nullptr);
   s->env = V3(env_radiance[0], env_radiance[1], env_radiance[2]);
 
   KdParams kp;
@@ -1557,6 +1645,12 @@ double orc_filter(int type, double support, const double *p, double x, double y)
 
 // Point lights: 6 floats each, position and power; radiance P / (4 pi)
 // (emitters/OmniEmitter.cpp:15-17; Emitter.cpp:44-46 for a zero area).
+// Emitting meshes and spheres: the shape of each triangle and the spheres'
+// transforms (both optional), as in mpt_scene_desc.
+extern "C" void orc_scene_set_emitters(orc_scene *s, const uint32_t *tri_shape, const float *sphere_frames) {
+  s->build_lights(tri_shape, sphere_frames);
+}
+
 extern "C" void orc_scene_set_points(orc_scene *s, uint32_t n, const float *pts) {
   s->points.clear();
   for (uint32_t i = 0; i < n; ++i) {

@@ -134,6 +134,12 @@ double orc_fresnel(const double *wi, const double *n, double n1, double n2);
  * reference's defaults, support 0 means 2.  Returns 0, or -1 on bad
  * arguments. */
 void orc_scene_set_points(orc_scene *s, uint32_t n, const float *points);
+/* Emitting meshes and spheres (Scene::getLights over every emitting shape):
+ * tri_shape (optional) names each triangle's shape, else triangles that
+ * share an emitting material form one; sphere_frames (optional, 9 floats per
+ * sphere, the images of the object axes) carry Sphere::getPosition's
+ * object-space samples to the world, else 2r times the identity. */
+void orc_scene_set_emitters(orc_scene *s, const uint32_t *tri_shape, const float *sphere_frames);
 int orc_scene_set_env_map(orc_scene *s, uint32_t w, uint32_t h, const float *rgb, float repeat_u, float repeat_v,
                           int filter, uint32_t support, const float *params);
 /* Background radiance seen by a ray leaving the scene in direction d. */

@@ -15,7 +15,7 @@ LIB_PATH = os.path.join(HERE, "liboracle_pt.so")
 MISS = 0xFFFFFFFF
 
 
-INTERFACE_VERSION = 9
+INTERFACE_VERSION = 10
 
 
 def build():
@@ -70,6 +70,7 @@ def lib():
         L.orc_scene_set_env_map.argtypes = [vp, u32, u32, vp, ctypes.c_float, ctypes.c_float, i32, u32, vp]
         L.orc_background.argtypes = [vp, vp, vp]
         L.orc_scene_set_points.argtypes = [vp, u32, vp]
+        L.orc_scene_set_emitters.argtypes = [vp, vp, vp]
         L.orc_render_film.argtypes = [vp, ctypes.POINTER(OrcJob), vp, i32]
         _lib = L
     return _lib
@@ -111,6 +112,12 @@ class OracleScene:
             kd_min_prims, kd_max_depth, 1 if naive else 0, self.threads)
         if not self.h:
             raise ValueError("oracle rejected the scene")
+        ts, sf = getattr(a, "tri_shape", None), getattr(a, "sphere_frames", None)
+        if ts is not None or sf is not None:
+            ts = None if ts is None else np.ascontiguousarray(ts, np.uint32)
+            sf = None if sf is None else np.ascontiguousarray(sf, np.float32)
+            self._keep += [x for x in (ts, sf) if x is not None]
+            lib().orc_scene_set_emitters(self.h, _p(ts), _p(sf))
         pts = getattr(a, "points", None)
         if pts is not None and len(pts):
             pts = np.ascontiguousarray(pts, np.float32).reshape(-1, 6)

@@ -167,7 +167,7 @@ def test_output_tonemap(name, want):
     ('"cone" : { }', "cone"),
     ('"material" : { "emitter" : { "type" : "oriented", "power" : 1 }, "cube" : { } }', "cubes"),
     ('"transform" : { "arbitraryTransform" : [1,0.5,0,0, 0,1,0,0, 0,0,1,0, 0,0,0,1], "cube" : { } }', "sheared"),
-    ('"material" : { "emitter" : { "type" : "oriented", "power" : 1 }, "sphere" : { } }', "spheres"),
+    ('"material" : { "emitter" : { "type" : "oriented", "power" : 1 }, "triangle" : { } }', "triangles"),
     ('"material" : { "emitter" : { "type" : "omni", "power" : 1 }, "plane" : { } }', "omni"),
     ('"transform" : { "scale" : [1, 2, 1], "sphere" : { } }', "ellipsoid"),
     ('"material" : { "bsdf" : { "type" : "velvet" }, "plane" : { } }', "velvet"),
@@ -279,6 +279,10 @@ def test_reference_scene_files():
     le = a.materials[a.quad_material[0]]["Le"]
     np.testing.assert_allclose(le, np.array([2e6, 2e6, 1.6e6]) / (math.pi * 140 * 125), rtol=1e-5)
     assert list(job.camera.eye) == pytest.approx([278, 273, -650])
+    # plates.js: its lights are emitting spheres (area pi, Sphere.cpp:138-141) and planes
+    a, _ = m.load_scenefile(path=os.path.join(REFERENCE_SCENES, "plates.js"))
+    emitting = [a.materials[s]["Le"].max() > 0 for s in a.sphere_material]
+    assert sum(emitting) == 2 and a.sphere_frames is not None
 
 
 def _quad_normals(a):
@@ -331,3 +335,32 @@ def test_cube_hits_closed_form():
     t, prim = osc.trace_closest(rays)
     assert np.allclose(t[:5], [3, 3, 3, 1, 1.25], rtol=1e-12)
     assert (prim[:5] != 0xFFFFFFFF).all() and prim[5] == 0xFFFFFFFF and np.isinf(t[5])
+
+
+def test_emitting_spheres_and_meshes(tmp_path):
+    """An emitting sphere gets Le = P / (pi * pi) (Sphere::_getSurfaceArea is
+    pi whatever the scale, shapes/Sphere.cpp:138-141) and its transform's
+    linear part as its frame; an emitting mesh gets Le = P / (pi A) with A
+    its world-space area (Mesh::_getSurfaceArea, shapes/Mesh.cpp:452-466),
+    and its triangles share one shape id."""
+    obj = tmp_path / "quad.obj"
+    obj.write_text("v 0 0 0\nv 1 0 0\nv 1 1 0\nv 0 1 0\nf 1 2 3 4\n")
+    scene = tmp_path / "s.js"
+    scene.write_text('{ "scenefile" : { "scene" : { '
+                     '"material" : { "emitter" : { "type" : "oriented", "power" : [3, 2, 1] }, '
+                     '"transform" : { "translate" : [0, 2, 0], "rotate" : [0, 0, 0, 1, 0, 0, 90], '
+                     '"scale" : [0.4, 0.4, 0.4], "sphere" : { } } }, '
+                     '"material" : { "emitter" : { "type" : "oriented", "power" : 5 }, '
+                     '"transform" : { "scale" : [2, 3, 1], "mesh" : { "path" : "quad.obj" } } }, '
+                     '"plane" : { } } } }')
+    a, _ = m.load_scenefile(path=str(scene))
+    assert len(a.spheres) == 1 and len(a.indices) == 2
+    le = a.materials[a.sphere_material[0]]["Le"]
+    np.testing.assert_allclose(le, np.array([3, 2, 1]) / math.pi ** 2, rtol=1e-6)
+    np.testing.assert_allclose(a.spheres[0], [0, 2, 0, 0.2], atol=1e-6)
+    # the object axes after a 90 degree turn about x, scaled by 0.4
+    np.testing.assert_allclose(a.sphere_frames[0].reshape(3, 3), [[0.4, 0, 0], [0, 0, 0.4], [0, -0.4, 0]],
+                               atol=1e-6)
+    assert a.tri_shape[0] == a.tri_shape[1]
+    le = a.materials[a.tri_material[0]]["Le"]
+    np.testing.assert_allclose(le, 5 / (math.pi * 6.0), rtol=1e-6)   # the quad scaled to 2 x 3
