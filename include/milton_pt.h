@@ -468,7 +468,10 @@ int mpt_tune_scene(mpt_context *ctx, const mpt_scene_desc *desc, const mpt_kd_pa
 typedef struct mpt_server mpt_server;
 
 typedef struct mpt_server_config {
-  uint64_t hbm_budget;   /* bytes for the films of admitted jobs (0 = 16 GiB) */
+  uint64_t hbm_budget;   /* device bytes for the ray queues of the server's batches and the films of
+                            admitted jobs (0 = 64 GiB).  The queues take at most half: the server's
+                            batch size is lowered until every lane's workspace fits there, and a job
+                            is admitted while its film fits the rest. */
   uint32_t max_active;   /* jobs admitted at once (0 = 256) */
   uint32_t max_inflight; /* batches enqueued ahead of completion (0 = 2) */
 } mpt_server_config;
@@ -502,6 +505,9 @@ int mpt_server_start(mpt_server *server);
 int mpt_server_stop(mpt_server *server);
 /* out: [batches, paths, extension rays, shadow rays, jobs done, peak film bytes] */
 int mpt_server_stats(mpt_server *server, uint64_t out[6]);
+/* out: [budget, bytes reserved for the lanes' ray queues, film bytes now, peak film bytes,
+ * paths per batch, ray-queue bytes the lanes hold now] */
+int mpt_server_memory(mpt_server *server, uint64_t out[6]);
 
 /* Paths per batch the engine plans for a job with direct_samples light
  * samples per area light (0 = 1) on a scene with num_lights area lights and

@@ -195,6 +195,7 @@ def lib():
         "mpt_server_start": ([vp], i32),
         "mpt_server_stop": ([vp], i32),
         "mpt_server_stats": ([vp, ctypes.POINTER(ctypes.c_uint64)], i32),
+        "mpt_server_memory": ([vp, ctypes.POINTER(ctypes.c_uint64)], i32),
         "mpt_tune": ([vp, ctypes.POINTER(Job), u64, ctypes.POINTER(TuneResult)], i32),
         "mpt_tune_scene": ([vp, ctypes.POINTER(SceneDesc), ctypes.POINTER(KdParams), ctypes.POINTER(Job), u64,
                             ctypes.POINTER(vp), ctypes.POINTER(TuneResult)], i32),
@@ -710,6 +711,12 @@ class Server:
         out = (ctypes.c_uint64 * 6)()
         self.ctx.check(self.ctx.L.mpt_server_stats(self.h, out), "mpt_server_stats")
         return dict(zip(["batches", "paths", "extension_rays", "shadow_rays", "jobs_done", "peak_film_bytes"],
+                        list(out)))
+
+    def memory(self):
+        out = (ctypes.c_uint64 * 6)()
+        self.ctx.check(self.ctx.L.mpt_server_memory(self.h, out), "mpt_server_memory")
+        return dict(zip(["budget", "queue_bytes", "film_bytes", "peak_film_bytes", "batch_paths", "lane_bytes"],
                         list(out)))
 
     def close(self):

@@ -157,6 +157,7 @@ struct BatchPlan {
   uint32_t n_paths = 0, n_pixels = 0, max_depth = 0;
   uint32_t max_direct = 1;   // most light samples per light of its jobs
   bool filtered = false;     // some job reconstructs its film through a filter
+  uint32_t capacity = 0;     // paths the workspace is sized for (0: the engine's batch capacity)
 };
 
 }  // namespace mpt_engine
@@ -235,6 +236,9 @@ int ensure_workspace(mpt_context *ctx, uint32_t k, uint32_t cap, uint32_t depth,
 // batch's jobs: the context's batch size, lowered so that the shadow rays
 // of one batch (paths x shadow rays per vertex) fit kShadowRayBudget.
 uint32_t batch_capacity(const mpt_context *ctx, const mpt_scene *s, uint32_t max_direct);
+// Device bytes of one lane's workspace for batches of `cap` paths, `depth`
+// bounces and `shadow` shadow rays (what ensure_workspace allocates).
+uint64_t workspace_bytes(uint64_t cap, uint32_t depth, uint64_t shadow);
 
 // Enqueues one batch on lane `lane` (asynchronous).  With L_host the
 // per-path radiance of the batch is copied to L_host (slot order) and the

@@ -1382,6 +1382,14 @@ uint32_t batch_capacity(const mpt_context *ctx, const mpt_scene *s, uint32_t max
   return (uint32_t)mpt_batch_paths_for(ctx->batch_paths, s->dev.num_lights, s->dev.num_points, max_direct);
 }
 
+uint64_t workspace_bytes(uint64_t cap, uint32_t depth, uint64_t shadow) {
+  // q[2], hits, sg, sr, Ls, flags, pjs, pos, counts, totals, desc
+  return cap * (2 * 3 * sizeof(float4) + sizeof(float2) + 2 * sizeof(float4) + sizeof(float4) + sizeof(uint32_t) +
+                sizeof(uint2) + sizeof(float2)) +
+         shadow * 2 * sizeof(float4) + (3ull * depth + 1) * sizeof(uint32_t) + 8 * sizeof(unsigned long long) +
+         sizeof(BatchDesc);
+}
+
 int ensure_lane(mpt_context *ctx, uint32_t k) {
   Lane &ln = ctx->lanes[k];
   if (ln.stream) return MPT_OK;
@@ -1512,8 +1520,8 @@ int run_batch(mpt_context *ctx, mpt_scene *s, const BatchPlan &plan, float *L_ho
   // numbered 1 + light * samples + sample below 256
   const uint32_t shadow_per_vertex = s->dev.num_lights * std::max(1u, plan.max_direct) + s->dev.num_points;
   if (shadow_per_vertex > 254) return fail(ctx, MPT_E_INVALID, "lights x direct_samples above 254");
-  int rc = ensure_workspace(ctx, lane, std::max(plan.n_paths, batch_capacity(ctx, s, plan.max_direct)),
-                            plan.max_depth, shadow_per_vertex);
+  const uint32_t cap = plan.capacity ? plan.capacity : batch_capacity(ctx, s, plan.max_direct);
+  int rc = ensure_workspace(ctx, lane, std::max(plan.n_paths, cap), plan.max_depth, shadow_per_vertex);
   if (rc) return rc;
   Lane &ln = ctx->lanes[lane];
   if ((rc = stage_plan(ctx, ln, plan))) return rc;

@@ -5,8 +5,11 @@
 // producer thread and N consumer threads over one film and returns when it is
 // done (renderers/PointSampleRenderer.cpp:24-81).  Here a server keeps
 // scenes resident and runs continuous batching:
-//   admission   queued jobs are admitted in arrival order while their films
-//               fit the HBM budget (and the active-job limit);
+//   admission   the HBM budget covers the lanes' ray queues and the films:
+//               the server's batch size is set so that every lane's
+//               workspace fits half the budget, and queued jobs are
+//               admitted in arrival order while their films fit the rest
+//               (and the active-job limit);
 //   packing     a round picks the scene of the oldest admitted job with work
 //               left and fills one batch with whole-pixel chunks of every
 //               admitted job on that scene, oldest first;
@@ -66,6 +69,8 @@ struct mpt_server {
   std::vector<hipEvent_t> free_events;
   uint64_t next_handle = 1;
   uint64_t film_bytes = 0, peak_film_bytes = 0;
+  uint32_t cap = 0;           // paths per batch (the lanes' workspaces fit queue_bytes)
+  uint64_t queue_bytes = 0;   // device bytes reserved for the lanes' ray queues
   uint64_t batches = 0, paths = 0, jobs_done = 0;
   std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now();
   std::thread worker;
@@ -95,7 +100,7 @@ void admit(mpt_server *sv) {
       sv->queued.pop_front();
       continue;
     }
-    if (sv->film_bytes + j->film_bytes() > sv->cfg.hbm_budget) break;
+    if (sv->queue_bytes + sv->film_bytes + j->film_bytes() > sv->cfg.hbm_budget) break;
     if (j->film.reserve(j->npix) != hipSuccess) break;
     // a filtered film is accumulated into, so it starts at zero; the clear
     // goes on the context stream, which every lane waits for before a batch
@@ -190,8 +195,11 @@ int round(mpt_server *sv, std::unique_lock<std::mutex> &lk, uint32_t *ran) {
     if (j->scene != scene) continue;
     if (plan.jobs.size() == kMaxJobs || plan.chunks.size() == kMaxChunks) break;
     const uint32_t spp = j->job.spp;
-    // a job with more light samples per vertex lowers the batch's capacity
-    const uint32_t cap = batch_capacity(ctx, scene, std::max(plan.max_direct, std::max(1u, j->job.direct_samples)));
+    // a job with more light samples per vertex lowers the batch's capacity:
+    // its shadow rays (paths x light samples) stay within the lane's queue
+    const uint32_t md = std::max(plan.max_direct, std::max(1u, j->job.direct_samples));
+    const uint32_t spv = std::max(1u, scene->dev.num_lights * md + scene->dev.num_points);
+    const uint32_t cap = std::max(1u, std::min(std::min(sv->cap, batch_capacity(ctx, scene, md)), sv->cap / spv));
     const uint64_t room = plan.n_paths >= cap ? 0 : (cap - plan.n_paths) / spp;
     if (room == 0) {
       if (plan.n_paths == 0) {
@@ -220,6 +228,8 @@ int round(mpt_server *sv, std::unique_lock<std::mutex> &lk, uint32_t *ran) {
     if (plan.n_paths >= cap) break;
   }
   if (plan.chunks.empty()) return MPT_OK;
+  plan.capacity = std::max(1u, std::min(sv->cap, sv->cap / std::max(1u, scene->dev.num_lights * plan.max_direct +
+                                                                              scene->dev.num_points)));
   while (sv->inflight.size() >= sv->cfg.max_inflight) retire_one(sv, lk);
   // Batches alternate over the context's lanes; a lane first waits for the
   // context stream, where new films are cleared.
@@ -294,13 +304,22 @@ int mpt_server_create(mpt_context *ctx, const mpt_server_config *cfg, mpt_server
   if (!sv) return MPT_E_NOMEM;
   sv->ctx = ctx;
   if (cfg) sv->cfg = *cfg;
-  if (!sv->cfg.hbm_budget) sv->cfg.hbm_budget = 16ull << 30;
+  if (!sv->cfg.hbm_budget) sv->cfg.hbm_budget = 64ull << 30;
   if (!sv->cfg.max_active) sv->cfg.max_active = 256;
   // two batches in flight per lane by default; at most three per lane, since
   // each lane's descriptor staging ring holds four
   const uint32_t lanes = std::max(1u, std::min(ctx->num_lanes, kMaxLanes));
   if (!sv->cfg.max_inflight) sv->cfg.max_inflight = 2 * lanes;
   sv->cfg.max_inflight = std::min<uint32_t>(sv->cfg.max_inflight, 3 * lanes);
+  // The lanes' ray queues take at most half the budget: halve the batch
+  // until every lane's workspace (batches up to depth 64, one shadow ray per
+  // path) fits there.
+  uint64_t cap = ctx->batch_paths;
+  while (cap > kMinBatchPaths && lanes * workspace_bytes(cap, 64, cap) > sv->cfg.hbm_budget / 2) cap /= 2;
+  if (lanes * workspace_bytes(cap, 64, cap) > sv->cfg.hbm_budget / 2)
+    return fail(ctx, MPT_E_BUDGET, "HBM budget too small for the ray queues of the server's lanes");
+  sv->cap = (uint32_t)cap;
+  sv->queue_bytes = lanes * workspace_bytes(cap, 64, cap);
   if (reset_totals(ctx)) return MPT_E_DEVICE;
   ctx->servers.push_back(sv.get());
   *out = sv.release();
@@ -331,7 +350,8 @@ int mpt_submit(mpt_server *sv, mpt_scene *scene, const mpt_job *job, uint64_t *h
   j->scene = scene;
   j->job = *job;
   j->npix = (uint64_t)job->width * job->height;
-  if (j->film_bytes() > sv->cfg.hbm_budget) return fail(sv->ctx, MPT_E_BUDGET, "film exceeds the HBM budget");
+  if (sv->queue_bytes + j->film_bytes() > sv->cfg.hbm_budget)
+    return fail(sv->ctx, MPT_E_BUDGET, "film exceeds the HBM budget left beside the ray queues");
   std::lock_guard<std::mutex> lk(sv->mu);
   j->handle = sv->next_handle++;
   j->submit_ms = sv->now_ms();
@@ -450,6 +470,20 @@ int mpt_server_stop(mpt_server *sv) {
   }
   sv->worker.join();
   sv->running = false;
+  return MPT_OK;
+}
+
+int mpt_server_memory(mpt_server *sv, uint64_t out[6]) {
+  if (!sv || !out) return MPT_E_INVALID;
+  std::lock_guard<std::mutex> lk(sv->mu);
+  out[0] = sv->cfg.hbm_budget;
+  out[1] = sv->queue_bytes;
+  out[2] = sv->film_bytes;
+  out[3] = sv->peak_film_bytes;
+  out[4] = sv->cap;
+  uint64_t held = 0;
+  for (const Lane &ln : sv->ctx->lanes) held += ln.ws.bytes();
+  out[5] = held;
   return MPT_OK;
 }
 

@@ -56,11 +56,20 @@ def test_mixed_batches_match_alone(ctx, scenes):
 
 
 def test_budget_admission_and_cancel(ctx, scenes):
+    """The HBM budget covers the lanes' ray queues and the films: the queues
+    take at most half of it, and jobs are admitted while their films fit the
+    rest.  Films here are sized so that at most two fit at once."""
     (room, sroom), _ = scenes
-    job = m.make_job(room.camera, 64, 64, 4, 4, seed=3)
-    film_bytes = 64 * 64 * 16
-    srv = m.Server(ctx, hbm_budget=2 * film_bytes)
-    big = m.make_job(room.camera, 128, 128, 4, 4, seed=3)
+    budget = 64 << 20
+    srv = m.Server(ctx, hbm_budget=budget)
+    mem = srv.memory()
+    assert 0 < mem["queue_bytes"] <= budget // 2 and mem["batch_paths"] >= 1024
+    avail = budget - mem["queue_bytes"]
+    side = int((avail / 2.5 / 16) ** 0.5)
+    film_bytes = side * side * 16
+    assert 2 * film_bytes <= avail < 3 * film_bytes
+    job = m.make_job(room.camera, side, side, 1, 3, seed=3)
+    big = m.make_job(room.camera, 2 * side, 2 * side, 1, 3, seed=3)
     with pytest.raises(m.MptError, match="budget"):
         srv.submit(sroom, big)
     handles = [srv.submit(sroom, job) for _ in range(5)]
@@ -70,17 +79,48 @@ def test_budget_admission_and_cancel(ctx, scenes):
     states = [srv.status(h)["state"] for h in handles]
     assert states.count(m.JOB_RUNNING) + states.count(m.JOB_DONE) <= 2
     srv.drain()
-    st = srv.stats()
-    assert st["peak_film_bytes"] <= 2 * film_bytes
-    ref = sroom.render(job)[0]
+    mem = srv.memory()
+    print(f"[budget] {mem}", flush=True)
+    assert mem["peak_film_bytes"] <= 2 * film_bytes
+    assert mem["peak_film_bytes"] + mem["queue_bytes"] <= budget
+    results = []
     for h in handles[:4]:
         s = srv.status(h)
         assert s["state"] == m.JOB_DONE and s["done_ms"] >= s["start_ms"] >= s["submit_ms"]
-        assert np.array_equal(srv.result(h, 64, 64), ref)
+        results.append(srv.result(h, side, side))
     assert srv.status(handles[4])["state"] == m.JOB_CANCELLED
     with pytest.raises(m.MptError, match="cancelled"):
-        srv.result(handles[4], 64, 64)
+        srv.result(handles[4], side, side)
     srv.close()
+    ref = sroom.render(job)[0]
+    assert all(np.array_equal(r, ref) for r in results)
+
+
+def test_queue_memory_stays_in_budget():
+    """On a fresh context, the lanes' workspaces after serving stay within
+    the bytes the server reserved for them, also for jobs with several light
+    samples per vertex (their batches hold fewer paths)."""
+    c = m.Context(0)
+    try:
+        a = m.generate(m.MPT_SCENE_ROOM)
+        a.points = np.array([[0.3, 0.7, 0.4, 0.8, 0.7, 0.5]], np.float32)
+        sc = m.Scene(c, a)
+        srv = m.Server(c, hbm_budget=96 << 20)
+        jobs = []
+        for i in range(6):
+            j = m.make_job(a.camera, 96, 80, 16, 5, seed=40 + i, job_id=i)
+            j.direct_samples = 1 + 3 * (i % 2)
+            jobs.append(j)
+        hs = [srv.submit(sc, j) for j in jobs]
+        srv.drain()
+        mem = srv.memory()
+        print(f"[queues] {mem}", flush=True)
+        assert mem["lane_bytes"] <= mem["queue_bytes"]
+        films = [srv.result(h, j.width, j.height) for h, j in zip(hs, jobs)]
+        srv.close()
+        assert all(np.array_equal(f, sc.render(j)[0]) for f, j in zip(films, jobs))
+    finally:
+        c.close()
 
 
 def test_worker_thread_serving(ctx, scenes):
