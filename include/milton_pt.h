@@ -264,9 +264,13 @@ enum {
                               blocks per CU; 1 (default) persistent waves that refill finished
                               lanes from 128-ray chunks of the queue each wave claims; 2 one ray
                               per lane, one grid slot per queue slot (also the shadow-ray form) */
-  MPT_OPT_LANES = 7        /* streams (each with its own batch workspace) that consecutive batches
+  MPT_OPT_LANES = 7,       /* streams (each with its own batch workspace) that consecutive batches
                               alternate over, 1..4, default 3: one batch's kernel tails overlap
                               another's bulk */
+  MPT_OPT_FAIL_BATCH = 8   /* failure injection for tests of the callers' error paths: n > 0 makes
+                              the n-th batch started from now on fail with MPT_E_DEVICE after its
+                              descriptors are staged and before its kernels are enqueued (the
+                              setting then returns to 0); 0 (default) off */
 };
 
 typedef struct mpt_context mpt_context;
@@ -279,6 +283,9 @@ int mpt_open(int device, mpt_context **ctx);
 /* Closes the context and destroys the scenes and films still open on it;
  * their handles become invalid. */
 int mpt_close(mpt_context *ctx);
+/* The message of the context's last failure, from any thread (a server's
+ * worker records its batches' failures here too); the string is the calling
+ * thread's copy, valid until that thread calls mpt_last_error again. */
 const char *mpt_last_error(const mpt_context *ctx);
 int mpt_abi_version(void);
 

@@ -14,6 +14,8 @@
 #include <stdint.h>
 
 #include <algorithm>
+#include <atomic>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -169,6 +171,7 @@ struct mpt_context {
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   int num_cus = 256;
+  std::mutex err_mu;                 // err is written by a server's worker thread too
   std::string err;
   mpt_engine::Lane lanes[mpt_engine::kMaxLanes];
   uint32_t num_lanes = 3;            // lanes batches alternate over
@@ -181,6 +184,7 @@ struct mpt_context {
   uint32_t stack = 8;                // short-stack entries of the traversal kernels
   uint32_t persistent = 1;           // closest-hit kernel form: 0 grid-stride, 1 persistent waves with
                                      // lane refill, 2 one ray per lane over a capacity-sized grid
+  std::atomic<uint32_t> fail_batch{0};   // MPT_OPT_FAIL_BATCH countdown (0: off); a server's worker reads it
   std::vector<hipEvent_t> events;    // timing pool, pairs per launch
   std::vector<std::pair<int, size_t>> timed;   // instrumented launches: (kind, first event) since the render began
   std::vector<mpt_scene *> scenes;   // live scenes, films and servers, released by mpt_close

@@ -1271,7 +1271,10 @@ void drop_graphs(mpt_context *ctx, uint64_t scene_uid) {
 }
 
 int fail(mpt_context *ctx, int code, const std::string &msg) {
-  if (ctx) ctx->err = msg;
+  if (ctx) {
+    std::lock_guard<std::mutex> lk(ctx->err_mu);
+    ctx->err = msg;
+  }
   return code;
 }
 
@@ -1525,6 +1528,12 @@ int run_batch(mpt_context *ctx, mpt_scene *s, const BatchPlan &plan, float *L_ho
   if (rc) return rc;
   Lane &ln = ctx->lanes[lane];
   if ((rc = stage_plan(ctx, ln, plan))) return rc;
+  // injected failure: count this batch down, and fail it when the count ends
+  for (uint32_t n = ctx->fail_batch.load(); n != 0;)
+    if (ctx->fail_batch.compare_exchange_weak(n, n - 1)) {
+      if (n == 1) return fail(ctx, MPT_E_DEVICE, "injected batch failure (MPT_OPT_FAIL_BATCH)");
+      break;
+    }
   const uint32_t D = plan.max_depth;
   const bool multi = plan.max_direct > 1;
   bool filtered = false;
@@ -2072,7 +2081,15 @@ int mpt_close(mpt_context *ctx) {
   return MPT_OK;
 }
 
-const char *mpt_last_error(const mpt_context *ctx) { return ctx ? ctx->err.c_str() : "no context"; }
+// The message is copied per calling thread, so a server's worker thread
+// recording a failure does not change a string the caller is reading.
+const char *mpt_last_error(const mpt_context *ctx) {
+  if (!ctx) return "no context";
+  thread_local std::string copy;
+  std::lock_guard<std::mutex> lk(const_cast<mpt_context *>(ctx)->err_mu);
+  copy = ctx->err;
+  return copy.c_str();
+}
 
 int mpt_set_option(mpt_context *ctx, uint32_t option, uint64_t value) {
   if (!ctx) return MPT_E_INVALID;
@@ -2100,6 +2117,10 @@ int mpt_set_option(mpt_context *ctx, uint32_t option, uint64_t value) {
       ctx->num_lanes = (uint32_t)value;
       ctx->next_lane = 0;
       return MPT_OK;
+    case MPT_OPT_FAIL_BATCH:
+      if (value > 0xffffffffull) return fail(ctx, MPT_E_INVALID, "fail_batch out of range");
+      ctx->fail_batch.store((uint32_t)value);
+      return MPT_OK;
     case MPT_OPT_STACK:
       if (value != 8 && value != 12 && value != 16 && value != 24)
         return fail(ctx, MPT_E_INVALID, "stack depth must be 8, 12, 16 or 24");
@@ -2121,6 +2142,7 @@ int mpt_get_option(mpt_context *ctx, uint32_t option, uint64_t *value) {
     case MPT_OPT_STACK: *value = ctx->stack; return MPT_OK;
     case MPT_OPT_PERSISTENT: *value = ctx->persistent; return MPT_OK;
     case MPT_OPT_LANES: *value = ctx->num_lanes; return MPT_OK;
+    case MPT_OPT_FAIL_BATCH: *value = ctx->fail_batch.load(); return MPT_OK;
     default: return fail(ctx, MPT_E_INVALID, "unknown option");
   }
 }

@@ -247,6 +247,9 @@ int round(mpt_server *sv, std::unique_lock<std::mutex> &lk, uint32_t *ran) {
     if (!rc && hipEventRecord(ev, ctx->lanes[lane].stream) != hipSuccess) rc = MPT_E_DEVICE;
   }
   if (rc) {
+    // whatever the failed batch did enqueue finishes before its films are
+    // released (their memory may be handed to the next admitted job)
+    (void)hipStreamSynchronize(ctx->lanes[lane].stream);
     for (auto &part : parts) {
       Job *j = part.first;
       --j->inflight;

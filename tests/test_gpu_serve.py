@@ -145,3 +145,51 @@ def test_worker_thread_serving(ctx, scenes):
     finally:
         srv.stop()
         srv.close()
+
+
+@pytest.mark.parametrize("worker", [False, True], ids=["inline", "worker"])
+def test_injected_batch_failure(ctx, scenes, worker):
+    """A batch that fails (MPT_OPT_FAIL_BATCH makes the third batch from now
+    on fail before its kernels are enqueued) fails the jobs it carried and
+    only those: the others finish with the same films as alone, their films
+    are released, and the server takes new jobs afterwards."""
+    (room, sroom), _ = scenes
+    jobs = [m.make_job(room.camera, 48, 40, 4, 4, seed=60 + i, job_id=i) for i in range(6)]
+    alone = [sroom.render(j)[0] for j in jobs]
+    saved = ctx.get_option(m.MPT_OPT_BATCH_PATHS)
+    ctx.set_option(m.MPT_OPT_BATCH_PATHS, 4096)   # about two jobs per batch
+    try:
+        srv = m.Server(ctx)
+        ctx.set_option(m.MPT_OPT_FAIL_BATCH, 3)
+        if worker:
+            srv.start()
+        handles = [srv.submit(sroom, j) for j in jobs]
+        if worker:
+            for h in handles:
+                assert srv.wait(h, 120000)
+        else:
+            with pytest.raises(m.MptError, match="injected"):
+                srv.drain()
+        assert ctx.get_option(m.MPT_OPT_FAIL_BATCH) == 0
+        states = [srv.status(h)["state"] for h in handles]
+        print(f"[failure {'worker' if worker else 'inline'}] states={states}", flush=True)
+        assert states.count(m.JOB_FAILED) >= 1 and states.count(m.JOB_DONE) >= 1
+        assert set(states) <= {m.JOB_FAILED, m.JOB_DONE}
+        for h, j, ref, s in zip(handles, jobs, alone, states):
+            if s == m.JOB_DONE:
+                assert np.array_equal(srv.result(h, j.width, j.height), ref)
+            else:
+                assert srv.status(h)["error"] == 2   # -MPT_E_DEVICE
+                with pytest.raises(m.MptError):
+                    srv.result(h, j.width, j.height)
+        if worker:
+            srv.stop()
+        assert srv.memory()["film_bytes"] == 0
+        # the server keeps serving
+        h = srv.submit(sroom, jobs[0])
+        srv.drain()
+        assert np.array_equal(srv.result(h, jobs[0].width, jobs[0].height), alone[0])
+        srv.close()
+    finally:
+        ctx.set_option(m.MPT_OPT_FAIL_BATCH, 0)
+        ctx.set_option(m.MPT_OPT_BATCH_PATHS, saved)
