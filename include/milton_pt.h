@@ -44,7 +44,7 @@
 extern "C" {
 #endif
 
-#define MPT_ABI_VERSION 5u
+#define MPT_ABI_VERSION 6u
 #define MPT_MISS 0xFFFFFFFFu
 
 enum {
@@ -222,6 +222,18 @@ typedef struct mpt_job {
    * efficientDirect = true): emission only at depth 0 or after a reflective
    * specular bounce, next-event estimation at every non-specular vertex. */
   uint32_t direct_off;
+  /* Next-event estimation's choice of lights.  0 (the reference's
+   * DirectIllumination, renderers/DirectIllumination.cpp:90-135): every
+   * light gets direct_samples shadow rays at each vertex.  1: each of the
+   * direct_samples shadow rays goes to one light drawn with probability
+   * proportional to its emitted power, as the scene's EmitterSampler draws
+   * lights (stats/samplers/EmitterSampler.cpp:19-61: power averaged over the
+   * channels, the normalized powers walked in order until their sum reaches
+   * the uniform), its contribution divided by that probability; point lights
+   * are drawn with the area lights.  Same expectation, direct_samples shadow
+   * rays per vertex however many lights the scene has.  With one light and
+   * no point light both choices give the same paths. */
+  uint32_t light_select;
 } mpt_job;
 
 typedef struct mpt_ray {
@@ -327,6 +339,38 @@ int mpt_trace_closest_stats(mpt_scene *scene, const mpt_ray *rays, uint64_t n, m
  * (core/Scene.cpp:110-114) -> kdTreeAccel::intersects (:1018-1094). */
 int mpt_trace_any(mpt_scene *scene, const mpt_ray *rays, const float *tmax, uint64_t n,
                   uint8_t *occluded);
+
+/* One emitter point per sample, as EmitterSampler::sample and ::getPd draw
+ * them for a light subpath's first vertex (stats/samplers/EmitterSampler.cpp:
+ * 51-96, called from renderers/utils/Path.cpp:233-236): a light drawn by
+ * emitted power (the rule of mpt_job.light_select), then a point from that
+ * shape's getRandomPoint:
+ *   quad    Shape::getRandomPoint (shapes/Shape.cpp:45-50): Plane::getPosition
+ *           at two uniforms;
+ *   mesh    Shape::getRandomPoint with Mesh::getPosition (shapes/Mesh.cpp:
+ *           442-450): a triangle by u, barycentrics (r1, v, r2) / sum;
+ *   sphere  Sphere::getRandomPoint (shapes/Sphere.cpp:107-115): a uniform
+ *           point s on the unit sphere, its (u, v) from Sphere::_getUV
+ *           (u = 1 - atan2(s_z, s_x) / 2pi, v = 1/2 - asin(2 s_y) / pi), then
+ *           Sphere::getPosition(u, v) through the sphere's transform.  For
+ *           |s_y| > 1/2 (half of the draws) asin's argument leaves [-1, 1] and,
+ *           as in the reference, position and normal are NaN;
+ *   point   its position, normal 0 (PointShape has none).
+ * pdf_area is getPd's value: 1 / (sum of the lights' surface areas), the
+ * same for every sample (points have area 0; 1 if the sum is 0);
+ * pdf_select the drawn light's probability.  Light indices number the
+ * scene's area lights (emitting quads, spheres, then meshes) followed by its
+ * point lights.  Sample i uses Philox keyed by (seed, i): runs are
+ * reproducible.  MPT_E_INVALID for a scene without lights. */
+typedef struct mpt_emitter_point {
+  float p[3];
+  uint32_t light;
+  float n[3];
+  float pdf_area;
+  float pdf_select;
+  uint32_t pad;
+} mpt_emitter_point;
+int mpt_emitter_sample(mpt_scene *scene, uint64_t n, uint32_t seed, mpt_emitter_point *out);
 
 /* Renders a job and copies the film (numerator / denominator per pixel,
  * RGB float, row-major) to film_rgb (W*H*3 floats).  Replaces

@@ -19,7 +19,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 # compiled with different kernel tunables); the in-tree build by default.
 LIB_PATH = os.environ.get("MPT_LIBRARY") or os.path.join(_HERE, "libmilton_pt.so")
 
-MPT_ABI_VERSION = 5
+MPT_ABI_VERSION = 6
 MPT_TRI_ONE_SIDED = 1
 MPT_MISS = 0xFFFFFFFF
 MPT_BSDF_DIFFUSE, MPT_BSDF_PHONG, MPT_BSDF_DIELECTRIC = 0, 1, 2
@@ -84,7 +84,11 @@ class Job(ctypes.Structure):
                 ("spp", ctypes.c_uint32), ("max_depth", ctypes.c_uint32),
                 ("seed", ctypes.c_uint32), ("job_id", ctypes.c_uint32), ("direct_samples", ctypes.c_uint32),
                 ("filter", ctypes.c_uint32), ("filter_support", ctypes.c_uint32), ("filter_params", ctypes.c_float * 2),
-                ("direct_off", ctypes.c_uint32)]
+                ("direct_off", ctypes.c_uint32), ("light_select", ctypes.c_uint32)]
+
+
+EMITTER_POINT_DTYPE = np.dtype([("p", np.float32, 3), ("light", np.uint32), ("n", np.float32, 3),
+                                ("pdf_area", np.float32), ("pdf_select", np.float32), ("pad", np.uint32)])
 
 
 class Ray(ctypes.Structure):
@@ -167,6 +171,7 @@ def lib():
         "mpt_scene_stats": ([vp, ctypes.POINTER(ctypes.c_uint64)], i32),
         "mpt_trace_closest": ([vp, vp, u64, vp], i32),
         "mpt_trace_any": ([vp, vp, vp, u64, vp], i32),
+        "mpt_emitter_sample": ([vp, u64, u32, vp], i32),
         "mpt_trace_closest_stats": ([vp, vp, u64, vp, ctypes.POINTER(ctypes.c_uint64)], i32),
         "mpt_render": ([vp, ctypes.POINTER(Job), vp, ctypes.POINTER(RenderStats)], i32),
         "mpt_film_create": ([vp, u32, u32, ctypes.POINTER(vp)], i32),
@@ -529,6 +534,12 @@ class Scene:
         self.ctx.check(self.ctx.L.mpt_trace_any(self.h, _ptr(rays), _ptr(tmax), len(rays), _ptr(occ)),
                        "mpt_trace_any")
         return occ
+
+    def emitter_sample(self, n, seed):
+        """EmitterSampler draws on the device (mpt_emitter_sample)."""
+        out = np.zeros(n, EMITTER_POINT_DTYPE)
+        self.ctx.check(self.ctx.L.mpt_emitter_sample(self.h, n, seed, _ptr(out)), "mpt_emitter_sample")
+        return out
 
     def render(self, job):
         film = np.zeros((job.height, job.width, 3), np.float32)

@@ -79,6 +79,10 @@ struct Scene {
   // point lights: (position, 0), (radiance P / (4 pi), 0) per light
   const float4 *points;
   uint32_t num_points;
+  // emitted power of the area lights then the point lights, normalized:
+  // (sum through this light, this light's share) per light
+  const float2 *light_sel;
+  float emit_area;   // the area lights' summed surface area
 };
 
 MPT_HD float3 f3(float x, float y, float z) { return make_float3(x, y, z); }

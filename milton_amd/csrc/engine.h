@@ -48,7 +48,8 @@ struct JobDev {
   uint32_t filter_support;
   float filter_p0, filter_p1;
   uint32_t direct_off;   // efficientDirect = false: emission at every vertex, no NEE
-  uint32_t pad[3];
+  uint32_t light_select; // 1: each light sample draws its light by emitted power
+  uint32_t pad[2];
 };
 
 struct ChunkDev {
@@ -202,6 +203,7 @@ struct mpt_scene {
   mpt_engine::DBuf<uint32_t> sphere_mat;
   mpt_engine::DBuf<mpt_dev::LightDev> lights;
   mpt_engine::DBuf<float4> light_tris;
+  mpt_engine::DBuf<float2> light_sel;
   mpt_engine::DBuf<mpt_material> mats;
   mpt_engine::DBuf<float4> env_tex, points;
   mpt_engine::DBuf<double> env_kernel;
@@ -210,7 +212,7 @@ struct mpt_scene {
   uint32_t leaf_size = 0;   // kd leaf size the tree was built with
   uint64_t device_bytes() const {
     return nodes.bytes() + recs.bytes() + tshade.bytes() + spheres.bytes() + quads.bytes() + sphere_mat.bytes() +
-           lights.bytes() + light_tris.bytes() + mats.bytes() + env_tex.bytes() + env_kernel.bytes();
+           lights.bytes() + light_tris.bytes() + light_sel.bytes() + mats.bytes() + env_tex.bytes() + env_kernel.bytes();
   }
 };
 

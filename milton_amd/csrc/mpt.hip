@@ -444,6 +444,48 @@ __device__ __forceinline__ float3 env_radiance(const Scene &sc, float3 dir) {
   return env_radiance_at(sc.env_tex, sc.env_kernel, sc.env_w, sc.env_h, sc.env_m, sc.env_rep_u, sc.env_rep_v, dir);
 }
 
+// Shape::getPoint (shapes/Shape.cpp:52-64) of an area light at (u, v), with
+// r1, r2 the two further uniforms a mesh triangle takes: the light point,
+// its normal, and the primitive a shadow ray to it may skip (a planar
+// light's only crossing is the end point; kMiss for a sphere).
+__device__ __forceinline__ void light_point(const Scene &sc, const LightDev &ld, float u, float v, float r1, float r2,
+                                            float3 &lp, float3 &ln, uint32_t &target) {
+  target = kMiss;
+  if (ld.kind == LIGHT_QUAD) {
+    // Plane::getPosition (shapes/Plane.cpp:91-93)
+    const float4 *lq = sc.quads + 4ull * ld.index;
+    const float4 c0 = lq[0], ex = lq[1], ez = lq[2], qn = lq[3];
+    lp = add(xyz(c0), add(mul(xyz(ex), u - 0.5f), mul(xyz(ez), v - 0.5f)));
+    ln = xyz(qn);
+    target = sc.nt + sc.ns + ld.index;
+  } else if (ld.kind == LIGHT_SPHERE) {
+    // Sphere::getPosition (shapes/Sphere.cpp:117-124): an object-space point
+    // at distance 1 from the centre near +z, mapped by the sphere's
+    // transform; the normal is that point's direction (Sphere.cpp:86-92, a
+    // similarity transform)
+    const float x = fminf(0.5f, fmaxf(-0.5f, tanf((1.0f - u) * (0.5f / kPi))));
+    const float y = fminf(0.5f, fmaxf(-0.5f, 0.5f * sinf((0.5f - v) * (1.0f / kPi))));
+    const float z = sqrtf(1.0f - x * x - y * y);
+    const float3 q = add(add(mul(xyz(ld.frame[0]), x), mul(xyz(ld.frame[1]), y)), mul(xyz(ld.frame[2]), z));
+    lp = add(xyz(sc.spheres[ld.index]), q);
+    ln = normalize(q);
+  } else {
+    // Mesh::getPosition (shapes/Mesh.cpp:442-450): the triangle picked by u,
+    // barycentrics (r1, v, r2) / (r1 + v + r2) (MeshTriangle.cpp:299-333);
+    // the normal interpolates the vertex normals there
+    // (Mesh::_getGeometricNormal, :419-433)
+    const uint32_t ti = ld.first + min((uint32_t)floor((double)u * ld.count), ld.count - 1u);
+    const float4 *lt = sc.light_tris + 3ull * ti;
+    const float4 t0 = lt[0], t1 = lt[1], t2 = lt[2];
+    const float ks = 1.0f / (r1 + v + r2);
+    const float b0 = r1 * ks, b1 = v * ks, b2 = r2 * ks;
+    lp = add(add(mul(xyz(t0), b0), mul(xyz(t1), b1)), mul(xyz(t2), b2));
+    target = __float_as_uint(t0.w);
+    const float4 *ts = sc.tshade + 3ull * target;
+    ln = normalize(add(add(mul(xyz(ts[0]), b0), mul(xyz(ts[1]), b1)), mul(xyz(ts[2]), b2)));
+  }
+}
+
 #ifndef MPT_SHADE_BLOCK
 #define MPT_SHADE_BLOCK 256
 #endif
@@ -455,8 +497,9 @@ constexpr int kShadeBlock = MPT_SHADE_BLOCK;   // threads per shading block
 #define MPT_SHADE_WAVES 5
 #endif
 
-// kMulti: some job of the batch draws several light samples per light; the
-// common single-sample form keeps the NEE loop's counters out of registers.
+// kMulti: some job of the batch draws several light samples per light, or
+// draws its lights by power; the common single-sample form keeps the NEE
+// loop's counters out of registers.
 // kEnv: the scene's background is an environment image (a constant
 // otherwise).
 //
@@ -496,6 +539,7 @@ __global__ __launch_bounds__(kShadeBlock) __attribute__((amdgpu_waves_per_eu(kMu
     uint32_t pixel = 0, sample = 0, seed = 0, job_id = 0, max_depth = 0;
     uint32_t ndir_job = 1;   // light samples per light
     bool direct_off = false; // efficientDirect = false
+    bool select = false;     // light samples drawn by emitted power
     if (active) {
       // (origin, slot) (direction, origin primitive) (throughput, flags)
       const float4 a = qin[3ull * i], b = qin[3ull * i + 1], c = qin[3ull * i + 2];
@@ -511,6 +555,7 @@ __global__ __launch_bounds__(kShadeBlock) __attribute__((amdgpu_waves_per_eu(kMu
       max_depth = job.max_depth;
       if (kMulti) ndir_job = job.direct;
       direct_off = job.direct_off != 0;
+      if (kMulti) select = job.light_select != 0;
       beta = xyz(c);
       fl = __float_as_uint(c.w);
       const uint32_t rec = __float_as_uint(h.y);
@@ -569,9 +614,11 @@ __global__ __launch_bounds__(kShadeBlock) __attribute__((amdgpu_waves_per_eu(kMu
     // Next-event estimation (DirectIllumination.cpp:90-135): per light, n
     // shadow rays to points jittered over floor(sqrt(n))^2 strata of the
     // light, each weighted area / n; skipped at specular vertices (:57) and
-    // on the light itself (:93).  The rays go to the slot's own run of the
-    // shadow buffer in (light, sample) order; one group entry per vertex
-    // points k_shadow at them.
+    // on the light itself (:93).  With light selection by power, each of the
+    // n rays goes to one light drawn from the emitted-power table instead,
+    // weighted by 1 / its probability.  The rays go to the slot's own run of
+    // the shadow buffer in sample order; one group entry per vertex points
+    // k_shadow at them.
     uint32_t n_sh = 0;
     if (surface && !specular && spv && !direct_off) {
       float4 *run = sr + 2ull * ((uint64_t)slot * spv);
@@ -579,82 +626,46 @@ __global__ __launch_bounds__(kShadeBlock) __attribute__((amdgpu_waves_per_eu(kMu
       if (kMulti)
         while ((side + 1) * (side + 1) <= ndir_job) ++side;
       const float inv = 1.0f / (float)side;
-      for (uint32_t li = 0; li < sc.num_lights; ++li) {
+      // sample k of area light li from random stream `stream`, weighted
+      // area / (n prob)
+      auto area_sample = [&](uint32_t li, uint32_t k, uint32_t stream, float prob) {
         const LightDev &ld = sc.lights[li];
         // not the light the vertex lies on (DirectIllumination.cpp:93)
         if ((ld.kind == LIGHT_QUAD && prim == sc.nt + sc.ns + ld.index) ||
             (ld.kind == LIGHT_SPHERE && prim == sc.nt + ld.index) || (ld.kind == LIGHT_MESH && hi.light == li))
-          continue;
+          return;
         // OrientedEmitter::getLe (emitters/OrientedEmitter.cpp:17-23)
         const float3 lle = le_of(sc.mats[ld.mat]);
-        for (uint32_t k = 0; k < ndir_job; ++k) {
-          const mpt_rng::U4 r = mpt_rng::path_draw(seed, job_id, pixel, sample, depth, 1 + li * ndir_job + k);
-          const uint32_t cell = kMulti ? k % (side * side) : 0u;
-          const float u = ((float)(cell % side) + mpt_rng::unitf(r.x)) * inv;
-          const float v = ((float)(cell / side) + mpt_rng::unitf(r.y)) * inv;
-          // Shape::getPoint (shapes/Shape.cpp:52-64): the light point, its
-          // normal, and the primitive the shadow ray may skip (its only
-          // crossing of a planar light is the end point)
-          float3 lp, ln;
-          uint32_t target = kMiss;
-          if (ld.kind == LIGHT_QUAD) {
-            // Plane::getPosition (shapes/Plane.cpp:91-93)
-            const float4 *lq = sc.quads + 4ull * ld.index;
-            const float4 c0 = lq[0], ex = lq[1], ez = lq[2], qn = lq[3];
-            lp = add(xyz(c0), add(mul(xyz(ex), u - 0.5f), mul(xyz(ez), v - 0.5f)));
-            ln = xyz(qn);
-            target = sc.nt + sc.ns + ld.index;
-          } else if (ld.kind == LIGHT_SPHERE) {
-            // Sphere::getPosition (shapes/Sphere.cpp:117-124): an object-space
-            // point at distance 1 from the centre near +z, mapped by the
-            // sphere's transform; the normal is that point's direction
-            // (Sphere.cpp:86-92, a similarity transform)
-            const float x = fminf(0.5f, fmaxf(-0.5f, tanf((1.0f - u) * (0.5f / kPi))));
-            const float y = fminf(0.5f, fmaxf(-0.5f, 0.5f * sinf((0.5f - v) * (1.0f / kPi))));
-            const float z = sqrtf(1.0f - x * x - y * y);
-            const float3 q = add(add(mul(xyz(ld.frame[0]), x), mul(xyz(ld.frame[1]), y)), mul(xyz(ld.frame[2]), z));
-            lp = add(xyz(sc.spheres[ld.index]), q);
-            ln = normalize(q);
-          } else {
-            // Mesh::getPosition (shapes/Mesh.cpp:442-450): the triangle picked
-            // by u, barycentrics (r1, v, r2) / (r1 + v + r2)
-            // (MeshTriangle.cpp:299-333); the normal interpolates the vertex
-            // normals there (Mesh::_getGeometricNormal, :419-433)
-            const uint32_t ti = ld.first + min((uint32_t)floor((double)u * ld.count), ld.count - 1u);
-            const float4 *lt = sc.light_tris + 3ull * ti;
-            const float4 t0 = lt[0], t1 = lt[1], t2 = lt[2];
-            const float r1 = mpt_rng::unitf(r.z), r2 = mpt_rng::unitf(r.w);
-            const float ks = 1.0f / (r1 + v + r2);
-            const float b0 = r1 * ks, b1 = v * ks, b2 = r2 * ks;
-            lp = add(add(mul(xyz(t0), b0), mul(xyz(t1), b1)), mul(xyz(t2), b2));
-            target = __float_as_uint(t0.w);
-            const float4 *ts = sc.tshade + 3ull * target;
-            ln = normalize(add(add(mul(xyz(ts[0]), b0), mul(xyz(ts[1]), b1)), mul(xyz(ts[2]), b2)));
-          }
-          float3 w = sub(lp, p);
-          const float dist = len(w);
-          w = mul(w, 1.0f / dist);
-          const float cos_wo = fabsf(dot(hi.n, w));
-          const float cos_wi = fabsf(dot(ln, w));
-          const float3 fr = bsdf_eval(m, hi.n, d, w);
-          const bool lit = dot(ln, w) < 0.0f;
-          if (cos_wo > 0.0f && cos_wi > 0.0f && !is_black(fr) && lit) {
-            const float g = cos_wo * cos_wi / (dist * dist) * (ld.area / (float)ndir_job);
-            const float3 contrib = mul(mul(beta, mul(fr, lle)), g);
-            if (!is_black(contrib)) {
-              // Scene::intersects(ray, t - EPSILON) over the kd window
-              // t < tMax - EPSILON (DirectIllumination.cpp:123, Scene.cpp:113)
-              run[2 * n_sh] = make_float4(w.x, w.y, w.z, dist - 2.0f * kEps);
-              run[2 * n_sh + 1] = make_float4(contrib.x, contrib.y, contrib.z, __uint_as_float(target));
-              ++n_sh;
-            }
+        const mpt_rng::U4 r = mpt_rng::path_draw(seed, job_id, pixel, sample, depth, stream);
+        const uint32_t cell = kMulti ? k % (side * side) : 0u;
+        const float u = ((float)(cell % side) + mpt_rng::unitf(r.x)) * inv;
+        const float v = ((float)(cell / side) + mpt_rng::unitf(r.y)) * inv;
+        float3 lp, ln;
+        uint32_t target;
+        light_point(sc, ld, u, v, mpt_rng::unitf(r.z), mpt_rng::unitf(r.w), lp, ln, target);
+        float3 w = sub(lp, p);
+        const float dist = len(w);
+        w = mul(w, 1.0f / dist);
+        const float cos_wo = fabsf(dot(hi.n, w));
+        const float cos_wi = fabsf(dot(ln, w));
+        const float3 fr = bsdf_eval(m, hi.n, d, w);
+        const bool lit = dot(ln, w) < 0.0f;
+        if (cos_wo > 0.0f && cos_wi > 0.0f && !is_black(fr) && lit) {
+          const float g = cos_wo * cos_wi / (dist * dist) * (ld.area / ((float)ndir_job * prob));
+          const float3 contrib = mul(mul(beta, mul(fr, lle)), g);
+          if (!is_black(contrib)) {
+            // Scene::intersects(ray, t - EPSILON) over the kd window
+            // t < tMax - EPSILON (DirectIllumination.cpp:123, Scene.cpp:113)
+            run[2 * n_sh] = make_float4(w.x, w.y, w.z, dist - 2.0f * kEps);
+            run[2 * n_sh + 1] = make_float4(contrib.x, contrib.y, contrib.z, __uint_as_float(target));
+            ++n_sh;
           }
         }
-      }
+      };
       // Point lights (DirectIllumination.cpp:96-121 with isPoint): one
       // sample, no area weight, and a point has no normal, so its cosine is
-      // 1 (SurfacePoint's normal dot, core/SurfacePoint.h:143).
-      for (uint32_t li = 0; li < sc.num_points; ++li) {
+      // 1 (SurfacePoint's normal dot, core/SurfacePoint.h:143); divided by q.
+      auto point_sample = [&](uint32_t li, float q) {
         const float4 lp = sc.points[2 * li], le = sc.points[2 * li + 1];
         float3 w = sub(xyz(lp), p);
         const float dist = len(w);
@@ -662,13 +673,31 @@ __global__ __launch_bounds__(kShadeBlock) __attribute__((amdgpu_waves_per_eu(kMu
         const float cos_wo = fabsf(dot(hi.n, w));
         const float3 fr = bsdf_eval(m, hi.n, d, w);
         if (dist > 0.0f && cos_wo > 0.0f && !is_black(fr)) {
-          const float3 contrib = mul(mul(beta, mul(fr, xyz(le))), cos_wo / (dist * dist));
+          const float3 contrib = mul(mul(beta, mul(fr, xyz(le))), cos_wo / (dist * dist) / q);
           if (!is_black(contrib)) {
             run[2 * n_sh] = make_float4(w.x, w.y, w.z, dist - 2.0f * kEps);
             run[2 * n_sh + 1] = make_float4(contrib.x, contrib.y, contrib.z, __uint_as_float(kMiss));
             ++n_sh;
           }
         }
+      };
+      if (select) {
+        // EmitterSampler::sample's draw (EmitterSampler.cpp:51-61,
+        // Random::sampleNormalizedCDF): the first light whose running share
+        // reaches the uniform, else the last
+        const uint32_t nl = sc.num_lights + sc.num_points;
+        for (uint32_t k = 0; k < ndir_job; ++k) {
+          const float x = mpt_rng::unitf(mpt_rng::path_draw(seed, job_id, pixel, sample, depth, 1 + ndir_job + k).x);
+          uint32_t j = 0;
+          while (j + 1 < nl && !(x <= sc.light_sel[j].x)) ++j;
+          const float pj = sc.light_sel[j].y;
+          if (j < sc.num_lights) area_sample(j, k, 1 + k, pj);
+          else point_sample(j - sc.num_lights, (float)ndir_job * pj);
+        }
+      } else {
+        for (uint32_t li = 0; li < sc.num_lights; ++li)
+          for (uint32_t k = 0; k < ndir_job; ++k) area_sample(li, k, 1 + li * ndir_job + k, 1.0f);
+        for (uint32_t li = 0; li < sc.num_points; ++li) point_sample(li, 1.0f);
       }
     }
     if (surface) {
@@ -982,6 +1011,69 @@ __global__ __launch_bounds__(256) void k_resolve_hits(Scene sc, const mpt_ray *_
     o.prim = __float_as_uint(rp[0].w);
   }
   out[i] = o;
+}
+
+// EmitterSampler::sample and ::getPd (stats/samplers/EmitterSampler.cpp:
+// 51-96): a light by emitted power, then the shape's getRandomPoint.  Sample
+// i draws from Philox keyed by (seed, i): stream 0 gives the light's uniform
+// and (u, v), stream 1 the mesh triangle's two further uniforms.
+__device__ __forceinline__ float clamp_ref(float x, float lo, float hi) {
+  // the reference's CLAMP: a NaN passes through
+  return x >= hi ? hi : (x <= lo ? lo : x);
+}
+
+__global__ __launch_bounds__(256) void k_emitter_sample(Scene sc, uint32_t seed, uint64_t n,
+                                                        mpt_emitter_point *__restrict__ out) {
+  const uint32_t nl = sc.num_lights + sc.num_points;
+  const float pdf_area = sc.emit_area > 0.0f ? 1.0f / sc.emit_area : 1.0f;
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
+    const mpt_rng::U4 r = mpt_rng::path_draw(seed, 0, (uint32_t)i, (uint32_t)(i >> 32), 0, 0);
+    const float x = mpt_rng::unitf(r.x), u = mpt_rng::unitf(r.y), v = mpt_rng::unitf(r.z);
+    uint32_t j = 0;
+    while (j + 1 < nl && !(x <= sc.light_sel[j].x)) ++j;
+    float3 lp = f3(0, 0, 0), ln = f3(0, 0, 0);
+    if (j >= sc.num_lights) {
+      // PointShape: its position, no normal
+      lp = xyz(sc.points[2 * (j - sc.num_lights)]);
+    } else {
+      const LightDev &ld = sc.lights[j];
+      if (ld.kind == LIGHT_SPHERE) {
+        // Sphere::getRandomPoint (shapes/Sphere.cpp:107-115): a uniform point
+        // s on the unit sphere, its (u, v) by Sphere::_getUV (:79-86), then
+        // Sphere::getPosition (:117-124); asin leaves its domain for
+        // |s_y| > 1/2 and the point is NaN there, as in the reference
+        const float sz = 1.0f - 2.0f * u, rho = sqrtf(fmaxf(0.0f, 1.0f - sz * sz));
+        const float phi = 2.0f * kPi * v;
+        const float sx = rho * cosf(phi), sy = rho * sinf(phi);
+        const float su = 1.0f - atan2f(sz, sx) * (0.5f / kPi);
+        const float sv = 0.5f - asinf(2.0f * sy) * (1.0f / kPi);
+        const float px = clamp_ref(tanf((1.0f - su) * (0.5f / kPi)), -0.5f, 0.5f);
+        const float py = clamp_ref(0.5f * sinf((0.5f - sv) * (1.0f / kPi)), -0.5f, 0.5f);
+        const float pz = sqrtf(1.0f - px * px - py * py);
+        const float3 q = add(add(mul(xyz(ld.frame[0]), px), mul(xyz(ld.frame[1]), py)), mul(xyz(ld.frame[2]), pz));
+        lp = add(xyz(sc.spheres[ld.index]), q);
+        ln = mul(q, 1.0f / len(q));
+      } else {
+        // Shape::getRandomPoint (shapes/Shape.cpp:45-50): getPoint at two
+        // uniforms (Plane::getPosition, or Mesh::getPosition with two more)
+        const mpt_rng::U4 r2 = mpt_rng::path_draw(seed, 0, (uint32_t)i, (uint32_t)(i >> 32), 0, 1);
+        uint32_t target;
+        light_point(sc, ld, u, v, mpt_rng::unitf(r2.x), mpt_rng::unitf(r2.y), lp, ln, target);
+      }
+    }
+    mpt_emitter_point e;
+    e.p[0] = lp.x;
+    e.p[1] = lp.y;
+    e.p[2] = lp.z;
+    e.light = j;
+    e.n[0] = ln.x;
+    e.n[1] = ln.y;
+    e.n[2] = ln.z;
+    e.pdf_area = pdf_area;
+    e.pdf_select = sc.light_sel[j].y;
+    e.pad = 0;
+    out[i] = e;
+  }
 }
 
 template <bool ANY>
@@ -1318,6 +1410,7 @@ JobDev make_job(const mpt_job &job, float4 *film) {
   j.filter_p0 = job.filter_params[0];
   j.filter_p1 = job.filter_params[1];
   j.direct_off = job.direct_off ? 1u : 0u;
+  j.light_select = job.light_select ? 1u : 0u;
   return j;
 }
 
@@ -1330,6 +1423,7 @@ int check_job(mpt_context *ctx, const mpt_job *job) {
   if (job->spp >= (1u << kJobBits)) return fail(ctx, MPT_E_INVALID, "spp at or above 2^20");
   if (job->direct_samples > kMaxDirect) return fail(ctx, MPT_E_INVALID, "direct_samples above 64");
   if (job->direct_off > 1) return fail(ctx, MPT_E_INVALID, "direct_off must be 0 or 1");
+  if (job->light_select > 1) return fail(ctx, MPT_E_INVALID, "light_select must be 0 or 1");
   const float *l = job->camera.look, *u = job->camera.up;
   const double ll = l[0] * l[0] + l[1] * l[1] + l[2] * l[2];
   const double cx = u[1] * l[2] - u[2] * l[1], cy = u[2] * l[0] - u[0] * l[2], cz = u[0] * l[1] - u[1] * l[0];
@@ -1535,7 +1629,9 @@ int run_batch(mpt_context *ctx, mpt_scene *s, const BatchPlan &plan, float *L_ho
       break;
     }
   const uint32_t D = plan.max_depth;
-  const bool multi = plan.max_direct > 1;
+  // the multi-sample form also carries light selection by power
+  bool multi = plan.max_direct > 1;
+  for (const JobDev &j : plan.jobs) multi = multi || j.light_select != 0;
   bool filtered = false;
   for (const JobDev &j : plan.jobs) filtered = filtered || j.filter != 0;
   if (filtered && ln.ws.pos.n < ln.ws.cap) {
@@ -1696,6 +1792,8 @@ struct HostScene {
   uint32_t env_w = 0, env_h = 0, env_m = 0;
   float env_rep_u = 1, env_rep_v = 1;
   std::vector<float4> points;   // 2 per point light, the device layout
+  std::vector<float2> light_sel;   // Scene::light_sel
+  double emit_area = 0;
 };
 
 int prepare_scene(mpt_context *ctx, const mpt_scene_desc *d, const mpt_kd_params *kd, HostScene &hs) {
@@ -1889,6 +1987,30 @@ int prepare_scene(mpt_context *ctx, const mpt_scene_desc *d, const mpt_kd_params
         hs.tshade[3ull * t + 2].w = lf;
       }
       lights.push_back(l);
+    }
+  }
+  {
+    // EmitterSampler::init (stats/samplers/EmitterSampler.cpp:19-48): each
+    // light's power averaged over the channels, normalized by the total;
+    // the draw walks the running sum in light order.  An area light's power
+    // is Le pi A (Le = P / (pi A), Emitter.cpp:44; OrientedEmitter.cpp:17-23),
+    // a point light's the given P.
+    std::vector<double> power;
+    for (const LightDev &l : lights) {
+      const mpt_material &lm = d->materials[l.mat];
+      power.push_back((lm.Le[0] + (double)lm.Le[1] + lm.Le[2]) / 3.0 * M_PI * (double)l.area);
+      hs.emit_area += (double)l.area;
+    }
+    for (uint32_t i = 0; i < d->num_points; ++i) {
+      const float *pt = d->points + 6ull * i;
+      power.push_back(((double)pt[3] + pt[4] + pt[5]) / 3.0);
+    }
+    double total = 0, run = 0;
+    for (double w : power) total += w;
+    for (double w : power) {
+      const double share = total > 0 ? w / total : 0.0;
+      run += share;
+      hs.light_sel.push_back(make_float2((float)run, (float)share));
     }
   }
 
@@ -2175,6 +2297,7 @@ int mpt_scene_create(mpt_context *ctx, const mpt_scene_desc *d, const mpt_kd_par
   if ((rc = upload(ctx, s->mats, hs.mats.data(), hs.mats.size()))) return rc;
   if ((rc = upload(ctx, s->lights, hs.lights.data(), hs.lights.size()))) return rc;
   if ((rc = upload(ctx, s->light_tris, hs.light_tris.data(), hs.light_tris.size()))) return rc;
+  if ((rc = upload(ctx, s->light_sel, hs.light_sel.data(), hs.light_sel.size()))) return rc;
   if ((rc = upload(ctx, s->env_tex, hs.env_tex.data(), hs.env_tex.size()))) return rc;
   if ((rc = upload(ctx, s->env_kernel, hs.env_kernel.data(), hs.env_kernel.size()))) return rc;
   if ((rc = upload(ctx, s->points, hs.points.data(), hs.points.size()))) return rc;
@@ -2191,6 +2314,8 @@ int mpt_scene_create(mpt_context *ctx, const mpt_scene_desc *d, const mpt_kd_par
   s->dev.env_rep_v = hs.env_rep_v;
   s->dev.points = s->points.p;
   s->dev.num_points = (uint32_t)(hs.points.size() / 2);
+  s->dev.light_sel = s->light_sel.p;
+  s->dev.emit_area = (float)hs.emit_area;
   s->stats[0] = ko.n_nodes;
   s->stats[1] = ko.n_leaves;
   s->stats[2] = ko.n_refs;
@@ -2369,6 +2494,20 @@ int mpt_trace_any(mpt_scene *s, const mpt_ray *rays, const float *tmax, uint64_t
   k_trace_rays<true><<<grid_for(ctx, n, 8), kBlock, 0, ctx->stream>>>(s->dev, dr.p, dt.p, n, nullptr, dout.p);
   MPT_LAUNCHED(ctx, "k_trace_rays");
   MPT_HIP(ctx, hipMemcpyAsync(occluded, dout.p, n, hipMemcpyDeviceToHost, ctx->stream));
+  MPT_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return MPT_OK;
+}
+
+int mpt_emitter_sample(mpt_scene *s, uint64_t n, uint32_t seed, mpt_emitter_point *out) {
+  if (!s || (n && !out)) return fail(s ? s->ctx : nullptr, MPT_E_INVALID, "NULL argument");
+  mpt_context *ctx = s->ctx;
+  if (s->dev.num_lights + s->dev.num_points == 0) return fail(ctx, MPT_E_INVALID, "the scene has no lights");
+  if (!n) return MPT_OK;
+  DBuf<mpt_emitter_point> dout;
+  MPT_HIP(ctx, dout.reserve(n));
+  k_emitter_sample<<<grid_for(ctx, n, 8), 256, 0, ctx->stream>>>(s->dev, seed, n, dout.p);
+  MPT_LAUNCHED(ctx, "k_emitter_sample");
+  MPT_HIP(ctx, hipMemcpyAsync(out, dout.p, n * sizeof(mpt_emitter_point), hipMemcpyDeviceToHost, ctx->stream));
   MPT_HIP(ctx, hipStreamSynchronize(ctx->stream));
   return MPT_OK;
 }

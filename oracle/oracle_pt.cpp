@@ -775,6 +775,43 @@ struct orc_scene {
   // the shape of each triangle (else triangles sharing an emitting material
   // form one shape), sphere_frames (optional) the spheres' transforms (else
   // 2r times the identity).
+  // EmitterSampler::init (stats/samplers/EmitterSampler.cpp:19-48): every
+  // light's power averaged over the channels, over their total; area lights
+  // then point lights.  An area light's power is Le pi A (Le = P / (pi A),
+  // materials/Emitter.cpp:44, emitters/OrientedEmitter.cpp:17-23), a point
+  // light's P = 4 pi Le (emitters/OmniEmitter.cpp:15-17).
+  std::vector<double> light_share;
+  double emit_area = 0;   // EmitterSampler::m_area: the lights' summed surface area
+  void update_shares() {
+    std::vector<double> power;
+    emit_area = 0;
+    for (const Light &l : lights) {
+      const V3 &le = mats[l.mat].Le;
+      power.push_back((le.x + le.y + le.z) / 3.0 * M_PI * l.area);
+      emit_area += l.area;
+    }
+    for (const Point &pl : points) power.push_back((pl.Le.x + pl.Le.y + pl.Le.z) / 3.0 * 4.0 * M_PI);
+    double total = 0;
+    for (double w : power) total += w;
+    light_share.clear();
+    for (double w : power) light_share.push_back(total > 0 ? w / total : 0.0);
+  }
+  // Random::sampleNormalizedCDF (stats/Random.h:125-139): the first light
+  // whose running share reaches x, else the last
+  uint32_t select_light(double x) const {
+    const uint32_t nl = (uint32_t)light_share.size();
+    double sum = 0;
+    for (uint32_t i = 0; i + 1 < nl; ++i) {
+      sum += light_share[i];
+      if (x <= sum) return i;
+    }
+    return nl - 1;
+  }
+
+  // Shape::getPoint (shapes/Shape.cpp:52-64) of area light L at (u, v), with
+  // r1, r2 the further uniforms of a mesh triangle: position and normal.
+  void light_point(const Light &L, double u, double v, double r1, double r2, V3 &lp, V3 &ln) const;
+
   void build_lights(const uint32_t *tri_shape, const float *sphere_frames) {
     lights.clear();
     tri_light.assign(nt, ORC_MISS);
@@ -818,8 +855,35 @@ struct orc_scene {
         tri_area[t] = l.area;
       }
     }
+    update_shares();
   }
 };
+
+void orc_scene::light_point(const Light &L, double u, double v, double r1, double r2, V3 &lp, V3 &ln) const {
+  if (L.kind == 0) {
+    const Quad &lq = quads[L.index];
+    lp = lq.c + lq.ex * (u - 0.5) + lq.ez * (v - 0.5);   // Plane.cpp:91-93
+    ln = lq.n;
+  } else if (L.kind == 1) {
+    // Sphere::getPosition (shapes/Sphere.cpp:117-124) through the transform;
+    // the normal is the point's object-space direction carried to world
+    // space (Sphere.cpp:86-92)
+    const double x = std::min(0.5, std::max(-0.5, std::tan((1.0 - u) / (2 * M_PI))));
+    const double y = std::min(0.5, std::max(-0.5, std::sin((0.5 - v) / M_PI) * 0.5));
+    const double z = std::sqrt(1 - x * x - y * y);
+    const V3 q = L.frame[0] * x + L.frame[1] * y + L.frame[2] * z;
+    lp = spheres[L.index].c + q;
+    ln = normalized(q);
+  } else {
+    // Mesh::getPosition (shapes/Mesh.cpp:442-450, MESH_GET_INDEX :26) with
+    // MeshTriangle::getPosition (MeshTriangle.cpp:299-333)
+    const uint32_t n_tri = (uint32_t)L.tris.size();
+    const uint32_t t = L.tris[std::min((uint32_t)std::floor(u * n_tri), n_tri - 1)];
+    const double sum = r1 + v + r2;
+    lp = verts[idx[3 * t]] * (r1 / sum) + verts[idx[3 * t + 1]] * (v / sum) + verts[idx[3 * t + 2]] * (r2 / sum);
+    ln = normal_at(t, lp);   // Mesh::_getGeometricNormal at that point
+  }
+}
 
 namespace {
 
@@ -1101,76 +1165,79 @@ V3 trace_path(const orc_scene &sc, const orc_job &job, const Camera &cam, uint32
       const uint32_t ns = std::max(1u, job.direct_samples);
       uint32_t side = 1;
       while ((side + 1) * (side + 1) <= ns) ++side;
-      V3 direct;
-      for (uint32_t li = (uint32_t)sc.lights.size(); li--;) {
+      // the vertex lies on area light li (DirectIllumination.cpp:93 skips it)
+      auto on_light = [&](uint32_t li) {
         const orc_scene::Light &L = sc.lights[li];
-        // not the light the vertex lies on (DirectIllumination.cpp:93)
-        if ((L.kind == 0 && gid == sc.nt + sc.ns + L.index) || (L.kind == 1 && gid == sc.nt + L.index) ||
-            (L.kind == 2 && gid < sc.nt && sc.tri_light[gid] == li))
-          continue;
-        V3 dl;
-        for (uint32_t k = 0; k < ns; ++k) {
-          const Uniforms lu = draw(job.seed, job.job_id, pixel, sample, depth, 1 + li * ns + k);
-          const uint32_t cell = k % (side * side);
-          const double u = ((double)(cell % side) + lu.u[0]) / side, v = ((double)(cell / side) + lu.u[1]) / side;
-          // Shape::getPoint (shapes/Shape.cpp:52-64): position and normal
-          V3 lp, ln;
-          if (L.kind == 0) {
-            const Quad &lq = sc.quads[L.index];
-            lp = lq.c + lq.ex * (u - 0.5) + lq.ez * (v - 0.5);   // Plane.cpp:91-93
-            ln = lq.n;
-          } else if (L.kind == 1) {
-            // Sphere::getPosition (shapes/Sphere.cpp:117-124) through the
-            // transform; the normal is the point's object-space direction
-            // carried to world space (Sphere.cpp:86-92)
-            const double x = std::min(0.5, std::max(-0.5, std::tan((1.0 - u) / (2 * M_PI))));
-            const double y = std::min(0.5, std::max(-0.5, std::sin((0.5 - v) / M_PI) * 0.5));
-            const double z = std::sqrt(1 - x * x - y * y);
-            const V3 q = L.frame[0] * x + L.frame[1] * y + L.frame[2] * z;
-            lp = sc.spheres[L.index].c + q;
-            ln = normalized(q);
-          } else {
-            // Mesh::getPosition (shapes/Mesh.cpp:442-450, MESH_GET_INDEX :26)
-            // with MeshTriangle::getPosition (MeshTriangle.cpp:299-333)
-            const uint32_t n_tri = (uint32_t)L.tris.size();
-            const uint32_t t = L.tris[std::min((uint32_t)std::floor(u * n_tri), n_tri - 1)];
-            const double r1 = lu.u[2], r2 = lu.u[3], sum = r1 + v + r2;
-            lp = sc.verts[sc.idx[3 * t]] * (r1 / sum) + sc.verts[sc.idx[3 * t + 1]] * (v / sum) +
-                 sc.verts[sc.idx[3 * t + 2]] * (r2 / sum);
-            ln = sc.normal_at(t, lp);   // Mesh::_getGeometricNormal at that point
-          }
-          V3 wo = lp - p;
-          const double dist = len(wo);
-          wo = wo / dist;
-          const double cos_wo = std::fabs(dot(n, wo));
-          const double cos_wi = std::fabs(dot(ln, wo));
-          const V3 fr = bsdf_eval(m, n, ray.d, wo);
-          if (cos_wo > 0 && cos_wi > 0 && !is_black(fr)) {
-            ctr.shadow++;
-            if (!sc.occluded(Ray(p, wo), dist - kEps)) {
-              const Material &lm = sc.mats[L.mat];
-              const V3 Le = dot(ln, -wo) > 0 ? lm.Le : V3();   // OrientedEmitter.cpp:17-23
-              dl = dl + mul(fr, Le) * (cos_wo * cos_wi / (dist * dist));
-            }
+        return (L.kind == 0 && gid == sc.nt + sc.ns + L.index) || (L.kind == 1 && gid == sc.nt + L.index) ||
+               (L.kind == 2 && gid < sc.nt && sc.tri_light[gid] == li);
+      };
+      // f Le cos cos / r^2 of sample k (stratum k) of area light li, drawn
+      // from random stream `stream`; 0 when occluded
+      auto area_term = [&](uint32_t li, uint32_t k, uint32_t stream) {
+        const orc_scene::Light &L = sc.lights[li];
+        const Uniforms lu = draw(job.seed, job.job_id, pixel, sample, depth, stream);
+        const uint32_t cell = k % (side * side);
+        const double u = ((double)(cell % side) + lu.u[0]) / side, v = ((double)(cell / side) + lu.u[1]) / side;
+        // Shape::getPoint (shapes/Shape.cpp:52-64): position and normal
+        V3 lp, ln;
+        sc.light_point(L, u, v, lu.u[2], lu.u[3], lp, ln);
+        V3 wo = lp - p;
+        const double dist = len(wo);
+        wo = wo / dist;
+        const double cos_wo = std::fabs(dot(n, wo));
+        const double cos_wi = std::fabs(dot(ln, wo));
+        const V3 fr = bsdf_eval(m, n, ray.d, wo);
+        if (cos_wo > 0 && cos_wi > 0 && !is_black(fr)) {
+          ctr.shadow++;
+          if (!sc.occluded(Ray(p, wo), dist - kEps)) {
+            const Material &lm = sc.mats[L.mat];
+            const V3 Le = dot(ln, -wo) > 0 ? lm.Le : V3();   // OrientedEmitter.cpp:17-23
+            return mul(fr, Le) * (cos_wo * cos_wi / (dist * dist));
           }
         }
-        direct = direct + dl * (L.area / ns);
-      }
+        return V3();
+      };
       // Point lights (DirectIllumination.cpp:96-121, isPoint): one sample, no
       // area weight; a PointShape has no normal, so SurfacePoint's normal
       // dot is 1 (core/SurfacePoint.h:143) and so is cos_wi.
-      for (uint32_t li = (uint32_t)sc.points.size(); li--;) {
+      auto point_term = [&](uint32_t li) {
         const orc_scene::Point &pl = sc.points[li];
         V3 wo = pl.p - p;
         const double dist = len(wo);
-        if (!(dist > 0)) continue;
+        if (!(dist > 0)) return V3();
         wo = wo / dist;
         const double cos_wo = std::fabs(dot(n, wo));
         const V3 fr = bsdf_eval(m, n, ray.d, wo);
         if (cos_wo > 0 && !is_black(fr)) {
           ctr.shadow++;
-          if (!sc.occluded(Ray(p, wo), dist - kEps)) direct = direct + mul(fr, pl.Le) * (cos_wo / (dist * dist));
+          if (!sc.occluded(Ray(p, wo), dist - kEps)) return mul(fr, pl.Le) * (cos_wo / (dist * dist));
         }
+        return V3();
+      };
+      V3 direct;
+      if (job.light_select) {
+        // each sample draws one light as EmitterSampler::sample does
+        // (stats/samplers/EmitterSampler.cpp:51-61 with
+        // Random::sampleNormalizedCDF, stats/Random.h:125-139) and divides
+        // by its probability
+        for (uint32_t k = 0; k < ns; ++k) {
+          const double x = draw(job.seed, job.job_id, pixel, sample, depth, 1 + ns + k).u[0];
+          const uint32_t j = sc.select_light(x);
+          const double pj = sc.light_share[j];
+          if (j < sc.lights.size()) {
+            if (!on_light(j)) direct = direct + area_term(j, k, 1 + k) * (sc.lights[j].area / (ns * pj));
+          } else {
+            direct = direct + point_term(j - (uint32_t)sc.lights.size()) / (ns * pj);
+          }
+        }
+      } else {
+        for (uint32_t li = (uint32_t)sc.lights.size(); li--;) {
+          if (on_light(li)) continue;
+          V3 dl;
+          for (uint32_t k = 0; k < ns; ++k) dl = dl + area_term(li, k, 1 + li * ns + k);
+          direct = direct + dl * (sc.lights[li].area / ns);
+        }
+        for (uint32_t li = (uint32_t)sc.points.size(); li--;) direct = direct + point_term(li);
       }
       L = L + mul(beta, direct);
     }
@@ -1225,7 +1292,7 @@ void orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t ou
 
 // Bumped with every change of an exported signature; pyoracle refuses a
 // library built from older sources.
-int orc_interface_version(void) { return 10; }
+int orc_interface_version(void) { return 11; }
 
 orc_scene *orc_scene_create(uint32_t nv, const float *vertices, uint32_t nt, const uint32_t *indices,
                             const uint32_t *tri_material, uint32_t ns, const float *spheres,
@@ -1651,6 +1718,60 @@ extern "C" void orc_scene_set_emitters(orc_scene *s, const uint32_t *tri_shape, 
   s->build_lights(tri_shape, sphere_frames);
 }
 
+// EmitterSampler::sample / ::getPd (stats/samplers/EmitterSampler.cpp:51-96):
+// a light by power (Random::sampleNormalizedCDF), then its getRandomPoint.
+// Sample i's uniforms: Philox (seed, i), stream 0 for the light and (u, v),
+// stream 1 for a mesh triangle's further two.
+extern "C" int orc_emitter_sample(const orc_scene *s, uint64_t n, uint32_t seed, double *out) {
+  const uint32_t nl = (uint32_t)(s->lights.size() + s->points.size());
+  if (nl == 0) return -1;
+  const double pdf_area = s->emit_area > 0 ? 1.0 / s->emit_area : 1.0;   // EmitterSampler.cpp:86-93
+  for (uint64_t i = 0; i < n; ++i) {
+    const Uniforms r = draw(seed, 0, (uint32_t)i, (uint32_t)(i >> 32), 0, 0);
+    const uint32_t j = s->select_light(r.u[0]);
+    const double u = r.u[1], v = r.u[2];
+    V3 lp, ln;
+    if (j >= s->lights.size()) {
+      lp = s->points[j - s->lights.size()].p;   // PointShape: its position, no normal
+    } else {
+      const orc_scene::Light &L = s->lights[j];
+      if (L.kind == 1) {
+        // Sphere::getRandomPoint (shapes/Sphere.cpp:107-115): a uniform point
+        // on the unit sphere (z = 1 - 2u, azimuth 2 pi v), Sphere::_getUV
+        // (:79-86), Sphere::getPosition (:117-124) with the reference's
+        // CLAMP, through which a NaN passes
+        const double sz = 1.0 - 2.0 * u, rho = std::sqrt(std::max(0.0, 1.0 - sz * sz)), phi = 2.0 * M_PI * v;
+        const double sx = rho * std::cos(phi), sy = rho * std::sin(phi);
+        const double su = 1.0 - std::atan2(sz, sx) * (1.0 / (2.0 * M_PI));
+        const double sv = 0.5 - std::asin(sy * 2.0) * (1.0 / M_PI);
+        auto clamp = [](double x, double lo, double hi) { return x >= hi ? hi : (x <= lo ? lo : x); };
+        const double x = clamp(std::tan((1.0 - su) / (2.0 * M_PI)), -0.5, 0.5);
+        const double y = clamp(std::sin((0.5 - sv) / M_PI) * 0.5, -0.5, 0.5);
+        const double z = std::sqrt(1.0 - x * x - y * y);
+        const V3 q = L.frame[0] * x + L.frame[1] * y + L.frame[2] * z;
+        lp = s->spheres[L.index].c + q;
+        ln = q / len(q);
+      } else {
+        // Shape::getRandomPoint (shapes/Shape.cpp:45-50): getPoint at two
+        // uniforms
+        const Uniforms r2 = draw(seed, 0, (uint32_t)i, (uint32_t)(i >> 32), 0, 1);
+        s->light_point(L, u, v, r2.u[0], r2.u[1], lp, ln);
+      }
+    }
+    double *o = out + 9 * i;
+    o[0] = lp.x;
+    o[1] = lp.y;
+    o[2] = lp.z;
+    o[3] = ln.x;
+    o[4] = ln.y;
+    o[5] = ln.z;
+    o[6] = j;
+    o[7] = pdf_area;
+    o[8] = s->light_share[j];
+  }
+  return 0;
+}
+
 extern "C" void orc_scene_set_points(orc_scene *s, uint32_t n, const float *pts) {
   s->points.clear();
   for (uint32_t i = 0; i < n; ++i) {
@@ -1660,6 +1781,7 @@ extern "C" void orc_scene_set_points(orc_scene *s, uint32_t n, const float *pts)
     pl.Le = V3(q[3], q[4], q[5]) / (4.0 * M_PI);
     s->points.push_back(pl);
   }
+  s->update_shares();
 }
 
 extern "C" int orc_scene_set_env_map(orc_scene *s, uint32_t w, uint32_t h, const float *rgb, float repeat_u,

@@ -44,7 +44,7 @@ typedef struct orc_scene orc_scene;
  * mesh triangles) with the reference build parameters.  kd_min_prims /
  * kd_max_depth of 0 select the reference defaults (3, 24).  use_naive != 0
  * replaces both kd-trees by NaiveSpatialAccel.  Returns NULL on bad input. */
-int orc_interface_version(void);   /* 8 */
+int orc_interface_version(void);   /* 11 */
 orc_scene *orc_scene_create(uint32_t nv, const float *vertices,
                             uint32_t nt, const uint32_t *indices, const uint32_t *tri_material,
                             uint32_t ns, const float *spheres, const uint32_t *sphere_material,
@@ -90,6 +90,8 @@ typedef struct orc_job {
   uint32_t filter_support; /* 0 = 2 */
   double filter_params[2]; /* gaussian sigma; mitchell B, C; lanczosSinc tau (0 = defaults) */
   uint32_t direct_off;     /* 1: efficientDirect = false (PathTracer.cpp:50-52, 70-73) */
+  uint32_t light_select;   /* 1: each light sample draws one light by emitted power
+                              (EmitterSampler's draw) and divides by its probability */
 } orc_job;
 
 /* Renders pixels [pix_begin, pix_end) of the job.  sum/sumsq are
@@ -149,6 +151,12 @@ void orc_background(const orc_scene *s, const double *d, double *rgb);
  * quadruples; mode 0 Reinhard (renderers/tonemaps/ReinhardToneMap.cpp:25-98),
  * otherwise gamma only (NullToneMap.cpp:21-46); rgba gets n * 4 bytes
  * (r, g, b, 255). */
+/* EmitterSampler::sample + ::getPd (stats/samplers/EmitterSampler.cpp:
+ * 51-96) for samples 0..n-1 of `seed`: out double[9*n] = position (3),
+ * normal (3), light index, pdf_area (1 / the lights' summed area), the
+ * light's selection probability.  Returns 0, or -1 without lights. */
+int orc_emitter_sample(const orc_scene *s, uint64_t n, uint32_t seed, double *out);
+
 void orc_tonemap(const float *rgbw, uint64_t n, int mode, uint8_t *rgba);
 
 #ifdef __cplusplus

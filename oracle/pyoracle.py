@@ -15,7 +15,7 @@ LIB_PATH = os.path.join(HERE, "liboracle_pt.so")
 MISS = 0xFFFFFFFF
 
 
-INTERFACE_VERSION = 10
+INTERFACE_VERSION = 11
 
 
 def build():
@@ -35,7 +35,7 @@ class OrcJob(ctypes.Structure):
                 ("job_id", ctypes.c_uint32), ("pix_begin", ctypes.c_uint32), ("pix_end", ctypes.c_uint32),
                 ("direct_samples", ctypes.c_uint32), ("filter", ctypes.c_uint32),
                 ("filter_support", ctypes.c_uint32), ("filter_params", ctypes.c_double * 2),
-                ("direct_off", ctypes.c_uint32)]
+                ("direct_off", ctypes.c_uint32), ("light_select", ctypes.c_uint32)]
 
 
 _lib = None
@@ -72,6 +72,8 @@ def lib():
         L.orc_scene_set_points.argtypes = [vp, u32, vp]
         L.orc_scene_set_emitters.argtypes = [vp, vp, vp]
         L.orc_render_film.argtypes = [vp, ctypes.POINTER(OrcJob), vp, i32]
+        L.orc_emitter_sample.argtypes = [vp, u64, u32, vp]
+        L.orc_emitter_sample.restype = i32
         _lib = L
     return _lib
 
@@ -193,6 +195,15 @@ class OracleScene:
         lib().orc_render_paths(self.h, ctypes.byref(j), len(px), _p(px), _p(sm), _p(L), threads or self.threads)
         return L
 
+    def emitter_sample(self, n, seed):
+        """EmitterSampler draws: dict of position (n, 3), normal (n, 3), light,
+        pdf_area, pdf_select."""
+        out = np.zeros((n, 9), np.float64)
+        if lib().orc_emitter_sample(self.h, n, seed, _p(out)) != 0:
+            raise ValueError("the scene has no lights")
+        return {"p": out[:, 0:3], "n": out[:, 3:6], "light": out[:, 6].astype(np.int64), "pdf_area": out[:, 7],
+                "pdf_select": out[:, 8]}
+
     def close(self):
         if self.h:
             lib().orc_scene_destroy(self.h)
@@ -217,6 +228,7 @@ def make_job(job, pix_begin=0, pix_end=None):
     j.seed, j.job_id = job.seed, job.job_id
     j.direct_samples = getattr(job, "direct_samples", 0)
     j.direct_off = getattr(job, "direct_off", 0)
+    j.light_select = getattr(job, "light_select", 0)
     j.filter, j.filter_support = getattr(job, "filter", 0), getattr(job, "filter_support", 0)
     fp = getattr(job, "filter_params", (0.0, 0.0))
     j.filter_params[0], j.filter_params[1] = fp[0], fp[1]
