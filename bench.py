@@ -159,7 +159,8 @@ def measured_traffic(engine, kd_leaf):
 
 def engine_settings(ctx, m):
     return {"batch_paths": ctx.get_option(m.MPT_OPT_BATCH_PATHS), "stack": ctx.get_option(m.MPT_OPT_STACK),
-            "persistent": ctx.get_option(m.MPT_OPT_PERSISTENT), "lanes": ctx.get_option(m.MPT_OPT_LANES)}
+            "persistent": ctx.get_option(m.MPT_OPT_PERSISTENT), "lanes": ctx.get_option(m.MPT_OPT_LANES),
+            "tree_top": ctx.get_option(m.MPT_OPT_TREE_TOP)}
 
 
 def system_scenes(m, ctx, reuse=None):
@@ -303,6 +304,8 @@ def apply_engine_args(m, ctx, args):
         ctx.set_option(m.MPT_OPT_PERSISTENT, args.persistent)
     if args.lanes:
         ctx.set_option(m.MPT_OPT_LANES, args.lanes)
+    if args.tree_top >= 0:
+        ctx.set_option(m.MPT_OPT_TREE_TOP, args.tree_top)
 
 
 def main_systems(args):
@@ -351,6 +354,8 @@ def main():
     ap.add_argument("--persistent", type=int, default=-1,
                     help="closest-hit kernel form: 0 grid-stride, 1 persistent waves, 2 one ray per lane")
     ap.add_argument("--lanes", type=int, default=0, help="engine lanes (streams) batches alternate over")
+    ap.add_argument("--tree-top", type=int, default=-1,
+                    help="1: the persistent closest-hit kernel reads the kd-tree's top from LDS (-1: library default)")
     ap.add_argument("--kd-leaf", type=int, default=0, help="kd leaf size (0: library default)")
     ap.add_argument("--kd-ci", type=float, default=0.0, help="kd SAH intersection cost (0: default)")
     ap.add_argument("--kd-eb", type=float, default=0.0, help="kd SAH empty-side factor (0: default)")

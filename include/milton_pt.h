@@ -279,6 +279,10 @@ enum {
   MPT_OPT_LANES = 7,       /* streams (each with its own batch workspace) that consecutive batches
                               alternate over, 1..4, default 3: one batch's kernel tails overlap
                               another's bulk */
+  MPT_OPT_TREE_TOP = 9,    /* 1: the closest-hit kernel in persistent waves (form 1) stages the top
+                              of the kd-tree (its first 512 nodes, laid out breadth-first) in LDS
+                              and reads those nodes from there; 0 (default) reads every node
+                              from memory */
   MPT_OPT_FAIL_BATCH = 8   /* failure injection for tests of the callers' error paths: n > 0 makes
                               the n-th batch started from now on fail with MPT_E_DEVICE after its
                               descriptors are staged and before its kernels are enqueued (the

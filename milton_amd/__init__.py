@@ -27,6 +27,7 @@ MPT_SCENE_ROOM, MPT_SCENE_SOUP, MPT_SCENE_TERRAIN = 1, 2, 3
 MPT_OPT_BATCH_PATHS, MPT_OPT_TIMING, MPT_OPT_COUNTERS, MPT_OPT_GRAPHS, MPT_OPT_STACK, MPT_OPT_PERSISTENT = 1, 2, 3, 4, 5, 6
 MPT_OPT_LANES = 7
 MPT_OPT_FAIL_BATCH = 8
+MPT_OPT_TREE_TOP = 9
 MPT_TONEMAP_REINHARD, MPT_TONEMAP_GAMMA = 0, 1
 MPT_FILTER_BOX, MPT_FILTER_TRIANGLE, MPT_FILTER_GAUSSIAN, MPT_FILTER_MITCHELL, MPT_FILTER_LANCZOS = 0, 1, 2, 3, 4
 MPT_SAVE_RADIANCE = 1

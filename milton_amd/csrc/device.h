@@ -65,6 +65,7 @@ struct Scene {
   const uint32_t *sphere_mat;
   const float4 *quads;       // 4 per quad: (c, area), (ex, 0), (ez, 0), (n, material bits)
   const mpt_material *mats;
+  uint32_t top_bytes;        // the tree's top: node bytes [0, top_bytes), which a kernel may stage in LDS
   const struct LightDev *lights;   // area lights: emitting quads, spheres, meshes
   uint32_t num_lights;
   const float4 *light_tris;  // 3 per triangle of an emitting mesh: (v0, triangle id) (v1, 0) (v2, 0)
@@ -334,12 +335,25 @@ struct Walker {
     return tmin <= tmax;
   }
 
-  template <bool COUNT>
+  // TOP: node bytes below sc.top_bytes are read from `top_lds`, the
+  // kernel's LDS copy of the top of the tree.
+  template <bool COUNT, bool TOP = false>
   MPT_HD bool step(const Scene &sc, const RaySetup &r, uint2 *stk, uint32_t *n_nodes,
-                                       uint32_t *n_refs) {
+                                       uint32_t *n_refs, const uint2 *top_lds = nullptr) {
     const char *nodes = reinterpret_cast<const char *>(sc.nodes);
+    const char *tops = reinterpret_cast<const char *>(top_lds);
     char *ring = reinterpret_cast<char *>(stk);
-    uint2 nd = *reinterpret_cast<const uint2 *>(nodes + node);
+    auto fetch = [&](uint32_t off) {
+      // the LDS copy is read as one 64-bit word: a load of another type than
+      // the global one keeps the compiler from merging the two into a
+      // generic-address load
+      if (TOP && off < sc.top_bytes) {
+        const unsigned long long v = *reinterpret_cast<const unsigned long long *>(tops + off);
+        return make_uint2((uint32_t)v, (uint32_t)(v >> 32));
+      }
+      return *reinterpret_cast<const uint2 *>(nodes + off);
+    };
+    uint2 nd = fetch(node);
     if (COUNT) ++*n_nodes;
     while ((nd.x & 3u) != 3u) {
       const uint32_t axis = nd.x & 3u;
@@ -370,7 +384,7 @@ struct Walker {
       }
       tmax = both ? tp : tmax;
       node = far_only ? second : first;
-      nd = *reinterpret_cast<const uint2 *>(nodes + node);
+      nd = fetch(node);
       if (COUNT) ++*n_nodes;
     }
     const uint32_t count = nd.x >> 2;

@@ -183,6 +183,7 @@ struct mpt_context {
   bool counters = false;
   bool graphs = true;
   uint32_t stack = 8;                // short-stack entries of the traversal kernels
+  bool tree_top = false;             // persistent closest hit reads the tree's top from LDS
   uint32_t persistent = 1;           // closest-hit kernel form: 0 grid-stride, 1 persistent waves with
                                      // lane refill, 2 one ray per lane over a capacity-sized grid
   std::atomic<uint32_t> fail_batch{0};   // MPT_OPT_FAIL_BATCH countdown (0: off); a server's worker reads it

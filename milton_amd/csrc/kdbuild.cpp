@@ -15,6 +15,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <deque>
 #include <memory>
 #include <thread>
 
@@ -251,6 +252,36 @@ struct Flattener {
     run(b.kids[0], child, depth + 1);
     run(b.kids[1], child + 1, depth + 1);
   }
+  // The top first, breadth-first while a node's children still fit in
+  // `budget` slots, so that the top is a prefix of the array (a kernel can
+  // keep it in local memory); every node after that, depth-first.
+  void run_top_first(const BNode &root, uint64_t budget) {
+    struct Item {
+      const BNode *b;
+      uint64_t idx, depth;
+    };
+    std::deque<Item> open{{&root, 0, 0}};
+    std::vector<Item> rest;
+    while (!open.empty()) {
+      const Item it = open.front();
+      open.pop_front();
+      if (it.b->axis == 3 || out.nodes.size() / 2 + 2 > budget) {
+        rest.push_back(it);
+        continue;
+      }
+      out.max_depth = std::max(out.max_depth, it.depth);
+      const uint64_t child = out.nodes.size() / 2;
+      out.nodes.resize(out.nodes.size() + 4);
+      uint32_t bits;
+      std::memcpy(&bits, &it.b->split, 4);
+      out.nodes[2 * it.idx] = ((uint32_t)child << 3) | it.b->axis;
+      out.nodes[2 * it.idx + 1] = bits;
+      open.push_back({&it.b->kids[0], child, it.depth + 1});
+      open.push_back({&it.b->kids[1], child + 1, it.depth + 1});
+    }
+    out.top_slots = out.nodes.size() / 2;
+    for (const Item &it : rest) run(*it.b, it.idx, it.depth);
+  }
 };
 
 }  // namespace
@@ -284,7 +315,7 @@ void build(const std::vector<Box> &prims, const Params &p,
   BNode root;
   Builder(p, max_depth, spawn, p.clip ? tri_verts : nullptr, num_tris).build(&root, std::move(refs), box, 0);
   out.nodes.assign(4, 0);
-  Flattener{write_record, out}.run(root, 0, 0);
+  Flattener{write_record, out}.run_top_first(root, p.top_nodes);
   out.n_nodes = out.nodes.size() / 2 - 1;
   out.build_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
 }

@@ -12,7 +12,10 @@
 //            starts at an even node, so the offsets of the two differ only in
 //            bit 3); (count << 2) | 3 for a leaf;
 //            word1 = split position (float bits) or the leaf's first record;
-//            node 0 is the root, node 1 unused (it aligns the pairs);
+//            node 0 is the root, node 1 unused (it aligns the pairs); the
+//            top of the tree comes first, breadth-first (every node of the
+//            first Params::top_nodes slots is shallower than any node after
+//            them), the rest depth-first;
 //   record = 12 x f32 per leaf reference, stored in leaf order so a leaf is a
 //            contiguous run: (v0, id), (v1, kind), (v2, 0) for a triangle
 //            (kind 0 two-sided, 1 one-sided),
@@ -43,6 +46,7 @@ struct Params {
   uint32_t num_threads = 0;        // 0: hardware concurrency
   uint32_t bins = 32;
   bool clip = true;                // clip triangles to the cells they are split across
+  uint32_t top_nodes = 512;        // node slots laid out breadth-first at the front of the array
 };
 
 struct Output {
@@ -50,6 +54,7 @@ struct Output {
   std::vector<float> records;      // 12 floats per reference
   Box box;
   uint64_t n_nodes = 0, n_leaves = 0, n_refs = 0, max_depth = 0, max_leaf = 0;   // n_nodes: root and children
+  uint64_t top_slots = 0;          // node slots [0, top_slots) hold the top of the tree, breadth-first
   bool too_large = false;          // node offsets past 2^32 bytes: the tree is unusable
   double build_ms = 0;
 };

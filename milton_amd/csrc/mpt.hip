@@ -247,12 +247,22 @@ __global__ __launch_bounds__(256) void k_extend(Scene sc, const float4 *__restri
 constexpr uint32_t kRefill = MPT_REFILL;   // idle lanes that trigger a refill
 constexpr uint32_t kChunk = MPT_CHUNK;     // rays per claim
 
-template <bool COUNT, int STACK>
+// TOP: the block first copies the top of the tree (up to kTopNodes nodes,
+// the breadth-first prefix of the node array) into LDS and reads those
+// nodes from there.
+constexpr uint32_t kTopNodes = 512;
+
+template <bool COUNT, int STACK, bool TOP>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(STACK <= 8 ? 8 : (STACK <= 16 ? 4 : 2)))) void k_extend_persistent(
     Scene sc, const float4 *__restrict__ q, const uint32_t *__restrict__ count, float2 *__restrict__ hits,
     unsigned long long *__restrict__ counters, uint32_t *__restrict__ fetch) {
   __shared__ uint2 stack[STACK * kBlock];
+  __shared__ uint2 top_lds[TOP ? kTopNodes : 1];
   uint2 *stk = stack + threadIdx.x;
+  if (TOP) {
+    for (uint32_t i = threadIdx.x; i < sc.top_bytes / 8; i += blockDim.x) top_lds[i] = sc.nodes[i];
+    __syncthreads();
+  }
   const uint32_t n = *count;
   uint32_t next = 0, end = 0;   // wave-uniform: the unassigned part of the claimed chunk
   bool exhausted = false;       // wave-uniform: the queue has no unclaimed rays left
@@ -292,7 +302,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(STACK <= 8 
       if (exhausted && next >= end) break;
       continue;
     }
-    if (ray != kMiss && w.template step<COUNT>(sc, r, stk, &n_nodes, &n_refs)) {
+    if (ray != kMiss && w.template step<COUNT, TOP>(sc, r, stk, &n_nodes, &n_refs, top_lds)) {
       float t;
       uint32_t rec;
       w.raw(t, rec);
@@ -1182,10 +1192,17 @@ template <int STACK>
 void launch_extend_t(mpt_context *ctx, hipStream_t st, uint32_t cap, uint32_t g, const Scene &sc, const float4 *q,
                      const uint32_t *count, float2 *hits, unsigned long long *counters, uint32_t *fetch) {
   if (ctx->persistent == 1) {
-    if (counters)
-      k_extend_persistent<true, STACK><<<g, kBlock, 0, st>>>(sc, q, count, hits, counters, fetch);
-    else
-      k_extend_persistent<false, STACK><<<g, kBlock, 0, st>>>(sc, q, count, hits, nullptr, fetch);
+    if (ctx->tree_top) {
+      if (counters)
+        k_extend_persistent<true, STACK, true><<<g, kBlock, 0, st>>>(sc, q, count, hits, counters, fetch);
+      else
+        k_extend_persistent<false, STACK, true><<<g, kBlock, 0, st>>>(sc, q, count, hits, nullptr, fetch);
+    } else {
+      if (counters)
+        k_extend_persistent<true, STACK, false><<<g, kBlock, 0, st>>>(sc, q, count, hits, counters, fetch);
+      else
+        k_extend_persistent<false, STACK, false><<<g, kBlock, 0, st>>>(sc, q, count, hits, nullptr, fetch);
+    }
     return;
   }
   // form 0: grid-stride over a fixed grid of 8 blocks per CU; form 2: one
@@ -2239,6 +2256,11 @@ int mpt_set_option(mpt_context *ctx, uint32_t option, uint64_t value) {
       ctx->num_lanes = (uint32_t)value;
       ctx->next_lane = 0;
       return MPT_OK;
+    case MPT_OPT_TREE_TOP:
+      if (value > 1) return fail(ctx, MPT_E_INVALID, "tree_top must be 0 or 1");
+      ctx->tree_top = value != 0;
+      drop_graphs(ctx, 0);
+      return MPT_OK;
     case MPT_OPT_FAIL_BATCH:
       if (value > 0xffffffffull) return fail(ctx, MPT_E_INVALID, "fail_batch out of range");
       ctx->fail_batch.store((uint32_t)value);
@@ -2265,6 +2287,7 @@ int mpt_get_option(mpt_context *ctx, uint32_t option, uint64_t *value) {
     case MPT_OPT_PERSISTENT: *value = ctx->persistent; return MPT_OK;
     case MPT_OPT_LANES: *value = ctx->num_lanes; return MPT_OK;
     case MPT_OPT_FAIL_BATCH: *value = ctx->fail_batch.load(); return MPT_OK;
+    case MPT_OPT_TREE_TOP: *value = ctx->tree_top; return MPT_OK;
     default: return fail(ctx, MPT_E_INVALID, "unknown option");
   }
 }
@@ -2315,6 +2338,7 @@ int mpt_scene_create(mpt_context *ctx, const mpt_scene_desc *d, const mpt_kd_par
   s->dev.points = s->points.p;
   s->dev.num_points = (uint32_t)(hs.points.size() / 2);
   s->dev.light_sel = s->light_sel.p;
+  s->dev.top_bytes = (uint32_t)(std::min<uint64_t>(ko.top_slots, kTopNodes) * sizeof(uint2));
   s->dev.emit_area = (float)hs.emit_area;
   s->stats[0] = ko.n_nodes;
   s->stats[1] = ko.n_leaves;

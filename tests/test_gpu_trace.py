@@ -161,3 +161,29 @@ def test_device_counters_match_host_replay(ctx):
     assert (plain["prim"] == hits["prim"]).mean() >= 0.99999
     sc.close()
     hk.close()
+
+
+def test_tree_top_in_lds_keeps_results(ctx):
+    """The persistent closest-hit kernel reading the top of the tree (the
+    breadth-first prefix of the node array) from LDS finds the same hits,
+    counts the same nodes and references, and renders the same film."""
+    arrays = m.generate(m.MPT_SCENE_SOUP, 200_000, 1)
+    rays = m.generate_rays(m.MPT_SCENE_SOUP, 200_000, 500_000, seed=4)
+    saved = {o: ctx.get_option(o) for o in (m.MPT_OPT_TREE_TOP, m.MPT_OPT_PERSISTENT)}
+    sc = m.Scene(ctx, arrays)
+    job = m.make_job(arrays.camera, 160, 90, 4, 4, seed=5)
+    out = {}
+    try:
+        ctx.set_option(m.MPT_OPT_PERSISTENT, 1)
+        for top in (0, 1):
+            ctx.set_option(m.MPT_OPT_TREE_TOP, top)
+            hits, cnt = sc.trace_closest_stats(rays)
+            out[top] = (hits, cnt, sc.render(job)[0])
+    finally:
+        for o, v in saved.items():
+            ctx.set_option(o, v)
+    (h0, c0, f0), (h1, c1, f1) = out[0], out[1]
+    print(f"[tree top] nodes {c0['nodes']} / {c1['nodes']}, refs {c0['refs']} / {c1['refs']}", flush=True)
+    assert np.array_equal(h0.view(np.uint32), h1.view(np.uint32))
+    assert c0 == c1
+    assert np.array_equal(f0, f1)
