@@ -28,7 +28,11 @@ COMMON = [
     "-I", os.path.join(ROOT, "include"),
 ]
 
-SOURCES = ["mpt.hip", "kdbuild.cpp", "scenegen.cpp", "serve.cpp", "tune.cpp", "scenefile.cpp", "filmio.cpp"]
+SOURCES = ["mpt.hip", "scenegen.cpp", "serve.cpp", "tune.cpp", "scenefile.cpp"]
+# Host-only sources (no HIP): compiled by the host compiler, which optimises
+# the kd builder's loops better than hipcc's host pass does (about 2x).
+HOST_SOURCES = ["kdbuild.cpp", "filmio.cpp"]
+HOST_FLAGS = ["-O3", "-std=c++17", "-fPIC", "-Wall", "-pthread", "-I", os.path.join(ROOT, "include")]
 
 
 def _run(cmd):
@@ -38,8 +42,15 @@ def _run(cmd):
 
 def build(verbose=True):
     srcs = [os.path.join(CSRC, s) for s in SOURCES if os.path.exists(os.path.join(CSRC, s))]
+    objdir = os.path.join(HERE, "_build")
+    os.makedirs(objdir, exist_ok=True)
+    objs = []
+    for src in HOST_SOURCES:
+        obj = os.path.join(objdir, os.path.splitext(src)[0] + ".o")
+        _run(["g++", *HOST_FLAGS, "-c", os.path.join(CSRC, src), "-o", obj])
+        objs.append(obj)
     tmp = LIB + ".tmp"
-    _run([HIPCC, "-x", "hip", *srcs, *COMMON, "-pthread", "-o", tmp, "-lz"])
+    _run([HIPCC, "-x", "hip", *srcs, *COMMON, "-pthread", "-x", "none", *objs, "-o", tmp, "-lz"])
     os.replace(tmp, LIB)
     plugin_src = os.path.join(CSRC, "plugin.cpp")
     if os.path.exists(plugin_src):

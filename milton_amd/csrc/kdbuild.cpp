@@ -12,6 +12,9 @@
 #include "kdbuild.h"
 
 #include <algorithm>
+#include <mutex>
+#include <list>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <cstring>
@@ -28,10 +31,46 @@ struct Ref {
 };
 
 struct BNode {
-  uint32_t axis = 3;   // 3 == leaf
+  uint32_t axis = 3;          // 3 == leaf
   float split = 0;
-  std::unique_ptr<BNode[]> kids;
-  std::vector<uint32_t> prims;
+  BNode *kids = nullptr;      // interior: the child pair, in an arena
+  const uint32_t *prims = nullptr;   // leaf: its references, in an arena
+  uint32_t count = 0;         // leaf: number of references
+  uint64_t sub_slots = 0;     // node slots the subtree below this node takes (2 per interior node)
+  uint64_t sub_refs = 0;      // references in the subtree's leaves
+};
+
+// Nodes and leaf references of the subtrees one thread builds, allocated in
+// chunks and released together (tens of millions of nodes would otherwise
+// be as many allocations to make and free).
+class Arena {
+ public:
+  BNode *pair() {
+    if (nodes_used_ + 2 > kNodeChunk) {
+      nodes_.emplace_back(new BNode[kNodeChunk]);
+      nodes_used_ = 0;
+    }
+    BNode *p = nodes_.back().get() + nodes_used_;
+    nodes_used_ += 2;
+    return p;
+  }
+  uint32_t *prims(size_t n) {
+    if (n == 0) return nullptr;   // an empty leaf
+    if (prims_used_ + n > prims_cap_) {
+      prims_cap_ = std::max<size_t>(kPrimChunk, n);
+      prims_.emplace_back(new uint32_t[prims_cap_]);
+      prims_used_ = 0;
+    }
+    uint32_t *p = prims_.back().get() + prims_used_;
+    prims_used_ += n;
+    return p;
+  }
+
+ private:
+  static constexpr size_t kNodeChunk = 1 << 16, kPrimChunk = 1 << 18;
+  std::vector<std::unique_ptr<BNode[]>> nodes_;
+  std::vector<std::unique_ptr<uint32_t[]>> prims_;
+  size_t nodes_used_ = kNodeChunk, prims_used_ = 0, prims_cap_ = 0;
 };
 
 inline float area(const float d[3]) { return 2.0f * (d[0] * d[1] + d[1] * d[2] + d[2] * d[0]); }
@@ -87,28 +126,128 @@ bool clip_triangle(const float *v, const Box &cell, Box &out) {
 
 class Builder {
  public:
-  Builder(const Params &p, uint32_t max_depth, int spawn_depth, const float *tri_verts, uint32_t num_tris)
-      : p_(p), max_depth_(max_depth), spawn_depth_(spawn_depth), tri_verts_(tri_verts), num_tris_(num_tris) {}
+  Builder(const Params &p, uint32_t max_depth, const float *tri_verts, uint32_t num_tris, unsigned threads)
+      : p_(p), max_depth_(max_depth), tri_verts_(tri_verts), num_tris_(num_tris), threads_(threads) {}
 
-  void build(BNode *node, std::vector<Ref> refs, Box cell, uint32_t depth) {
+  // A new arena for a subtree built on its own thread; it lives as long as
+  // the builder.
+  Arena &new_arena() {
+    std::lock_guard<std::mutex> lk(mu_);
+    arenas_.emplace_back();
+    return arenas_.back();
+  }
+
+  void build(BNode *node, std::vector<Ref> refs, Box cell, uint32_t depth, Arena &ar) {
     const size_t n = refs.size();
     if (n <= p_.leaf_size || depth >= max_depth_) {
-      make_leaf(node, refs);
+      make_leaf(node, refs, ar);
       return;
     }
     uint32_t axis;
     float split;
     if (!find_split(refs, cell, axis, split)) {
-      make_leaf(node, refs);
+      make_leaf(node, refs, ar);
       return;
     }
     Box lcell = cell, rcell = cell;
     lcell.hi[axis] = split;
     rcell.lo[axis] = split;
     std::vector<Ref> left, right;
-    left.reserve(n / 2 + 16);
-    right.reserve(n / 2 + 16);
-    for (const Ref &r : refs) {
+    const unsigned parts = helpers(n);
+    if (parts <= 1) {
+      left.reserve(n / 2 + 16);
+      right.reserve(n / 2 + 16);
+      partition(refs, 0, n, axis, split, lcell, rcell, left, right);
+    } else {
+      // consecutive runs of the references on helper threads, joined in
+      // order: the same children as one pass
+      std::vector<std::vector<Ref>> pl(parts), pr(parts);
+      parallel_parts(n, parts, [&](unsigned t, size_t b, size_t e) {
+        partition(refs, b, e, axis, split, lcell, rcell, pl[t], pr[t]);
+      });
+      size_t nl = 0, nr = 0;
+      for (unsigned t = 0; t < parts; ++t) {
+        nl += pl[t].size();
+        nr += pr[t].size();
+      }
+      left.reserve(nl);
+      right.reserve(nr);
+      for (unsigned t = 0; t < parts; ++t) {
+        left.insert(left.end(), pl[t].begin(), pl[t].end());
+        right.insert(right.end(), pr[t].begin(), pr[t].end());
+      }
+    }
+    if (left.size() == n && right.size() == n) {
+      make_leaf(node, refs, ar);
+      return;
+    }
+    std::vector<Ref>().swap(refs);
+    node->axis = axis;
+    node->split = split;
+    node->kids = ar.pair();
+    // the right subtree on a thread of its own while fewer than `threads_`
+    // build and both sides are large; a split that only cuts empty space off
+    // (one side nearly empty) keeps its thread for the other side
+    if (left.size() > 4096 && right.size() > 4096 && live_.fetch_add(1) < (int)threads_) {
+      BNode *rk = &node->kids[1];
+      Arena &rar = new_arena();
+      std::thread t([this, rk, &right, rcell, depth, &rar]() {
+        build(rk, std::move(right), rcell, depth + 1, rar);
+        live_.fetch_sub(1);
+      });
+      build(&node->kids[0], std::move(left), lcell, depth + 1, ar);
+      t.join();
+    } else {
+      if (left.size() > 4096 && right.size() > 4096) live_.fetch_sub(1);
+      build(&node->kids[0], std::move(left), lcell, depth + 1, ar);
+      build(&node->kids[1], std::move(right), rcell, depth + 1, ar);
+    }
+    node->sub_slots = 2 + node->kids[0].sub_slots + node->kids[1].sub_slots;
+    node->sub_refs = node->kids[0].sub_refs + node->kids[1].sub_refs;
+  }
+
+ private:
+  static Box intersect(const Box &a, const Box &b) {
+    Box r;
+    for (int k = 0; k < 3; ++k) {
+      r.lo[k] = std::max(a.lo[k], b.lo[k]);
+      r.hi[k] = std::min(a.hi[k], b.hi[k]);
+      if (r.lo[k] > r.hi[k]) r.lo[k] = r.hi[k];
+    }
+    return r;
+  }
+
+  void make_leaf(BNode *node, const std::vector<Ref> &refs, Arena &ar) {
+    node->axis = 3;
+    uint32_t *pr = ar.prims(refs.size());
+    for (size_t i = 0; i < refs.size(); ++i) pr[i] = refs[i].prim;
+    node->prims = pr;
+    node->count = (uint32_t)refs.size();
+    node->sub_slots = 0;
+    node->sub_refs = refs.size();
+  }
+
+  // Threads to split n references over: several only for the large nodes
+  // near the root, whose subtrees have not yet spread over threads.
+  unsigned helpers(size_t n) const {
+    if (n < (1u << 17)) return 1;
+    return (unsigned)std::min<size_t>(threads_, n >> 16);
+  }
+
+  template <class F>
+  static void parallel_parts(size_t n, unsigned parts, F &&f) {
+    std::vector<std::thread> pool;
+    for (unsigned t = 1; t < parts; ++t) pool.emplace_back([&, t]() { f(t, n * t / parts, n * (t + 1) / parts); });
+    f(0, 0, n / parts);
+    for (auto &x : pool) x.join();
+  }
+
+  // References [b, e) into the children of the split (clipped where it
+  // cuts them), appended in order.
+  void partition(const std::vector<Ref> &refs, size_t b0, size_t e0, uint32_t axis, float split, const Box &lcell,
+                 const Box &rcell, std::vector<Ref> &left, std::vector<Ref> &right) const {
+    for (size_t i = b0; i < e0; ++i) {
+      const Ref &r = refs[i];
       const Box &b = r.box;
       if (b.lo[axis] < split && b.hi[axis] > split) {
         // The split cuts the reference: clip it into each child.
@@ -134,40 +273,28 @@ class Builder {
         if (b.hi[axis] >= split) right.push_back(r);
       }
     }
-    if (left.size() == n && right.size() == n) {
-      make_leaf(node, refs);
-      return;
-    }
-    std::vector<Ref>().swap(refs);
-    node->axis = axis;
-    node->split = split;
-    node->kids.reset(new BNode[2]);
-    if ((int)depth < spawn_depth_ && left.size() + right.size() > 4096) {
-      BNode *rk = &node->kids[1];
-      std::thread t([this, rk, &right, rcell, depth]() { build(rk, std::move(right), rcell, depth + 1); });
-      build(&node->kids[0], std::move(left), lcell, depth + 1);
-      t.join();
-    } else {
-      build(&node->kids[0], std::move(left), lcell, depth + 1);
-      build(&node->kids[1], std::move(right), rcell, depth + 1);
-    }
   }
 
- private:
-  static Box intersect(const Box &a, const Box &b) {
-    Box r;
-    for (int k = 0; k < 3; ++k) {
-      r.lo[k] = std::max(a.lo[k], b.lo[k]);
-      r.hi[k] = std::min(a.hi[k], b.hi[k]);
-      if (r.lo[k] > r.hi[k]) r.lo[k] = r.hi[k];
+  // Bin counts of references [b, e) on every axis: first and last bin of
+  // their cell-clipped bounds.
+  void count_bins(const std::vector<Ref> &refs, size_t b0, size_t e0, const Box &cell, uint32_t K,
+                  uint32_t *start, uint32_t *end) const {
+    for (uint32_t a = 0; a < 3; ++a) {
+      const float ext = cell.hi[a] - cell.lo[a];
+      if (!(ext > 0)) continue;
+      const float scale = (float)K / ext;
+      const float lo0 = cell.lo[a];
+      uint32_t *st = start + a * K, *en = end + a * K;
+      for (size_t i = b0; i < e0; ++i) {
+        const Ref &r = refs[i];
+        const float lo = std::max(r.box.lo[a], cell.lo[a]), hi = std::min(r.box.hi[a], cell.hi[a]);
+        int bl = (int)((lo - lo0) * scale), bh = (int)((hi - lo0) * scale);
+        bl = std::min(std::max(bl, 0), (int)K - 1);
+        bh = std::min(std::max(bh, 0), (int)K - 1);
+        ++st[bl];
+        ++en[bh];
+      }
     }
-    return r;
-  }
-
-  void make_leaf(BNode *node, const std::vector<Ref> &refs) {
-    node->axis = 3;
-    node->prims.resize(refs.size());
-    for (size_t i = 0; i < refs.size(); ++i) node->prims[i] = refs[i].prim;
   }
 
   // Binned SAH over the (cell-clipped) reference bounds.
@@ -180,26 +307,30 @@ class Builder {
     const double n = (double)refs.size();
     double best = (double)p_.cost_intersect * n;   // leaf cost
     bool found = false;
-    uint32_t start[256], end[256];
+    std::vector<uint32_t> start(3 * K, 0), end(3 * K, 0);
+    const unsigned parts = helpers(refs.size());
+    if (parts <= 1) {
+      count_bins(refs, 0, refs.size(), cell, K, start.data(), end.data());
+    } else {
+      std::vector<std::vector<uint32_t>> ps(parts, std::vector<uint32_t>(3 * K, 0)), pe = ps;
+      parallel_parts(refs.size(), parts, [&](unsigned t, size_t b, size_t e) {
+        count_bins(refs, b, e, cell, K, ps[t].data(), pe[t].data());
+      });
+      for (unsigned t = 0; t < parts; ++t)
+        for (uint32_t i = 0; i < 3 * K; ++i) {
+          start[i] += ps[t][i];
+          end[i] += pe[t][i];
+        }
+    }
     for (uint32_t a = 0; a < 3; ++a) {
       const float ext = d[a];
       if (!(ext > 0)) continue;
-      std::memset(start, 0, sizeof(uint32_t) * K);
-      std::memset(end, 0, sizeof(uint32_t) * K);
-      const float scale = (float)K / ext;
       const float lo0 = cell.lo[a];
-      for (const Ref &r : refs) {
-        const float lo = std::max(r.box.lo[a], cell.lo[a]), hi = std::min(r.box.hi[a], cell.hi[a]);
-        int bl = (int)((lo - lo0) * scale), bh = (int)((hi - lo0) * scale);
-        bl = std::min(std::max(bl, 0), (int)K - 1);
-        bh = std::min(std::max(bh, 0), (int)K - 1);
-        ++start[bl];
-        ++end[bh];
-      }
+      const uint32_t *st = start.data() + a * K, *en = end.data() + a * K;
       uint32_t nl = 0, nr = (uint32_t)refs.size();
       for (uint32_t i = 1; i < K; ++i) {
-        nl += start[i - 1];
-        nr -= end[i - 1];
+        nl += st[i - 1];
+        nr -= en[i - 1];
         const float s = lo0 + ext * ((float)i / (float)K);
         if (!(s > cell.lo[a] && s < cell.hi[a])) continue;
         float dl[3] = {d[0], d[1], d[2]}, dr[3] = {d[0], d[1], d[2]};
@@ -221,57 +352,76 @@ class Builder {
 
   const Params &p_;
   uint32_t max_depth_;
-  int spawn_depth_;
   const float *tri_verts_;
   uint32_t num_tris_;
+  unsigned threads_;
+  std::atomic<int> live_{1};   // threads building subtrees
+  std::mutex mu_;
+  std::list<Arena> arenas_;
 };
 
+// Writes the device layout.  Every subtree knows how many node slots and
+// references it takes, so the array is sized once and the subtrees below
+// the breadth-first top are written on several threads, each into its own
+// range: the same bytes as writing them one after another.
 struct Flattener {
   const std::function<void(uint32_t, float *)> &write;
   Output &out;
-  void run(const BNode &b, uint64_t idx, uint64_t depth) {
-    out.max_depth = std::max(out.max_depth, depth);
+  struct Stats {
+    uint64_t max_depth = 0, n_leaves = 0, n_refs = 0, max_leaf = 0;
+  };
+  // the subtree of b at slot idx, its children from slot `slot` on and its
+  // references from record `ref` on, depth-first
+  void run(const BNode &b, uint64_t idx, uint64_t depth, uint64_t &slot, uint64_t &ref, Stats &st) {
+    st.max_depth = std::max(st.max_depth, depth);
     if (b.axis == 3) {
-      const uint64_t first = out.records.size() / 12;
-      out.nodes[2 * idx] = ((uint32_t)b.prims.size() << 2) | 3u;
-      out.nodes[2 * idx + 1] = (uint32_t)first;
-      out.records.resize(out.records.size() + 12 * b.prims.size());
-      for (size_t i = 0; i < b.prims.size(); ++i) write(b.prims[i], &out.records[12 * (first + i)]);
-      ++out.n_leaves;
-      out.n_refs += b.prims.size();
-      out.max_leaf = std::max<uint64_t>(out.max_leaf, b.prims.size());
+      out.nodes[2 * idx] = (b.count << 2) | 3u;
+      out.nodes[2 * idx + 1] = (uint32_t)ref;
+      for (uint32_t i = 0; i < b.count; ++i) write(b.prims[i], &out.records[12 * (ref + i)]);
+      ref += b.count;
+      ++st.n_leaves;
+      st.n_refs += b.count;
+      st.max_leaf = std::max<uint64_t>(st.max_leaf, b.count);
       return;
     }
-    const uint64_t child = out.nodes.size() / 2;   // even: node 1 pads the root
-    if (child + 2 > (1ull << 29)) out.too_large = true;
-    out.nodes.resize(out.nodes.size() + 4);
+    const uint64_t child = slot;
+    slot += 2;
     uint32_t bits;
     std::memcpy(&bits, &b.split, 4);
     out.nodes[2 * idx] = ((uint32_t)child << 3) | b.axis;
     out.nodes[2 * idx + 1] = bits;
-    run(b.kids[0], child, depth + 1);
-    run(b.kids[1], child + 1, depth + 1);
+    run(b.kids[0], child, depth + 1, slot, ref, st);
+    run(b.kids[1], child + 1, depth + 1, slot, ref, st);
   }
   // The top first, breadth-first while a node's children still fit in
   // `budget` slots, so that the top is a prefix of the array (a kernel can
   // keep it in local memory); every node after that, depth-first.
-  void run_top_first(const BNode &root, uint64_t budget) {
+  void run_top_first(const BNode &root, uint64_t budget, unsigned threads) {
+    const uint64_t slots = 2 + root.sub_slots;   // the root and the padding slot first
+    if (slots > (1ull << 29)) {
+      out.too_large = true;
+      return;
+    }
+    out.nodes.assign(2 * slots, 0);
+    out.records.assign(12 * root.sub_refs, 0.0f);
     struct Item {
       const BNode *b;
       uint64_t idx, depth;
     };
     std::deque<Item> open{{&root, 0, 0}};
     std::vector<Item> rest;
+    uint64_t slot = 2;
+    Stats top;
     while (!open.empty()) {
       const Item it = open.front();
       open.pop_front();
-      if (it.b->axis == 3 || out.nodes.size() / 2 + 2 > budget) {
+      if (it.b->axis == 3 || slot + 2 > budget) {
         rest.push_back(it);
         continue;
       }
-      out.max_depth = std::max(out.max_depth, it.depth);
-      const uint64_t child = out.nodes.size() / 2;
-      out.nodes.resize(out.nodes.size() + 4);
+      top.max_depth = std::max(top.max_depth, it.depth);
+      const uint64_t child = slot;
+      slot += 2;
       uint32_t bits;
       std::memcpy(&bits, &it.b->split, 4);
       out.nodes[2 * it.idx] = ((uint32_t)child << 3) | it.b->axis;
@@ -279,8 +429,36 @@ struct Flattener {
       open.push_back({&it.b->kids[0], child, it.depth + 1});
       open.push_back({&it.b->kids[1], child + 1, it.depth + 1});
     }
-    out.top_slots = out.nodes.size() / 2;
-    for (const Item &it : rest) run(*it.b, it.idx, it.depth);
+    out.top_slots = slot;
+    // each remaining subtree's ranges, in the order a sequential pass would
+    // give them
+    std::vector<uint64_t> slot0(rest.size()), ref0(rest.size());
+    uint64_t ref = 0;
+    for (size_t i = 0; i < rest.size(); ++i) {
+      slot0[i] = slot;
+      ref0[i] = ref;
+      slot += rest[i].b->sub_slots;
+      ref += rest[i].b->sub_refs;
+    }
+    std::vector<Stats> st(std::max(1u, threads));
+    std::atomic<size_t> next(0);
+    auto work = [&](unsigned t) {
+      for (size_t i; (i = next.fetch_add(1)) < rest.size();) {
+        uint64_t sl = slot0[i], rf = ref0[i];
+        run(*rest[i].b, rest[i].idx, rest[i].depth, sl, rf, st[t]);
+      }
+    };
+    std::vector<std::thread> pool;
+    for (unsigned t = 1; t < st.size(); ++t) pool.emplace_back(work, t);
+    work(0);
+    for (auto &x : pool) x.join();
+    out.max_depth = top.max_depth;
+    for (const Stats &x : st) {
+      out.max_depth = std::max(out.max_depth, x.max_depth);
+      out.n_leaves += x.n_leaves;
+      out.n_refs += x.n_refs;
+      out.max_leaf = std::max(out.max_leaf, x.max_leaf);
+    }
   }
 };
 
@@ -309,14 +487,13 @@ void build(const std::vector<Box> &prims, const Params &p,
   max_depth = std::min<uint32_t>(max_depth, 48);
   unsigned threads = p.num_threads ? p.num_threads : std::max(1u, std::thread::hardware_concurrency());
   threads = std::min(threads, 64u);
-  const int spawn = threads > 1 ? (int)std::ceil(std::log2((double)threads)) + 2 : 0;
   std::vector<Ref> refs(prims.size());
   for (uint32_t i = 0; i < refs.size(); ++i) refs[i] = Ref{i, prims[i]};
   BNode root;
-  Builder(p, max_depth, spawn, p.clip ? tri_verts : nullptr, num_tris).build(&root, std::move(refs), box, 0);
-  out.nodes.assign(4, 0);
-  Flattener{write_record, out}.run_top_first(root, p.top_nodes);
-  out.n_nodes = out.nodes.size() / 2 - 1;
+  Builder builder(p, max_depth, p.clip ? tri_verts : nullptr, num_tris, threads);
+  builder.build(&root, std::move(refs), box, 0, builder.new_arena());
+  Flattener{write_record, out}.run_top_first(root, p.top_nodes, threads);
+  out.n_nodes = out.nodes.empty() ? 0 : out.nodes.size() / 2 - 1;
   out.build_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
 }
 

@@ -4,8 +4,11 @@
 // (accel/kdTreeAccel.cpp:403-472, 533-630, 722-877).  The reference sorts the
 // primitive extrema of every node (O(N log^2 N), single-threaded) and stores
 // 16-byte pointer nodes with per-leaf heap vectors.  This builder bins the
-// clipped primitive bounds per node (O(N log N)), builds subtrees on several
-// threads, and emits the device layout directly:
+// clipped primitive bounds per node (O(N log N)); the large nodes near the
+// root bin and partition on several threads, subtrees go to threads as they
+// free up, nodes live in per-thread arenas, and the device layout is written
+// by several threads into precomputed ranges.  The result does not depend on
+// the thread count:
 //   node   = 2 x u32: word0 = (child << 3) | axis for an interior node, i.e.
 //            the byte offset of its left child in the node array plus the
 //            axis (the right child follows at + 8, and every child pair
