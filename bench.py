@@ -231,6 +231,7 @@ def bench_batched(m, ctx, scenes):
     # where the per-job time goes: jobs under 4 M paths against the rest
     paths = np.array([j.width * j.height * j.spp for _, j in jobs], np.float64)
     small = paths < 4e6
+    sat = saturated_bound(m, ctx, scenes, jobs, cams)
     log(f"batched: {len(jobs)} jobs, {total} paths, classic per-job {t_classic:.3f}s, per-job {t_plain:.3f}s, "
         f"batched {t_batched:.3f}s, {st['batches']} batches, identical films {same}/{len(jobs)} "
         f"(vs classic {same_classic}/{len(jobs)}); small jobs {int(small.sum())} = "
@@ -241,7 +242,37 @@ def bench_batched(m, ctx, scenes):
             "speedup_vs_per_job": t_plain / t_batched, "speedup_vs_classic": t_classic / t_batched,
             "identical_films": same, "identical_films_classic": same_classic, "batches": st["batches"],
             "extension_rays": st["extension_rays"], "shadow_rays": st["shadow_rays"],
-            "small_jobs": int(small.sum()), "small_job_share_of_per_job_time": float(job_s[small].sum() / job_s.sum())}
+            "small_jobs": int(small.sum()), "small_job_share_of_per_job_time": float(job_s[small].sum() / job_s.sum()),
+            "saturated_seconds": sat["seconds"], "saturated_rates": sat["rates"],
+            "batched_of_saturated": sat["seconds"] / t_batched, "per_job_of_saturated": sat["seconds"] / t_plain,
+            "max_speedup_vs_per_job": t_plain / sat["seconds"]}
+
+
+def saturated_bound(m, ctx, scenes, jobs, cams, side=1024, spp=100):
+    """The mix's time if every job ran at its scene's saturated throughput:
+    for each (scene, max depth) in the mix, one large job of that scene's
+    camera (1024 x 1024 x 100 = 105 M paths, several full batches over the
+    lanes) gives paths/s with the GPU full; the bound is the sum over jobs of
+    paths / that rate.  Work per path does not depend on the resolution or on
+    the sample count, so no schedule of the mix's jobs can finish much
+    sooner than this."""
+    rates = {}
+    film = m.Film(ctx, side, side)
+    for kind, d in sorted({(kind, j.max_depth) for kind, j in jobs}):
+        cam = dict(cams[kind])
+        cam["aspect"] = 1.0
+        big = m.make_job(cam, side, side, spp, d, seed=5, job_id=0)
+        film.render(scenes[kind][1], big)      # warm-up (graphs, workspace)
+        ctx.sync()
+        t0 = time.perf_counter()
+        film.render(scenes[kind][1], big)
+        ctx.sync()
+        rates[f"{kind}/{d}"] = side * side * spp / (time.perf_counter() - t0)
+    film.close()
+    seconds = sum(j.width * j.height * j.spp / rates[f"{kind}/{j.max_depth}"] for kind, j in jobs)
+    log(f"saturated bound: {seconds:.3f}s; rates (scene/depth: G paths/s) "
+        f"{ {k: round(v / 1e9, 3) for k, v in rates.items()} }")
+    return {"seconds": seconds, "rates": rates}
 
 
 def bench_serving(m, ctx, scenes, rate, seconds):
@@ -284,7 +315,23 @@ def systems(m, ctx, args, reuse=None):
     """The batched and serving workloads on the three resident scenes."""
     scenes, build = system_scenes(m, ctx, reuse)
     out = {"scene_build_s": build}
+    tuned = None
+    if not args.no_systems_tune:
+        # engine settings picked by the autotuner on a sample of the mix's
+        # largest job, 2^27 paths so that every candidate batch size runs
+        # several batches over the lanes (a one-batch sample would favour
+        # small batches, which overlap); the per-job, classic and batched runs
+        # all use them
+        from milton_amd import workloads
+        cams = {k: v[0].camera for k, v in scenes.items()}
+        kind, big = max(workloads.batched_jobs(cams, 64, seed=11),
+                        key=lambda kj: kj[1].width * kj[1].height * kj[1].spp)
+        tuned = scenes[kind][1].tune(big, 1 << 27)
+        tuned["scene"] = int(kind)
+        log(f"systems tuned on the largest job: {tuned}")
     out["batched"] = bench_batched(m, ctx, scenes)
+    out["batched"]["tuned"] = tuned
+    out["batched"]["engine"] = engine_settings(ctx, m)
     if args.serve_seconds > 0:
         # Poisson arrivals at 60% of the job rate the batched run sustained
         out["serving"] = bench_serving(m, ctx, scenes, 0.6 * out["batched"]["jobs"] / out["batched"]["seconds"],
@@ -366,6 +413,8 @@ def main():
                     help="soup: the headline kd-tree stress scene (plus the systems workloads unless "
                          "--no-systems); batched / serving: only the systems workloads")
     ap.add_argument("--serve-seconds", type=float, default=10.0)
+    ap.add_argument("--no-systems-tune", action="store_true",
+                    help="batched / serving at the context's settings instead of the autotuner's")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a 256 MiB device-to-device copy (reference dispatch for HBM counters)")
     args = ap.parse_args()
