@@ -44,7 +44,7 @@
 extern "C" {
 #endif
 
-#define MPT_ABI_VERSION 6u
+#define MPT_ABI_VERSION 7u
 #define MPT_MISS 0xFFFFFFFFu
 
 enum {
@@ -234,7 +234,19 @@ typedef struct mpt_job {
    * rays per vertex however many lights the scene has.  With one light and
    * no point light both choices give the same paths. */
   uint32_t light_select;
+  /* MPT_RENDERER_PATH (0): PathTracer (renderers/renderers/PathTracer.cpp).
+   * MPT_RENDERER_BIDIR (1): BidirectionalPathTracer (renderers/renderers/
+   * BidirectionalPathTracer.cpp:34-174 over renderers/utils/Path.cpp): an
+   * eye and a light subpath, every split of every prefix/suffix weighted by
+   * the power heuristic, the sum clamped to [0, 1]; max_depth caps the
+   * bounces of each subpath; direct_samples, direct_off and light_select do
+   * not apply.  It needs a pinhole camera and at least one quad, mesh or
+   * point light (emitting spheres: MPT_E_INVALID); the background is
+   * not an emitter of this renderer. */
+  uint32_t renderer;
 } mpt_job;
+
+enum { MPT_RENDERER_PATH = 0, MPT_RENDERER_BIDIR = 1 };
 
 typedef struct mpt_ray {
   float o[3];

@@ -19,7 +19,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 # compiled with different kernel tunables); the in-tree build by default.
 LIB_PATH = os.environ.get("MPT_LIBRARY") or os.path.join(_HERE, "libmilton_pt.so")
 
-MPT_ABI_VERSION = 6
+MPT_ABI_VERSION = 7
 MPT_TRI_ONE_SIDED = 1
 MPT_MISS = 0xFFFFFFFF
 MPT_BSDF_DIFFUSE, MPT_BSDF_PHONG, MPT_BSDF_DIELECTRIC = 0, 1, 2
@@ -29,6 +29,7 @@ MPT_OPT_LANES = 7
 MPT_OPT_FAIL_BATCH = 8
 MPT_OPT_TREE_TOP = 9
 MPT_TONEMAP_REINHARD, MPT_TONEMAP_GAMMA = 0, 1
+MPT_RENDERER_PATH, MPT_RENDERER_BIDIR = 0, 1
 MPT_FILTER_BOX, MPT_FILTER_TRIANGLE, MPT_FILTER_GAUSSIAN, MPT_FILTER_MITCHELL, MPT_FILTER_LANCZOS = 0, 1, 2, 3, 4
 MPT_SAVE_RADIANCE = 1
 MPT_E_IO = -7
@@ -85,7 +86,7 @@ class Job(ctypes.Structure):
                 ("spp", ctypes.c_uint32), ("max_depth", ctypes.c_uint32),
                 ("seed", ctypes.c_uint32), ("job_id", ctypes.c_uint32), ("direct_samples", ctypes.c_uint32),
                 ("filter", ctypes.c_uint32), ("filter_support", ctypes.c_uint32), ("filter_params", ctypes.c_float * 2),
-                ("direct_off", ctypes.c_uint32), ("light_select", ctypes.c_uint32)]
+                ("direct_off", ctypes.c_uint32), ("light_select", ctypes.c_uint32), ("renderer", ctypes.c_uint32)]
 
 
 EMITTER_POINT_DTYPE = np.dtype([("p", np.float32, 3), ("light", np.uint32), ("n", np.float32, 3),
@@ -398,8 +399,10 @@ def camera_dict(cam):
             "height_angle": cam.height_angle, "aspect": cam.aspect}
 
 
-def make_job(camera, width, height, spp, max_depth, seed=1, job_id=0, direct_samples=1, film_filter=None):
-    """film_filter: None (pixel bin) or (MPT_FILTER_*, support, (p0, p1))."""
+def make_job(camera, width, height, spp, max_depth, seed=1, job_id=0, direct_samples=1, film_filter=None,
+             renderer=0):
+    """film_filter: None (pixel bin) or (MPT_FILTER_*, support, (p0, p1)); renderer:
+    MPT_RENDERER_PATH or MPT_RENDERER_BIDIR."""
     j = Job()
     for k in range(3):
         j.camera.eye[k] = camera["eye"][k]
@@ -417,6 +420,7 @@ def make_job(camera, width, height, spp, max_depth, seed=1, job_id=0, direct_sam
     j.camera.focal_point[0], j.camera.focal_point[1] = fp[0], fp[1]
     j.width, j.height, j.spp, j.max_depth, j.seed, j.job_id = width, height, spp, max_depth, seed, job_id
     j.direct_samples = direct_samples
+    j.renderer = renderer
     if film_filter is not None:
         ff = list(film_filter)
         kind = ff[0]

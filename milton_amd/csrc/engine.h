@@ -49,7 +49,8 @@ struct JobDev {
   float filter_p0, filter_p1;
   uint32_t direct_off;   // efficientDirect = false: emission at every vertex, no NEE
   uint32_t light_select; // 1: each light sample draws its light by emitted power
-  uint32_t pad[2];
+  uint32_t renderer;     // MPT_RENDERER_PATH or MPT_RENDERER_BIDIR
+  uint32_t pad;
 };
 
 struct ChunkDev {
@@ -120,9 +121,17 @@ struct Workspace {
   DBuf<uint32_t> counts;     // [0, D] queue sizes, [D+1, 2D] shadow group counts, [2D+1, 3D] claim cursors
   DBuf<unsigned long long> totals;
   DBuf<BatchDesc> desc;
+  // Bidirectional batches: per slot its subpaths' vertex records (V per slot,
+  // 5 float4 each), (eye vertices, light vertices, flags), the visibility
+  // pairs it queues and its occlusion mask.
+  DBuf<float4> bverts;
+  DBuf<uint4> binfo;
+  DBuf<uint2> bpairs;
+  DBuf<uint32_t> bmasks;
   size_t bytes() const {
     return q[0].bytes() + q[1].bytes() + hits.bytes() + sg.bytes() + sr.bytes() + Ls.bytes() + flags.bytes() +
-           pjs.bytes() + pos.bytes() + counts.bytes() + totals.bytes() + desc.bytes();
+           pjs.bytes() + pos.bytes() + counts.bytes() + totals.bytes() + desc.bytes() + bverts.bytes() +
+           binfo.bytes() + bpairs.bytes() + bmasks.bytes();
   }
 };
 
@@ -149,6 +158,7 @@ struct GraphEntry {
   uint32_t depth = 0, cap = 0, lights = 0;
   bool multi = false;      // several light samples per light
   bool filtered = false;   // the batch splats filtered samples
+  bool bidir = false;      // bidirectional path tracing
   uint64_t generation = 0;
   hipGraphExec_t exec = nullptr;
 };
@@ -160,6 +170,7 @@ struct BatchPlan {
   uint32_t n_paths = 0, n_pixels = 0, max_depth = 0;
   uint32_t max_direct = 1;   // most light samples per light of its jobs
   bool filtered = false;     // some job reconstructs its film through a filter
+  bool bidir = false;        // every job renders with MPT_RENDERER_BIDIR
   uint32_t capacity = 0;     // paths the workspace is sized for (0: the engine's batch capacity)
 };
 
@@ -211,6 +222,7 @@ struct mpt_scene {
   mpt_dev::Scene dev{};
   uint64_t stats[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   uint32_t leaf_size = 0;   // kd leaf size the tree was built with
+  uint32_t sphere_lights = 0;   // emitting spheres among the area lights
   uint64_t device_bytes() const {
     return nodes.bytes() + recs.bytes() + tshade.bytes() + spheres.bytes() + quads.bytes() + sphere_mat.bytes() +
            lights.bytes() + light_tris.bytes() + light_sel.bytes() + mats.bytes() + env_tex.bytes() + env_kernel.bytes();
@@ -246,6 +258,18 @@ uint32_t batch_capacity(const mpt_context *ctx, const mpt_scene *s, uint32_t max
 // Device bytes of one lane's workspace for batches of `cap` paths, `depth`
 // bounces and `shadow` shadow rays (what ensure_workspace allocates).
 uint64_t workspace_bytes(uint64_t cap, uint32_t depth, uint64_t shadow);
+// Bidirectional batches: the device bytes per path slot of their subpath
+// vertices, visibility pairs and masks for subpaths of `depth` bounces, and
+// the paths per batch that fit `lane_bytes` of them (at most the context's
+// batch size, at least kMinBatchPaths).
+constexpr uint64_t kBidirLaneBytes = 4ull << 30;
+uint64_t bidir_slot_bytes(uint32_t depth);
+uint32_t bidir_capacity(const mpt_context *ctx, uint32_t depth, uint64_t lane_bytes);
+// Deepest subpaths (bounces) of a bidirectional job.
+uint32_t bdpt_max_depth();
+// Scene-dependent conditions of a job: a bidirectional job needs a light
+// (quad, mesh or point) and no emitting sphere.
+int check_scene_job(mpt_context *ctx, const mpt_scene *s, const mpt_job *job);
 
 // Enqueues one batch on lane `lane` (asynchronous).  With L_host the
 // per-path radiance of the batch is copied to L_host (slot order) and the

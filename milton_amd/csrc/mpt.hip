@@ -1116,6 +1116,8 @@ namespace {
 
 bool emissive_host(const mpt_material &m) { return m.Le[0] > 0 || m.Le[1] > 0 || m.Le[2] > 0; }
 
+#include "bdpt.h"
+
 int hip_fail(mpt_context *ctx, hipError_t e, const char *what) {
   return fail(ctx, MPT_E_DEVICE, std::string(what) + ": " + hipGetErrorString(e));
 }
@@ -1319,6 +1321,93 @@ int enqueue_batch(mpt_context *ctx, Lane &ln, mpt_scene *s, uint32_t D, bool mul
   return MPT_OK;
 }
 
+// The kernel sequence of one bidirectional batch (the part a graph
+// captures).  counts: eye subpath queue sizes per bounce [0, D], light
+// subpath queue sizes [D + 1, 2D + 1], the pair queue's size, then the
+// closest-hit claim cursors of the eye and the light bounces.
+int enqueue_batch_bidir(mpt_context *ctx, Lane &ln, mpt_scene *s, uint32_t D, bool filtered, bool instrument) {
+  Workspace &w = ln.ws;
+  const hipStream_t st = ln.stream;
+  const uint32_t cap = w.cap, V = 2 * (D + 1);
+  const uint32_t g = grid_for(ctx, cap, 8);
+  const uint32_t gcap = (cap + 255) / 256;
+  uint32_t *cnt_eye = w.counts.p, *cnt_light = w.counts.p + (D + 1), *cnt_pairs = w.counts.p + 2 * (D + 1);
+  uint32_t *cur_eye = cnt_pairs + 1, *cur_light = cur_eye + D;
+  unsigned long long *ctr = instrument && ctx->counters ? w.totals.p + 3 : nullptr;
+  int rc;
+  auto tic = [&](int kind) -> int {
+    if (!instrument || !ctx->timing) return MPT_OK;
+    size_t used = 0;
+    for (const auto &t : ctx->timed) used = std::max(used, t.second + 2);
+    while (used + 2 > ctx->events.size()) {
+      hipEvent_t e;
+      MPT_HIP(ctx, hipEventCreate(&e));
+      ctx->events.push_back(e);
+    }
+    ctx->timed.push_back({kind, used});
+    MPT_HIP(ctx, hipEventRecord(ctx->events[used], st));
+    return MPT_OK;
+  };
+  auto toc = [&]() -> int {
+    if (!instrument || !ctx->timing) return MPT_OK;
+    MPT_HIP(ctx, hipEventRecord(ctx->events[ctx->timed.back().second + 1], st));
+    return MPT_OK;
+  };
+  k_reset<<<1, 64, 0, st>>>(w.counts.p, 4 * D + 3);
+  bdpt::k_bd_camera<<<gcap, 256, 0, st>>>(w.desc.p, w.q[0].p, w.pjs.p, filtered ? w.pos.p : nullptr, cnt_eye,
+                                          w.bverts.p, w.binfo.p, V);
+  MPT_LAUNCHED(ctx, "k_bd_camera");
+  // the eye subpaths, then the light subpaths, one bounce per launch pair
+  for (int side = 0; side < 2; ++side) {
+    uint32_t *cnt = side == 0 ? cnt_eye : cnt_light, *cur = side == 0 ? cur_eye : cur_light;
+    if (side == 1) {
+      bdpt::k_bd_light_start<<<g, 256, 0, st>>>(s->dev, w.desc.p, w.q[0].p, cnt_light, w.bverts.p, w.binfo.p, V,
+                                                w.pjs.p);
+      MPT_LAUNCHED(ctx, "k_bd_light_start");
+    }
+    for (uint32_t b = 0; b < D; ++b) {
+      float4 *qin = w.q[b & 1].p, *qout = w.q[(b + 1) & 1].p;
+      if ((rc = tic(0))) return rc;
+      launch_extend(ctx, st, cap, g, s->dev, qin, cnt + b, w.hits.p, ctr, cur + b);
+      if ((rc = toc())) return rc;
+      if ((rc = tic(1))) return rc;
+      if (side == 0)
+        bdpt::k_bd_step<true><<<g, 256, 0, st>>>(s->dev, w.desc.p, b, qin, w.hits.p, cnt + b, qout, cnt + b + 1,
+                                                 w.bverts.p, w.binfo.p, V, w.pjs.p);
+      else
+        bdpt::k_bd_step<false><<<g, 256, 0, st>>>(s->dev, w.desc.p, b, qin, w.hits.p, cnt + b, qout, cnt + b + 1,
+                                                  w.bverts.p, w.binfo.p, V, w.pjs.p);
+      if ((rc = toc())) return rc;
+      MPT_LAUNCHED(ctx, "subpath kernels");
+    }
+  }
+  bdpt::k_bd_join<<<g, 256, 0, st>>>(s->dev, w.desc.p, w.bverts.p, w.binfo.p, V, w.pjs.p, w.bpairs.p, cnt_pairs,
+                                     w.bmasks.p);
+  MPT_LAUNCHED(ctx, "k_bd_join");
+  if ((rc = tic(2))) return rc;
+  switch (ctx->stack) {
+    case 8: bdpt::k_bd_vis<8><<<g, 256, 0, st>>>(s->dev, w.bpairs.p, cnt_pairs, w.bverts.p, w.binfo.p, V, w.bmasks.p); break;
+    case 12: bdpt::k_bd_vis<12><<<g, 256, 0, st>>>(s->dev, w.bpairs.p, cnt_pairs, w.bverts.p, w.binfo.p, V, w.bmasks.p); break;
+    case 24: bdpt::k_bd_vis<24><<<g, 256, 0, st>>>(s->dev, w.bpairs.p, cnt_pairs, w.bverts.p, w.binfo.p, V, w.bmasks.p); break;
+    default: bdpt::k_bd_vis<16><<<g, 256, 0, st>>>(s->dev, w.bpairs.p, cnt_pairs, w.bverts.p, w.binfo.p, V, w.bmasks.p); break;
+  }
+  if ((rc = toc())) return rc;
+  MPT_LAUNCHED(ctx, "k_bd_vis");
+  if (V <= 8)
+    bdpt::k_bd_combine<8><<<gcap, 256, 0, st>>>(s->dev, w.desc.p, w.bverts.p, w.binfo.p, V, w.bmasks.p, w.Ls.p);
+  else if (V <= 14)
+    bdpt::k_bd_combine<14><<<gcap, 256, 0, st>>>(s->dev, w.desc.p, w.bverts.p, w.binfo.p, V, w.bmasks.p, w.Ls.p);
+  else
+    bdpt::k_bd_combine<bdpt::kMaxVerts><<<gcap, 256, 0, st>>>(s->dev, w.desc.p, w.bverts.p, w.binfo.p, V,
+                                                              w.bmasks.p, w.Ls.p);
+  MPT_LAUNCHED(ctx, "k_bd_combine");
+  k_film<<<gcap, 256, 0, st>>>(w.desc.p, w.Ls.p);
+  if (filtered) k_film_filtered<<<gcap, 256, 0, st>>>(w.desc.p, w.Ls.p, w.pos.p);
+  bdpt::k_bd_totals<<<1, 64, 0, st>>>(w.counts.p, D, w.totals.p);
+  MPT_LAUNCHED(ctx, "film kernels");
+  return MPT_OK;
+}
+
 // Copies a batch plan into the next pinned staging descriptor and enqueues
 // its upload.
 int stage_plan(mpt_context *ctx, Lane &ln, const BatchPlan &plan) {
@@ -1428,6 +1517,7 @@ JobDev make_job(const mpt_job &job, float4 *film) {
   j.filter_p1 = job.filter_params[1];
   j.direct_off = job.direct_off ? 1u : 0u;
   j.light_select = job.light_select ? 1u : 0u;
+  j.renderer = job.renderer;
   return j;
 }
 
@@ -1441,6 +1531,13 @@ int check_job(mpt_context *ctx, const mpt_job *job) {
   if (job->direct_samples > kMaxDirect) return fail(ctx, MPT_E_INVALID, "direct_samples above 64");
   if (job->direct_off > 1) return fail(ctx, MPT_E_INVALID, "direct_off must be 0 or 1");
   if (job->light_select > 1) return fail(ctx, MPT_E_INVALID, "light_select must be 0 or 1");
+  if (job->renderer > MPT_RENDERER_BIDIR) return fail(ctx, MPT_E_INVALID, "unknown renderer");
+  if (job->renderer == MPT_RENDERER_BIDIR) {
+    if (job->max_depth > bdpt::kMaxDepth)
+      return fail(ctx, MPT_E_INVALID, "the bidirectional renderer takes max_depth up to 12");
+    if (job->camera.aperture > 0)
+      return fail(ctx, MPT_E_INVALID, "the bidirectional renderer needs a pinhole camera (aperture 0)");
+  }
   const float *l = job->camera.look, *u = job->camera.up;
   const double ll = l[0] * l[0] + l[1] * l[1] + l[2] * l[2];
   const double cx = u[1] * l[2] - u[2] * l[1], cy = u[2] * l[0] - u[0] * l[2], cz = u[0] * l[1] - u[1] * l[0];
@@ -1500,8 +1597,36 @@ uint64_t workspace_bytes(uint64_t cap, uint32_t depth, uint64_t shadow) {
   // q[2], hits, sg, sr, Ls, flags, pjs, pos, counts, totals, desc
   return cap * (2 * 3 * sizeof(float4) + sizeof(float2) + 2 * sizeof(float4) + sizeof(float4) + sizeof(uint32_t) +
                 sizeof(uint2) + sizeof(float2)) +
-         shadow * 2 * sizeof(float4) + (3ull * depth + 1) * sizeof(uint32_t) + 8 * sizeof(unsigned long long) +
+         shadow * 2 * sizeof(float4) + (4ull * depth + 3) * sizeof(uint32_t) + 8 * sizeof(unsigned long long) +
          sizeof(BatchDesc);
+}
+
+uint64_t bidir_slot_bytes(uint32_t depth) {
+  const uint64_t V = 2ull * (depth + 1), P = V * (V - 1) / 2;
+  return V * bdpt::kVtx * sizeof(float4) + sizeof(uint4) + P * sizeof(uint2) + bdpt::kMaskWords * sizeof(uint32_t);
+}
+
+uint32_t bdpt_max_depth() { return bdpt::kMaxDepth; }
+
+uint32_t bidir_capacity(const mpt_context *ctx, uint32_t depth, uint64_t lane_bytes) {
+  const uint64_t V = 2ull * (depth + 1), P = V * (V - 1) / 2;
+  uint64_t cap = std::min<uint64_t>(ctx->batch_paths, lane_bytes / bidir_slot_bytes(depth));
+  // the pair queue's count is 32-bit
+  cap = std::min<uint64_t>(cap, 0xFFFFFFFFull / P);
+  cap = cap / 256 * 256;
+  return (uint32_t)std::max<uint64_t>(cap, kMinBatchPaths);
+}
+
+int check_scene_job(mpt_context *ctx, const mpt_scene *s, const mpt_job *job) {
+  if (job->renderer != MPT_RENDERER_BIDIR) return MPT_OK;
+  // EmitterSampler::sample asserts a light (stats/samplers/EmitterSampler.cpp:51-53)
+  if (s->dev.num_lights + s->dev.num_points == 0)
+    return fail(ctx, MPT_E_INVALID, "the bidirectional renderer needs a light (an emitting quad, mesh or point)");
+  // Sphere::getRandomPoint's uv round trip yields NaN points for half its
+  // draws (shapes/Sphere.cpp:79-124)
+  if (s->sphere_lights)
+    return fail(ctx, MPT_E_INVALID, "the bidirectional renderer does not sample emitting spheres");
+  return MPT_OK;
 }
 
 int ensure_lane(mpt_context *ctx, uint32_t k) {
@@ -1547,7 +1672,7 @@ int ensure_workspace(mpt_context *ctx, uint32_t k, uint32_t cap, uint32_t depth,
   MPT_HIP(ctx, w.pjs.reserve(cap));
   MPT_HIP(ctx, w.sg.reserve(2ull * cap));
   MPT_HIP(ctx, w.sr.reserve(2ull * std::max<uint64_t>(shadow, 1)));
-  MPT_HIP(ctx, w.counts.reserve(3ull * depth + 1));
+  MPT_HIP(ctx, w.counts.reserve(4ull * depth + 3));
   MPT_HIP(ctx, w.desc.reserve(1));
   if (!w.totals.p) {
     // totals persist across reallocation: a render's counts are zeroed once
@@ -1558,6 +1683,24 @@ int ensure_workspace(mpt_context *ctx, uint32_t k, uint32_t cap, uint32_t depth,
   w.cap = cap;
   w.depth = depth;
   w.shadow_entries = shadow;
+  ++w.generation;
+  drop_graphs(ctx, 0);
+  return MPT_OK;
+}
+
+// Makes sure lane k holds the subpath records, pairs and masks of a
+// bidirectional batch of `cap` paths with subpaths of `depth` bounces.
+int ensure_bidir(mpt_context *ctx, uint32_t k, uint32_t cap, uint32_t depth) {
+  Lane &ln = ctx->lanes[k];
+  Workspace &w = ln.ws;
+  const uint64_t V = 2ull * (depth + 1), P = V * (V - 1) / 2;
+  const uint64_t nv = (uint64_t)cap * V * bdpt::kVtx, np = (uint64_t)cap * P, nm = (uint64_t)cap * bdpt::kMaskWords;
+  if (w.bverts.n >= nv && w.binfo.n >= cap && w.bpairs.n >= np && w.bmasks.n >= nm) return MPT_OK;
+  MPT_HIP(ctx, hipStreamSynchronize(ln.stream));
+  MPT_HIP(ctx, w.bverts.reserve(std::max<uint64_t>(nv, w.bverts.n)));
+  MPT_HIP(ctx, w.binfo.reserve(std::max<uint64_t>(cap, w.binfo.n)));
+  MPT_HIP(ctx, w.bpairs.reserve(std::max<uint64_t>(np, w.bpairs.n)));
+  MPT_HIP(ctx, w.bmasks.reserve(std::max<uint64_t>(nm, w.bmasks.n)));
   ++w.generation;
   drop_graphs(ctx, 0);
   return MPT_OK;
@@ -1630,13 +1773,21 @@ int run_batch(mpt_context *ctx, mpt_scene *s, const BatchPlan &plan, float *L_ho
   if (plan.jobs.size() > kMaxJobs || plan.chunks.size() > kMaxChunks)
     return fail(ctx, MPT_E_INVALID, "batch plan exceeds the job or chunk table");
   if (lane >= kMaxLanes) return fail(ctx, MPT_E_INVALID, "bad lane");
-  // shadow rays per vertex: one per light sample; their random streams are
-  // numbered 1 + light * samples + sample below 256
-  const uint32_t shadow_per_vertex = s->dev.num_lights * std::max(1u, plan.max_direct) + s->dev.num_points;
-  if (shadow_per_vertex > 254) return fail(ctx, MPT_E_INVALID, "lights x direct_samples above 254");
-  const uint32_t cap = plan.capacity ? plan.capacity : batch_capacity(ctx, s, plan.max_direct);
-  int rc = ensure_workspace(ctx, lane, std::max(plan.n_paths, cap), plan.max_depth, shadow_per_vertex);
-  if (rc) return rc;
+  int rc;
+  if (plan.bidir) {
+    if (plan.max_depth > bdpt::kMaxDepth) return fail(ctx, MPT_E_INVALID, "bidirectional batch deeper than 12");
+    const uint32_t cap = plan.capacity ? plan.capacity : bidir_capacity(ctx, plan.max_depth, kBidirLaneBytes);
+    if ((rc = ensure_workspace(ctx, lane, std::max(plan.n_paths, cap), plan.max_depth, 1))) return rc;
+    if ((rc = ensure_bidir(ctx, lane, std::max(plan.n_paths, cap), plan.max_depth))) return rc;
+  } else {
+    // shadow rays per vertex: one per light sample; their random streams are
+    // numbered 1 + light * samples + sample below 256
+    const uint32_t shadow_per_vertex = s->dev.num_lights * std::max(1u, plan.max_direct) + s->dev.num_points;
+    if (shadow_per_vertex > 254) return fail(ctx, MPT_E_INVALID, "lights x direct_samples above 254");
+    const uint32_t cap = plan.capacity ? plan.capacity : batch_capacity(ctx, s, plan.max_direct);
+    if ((rc = ensure_workspace(ctx, lane, std::max(plan.n_paths, cap), plan.max_depth, shadow_per_vertex)))
+      return rc;
+  }
   Lane &ln = ctx->lanes[lane];
   if ((rc = stage_plan(ctx, ln, plan))) return rc;
   // injected failure: count this batch down, and fail it when the count ends
@@ -1666,14 +1817,15 @@ int run_batch(mpt_context *ctx, mpt_scene *s, const BatchPlan &plan, float *L_ho
     const void *key = reinterpret_cast<const void *>((uintptr_t)s->uid);
     for (const GraphEntry &e : ctx->graphs_cache)
       if (e.scene == key && e.lane == lane && e.depth == D && e.cap == w.cap && e.lights == s->dev.num_lights &&
-          e.multi == multi && e.filtered == filtered && e.generation == w.generation) {
+          e.multi == multi && e.filtered == filtered && e.bidir == plan.bidir && e.generation == w.generation) {
         exec = e.exec;
         break;
       }
     if (!exec) {
       hipGraph_t graph = nullptr;
       MPT_HIP(ctx, hipStreamBeginCapture(ln.stream, hipStreamCaptureModeThreadLocal));
-      rc = enqueue_batch(ctx, ln, s, D, multi, filtered, false);
+      rc = plan.bidir ? enqueue_batch_bidir(ctx, ln, s, D, filtered, false)
+                      : enqueue_batch(ctx, ln, s, D, multi, filtered, false);
       hipGraph_t captured = nullptr;
       const hipError_t ec = hipStreamEndCapture(ln.stream, &captured);
       if (rc) {
@@ -1693,13 +1845,16 @@ int run_batch(mpt_context *ctx, mpt_scene *s, const BatchPlan &plan, float *L_ho
       e.lights = s->dev.num_lights;
       e.multi = multi;
       e.filtered = filtered;
+      e.bidir = plan.bidir;
       e.generation = w.generation;
       e.exec = exec;
       ctx->graphs_cache.push_back(e);
     }
     MPT_HIP(ctx, hipGraphLaunch(exec, ln.stream));
   } else {
-    if ((rc = enqueue_batch(ctx, ln, s, D, multi, filtered, true))) return rc;
+    rc = plan.bidir ? enqueue_batch_bidir(ctx, ln, s, D, filtered, true)
+                    : enqueue_batch(ctx, ln, s, D, multi, filtered, true);
+    if (rc) return rc;
   }
   if (L_host) {
     MPT_HIP(ctx, hipMemcpy2DAsync(L_host, 3 * sizeof(float), ln.ws.Ls.p, sizeof(float4), 3 * sizeof(float),
@@ -1719,8 +1874,12 @@ int render_range(mpt_scene *s, const mpt_job *job, uint32_t pix_begin, uint32_t 
   mpt_job focused = *job;
   if ((rc = resolve_focus(s, focused))) return rc;
   job = &focused;
+  if ((rc = check_scene_job(ctx, s, job))) return rc;
   const uint32_t spp = job->spp;
-  const uint32_t batch_pix = std::max<uint32_t>(1, batch_capacity(ctx, s, std::max(1u, job->direct_samples)) / spp);
+  const bool bidir = job->renderer == MPT_RENDERER_BIDIR;
+  const uint32_t cap = bidir ? bidir_capacity(ctx, job->max_depth, kBidirLaneBytes)
+                             : batch_capacity(ctx, s, std::max(1u, job->direct_samples));
+  const uint32_t batch_pix = std::max<uint32_t>(1, cap / spp);
   if ((rc = reset_totals(ctx))) return rc;
   MPT_HIP(ctx, hipEventRecord(ctx->ev0, ctx->stream));
   if ((rc = fork_lanes(ctx))) return rc;
@@ -1728,6 +1887,7 @@ int render_range(mpt_scene *s, const mpt_job *job, uint32_t pix_begin, uint32_t 
   plan.jobs.push_back(make_job(*job, film));
   plan.max_depth = job->max_depth;
   plan.max_direct = plan.jobs[0].direct;
+  plan.bidir = bidir;
   uint64_t batches = 0;
   for (uint32_t p0 = pix_begin; p0 < pix_end; p0 += batch_pix) {
     const uint32_t np = std::min(batch_pix, pix_end - p0);
@@ -2319,6 +2479,8 @@ int mpt_scene_create(mpt_context *ctx, const mpt_scene_desc *d, const mpt_kd_par
   if ((rc = upload(ctx, s->quads, hs.quads.data(), hs.quads.size()))) return rc;
   if ((rc = upload(ctx, s->mats, hs.mats.data(), hs.mats.size()))) return rc;
   if ((rc = upload(ctx, s->lights, hs.lights.data(), hs.lights.size()))) return rc;
+  s->sphere_lights = 0;
+  for (const LightDev &l : hs.lights) s->sphere_lights += l.kind == LIGHT_SPHERE ? 1u : 0u;
   if ((rc = upload(ctx, s->light_tris, hs.light_tris.data(), hs.light_tris.size()))) return rc;
   if ((rc = upload(ctx, s->light_sel, hs.light_sel.data(), hs.light_sel.size()))) return rc;
   if ((rc = upload(ctx, s->env_tex, hs.env_tex.data(), hs.env_tex.size()))) return rc;

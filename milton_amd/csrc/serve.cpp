@@ -70,7 +70,9 @@ struct mpt_server {
   uint64_t next_handle = 1;
   uint64_t film_bytes = 0, peak_film_bytes = 0;
   uint32_t cap = 0;           // paths per batch (the lanes' workspaces fit queue_bytes)
-  uint64_t queue_bytes = 0;   // device bytes reserved for the lanes' ray queues
+  uint32_t bidir_cap = 0;     // paths per bidirectional batch (their subpath records fit bidir_lane_bytes)
+  uint64_t bidir_lane_bytes = 0;
+  uint64_t queue_bytes = 0;   // device bytes reserved for the lanes' ray queues and subpath records
   uint64_t batches = 0, paths = 0, jobs_done = 0;
   std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now();
   std::thread worker;
@@ -188,18 +190,23 @@ int round(mpt_server *sv, std::unique_lock<std::mutex> &lk, uint32_t *ran) {
     if (sv->active.empty()) return MPT_OK;
   }
   mpt_context *ctx = sv->ctx;
+  // a batch holds jobs of one scene and one renderer
   mpt_scene *scene = sv->active.front()->scene;
+  const uint32_t renderer = sv->active.front()->job.renderer;
+  const bool bidir = renderer == MPT_RENDERER_BIDIR;
   BatchPlan plan;
+  plan.bidir = bidir;
   std::vector<std::pair<Job *, uint32_t>> parts;
   for (Job *j : sv->active) {
-    if (j->scene != scene) continue;
+    if (j->scene != scene || j->job.renderer != renderer) continue;
     if (plan.jobs.size() == kMaxJobs || plan.chunks.size() == kMaxChunks) break;
     const uint32_t spp = j->job.spp;
     // a job with more light samples per vertex lowers the batch's capacity:
     // its shadow rays (paths x light samples) stay within the lane's queue
     const uint32_t md = std::max(plan.max_direct, std::max(1u, j->job.direct_samples));
     const uint32_t spv = std::max(1u, scene->dev.num_lights * md + scene->dev.num_points);
-    const uint32_t cap = std::max(1u, std::min(std::min(sv->cap, batch_capacity(ctx, scene, md)), sv->cap / spv));
+    const uint32_t cap = bidir ? sv->bidir_cap
+                               : std::max(1u, std::min(std::min(sv->cap, batch_capacity(ctx, scene, md)), sv->cap / spv));
     const uint64_t room = plan.n_paths >= cap ? 0 : (cap - plan.n_paths) / spp;
     if (room == 0) {
       if (plan.n_paths == 0) {
@@ -228,8 +235,9 @@ int round(mpt_server *sv, std::unique_lock<std::mutex> &lk, uint32_t *ran) {
     if (plan.n_paths >= cap) break;
   }
   if (plan.chunks.empty()) return MPT_OK;
-  plan.capacity = std::max(1u, std::min(sv->cap, sv->cap / std::max(1u, scene->dev.num_lights * plan.max_direct +
-                                                                              scene->dev.num_points)));
+  plan.capacity = bidir ? sv->bidir_cap
+                        : std::max(1u, std::min(sv->cap, sv->cap / std::max(1u, scene->dev.num_lights * plan.max_direct +
+                                                                                    scene->dev.num_points)));
   while (sv->inflight.size() >= sv->cfg.max_inflight) retire_one(sv, lk);
   // Batches alternate over the context's lanes; a lane first waits for the
   // context stream, where new films are cleared.
@@ -316,13 +324,21 @@ int mpt_server_create(mpt_context *ctx, const mpt_server_config *cfg, mpt_server
   sv->cfg.max_inflight = std::min<uint32_t>(sv->cfg.max_inflight, 3 * lanes);
   // The lanes' ray queues take at most half the budget: halve the batch
   // until every lane's workspace (batches up to depth 64, one shadow ray per
-  // path) fits there.
+  // path) and the subpath records of a bidirectional batch (half as many
+  // bytes, at most kBidirLaneBytes) fit there.
+  auto lane_bytes = [](uint64_t c) {
+    const uint64_t ws = workspace_bytes(c, 64, c);
+    return ws + std::min(kBidirLaneBytes, ws / 2);
+  };
   uint64_t cap = ctx->batch_paths;
-  while (cap > kMinBatchPaths && lanes * workspace_bytes(cap, 64, cap) > sv->cfg.hbm_budget / 2) cap /= 2;
-  if (lanes * workspace_bytes(cap, 64, cap) > sv->cfg.hbm_budget / 2)
+  while (cap > kMinBatchPaths && lanes * lane_bytes(cap) > sv->cfg.hbm_budget / 2) cap /= 2;
+  if (lanes * lane_bytes(cap) > sv->cfg.hbm_budget / 2)
     return fail(ctx, MPT_E_BUDGET, "HBM budget too small for the ray queues of the server's lanes");
   sv->cap = (uint32_t)cap;
-  sv->queue_bytes = lanes * workspace_bytes(cap, 64, cap);
+  sv->bidir_lane_bytes = lane_bytes(cap) - workspace_bytes(cap, 64, cap);
+  // sized for the deepest subpaths, so every bidirectional batch fits
+  sv->bidir_cap = std::min<uint32_t>(sv->cap, bidir_capacity(ctx, bdpt_max_depth(), sv->bidir_lane_bytes));
+  sv->queue_bytes = lanes * lane_bytes(cap);
   if (reset_totals(ctx)) return MPT_E_DEVICE;
   ctx->servers.push_back(sv.get());
   *out = sv.release();
@@ -347,6 +363,7 @@ int mpt_submit(mpt_server *sv, mpt_scene *scene, const mpt_job *job, uint64_t *h
   if (!sv || !scene || !job || !handle) return MPT_E_INVALID;
   if (scene->ctx != sv->ctx) return fail(sv->ctx, MPT_E_INVALID, "scene belongs to another context");
   int rc = check_job(sv->ctx, job);
+  if (!rc) rc = check_scene_job(sv->ctx, scene, job);
   if (rc) return rc;
   std::unique_ptr<Job> j(new (std::nothrow) Job());
   if (!j) return MPT_E_NOMEM;

@@ -1262,6 +1262,558 @@ V3 trace_path(const orc_scene &sc, const orc_job &job, const Camera &cam, uint32
   return L;
 }
 
+// ---------------------------------------------------------------------------
+// Bidirectional path tracing: BidirectionalPathTracer::sample
+// (renderers/renderers/BidirectionalPathTracer.cpp:34-174) over Path
+// (renderers/utils/Path.cpp) and PathVertex (renderers/utils/PathVertex.cpp).
+//
+// One sample builds an eye subpath (generateE, :233-249) and, unless it ended
+// on an emitter, a light subpath (generateL, :222-231); Path::append(Path)
+// (Path.cpp:339-515) joins them and completes every vertex's cumulative
+// light-side and eye-side values; every split (s, t) of every prefix/suffix
+// length k is then weighted by the power heuristic over its k + 1 densities
+// (:111-165) and the sum is clamped to [0, 1] per channel (RADIANCE_HACK,
+// :167-171).
+//
+// Random numbers (shared with the device): the camera stream as for the
+// path tracer; eye vertex j samples its continuation from (depth j, stream
+// 2) plus stream 3 for a dielectric's channel and draws the event of its
+// construction (PathVertex::_init, PathVertex.cpp:58-66, whose Phong lobe a
+// last vertex keeps) from stream 4; the light subpath uses depths
+// kLightDepth + i the same way, its start (EmitterSampler::sample) streams 0
+// and 1 of depth kLightDepth; the join of the two (stream 0 of kJoinDepth).
+// The device supports pinhole cameras and quad, mesh and point lights here;
+// the oracle returns NaN for a thin lens or an emitting sphere.
+// ---------------------------------------------------------------------------
+const uint32_t kLightDepth = 1024;
+const uint32_t kJoinDepth = 2047;   // the join's forced roulette (Path.cpp:364-376)
+
+// EQ (common/utilities.h:240) with its precedence as written; Vector::
+// operator== (common/math/Vector.inl:73-80) compares component by component.
+inline bool eq_ref(double a, double b) {
+  return std::fabs(a - b) < kEps || std::fabs(a - b / ((std::fabs(b) > std::fabs(a)) ? b : a)) <= kEps;
+}
+inline bool veq(V3 a, V3 b) { return eq_ref(a.x, b.x) && eq_ref(a.y, b.y) && eq_ref(a.z, b.z); }
+// fs == SpectralSampleSet::black() (SpectralSampleSet.inl:141-153): every
+// channel EQ 0, i.e. |c| <= EPSILON
+inline bool black_ref(V3 a) { return std::fabs(a.x) <= kEps && std::fabs(a.y) <= kEps && std::fabs(a.z) <= kEps; }
+// SpectralSampleSet::getMaxSample (SpectralSampleSet.inl:388-398)
+inline double max_sample(V3 a) {
+  double m = a.x;
+  if (a.y > m) m = a.y;
+  if (a.z > m) m = a.z;
+  return m;
+}
+
+enum { BV_SURFACE = 0, BV_EMITTER = 1, BV_SENSOR = 2 };
+enum { LOBE_ABSORB = 0, LOBE_DIFFUSE = 1, LOBE_SPECULAR = 2 };
+
+struct BVert {
+  V3 p, n;
+  bool has_normal = true;      // a point light has none (SurfacePointNormal::dot = 1)
+  int kind = BV_SURFACE;       // the BSDF it scatters with: material, emitter or sensor
+  uint32_t gid = ORC_MISS;
+  const Material *m = nullptr; // material of the surface (emission of an emitter vertex)
+  int point = -1;              // point light of a light subpath that starts on one
+  int lobe = LOBE_DIFFUSE;     // ModifiedPhongBSDF event index of PathVertex::event
+  double area = 0;             // Shape::getSurfaceArea of its shape
+  double GL = 1, GE = 1, pdfL = 1, pdfE = 1, pL = 1, pE = 1;
+  V3 alphaL{1, 1, 1}, alphaE{1, 1, 1}, fs;
+};
+
+inline double ndot(const BVert &v, V3 w) { return v.has_normal ? dot(v.n, w) : 1.0; }
+
+// pt->emitter->isEmitter() of the vertex's surface point
+inline bool on_emitter(const BVert &v) { return v.point >= 0 || (v.m && v.m->emitter()); }
+// BSDF::isSpecular: dielectric and sensor (DielectricBSDF.h:64, Sensor.h:153)
+inline bool bv_specular(const BVert &v) {
+  return v.kind == BV_SENSOR || (v.kind == BV_SURFACE && v.m->specular());
+}
+
+inline double channel_ior(const Material &m, int c) {
+  return c > 0 && m.ior_gb[c - 1] > 0 ? m.ior_gb[c - 1] : m.ior;
+}
+
+// Emitter::getLe of the vertex's emitter: OrientedEmitter (emitters/
+// OrientedEmitter.cpp:17-23) on the normal's side, OmniEmitter (OmniEmitter.
+// cpp:15-17) everywhere; NullEmitter (NullEmitter.cpp:20-22) gives black.
+inline V3 bv_Le(const orc_scene &sc, const BVert &v, V3 wo) {
+  if (v.point >= 0) return sc.points[v.point].Le;
+  if (!v.m || !v.m->emitter()) return V3();
+  return dot(v.n, wo) > 0 ? v.m->Le : V3();
+}
+// Emitter::getLe0 = radiant exitance, power / area (Emitter.cpp:30-49,
+// 64-66): pi Le for an oriented emitter, the power 4 pi Le for a point.
+inline V3 bv_Le0(const orc_scene &sc, const BVert &v) {
+  if (v.point >= 0) return sc.points[v.point].Le * (4 * M_PI);
+  return v.m->Le * M_PI;
+}
+
+// BSDF::getPd(Event(dir, event)) of the vertex with its incident direction
+// set to wi.
+double bv_pdf(const BVert &v, V3 wi, V3 dir) {
+  if (v.kind == BV_SENSOR) {
+    // Sensor::getPd (materials/Sensor.cpp:85-95)
+    if (veq(wi, -v.n)) return 1;
+    return veq(dir, v.n) ? 1 : 0;
+  }
+  if (v.kind == BV_EMITTER) {
+    // Emitter::getPd -> its sampler (Emitter.cpp:160-165): UniformOnSphere
+    // returns 1 (stats/samplers/UniformOnSphereSampler.cpp:32-34); an
+    // oriented emitter's DiffuseBSDF sees wi = -normalS always
+    // (OrientedEmitter.cpp:25-31), so only the side of dir counts.
+    if (v.point >= 0) return 1;
+    return dot(v.n, dir) > 0 ? 1.0 / M_PI : 0.0;
+  }
+  const Material &m = *v.m;
+  if (m.type == MAT_DIFFUSE) return bsdf_pdf_diffuse(v.n, wi, dir);   // DiffuseBSDF.cpp:30-38
+  if (m.type == MAT_PHONG) {
+    // ModifiedPhongBSDF::getPd (ModifiedPhongBSDF.cpp:131-162) by event index
+    const double kda = avg(m.kd), ksa = avg(m.ks);
+    if (v.lobe == LOBE_ABSORB) return 1 - (kda + ksa);
+    const double cosA = dot(v.n, dir);
+    if (cosA <= 0) return 0;
+    double pdf;
+    if (v.lobe == LOBE_SPECULAR) {
+      const double c = std::fabs(dot(reflect(wi, v.n), dir));
+      pdf = std::min(1.0, (m.exponent + 1) / (2 * M_PI) * std::pow(c, m.exponent));
+    } else {
+      pdf = std::fabs(dot(v.n, dir)) / M_PI;
+    }
+    return pdf / cosA;
+  }
+  // DielectricBSDF::getPd (DielectricBSDF.cpp:106-127) with iorIndex 0: the
+  // share of channels whose refraction is dir, or 1 for the mirror image
+  const V3 N = dot(v.n, wi) > 0 ? -v.n : v.n;
+  if (dot(dir, N) < 0) {
+    int cnt = 0;
+    for (int c = 0; c < 3; ++c) cnt += veq(dir, refract(wi, v.n, kAirIor, channel_ior(m, c))) ? 1 : 0;
+    return cnt / 3.0;
+  }
+  return veq(dir, reflect(wi, N)) ? 1 : 0;
+}
+
+// BSDF::evaluate(wi, dir) of the vertex (light arriving along wi, leaving
+// along dir).
+V3 bv_f(const BVert &v, V3 wi, V3 dir) {
+  if (v.kind == BV_SENSOR) {
+    // Sensor::evaluate = getWe1(-wi) = (-wi == normalS) (Sensor.cpp:47-62, 98-107)
+    return veq(-wi, v.n) ? V3(1, 1, 1) : V3();
+  }
+  if (v.kind == BV_EMITTER) {
+    // Emitter::evaluate = getLe1(dir) = getLe(dir) / getLe0() (Emitter.cpp:
+    // 68-70, 167-171): 1 / (4 pi) for a point, 1 / pi on an oriented
+    // emitter's side (a channel with no emission gives 0; the reference
+    // asserts against dividing by it)
+    const V3 le0 = v.point >= 0 ? V3(4 * M_PI, 4 * M_PI, 4 * M_PI) : V3(M_PI, M_PI, M_PI);
+    V3 le = v.point >= 0 ? V3(1, 1, 1) : (dot(v.n, dir) > 0 ? V3(1, 1, 1) : V3());
+    const V3 ch = v.point >= 0 ? V3(1, 1, 1) : v.m->Le;   // channels that emit
+    V3 r;
+    for (int c = 0; c < 3; ++c) r[c] = ch[c] != 0 ? le[c] / le0[c] : 0.0;
+    return r;
+  }
+  const Material &m = *v.m;
+  if (m.type != MAT_DIELECTRIC) return bsdf_eval(m, v.n, wi, dir);
+  // DielectricBSDF::evaluate (DielectricBSDF.cpp:129-158)
+  const V3 N = dot(v.n, wi) > 0 ? -v.n : v.n;
+  if (dot(dir, N) < 0) {
+    V3 s;
+    for (int c = 0; c < 3; ++c) s[c] = veq(dir, refract(wi, v.n, kAirIor, channel_ior(m, c))) ? 1 : 0;
+    return mul(m.ks, s);
+  }
+  return veq(dir, reflect(wi, N)) ? m.ks : V3();
+}
+
+// BSDF::sample of the vertex: the event's direction (zero when absorbed)
+// and Phong lobe.  u: (u0, u1) direction, u2 lobe / refraction choice; uch
+// a dielectric's channel (Random::sampleInt(0, 3), stats/Random.h:87-89).
+V3 bv_sample(const BVert &v, V3 wi, const double u[4], double uch, int &lobe) {
+  lobe = LOBE_DIFFUSE;
+  if (v.kind == BV_SENSOR) return v.n;   // Sensor::sample (Sensor.cpp:72-83)
+  if (v.kind == BV_EMITTER) {
+    if (v.point >= 0) {
+      // UniformOnSphereSampler::sample (UniformOnSphereSampler.cpp:22-30)
+      const double z = 1 - 2 * u[0], r = std::sqrt(std::max(0.0, 1 - z * z)), phi = 2 * M_PI * u[1];
+      return V3(r * std::cos(phi), r * std::sin(phi), z);
+    }
+    return cos_random(v.n, u[0], u[1]);   // DiffuseBSDF::sample with wi = -normalS
+  }
+  const Material &m = *v.m;
+  if (m.type == MAT_DIFFUSE) {
+    if (dot(v.n, wi) > 0) return V3();   // DiffuseBSDF.cpp:21-28
+    return cos_random(v.n, u[0], u[1]);
+  }
+  if (m.type == MAT_PHONG) {
+    // ModifiedPhongBSDF::sample (ModifiedPhongBSDF.cpp:84-129): the event's
+    // direction as drawn, below the surface or not
+    const double kda = avg(m.kd), ksa = avg(m.ks);
+    const double p0 = u[2] - kEps;
+    if (p0 <= kda) return cos_random(v.n, u[0], u[1]);
+    if (p0 <= kda + ksa) {
+      lobe = LOBE_SPECULAR;
+      const double alpha = std::acos(std::pow(u[0], 1.0 / (m.exponent + 1)));
+      const double phi = 2.0 * M_PI * u[1];
+      const V3 R = reflect(wi, v.n);
+      V3 r = R, uu, vv;
+      if (std::fabs(dot(v.n, r)) < .95) {
+        uu = normalized(cross(r, v.n));
+        vv = cross(uu, r);
+      } else {
+        onb(r, vv, uu);
+        r = normalized(r);
+      }
+      const V3 w = from_spherical(alpha, phi);
+      const V3 wo = uu * w.x + r * w.y + vv * w.z;
+      return dot(v.n, wo) > 0 ? wo : R;
+    }
+    lobe = LOBE_ABSORB;
+    return V3();
+  }
+  // DielectricBSDF::sample (DielectricBSDF.cpp:55-104), iorIndex 0: a random
+  // channel's IOR
+  const V3 N = dot(v.n, wi) > 0 ? -v.n : v.n;
+  const int c = std::min(2, (int)std::floor(uch * 3));
+  const double ior = channel_ior(m, c);
+  const V3 wt = refract(wi, v.n, kAirIor, ior);
+  if (!is_black(wt)) {
+    const double Fr = fresnel(wi, v.n, kAirIor, ior);
+    if (u[2] < m.transparency * (1 - Fr)) return wt;
+  }
+  return reflect(wi, N);
+}
+
+// The Phong lobe of a new vertex's construction event (PathVertex::_init).
+inline int init_lobe(const BVert &v, double u) {
+  if (v.kind != BV_SURFACE || v.m->type != MAT_PHONG) return LOBE_DIFFUSE;
+  const double kda = avg(v.m->kd), ksa = avg(v.m->ks);
+  const double p0 = u - kEps;
+  return p0 <= kda ? LOBE_DIFFUSE : (p0 <= kda + ksa ? LOBE_SPECULAR : LOBE_ABSORB);
+}
+
+// Path::_initE (Path.cpp:647-693): eye vertex z scattering from wi (toward
+// the camera side) to wo (a vertex at distance t with point pt); rr: the
+// uniform of the roulette, taken when `roulette` or when fs is black.
+bool init_E(BVert &z, V3 wi, V3 wo, double t, const BVert &pt, bool roulette, double rr, V3 &alphaE, double &pE) {
+  if (t < kEps) {   // GEOMETRY_BOUND
+    z.GE = 0;
+    return false;
+  }
+  z.GE = std::fabs(ndot(z, wo) * ndot(pt, -wo)) / (t * t);
+  z.pdfL = bv_pdf(z, -wo, -wi);
+  z.fs = bv_f(z, -wo, -wi);
+  z.pdfE = bv_pdf(z, wi, wo);
+  if (z.GE == 0 || z.pdfE == 0) return false;
+  if (roulette || black_ref(z.fs)) {
+    const double q = std::min(0.95, max_sample(z.fs) / z.pdfE);
+    if (q <= rr) {
+      z.pdfE = 0;
+      return false;
+    }
+    z.pdfE *= q;
+  }
+  pE = (z.pdfE * z.GE) * z.pE;
+  alphaE = mul(z.fs / z.pdfE, z.alphaE);
+  return true;
+}
+
+// Path::_initL (Path.cpp:600-645): the light-side twin.
+bool init_L(BVert &y, V3 wi, V3 wo, double t, const BVert &pt, bool roulette, double rr, V3 &alphaL, double &pL) {
+  if (t < kEps) {
+    y.GL = 0;
+    return false;
+  }
+  y.GL = std::fabs(ndot(y, wo) * ndot(pt, -wo)) / (t * t);
+  y.pdfE = bv_pdf(y, -wo, -wi);
+  y.fs = bv_f(y, wi, wo);
+  y.pdfL = bv_pdf(y, wi, wo);
+  if (y.GL == 0 || y.pdfL == 0) return false;
+  if (roulette || black_ref(y.fs)) {
+    const double q = std::min(0.95, max_sample(y.fs) / y.pdfL);
+    if (q <= rr) {
+      y.pdfL = 0;
+      return false;
+    }
+    y.pdfL *= q;
+  }
+  pL = (y.pdfL * y.GL) * y.pL;
+  alphaL = mul(y.fs / y.pdfL, y.alphaL);
+  return true;
+}
+
+// A surface vertex at a closest hit (SurfacePoint::init, core/SurfacePoint.
+// cpp:49-78).
+BVert surface_vertex(const orc_scene &sc, uint32_t gid, V3 p) {
+  BVert v;
+  v.p = p;
+  v.gid = gid;
+  v.n = sc.normal_at(gid, p);
+  v.m = &sc.material_of(gid);
+  // Shape::getSurfaceArea: a triangle's mesh is taken to have an area of at
+  // least EPSILON; quads and spheres (pi) have theirs
+  v.area = gid < sc.nt ? 1.0 : sc.shape_area(gid);
+  return v;
+}
+
+struct Visibility {
+  const orc_scene &sc;
+  PathCounters &ctr;
+  // Scene::intersects(Ray(a, dir(a -> b)), |b - a|) (core/Scene.cpp:110-114)
+  bool visible(V3 a, V3 b) {
+    V3 w = b - a;
+    const double d = len(w);
+    ctr.shadow++;
+    return !sc.occluded(Ray(a, w / d), d);
+  }
+};
+
+V3 bdpt_path(const orc_scene &sc, const orc_job &job, const Camera &cam, uint32_t pixel, uint32_t sample,
+             PathCounters &ctr) {
+  const uint32_t cap = job.max_depth + 1;   // vertices per subpath (the reference has no cap)
+  const double nan = std::numeric_limits<double>::quiet_NaN();
+  if (cam.aperture > 0) return V3(nan, nan, nan);
+  for (const orc_scene::Light &l : sc.lights)
+    if (l.kind == 1) return V3(nan, nan, nan);
+  if (sc.lights.empty() && sc.points.empty()) return V3(nan, nan, nan);   // EmitterSampler.cpp:51-53 asserts
+  auto rnd = [&](uint32_t depth, uint32_t stream) { return draw(job.seed, job.job_id, pixel, sample, depth, stream); };
+  auto trace = [&](V3 o, V3 d, uint32_t &gid) {
+    ctr.extension++;
+    return sc.closest(Ray(o, d), gid);
+  };
+
+  // --- eye subpath (generateE; Path::prepend, :277-306 and _samplePathVertex
+  // :517-598 with adjoint = true) ------------------------------------------
+  std::vector<BVert> eye;
+  {
+    const Uniforms cu = rnd(kCameraDepth, 0);
+    double fx, fy;
+    film_position(job, pixel, sample, cu.u[0], cu.u[1], fx, fy);
+    // Camera::getPoint at the film position (BidirectionalPathTracer.cpp:
+    // 38-45): the eye, normal = the camera ray's direction
+    // (Camera::_getGeometricNormal, cameras/Camera.cpp:48-50); pE = 1
+    BVert c;
+    c.p = cam.eye;
+    c.n = cam.dir(fx, fy);
+    c.kind = BV_SENSOR;
+    c.area = 0;   // Camera::_getSurfaceArea is Shape's 0
+    eye.push_back(c);
+    ctr.camera++;
+  }
+  for (;;) {
+    const uint32_t T = (uint32_t)eye.size();
+    if (T >= cap) break;
+    const bool roulette = T >= 2;
+    BVert &v = eye.back();
+    V3 wi;
+    if (T > 1) wi = normalized(v.p - eye[T - 2].p);
+    const Uniforms su = rnd(T - 1, 2);
+    const double uch = rnd(T - 1, 3).u[0];
+    int lobe;
+    const V3 wo = bv_sample(v, wi, su.u, uch, lobe);
+    v.lobe = lobe;
+    if (is_black(wo)) break;
+    uint32_t gid;
+    const double t = trace(v.p, wo, gid);
+    if (!(t > kEps && t < kInf)) break;
+    if (T == 1) wi = -wo;
+    BVert nv = surface_vertex(sc, gid, v.p + wo * t);
+    V3 alphaE;
+    double pE;
+    if (!init_E(v, wi, wo, t, nv, roulette, su.u[3], alphaE, pE)) break;
+    nv.GL = v.GE;
+    nv.GE = 1;
+    nv.alphaL = V3(1, 1, 1);
+    nv.alphaE = alphaE;
+    nv.pL = 1;
+    nv.pE = pE;
+    nv.lobe = init_lobe(nv, rnd(T, 4).u[0]);
+    eye.push_back(nv);
+    if (on_emitter(eye.back())) break;
+  }
+
+  // --- light subpath (generateL; Path::append, :255-275 with
+  // EmitterSampler::sample / getPd, stats/samplers/EmitterSampler.cpp:51-96)
+  std::vector<BVert> light;
+  if (!on_emitter(eye.back())) {
+    const Uniforms a = rnd(kLightDepth, 0), b = rnd(kLightDepth, 1);
+    const uint32_t j = sc.select_light(a.u[0]);
+    BVert e;
+    e.kind = BV_EMITTER;
+    if (j >= sc.lights.size()) {
+      e.point = (int)(j - sc.lights.size());
+      e.p = sc.points[e.point].p;
+      e.has_normal = false;
+      e.area = 0;
+    } else {
+      const orc_scene::Light &L = sc.lights[j];
+      V3 ln;
+      sc.light_point(L, a.u[1], a.u[2], a.u[3], b.u[0], e.p, ln);
+      e.n = ln;
+      e.m = &sc.mats[L.mat];
+      e.area = L.area;
+      e.gid = L.kind == 0 ? sc.nt + sc.ns + L.index : ORC_MISS;
+    }
+    const double pA = sc.emit_area > 0 ? 1.0 / sc.emit_area : 1.0;
+    // PathVertex(pt, pA, 1) (PathVertex.cpp:31-56): alphaL = Le0 / pL
+    e.pL = pA;
+    e.pE = 1;
+    e.alphaL = bv_Le0(sc, e) / pA;
+    e.alphaE = V3(1, 1, 1);
+    light.push_back(e);
+    for (;;) {
+      const uint32_t S = (uint32_t)light.size();
+      if (S >= cap) break;
+      BVert &v = light.back();
+      V3 wi;
+      if (S > 1) wi = normalized(v.p - light[S - 2].p);
+      const Uniforms su = rnd(kLightDepth + S - 1, 2);
+      const double uch = rnd(kLightDepth + S - 1, 3).u[0];
+      int lobe;
+      const V3 wo = bv_sample(v, wi, su.u, uch, lobe);
+      v.lobe = lobe;
+      if (is_black(wo)) break;
+      uint32_t gid;
+      const double t = trace(v.p, wo, gid);
+      if (!(t > kEps && t < kInf)) break;
+      if (S == 1) wi = -wo;
+      BVert nv = surface_vertex(sc, gid, v.p + wo * t);
+      V3 alphaL;
+      double pL;
+      if (!init_L(v, wi, wo, t, nv, true, su.u[3], alphaL, pL)) break;
+      nv.GL = 1;
+      nv.GE = v.GL;
+      nv.alphaL = alphaL;
+      nv.alphaE = V3(1, 1, 1);
+      nv.pL = pL;
+      nv.pE = 1;
+      nv.lobe = init_lobe(nv, rnd(kLightDepth + S, 4).u[0]);
+      light.push_back(nv);
+    }
+  }
+
+  // --- Path::append(const Path &) (Path.cpp:339-515) ----------------------
+  const uint32_t S = (uint32_t)light.size(), T = (uint32_t)eye.size(), n = S + T;
+  std::vector<BVert> m(light);
+  for (uint32_t j = T; j--;) m.push_back(eye[j]);
+  if (on_emitter(m[0])) m[0].kind = BV_EMITTER;
+  Visibility vis{sc, ctr};
+  bool valid = true;
+  if (S > 0 && T > 0) {
+    BVert &y = m[S - 1], &z = m[S];
+    V3 wo = z.p - y.p;
+    const double d = len(wo);
+    wo = wo / d;
+    V3 alpha;
+    double pA;
+    // the roulette a black fs forces at either end draws from its own stream
+    const Uniforms ju = rnd(kJoinDepth, 0);
+    V3 wi = S > 1 ? normalized(y.p - m[S - 2].p) : -wo;
+    if (!init_L(y, wi, wo, d, z, false, ju.u[0], alpha, pA)) valid = false;
+    wi = T > 1 ? normalized(z.p - m[S + 1].p) : wo;
+    if (!init_E(z, wi, -wo, d, y, false, ju.u[1], alpha, pA)) valid = false;
+    if (valid && !vis.visible(y.p, z.p)) {
+      y.GL = z.GE = 0;
+      valid = false;
+    }
+  }
+  // the eye-side values of the light vertices ...
+  for (uint32_t i = S; i--;) {
+    BVert &y = m[i];
+    const BVert &z = m[i + 1];
+    y.alphaE = (z.pdfE > 0 && z.GE > 0) ? mul(z.alphaE, z.fs / z.pdfE) : V3();
+    y.pE = z.pE * z.pdfE * z.GE;
+  }
+  // ... and the light-side values of the eye vertices
+  for (uint32_t i = 0; i < T; ++i) {
+    BVert &z = m[S + i];
+    if (S + i == 0) {
+      // an eye subpath that ended on an emitter: it starts the light side
+      // with the emitter's EmitterSampler density
+      if (on_emitter(z)) {
+        const double pA = sc.emit_area > 0 ? 1.0 / sc.emit_area : 1.0;
+        const V3 wo = n > 1 ? normalized(m[1].p - z.p) : V3();
+        z.kind = BV_EMITTER;
+        z.alphaL = bv_Le0(sc, z) / pA;
+        z.pL = pA;
+        z.fs = bv_f(z, V3(), wo);
+        z.pdfL = bv_pdf(z, V3(), wo);
+      } else {
+        z.alphaL = V3();
+        z.pL = 0;
+        z.fs = V3();
+        z.pdfL = 0;
+      }
+    } else {
+      const BVert &y = m[S + i - 1];
+      z.alphaL = (y.pdfL > 0 && y.GL > 0) ? mul(y.alphaL, y.fs / y.pdfL) : V3();
+      z.pL = y.pL * y.pdfL * y.GL;
+    }
+  }
+
+  // --- getPd / getContribution of every split (Path.cpp:48-197) ----------
+  // visibility of the pair (u, v), traced at most once
+  std::vector<int8_t> seen((size_t)n * n, -1);
+  auto pair_visible = [&](uint32_t u, uint32_t v) {
+    int8_t &c = seen[(size_t)u * n + v];
+    if (c < 0) c = vis.visible(m[u].p, m[v].p) ? 1 : 0;
+    return c == 1;
+  };
+  auto pd = [&](uint32_t s, uint32_t t) -> double {
+    const uint32_t u = s - 1, v = n - t;
+    if (s == 0) return m[v].area < kEps ? 0.0 : m[v].pE;
+    if (t == 0) return m[u].area < kEps ? 0.0 : m[u].pL;
+    const BVert &y = m[u], &z = m[v];
+    if (bv_specular(y) || bv_specular(z)) return 0;
+    if (s + t == n) return y.pL * z.pE;
+    if (len(z.p - y.p) < kEps) return 0;
+    if (!pair_visible(u, v)) return 0;
+    return y.pL * z.pE;
+  };
+  auto contribution = [&](uint32_t s, uint32_t t) -> V3 {
+    const uint32_t u = s - 1, v = n - t;
+    if (s == 0) {
+      if (m[v].area < kEps) return V3();
+      const V3 wo = v < n - 1 ? normalized(m[v + 1].p - m[v].p) : V3();
+      return mul(bv_Le(sc, m[v], wo), m[v].alphaE);
+    }
+    if (t == 0) return V3();   // NullSensor::getWe is black off the camera, which has no area
+    const BVert &y = m[u], &z = m[v];
+    if (bv_specular(y) || bv_specular(z)) return V3();
+    if (s + t == n) return mul(mul(mul(y.alphaL, y.fs) * y.GL, z.fs), z.alphaE);
+    V3 wo = z.p - y.p;
+    const double d = len(wo);
+    if (d < kEps) return V3();
+    if (!pair_visible(u, v)) return V3();
+    wo = wo / d;
+    const V3 ywi = u > 0 ? normalized(y.p - m[u - 1].p) : V3();
+    const V3 wo2 = v < n - 1 ? normalized(m[v + 1].p - z.p) : -wo;
+    const double G = std::fabs(ndot(y, wo) * ndot(z, -wo)) / (d * d);
+    return mul(mul(mul(y.alphaL, bv_f(y, ywi, wo)) * G, bv_f(z, wo, wo2)), z.alphaE);
+  };
+  const uint32_t length = n - (valid ? 0 : 1);
+  V3 L;
+  std::vector<double> pdf(n + 2);
+  for (uint32_t k = 2; k <= length; ++k) {
+    double sum = 0;
+    for (uint32_t s = 0; s <= k; ++s) {
+      const double p = pd(s, k - s);
+      pdf[s] = p * p;   // power heuristic, beta = 2
+      sum += pdf[s];
+    }
+    if (sum == 0) continue;
+    for (uint32_t s = 0; s <= k; ++s)
+      if (pdf[s] > 0) L = L + contribution(s, k - s) * (pdf[s] / sum);
+  }
+  // RADIANCE_HACK: CLAMP(value, 0, 1) per channel (common/utilities.h:195-200)
+  for (int c = 0; c < 3; ++c) L[c] = L[c] >= 1 ? 1.0 : (L[c] <= 0 ? 0.0 : L[c]);
+  return L;
+}
+
+V3 sample_path(const orc_scene &sc, const orc_job &job, const Camera &cam, uint32_t pixel, uint32_t sample,
+               PathCounters &ctr) {
+  if (job.renderer == 1) return bdpt_path(sc, job, cam, pixel, sample, ctr);
+  return trace_path(sc, job, cam, pixel, sample, ctr);
+}
+
 template <class F>
 void parallel_for(uint64_t n, int nthreads, uint64_t chunk, F &&f) {
   if (nthreads <= 1 || n <= chunk) {
@@ -1292,7 +1844,7 @@ void orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t ou
 
 // Bumped with every change of an exported signature; pyoracle refuses a
 // library built from older sources.
-int orc_interface_version(void) { return 11; }
+int orc_interface_version(void) { return 12; }
 
 orc_scene *orc_scene_create(uint32_t nv, const float *vertices, uint32_t nt, const uint32_t *indices,
                             const uint32_t *tri_material, uint32_t ns, const float *spheres,
@@ -1453,7 +2005,7 @@ void orc_render(const orc_scene *s, const orc_job *job, double *sum, double *sum
       const uint32_t pixel = job->pix_begin + (uint32_t)i;
       V3 acc, acc2;
       for (uint32_t smp = 0; smp < job->spp; ++smp) {
-        const V3 L = trace_path(*s, *job, cam, pixel, smp, pc[tid]);
+        const V3 L = sample_path(*s, *job, cam, pixel, smp, pc[tid]);
         acc = acc + L;
         acc2 = acc2 + mul(L, L);
       }
@@ -1483,7 +2035,7 @@ void orc_render_paths(const orc_scene *s, const orc_job *job, uint64_t n, const 
   parallel_for(n, std::max(1, nthreads), 1024, [&](uint64_t b, uint64_t e, int) {
     PathCounters pc;
     for (uint64_t i = b; i < e; ++i) {
-      const V3 L = trace_path(*s, *job, cam, pixel[i], sample[i], pc);
+      const V3 L = sample_path(*s, *job, cam, pixel[i], sample[i], pc);
       L_out[3 * i] = L.x;
       L_out[3 * i + 1] = L.y;
       L_out[3 * i + 2] = L.z;
@@ -1503,7 +2055,7 @@ void orc_render_film(const orc_scene *s, const orc_job *job, double *rgbw_out, i
   parallel_for(n, std::max(1, nthreads), 1024, [&](uint64_t b, uint64_t e, int) {
     PathCounters pc;
     for (uint64_t i = b; i < e; ++i) {
-      const V3 v = trace_path(*s, *job, cam, (uint32_t)(i / spp), (uint32_t)(i % spp), pc);
+      const V3 v = sample_path(*s, *job, cam, (uint32_t)(i / spp), (uint32_t)(i % spp), pc);
       L[3 * i] = v.x;
       L[3 * i + 1] = v.y;
       L[3 * i + 2] = v.z;

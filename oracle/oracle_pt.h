@@ -44,7 +44,7 @@ typedef struct orc_scene orc_scene;
  * mesh triangles) with the reference build parameters.  kd_min_prims /
  * kd_max_depth of 0 select the reference defaults (3, 24).  use_naive != 0
  * replaces both kd-trees by NaiveSpatialAccel.  Returns NULL on bad input. */
-int orc_interface_version(void);   /* 11 */
+int orc_interface_version(void);   /* 12 */
 orc_scene *orc_scene_create(uint32_t nv, const float *vertices,
                             uint32_t nt, const uint32_t *indices, const uint32_t *tri_material,
                             uint32_t ns, const float *spheres, const uint32_t *sphere_material,
@@ -92,6 +92,8 @@ typedef struct orc_job {
   uint32_t direct_off;     /* 1: efficientDirect = false (PathTracer.cpp:50-52, 70-73) */
   uint32_t light_select;   /* 1: each light sample draws one light by emitted power
                               (EmitterSampler's draw) and divides by its probability */
+  uint32_t renderer;       /* 0 path tracer (PathTracer.cpp), 1 bidirectional path tracer
+                              (BidirectionalPathTracer.cpp:34-174); max_depth caps each subpath */
 } orc_job;
 
 /* Renders pixels [pix_begin, pix_end) of the job.  sum/sumsq are

@@ -15,7 +15,7 @@ LIB_PATH = os.path.join(HERE, "liboracle_pt.so")
 MISS = 0xFFFFFFFF
 
 
-INTERFACE_VERSION = 11
+INTERFACE_VERSION = 12
 
 
 def build():
@@ -35,7 +35,8 @@ class OrcJob(ctypes.Structure):
                 ("job_id", ctypes.c_uint32), ("pix_begin", ctypes.c_uint32), ("pix_end", ctypes.c_uint32),
                 ("direct_samples", ctypes.c_uint32), ("filter", ctypes.c_uint32),
                 ("filter_support", ctypes.c_uint32), ("filter_params", ctypes.c_double * 2),
-                ("direct_off", ctypes.c_uint32), ("light_select", ctypes.c_uint32)]
+                ("direct_off", ctypes.c_uint32), ("light_select", ctypes.c_uint32),
+                ("renderer", ctypes.c_uint32)]
 
 
 _lib = None
@@ -229,6 +230,7 @@ def make_job(job, pix_begin=0, pix_end=None):
     j.direct_samples = getattr(job, "direct_samples", 0)
     j.direct_off = getattr(job, "direct_off", 0)
     j.light_select = getattr(job, "light_select", 0)
+    j.renderer = getattr(job, "renderer", 0)
     j.filter, j.filter_support = getattr(job, "filter", 0), getattr(job, "filter_support", 0)
     fp = getattr(job, "filter_params", (0.0, 0.0))
     j.filter_params[0], j.filter_params[1] = fp[0], fp[1]
