@@ -14,7 +14,8 @@ bounded random sample of the same job's pixels at 65,536 spp, on all of the
 process's host cores, which also gives the RMSE against an FP64 reference),
 and the systems workloads: `batched` (64 jobs over the room, the soup and the
 2048^2 terrain through one server, against the same jobs one at a time) and
-`serving` (Poisson arrivals, p50 / p99 latency).
+`serving` (Poisson arrivals, p50 / p99 latency), and `bidir`: the
+bidirectional renderer on the room at 256^2 and 1024^2 beside its oracle.
 """
 import argparse
 import json
@@ -342,6 +343,59 @@ def systems(m, ctx, args, reuse=None):
     return out
 
 
+def bench_bidir(m, ctx, args, threads):
+    """The bidirectional renderer (MPT_RENDERER_BIDIR) on the room: the
+    CPU-runnable room job (256^2, 16 spp, depth 5) and the same camera at
+    1024^2 (16.8 M paths), paths/s and rays/s (camera, subpath extension and
+    connection visibility rays), beside the oracle's bidirectional
+    restatement on host threads over a bounded sample of the small job."""
+    arrays = m.generate(m.MPT_SCENE_ROOM)
+    scene = m.Scene(ctx, arrays)
+    out = {"renderer": "bidirectional (BidirectionalPathTracer.cpp:34-174)", "scene": "room"}
+    for key, (w, h) in (("room_256", (256, 256)), ("room_1024", (1024, 1024))):
+        job = m.make_job(arrays.camera, w, h, 16, 5, seed=1, renderer=m.MPT_RENDERER_BIDIR)
+        film = m.Film(ctx, w, h)
+        for _ in range(2):
+            film.render(scene, job)
+        ctx.sync()
+        steps = 5
+        t0 = time.perf_counter()
+        runs = [film.render(scene, job) for _ in range(steps)]
+        ctx.sync()
+        el = time.perf_counter() - t0
+        r = runs[0]
+        rays = r["extension_rays"] + r["shadow_rays"]
+        out[key] = {"width": w, "height": h, "spp": 16, "max_depth": 5, "ms_per_job": 1000 * el / steps,
+                    "paths_per_s": w * h * 16 * steps / el, "mrays_per_s": rays * steps / el / 1e6,
+                    "extension_rays": r["extension_rays"], "visibility_rays": r["shadow_rays"]}
+        film.close()
+        log(f"bidir {key}: {out[key]}")
+    if not args.no_cpu:
+        # the oracle on host threads over whole rows of the 1024^2 job, in a
+        # fixed random order, until about cpu_seconds (at most 10 s) of work
+        from oracle import pyoracle as po
+        job = m.make_job(arrays.camera, 1024, 1024, 16, 5, seed=1, renderer=m.MPT_RENDERER_BIDIR)
+        osc = po.OracleScene(arrays, threads=threads)
+        rows = np.random.default_rng(4321).permutation(1024)
+        done, t_cpu = 0, 0.0
+        while done < 1024 and t_cpu < min(args.cpu_seconds, 10.0):
+            px = np.repeat(np.arange(rows[done] * 1024, (rows[done] + 1) * 1024, dtype=np.uint32), 16)
+            sm = np.tile(np.arange(16, dtype=np.uint32), 1024)
+            t = time.perf_counter()
+            osc.render_paths(job, px, sm, threads=threads)
+            t_cpu += time.perf_counter() - t
+            done += 1
+        osc.close()
+        paths = done * 1024 * 16
+        out["cpu_baseline"] = {"value": paths / t_cpu, "unit": "paths/s", "cores": threads, "kind": "port",
+                               "sample": f"{done} random rows of the 1024^2 x 16 spp room job ({paths} paths, "
+                                         f"{t_cpu:.1f} s on {threads} threads)"}
+        out["speedup_vs_cpu"] = out["room_1024"]["paths_per_s"] / out["cpu_baseline"]["value"]
+        log(f"bidir cpu baseline: {out['cpu_baseline']}")
+    scene.close()
+    return out
+
+
 def apply_engine_args(m, ctx, args):
     if args.batch_paths:
         ctx.set_option(m.MPT_OPT_BATCH_PATHS, args.batch_paths)
@@ -375,6 +429,25 @@ def main_systems(args):
     print(json.dumps(out), flush=True)
 
 
+def main_bidir(args):
+    """--workload bidir: only the bidirectional renderer's room jobs."""
+    import milton_amd as m
+    ctx = m.Context(0)
+    apply_engine_args(m, ctx, args)
+    nproc = os.cpu_count() or 1
+    threads = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else nproc
+    b = bench_bidir(m, ctx, args, threads)
+    r = b["room_1024"]
+    out = {"metric": "paths/s", "value": r["paths_per_s"], "unit": "paths/s", "n_gpus": 1, "steps": 5,
+           "warmup": 2, "ms_per_step": r["ms_per_job"], "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+           "config": {"workload": "bidirectional path tracing of the room, 1024x1024, 16 spp, depth 5",
+                      "engine": engine_settings(ctx, m)},
+           "cpu_baseline": b.get("cpu_baseline"), "bidir": b}
+    ctx.close()
+    print(json.dumps(out), flush=True)
+
+
 def timed_render(m, ctx, film, scene, job, lanes):
     """One render with per-launch HIP event timing on `lanes` lanes."""
     saved = ctx.get_option(m.MPT_OPT_LANES)
@@ -396,6 +469,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline (and the FP64 RMSE it yields)")
     ap.add_argument("--no-rmse", action="store_true", help="skip the RMSE against a 65,536-spp device reference")
     ap.add_argument("--no-systems", action="store_true", help="skip the batched and serving workloads")
+    ap.add_argument("--no-bidir", action="store_true", help="skip the bidirectional renderer's room jobs")
     ap.add_argument("--batch-paths", type=int, default=0)
     ap.add_argument("--stack", type=int, default=0, help="traversal short-stack rows (8, 12, 16, 24)")
     ap.add_argument("--persistent", type=int, default=-1,
@@ -409,9 +483,10 @@ def main():
     ap.add_argument("--no-clip", action="store_true", help="kd build without triangle clipping")
     ap.add_argument("--tune", action="store_true",
                     help="autotune (kd leaf size and engine settings) on a sample of the job before timing")
-    ap.add_argument("--workload", default="soup", choices=["soup", "batched", "serving"],
+    ap.add_argument("--workload", default="soup", choices=["soup", "batched", "serving", "bidir"],
                     help="soup: the headline kd-tree stress scene (plus the systems workloads unless "
-                         "--no-systems); batched / serving: only the systems workloads")
+                         "--no-systems); batched / serving: only the systems workloads; bidir: only the "
+                         "bidirectional renderer's room jobs")
     ap.add_argument("--serve-seconds", type=float, default=10.0)
     ap.add_argument("--no-systems-tune", action="store_true",
                     help="batched / serving at the context's settings instead of the autotuner's")
@@ -422,6 +497,9 @@ def main():
         print(f"bench.py measures one GPU: this build runs on a single MI355X; --gpus {args.gpus} is not "
               f"supported (paths shard by pixel range; see the design notes)", file=sys.stderr)
         sys.exit(2)
+    if args.workload == "bidir":
+        main_bidir(args)
+        return
     if args.workload != "soup":
         main_systems(args)
         return
@@ -532,6 +610,10 @@ def main():
     else:
         out["cpu_baseline"] = None
     film.close()
+    if not args.no_bidir:
+        nproc = os.cpu_count() or 1
+        threads = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else nproc
+        out["bidir"] = bench_bidir(m, ctx, args, threads)
     if not args.no_systems:
         out.update(systems(m, ctx, args, reuse={m.MPT_SCENE_SOUP: (arrays, scene)}))
     scene.close()

@@ -52,7 +52,9 @@ class PathTracer {
    *             "maxDepth" (the cap), "seed" (-seed=, MiltonApp.cpp:329),
    *             "efficientDirect" (PathTracer.cpp:23),
    *             "noDirectSamples" (1..64, DirectIllumination.cpp:33),
-   *             "device" (HIP device), "batchPaths" (queue depth)
+   *             "device" (HIP device), "batchPaths" (queue depth),
+   *             "bidirectional" (1: BidirectionalPathTracer instead,
+   *             MiltonJSONSceneLoader.cpp:720-721)
    *   output    "width" / "height" (RenderOutput.cpp:34)
    *   kd-tree   "kdMinPrimitives" (leaf size), "kdMaxDepth", "kdCostTraversal",
    *             "kdEmptyBias", "kdNoThreads" (kdTreeAccel.h:97-104)       */

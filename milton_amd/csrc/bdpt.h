@@ -53,6 +53,7 @@ enum { LOBE_ABSORB = 0, LOBE_DIFFUSE = 1, LOBE_SPECULAR = 2 };
 constexpr uint32_t I_NO_NORMAL = 16u;   // a point light: SurfacePointNormal::dot is 1
 constexpr uint32_t I_AREA = 32u;        // its shape's area is at least EPSILON
 constexpr uint32_t I_POINT = 64u;       // the emitter is point light (info >> 8)
+constexpr uint32_t I_SPEC = 128u;       // k_bd_combine's copy: the vertex scatters specularly
 // slot flags
 constexpr uint32_t S_EYE_ON_EMITTER = 1u;   // the eye subpath ended on an emitter: no light subpath
 constexpr uint32_t S_JOIN_OK = 2u;          // the join's _initL and _initE succeeded; its pair is queued
@@ -614,7 +615,8 @@ __device__ __forceinline__ uint32_t rec_of(uint32_t i, uint32_t S, uint32_t T, u
 }
 
 // Occlusion bit of the merged pair (u, v) in a slot's mask.
-__device__ __forceinline__ uint32_t pair_bit(uint32_t u, uint32_t v) { return u * kMaxVerts + v; }
+// V: the batch's vertex records per slot.
+__device__ __forceinline__ uint32_t pair_bit(uint32_t u, uint32_t v, uint32_t V) { return u * V + v; }
 constexpr uint32_t kMaskWords = (kMaxVerts * kMaxVerts + 31) / 32;
 
 // Whether split (u, v) of a path needs a visibility test (both ends
@@ -729,7 +731,7 @@ __global__ __launch_bounds__(256) void k_bd_vis(Scene sc, const uint2 *__restric
     uint32_t id;
     if (trace<true, false, STACK>(sc, r, tlimit, pa, stk, t, id, nullptr, nullptr, nullptr,
                                   sphere_target ? kMiss : pb)) {
-      const uint32_t bit = pair_bit(u, v);
+      const uint32_t bit = pair_bit(u, v, V);
       atomicOr(masks + (uint64_t)slot * kMaskWords + bit / 32, 1u << (bit % 32));
     }
   }
@@ -743,7 +745,7 @@ __global__ __launch_bounds__(256) void k_bd_vis(Scene sc, const uint2 *__restric
 // VM: the most vertices a path of the batch can have (the per-lane arrays
 // are sized by it).
 template <int VM>
-__global__ __launch_bounds__(256) void k_bd_combine(Scene sc, const BatchDesc *__restrict__ bd,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_bd_combine(Scene sc, const BatchDesc *__restrict__ bd,
                                                     const float4 *__restrict__ verts, const uint4 *__restrict__ info,
                                                     uint32_t V, const uint32_t *__restrict__ masks,
                                                     float4 *__restrict__ Ls) {
@@ -753,10 +755,19 @@ __global__ __launch_bounds__(256) void k_bd_combine(Scene sc, const BatchDesc *_
   const uint4 in = info[slot];
   const uint32_t T = in.x, S = in.y, flags = in.z, n = S + T;
   const float4 *vs = verts + (uint64_t)slot * V * kVtx;
+  // the slot's occlusion mask in registers: its words are picked by a select
+  // chain over a compile-time count, so no lookup goes back to memory
+  constexpr uint32_t kW = (VM * VM + 31) / 32;
+  uint32_t mw[kW];
   const uint32_t *mask = masks + (uint64_t)slot * kMaskWords;
+#pragma unroll
+  for (uint32_t j = 0; j < kW; ++j) mw[j] = mask[j];
   auto occluded = [&](uint32_t u, uint32_t v) {
-    const uint32_t bit = pair_bit(u, v);
-    return (mask[bit / 32] >> (bit % 32)) & 1u;
+    const uint32_t bit = pair_bit(u, v, V), w = bit >> 5;
+    uint32_t word = mw[0];
+#pragma unroll
+    for (uint32_t j = 1; j < kW; ++j) word = w == j ? mw[j] : word;
+    return (word >> (bit & 31u)) & 1u;
   };
   // the merged path's values in the reference's terms: GL[i] is the
   // geometry term of edge (i, i + 1), GE[i] of edge (i - 1, i)
@@ -764,6 +775,7 @@ __global__ __launch_bounds__(256) void k_bd_combine(Scene sc, const BatchDesc *_
   double pL[VM], pE[VM];
   float pdfL[VM], pdfE[VM], GL[VM], GE[VM];
   uint32_t inf[VM];
+  uint32_t emits = 0;   // bit i: vertex i's surface emits
   for (uint32_t i = 0; i < n; ++i) {
     const Vtx v = load_vtx(vs + rec_of(i, S, T, E) * kVtx);
     P[i] = v.p;
@@ -780,6 +792,11 @@ __global__ __launch_bounds__(256) void k_bd_combine(Scene sc, const BatchDesc *_
   }
   // the emitter an eye subpath ended on scatters as that emitter
   if (S == 0) inf[0] = (inf[0] & ~3u) | K_EMITTER;
+  // specular and emitting flags, looked up once per vertex
+  for (uint32_t i = 0; i < n; ++i) {
+    if (specular_of(sc, inf[i])) inf[i] |= I_SPEC;
+    if (on_emitter(sc, inf[i])) emits |= 1u << i;
+  }
   bool valid = true;
   if (S > 0) {
     if (!(flags & S_JOIN_OK)) {
@@ -822,7 +839,7 @@ __global__ __launch_bounds__(256) void k_bd_combine(Scene sc, const BatchDesc *_
     const uint32_t u = s - 1, v = n - t;
     if (s == 0) return has_area(v) ? pE[v] : 0.0;
     if (t == 0) return has_area(u) ? pL[u] : 0.0;
-    if (specular_of(sc, inf[u]) || specular_of(sc, inf[v])) return 0.0;
+    if ((inf[u] | inf[v]) & I_SPEC) return 0.0;
     if (s + t == n) return pL[u] * pE[v];
     if (len(sub(P[v], P[u])) < kEps) return 0.0;
     if (occluded(u, v)) return 0.0;
@@ -836,11 +853,11 @@ __global__ __launch_bounds__(256) void k_bd_combine(Scene sc, const BatchDesc *_
       const float3 wo = v < n - 1 ? normalize(sub(P[v + 1], P[v])) : f3(0, 0, 0);
       // Emitter::getLe of the vertex's surface (OrientedEmitter.cpp:17-23;
       // NullEmitter is black)
-      if (!on_emitter(sc, inf[v]) || !(dot(Nn[v], wo) > 0.0f)) return f3(0, 0, 0);
+      if (!((emits >> v) & 1u) || !(dot(Nn[v], wo) > 0.0f)) return f3(0, 0, 0);
       return mul(le_of(sc.mats[mat_of(inf[v])]), aE[v]);
     }
     if (t == 0) return f3(0, 0, 0);   // NullSensor::getWe is black off the camera, which has no area
-    if (specular_of(sc, inf[u]) || specular_of(sc, inf[v])) return f3(0, 0, 0);
+    if ((inf[u] | inf[v]) & I_SPEC) return f3(0, 0, 0);
     if (s + t == n) return mul(mul(mul(aL[u], fs[u]), GL[u]), mul(fs[v], aE[v]));
     float3 wo = sub(P[v], P[u]);
     const float d = len(wo);

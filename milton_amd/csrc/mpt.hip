@@ -1393,13 +1393,15 @@ int enqueue_batch_bidir(mpt_context *ctx, Lane &ln, mpt_scene *s, uint32_t D, bo
   }
   if ((rc = toc())) return rc;
   MPT_LAUNCHED(ctx, "k_bd_vis");
-  if (V <= 8)
-    bdpt::k_bd_combine<8><<<gcap, 256, 0, st>>>(s->dev, w.desc.p, w.bverts.p, w.binfo.p, V, w.bmasks.p, w.Ls.p);
-  else if (V <= 14)
-    bdpt::k_bd_combine<14><<<gcap, 256, 0, st>>>(s->dev, w.desc.p, w.bverts.p, w.binfo.p, V, w.bmasks.p, w.Ls.p);
-  else
-    bdpt::k_bd_combine<bdpt::kMaxVerts><<<gcap, 256, 0, st>>>(s->dev, w.desc.p, w.bverts.p, w.binfo.p, V,
-                                                              w.bmasks.p, w.Ls.p);
+  // per-lane arrays sized by the batch's path length: subpaths of 3, 5, 7
+  // or 12 bounces
+  auto combine = [&](auto kern) {
+    kern<<<gcap, 256, 0, st>>>(s->dev, w.desc.p, w.bverts.p, w.binfo.p, V, w.bmasks.p, w.Ls.p);
+  };
+  if (V <= 8) combine(bdpt::k_bd_combine<8>);
+  else if (V <= 12) combine(bdpt::k_bd_combine<12>);
+  else if (V <= 16) combine(bdpt::k_bd_combine<16>);
+  else combine(bdpt::k_bd_combine<bdpt::kMaxVerts>);
   MPT_LAUNCHED(ctx, "k_bd_combine");
   k_film<<<gcap, 256, 0, st>>>(w.desc.p, w.Ls.p);
   if (filtered) k_film_filtered<<<gcap, 256, 0, st>>>(w.desc.p, w.Ls.p, w.pos.p);

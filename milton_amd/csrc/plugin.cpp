@@ -153,6 +153,8 @@ int PathTracer::render() {
   job.seed = (uint32_t)getValue("seed", 1);
   job.direct_samples = (uint32_t)getValue("noDirectSamples", 1);
   job.direct_off = getValue("efficientDirect", 1) == 0 ? 1u : 0u;   // PathTracer.cpp:23
+  // 1: the loader's bidirectionalPathTracer (MiltonJSONSceneLoader.cpp:720-721)
+  job.renderer = getValue("bidirectional", 0) != 0 ? MPT_RENDERER_BIDIR : MPT_RENDERER_PATH;
   film_.assign(3ull * job.width * job.height, 0.0f);
   return mpt_render(scene_, &job, film_.data(), &stats_);
 }

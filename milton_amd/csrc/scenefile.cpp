@@ -31,9 +31,11 @@
 //   output             width, height, size, dimensions (default 480 x 480),
 //                      tonemap (:783; "default" = Reinhard), type and filter
 //                      (box, triangle, gaussian, mitchell, lanczosSinc)
-//   renderer           noSuperSamples (spp), noDirectSamples, seed, maxDepth,
-//                      efficientDirect
-//                      (a cap the reference does not have; default 8)
+//   renderer           type (bidirectionalPathTracer / bidirPathTracer select
+//                      the bidirectional renderer, :720-721; any other type
+//                      renders with the path tracer), noSuperSamples (spp),
+//                      noDirectSamples, seed, maxDepth, efficientDirect
+//                      (maxDepth: a cap the reference does not have; default 8)
 //   point              with an "omni" emitter: a point light (PointShape,
 //                      OmniEmitter)
 // Anything else that would change the image (cone, cylinder, blob,
@@ -297,6 +299,7 @@ struct Settings {
   bool autofocus = false;
   uint32_t width = 480, height = 480, spp = 1, max_depth = 8, seed = 1, direct = 1;
   uint32_t direct_off = 0;   // the renderer's efficientDirect = false
+  uint32_t renderer = MPT_RENDERER_PATH;
   uint32_t tonemap = MPT_TONEMAP_REINHARD;   // RenderOutput::init's "default"
   uint32_t filter = 0, filter_support = 0;    // mpt_job.filter: 0 pixel bin, else 1 + MPT_FILTER_*
   float filter_params[2] = {0, 0};
@@ -365,6 +368,10 @@ class Loader {
     if (num(v.find("maxDepth"), x)) set_.max_depth = (uint32_t)std::max(1.0, x);
     if (num(v.find("seed"), x)) set_.seed = (uint32_t)x;
     if (num(v.find("noDirectSamples"), x)) set_.direct = (uint32_t)std::max(1.0, x);
+    // MiltonJSONSceneLoader::_parseRenderer (loaders/MiltonJSONSceneLoader.cpp:720-721)
+    const JVal *type = v.find("type");
+    if (type && type->kind == JVal::STR && (type->str == "bidirectionalPathTracer" || type->str == "bidirPathTracer"))
+      set_.renderer = MPT_RENDERER_BIDIR;
     // PathTracer::init (renderers/renderers/PathTracer.cpp:22-26)
     const JVal *ed = v.find("efficientDirect");
     if (ed) {
@@ -1184,6 +1191,7 @@ int mpt_scenefile_job(const mpt_scenefile *sf, mpt_job *job) {
   job->seed = s.seed;
   job->direct_samples = s.direct;
   job->direct_off = s.direct_off;
+  job->renderer = s.renderer;
   job->filter = s.filter;
   job->filter_support = s.filter_support;
   job->filter_params[0] = s.filter_params[0];

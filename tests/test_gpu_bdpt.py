@@ -112,20 +112,29 @@ def test_films_are_deterministic_and_batch_independent(ctx):
     assert np.allclose(film.reshape(-1, 3), L.mean(axis=1), rtol=1e-5, atol=1e-6)
 
 
-def test_served_beside_path_tracing_jobs(ctx):
-    a = m.generate(m.MPT_SCENE_ROOM)
-    sc = m.Scene(ctx, a)
-    jobs = [_job(a, 40, 32, 8, 5, 20 + i, job_id=i) for i in range(3)]
-    pt = [m.make_job(a.camera, 48, 40, 8, 5, seed=30 + i, job_id=10 + i) for i in range(3)]
-    srv = m.Server(ctx)
-    hs = [srv.submit(sc, j) for j in jobs + pt]
-    srv.drain()
-    films = [srv.result(h, j.width, j.height) for h, j in zip(hs, jobs + pt)]
-    mem = srv.memory()
-    srv.close()
-    assert mem["lane_bytes"] <= mem["queue_bytes"]
-    for f, j in zip(films, jobs + pt):
-        assert np.array_equal(f, sc.render(j)[0])
+def test_served_beside_path_tracing_jobs():
+    """On a fresh context (the lanes hold only what the server asked for),
+    bidirectional and path-tracing jobs through one server: each film equals
+    the job rendered alone, and the lanes stay within the bytes the server
+    reserved for ray queues and subpath records."""
+    c = m.Context(0)
+    try:
+        a = m.generate(m.MPT_SCENE_ROOM)
+        sc = m.Scene(c, a)
+        jobs = [_job(a, 40, 32, 8, 5, 20 + i, job_id=i) for i in range(3)]
+        pt = [m.make_job(a.camera, 48, 40, 8, 5, seed=30 + i, job_id=10 + i) for i in range(3)]
+        srv = m.Server(c)
+        hs = [srv.submit(sc, j) for j in jobs + pt]
+        srv.drain()
+        films = [srv.result(h, j.width, j.height) for h, j in zip(hs, jobs + pt)]
+        mem = srv.memory()
+        print(f"[bidir served] {mem}", flush=True)
+        srv.close()
+        assert mem["lane_bytes"] <= mem["queue_bytes"]
+        for f, j in zip(films, jobs + pt):
+            assert np.array_equal(f, sc.render(j)[0])
+    finally:
+        c.close()
 
 
 def test_refused_scenes_and_cameras(ctx):

@@ -155,6 +155,17 @@ def test_defaults():
     assert a.tonemap == m.MPT_TONEMAP_REINHARD
 
 
+@pytest.mark.parametrize("kind, want", [("bidirectionalPathTracer", m.MPT_RENDERER_BIDIR),
+                                        ("bidirPathTracer", m.MPT_RENDERER_BIDIR),
+                                        ("pathTracer", m.MPT_RENDERER_PATH), (None, m.MPT_RENDERER_PATH)])
+def test_renderer_type(kind, want):
+    """MiltonJSONSceneLoader.cpp:720-721: both names select the bidirectional
+    renderer; other types render with the path tracer."""
+    r = '"renderer" : { "type" : "%s" }, ' % kind if kind else ""
+    _, job = m.load_scenefile(text='{ "scenefile" : { %s"scene" : { "sphere" : { } } } }' % r)
+    assert job.renderer == want
+
+
 @pytest.mark.parametrize("name, want", [("reinhard", m.MPT_TONEMAP_REINHARD), ("default", m.MPT_TONEMAP_REINHARD),
                                         ("null", m.MPT_TONEMAP_GAMMA)])
 def test_output_tonemap(name, want):
