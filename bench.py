@@ -142,18 +142,19 @@ def rmse_vs_reference(m, ctx, scene, job):
 def measured_traffic(engine, kd_leaf):
     """HBM bytes per k_extend launch from the committed counter summary
     (FETCH_SIZE + WRITE_SIZE, copy-calibrated), if it was taken with the same
-    batch size, stack, kernel form and kd leaf size; else None."""
+    batch size, stack, kernel form, block size and kd leaf size; else None."""
     import re
     try:
         text = open(TRAFFIC_PROFILE).read()
     except OSError:
         return None
     m = re.search(r"traffic_bytes_per_launch k_extend batch_paths=(\d+) stack=(\d+) persistent=(\d+) "
-                  r"kd_leaf=(\d+): (\d+)", text)
+                  r"trace_block=(\d+) kd_leaf=(\d+): (\d+)", text)
     if not m:
         return None
-    bp, st, pe, leaf, val = (int(x) for x in m.groups())
-    if (bp, st, pe, leaf) != (engine["batch_paths"], engine["stack"], engine["persistent"], kd_leaf):
+    bp, st, pe, tb, leaf, val = (int(x) for x in m.groups())
+    if (bp, st, pe, tb, leaf) != (engine["batch_paths"], engine["stack"], engine["persistent"], engine["trace_block"],
+                                  kd_leaf):
         return None
     return val
 
@@ -161,7 +162,8 @@ def measured_traffic(engine, kd_leaf):
 def engine_settings(ctx, m):
     return {"batch_paths": ctx.get_option(m.MPT_OPT_BATCH_PATHS), "stack": ctx.get_option(m.MPT_OPT_STACK),
             "persistent": ctx.get_option(m.MPT_OPT_PERSISTENT), "lanes": ctx.get_option(m.MPT_OPT_LANES),
-            "tree_top": ctx.get_option(m.MPT_OPT_TREE_TOP)}
+            "tree_top": ctx.get_option(m.MPT_OPT_TREE_TOP), "trace_block": ctx.get_option(m.MPT_OPT_TRACE_BLOCK),
+            "shade_block": ctx.get_option(m.MPT_OPT_SHADE_BLOCK)}
 
 
 def system_scenes(m, ctx, reuse=None):
@@ -407,6 +409,10 @@ def apply_engine_args(m, ctx, args):
         ctx.set_option(m.MPT_OPT_LANES, args.lanes)
     if args.tree_top >= 0:
         ctx.set_option(m.MPT_OPT_TREE_TOP, args.tree_top)
+    if args.trace_block:
+        ctx.set_option(m.MPT_OPT_TRACE_BLOCK, args.trace_block)
+    if args.shade_block:
+        ctx.set_option(m.MPT_OPT_SHADE_BLOCK, args.shade_block)
 
 
 def main_systems(args):
@@ -477,6 +483,8 @@ def main():
     ap.add_argument("--lanes", type=int, default=0, help="engine lanes (streams) batches alternate over")
     ap.add_argument("--tree-top", type=int, default=-1,
                     help="1: the persistent closest-hit kernel reads the kd-tree's top from LDS (-1: library default)")
+    ap.add_argument("--trace-block", type=int, default=0, help="threads per persistent closest-hit block (128, 256)")
+    ap.add_argument("--shade-block", type=int, default=0, help="threads per shading block (256, 512)")
     ap.add_argument("--kd-leaf", type=int, default=0, help="kd leaf size (0: library default)")
     ap.add_argument("--kd-ci", type=float, default=0.0, help="kd SAH intersection cost (0: default)")
     ap.add_argument("--kd-eb", type=float, default=0.0, help="kd SAH empty-side factor (0: default)")

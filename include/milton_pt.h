@@ -291,6 +291,11 @@ enum {
   MPT_OPT_LANES = 7,       /* streams (each with its own batch workspace) that consecutive batches
                               alternate over, 1..4, default 3: one batch's kernel tails overlap
                               another's bulk */
+  MPT_OPT_TRACE_BLOCK = 10, /* threads per block of the closest-hit kernel in persistent waves
+                               (form 1): 128 (default) or 256 (the same waves in half the blocks;
+                               with MPT_OPT_TREE_TOP, 256) */
+  MPT_OPT_SHADE_BLOCK = 11, /* threads per block of the shading kernel: 512 (default) or 256 (one
+                               queue atomic per block, so 512 halves the atomics) */
   MPT_OPT_TREE_TOP = 9,    /* 1: the closest-hit kernel in persistent waves (form 1) stages the top
                               of the kd-tree (its first 512 nodes, laid out breadth-first) in LDS
                               and reads those nodes from there; 0 (default) reads every node
@@ -512,10 +517,11 @@ typedef struct mpt_tune_result {
   uint32_t trials;              /* sample renders timed */
   double paths_per_s;           /* sampled rate at the chosen settings */
   double baseline_paths_per_s;  /* sampled rate at the settings on entry */
+  uint32_t trace_block, shade_block;   /* MPT_OPT_TRACE_BLOCK, MPT_OPT_SHADE_BLOCK chosen */
 } mpt_tune_result;
 
-/* Engine settings (persistent, stack, batch paths) for a resident scene; the
- * fastest are left set on the scene's context. */
+/* Engine settings (persistent, stack, block sizes, batch paths) for a
+ * resident scene; the fastest are left set on the scene's context. */
 int mpt_tune(mpt_scene *scene, const mpt_job *job, uint64_t sample_paths, mpt_tune_result *result);
 /* Also the kd leaf size (1..16, hill climb from kd->leaf_size or 4): builds
  * the scene once per candidate and returns the fastest build in *scene. */

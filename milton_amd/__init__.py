@@ -28,6 +28,7 @@ MPT_OPT_BATCH_PATHS, MPT_OPT_TIMING, MPT_OPT_COUNTERS, MPT_OPT_GRAPHS, MPT_OPT_S
 MPT_OPT_LANES = 7
 MPT_OPT_FAIL_BATCH = 8
 MPT_OPT_TREE_TOP = 9
+MPT_OPT_TRACE_BLOCK, MPT_OPT_SHADE_BLOCK = 10, 11
 MPT_TONEMAP_REINHARD, MPT_TONEMAP_GAMMA = 0, 1
 MPT_RENDERER_PATH, MPT_RENDERER_BIDIR = 0, 1
 MPT_FILTER_BOX, MPT_FILTER_TRIANGLE, MPT_FILTER_GAUSSIAN, MPT_FILTER_MITCHELL, MPT_FILTER_LANCZOS = 0, 1, 2, 3, 4
@@ -117,7 +118,8 @@ class ServerConfig(ctypes.Structure):
 class TuneResult(ctypes.Structure):
     _fields_ = [("batch_paths", ctypes.c_uint64), ("stack", ctypes.c_uint32), ("persistent", ctypes.c_uint32),
                 ("leaf_size", ctypes.c_uint32), ("trials", ctypes.c_uint32), ("paths_per_s", ctypes.c_double),
-                ("baseline_paths_per_s", ctypes.c_double)]
+                ("baseline_paths_per_s", ctypes.c_double), ("trace_block", ctypes.c_uint32),
+                ("shade_block", ctypes.c_uint32)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

@@ -286,7 +286,8 @@ MPT_HD float refine_hit(const float4 *rec, float3 o, float3 d, float t32, float2
 // Sugerman 2005).  `stk` points at this lane's column of a [STACK][kBlock]
 // uint2 array: a ring of STACK rows that holds at most STACK - 1 entries, so
 // the row after the newest entry is always free and every descent step can
-// store into it whether or not it pushes.  skip_origin is the primitive the
+// store into it whether or not it pushes (BLOCK: the threads of the block,
+// one column each).  skip_origin is the primitive the
 // ray leaves (self-intersection guard in place of the reference's FP64
 // t > EPSILON); skip_target the triangle or quad a shadow ray aims at (its
 // only crossing of that primitive is the end point, which the reference's
@@ -303,9 +304,9 @@ MPT_HD float refine_hit(const float4 *rec, float3 o, float3 d, float t32, float2
 // Node references and ring positions are byte offsets, so the descent does
 // no index arithmetic: a child is (word0 & ~3) + 8 * side, a ring row is
 // row * kRow.
-template <bool ANY, int STACK>
+template <bool ANY, int STACK, int BLOCK = kBlock>
 struct Walker {
-  static constexpr uint32_t kRow = kBlock * sizeof(uint2);           // bytes per ring row
+  static constexpr uint32_t kRow = BLOCK * sizeof(uint2);            // bytes per ring row
   static constexpr uint32_t kCap = (STACK - 1) * kRow;               // entries held, in bytes
   static constexpr bool kPow2 = (STACK & (STACK - 1)) == 0;
   float tmin, tmax, troot, best;

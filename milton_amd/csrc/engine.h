@@ -195,6 +195,8 @@ struct mpt_context {
   bool graphs = true;
   uint32_t stack = 8;                // short-stack entries of the traversal kernels
   bool tree_top = false;             // persistent closest hit reads the tree's top from LDS
+  uint32_t trace_block = 128;        // threads per block of the persistent closest-hit kernel (128 or 256)
+  uint32_t shade_block = 512;        // threads per block of the shading kernel (256 or 512)
   uint32_t persistent = 1;           // closest-hit kernel form: 0 grid-stride, 1 persistent waves with
                                      // lane refill, 2 one ray per lane over a capacity-sized grid
   std::atomic<uint32_t> fail_batch{0};   // MPT_OPT_FAIL_BATCH countdown (0: off); a server's worker reads it

@@ -252,11 +252,11 @@ constexpr uint32_t kChunk = MPT_CHUNK;     // rays per claim
 // nodes from there.
 constexpr uint32_t kTopNodes = 512;
 
-template <bool COUNT, int STACK, bool TOP>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(STACK <= 8 ? 8 : (STACK <= 16 ? 4 : 2)))) void k_extend_persistent(
+template <bool COUNT, int STACK, bool TOP, int BLOCK>
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(STACK <= 8 ? 8 : (STACK <= 16 ? 4 : 2)))) void k_extend_persistent(
     Scene sc, const float4 *__restrict__ q, const uint32_t *__restrict__ count, float2 *__restrict__ hits,
     unsigned long long *__restrict__ counters, uint32_t *__restrict__ fetch) {
-  __shared__ uint2 stack[STACK * kBlock];
+  __shared__ uint2 stack[STACK * BLOCK];
   __shared__ uint2 top_lds[TOP ? kTopNodes : 1];
   uint2 *stk = stack + threadIdx.x;
   if (TOP) {
@@ -269,7 +269,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(STACK <= 8 
   uint32_t n_nodes = 0, n_refs = 0;
   uint32_t ray = kMiss;
   RaySetup r;
-  Walker<false, STACK> w;
+  Walker<false, STACK, BLOCK> w;
   for (;;) {
     const bool need = ray == kMiss;
     const uint64_t idle = __ballot(need);
@@ -496,10 +496,6 @@ __device__ __forceinline__ void light_point(const Scene &sc, const LightDev &ld,
   }
 }
 
-#ifndef MPT_SHADE_BLOCK
-#define MPT_SHADE_BLOCK 256
-#endif
-constexpr int kShadeBlock = MPT_SHADE_BLOCK;   // threads per shading block
 // Waves per SIMD the shading kernel is compiled for (its register budget):
 // the most at which the single-sample form does not spill; the form with
 // several light samples per light gets one fewer.
@@ -516,7 +512,7 @@ constexpr int kShadeBlock = MPT_SHADE_BLOCK;   // threads per shading block
 // The grid holds as many blocks as fit on the device at once (every block
 // strides over the queue the same number of times, give or take one), and
 // each pass appends its shadow groups and extension rays with one exchange.
-template <bool kMulti, bool kEnv>
+template <bool kMulti, bool kEnv, int kShadeBlock>
 __global__ __launch_bounds__(kShadeBlock) __attribute__((amdgpu_waves_per_eu(kMulti ? MPT_SHADE_WAVES - 1 : MPT_SHADE_WAVES))) void k_shade(
     Scene sc, const BatchDesc *__restrict__ bd, uint32_t depth, const float4 *__restrict__ qin,
     const float2 *__restrict__ hits, const uint32_t *__restrict__ count_in, float4 *__restrict__ qout,
@@ -1196,14 +1192,20 @@ void launch_extend_t(mpt_context *ctx, hipStream_t st, uint32_t cap, uint32_t g,
   if (ctx->persistent == 1) {
     if (ctx->tree_top) {
       if (counters)
-        k_extend_persistent<true, STACK, true><<<g, kBlock, 0, st>>>(sc, q, count, hits, counters, fetch);
+        k_extend_persistent<true, STACK, true, 256><<<g, 256, 0, st>>>(sc, q, count, hits, counters, fetch);
       else
-        k_extend_persistent<false, STACK, true><<<g, kBlock, 0, st>>>(sc, q, count, hits, nullptr, fetch);
+        k_extend_persistent<false, STACK, true, 256><<<g, 256, 0, st>>>(sc, q, count, hits, nullptr, fetch);
+    } else if (ctx->trace_block == 128) {
+      // the same waves in twice as many blocks
+      if (counters)
+        k_extend_persistent<true, STACK, false, 128><<<2 * g, 128, 0, st>>>(sc, q, count, hits, counters, fetch);
+      else
+        k_extend_persistent<false, STACK, false, 128><<<2 * g, 128, 0, st>>>(sc, q, count, hits, nullptr, fetch);
     } else {
       if (counters)
-        k_extend_persistent<true, STACK, false><<<g, kBlock, 0, st>>>(sc, q, count, hits, counters, fetch);
+        k_extend_persistent<true, STACK, false, 256><<<g, 256, 0, st>>>(sc, q, count, hits, counters, fetch);
       else
-        k_extend_persistent<false, STACK, false><<<g, kBlock, 0, st>>>(sc, q, count, hits, nullptr, fetch);
+        k_extend_persistent<false, STACK, false, 256><<<g, 256, 0, st>>>(sc, q, count, hits, nullptr, fetch);
     }
     return;
   }
@@ -1293,17 +1295,24 @@ int enqueue_batch(mpt_context *ctx, Lane &ln, mpt_scene *s, uint32_t D, bool mul
                   instrument && ctx->counters ? w.totals.p + 3 : nullptr, w.counts.p + 2 * D + 1 + d);
     if ((rc = toc())) return rc;
     if ((rc = tic(1))) return rc;
-    auto shade = [&](auto kern) {
-      const uint32_t sgrid = resident_grid(ctx, kern, kShadeBlock);
-      kern<<<sgrid, kShadeBlock, 0, st>>>(s->dev, w.desc.p, d, qin, w.hits.p, w.counts.p + d, qout,
-                                          w.counts.p + d + 1, w.sg.p, w.sr.p, w.counts.p + D + 1 + d, w.Ls.p,
-                                          w.flags.p, w.pjs.p);
+    auto shade = [&](auto kern, int block) {
+      const uint32_t sgrid = resident_grid(ctx, kern, block);
+      kern<<<sgrid, block, 0, st>>>(s->dev, w.desc.p, d, qin, w.hits.p, w.counts.p + d, qout,
+                                    w.counts.p + d + 1, w.sg.p, w.sr.p, w.counts.p + D + 1 + d, w.Ls.p,
+                                    w.flags.p, w.pjs.p);
     };
     const bool env = s->dev.env_w != 0;
-    if (multi)
-      env ? shade(k_shade<true, true>) : shade(k_shade<true, false>);
-    else
-      env ? shade(k_shade<false, true>) : shade(k_shade<false, false>);
+    if (ctx->shade_block == 512) {
+      if (multi)
+        env ? shade(k_shade<true, true, 512>, 512) : shade(k_shade<true, false, 512>, 512);
+      else
+        env ? shade(k_shade<false, true, 512>, 512) : shade(k_shade<false, false, 512>, 512);
+    } else {
+      if (multi)
+        env ? shade(k_shade<true, true, 256>, 256) : shade(k_shade<true, false, 256>, 256);
+      else
+        env ? shade(k_shade<false, true, 256>, 256) : shade(k_shade<false, false, 256>, 256);
+    }
     if ((rc = toc())) return rc;
     if (L) {
       if ((rc = tic(2))) return rc;
@@ -2423,6 +2432,16 @@ int mpt_set_option(mpt_context *ctx, uint32_t option, uint64_t value) {
       ctx->tree_top = value != 0;
       drop_graphs(ctx, 0);
       return MPT_OK;
+    case MPT_OPT_TRACE_BLOCK:
+      if (value != 128 && value != 256) return fail(ctx, MPT_E_INVALID, "trace block must be 128 or 256");
+      ctx->trace_block = (uint32_t)value;
+      drop_graphs(ctx, 0);
+      return MPT_OK;
+    case MPT_OPT_SHADE_BLOCK:
+      if (value != 256 && value != 512) return fail(ctx, MPT_E_INVALID, "shade block must be 256 or 512");
+      ctx->shade_block = (uint32_t)value;
+      drop_graphs(ctx, 0);
+      return MPT_OK;
     case MPT_OPT_FAIL_BATCH:
       if (value > 0xffffffffull) return fail(ctx, MPT_E_INVALID, "fail_batch out of range");
       ctx->fail_batch.store((uint32_t)value);
@@ -2450,6 +2469,8 @@ int mpt_get_option(mpt_context *ctx, uint32_t option, uint64_t *value) {
     case MPT_OPT_LANES: *value = ctx->num_lanes; return MPT_OK;
     case MPT_OPT_FAIL_BATCH: *value = ctx->fail_batch.load(); return MPT_OK;
     case MPT_OPT_TREE_TOP: *value = ctx->tree_top; return MPT_OK;
+    case MPT_OPT_TRACE_BLOCK: *value = ctx->trace_block; return MPT_OK;
+    case MPT_OPT_SHADE_BLOCK: *value = ctx->shade_block; return MPT_OK;
     default: return fail(ctx, MPT_E_INVALID, "unknown option");
   }
 }

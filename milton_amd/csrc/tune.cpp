@@ -8,6 +8,8 @@
 //   persistent  closest-hit kernel form (one ray per lane, or persistent
 //               waves with lane refill),
 //   stack       short-stack entries of the kd walk (8, 12, 16),
+//   blocks      threads per block of the persistent closest-hit kernel (128,
+//               256) and of the shading kernel (256, 512),
 //   batch paths queue depth of a wavefront batch (2^20 .. 2^26; a sample
 //               smaller than a candidate runs as one batch),
 //   leaf size   kd leaf size (1 .. 16, mpt_tune_scene only; rebuilds the tree).
@@ -62,16 +64,26 @@ int time_sample(mpt_scene *scene, const Sample &smp, mpt_film *film, double &rat
 }
 
 struct Settings {
-  uint32_t persistent, stack, batch_paths;
+  uint32_t persistent, stack, batch_paths, trace_block, shade_block;
+  bool operator==(const Settings &o) const {
+    return persistent == o.persistent && stack == o.stack && batch_paths == o.batch_paths &&
+           trace_block == o.trace_block && shade_block == o.shade_block;
+  }
 };
 
-Settings current(const mpt_context *ctx) { return {ctx->persistent, ctx->stack, ctx->batch_paths}; }
+Settings current(const mpt_context *ctx) {
+  return {ctx->persistent, ctx->stack, ctx->batch_paths, ctx->trace_block, ctx->shade_block};
+}
 
 void apply(mpt_context *ctx, const Settings &s) {
-  if (ctx->persistent != s.persistent || ctx->stack != s.stack) drop_graphs(ctx, 0);
+  if (ctx->persistent != s.persistent || ctx->stack != s.stack || ctx->trace_block != s.trace_block ||
+      ctx->shade_block != s.shade_block)
+    drop_graphs(ctx, 0);
   ctx->persistent = s.persistent;
   ctx->stack = s.stack;
   ctx->batch_paths = s.batch_paths;
+  ctx->trace_block = s.trace_block;
+  ctx->shade_block = s.shade_block;
 }
 
 // One coordinate pass over the engine settings on `scene`.
@@ -84,8 +96,7 @@ int tune_engine(mpt_scene *scene, const Sample &smp, mpt_film *film, mpt_tune_re
   ++res.trials;
   res.baseline_paths_per_s = best_rate;
   auto trial = [&](Settings s) -> int {
-    if (s.persistent == best.persistent && s.stack == best.stack && s.batch_paths == best.batch_paths)
-      return MPT_OK;
+    if (s == best) return MPT_OK;
     apply(ctx, s);
     double r = 0;
     int e = time_sample(scene, smp, film, r);
@@ -108,6 +119,18 @@ int tune_engine(mpt_scene *scene, const Sample &smp, mpt_film *film, mpt_tune_re
     s.stack = k;
     if ((rc = trial(s))) return rc;
   }
+  if (best.persistent == 1) {
+    for (uint32_t t : {128u, 256u}) {
+      Settings s = best;
+      s.trace_block = t;
+      if ((rc = trial(s))) return rc;
+    }
+  }
+  for (uint32_t t : {256u, 512u}) {
+    Settings s = best;
+    s.shade_block = t;
+    if ((rc = trial(s))) return rc;
+  }
   for (uint32_t b = 20; b <= 26; ++b) {
     Settings s = best;
     s.batch_paths = 1u << b;
@@ -117,6 +140,8 @@ int tune_engine(mpt_scene *scene, const Sample &smp, mpt_film *film, mpt_tune_re
   res.persistent = best.persistent;
   res.stack = best.stack;
   res.batch_paths = best.batch_paths;
+  res.trace_block = best.trace_block;
+  res.shade_block = best.shade_block;
   res.paths_per_s = best_rate;
   return MPT_OK;
 }

@@ -25,7 +25,8 @@ def _restore(ctx, saved):
 
 
 def test_tune_engine_settings_keep_results(ctx):
-    opts = (m.MPT_OPT_BATCH_PATHS, m.MPT_OPT_STACK, m.MPT_OPT_PERSISTENT)
+    opts = (m.MPT_OPT_BATCH_PATHS, m.MPT_OPT_STACK, m.MPT_OPT_PERSISTENT, m.MPT_OPT_TRACE_BLOCK,
+            m.MPT_OPT_SHADE_BLOCK)
     saved = {o: ctx.get_option(o) for o in opts}
     try:
         a = m.generate(m.MPT_SCENE_ROOM, 0, 1)
@@ -41,8 +42,18 @@ def test_tune_engine_settings_keep_results(ctx):
         assert res["paths_per_s"] >= res["baseline_paths_per_s"] > 0
         assert ctx.get_option(m.MPT_OPT_STACK) == res["stack"]
         assert ctx.get_option(m.MPT_OPT_PERSISTENT) == res["persistent"]
+        assert res["trace_block"] in (128, 256) and res["shade_block"] in (256, 512)
+        assert ctx.get_option(m.MPT_OPT_TRACE_BLOCK) == res["trace_block"]
+        assert ctx.get_option(m.MPT_OPT_SHADE_BLOCK) == res["shade_block"]
         after = _film(ctx, scene, job)
         assert np.array_equal(before, after)
+        # every block size renders the same film
+        ctx.set_option(m.MPT_OPT_PERSISTENT, 1)
+        for tb in (128, 256):
+            for sb in (256, 512):
+                ctx.set_option(m.MPT_OPT_TRACE_BLOCK, tb)
+                ctx.set_option(m.MPT_OPT_SHADE_BLOCK, sb)
+                assert np.array_equal(before, _film(ctx, scene, job)), (tb, sb)
         # every persistent/stack form renders the same film
         for p in (0, 1, 2):
             for k in (8, 16):
@@ -55,7 +66,8 @@ def test_tune_engine_settings_keep_results(ctx):
 
 
 def test_tune_scene_leaf_size(ctx):
-    opts = (m.MPT_OPT_BATCH_PATHS, m.MPT_OPT_STACK, m.MPT_OPT_PERSISTENT)
+    opts = (m.MPT_OPT_BATCH_PATHS, m.MPT_OPT_STACK, m.MPT_OPT_PERSISTENT, m.MPT_OPT_TRACE_BLOCK,
+            m.MPT_OPT_SHADE_BLOCK)
     saved = {o: ctx.get_option(o) for o in opts}
     try:
         a = m.generate(m.MPT_SCENE_SOUP, 50_000, 1)
