@@ -10,6 +10,7 @@ import numpy as np
 import pytest
 
 import milton_amd as m
+from oracle import pyoracle as po
 from test_bdpt import KD, H, _floor_mats, _scene
 from test_gpu_render import _per_path
 
@@ -157,3 +158,28 @@ def test_refused_scenes_and_cameras(ctx):
     b.camera = a.camera
     with pytest.raises(m.MptError, match="sphere"):
         m.Scene(ctx, b).render(_job(b, 8, 8, 1, 3, 1))
+
+
+@pytest.mark.parametrize("kind, support", [(m.MPT_FILTER_MITCHELL, 2), (m.MPT_FILTER_GAUSSIAN, 2)])
+def test_filtered_film_matches_oracle(ctx, kind, support):
+    """A bidirectional job with a reconstruction filter: each sample splats
+    its clamped value onto the 3 x 3 pixels around it (the film kernels the
+    path tracer uses), as the oracle's ReconstructionRenderOutput does."""
+    a = m.generate(m.MPT_SCENE_ROOM)
+    job = m.make_job(a.camera, 40, 36, 9, 5, seed=17, film_filter=(kind, support), renderer=BIDIR)
+    f = m.Film(ctx, 40, 36)
+    f.render(m.Scene(ctx, a), job)
+    dev = f.download_raw().astype(np.float64)
+    again = m.Film(ctx, 40, 36)
+    again.render(m.Scene(ctx, a), job)
+    assert (again.download_raw() == dev).all()
+    f.close()
+    again.close()
+    orc = po.OracleScene(a).render_film(job)
+    dv = dev[..., :3] / np.maximum(dev[..., 3:], 1e-30)
+    ov = orc[..., :3] / np.maximum(orc[..., 3:], 1e-30)
+    close = np.abs(dv - ov).max(axis=2) <= 1e-3 * np.maximum(1, np.abs(ov).max(axis=2))
+    dden = np.abs(dev[..., 3] - orc[..., 3]) / np.maximum(orc[..., 3], 1e-30)
+    print(f"[bidir filtered {kind}] pixels close {close.mean():.5f}, denominators {np.mean(dden <= 1e-4):.5f}",
+          flush=True)
+    assert close.mean() >= 0.97 and np.mean(dden <= 1e-4) == 1.0
