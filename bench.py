@@ -390,8 +390,8 @@ def bench_bidir(m, ctx, args, threads):
         osc.close()
         paths = done * 1024 * 16
         out["cpu_baseline"] = {"value": paths / t_cpu, "unit": "paths/s", "cores": threads, "kind": "port",
-                               "sample": f"{done} random rows of the 1024^2 x 16 spp room job ({paths} paths, "
-                                         f"{t_cpu:.1f} s on {threads} threads)"}
+                               "sample": f"{done} of the 1024 rows (in a fixed random order) of the 1024^2 x 16 "
+                                         f"spp room job ({paths} paths, {t_cpu:.1f} s on {threads} threads)"}
         out["speedup_vs_cpu"] = out["room_1024"]["paths_per_s"] / out["cpu_baseline"]["value"]
         log(f"bidir cpu baseline: {out['cpu_baseline']}")
     scene.close()

@@ -737,6 +737,61 @@ __global__ __launch_bounds__(256) void k_bd_vis(Scene sc, const uint2 *__restric
   }
 }
 
+// The combine kernel's work grows with the square of a path's vertex count,
+// so its lanes take the slots ordered by that count (a counting sort of the
+// slot indices): a wave then holds paths of one length instead of waiting
+// for its longest.  The order within one length is free (results are per
+// slot).
+__global__ __launch_bounds__(256) void k_bd_len_count(const BatchDesc *__restrict__ bd,
+                                                      const uint4 *__restrict__ info, uint32_t *__restrict__ bins) {
+  __shared__ uint32_t sh[kMaxVerts + 1];
+  for (uint32_t k = threadIdx.x; k <= kMaxVerts; k += blockDim.x) sh[k] = 0;
+  __syncthreads();
+  const uint32_t n = bd->hdr.n_paths;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const uint4 in = info[i];
+    atomicAdd(&sh[min(in.x + in.y, kMaxVerts)], 1u);
+  }
+  __syncthreads();
+  for (uint32_t k = threadIdx.x; k <= kMaxVerts; k += blockDim.x)
+    if (sh[k]) atomicAdd(&bins[k], sh[k]);
+}
+
+// bins -> exclusive offsets
+__global__ void k_bd_len_scan(uint32_t *__restrict__ bins) {
+  if (threadIdx.x != 0) return;
+  uint32_t sum = 0;
+  for (uint32_t k = 0; k <= kMaxVerts; ++k) {
+    const uint32_t c = bins[k];
+    bins[k] = sum;
+    sum += c;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_bd_len_scatter(const BatchDesc *__restrict__ bd,
+                                                        const uint4 *__restrict__ info,
+                                                        uint32_t *__restrict__ cursors, uint32_t *__restrict__ order) {
+  __shared__ uint32_t cnt[kMaxVerts + 1], base[kMaxVerts + 1];
+  const uint32_t n = bd->hdr.n_paths;
+  for (uint32_t b0 = blockIdx.x * blockDim.x; b0 < n; b0 += gridDim.x * blockDim.x) {
+    for (uint32_t k = threadIdx.x; k <= kMaxVerts; k += blockDim.x) cnt[k] = 0;
+    __syncthreads();
+    const uint32_t i = b0 + threadIdx.x;
+    uint32_t len = 0, rank = 0;
+    if (i < n) {
+      const uint4 in = info[i];
+      len = min(in.x + in.y, kMaxVerts);
+      rank = atomicAdd(&cnt[len], 1u);
+    }
+    __syncthreads();
+    for (uint32_t k = threadIdx.x; k <= kMaxVerts; k += blockDim.x)
+      base[k] = cnt[k] ? atomicAdd(&cursors[k], cnt[k]) : 0u;
+    __syncthreads();
+    if (i < n) order[base[len] + rank] = i;
+    __syncthreads();
+  }
+}
+
 // The rest of BidirectionalPathTracer::sample, one slot per lane: Path::
 // append's cumulative values across the join (Path.cpp:397-512), getPd and
 // getContribution of every split (:48-197), the power heuristic per path
@@ -748,9 +803,10 @@ template <int VM>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_bd_combine(Scene sc, const BatchDesc *__restrict__ bd,
                                                     const float4 *__restrict__ verts, const uint4 *__restrict__ info,
                                                     uint32_t V, const uint32_t *__restrict__ masks,
-                                                    float4 *__restrict__ Ls) {
-  const uint32_t slot = blockIdx.x * blockDim.x + threadIdx.x;
-  if (slot >= bd->hdr.n_paths) return;
+                                                    const uint32_t *__restrict__ order, float4 *__restrict__ Ls) {
+  const uint32_t lane_slot = blockIdx.x * blockDim.x + threadIdx.x;
+  if (lane_slot >= bd->hdr.n_paths) return;
+  const uint32_t slot = order[lane_slot];
   const uint32_t E = V / 2;
   const uint4 in = info[slot];
   const uint32_t T = in.x, S = in.y, flags = in.z, n = S + T;

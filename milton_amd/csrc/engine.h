@@ -128,10 +128,11 @@ struct Workspace {
   DBuf<uint4> binfo;
   DBuf<uint2> bpairs;
   DBuf<uint32_t> bmasks;
+  DBuf<uint32_t> border;     // slot indices ordered by path length for the combine kernel
   size_t bytes() const {
     return q[0].bytes() + q[1].bytes() + hits.bytes() + sg.bytes() + sr.bytes() + Ls.bytes() + flags.bytes() +
            pjs.bytes() + pos.bytes() + counts.bytes() + totals.bytes() + desc.bytes() + bverts.bytes() +
-           binfo.bytes() + bpairs.bytes() + bmasks.bytes();
+           binfo.bytes() + bpairs.bytes() + bmasks.bytes() + border.bytes();
   }
 };
 

@@ -1362,7 +1362,7 @@ int enqueue_batch_bidir(mpt_context *ctx, Lane &ln, mpt_scene *s, uint32_t D, bo
     MPT_HIP(ctx, hipEventRecord(ctx->events[ctx->timed.back().second + 1], st));
     return MPT_OK;
   };
-  k_reset<<<1, 64, 0, st>>>(w.counts.p, 4 * D + 3);
+  k_reset<<<1, 64, 0, st>>>(w.counts.p, 4 * D + 3 + bdpt::kMaxVerts + 1);
   bdpt::k_bd_camera<<<gcap, 256, 0, st>>>(w.desc.p, w.q[0].p, w.pjs.p, filtered ? w.pos.p : nullptr, cnt_eye,
                                           w.bverts.p, w.binfo.p, V);
   MPT_LAUNCHED(ctx, "k_bd_camera");
@@ -1402,10 +1402,17 @@ int enqueue_batch_bidir(mpt_context *ctx, Lane &ln, mpt_scene *s, uint32_t D, bo
   }
   if ((rc = toc())) return rc;
   MPT_LAUNCHED(ctx, "k_bd_vis");
+  // slot indices by path length: combine's lanes then walk similar numbers
+  // of splits
+  uint32_t *len_bins = cur_light + D;
+  bdpt::k_bd_len_count<<<g, 256, 0, st>>>(w.desc.p, w.binfo.p, len_bins);
+  bdpt::k_bd_len_scan<<<1, 64, 0, st>>>(len_bins);
+  bdpt::k_bd_len_scatter<<<g, 256, 0, st>>>(w.desc.p, w.binfo.p, len_bins, w.border.p);
+  MPT_LAUNCHED(ctx, "path-length order");
   // per-lane arrays sized by the batch's path length: subpaths of 3, 5, 7
   // or 12 bounces
   auto combine = [&](auto kern) {
-    kern<<<gcap, 256, 0, st>>>(s->dev, w.desc.p, w.bverts.p, w.binfo.p, V, w.bmasks.p, w.Ls.p);
+    kern<<<gcap, 256, 0, st>>>(s->dev, w.desc.p, w.bverts.p, w.binfo.p, V, w.bmasks.p, w.border.p, w.Ls.p);
   };
   if (V <= 8) combine(bdpt::k_bd_combine<8>);
   else if (V <= 12) combine(bdpt::k_bd_combine<12>);
@@ -1608,13 +1615,15 @@ uint64_t workspace_bytes(uint64_t cap, uint32_t depth, uint64_t shadow) {
   // q[2], hits, sg, sr, Ls, flags, pjs, pos, counts, totals, desc
   return cap * (2 * 3 * sizeof(float4) + sizeof(float2) + 2 * sizeof(float4) + sizeof(float4) + sizeof(uint32_t) +
                 sizeof(uint2) + sizeof(float2)) +
-         shadow * 2 * sizeof(float4) + (4ull * depth + 3) * sizeof(uint32_t) + 8 * sizeof(unsigned long long) +
+         shadow * 2 * sizeof(float4) + (4ull * depth + 3 + bdpt::kMaxVerts + 1) * sizeof(uint32_t) +
+         8 * sizeof(unsigned long long) +
          sizeof(BatchDesc);
 }
 
 uint64_t bidir_slot_bytes(uint32_t depth) {
   const uint64_t V = 2ull * (depth + 1), P = V * (V - 1) / 2;
-  return V * bdpt::kVtx * sizeof(float4) + sizeof(uint4) + P * sizeof(uint2) + bdpt::kMaskWords * sizeof(uint32_t);
+  return V * bdpt::kVtx * sizeof(float4) + sizeof(uint4) + P * sizeof(uint2) + bdpt::kMaskWords * sizeof(uint32_t) +
+         sizeof(uint32_t);
 }
 
 uint32_t bdpt_max_depth() { return bdpt::kMaxDepth; }
@@ -1683,7 +1692,7 @@ int ensure_workspace(mpt_context *ctx, uint32_t k, uint32_t cap, uint32_t depth,
   MPT_HIP(ctx, w.pjs.reserve(cap));
   MPT_HIP(ctx, w.sg.reserve(2ull * cap));
   MPT_HIP(ctx, w.sr.reserve(2ull * std::max<uint64_t>(shadow, 1)));
-  MPT_HIP(ctx, w.counts.reserve(4ull * depth + 3));
+  MPT_HIP(ctx, w.counts.reserve(4ull * depth + 3 + bdpt::kMaxVerts + 1));
   MPT_HIP(ctx, w.desc.reserve(1));
   if (!w.totals.p) {
     // totals persist across reallocation: a render's counts are zeroed once
@@ -1706,12 +1715,14 @@ int ensure_bidir(mpt_context *ctx, uint32_t k, uint32_t cap, uint32_t depth) {
   Workspace &w = ln.ws;
   const uint64_t V = 2ull * (depth + 1), P = V * (V - 1) / 2;
   const uint64_t nv = (uint64_t)cap * V * bdpt::kVtx, np = (uint64_t)cap * P, nm = (uint64_t)cap * bdpt::kMaskWords;
-  if (w.bverts.n >= nv && w.binfo.n >= cap && w.bpairs.n >= np && w.bmasks.n >= nm) return MPT_OK;
+  if (w.bverts.n >= nv && w.binfo.n >= cap && w.bpairs.n >= np && w.bmasks.n >= nm && w.border.n >= cap)
+    return MPT_OK;
   MPT_HIP(ctx, hipStreamSynchronize(ln.stream));
   MPT_HIP(ctx, w.bverts.reserve(std::max<uint64_t>(nv, w.bverts.n)));
   MPT_HIP(ctx, w.binfo.reserve(std::max<uint64_t>(cap, w.binfo.n)));
   MPT_HIP(ctx, w.bpairs.reserve(std::max<uint64_t>(np, w.bpairs.n)));
   MPT_HIP(ctx, w.bmasks.reserve(std::max<uint64_t>(nm, w.bmasks.n)));
+  MPT_HIP(ctx, w.border.reserve(std::max<uint64_t>(cap, w.border.n)));
   ++w.generation;
   drop_graphs(ctx, 0);
   return MPT_OK;
