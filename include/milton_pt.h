@@ -44,7 +44,7 @@
 extern "C" {
 #endif
 
-#define MPT_ABI_VERSION 7u
+#define MPT_ABI_VERSION 8u
 #define MPT_MISS 0xFFFFFFFFu
 
 enum {
@@ -242,11 +242,29 @@ typedef struct mpt_job {
    * bounces of each subpath; direct_samples, direct_off and light_select do
    * not apply.  It needs a pinhole camera and at least one quad, mesh or
    * point light (emitting spheres: MPT_E_INVALID); the background is
-   * not an emitter of this renderer. */
+   * not an emitter of this renderer.
+   * MPT_RENDERER_MLT (2): MLTRenderer (renderers/mlt/MLTRenderer.cpp:
+   * 70-235, MLTMarkovProcess.cpp:38-176): mlt_chains Markov chains (the
+   * reference's noRenderThreads) of mlt_mutations mutations each, started
+   * from the splits of mlt_seed_paths generated paths weighted by their
+   * contribution; the film is RenderOutput's MLT histogram
+   * (renderers/RenderOutput.cpp:129-216).  width and height give the film;
+   * spp, max_depth, direct_samples, direct_off, light_select and filter do
+   * not apply (filter must be 0).  Needs a pinhole camera and a quad, mesh
+   * or point light (emitting spheres: MPT_E_INVALID). */
   uint32_t renderer;
+  /* Metropolis light transport (MPT_RENDERER_MLT only). */
+  uint32_t mlt_chains;         /* Markov chains, >= 1 */
+  uint32_t mlt_mutations;      /* mutations per chain, at most 32767 */
+  uint32_t mlt_seed_paths;     /* mltNoInitialPaths (reference default 10000; 0 means 1) */
+  uint32_t mlt_max_depth;      /* mltMaxDepth: most vertices of a path, 2..16 (reference default 10) */
+  uint32_t mlt_max_rejections; /* mltMaxConsequtiveRejections (reference default 500; 0: no limit) */
+  uint32_t mlt_flags;          /* bit 0: mltFilterProposed = false (the proposal-count median filter off) */
+  float mlt_probs[3];          /* mltBidirPathMutationProb, mltLensSubpathMutationProb,
+                                  mltPerturbationPathMutationProb (reference default 1, 1, 1) */
 } mpt_job;
 
-enum { MPT_RENDERER_PATH = 0, MPT_RENDERER_BIDIR = 1 };
+enum { MPT_RENDERER_PATH = 0, MPT_RENDERER_BIDIR = 1, MPT_RENDERER_MLT = 2 };
 
 typedef struct mpt_ray {
   float o[3];
@@ -274,6 +292,11 @@ typedef struct mpt_render_stats {
                               different lanes overlap, so this is below extend_ms */
   double shade_wall_ms;
   double shadow_wall_ms;
+  /* Metropolis light transport: mutations accepted, samples added to the
+   * film (RenderOutput's sample count) and b, the seed paths' mean weight */
+  uint64_t mlt_accepted;
+  uint64_t mlt_samples;
+  double mlt_b;
 } mpt_render_stats;
 
 /* Context options (mpt_set_option).  No reference counterpart: these are the
@@ -482,6 +505,17 @@ int mpt_image_read(const char *path, uint32_t *width, uint32_t *height, float *r
 /* Per-path radiance of pixels [pix_begin, pix_end) of a job, all samples,
  * ordered (pixel, sample): L_out holds (pix_end-pix_begin)*spp*3 floats.
  * Test hook: the value PathTracer::_evaluate returns for each camera ray. */
+/* Metropolis light transport's chains step by step, for checking: an
+ * MPT_RENDERER_MLT job's seeds and chain starts, then its first `steps`
+ * mutations (at most mlt_mutations) of every chain, recording per chain and
+ * mutation six doubles: the mutation (0 bidirectional, 1 lens subpath,
+ * 2 perturbation), its acceptance probability, accepted (0/1), the mutated
+ * path's vertices (0 when rejected as invalid), the tentative sample's film
+ * bin, the chain's path vertices after.  rec holds mlt_chains * steps * 6
+ * doubles; b and entries (optional) get the seeds' mean weight and count.
+ * No reference counterpart: the reference's chains run unobserved. */
+int mpt_mlt_trace(mpt_scene *scene, const mpt_job *job, uint32_t steps, double *rec, double *b,
+                  uint64_t *entries);
 int mpt_render_paths(mpt_scene *scene, const mpt_job *job, uint32_t pix_begin, uint32_t pix_end,
                      float *L_out);
 

@@ -19,7 +19,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 # compiled with different kernel tunables); the in-tree build by default.
 LIB_PATH = os.environ.get("MPT_LIBRARY") or os.path.join(_HERE, "libmilton_pt.so")
 
-MPT_ABI_VERSION = 7
+MPT_ABI_VERSION = 8
 MPT_TRI_ONE_SIDED = 1
 MPT_MISS = 0xFFFFFFFF
 MPT_BSDF_DIFFUSE, MPT_BSDF_PHONG, MPT_BSDF_DIELECTRIC = 0, 1, 2
@@ -30,7 +30,7 @@ MPT_OPT_FAIL_BATCH = 8
 MPT_OPT_TREE_TOP = 9
 MPT_OPT_TRACE_BLOCK, MPT_OPT_SHADE_BLOCK = 10, 11
 MPT_TONEMAP_REINHARD, MPT_TONEMAP_GAMMA = 0, 1
-MPT_RENDERER_PATH, MPT_RENDERER_BIDIR = 0, 1
+MPT_RENDERER_PATH, MPT_RENDERER_BIDIR, MPT_RENDERER_MLT = 0, 1, 2
 MPT_FILTER_BOX, MPT_FILTER_TRIANGLE, MPT_FILTER_GAUSSIAN, MPT_FILTER_MITCHELL, MPT_FILTER_LANCZOS = 0, 1, 2, 3, 4
 MPT_SAVE_RADIANCE = 1
 MPT_E_IO = -7
@@ -87,7 +87,11 @@ class Job(ctypes.Structure):
                 ("spp", ctypes.c_uint32), ("max_depth", ctypes.c_uint32),
                 ("seed", ctypes.c_uint32), ("job_id", ctypes.c_uint32), ("direct_samples", ctypes.c_uint32),
                 ("filter", ctypes.c_uint32), ("filter_support", ctypes.c_uint32), ("filter_params", ctypes.c_float * 2),
-                ("direct_off", ctypes.c_uint32), ("light_select", ctypes.c_uint32), ("renderer", ctypes.c_uint32)]
+                ("direct_off", ctypes.c_uint32), ("light_select", ctypes.c_uint32), ("renderer", ctypes.c_uint32),
+                ("mlt_chains", ctypes.c_uint32), ("mlt_mutations", ctypes.c_uint32),
+                ("mlt_seed_paths", ctypes.c_uint32), ("mlt_max_depth", ctypes.c_uint32),
+                ("mlt_max_rejections", ctypes.c_uint32), ("mlt_flags", ctypes.c_uint32),
+                ("mlt_probs", ctypes.c_float * 3)]
 
 
 EMITTER_POINT_DTYPE = np.dtype([("p", np.float32, 3), ("light", np.uint32), ("n", np.float32, 3),
@@ -105,7 +109,8 @@ class RenderStats(ctypes.Structure):
                 ("shade_ms", ctypes.c_double), ("shadow_ms", ctypes.c_double),
                 ("extend_launches", ctypes.c_uint64), ("nodes_visited", ctypes.c_uint64),
                 ("prims_tested", ctypes.c_uint64), ("extend_wall_ms", ctypes.c_double),
-                ("shade_wall_ms", ctypes.c_double), ("shadow_wall_ms", ctypes.c_double)]
+                ("shade_wall_ms", ctypes.c_double), ("shadow_wall_ms", ctypes.c_double),
+                ("mlt_accepted", ctypes.c_uint64), ("mlt_samples", ctypes.c_uint64), ("mlt_b", ctypes.c_double)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -193,6 +198,8 @@ def lib():
         "mpt_image_write_hdr": ([ctypes.c_char_p, u32, u32, vp], i32),
         "mpt_image_read": ([ctypes.c_char_p, ctypes.POINTER(u32), ctypes.POINTER(u32), vp, ctypes.c_size_t], i32),
         "mpt_render_paths": ([vp, ctypes.POINTER(Job), u32, u32, vp], i32),
+        "mpt_mlt_trace": ([vp, ctypes.POINTER(Job), u32, vp, ctypes.POINTER(ctypes.c_double),
+                           ctypes.POINTER(ctypes.c_uint64)], i32),
         "mpt_server_create": ([vp, ctypes.POINTER(ServerConfig), ctypes.POINTER(vp)], i32),
         "mpt_server_destroy": ([vp], i32),
         "mpt_submit": ([vp, vp, ctypes.POINTER(Job), ctypes.POINTER(ctypes.c_uint64)], i32),
@@ -402,9 +409,12 @@ def camera_dict(cam):
 
 
 def make_job(camera, width, height, spp, max_depth, seed=1, job_id=0, direct_samples=1, film_filter=None,
-             renderer=0):
+             renderer=0, mlt=None):
     """film_filter: None (pixel bin) or (MPT_FILTER_*, support, (p0, p1)); renderer:
-    MPT_RENDERER_PATH or MPT_RENDERER_BIDIR."""
+    MPT_RENDERER_PATH, MPT_RENDERER_BIDIR or MPT_RENDERER_MLT.  mlt: the
+    Metropolis parameters over the reference's defaults: chains (1),
+    mutations (spp * pixels / chains), seed_paths (10000), max_depth (10),
+    max_rejections (500), filter_proposed (True), probs ((1, 1, 1))."""
     j = Job()
     for k in range(3):
         j.camera.eye[k] = camera["eye"][k]
@@ -423,6 +433,16 @@ def make_job(camera, width, height, spp, max_depth, seed=1, job_id=0, direct_sam
     j.width, j.height, j.spp, j.max_depth, j.seed, j.job_id = width, height, spp, max_depth, seed, job_id
     j.direct_samples = direct_samples
     j.renderer = renderer
+    p = dict(mlt or {})
+    j.mlt_chains = p.get("chains", 1)
+    j.mlt_mutations = p.get("mutations", max(1, spp * width * height // max(1, j.mlt_chains)))
+    j.mlt_seed_paths = p.get("seed_paths", 10000)
+    j.mlt_max_depth = p.get("max_depth", 10)
+    j.mlt_max_rejections = p.get("max_rejections", 500)
+    j.mlt_flags = 0 if p.get("filter_proposed", True) else 1
+    probs = p.get("probs", (1.0, 1.0, 1.0))
+    for k in range(3):
+        j.mlt_probs[k] = probs[k]
     if film_filter is not None:
         ff = list(film_filter)
         kind = ff[0]
@@ -553,6 +573,18 @@ class Scene:
         st = RenderStats()
         self.ctx.check(self.ctx.L.mpt_render(self.h, ctypes.byref(job), _ptr(film), ctypes.byref(st)), "mpt_render")
         return film, st.as_dict()
+
+    def mlt_trace(self, job, steps):
+        """The first `steps` mutations of every chain of a Metropolis job
+        (mpt_mlt_trace): (chains, steps, 6) records (mutation, alpha,
+        accepted, mutated path length, tentative film bin, path length
+        after), b and the seed entries."""
+        rec = np.zeros((job.mlt_chains, steps, 6), np.float64)
+        b = ctypes.c_double(0)
+        n = ctypes.c_uint64(0)
+        self.ctx.check(self.ctx.L.mpt_mlt_trace(self.h, ctypes.byref(job), steps, _ptr(rec), ctypes.byref(b),
+                                                ctypes.byref(n)), "mpt_mlt_trace")
+        return rec, b.value, n.value
 
     def render_paths(self, job, pix_begin, pix_end):
         L = np.zeros(((pix_end - pix_begin) * job.spp, 3), np.float32)

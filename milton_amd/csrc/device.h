@@ -467,11 +467,11 @@ struct Walker {
 };
 
 // kd traversal of one ray to completion (ANY: first hit below tlimit).
-template <bool ANY, bool COUNT = false, int STACK = kStack>
+template <bool ANY, bool COUNT = false, int STACK = kStack, int BLOCK = kBlock>
 MPT_HD bool trace(const Scene &sc, const RaySetup &r, float tlimit, uint32_t skip_origin, uint2 *stk,
                   float &t_out, uint32_t &id_out, uint32_t *n_nodes = nullptr, uint32_t *n_refs = nullptr,
                   float2 *uv = nullptr, uint32_t skip_target = kMiss) {
-  Walker<ANY, STACK> w;
+  Walker<ANY, STACK, BLOCK> w;
   if (!w.start(sc, r, tlimit, skip_origin, skip_target)) {
     t_out = INFINITY;
     id_out = kMiss;

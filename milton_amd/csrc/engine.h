@@ -101,6 +101,23 @@ struct DBuf {
   size_t bytes() const { return n * sizeof(T); }
 };
 
+// Metropolis light transport's device buffers (the kernels' view is
+// mlt::MltBufs), kept across renders and grown as jobs need.
+struct MltWork {
+  DBuf<float4> seed_v, seed_tmp, seed_tmp2, chain_v;
+  DBuf<uint32_t> seed_len, seed_att, seed_cnt, ent_pos, ent_ks, order, proposed;
+  DBuf<double> ent_w, cdf, rec;
+  DBuf<unsigned char> desc, states;
+  DBuf<uint16_t> strat;
+  DBuf<unsigned long long> num, counters;
+  uint64_t bytes() const {
+    return seed_v.bytes() + seed_tmp.bytes() + seed_tmp2.bytes() + chain_v.bytes() + seed_len.bytes() +
+           seed_att.bytes() + seed_cnt.bytes() + ent_pos.bytes() + ent_ks.bytes() + order.bytes() +
+           proposed.bytes() + ent_w.bytes() + cdf.bytes() + rec.bytes() + desc.bytes() + states.bytes() +
+           strat.bytes() + num.bytes() + counters.bytes();
+  }
+};
+
 // Path and queue buffers, sized by the batch capacity.
 struct Workspace {
   uint32_t cap = 0, depth = 0;
@@ -208,6 +225,7 @@ struct mpt_context {
   std::vector<mpt_server *> servers;
   std::vector<mpt_engine::GraphEntry> graphs_cache;
   std::vector<std::pair<const void *, uint32_t>> grid_cache;   // resident blocks per kernel
+  mpt_engine::MltWork mlt;           // Metropolis light transport buffers
 };
 
 struct mpt_scene {
@@ -273,6 +291,12 @@ uint32_t bdpt_max_depth();
 // Scene-dependent conditions of a job: a bidirectional job needs a light
 // (quad, mesh or point) and no emitting sphere.
 int check_scene_job(mpt_context *ctx, const mpt_scene *s, const mpt_job *job);
+
+// Metropolis light transport of a whole job into film (W*H, finalised
+// values with weight 1), or, with rec, its first rec_steps mutations of
+// every chain recorded (mpt_mlt_trace); b_out / entries_out optional.
+int render_mlt(mpt_scene *s, const mpt_job *job, float4 *film, mpt_render_stats *stats, double *rec,
+               uint32_t rec_steps, double *b_out, uint64_t *entries_out);
 
 // Enqueues one batch on lane `lane` (asynchronous).  With L_host the
 // per-path radiance of the batch is copied to L_host (slot order) and the

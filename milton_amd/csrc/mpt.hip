@@ -1113,6 +1113,7 @@ namespace {
 bool emissive_host(const mpt_material &m) { return m.Le[0] > 0 || m.Le[1] > 0 || m.Le[2] > 0; }
 
 #include "bdpt.h"
+#include "mlt.h"
 
 int hip_fail(mpt_context *ctx, hipError_t e, const char *what) {
   return fail(ctx, MPT_E_DEVICE, std::string(what) + ": " + hipGetErrorString(e));
@@ -1541,15 +1542,36 @@ JobDev make_job(const mpt_job &job, float4 *film) {
 
 int check_job(mpt_context *ctx, const mpt_job *job) {
   if (!job) return fail(ctx, MPT_E_INVALID, "job is NULL");
-  if (job->width == 0 || job->height == 0 || job->spp == 0 || job->max_depth == 0)
+  if (job->renderer == MPT_RENDERER_MLT) {
+    if (job->width == 0 || job->height == 0) return fail(ctx, MPT_E_INVALID, "job needs width and height > 0");
+  } else if (job->width == 0 || job->height == 0 || job->spp == 0 || job->max_depth == 0) {
     return fail(ctx, MPT_E_INVALID, "job needs width, height, spp and max_depth > 0");
+  }
   if (job->max_depth > 64) return fail(ctx, MPT_E_INVALID, "max_depth above 64");
   if ((uint64_t)job->width * job->height > (1ull << 30)) return fail(ctx, MPT_E_INVALID, "film too large");
   if (job->spp >= (1u << kJobBits)) return fail(ctx, MPT_E_INVALID, "spp at or above 2^20");
   if (job->direct_samples > kMaxDirect) return fail(ctx, MPT_E_INVALID, "direct_samples above 64");
   if (job->direct_off > 1) return fail(ctx, MPT_E_INVALID, "direct_off must be 0 or 1");
   if (job->light_select > 1) return fail(ctx, MPT_E_INVALID, "light_select must be 0 or 1");
-  if (job->renderer > MPT_RENDERER_BIDIR) return fail(ctx, MPT_E_INVALID, "unknown renderer");
+  if (job->renderer > MPT_RENDERER_MLT) return fail(ctx, MPT_E_INVALID, "unknown renderer");
+  if (job->renderer == MPT_RENDERER_MLT) {
+    if (job->camera.aperture > 0)
+      return fail(ctx, MPT_E_INVALID, "Metropolis light transport needs a pinhole camera (aperture 0)");
+    if (job->mlt_chains == 0) return fail(ctx, MPT_E_INVALID, "mlt_chains must be at least 1");
+    if (job->mlt_mutations > 32767)
+      return fail(ctx, MPT_E_INVALID, "mlt_mutations above 32767 (the lens stratification's 16-bit counts)");
+    if (job->mlt_max_depth < 2 || job->mlt_max_depth > mlt::kMaxLen)
+      return fail(ctx, MPT_E_INVALID, "mlt_max_depth must be 2..16 path vertices");
+    if ((uint64_t)job->width * job->height < 2) return fail(ctx, MPT_E_INVALID, "the film needs two or more pixels");
+    if (job->filter) return fail(ctx, MPT_E_INVALID, "Metropolis light transport bins samples (filter 0)");
+    if (job->mlt_seed_paths > (1u << 22)) return fail(ctx, MPT_E_INVALID, "mlt_seed_paths above 2^22");
+    for (int k = 0; k < 3; ++k)
+      if (!std::isfinite(job->mlt_probs[k])) return fail(ctx, MPT_E_INVALID, "mlt_probs must be finite");
+    const uint64_t strat = (uint64_t)job->mlt_chains * job->width * job->height * sizeof(uint16_t);
+    if (strat > (64ull << 30))
+      return fail(ctx, MPT_E_INVALID,
+                  "mlt_chains x pixels x 2 B (the lens mutation's per-chain film counts) above 64 GiB");
+  }
   if (job->renderer == MPT_RENDERER_BIDIR) {
     if (job->max_depth > bdpt::kMaxDepth)
       return fail(ctx, MPT_E_INVALID, "the bidirectional renderer takes max_depth up to 12");
@@ -1638,14 +1660,14 @@ uint32_t bidir_capacity(const mpt_context *ctx, uint32_t depth, uint64_t lane_by
 }
 
 int check_scene_job(mpt_context *ctx, const mpt_scene *s, const mpt_job *job) {
-  if (job->renderer != MPT_RENDERER_BIDIR) return MPT_OK;
+  if (job->renderer == MPT_RENDERER_PATH) return MPT_OK;
+  const char *who = job->renderer == MPT_RENDERER_BIDIR ? "the bidirectional renderer" : "Metropolis light transport";
   // EmitterSampler::sample asserts a light (stats/samplers/EmitterSampler.cpp:51-53)
   if (s->dev.num_lights + s->dev.num_points == 0)
-    return fail(ctx, MPT_E_INVALID, "the bidirectional renderer needs a light (an emitting quad, mesh or point)");
+    return fail(ctx, MPT_E_INVALID, std::string(who) + " needs a light (an emitting quad, mesh or point)");
   // Sphere::getRandomPoint's uv round trip yields NaN points for half its
   // draws (shapes/Sphere.cpp:79-124)
-  if (s->sphere_lights)
-    return fail(ctx, MPT_E_INVALID, "the bidirectional renderer does not sample emitting spheres");
+  if (s->sphere_lights) return fail(ctx, MPT_E_INVALID, std::string(who) + " does not sample emitting spheres");
   return MPT_OK;
 }
 
@@ -1888,11 +1910,150 @@ int run_batch(mpt_context *ctx, mpt_scene *s, const BatchPlan &plan, float *L_ho
 
 // Renders pixels [pix_begin, pix_end) of one job as consecutive batches of
 // whole pixels.
+int render_mlt(mpt_scene *s, const mpt_job *job, float4 *film, mpt_render_stats *stats, double *rec,
+               uint32_t rec_steps, double *b_out, uint64_t *entries_out) {
+  mpt_context *ctx = s->ctx;
+  int rc = check_job(ctx, job);
+  if (rc) return rc;
+  if ((rc = check_scene_job(ctx, s, job))) return rc;
+  using mlt::MltDesc;
+  MltWork &w = ctx->mlt;
+  const uint32_t W = job->width, H = job->height, P = W * H, C = job->mlt_chains;
+  const uint32_t N = std::max(1u, job->mlt_seed_paths);
+  hipStream_t st = ctx->stream;
+  MltDesc d{};
+  d.cam = make_job(*job, nullptr).cam;
+  d.W = W;
+  d.H = H;
+  d.seed = job->seed;
+  d.job_id = job->job_id;
+  d.chains = C;
+  d.mutations = job->mlt_mutations;
+  d.seed_paths = N;
+  d.max_len = job->mlt_max_depth;
+  d.max_rejections = job->mlt_max_rejections;
+  d.filter_proposed = (job->mlt_flags & 1u) ? 0u : 1u;
+  {
+    // MLTAggregatePathMutation::init (MLTAggregatePathMutation.cpp:34-91):
+    // |weights|, all three when their sum is at most EPSILON, normalised
+    double a[3] = {std::fabs((double)job->mlt_probs[0]), std::fabs((double)job->mlt_probs[1]),
+                   std::fabs((double)job->mlt_probs[2])};
+    if (a[0] + a[1] + a[2] <= 1e-7) a[0] = a[1] = a[2] = 1;
+    d.n_mut = 0;
+    for (uint32_t i = 0; i < 3; ++i)
+      if (a[i] > 0) {
+        d.mut_id[d.n_mut] = i;
+        d.mut_w[d.n_mut++] = a[i];
+      }
+    double sum = 0;
+    for (uint32_t i = d.n_mut; i--;) sum += d.mut_w[i];
+    for (uint32_t i = d.n_mut; i--;) d.mut_w[i] /= sum;
+  }
+  const uint64_t rec_len = (uint64_t)mlt::kCap * mlt::kSlots;
+  MPT_HIP(ctx, w.seed_v.reserve(rec_len * N));
+  MPT_HIP(ctx, w.seed_tmp.reserve(rec_len * N));
+  MPT_HIP(ctx, w.seed_tmp2.reserve(rec_len * N));
+  MPT_HIP(ctx, w.seed_len.reserve(N));
+  MPT_HIP(ctx, w.seed_att.reserve(N));
+  MPT_HIP(ctx, w.seed_cnt.reserve(N));
+  MPT_HIP(ctx, w.ent_w.reserve((uint64_t)N * mlt::kMaxEntries));
+  MPT_HIP(ctx, w.ent_pos.reserve((uint64_t)N * mlt::kMaxEntries));
+  MPT_HIP(ctx, w.ent_ks.reserve((uint64_t)N * mlt::kMaxEntries));
+  MPT_HIP(ctx, w.cdf.reserve((uint64_t)N * mlt::kMaxEntries));
+  MPT_HIP(ctx, w.order.reserve((uint64_t)N * mlt::kMaxEntries));
+  MPT_HIP(ctx, w.chain_v.reserve(3 * rec_len * C));
+  MPT_HIP(ctx, w.states.reserve(sizeof(mlt::ChainState) * (uint64_t)C));
+  MPT_HIP(ctx, w.strat.reserve((uint64_t)C * P));
+  MPT_HIP(ctx, w.num.reserve(3ull * P));
+  MPT_HIP(ctx, w.proposed.reserve(P));
+  MPT_HIP(ctx, w.counters.reserve(4));
+  MPT_HIP(ctx, w.desc.reserve(sizeof(MltDesc)));
+  const uint32_t steps = rec ? std::min(rec_steps, job->mlt_mutations) : job->mlt_mutations;
+  if (rec) MPT_HIP(ctx, w.rec.reserve(std::max<uint64_t>(1, 6ull * C * steps)));
+  mlt::MltBufs B;
+  B.seed_v = w.seed_v.p;
+  B.seed_tmp = w.seed_tmp.p;
+  B.seed_tmp2 = w.seed_tmp2.p;
+  B.seed_len = w.seed_len.p;
+  B.seed_att = w.seed_att.p;
+  B.seed_cnt = w.seed_cnt.p;
+  B.ent_w = w.ent_w.p;
+  B.ent_pos = w.ent_pos.p;
+  B.ent_ks = w.ent_ks.p;
+  B.cdf = w.cdf.p;
+  B.order = w.order.p;
+  B.chain_v = w.chain_v.p;
+  B.st = reinterpret_cast<mlt::ChainState *>(w.states.p);
+  B.strat = w.strat.p;
+  B.num = w.num.p;
+  B.proposed = w.proposed.p;
+  B.counters = w.counters.p;
+  MltDesc *dd = reinterpret_cast<MltDesc *>(w.desc.p);
+  MPT_HIP(ctx, hipEventRecord(ctx->ev0, st));
+  MPT_HIP(ctx, hipMemsetAsync(w.counters.p, 0, 4 * sizeof(unsigned long long), st));
+  MPT_HIP(ctx, hipMemsetAsync(w.num.p, 0, 3ull * P * sizeof(unsigned long long), st));
+  MPT_HIP(ctx, hipMemsetAsync(w.proposed.p, 0, (uint64_t)P * sizeof(uint32_t), st));
+  MPT_HIP(ctx, hipMemsetAsync(w.strat.p, 0, (uint64_t)C * P * sizeof(uint16_t), st));
+  // seed paths, round after round while no seed carries light
+  // (MLTRenderer.cpp:144-205)
+  bool found = false;
+  for (uint32_t round = 0; round < mlt::kRounds && !found; ++round) {
+    d.round = round;
+    MPT_HIP(ctx, hipMemcpyAsync(dd, &d, sizeof(d), hipMemcpyHostToDevice, st));
+    mlt::k_mlt_seed<<<(N + mlt::kBlock - 1) / mlt::kBlock, mlt::kBlock, 0, st>>>(s->dev, dd, B);
+    MPT_LAUNCHED(ctx, "k_mlt_seed");
+    mlt::k_mlt_weights<<<1, 64, 0, st>>>(dd, B);
+    MPT_LAUNCHED(ctx, "k_mlt_weights");
+    MPT_HIP(ctx, hipMemcpyAsync(&d, dd, sizeof(d), hipMemcpyDeviceToHost, st));
+    MPT_HIP(ctx, hipStreamSynchronize(st));
+    found = d.has_seeds != 0;
+    if (!found) d.n_prev += d.n_entries;
+  }
+  if (!found) return fail(ctx, MPT_E_INVALID, "no seed path of the Metropolis job carries light");
+  const uint32_t gc = (C + mlt::kBlock - 1) / mlt::kBlock;
+  mlt::k_mlt_start<<<gc, mlt::kBlock, 0, st>>>(s->dev, dd, B);
+  MPT_LAUNCHED(ctx, "k_mlt_start");
+  // runs of mutations: about 2^22 chain mutations per launch
+  const uint32_t run = std::max<uint32_t>(1, std::min<uint64_t>(steps, (4ull << 20) / C));
+  for (uint32_t j0 = 0; j0 < steps; j0 += run) {
+    const uint32_t j1 = std::min(steps, j0 + run);
+    mlt::k_mlt_step<<<gc, mlt::kBlock, 0, st>>>(s->dev, dd, B, j0, j1, rec ? w.rec.p : nullptr, steps);
+    MPT_LAUNCHED(ctx, "k_mlt_step");
+  }
+  if (film) {
+    mlt::k_mlt_finish<<<(P + 255) / 256, 256, 0, st>>>(dd, B, film);
+    MPT_LAUNCHED(ctx, "k_mlt_finish");
+  }
+  MPT_HIP(ctx, hipEventRecord(ctx->ev1, st));
+  unsigned long long ctr[4];
+  MPT_HIP(ctx, hipMemcpyAsync(ctr, w.counters.p, sizeof(ctr), hipMemcpyDeviceToHost, st));
+  if (rec) MPT_HIP(ctx, hipMemcpyAsync(rec, w.rec.p, 6ull * C * steps * sizeof(double), hipMemcpyDeviceToHost, st));
+  MPT_HIP(ctx, hipStreamSynchronize(st));
+  if (b_out) *b_out = d.b;
+  if (entries_out) *entries_out = (uint64_t)d.n_entries + d.n_prev;
+  if (stats) {
+    float ms = 0;
+    MPT_HIP(ctx, hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+    std::memset(stats, 0, sizeof(*stats));
+    stats->paths = (uint64_t)C * steps;
+    stats->extension_rays = ctr[2];
+    stats->shadow_rays = ctr[3];
+    stats->render_ms = ms;
+    stats->mlt_accepted = ctr[1];
+    stats->mlt_samples = ctr[0];
+    stats->mlt_b = d.b;
+  }
+  return MPT_OK;
+}
+
 int render_range(mpt_scene *s, const mpt_job *job, uint32_t pix_begin, uint32_t pix_end, float4 *film,
                  float *L_host, mpt_render_stats *stats) {
   mpt_context *ctx = s->ctx;
   int rc = check_job(ctx, job);
   if (rc) return rc;
+  if (job->renderer == MPT_RENDERER_MLT)
+    return fail(ctx, MPT_E_INVALID,
+                "Metropolis light transport renders whole films (mpt_render, mpt_render_film), not pixel ranges");
   mpt_job focused = *job;
   if ((rc = resolve_focus(s, focused))) return rc;
   job = &focused;
@@ -2760,9 +2921,19 @@ int mpt_render_film(mpt_scene *s, const mpt_job *job, mpt_film *film, mpt_render
   if (!s || !film) return fail(s ? s->ctx : nullptr, MPT_E_INVALID, "NULL argument");
   if (!job || job->width != film->width || job->height != film->height)
     return fail(s->ctx, MPT_E_INVALID, "film size does not match the job");
+  if (job->renderer == MPT_RENDERER_MLT)
+    return render_mlt(s, job, film->data.p, stats, nullptr, 0, nullptr, nullptr);
   // a filtered render adds into the film
   if (job->filter) MPT_HIP(s->ctx, hipMemsetAsync(film->data.p, 0, film->data.bytes(), s->ctx->stream));
   return render_range(s, job, 0, job->width * job->height, film->data.p, nullptr, stats);
+}
+
+int mpt_mlt_trace(mpt_scene *s, const mpt_job *job, uint32_t steps, double *rec, double *b, uint64_t *entries) {
+  if (!s || !job || !rec) return fail(s ? s->ctx : nullptr, MPT_E_INVALID, "NULL argument");
+  if (job->renderer != MPT_RENDERER_MLT) return fail(s->ctx, MPT_E_INVALID, "not a Metropolis job");
+  if (steps == 0 || steps > job->mlt_mutations)
+    return fail(s->ctx, MPT_E_INVALID, "steps must be 1..mlt_mutations");
+  return render_mlt(s, job, nullptr, nullptr, rec, steps, b, entries);
 }
 
 int mpt_render_film_range(mpt_scene *s, const mpt_job *job, uint32_t pix_begin, uint32_t pix_end, mpt_film *film,

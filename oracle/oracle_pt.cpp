@@ -1834,6 +1834,8 @@ void parallel_for(uint64_t n, int nthreads, uint64_t chunk, F &&f) {
   for (auto &x : th) x.join();
 }
 
+#include "oracle_mlt.inc"
+
 }  // namespace
 
 extern "C" {
@@ -1844,7 +1846,7 @@ void orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t ou
 
 // Bumped with every change of an exported signature; pyoracle refuses a
 // library built from older sources.
-int orc_interface_version(void) { return 12; }
+int orc_interface_version(void) { return 13; }
 
 orc_scene *orc_scene_create(uint32_t nv, const float *vertices, uint32_t nt, const uint32_t *indices,
                             const uint32_t *tri_material, uint32_t ns, const float *spheres,
@@ -2367,3 +2369,152 @@ extern "C" void orc_background(const orc_scene *s, const double *d, double *rgb)
   rgb[1] = L.y;
   rgb[2] = L.z;
 }
+
+// ---------------------------------------------------------------------------
+// Metropolis light transport entry points.
+// ---------------------------------------------------------------------------
+namespace {
+
+// The scenes and cameras the MLT restatement covers: a pinhole and quad,
+// mesh and point lights (as the bidirectional renderer).
+bool mlt_supported(const orc_scene &sc, const Camera &cam, const orc_job &job, const orc_mlt &p) {
+  if (cam.aperture > 0) return false;
+  for (const orc_scene::Light &l : sc.lights)
+    if (l.kind == 1) return false;
+  if (sc.lights.empty() && sc.points.empty()) return false;
+  if (job.width * (uint64_t)job.height < 2) return false;
+  return p.max_depth >= 2 && p.chains > 0;
+}
+
+MltParams mlt_params(const orc_mlt &p) {
+  MltParams r;
+  r.chains = p.chains;
+  r.mutations = p.mutations;
+  r.seed_paths = p.seed_paths;
+  r.max_depth = p.max_depth;
+  r.max_rejections = p.max_rejections;
+  r.filter_proposed = p.filter_proposed;
+  for (int i = 0; i < 3; ++i) r.probs[i] = p.probs[i];
+  return r;
+}
+
+}  // namespace
+
+extern "C" {
+
+int orc_mlt_seeds(const orc_scene *s, const orc_job *job, const orc_mlt *p, double *b_out, uint64_t *n_out,
+                  double *w_out, uint32_t *len_out, uint64_t cap, int nthreads) {
+  const Camera cam(job->camera, s);
+  if (!mlt_supported(*s, cam, *job, *p)) return -1;
+  PathCounters ctr;
+  MltCtx cx{*s, cam, job->width, job->height, 1.0 / job->width, 1.0 / job->height, ctr};
+  SeedSet S;
+  if (!init_seeds(cx, *job, mlt_params(*p), S, nthreads)) return -2;
+  *b_out = S.b;
+  *n_out = S.w.size();
+  for (uint64_t i = 0; i < std::min<uint64_t>(cap, S.w.size()); ++i) {
+    if (w_out) w_out[i] = S.w[i];
+    if (len_out) len_out[i] = S.paths[i].len();
+  }
+  return 0;
+}
+
+int orc_mlt_render(const orc_scene *s, const orc_job *job, const orc_mlt *p, double *film_out, double *num_out,
+                   uint64_t *proposed_out, uint64_t *stats, double *b_out, int nthreads) {
+  const Camera cam(job->camera, s);
+  if (!mlt_supported(*s, cam, *job, *p)) return -1;
+  const MltParams mp = mlt_params(*p);
+  PathCounters ctr;
+  MltCtx cx{*s, cam, job->width, job->height, 1.0 / job->width, 1.0 / job->height, ctr};
+  SeedSet S;
+  if (!init_seeds(cx, *job, mp, S, nthreads)) return -2;
+  const Mutations mu(mp);
+  const uint32_t P = job->width * job->height;
+  const int nt = std::max(1, nthreads);
+  std::vector<MltFilm> films(nt);
+  std::vector<PathCounters> ctrs(nt);
+  std::vector<uint64_t> acc(nt, 0);
+  for (MltFilm &f : films) {
+    f.num.assign(3ull * P, 0.0);
+    f.proposed.assign(P, 0);
+  }
+  parallel_for(mp.chains, nthreads, 1, [&](uint64_t b, uint64_t e, int tid) {
+    MltCtx tc{*s, cam, job->width, job->height, 1.0 / job->width, 1.0 / job->height, ctrs[tid]};
+    for (uint64_t c = b; c < e; ++c) {
+      Chain ch;
+      chain_start(tc, S, (uint32_t)c, *job, ch);
+      for (uint32_t j = 0; j < mp.mutations; ++j) {
+        StepRec r;
+        chain_step(tc, mp, mu, S.b, ch, (uint32_t)c, j, *job, films[tid], &r);
+        acc[tid] += r.accepted;
+      }
+    }
+  });
+  MltFilm f;
+  f.num.assign(3ull * P, 0.0);
+  f.proposed.assign(P, 0);
+  uint64_t accepted = 0;
+  for (int t = 0; t < nt; ++t) {
+    for (uint64_t i = 0; i < 3ull * P; ++i) f.num[i] += films[t].num[i];
+    for (uint32_t i = 0; i < P; ++i) f.proposed[i] += films[t].proposed[i];
+    f.samples += films[t].samples;
+    ctr.extension += ctrs[t].extension;
+    ctr.shadow += ctrs[t].shadow;
+    accepted += acc[t];
+  }
+  finalize_film(f, job->width, job->height, mp.filter_proposed != 0, film_out);
+  if (num_out) std::copy(f.num.begin(), f.num.end(), num_out);
+  if (proposed_out) std::copy(f.proposed.begin(), f.proposed.end(), proposed_out);
+  if (stats) {
+    stats[0] = f.samples;
+    stats[1] = accepted;
+    stats[2] = (uint64_t)mp.chains * mp.mutations;
+    stats[3] = ctr.extension;
+    stats[4] = ctr.shadow;
+    stats[5] = S.w.size();
+  }
+  if (b_out) *b_out = S.b;
+  return 0;
+}
+
+int orc_mlt_trace(const orc_scene *s, const orc_job *job, const orc_mlt *p, uint32_t c0, uint32_t c1, uint32_t steps,
+                  double *rec, int nthreads) {
+  const Camera cam(job->camera, s);
+  if (!mlt_supported(*s, cam, *job, *p) || c1 < c0) return -1;
+  const MltParams mp = mlt_params(*p);
+  PathCounters ctr;
+  MltCtx cx{*s, cam, job->width, job->height, 1.0 / job->width, 1.0 / job->height, ctr};
+  SeedSet S;
+  if (!init_seeds(cx, *job, mp, S, nthreads)) return -2;
+  const Mutations mu(mp);
+  const uint32_t P = job->width * job->height;
+  const int nt = std::max(1, nthreads);
+  std::vector<MltFilm> films(nt);
+  std::vector<PathCounters> ctrs(nt);
+  for (MltFilm &f : films) {
+    f.num.assign(3ull * P, 0.0);
+    f.proposed.assign(P, 0);
+  }
+  parallel_for(c1 - c0, nthreads, 1, [&](uint64_t b, uint64_t e, int tid) {
+    MltCtx tc{*s, cam, job->width, job->height, 1.0 / job->width, 1.0 / job->height, ctrs[tid]};
+    for (uint64_t i = b; i < e; ++i) {
+      const uint32_t c = c0 + (uint32_t)i;
+      Chain ch;
+      chain_start(tc, S, c, *job, ch);
+      for (uint32_t j = 0; j < steps; ++j) {
+        StepRec r;
+        chain_step(tc, mp, mu, S.b, ch, c, j, *job, films[tid], &r);
+        double *o = rec + 6 * ((uint64_t)i * steps + j);
+        o[0] = r.mutation;
+        o[1] = r.alpha;
+        o[2] = r.accepted;
+        o[3] = r.y_len;
+        o[4] = r.tent_bin;
+        o[5] = r.x_len;
+      }
+    }
+  });
+  return 0;
+}
+
+}  // extern "C"

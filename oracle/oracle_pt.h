@@ -44,7 +44,7 @@ typedef struct orc_scene orc_scene;
  * mesh triangles) with the reference build parameters.  kd_min_prims /
  * kd_max_depth of 0 select the reference defaults (3, 24).  use_naive != 0
  * replaces both kd-trees by NaiveSpatialAccel.  Returns NULL on bad input. */
-int orc_interface_version(void);   /* 12 */
+int orc_interface_version(void);   /* 13 */
 orc_scene *orc_scene_create(uint32_t nv, const float *vertices,
                             uint32_t nt, const uint32_t *indices, const uint32_t *tri_material,
                             uint32_t ns, const float *spheres, const uint32_t *sphere_material,
@@ -160,6 +160,34 @@ void orc_background(const orc_scene *s, const double *d, double *rgb);
 int orc_emitter_sample(const orc_scene *s, uint64_t n, uint32_t seed, double *out);
 
 void orc_tonemap(const float *rgbw, uint64_t n, int mode, uint8_t *rgba);
+/* Metropolis light transport (renderers/mlt/): `chains` Markov chains of
+ * `mutations` mutations each, seeded from `seed_paths` generated paths of at
+ * most `max_depth` vertices (mltNoInitialPaths, mltMaxDepth); max_rejections
+ * = mltMaxConsequtiveRejections (0 = no limit); filter_proposed =
+ * mltFilterProposed; probs = mltBidirPathMutationProb,
+ * mltLensSubpathMutationProb, mltPerturbationPathMutationProb.  The job's
+ * seed and job_id key the random streams; width x height is the film. */
+typedef struct orc_mlt {
+  uint32_t chains, mutations, seed_paths, max_depth, max_rejections, filter_proposed;
+  double probs[3];
+} orc_mlt;
+/* The seed set: b (sum / entries), the number of entries, and up to cap
+ * normalised weights and path lengths.  0, -1 for an unsupported scene or
+ * camera, -2 when no seed path carries light. */
+int orc_mlt_seeds(const orc_scene *s, const orc_job *job, const orc_mlt *p, double *b_out, uint64_t *n_out,
+                  double *w_out, uint32_t *len_out, uint64_t cap, int nthreads);
+/* The whole job: film_out double[3*W*H] finalised (RenderOutput's MLT
+ * branch); optional num_out double[3*W*H] (numerators), proposed_out
+ * uint64[W*H]; stats uint64[6] = samples added, accepted mutations,
+ * mutations, extension rays, visibility rays, seed entries. */
+int orc_mlt_render(const orc_scene *s, const orc_job *job, const orc_mlt *p, double *film_out, double *num_out,
+                   uint64_t *proposed_out, uint64_t *stats, double *b_out, int nthreads);
+/* Chains [c0, c1), their first `steps` mutations: rec double[6 per step] =
+ * mutation (0 bidir, 1 lens subpath, 2 perturbation), acceptance
+ * probability, accepted, the mutated path's length (0 when rejected as
+ * invalid), the tentative sample's film bin, the chain's path length after. */
+int orc_mlt_trace(const orc_scene *s, const orc_job *job, const orc_mlt *p, uint32_t c0, uint32_t c1,
+                  uint32_t steps, double *rec, int nthreads);
 
 #ifdef __cplusplus
 }

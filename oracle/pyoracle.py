@@ -15,7 +15,7 @@ LIB_PATH = os.path.join(HERE, "liboracle_pt.so")
 MISS = 0xFFFFFFFF
 
 
-INTERFACE_VERSION = 12
+INTERFACE_VERSION = 13
 
 
 def build():
@@ -37,6 +37,12 @@ class OrcJob(ctypes.Structure):
                 ("filter_support", ctypes.c_uint32), ("filter_params", ctypes.c_double * 2),
                 ("direct_off", ctypes.c_uint32), ("light_select", ctypes.c_uint32),
                 ("renderer", ctypes.c_uint32)]
+
+
+class OrcMlt(ctypes.Structure):
+    _fields_ = [("chains", ctypes.c_uint32), ("mutations", ctypes.c_uint32), ("seed_paths", ctypes.c_uint32),
+                ("max_depth", ctypes.c_uint32), ("max_rejections", ctypes.c_uint32),
+                ("filter_proposed", ctypes.c_uint32), ("probs", ctypes.c_double * 3)]
 
 
 _lib = None
@@ -75,6 +81,12 @@ def lib():
         L.orc_render_film.argtypes = [vp, ctypes.POINTER(OrcJob), vp, i32]
         L.orc_emitter_sample.argtypes = [vp, u64, u32, vp]
         L.orc_emitter_sample.restype = i32
+        L.orc_mlt_seeds.argtypes = [vp, ctypes.POINTER(OrcJob), ctypes.POINTER(OrcMlt), vp, vp, vp, vp, u64, i32]
+        L.orc_mlt_seeds.restype = i32
+        L.orc_mlt_render.argtypes = [vp, ctypes.POINTER(OrcJob), ctypes.POINTER(OrcMlt), vp, vp, vp, vp, vp, i32]
+        L.orc_mlt_render.restype = i32
+        L.orc_mlt_trace.argtypes = [vp, ctypes.POINTER(OrcJob), ctypes.POINTER(OrcMlt), u32, u32, u32, vp, i32]
+        L.orc_mlt_trace.restype = i32
         _lib = L
     return _lib
 
@@ -205,6 +217,52 @@ class OracleScene:
         return {"p": out[:, 0:3], "n": out[:, 3:6], "light": out[:, 6].astype(np.int64), "pdf_area": out[:, 7],
                 "pdf_select": out[:, 8]}
 
+    def _mlt(self, job, call):
+        rc = call(make_job(job), make_mlt(job))
+        if rc == -1:
+            raise ValueError("Metropolis light transport takes a pinhole camera, quad, mesh or point lights, "
+                             "a film of two or more pixels, chains and a max_depth of at least 2")
+        if rc == -2:
+            raise ValueError("no seed path carries light")
+
+    def mlt_seeds(self, job, threads=None):
+        """MLTRenderer::_initSeedPaths: dict of b, the normalised weights and
+        the split paths' lengths."""
+        b = ctypes.c_double(0)
+        n = ctypes.c_uint64(0)
+        self._mlt(job, lambda j, p: lib().orc_mlt_seeds(self.h, ctypes.byref(j), ctypes.byref(p), ctypes.byref(b),
+                                                        ctypes.byref(n), None, None, 0, threads or self.threads))
+        w = np.zeros(n.value, np.float64)
+        ln = np.zeros(n.value, np.uint32)
+        self._mlt(job, lambda j, p: lib().orc_mlt_seeds(self.h, ctypes.byref(j), ctypes.byref(p), ctypes.byref(b),
+                                                        ctypes.byref(n), _p(w), _p(ln), len(w),
+                                                        threads or self.threads))
+        return {"b": b.value, "w": w, "len": ln}
+
+    def mlt_render(self, job, threads=None):
+        """The job's finalised film (H, W, 3), numerators (H, W, 3), proposal
+        counts (H, W) and stats."""
+        W, H = job.width, job.height
+        film = np.zeros((H, W, 3), np.float64)
+        num = np.zeros((H, W, 3), np.float64)
+        prop = np.zeros((H, W), np.uint64)
+        st = np.zeros(6, np.uint64)
+        b = ctypes.c_double(0)
+        self._mlt(job, lambda j, p: lib().orc_mlt_render(self.h, ctypes.byref(j), ctypes.byref(p), _p(film), _p(num),
+                                                         _p(prop), _p(st), ctypes.byref(b), threads or self.threads))
+        stats = dict(zip(["samples", "accepted", "mutations", "extension_rays", "visibility_rays", "seed_entries"],
+                         st.tolist()))
+        stats["b"] = b.value
+        return film, num, prop, stats
+
+    def mlt_trace(self, job, c0, c1, steps, threads=None):
+        """Per chain and mutation: mutation, alpha, accepted, Y length,
+        tentative bin, chain length after (c1 - c0, steps, 6)."""
+        rec = np.zeros((c1 - c0, steps, 6), np.float64)
+        self._mlt(job, lambda j, p: lib().orc_mlt_trace(self.h, ctypes.byref(j), ctypes.byref(p), c0, c1, steps,
+                                                        _p(rec), threads or self.threads))
+        return rec
+
     def close(self):
         if self.h:
             lib().orc_scene_destroy(self.h)
@@ -237,6 +295,21 @@ def make_job(job, pix_begin=0, pix_end=None):
     j.pix_begin = pix_begin
     j.pix_end = job.width * job.height if pix_end is None else pix_end
     return j
+
+
+def make_mlt(job):
+    """OrcMlt from a job's Metropolis fields (milton_amd.Job names)."""
+    p = OrcMlt()
+    p.chains = getattr(job, "mlt_chains", 0)
+    p.mutations = getattr(job, "mlt_mutations", 0)
+    p.seed_paths = getattr(job, "mlt_seed_paths", 10000)
+    p.max_depth = getattr(job, "mlt_max_depth", 10)
+    p.max_rejections = getattr(job, "mlt_max_rejections", 500)
+    p.filter_proposed = 0 if getattr(job, "mlt_flags", 0) & 1 else 1
+    probs = getattr(job, "mlt_probs", (1.0, 1.0, 1.0))
+    for i in range(3):
+        p.probs[i] = probs[i]
+    return p
 
 
 def camera_ray(job, pixel, sample):

@@ -59,7 +59,7 @@ def test_struct_layouts_match_header(tmp_path):
 def test_struct_layouts():
     assert ctypes.sizeof(m.Material) == 64
     assert ctypes.sizeof(m.Camera) == 44 + 24
-    assert ctypes.sizeof(m.Job) == 44 + 24 + 28 + 16 + 8 + 4   # + renderer
+    assert ctypes.sizeof(m.Job) == 44 + 24 + 28 + 16 + 8 + 4 + 36   # + renderer, Metropolis fields
     assert ctypes.sizeof(m.Ray) == 24
     assert m.HIT_DTYPE.itemsize == 8
     assert ctypes.sizeof(m.KdParams) == 32

@@ -1,0 +1,76 @@
+"""Metropolis light transport on the device (MPT_RENDERER_MLT) against the
+oracle's restatement of MLTRenderer / MLTMarkovProcess and the three
+mutations (renderers/mlt/): chain by chain and mutation by mutation on the
+same random streams, the films they make, and the closed form of a light
+that fills the view."""
+import numpy as np
+import pytest
+
+import milton_amd as m
+from oracle import pyoracle as po
+from test_mlt import fullscreen_light, LE
+
+pytestmark = pytest.mark.gpu
+MLT = m.MPT_RENDERER_MLT
+
+
+def _job(a, w, h, seed=3, **mlt):
+    return m.make_job(a.camera, w, h, 1, 1, seed=seed, renderer=MLT, mlt=mlt)
+
+
+def _compare_traces(ctx, a, job, steps, what, min_chains):
+    """Device and oracle records of the first `steps` mutations of every
+    chain; returns the share of chains whose records all agree."""
+    sc = m.Scene(ctx, a)
+    dev, b_dev, n_dev = sc.mlt_trace(job, steps)
+    orc = po.OracleScene(a)
+    sd = orc.mlt_seeds(job)
+    ref = orc.mlt_trace(job, 0, job.mlt_chains, steps)
+    assert n_dev == len(sd["w"]) and abs(b_dev - sd["b"]) <= 1e-3 * sd["b"], (what, b_dev, sd["b"])
+    same = ((dev[..., 0] == ref[..., 0]) & (dev[..., 2] == ref[..., 2]) & (dev[..., 3] == ref[..., 3]) &
+            (dev[..., 4] == ref[..., 4]) & (dev[..., 5] == ref[..., 5]) &
+            (np.abs(dev[..., 1] - ref[..., 1]) <= 1e-3 * np.maximum(1e-3, np.abs(ref[..., 1]))))
+    chains = same.all(axis=1)
+    # a chain's first disagreement, by mutation type
+    first = np.argmin(same, axis=1)
+    kinds = [int(ref[c, first[c], 0]) for c in np.nonzero(~chains)[0]]
+    print(f"[{what}] chains agreeing over {steps} mutations: {chains.mean():.4f}; steps agreeing "
+          f"{same.mean():.5f}; first disagreement by mutation {np.bincount(kinds, minlength=3).tolist()}; "
+          f"acceptance dev {dev[..., 2].mean():.4f} oracle {ref[..., 2].mean():.4f}", flush=True)
+    assert chains.mean() >= min_chains, what
+    return dev, ref
+
+
+def test_fullscreen_light_closed_form(ctx):
+    """Every seed path is emitter-camera: b = lum(Le) W H / 3 and the film's
+    mean is 2/3 Le times mutations / samples, as in the oracle."""
+    a = fullscreen_light()
+    W, H = 8, 6
+    job = _job(a, W, H, chains=64, mutations=W * H * 4, seed_paths=1000, max_rejections=0, filter_proposed=False,
+               probs=(1.0, 1.0, 0.0))
+    film, st = m.Scene(ctx, a).render(job)
+    lum = 0.2126 * LE[0] + 0.7152 * LE[1] + 0.0722 * LE[2]
+    assert abs(st["mlt_b"] - lum * W * H / 3) <= 1e-6 * st["mlt_b"]
+    want = 2.0 / 3.0 * np.asarray(LE) * (st["paths"] / st["mlt_samples"])
+    assert np.allclose(film.reshape(-1, 3).mean(axis=0), want, rtol=1e-5)
+
+
+def test_room_traces_match_oracle(ctx):
+    a = m.generate(m.MPT_SCENE_ROOM)
+    job = _job(a, 32, 24, chains=256, mutations=16, seed_paths=2000, max_depth=10)
+    _compare_traces(ctx, a, job, 16, "room", 0.9)
+
+
+def test_room_film_matches_oracle_and_is_deterministic(ctx):
+    a = m.generate(m.MPT_SCENE_ROOM)
+    job = _job(a, 24, 16, chains=128, mutations=64, seed_paths=2000)
+    sc = m.Scene(ctx, a)
+    film, st = sc.render(job)
+    again, _ = sc.render(job)
+    assert np.array_equal(film, again)
+    orc, _, _, ost = po.OracleScene(a).mlt_render(job)
+    print(f"[mlt room film] dev {film.mean(axis=(0, 1))} oracle {orc.mean(axis=(0, 1))} "
+          f"samples {st['mlt_samples']} / {ost['samples']} accepted {st['mlt_accepted']} / {ost['accepted']}",
+          flush=True)
+    assert abs(st["mlt_samples"] - ost["samples"]) <= 0.02 * ost["samples"]
+    assert np.allclose(film.mean(axis=(0, 1)), orc.mean(axis=(0, 1)), rtol=0.05)
