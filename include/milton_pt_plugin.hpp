@@ -54,7 +54,14 @@ class PathTracer {
    *             "noDirectSamples" (1..64, DirectIllumination.cpp:33),
    *             "device" (HIP device), "batchPaths" (queue depth),
    *             "bidirectional" (1: BidirectionalPathTracer instead,
-   *             MiltonJSONSceneLoader.cpp:720-721)
+   *             MiltonJSONSceneLoader.cpp:720-721), "mlt" (1: MLTRenderer,
+   *             :731-738, with "noRenderThreads" (chains),
+   *             "mltNoInitialPaths", "mltMaxDepth",
+   *             "mltMaxConsequtiveRejections", "mltBidirPathMutationProb",
+   *             "mltLensSubpathMutationProb",
+   *             "mltPerturbationPathMutationProb", "mltFilterProposed" and
+   *             "mltMutations" per chain, by default noSuperSamples x
+   *             pixels / chains: the reference runs until stopped)
    *   output    "width" / "height" (RenderOutput.cpp:34)
    *   kd-tree   "kdMinPrimitives" (leaf size), "kdMaxDepth", "kdCostTraversal",
    *             "kdEmptyBias", "kdNoThreads" (kdTreeAccel.h:97-104)       */

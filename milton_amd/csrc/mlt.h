@@ -45,7 +45,7 @@ constexpr uint32_t kMaxLen = 16;     // most vertices of a chain's path (mltMaxD
 constexpr uint32_t kCap = kMaxLen + 1;   // vertex records per path buffer: a subpath may grow one past the cap
 // splits (k, s) of one seed path of kMaxLen vertices: sum over k = 2..16 of k + 1
 constexpr uint32_t kMaxEntries = (kMaxLen + 1) * (kMaxLen + 2) / 2 - 3;
-constexpr uint32_t kSlots = 9;       // float4 per vertex record
+constexpr uint32_t kSlots = 12;      // float4 per vertex record
 constexpr uint32_t kBlock = 128;     // threads per block of the chain kernels
 constexpr uint32_t kStackRows = 8;   // traversal ring rows per lane
 constexpr double kFix = 16777216.0;  // film sums: units of b / 2^24
@@ -55,11 +55,23 @@ constexpr double kFix = 16777216.0;  // film sums: units of b / 2^24
 constexpr uint32_t E_META = 1u, E_REFL = 2u, E_CAM = 16u;
 __device__ __forceinline__ uint32_t ev_ch(uint32_t e) { return (e >> 2) & 3u; }
 
+// Spectra and the values built from them in double precision: a Phong
+// vertex's pdf is that of its event's lobe alone, and the contributions of
+// seed splits and mutated paths reach 1e44 and more, past FP32's range.
+struct D3 {
+  double x, y, z;
+};
+__device__ __forceinline__ D3 d3(double x, double y, double z) { return D3{x, y, z}; }
+__device__ __forceinline__ D3 d3(float3 a) { return D3{a.x, a.y, a.z}; }
+__device__ __forceinline__ D3 dmul(D3 a, D3 b) { return D3{a.x * b.x, a.y * b.y, a.z * b.z}; }
+__device__ __forceinline__ D3 dscale(D3 a, double s) { return D3{a.x * s, a.y * s, a.z * s}; }
+__device__ __forceinline__ bool dzero(D3 a) { return a.x == 0 && a.y == 0 && a.z == 0; }
+
 // PathVertex (renderers/utils/PathVertex.h:38-120): position and primitive,
 // normal and info word (the BSDF kind it scatters with, Phong lobe of its
-// event, flags, material), wi, the event's direction and mode, both sides'
-// cumulative contributions and geometry terms, fs, both pdfs, the film
-// point of a camera vertex, and pL / pE in double.
+// event, flags, material), wi, the event's direction and mode, the film
+// point of a camera vertex; in double both sides' geometry terms, pdfs,
+// cumulative densities and contributions, and fs.
 struct MVtx {
   float3 p;
   uint32_t prim;
@@ -68,14 +80,9 @@ struct MVtx {
   float3 wi;
   uint32_t evb;
   float3 ev;
-  float3 alphaL;
-  float GL;
-  float3 alphaE;
-  float GE;
-  float3 fs;
-  float pdfL;
-  float pdfE, fu, fv;
-  double pL, pE;
+  float fu, fv;
+  double GL, GE, pdfL, pdfE, pL, pE;
+  D3 alphaL, alphaE, fs;
 };
 
 // A chain's (or seed lane's) path buffer: slot k of vertex i at
@@ -87,9 +94,17 @@ struct Store {
   __device__ __forceinline__ float4 &at(uint32_t i, uint32_t k) const {
     return base[((uint64_t)i * kSlots + k) * stride];
   }
+  __device__ __forceinline__ void get2(uint32_t i, uint32_t k, double &a, double &b) const {
+    const float4 q = at(i, k);
+    a = __hiloint2double(__float_as_int(q.y), __float_as_int(q.x));
+    b = __hiloint2double(__float_as_int(q.w), __float_as_int(q.z));
+  }
+  __device__ __forceinline__ void put2(uint32_t i, uint32_t k, double a, double b) const {
+    at(i, k) = make_float4(__int_as_float(__double2loint(a)), __int_as_float(__double2hiint(a)),
+                           __int_as_float(__double2loint(b)), __int_as_float(__double2hiint(b)));
+  }
   __device__ MVtx get(uint32_t i) const {
-    const float4 a = at(i, 0), b = at(i, 1), c = at(i, 2), d = at(i, 3), e = at(i, 4), f = at(i, 5), g = at(i, 6),
-                 h = at(i, 7), q = at(i, 8);
+    const float4 a = at(i, 0), b = at(i, 1), c = at(i, 2), d = at(i, 3);
     MVtx v;
     v.p = xyz(a);
     v.prim = __float_as_uint(a.w);
@@ -98,30 +113,32 @@ struct Store {
     v.wi = xyz(c);
     v.evb = __float_as_uint(c.w);
     v.ev = xyz(d);
-    v.alphaL = xyz(e);
-    v.GL = e.w;
-    v.alphaE = xyz(f);
-    v.GE = f.w;
-    v.fs = xyz(g);
-    v.pdfL = g.w;
-    v.pdfE = h.x;
-    v.fu = h.y;
-    v.fv = h.z;
-    v.pL = __hiloint2double(__float_as_int(q.y), __float_as_int(q.x));
-    v.pE = __hiloint2double(__float_as_int(q.w), __float_as_int(q.z));
+    v.fu = d.w;
+    double fv;
+    get2(i, 4, v.GL, v.GE);
+    get2(i, 5, v.pdfL, v.pdfE);
+    get2(i, 6, v.pL, v.pE);
+    get2(i, 7, v.alphaL.x, v.alphaL.y);
+    get2(i, 8, v.alphaL.z, v.alphaE.x);
+    get2(i, 9, v.alphaE.y, v.alphaE.z);
+    get2(i, 10, v.fs.x, v.fs.y);
+    get2(i, 11, v.fs.z, fv);
+    v.fv = (float)fv;
     return v;
   }
   __device__ void put(uint32_t i, const MVtx &v) const {
     at(i, 0) = make_float4(v.p.x, v.p.y, v.p.z, __uint_as_float(v.prim));
     at(i, 1) = make_float4(v.n.x, v.n.y, v.n.z, __uint_as_float(v.info));
     at(i, 2) = make_float4(v.wi.x, v.wi.y, v.wi.z, __uint_as_float(v.evb));
-    at(i, 3) = make_float4(v.ev.x, v.ev.y, v.ev.z, 0.0f);
-    at(i, 4) = make_float4(v.alphaL.x, v.alphaL.y, v.alphaL.z, v.GL);
-    at(i, 5) = make_float4(v.alphaE.x, v.alphaE.y, v.alphaE.z, v.GE);
-    at(i, 6) = make_float4(v.fs.x, v.fs.y, v.fs.z, v.pdfL);
-    at(i, 7) = make_float4(v.pdfE, v.fu, v.fv, 0.0f);
-    at(i, 8) = make_float4(__int_as_float(__double2loint(v.pL)), __int_as_float(__double2hiint(v.pL)),
-                           __int_as_float(__double2loint(v.pE)), __int_as_float(__double2hiint(v.pE)));
+    at(i, 3) = make_float4(v.ev.x, v.ev.y, v.ev.z, v.fu);
+    put2(i, 4, v.GL, v.GE);
+    put2(i, 5, v.pdfL, v.pdfE);
+    put2(i, 6, v.pL, v.pE);
+    put2(i, 7, v.alphaL.x, v.alphaL.y);
+    put2(i, 8, v.alphaL.z, v.alphaE.x);
+    put2(i, 9, v.alphaE.y, v.alphaE.z);
+    put2(i, 10, v.fs.x, v.fs.y);
+    put2(i, 11, v.fs.z, (double)v.fv);
   }
   __device__ void copy(uint32_t dst, const Store &src, uint32_t i) const {
 #pragma unroll
@@ -173,27 +190,102 @@ __device__ __forceinline__ uint32_t material_key(const MVtx &v) {
   if (v.info & I_POINT) return 0x80000000u | mat_of(v.info);
   return mat_of(v.info);
 }
-__device__ __forceinline__ float ndot_m(const MVtx &v, float3 w) { return (v.info & I_NO_NORMAL) ? 1.0f : dot(v.n, w); }
-__device__ __forceinline__ double lum(float3 c) { return 0.2126 * (double)c.x + 0.7152 * (double)c.y + 0.0722 * (double)c.z; }
-__device__ __forceinline__ double lum_ratio(float3 y, float3 x) {
-  return 0.2126 * ((double)y.x / (double)x.x) + 0.7152 * ((double)y.y / (double)x.y) +
-         0.0722 * ((double)y.z / (double)x.z);
+__device__ __forceinline__ double ddot(float3 a, float3 b) {
+  return (double)a.x * b.x + (double)a.y * b.y + (double)a.z * b.z;
+}
+__device__ __forceinline__ double ndot_m(const MVtx &v, float3 w) { return (v.info & I_NO_NORMAL) ? 1.0 : ddot(v.n, w); }
+__device__ __forceinline__ double lum(D3 c) { return 0.2126 * c.x + 0.7152 * c.y + 0.0722 * c.z; }
+__device__ __forceinline__ double lum_ratio(D3 y, D3 x) {
+  return 0.2126 * (y.x / x.x) + 0.7152 * (y.y / x.y) + 0.0722 * (y.z / x.z);
 }
 // MIN (common/utilities.h:161-165)
 __device__ __forceinline__ double ref_min(double x, double y) { return x < y ? x : y; }
 
 // Emitter::getLe0: pi Le for an area light, 4 pi Le for a point
-__device__ float3 le0_of(const Scene &sc, const MVtx &v) {
-  if (v.info & I_POINT) return mul(xyz(sc.points[2 * mat_of(v.info) + 1]), 4.0f * kPi);
+__device__ D3 le0_of(const Scene &sc, const MVtx &v) {
+  if (v.info & I_POINT) return dscale(d3(xyz(sc.points[2 * mat_of(v.info) + 1])), 4.0 * M_PI);
   const mpt_material &m = sc.mats[mat_of(v.info)];
-  return emissive(m) ? mul(le_of(m), kPi) : f3(0, 0, 0);
+  return emissive(m) ? dscale(d3(le_of(m)), M_PI) : d3(0, 0, 0);
 }
 // Emitter::getLe of the vertex's emitter toward wo
-__device__ float3 le_toward(const Scene &sc, const MVtx &v, float3 wo) {
-  if (v.info & I_POINT) return xyz(sc.points[2 * mat_of(v.info) + 1]);
-  if (is_cam(v)) return f3(0, 0, 0);
+__device__ D3 le_toward(const Scene &sc, const MVtx &v, float3 wo) {
+  if (v.info & I_POINT) return d3(xyz(sc.points[2 * mat_of(v.info) + 1]));
+  if (is_cam(v)) return d3(0, 0, 0);
   const mpt_material &m = sc.mats[mat_of(v.info)];
-  return dot(v.n, wo) > 0.0f ? le_of(m) : f3(0, 0, 0);
+  return dot(v.n, wo) > 0.0f ? d3(le_of(m)) : d3(0, 0, 0);
+}
+
+// BSDF::getPd(Event(dir, event)) of vertex v with incident direction wi, in
+// double: the bidirectional kernels' pdf_v with every product widened.
+__device__ double pdf_d(const Scene &sc, const MVtx &v, float3 wi, float3 dir) {
+  const uint32_t k = kind_of(v.info);
+  if (k == K_SENSOR) {
+    if (same_dir(wi, mul(v.n, -1.0f))) return 1.0;
+    return same_dir(dir, v.n) ? 1.0 : 0.0;
+  }
+  if (k == K_EMITTER) {
+    if (v.info & I_POINT) return 1.0;
+    return dot(v.n, dir) > 0.0f ? 1.0 / M_PI : 0.0;
+  }
+  const mpt_material &m = sc.mats[mat_of(v.info)];
+  if (m.type == MPT_BSDF_DIFFUSE) {
+    if (dot(v.n, wi) > 0.0f) return 0.0;
+    return dot(v.n, dir) > 0.0f ? 1.0 / M_PI : 0.0;
+  }
+  if (m.type == MPT_BSDF_PHONG) {
+    const double kda = ((double)m.kd[0] + m.kd[1] + m.kd[2]) / 3.0, ksa = ((double)m.ks[0] + m.ks[1] + m.ks[2]) / 3.0;
+    const uint32_t lobe = lobe_of(v.info);
+    if (lobe == LOBE_ABSORB) return 1.0 - (kda + ksa);
+    const double cosA = ddot(v.n, dir);
+    if (cosA <= 0) return 0.0;
+    double pdf;
+    if (lobe == LOBE_SPECULAR) {
+      const double c = fabs(ddot(reflect(wi, v.n), dir));
+      pdf = fmin(1.0, ((double)m.exponent + 1.0) / (2.0 * M_PI) * pow(c, (double)m.exponent));
+    } else {
+      pdf = fabs(ddot(v.n, dir)) / M_PI;
+    }
+    return pdf / cosA;
+  }
+  return (double)pdf_v(sc, view(v), wi, dir);   // a dielectric: 0, 1/3, 2/3 or 1
+}
+
+// BSDF::evaluate(wi, dir) of vertex v in double.
+__device__ D3 f_d(const Scene &sc, const MVtx &v, float3 wi, float3 dir) {
+  const uint32_t k = kind_of(v.info);
+  if (k == K_SENSOR) return same_dir(mul(wi, -1.0f), v.n) ? d3(1, 1, 1) : d3(0, 0, 0);
+  if (k == K_EMITTER) {
+    if (v.info & I_POINT) {
+      const double f = 0.25 / M_PI;
+      return d3(f, f, f);
+    }
+    if (!(dot(v.n, dir) > 0.0f)) return d3(0, 0, 0);
+    const float3 le = le_of(sc.mats[mat_of(v.info)]);
+    const double f = 1.0 / M_PI;
+    return d3(le.x != 0.0f ? f : 0.0, le.y != 0.0f ? f : 0.0, le.z != 0.0f ? f : 0.0);
+  }
+  const mpt_material &m = sc.mats[mat_of(v.info)];
+  if (m.type == MPT_BSDF_DIFFUSE) {
+    if (dot(v.n, wi) >= 0.0f || dot(v.n, dir) <= 0.0f) return d3(0, 0, 0);
+    return dscale(d3(kd_of(m)), 1.0 / M_PI);
+  }
+  if (m.type == MPT_BSDF_PHONG) {
+    if (dot(v.n, dir) <= 0.0f) return d3(0, 0, 0);
+    const double a = fmin(1.0, fmax(0.0, ddot(reflect(wi, v.n), dir)));
+    const double kda = ((double)m.kd[0] + m.kd[1] + m.kd[2]) / 3.0, ksa = ((double)m.ks[0] + m.ks[1] + m.ks[2]) / 3.0;
+    const double an = pow(a, (double)m.exponent);
+    const double fs_s = ksa > 0 ? ((double)m.exponent + 2.0) / (2.0 * M_PI * ksa) * an : 0.0;
+    const double fs_d = kda > 0 ? 1.0 / (M_PI * kda) : 0.0;
+    return d3(m.kd[0] * fs_d + m.ks[0] * fs_s, m.kd[1] * fs_d + m.ks[1] * fs_s, m.kd[2] * fs_d + m.ks[2] * fs_s);
+  }
+  return d3(f_v(sc, view(v), wi, dir));   // a dielectric: ks on its events
+}
+__device__ __forceinline__ bool black_d(D3 a) { return fabs(a.x) <= kEps && fabs(a.y) <= kEps && fabs(a.z) <= kEps; }
+__device__ __forceinline__ double max_d(D3 a) {
+  double m = a.x;
+  if (a.y > m) m = a.y;
+  if (a.z > m) m = a.z;
+  return m;
 }
 
 // BSDF::sample(e) of v with incident direction wi; keep: the event whose
@@ -239,66 +331,66 @@ __device__ void mv_sample(const Scene &sc, MVtx &v, float3 wi, Seq &rng, const M
 
 // Path::_initL (Path.cpp:615-667) with the roulette's uniform drawn when it
 // is taken.
-__device__ bool init_L(const Scene &sc, MVtx &y, float3 wi, float3 wo, float t, const MVtx &pt, bool roulette,
-                       Seq &rng, float3 &alphaL, double &pL) {
+__device__ bool init_L(const Scene &sc, MVtx &y, float3 wi, float3 wo, double t, const MVtx &pt, bool roulette,
+                       Seq &rng, D3 &alphaL, double &pL) {
   if (t < kEps) {
-    y.GL = 0.0f;
+    y.GL = 0;
     return false;
   }
   const float3 mwo = mul(wo, -1.0f), mwi = mul(wi, -1.0f);
-  y.GL = fabsf(ndot_m(y, wo) * ndot_m(pt, mwo)) / (t * t);
-  y.pdfE = pdf_v(sc, view(y), mwo, mwi);
-  y.fs = f_v(sc, view(y), wi, wo);
-  y.pdfL = pdf_v(sc, view(y), wi, wo);
-  if (y.GL == 0.0f || y.pdfL == 0.0f) {
+  y.GL = fabs(ndot_m(y, wo) * ndot_m(pt, mwo)) / (t * t);
+  y.pdfE = pdf_d(sc, y, mwo, mwi);
+  y.fs = f_d(sc, y, wi, wo);
+  y.pdfL = pdf_d(sc, y, wi, wo);
+  if (y.GL == 0 || y.pdfL == 0) {
     pL = 0;
-    alphaL = f3(0, 0, 0);
+    alphaL = d3(0, 0, 0);
     return false;
   }
-  if (roulette || black_ref(y.fs)) {
-    const float q = fminf(0.95f, max_sample(y.fs) / y.pdfL);
-    if (q <= rng.next()) {
-      y.pdfL = 0.0f;
+  if (roulette || black_d(y.fs)) {
+    const double q = fmin(0.95, max_d(y.fs) / y.pdfL);
+    if (q <= (double)rng.next()) {
+      y.pdfL = 0;
       pL = 0;
-      alphaL = f3(0, 0, 0);
+      alphaL = d3(0, 0, 0);
       return false;
     }
     y.pdfL *= q;
   }
-  pL = ((double)y.pdfL * (double)y.GL) * y.pL;
-  alphaL = mul(mul(y.fs, 1.0f / y.pdfL), y.alphaL);
+  pL = (y.pdfL * y.GL) * y.pL;
+  alphaL = dmul(dscale(y.fs, 1.0 / y.pdfL), y.alphaL);
   return true;
 }
 
 // Path::_initE (Path.cpp:669-720).
-__device__ bool init_E(const Scene &sc, MVtx &z, float3 wi, float3 wo, float t, const MVtx &pt, bool roulette,
-                       Seq &rng, float3 &alphaE, double &pE) {
+__device__ bool init_E(const Scene &sc, MVtx &z, float3 wi, float3 wo, double t, const MVtx &pt, bool roulette,
+                       Seq &rng, D3 &alphaE, double &pE) {
   if (t < kEps) {
-    z.GE = 0.0f;
+    z.GE = 0;
     return false;
   }
   const float3 mwo = mul(wo, -1.0f), mwi = mul(wi, -1.0f);
-  z.GE = fabsf(ndot_m(z, wo) * ndot_m(pt, mwo)) / (t * t);
-  z.pdfL = pdf_v(sc, view(z), mwo, mwi);
-  z.fs = f_v(sc, view(z), mwo, mwi);
-  z.pdfE = pdf_v(sc, view(z), wi, wo);
-  if (z.GE == 0.0f || z.pdfE == 0.0f) {
+  z.GE = fabs(ndot_m(z, wo) * ndot_m(pt, mwo)) / (t * t);
+  z.pdfL = pdf_d(sc, z, mwo, mwi);
+  z.fs = f_d(sc, z, mwo, mwi);
+  z.pdfE = pdf_d(sc, z, wi, wo);
+  if (z.GE == 0 || z.pdfE == 0) {
     pE = 0;
-    alphaE = f3(0, 0, 0);
+    alphaE = d3(0, 0, 0);
     return false;
   }
-  if (roulette || black_ref(z.fs)) {
-    const float q = fminf(0.95f, max_sample(z.fs) / z.pdfE);
-    if (q <= rng.next()) {
-      z.pdfE = 0.0f;
+  if (roulette || black_d(z.fs)) {
+    const double q = fmin(0.95, max_d(z.fs) / z.pdfE);
+    if (q <= (double)rng.next()) {
+      z.pdfE = 0;
       pE = 0;
-      alphaE = f3(0, 0, 0);
+      alphaE = d3(0, 0, 0);
       return false;
     }
     z.pdfE *= q;
   }
-  pE = ((double)z.pdfE * (double)z.GE) * z.pE;
-  alphaE = mul(mul(z.fs, 1.0f / z.pdfE), z.alphaE);
+  pE = (z.pdfE * z.GE) * z.pE;
+  alphaE = dmul(dscale(z.fs, 1.0 / z.pdfE), z.alphaE);
   return true;
 }
 
@@ -321,11 +413,11 @@ __device__ MVtx camera_vertex(const Scene &sc, const Ctx &cx, float fx, float fy
   c.evb = E_CAM;
   c.wi = mul(c.n, -1.0f);
   c.ev = f3(0, 0, 0);
-  c.alphaL = f3(1, 1, 1);
-  c.alphaE = f3((float)(1.0 / pA), (float)(1.0 / pA), (float)(1.0 / pA));
-  c.GL = c.GE = 1.0f;
-  c.fs = f3(0, 0, 0);
-  c.pdfL = c.pdfE = 1.0f;
+  c.alphaL = d3(1, 1, 1);
+  c.alphaE = d3(1.0 / pA, 1.0 / pA, 1.0 / pA);
+  c.GL = c.GE = 1.0;
+  c.fs = d3(0, 0, 0);
+  c.pdfL = c.pdfE = 1.0;
   c.fu = fx;
   c.fv = fy;
   c.pL = 1.0;
@@ -343,10 +435,10 @@ __device__ MVtx hit_vertex(const Scene &sc, uint32_t prim, float3 p, float2 uv) 
   v.info = s.info;
   v.evb = 0;
   v.ev = f3(0, 0, 0);
-  v.alphaL = v.alphaE = f3(1, 1, 1);
-  v.GL = v.GE = 1.0f;
-  v.fs = f3(0, 0, 0);
-  v.pdfL = v.pdfE = 1.0f;
+  v.alphaL = v.alphaE = d3(1, 1, 1);
+  v.GL = v.GE = 1.0;
+  v.fs = d3(0, 0, 0);
+  v.pdfL = v.pdfE = 1.0;
   v.fu = v.fv = 0.0f;
   v.pL = v.pE = 1.0;
   return v;
@@ -386,29 +478,29 @@ __device__ bool sample_vertex(const Scene &sc, Ctx &cx, Sub &A, bool roulette, b
   v.wi = wi;
   MVtx nv = hit_vertex(sc, id, add(v.p, mul(wo, t)), uv);
   if (adjoint) {
-    float3 alphaE;
+    D3 alphaE;
     double pE;
     const bool ok = init_E(sc, v, wi, wo, t, nv, roulette, rng, alphaE, pE);
     A.s.put(n - 1, v);
     if (!ok) return false;
     nv.wi = wo;
     nv.GL = v.GE;
-    nv.GE = 1.0f;
-    nv.alphaL = f3(1, 1, 1);
+    nv.GE = 1.0;
+    nv.alphaL = d3(1, 1, 1);
     nv.alphaE = alphaE;
     nv.pL = 1.0;
     nv.pE = pE;
   } else {
-    float3 alphaL;
+    D3 alphaL;
     double pL;
     const bool ok = init_L(sc, v, wi, wo, t, nv, roulette, rng, alphaL, pL);
     A.s.put(n - 1, v);
     if (!ok) return false;
     nv.wi = wo;
-    nv.GL = 1.0f;
+    nv.GL = 1.0;
     nv.GE = v.GL;
     nv.alphaL = alphaL;
-    nv.alphaE = f3(1, 1, 1);
+    nv.alphaE = d3(1, 1, 1);
     nv.pL = pL;
     nv.pE = 1.0;
   }
@@ -450,11 +542,11 @@ __device__ bool append_light(const Scene &sc, Ctx &cx, Sub &A, bool roulette, bo
   const double pA = sc.emit_area > 0.0f ? 1.0 / (double)sc.emit_area : 1.0;
   e.pL = pA;
   e.pE = 1.0;
-  e.alphaL = mul(le0_of(sc, e), (float)(1.0 / pA));
-  e.alphaE = f3(1, 1, 1);
-  e.GL = e.GE = 1.0f;
-  e.fs = f3(0, 0, 0);
-  e.pdfL = e.pdfE = 1.0f;
+  e.alphaL = dscale(le0_of(sc, e), 1.0 / pA);
+  e.alphaE = d3(1, 1, 1);
+  e.GL = e.GE = 1.0;
+  e.fs = d3(0, 0, 0);
+  e.pdfL = e.pdfE = 1.0;
   e.fu = e.fv = 0.0f;
   mv_sample(sc, e, e.wi, rng, nullptr);
   A.s.put(0, e);
@@ -491,13 +583,13 @@ __device__ bool occluded(const Scene &sc, Ctx &cx, const MVtx &a, const MVtx &b)
 }
 
 // Path::_computeRadiance (Path.cpp:208-222).
-__device__ float3 path_radiance(const Scene &sc, const Store &P, uint32_t n) {
-  if (n < 2) return f3(0, 0, 0);
+__device__ D3 path_radiance(const Scene &sc, const Store &P, uint32_t n) {
+  if (n < 2) return d3(0, 0, 0);
   const MVtx f = P.get(0);
-  float3 r = (!is_cam(f) && on_emitter(sc, f.info)) ? le0_of(sc, f) : f3(0, 0, 0);
+  D3 r = (!is_cam(f) && on_emitter(sc, f.info)) ? le0_of(sc, f) : d3(0, 0, 0);
   for (uint32_t i = 0; i < n; ++i) {
     const MVtx v = i == 0 ? f : P.get(i);
-    r = mul(mul(r, v.fs), v.GL);
+    r = dscale(dmul(r, v.fs), v.GL);
   }
   return r;
 }
@@ -505,7 +597,7 @@ __device__ float3 path_radiance(const Scene &sc, const Store &P, uint32_t n) {
 // Path::append(const Path &) (Path.cpp:348-515): light subpath Y (len s)
 // and eye subpath T (len t, camera first) joined into Y, both sides'
 // cumulative values completed, the radiance.
-__device__ bool join(const Scene &sc, Ctx &cx, Sub &Y, const Sub &T, Seq &rng, float3 &rad) {
+__device__ bool join(const Scene &sc, Ctx &cx, Sub &Y, const Sub &T, Seq &rng, D3 &rad) {
   const uint32_t s = Y.len, t = T.len, k = s + t;
   for (uint32_t i = 0; i < t; ++i) Y.s.copy(s + i, T.s, t - 1 - i);
   Y.len = k;
@@ -522,14 +614,14 @@ __device__ bool join(const Scene &sc, Ctx &cx, Sub &Y, const Sub &T, Seq &rng, f
     float3 wo = sub(z.p, y.p);
     const float d = len(wo);
     wo = mul(wo, 1.0f / d);
-    float3 alpha;
+    D3 alpha;
     double pA;
     float3 wi = s > 1 ? normalize(sub(y.p, Y.s.get(s - 2).p)) : mul(wo, -1.0f);
     if (!init_L(sc, y, wi, wo, d, z, false, rng, alpha, pA)) invalid = true;
     wi = t > 1 ? normalize(sub(z.p, Y.s.get(s + 1).p)) : wo;
     if (!init_E(sc, z, wi, mul(wo, -1.0f), d, y, false, rng, alpha, pA)) invalid = true;
     if (!invalid && occluded(sc, cx, y, z)) {
-      y.GL = z.GE = 0.0f;
+      y.GL = z.GE = 0;
       invalid = true;
     }
     Y.s.put(s - 1, y);
@@ -545,21 +637,20 @@ __device__ bool join(const Scene &sc, Ctx &cx, Sub &Y, const Sub &T, Seq &rng, f
           mv_sample(sc, y, y.wi, rng, nullptr);
         }
         const float3 wi = i > 0 ? normalize(sub(y.p, Y.s.get(i - 1).p)) : y.wi;
-        const float ia = (float)(1.0 / pA);
-        y.alphaE = f3(ia, ia, ia);
+        y.alphaE = d3(1.0 / pA, 1.0 / pA, 1.0 / pA);
         y.pE = pA;
-        y.fs = f_v(sc, view(y), wi, mul(wi, -1.0f));
-        y.pdfE = pdf_v(sc, view(y), wi, mul(wi, -1.0f));
+        y.fs = f_d(sc, y, wi, mul(wi, -1.0f));
+        y.pdfE = pdf_d(sc, y, wi, mul(wi, -1.0f));
       } else {
-        y.alphaE = f3(0, 0, 0);
+        y.alphaE = d3(0, 0, 0);
         y.pE = 0;
-        y.fs = f3(0, 0, 0);
-        y.pdfE = 0.0f;
+        y.fs = d3(0, 0, 0);
+        y.pdfE = 0;
       }
     } else {
       const MVtx z = Y.s.get(i + 1);
-      y.alphaE = (z.pdfE > 0.0f && z.GE > 0.0f) ? mul(z.alphaE, mul(z.fs, 1.0f / z.pdfE)) : f3(0, 0, 0);
-      y.pE = z.pE * (double)z.pdfE * (double)z.GE;
+      y.alphaE = (z.pdfE > 0 && z.GE > 0) ? dmul(z.alphaE, dscale(z.fs, 1.0 / z.pdfE)) : d3(0, 0, 0);
+      y.pE = z.pE * z.pdfE * z.GE;
     }
     Y.s.put(i, y);
   }
@@ -569,20 +660,20 @@ __device__ bool join(const Scene &sc, Ctx &cx, Sub &Y, const Sub &T, Seq &rng, f
       if (on_em(sc, z)) {
         const double pA = sc.emit_area > 0.0f ? 1.0 / (double)sc.emit_area : 1.0;
         const float3 wo = s + i < k - 1 ? normalize(sub(Y.s.get(s + i + 1).p, z.p)) : mul(z.wi, -1.0f);
-        z.alphaL = mul(le0_of(sc, z), (float)(1.0 / pA));
+        z.alphaL = dscale(le0_of(sc, z), 1.0 / pA);
         z.pL = pA;
-        z.fs = f_v(sc, view(z), f3(0, 0, 0), wo);
-        z.pdfL = pdf_v(sc, view(z), f3(0, 0, 0), wo);
+        z.fs = f_d(sc, z, f3(0, 0, 0), wo);
+        z.pdfL = pdf_d(sc, z, f3(0, 0, 0), wo);
       } else {
-        z.alphaL = f3(0, 0, 0);
+        z.alphaL = d3(0, 0, 0);
         z.pL = 0;
-        z.fs = f3(0, 0, 0);
-        z.pdfL = 0.0f;
+        z.fs = d3(0, 0, 0);
+        z.pdfL = 0;
       }
     } else {
       const MVtx y = Y.s.get(s + i - 1);
-      z.alphaL = (y.pdfL > 0.0f && y.GL > 0.0f) ? mul(y.alphaL, mul(y.fs, 1.0f / y.pdfL)) : f3(0, 0, 0);
-      z.pL = y.pL * (double)y.pdfL * (double)y.GL;
+      z.alphaL = (y.pdfL > 0 && y.GL > 0) ? dmul(y.alphaL, dscale(y.fs, 1.0 / y.pdfL)) : d3(0, 0, 0);
+      z.pL = y.pL * y.pdfL * y.GL;
     }
     Y.s.put(s + i, z);
   }
@@ -591,27 +682,27 @@ __device__ bool join(const Scene &sc, Ctx &cx, Sub &Y, const Sub &T, Seq &rng, f
 }
 
 // Path::getContribution (Path.cpp:47-150) of split (s, t) of path P (len n).
-__device__ float3 contribution_st(const Scene &sc, Ctx &cx, const Store &P, uint32_t n, uint32_t s, uint32_t t) {
+__device__ D3 contribution_st(const Scene &sc, Ctx &cx, const Store &P, uint32_t n, uint32_t s, uint32_t t) {
   const uint32_t u = s - 1, v = n - t;
   if (s == 0) {
     const MVtx z = P.get(v);
-    if (!has_area(z) || is_cam(z)) return f3(0, 0, 0);
+    if (!has_area(z) || is_cam(z)) return d3(0, 0, 0);
     const float3 wo = v < n - 1 ? normalize(sub(P.get(v + 1).p, z.p)) : mul(z.wi, -1.0f);
-    return mul(le_toward(sc, z, wo), z.alphaE);
+    return dmul(le_toward(sc, z, wo), z.alphaE);
   }
-  if (t == 0) return f3(0, 0, 0);   // the camera has no area; NullSensor::getWe is black elsewhere
+  if (t == 0) return d3(0, 0, 0);   // the camera has no area; NullSensor::getWe is black elsewhere
   const MVtx y = P.get(u), z = P.get(v);
-  if (spec(sc, y) || spec(sc, z)) return f3(0, 0, 0);
-  if (s + t == n) return mul(mul(mul(mul(y.alphaL, y.fs), y.GL), z.fs), z.alphaE);
+  if (spec(sc, y) || spec(sc, z)) return d3(0, 0, 0);
+  if (s + t == n) return dmul(dmul(dscale(dmul(y.alphaL, y.fs), y.GL), z.fs), z.alphaE);
   float3 wo = sub(z.p, y.p);
   const float d = len(wo);
-  if (d < kEps) return f3(0, 0, 0);
-  if (occluded(sc, cx, y, z)) return f3(0, 0, 0);
+  if (d < kEps) return d3(0, 0, 0);
+  if (occluded(sc, cx, y, z)) return d3(0, 0, 0);
   wo = mul(wo, 1.0f / d);
   const float3 ywi = u > 0 ? normalize(sub(y.p, P.get(u - 1).p)) : y.wi;
   const float3 wo2 = v < n - 1 ? normalize(sub(P.get(v + 1).p, z.p)) : mul(wo, -1.0f);
-  const float G = fabsf(ndot_m(y, wo) * ndot_m(z, mul(wo, -1.0f))) / (d * d);
-  return mul(mul(mul(mul(y.alphaL, f_v(sc, view(y), ywi, wo)), G), f_v(sc, view(z), wo, wo2)), z.alphaE);
+  const double G = fabs(ndot_m(y, wo) * ndot_m(z, mul(wo, -1.0f))) / ((double)d * d);
+  return dmul(dmul(dscale(dmul(y.alphaL, f_d(sc, y, ywi, wo)), G), f_d(sc, z, wo, wo2)), z.alphaE);
 }
 
 // getPd of a full-length split (no visibility test), as the mutations use it.
@@ -650,7 +741,7 @@ __device__ __forceinline__ float clamp01(float v) { return v >= 1.0f ? 1.0f : (v
 // BidirectionalPathTracer::generate (BidirectionalPathTracer.cpp:212-249)
 // into L (the light subpath, then the whole path) with T as the eye
 // subpath; false once the path would exceed `cap` vertices.
-__device__ bool generate(const Scene &sc, Ctx &cx, Sub &L, Sub &T, uint32_t cap, Seq &rng, float3 &rad) {
+__device__ bool generate(const Scene &sc, Ctx &cx, Sub &L, Sub &T, uint32_t cap, Seq &rng, D3 &rad) {
   T.len = 0;
   L.len = 0;
   for (;;) {
@@ -672,14 +763,12 @@ __device__ bool generate(const Scene &sc, Ctx &cx, Sub &L, Sub &T, uint32_t cap,
 
 // MLTMarkovProcess::_initSample (MLTMarkovProcess.cpp:114-152): a black
 // value keeps the sample's previous film bin.
-__device__ void init_sample(const Ctx &cx, const Store &P, uint32_t n, float3 rad, double b, uint32_t &bin,
-                            float3 &val) {
-  if (n < 2 || (rad.x == 0.0f && rad.y == 0.0f && rad.z == 0.0f)) {
-    val = f3(0, 0, 0);
+__device__ void init_sample(const Ctx &cx, const Store &P, uint32_t n, D3 rad, double b, uint32_t &bin, D3 &val) {
+  if (n < 2 || dzero(rad)) {
+    val = d3(0, 0, 0);
     return;
   }
-  const double s = b / lum(rad);
-  val = f3((float)(rad.x * s), (float)(rad.y * s), (float)(rad.z * s));
+  val = dscale(rad, b / lum(rad));
   double x, y;
   project(cx.cam, P.get(n - 2).p, x, y);
   bin = film_bin(cx, x, y);
@@ -702,7 +791,7 @@ __device__ double pa1_of(uint32_t kd, uint32_t ka) {   // _getPa1 (:478-490)
   return 0.2 * (1.0 / (double)(1ull << (diff & 63)));
 }
 // _getMutationR (:205-365): T(y|x) over every split of the added subpath.
-__device__ double mutation_R(const Scene &sc, const Store &P, uint32_t n, float3 rad, uint32_t ka, double pd,
+__device__ double mutation_R(const Scene &sc, const Store &P, uint32_t n, D3 rad, uint32_t ka, double pd,
                              double pa1, int l) {
   if (lum(rad) <= 0) return 0;
   double T = 0;
@@ -716,11 +805,11 @@ __device__ double mutation_R(const Scene &sc, const Store &P, uint32_t n, float3
 struct ChainPaths {
   Store X, Y, T;
   uint32_t xlen;
-  float3 xrad;
+  D3 xrad;
 };
 
 // MLTBidirPathMutation::mutate (:29-202).
-__device__ double mutate_bidir(const Scene &sc, Ctx &cx, const ChainPaths &C, Sub &Y, float3 &yrad, uint32_t max_len,
+__device__ double mutate_bidir(const Scene &sc, Ctx &cx, const ChainPaths &C, Sub &Y, D3 &yrad, uint32_t max_len,
                                Seq &rng) {
   const uint32_t n = C.xlen;
   uint32_t kd = 0, ka = 0;
@@ -855,7 +944,7 @@ __device__ void rover_next(Strat &st, uint32_t W, uint32_t H, uint32_t &col, uin
 }
 
 // MLTLensSubpathMutation::mutate (:35-160).
-__device__ double mutate_lens(const Scene &sc, Ctx &cx, const ChainPaths &C, Sub &Y, float3 &yrad, Strat &st,
+__device__ double mutate_lens(const Scene &sc, Ctx &cx, const ChainPaths &C, Sub &Y, D3 &yrad, Strat &st,
                               Seq &rng) {
   const uint32_t n = C.xlen;
   // delete the lens subpath: the camera, zero or more specular vertices,
@@ -914,7 +1003,7 @@ __device__ float3 perturb(float3 N, Seq &rng) {
 }
 
 // MLTPerturbationPathMutation::mutate (:60-271).
-__device__ double mutate_perturb(const Scene &sc, Ctx &cx, const ChainPaths &C, Sub &Y, float3 &yrad, Seq &rng) {
+__device__ double mutate_perturb(const Scene &sc, Ctx &cx, const ChainPaths &C, Sub &Y, D3 &yrad, Seq &rng) {
   enum { LENS, MULTI, CAUSTIC };
   const uint32_t n = C.xlen;
   int type;
@@ -1024,9 +1113,7 @@ struct MltDesc {
 struct ChainState {
   uint32_t xbuf, xlen, cur_bin, tent_bin;
   uint32_t visits, smax, scur, seq;
-  float cur_val[3], tent_val[3];
-  float xrad[3];
-  uint32_t pad;
+  D3 cur_val, xrad;
 };
 
 struct MltBufs {
@@ -1088,7 +1175,7 @@ __global__ __launch_bounds__(kBlock) void k_mlt_seed(Scene sc, const MltDesc *__
   uint32_t cnt = 0;
   for (uint32_t a = 0; a < kAttempts; ++a) {
     Seq rng(d.seed, d.job_id, idx, a, kSeedKind);
-    float3 rad;
+    D3 rad;
     if (!generate(sc, cx, G, T, d.max_len, rng, rad)) continue;
     if (G.len > d.max_len) continue;
     B.seed_att[i] = a;
@@ -1103,7 +1190,7 @@ __global__ __launch_bounds__(kBlock) void k_mlt_seed(Scene sc, const MltDesc *__
         for (uint32_t j = 0; j < s; ++j) P.s.copy(j, G.s, j);
         Sub R{T.s, t};
         for (uint32_t j = 0; j < t; ++j) R.s.copy(j, G.s, L - 1 - j);
-        float3 prad;
+        D3 prad;
         const bool v = join(sc, cx, P, R, rng, prad);
         B.ent_w[e] = v ? lum(contribution_st(sc, cx, G.s, L, s, t)) : 0.0;
         ++cnt;
@@ -1202,7 +1289,7 @@ __global__ __launch_bounds__(kBlock) void k_mlt_start(Scene sc, const MltDesc *_
   Sub R{chain_store(B, d, 2, c), t};
   for (uint32_t j = 0; j < t; ++j) R.s.copy(j, G, L - 1 - j);
   Seq rng(d.seed, d.job_id, d.round * N + i, B.seed_att[i], kSeedKind, B.ent_pos[e]);
-  float3 rad;
+  D3 rad;
   (void)join(sc, cx, X, R, rng, rad);
   ChainState st;
   st.xbuf = 0;
@@ -1212,27 +1299,21 @@ __global__ __launch_bounds__(kBlock) void k_mlt_start(Scene sc, const MltDesc *_
   st.smax = 1;
   st.scur = 0;
   st.seq = 1;
-  float3 val;
+  D3 val;
   uint32_t bin = 0;
   init_sample(cx, X.s, X.len, rad, d.b, bin, val);
   st.cur_bin = bin;
-  st.cur_val[0] = val.x;
-  st.cur_val[1] = val.y;
-  st.cur_val[2] = val.z;
-  st.tent_val[0] = st.tent_val[1] = st.tent_val[2] = 0.0f;
-  st.xrad[0] = rad.x;
-  st.xrad[1] = rad.y;
-  st.xrad[2] = rad.z;
-  st.pad = 0;
+  st.cur_val = val;
+  st.xrad = rad;
   B.st[c] = st;
   flush_counters(B.counters, 0, 0, cx.n_ext, cx.n_vis);
 }
 
-__device__ __forceinline__ void add_sample(const MltBufs &B, double b, uint32_t bin, float3 val, double w) {
+__device__ __forceinline__ void add_sample(const MltBufs &B, double b, uint32_t bin, D3 val, double w) {
   // value / b * w in units of 2^-24 (values are at most 1 / 0.0722 times b)
   const double s = w / b * kFix;
   unsigned long long *o = B.num + 3ull * bin;
-  const double q[3] = {(double)val.x * s, (double)val.y * s, (double)val.z * s};
+  const double q[3] = {val.x * s, val.y * s, val.z * s};
 #pragma unroll
   for (int k = 0; k < 3; ++k)
     if (q[k] > 0) atomicAdd(o + k, (unsigned long long)llrint(q[k]));
@@ -1257,7 +1338,7 @@ __global__ __launch_bounds__(kBlock) void k_mlt_step(Scene sc, const MltDesc *__
     C.Y = chain_store(B, d, 1 - st.xbuf, c);
     C.T = chain_store(B, d, 2, c);
     C.xlen = st.xlen;
-    C.xrad = f3(st.xrad[0], st.xrad[1], st.xrad[2]);
+    C.xrad = st.xrad;
     Seq rng(d.seed, d.job_id, c, j, kStepKind);
     const double x = rng.next();
     uint32_t which = d.mut_id[d.n_mut - 1];
@@ -1272,7 +1353,7 @@ __global__ __launch_bounds__(kBlock) void k_mlt_step(Scene sc, const MltDesc *__
       }
     }
     Sub Y{C.Y, 0};
-    float3 yrad = f3(0, 0, 0);
+    D3 yrad = d3(0, 0, 0);
     double alpha;
     if (which == 0) alpha = mutate_bidir(sc, cx, C, Y, yrad, d.max_len, rng);
     else if (which == 1) alpha = mutate_lens(sc, cx, C, Y, yrad, sr, rng);
@@ -1280,15 +1361,15 @@ __global__ __launch_bounds__(kBlock) void k_mlt_step(Scene sc, const MltDesc *__
     alpha = ref_min(1.0, alpha);
     if (alpha == 0 || Y.len < 2 || Y.len > d.max_len) {
       Y.len = 0;
-      yrad = f3(0, 0, 0);
+      yrad = d3(0, 0, 0);
       alpha = 0;
     }
-    float3 tval;
+    D3 tval;
     init_sample(cx, Y.s, Y.len, yrad, d.b, st.tent_bin, tval);
     ++st.visits;
     if (d.max_rejections == 0 || st.visits < d.max_rejections) {
       if (1 - alpha > 0) {
-        add_sample(B, d.b, st.cur_bin, f3(st.cur_val[0], st.cur_val[1], st.cur_val[2]), 1 - alpha);
+        add_sample(B, d.b, st.cur_bin, st.cur_val, 1 - alpha);
         ++n_samples;
       }
       if (alpha > 0) {
@@ -1302,13 +1383,9 @@ __global__ __launch_bounds__(kBlock) void k_mlt_step(Scene sc, const MltDesc *__
       st.visits = 0;
       st.xbuf = 1 - st.xbuf;
       st.xlen = Y.len;
-      st.xrad[0] = yrad.x;
-      st.xrad[1] = yrad.y;
-      st.xrad[2] = yrad.z;
+      st.xrad = yrad;
       st.cur_bin = st.tent_bin;
-      st.cur_val[0] = tval.x;
-      st.cur_val[1] = tval.y;
-      st.cur_val[2] = tval.z;
+      st.cur_val = tval;
       ++n_acc;
     }
     if (rec && j < rec_steps) {

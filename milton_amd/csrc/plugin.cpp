@@ -153,8 +153,29 @@ int PathTracer::render() {
   job.seed = (uint32_t)getValue("seed", 1);
   job.direct_samples = (uint32_t)getValue("noDirectSamples", 1);
   job.direct_off = getValue("efficientDirect", 1) == 0 ? 1u : 0u;   // PathTracer.cpp:23
-  // 1: the loader's bidirectionalPathTracer (MiltonJSONSceneLoader.cpp:720-721)
+  // 1: the loader's bidirectionalPathTracer (MiltonJSONSceneLoader.cpp:720-721);
+  // "mlt" 1: its MLTRenderer (:731-738)
   job.renderer = getValue("bidirectional", 0) != 0 ? MPT_RENDERER_BIDIR : MPT_RENDERER_PATH;
+  if (getValue("mlt", 0) != 0) {
+    // MLTRenderer::init / render (renderers/mlt/MLTRenderer.cpp:52-91),
+    // MLTAggregatePathMutation::init (MLTAggregatePathMutation.cpp:41-61),
+    // RenderOutput::init (renderers/RenderOutput.cpp:63): the reference's
+    // keys and defaults; a chain per render thread; the reference runs
+    // until stopped, so the mutations default to noSuperSamples per pixel
+    job.renderer = MPT_RENDERER_MLT;
+    const double threads = getValue("noRenderThreads", 1);
+    job.mlt_chains = (uint32_t)std::max(1.0, threads);
+    const double seeds = getValue("mltNoInitialPaths", 10000);
+    job.mlt_seed_paths = (uint32_t)std::max(1.0, seeds);
+    job.mlt_max_depth = (uint32_t)getValue("mltMaxDepth", 10);
+    job.mlt_max_rejections = (uint32_t)getValue("mltMaxConsequtiveRejections", 500);
+    job.mlt_flags = getValue("mltFilterProposed", 1) != 0 ? 0u : 1u;
+    job.mlt_probs[0] = (float)getValue("mltBidirPathMutationProb", 1.0);
+    job.mlt_probs[1] = (float)getValue("mltLensSubpathMutationProb", 1.0);
+    job.mlt_probs[2] = (float)getValue("mltPerturbationPathMutationProb", 1.0);
+    const double per_chain = (double)job.spp * job.width * job.height / job.mlt_chains;
+    job.mlt_mutations = (uint32_t)getValue("mltMutations", std::max(1.0, std::min(32767.0, per_chain)));
+  }
   film_.assign(3ull * job.width * job.height, 0.0f);
   return mpt_render(scene_, &job, film_.data(), &stats_);
 }

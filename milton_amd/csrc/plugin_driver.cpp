@@ -83,6 +83,45 @@ int main(int argc, char **argv) {
   const bool same = std::memcmp(film.data(), ref.data(), 4 * ref.size()) == 0;
   std::printf("plugin render: mean %.6f, paths %llu, identical to C ABI: %s\n", mean,
               (unsigned long long)pt->getStats().paths, same ? "yes" : "no");
+  // Metropolis light transport through the plugin's "mlt" key, against the
+  // same job through the C ABI
+  bool mlt_same = false;
+  {
+    milton_amd::PathTracer *mp = static_cast<milton_amd::PathTracer *>(pnew());
+    mp->setScene(desc);
+    mp->setCamera(cam);
+    mp->setOutput(32, 24);
+    mp->setValue("mlt", 1);
+    mp->setValue("noRenderThreads", 64);
+    mp->setValue("mltMutations", 16);
+    mp->setValue("mltNoInitialPaths", 500);
+    mp->setValue("seed", 7);
+    int mrc = mp->init();
+    if (!mrc) mrc = mp->render();
+    mpt_job mj{};
+    mj.camera = cam;
+    mj.width = 32;
+    mj.height = 24;
+    mj.spp = 4;
+    mj.max_depth = 8;
+    mj.seed = 7;
+    mj.renderer = MPT_RENDERER_MLT;
+    mj.mlt_chains = 64;
+    mj.mlt_mutations = 16;
+    mj.mlt_seed_paths = 500;
+    mj.mlt_max_depth = 10;
+    mj.mlt_max_rejections = 500;
+    mj.mlt_probs[0] = mj.mlt_probs[1] = mj.mlt_probs[2] = 1.0f;
+    std::vector<float> mref(3 * 32 * 24);
+    if (!mrc && !mpt_render(scene, &mj, mref.data(), nullptr)) {
+      const std::vector<float> mf = mp->getOutput();
+      mlt_same = mf.size() == mref.size() && std::memcmp(mf.data(), mref.data(), 4 * mref.size()) == 0;
+    } else {
+      std::fprintf(stderr, "plugin MLT render failed (%d): %s\n", mrc, mp->lastError().c_str());
+    }
+    std::printf("MLT identical to C ABI: %s\n", mlt_same ? "yes" : "no");
+    pdel(mp);
+  }
   // a scene description with a missing array is rejected at init
   milton_amd::PathTracer *bad = static_cast<milton_amd::PathTracer *>(pnew());
   mpt_scene_desc broken = desc;
@@ -96,5 +135,5 @@ int main(int argc, char **argv) {
   mpt_close(ctx);
   pdel(pt);
   mpt_gen_destroy(gen);
-  return same && rejected && std::isfinite(mean) && mean > 0 ? 0 : 1;
+  return same && mlt_same && rejected && std::isfinite(mean) && mean > 0 ? 0 : 1;
 }

@@ -32,10 +32,16 @@
 //                      tonemap (:783; "default" = Reinhard), type and filter
 //                      (box, triangle, gaussian, mitchell, lanczosSinc)
 //   renderer           type (bidirectionalPathTracer / bidirPathTracer select
-//                      the bidirectional renderer, :720-721; any other type
+//                      the bidirectional renderer, :720-721; mlt / MLT
+//                      Metropolis light transport, :731-738; any other type
 //                      renders with the path tracer), noSuperSamples (spp),
 //                      noDirectSamples, seed, maxDepth, efficientDirect
-//                      (maxDepth: a cap the reference does not have; default 8)
+//                      (maxDepth: a cap the reference does not have; default
+//                      8); for mlt noRenderThreads (chains), mltNoInitialPaths,
+//                      mltMaxDepth, mltMaxConsequtiveRejections, the three
+//                      mlt*MutationProb, mltFilterProposed, and mltMutations
+//                      per chain (the reference runs until stopped; default
+//                      noSuperSamples x pixels / chains, at most 32767)
 //   point              with an "omni" emitter: a point light (PointShape,
 //                      OmniEmitter)
 // Anything else that would change the image (cone, cylinder, blob,
@@ -300,6 +306,12 @@ struct Settings {
   uint32_t width = 480, height = 480, spp = 1, max_depth = 8, seed = 1, direct = 1;
   uint32_t direct_off = 0;   // the renderer's efficientDirect = false
   uint32_t renderer = MPT_RENDERER_PATH;
+  // Metropolis light transport (MLTRenderer::init / render, renderers/mlt/
+  // MLTRenderer.cpp:52-91; MLTAggregatePathMutation.cpp:41-61; RenderOutput::
+  // init, renderers/RenderOutput.cpp:63): the reference's defaults
+  uint32_t mlt_chains = 1, mlt_mutations = 0, mlt_seed_paths = 10000, mlt_max_depth = 10;
+  uint32_t mlt_max_rejections = 500, mlt_flags = 0;
+  float mlt_probs[3] = {1, 1, 1};
   uint32_t tonemap = MPT_TONEMAP_REINHARD;   // RenderOutput::init's "default"
   uint32_t filter = 0, filter_support = 0;    // mpt_job.filter: 0 pixel bin, else 1 + MPT_FILTER_*
   float filter_params[2] = {0, 0};
@@ -372,6 +384,23 @@ class Loader {
     const JVal *type = v.find("type");
     if (type && type->kind == JVal::STR && (type->str == "bidirectionalPathTracer" || type->str == "bidirPathTracer"))
       set_.renderer = MPT_RENDERER_BIDIR;
+    if (type && type->kind == JVal::STR && (type->str == "mlt" || type->str == "MLT")) {
+      set_.renderer = MPT_RENDERER_MLT;
+      if (num(v.find("noRenderThreads"), x)) set_.mlt_chains = (uint32_t)std::max(1.0, x);
+      if (num(v.find("mltNoInitialPaths"), x)) set_.mlt_seed_paths = (uint32_t)std::max(1.0, x);
+      if (num(v.find("mltMaxDepth"), x)) set_.mlt_max_depth = (uint32_t)std::max(0.0, x);
+      if (num(v.find("mltMaxConsequtiveRejections"), x)) set_.mlt_max_rejections = (uint32_t)std::max(0.0, x);
+      if (num(v.find("mltMutations"), x)) set_.mlt_mutations = (uint32_t)std::max(1.0, x);
+      if (num(v.find("mltBidirPathMutationProb"), x)) set_.mlt_probs[0] = (float)x;
+      if (num(v.find("mltLensSubpathMutationProb"), x)) set_.mlt_probs[1] = (float)x;
+      if (num(v.find("mltPerturbationPathMutationProb"), x)) set_.mlt_probs[2] = (float)x;
+      const JVal *fp = v.find("mltFilterProposed");
+      if (fp) {
+        if (fp->kind == JVal::BOOL) set_.mlt_flags = fp->b ? 0u : 1u;
+        else if (fp->kind == JVal::NUM) set_.mlt_flags = fp->num == 0 ? 1u : 0u;
+        else return bad("renderer.mltFilterProposed must be a boolean");
+      }
+    }
     // PathTracer::init (renderers/renderers/PathTracer.cpp:22-26)
     const JVal *ed = v.find("efficientDirect");
     if (ed) {
@@ -1192,6 +1221,17 @@ int mpt_scenefile_job(const mpt_scenefile *sf, mpt_job *job) {
   job->direct_samples = s.direct;
   job->direct_off = s.direct_off;
   job->renderer = s.renderer;
+  job->mlt_chains = s.mlt_chains;
+  job->mlt_seed_paths = s.mlt_seed_paths;
+  job->mlt_max_depth = s.mlt_max_depth;
+  job->mlt_max_rejections = s.mlt_max_rejections;
+  job->mlt_flags = s.mlt_flags;
+  for (int k = 0; k < 3; ++k) job->mlt_probs[k] = s.mlt_probs[k];
+  job->mlt_mutations = s.mlt_mutations;
+  if (!job->mlt_mutations) {
+    const double per_chain = (double)s.spp * s.width * s.height / std::max(1u, s.mlt_chains);
+    job->mlt_mutations = (uint32_t)std::max(1.0, std::min(32767.0, per_chain));
+  }
   job->filter = s.filter;
   job->filter_support = s.filter_support;
   job->filter_params[0] = s.filter_params[0];

@@ -365,6 +365,11 @@ int mpt_submit(mpt_server *sv, mpt_scene *scene, const mpt_job *job, uint64_t *h
   int rc = check_job(sv->ctx, job);
   if (!rc) rc = check_scene_job(sv->ctx, scene, job);
   if (rc) return rc;
+  // a Metropolis job's chains splat over the whole film, so it cannot share
+  // batches of pixel ranges with other jobs
+  if (job->renderer == MPT_RENDERER_MLT)
+    return fail(sv->ctx, MPT_E_INVALID,
+                "Metropolis light transport jobs render through mpt_render / mpt_render_film, not the server");
   std::unique_ptr<Job> j(new (std::nothrow) Job());
   if (!j) return MPT_E_NOMEM;
   j->scene = scene;

@@ -18,7 +18,7 @@ def _job(a, w, h, seed=3, **mlt):
     return m.make_job(a.camera, w, h, 1, 1, seed=seed, renderer=MLT, mlt=mlt)
 
 
-def _compare_traces(ctx, a, job, steps, what, min_chains):
+def _compare_traces(ctx, a, job, steps, what, min_chains, b_rtol=1e-3):
     """Device and oracle records of the first `steps` mutations of every
     chain; returns the share of chains whose records all agree."""
     sc = m.Scene(ctx, a)
@@ -26,7 +26,7 @@ def _compare_traces(ctx, a, job, steps, what, min_chains):
     orc = po.OracleScene(a)
     sd = orc.mlt_seeds(job)
     ref = orc.mlt_trace(job, 0, job.mlt_chains, steps)
-    assert n_dev == len(sd["w"]) and abs(b_dev - sd["b"]) <= 1e-3 * sd["b"], (what, b_dev, sd["b"])
+    assert n_dev == len(sd["w"]) and abs(b_dev - sd["b"]) <= b_rtol * sd["b"], (what, b_dev, sd["b"])
     same = ((dev[..., 0] == ref[..., 0]) & (dev[..., 2] == ref[..., 2]) & (dev[..., 3] == ref[..., 3]) &
             (dev[..., 4] == ref[..., 4]) & (dev[..., 5] == ref[..., 5]) &
             (np.abs(dev[..., 1] - ref[..., 1]) <= 1e-3 * np.maximum(1e-3, np.abs(ref[..., 1]))))
@@ -74,3 +74,66 @@ def test_room_film_matches_oracle_and_is_deterministic(ctx):
           flush=True)
     assert abs(st["mlt_samples"] - ost["samples"]) <= 0.02 * ost["samples"]
     assert np.allclose(film.mean(axis=(0, 1)), orc.mean(axis=(0, 1)), rtol=0.05)
+
+
+def test_mixed_room_traces_match_oracle(ctx):
+    """Phong lobes, a glass sphere (caustic and multi-chain perturbations,
+    dielectric events that keep their mode) and a point light."""
+    from test_gpu_bdpt import _room_mixed
+    a = _room_mixed()
+    job = _job(a, 32, 24, chains=256, mutations=24, seed_paths=2000, max_depth=10)
+    _compare_traces(ctx, a, job, 24, "room, Phong, glass, point light", 0.9)
+
+
+def test_terrain_traces_match_oracle(ctx):
+    """The terrain's Phong (exponent 50) and glass.  A few seed splits there
+    weigh about 1e143: the pdfs of vertices whose event lobe is the specular
+    one (ModifiedPhongBSDF::getPd by event index) are cosines near 1e-5 to the
+    50th power, which FP32 positions change by tens of percent, and b is the
+    weights' mean; so b agrees to 50% while the chains agree step by step."""
+    a = m.generate(m.MPT_SCENE_TERRAIN, 256, 1)
+    job = _job(a, 32, 24, chains=256, mutations=16, seed_paths=1000, max_depth=8)
+    _compare_traces(ctx, a, job, 16, "terrain 256^2", 0.9, b_rtol=0.5)
+
+
+def test_chains_are_independent_of_each_other(ctx):
+    """A chain's records depend only on its index: the first 64 chains of a
+    256-chain job step exactly as a 64-chain job's, and runs repeat."""
+    a = m.generate(m.MPT_SCENE_ROOM)
+    sc = m.Scene(ctx, a)
+    big, _, _ = sc.mlt_trace(_job(a, 32, 24, chains=256, mutations=12, seed_paths=1000), 12)
+    small, _, _ = sc.mlt_trace(_job(a, 32, 24, chains=64, mutations=12, seed_paths=1000), 12)
+    again, _, _ = sc.mlt_trace(_job(a, 32, 24, chains=64, mutations=12, seed_paths=1000), 12)
+    assert np.array_equal(big[:64], small) and np.array_equal(small, again)
+
+
+def test_refused_jobs(ctx):
+    room = m.generate(m.MPT_SCENE_ROOM)
+    sr = m.Scene(ctx, room)
+    soup = m.generate(m.MPT_SCENE_SOUP, 1000, 1)   # lit by the environment only
+    with pytest.raises(m.MptError, match="light"):
+        m.Scene(ctx, soup).render(_job(soup, 8, 8, chains=4, mutations=4, seed_paths=16))
+    lens = dict(room.camera, aperture=0.05, focal_length=0.05, focal_distance=2.0)
+    with pytest.raises(m.MptError, match="pinhole"):
+        sr.render(m.make_job(lens, 8, 8, 1, 1, renderer=MLT, mlt=dict(chains=4, mutations=4)))
+    for bad, what in ((dict(max_depth=17), "mlt_max_depth"), (dict(max_depth=1), "mlt_max_depth"),
+                      (dict(chains=0), "mlt_chains"), (dict(mutations=40000), "32767")):
+        p = dict(chains=4, mutations=4, seed_paths=16)
+        p.update(bad)
+        with pytest.raises(m.MptError, match=what):
+            sr.render(_job(room, 8, 8, **p))
+    with pytest.raises(m.MptError, match="whole films"):
+        sr.render_paths(_job(room, 8, 8, chains=4, mutations=4, seed_paths=16), 0, 8)
+
+
+def test_server_refuses_metropolis_jobs(ctx):
+    """A Metropolis job's chains splat over the whole film: the server,
+    which batches pixel ranges of many jobs, refuses it."""
+    room = m.generate(m.MPT_SCENE_ROOM)
+    job = _job(room, 16, 12, chains=64, mutations=32, seed_paths=500)
+    srv = m.Server(ctx)
+    try:
+        with pytest.raises(m.MptError, match="server"):
+            srv.submit(m.Scene(ctx, room), job)
+    finally:
+        srv.close()

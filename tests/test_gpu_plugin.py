@@ -17,6 +17,7 @@ def test_plugin_render_matches_c_abi():
     print(out.stdout, out.stderr)
     assert out.returncode == 0
     assert "identical to C ABI: yes" in out.stdout
+    assert "MLT identical to C ABI: yes" in out.stdout
     assert "missing array rejected: yes" in out.stdout
 
 

@@ -166,6 +166,28 @@ def test_renderer_type(kind, want):
     assert job.renderer == want
 
 
+def test_mlt_renderer_keys():
+    """MiltonJSONSceneLoader.cpp:731-738 and MLTRenderer::init / render
+    (renderers/mlt/MLTRenderer.cpp:52-91): mlt selects Metropolis light
+    transport with the reference's defaults, one chain per render thread,
+    and mutations per chain = noSuperSamples x pixels / chains."""
+    _, job = m.load_scenefile(text='{ "scenefile" : { "renderer" : { "type" : "mlt", "noRenderThreads" : 8, '
+                                   '"noSuperSamples" : 16 }, "output" : { "width" : 32, "height" : 16 }, '
+                                   '"scene" : { "sphere" : { } } } }')
+    assert job.renderer == m.MPT_RENDERER_MLT
+    assert (job.mlt_chains, job.mlt_mutations, job.mlt_seed_paths, job.mlt_max_depth, job.mlt_max_rejections,
+            job.mlt_flags) == (8, 16 * 32 * 16 // 8, 10000, 10, 500, 0)
+    assert list(job.mlt_probs) == [1.0, 1.0, 1.0]
+    _, job = m.load_scenefile(text='{ "scenefile" : { "renderer" : { "type" : "MLT", "mltMaxDepth" : 7, '
+                                   '"mltNoInitialPaths" : 100, "mltMaxConsequtiveRejections" : 0, '
+                                   '"mltBidirPathMutationProb" : 0.5, "mltLensSubpathMutationProb" : 0, '
+                                   '"mltPerturbationPathMutationProb" : 2, "mltFilterProposed" : false, '
+                                   '"mltMutations" : 99 }, "scene" : { "sphere" : { } } } }')
+    assert (job.mlt_chains, job.mlt_mutations, job.mlt_seed_paths, job.mlt_max_depth, job.mlt_max_rejections,
+            job.mlt_flags) == (1, 99, 100, 7, 0, 1)
+    assert list(job.mlt_probs) == [0.5, 0.0, 2.0]
+
+
 @pytest.mark.parametrize("name, want", [("reinhard", m.MPT_TONEMAP_REINHARD), ("default", m.MPT_TONEMAP_REINHARD),
                                         ("null", m.MPT_TONEMAP_GAMMA)])
 def test_output_tonemap(name, want):
