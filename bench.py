@@ -435,6 +435,87 @@ def main_systems(args):
     print(json.dumps(out), flush=True)
 
 
+MLT_W, MLT_H, MLT_CHAINS, MLT_MUTATIONS = 256, 256, 65536, 256
+
+
+def bench_mlt(m, ctx, args, threads):
+    """Metropolis light transport (MLTRenderer, renderers/mlt/) of the room:
+    a 256^2 film, 65,536 chains of 256 mutations (16.8 M mutations), the
+    reference's defaults otherwise (10,000 seed paths, mltMaxDepth 10, all
+    three mutations, the proposal filter).  The oracle's rate on the host
+    threads comes from 8,192 chains of the same job, less its seed phase."""
+    arrays = m.generate(m.MPT_SCENE_ROOM)
+    scene = m.Scene(ctx, arrays)
+    film = m.Film(ctx, MLT_W, MLT_H)
+    job = m.make_job(arrays.camera, MLT_W, MLT_H, 1, 1, seed=1, renderer=m.MPT_RENDERER_MLT,
+                     mlt=dict(chains=MLT_CHAINS, mutations=MLT_MUTATIONS))
+    # the seed phase alone: seeds, chain starts and one mutation
+    one = m.make_job(arrays.camera, MLT_W, MLT_H, 1, 1, seed=1, renderer=m.MPT_RENDERER_MLT,
+                     mlt=dict(chains=MLT_CHAINS, mutations=1))
+    film.render(scene, job)   # warm-up
+    t = time.perf_counter()
+    st1 = film.render(scene, one)
+    seed_s = time.perf_counter() - t
+    steps = 3
+    ctx.sync()
+    t = time.perf_counter()
+    for _ in range(steps):
+        st = film.render(scene, job)
+    ctx.sync()
+    el = (time.perf_counter() - t) / steps
+    muts = MLT_CHAINS * MLT_MUTATIONS
+    out = {"renderer": "Metropolis light transport (MLTRenderer.cpp:70-235)", "scene": "room",
+           "width": MLT_W, "height": MLT_H, "chains": MLT_CHAINS, "mutations_per_chain": MLT_MUTATIONS,
+           "ms_per_job": 1000 * el, "mutations_per_s": muts / el,
+           "seed_phase_ms": 1000 * seed_s,
+           "mutations_per_s_after_seeds": muts / max(1e-9, el - seed_s),
+           "accepted": st["mlt_accepted"], "samples": st["mlt_samples"], "b": st["mlt_b"],
+           "extension_rays": st["extension_rays"], "visibility_rays": st["shadow_rays"],
+           "device_ms": st["render_ms"], "device_ms_seed_phase": st1["render_ms"]}
+    log(f"mlt room: {out}")
+    if not args.no_cpu:
+        from oracle import pyoracle as po
+        chains = 8192
+        cjob = m.make_job(arrays.camera, MLT_W, MLT_H, 1, 1, seed=1, renderer=m.MPT_RENDERER_MLT,
+                          mlt=dict(chains=chains, mutations=MLT_MUTATIONS))
+        osc = po.OracleScene(arrays, threads=threads)
+        osc.mlt_seeds(cjob, threads=threads)   # warm-up
+        t = time.perf_counter()
+        osc.mlt_seeds(cjob, threads=threads)
+        cseed = time.perf_counter() - t
+        t = time.perf_counter()
+        osc.mlt_render(cjob, threads=threads)
+        cel = time.perf_counter() - t
+        osc.close()
+        cm = chains * MLT_MUTATIONS
+        out["cpu_baseline"] = {"value": cm / max(1e-9, cel - cseed), "unit": "mutations/s", "cores": threads,
+                               "kind": "port",
+                               "sample": f"{chains} chains x {MLT_MUTATIONS} mutations of the same job "
+                                         f"({cel:.1f} s on {threads} threads, less its {cseed:.2f} s seed phase)"}
+        out["speedup_vs_cpu"] = out["mutations_per_s_after_seeds"] / out["cpu_baseline"]["value"]
+        log(f"mlt cpu baseline: {out['cpu_baseline']}")
+    film.close()
+    scene.close()
+    return out
+
+
+def main_mlt(args):
+    """--workload mlt: only the Metropolis light transport room job."""
+    import milton_amd as m
+    ctx = m.Context(0)
+    apply_engine_args(m, ctx, args)
+    threads = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    r = bench_mlt(m, ctx, args, threads)
+    out = {"metric": "mutations/s", "value": r["mutations_per_s"], "unit": "mutations/s", "n_gpus": 1,
+           "steps": 3, "warmup": 1, "ms_per_step": r["ms_per_job"], "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+           "config": {"workload": f"Metropolis light transport of the room, {MLT_W}x{MLT_H}, {MLT_CHAINS} chains x "
+                                  f"{MLT_MUTATIONS} mutations", "engine": engine_settings(ctx, m)},
+           "cpu_baseline": r.get("cpu_baseline"), "mlt": r}
+    ctx.close()
+    print(json.dumps(out), flush=True)
+
+
 def main_bidir(args):
     """--workload bidir: only the bidirectional renderer's room jobs."""
     import milton_amd as m
@@ -476,6 +557,7 @@ def main():
     ap.add_argument("--no-rmse", action="store_true", help="skip the RMSE against a 65,536-spp device reference")
     ap.add_argument("--no-systems", action="store_true", help="skip the batched and serving workloads")
     ap.add_argument("--no-bidir", action="store_true", help="skip the bidirectional renderer's room jobs")
+    ap.add_argument("--no-mlt", action="store_true", help="skip the Metropolis light transport room job")
     ap.add_argument("--batch-paths", type=int, default=0)
     ap.add_argument("--stack", type=int, default=0, help="traversal short-stack rows (8, 12, 16, 24)")
     ap.add_argument("--persistent", type=int, default=-1,
@@ -491,7 +573,7 @@ def main():
     ap.add_argument("--no-clip", action="store_true", help="kd build without triangle clipping")
     ap.add_argument("--tune", action="store_true",
                     help="autotune (kd leaf size and engine settings) on a sample of the job before timing")
-    ap.add_argument("--workload", default="soup", choices=["soup", "batched", "serving", "bidir"],
+    ap.add_argument("--workload", default="soup", choices=["soup", "batched", "serving", "bidir", "mlt"],
                     help="soup: the headline kd-tree stress scene (plus the systems workloads unless "
                          "--no-systems); batched / serving: only the systems workloads; bidir: only the "
                          "bidirectional renderer's room jobs")
@@ -507,6 +589,9 @@ def main():
         sys.exit(2)
     if args.workload == "bidir":
         main_bidir(args)
+        return
+    if args.workload == "mlt":
+        main_mlt(args)
         return
     if args.workload != "soup":
         main_systems(args)
@@ -618,10 +703,12 @@ def main():
     else:
         out["cpu_baseline"] = None
     film.close()
+    nproc = os.cpu_count() or 1
+    threads = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else nproc
     if not args.no_bidir:
-        nproc = os.cpu_count() or 1
-        threads = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else nproc
         out["bidir"] = bench_bidir(m, ctx, args, threads)
+    if not args.no_mlt:
+        out["mlt"] = bench_mlt(m, ctx, args, threads)
     if not args.no_systems:
         out.update(systems(m, ctx, args, reuse={m.MPT_SCENE_SOUP: (arrays, scene)}))
     scene.close()

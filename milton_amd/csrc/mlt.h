@@ -13,8 +13,8 @@
 //                   by its contribution's luminance; the random-stream
 //                   position of each split's join is kept so a chain can
 //                   rebuild the split it starts from
-//   k_mlt_weights   one lane: the weights' sum in the reference's order,
-//                   b = sum / entries, the normalised running sum that
+//   k_mlt_weights   one block: the entries in the reference's order, their
+//                   sum, b = sum / entries, the normalised running sum that
 //                   Random::sampleNormalizedCDF walks
 //   k_mlt_start     per chain: its seed entry, the split rebuilt, the first
 //                   sample
@@ -1203,41 +1203,86 @@ __global__ __launch_bounds__(kBlock) void k_mlt_seed(Scene sc, const MltDesc *__
   flush_counters(B.counters, 0, 0, cx.n_ext, cx.n_vis);
 }
 
-// One lane: the weights' sum in the reference's order (the entries of path
-// N - 1 first, MLTRenderer.cpp:150), b = sum / entries, and the normalised
-// running sum Random::sampleNormalizedCDF walks.
-__global__ void k_mlt_weights(MltDesc *dd, MltBufs B) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+// The seed entries in the reference's order (the entries of path N - 1
+// first, MLTRenderer.cpp:150) compacted into cdf / order, their sum in that
+// order taken backwards as the reference does (MLTRenderer.cpp:202-205),
+// b = sum / entries, and the normalised running sum Random::
+// sampleNormalizedCDF walks.  One block: the sums are sequential (so they
+// round as the oracle's), one lane adding what the block stages in LDS.
+constexpr uint32_t kWeightsBlock = 256, kWeightsChunk = 2048;
+__global__ __launch_bounds__(kWeightsBlock) void k_mlt_weights(MltDesc *dd, MltBufs B) {
+  __shared__ uint32_t scan[kWeightsBlock];
+  __shared__ double buf[kWeightsChunk];
+  __shared__ double sum_sh;
   MltDesc &d = *dd;
-  const uint32_t N = d.seed_paths;
-  double sum = 0;
-  uint32_t n = 0;
-  for (uint32_t i = 0; i < N; ++i) {
-    const uint32_t c = B.seed_cnt[i];
-    n += c;
-    for (uint32_t j = c; j--;) sum += B.ent_w[i * kMaxEntries + j];
+  const uint32_t N = d.seed_paths, t = threadIdx.x;
+  uint32_t base = 0;
+  for (uint32_t top = 0; top < N; top += kWeightsBlock) {
+    const uint32_t k = top + t;   // the k-th path in the reference's order
+    const uint32_t i = k < N ? N - 1 - k : 0;
+    const uint32_t c = k < N ? B.seed_cnt[i] : 0u;
+    scan[t] = c;
+    __syncthreads();
+    for (uint32_t o = 1; o < kWeightsBlock; o <<= 1) {
+      const uint32_t v = t >= o ? scan[t - o] : 0u;
+      __syncthreads();
+      scan[t] += v;
+      __syncthreads();
+    }
+    const uint32_t off = base + scan[t] - c;
+    for (uint32_t j = 0; j < c; ++j) {
+      B.cdf[off + j] = B.ent_w[i * kMaxEntries + j];
+      B.order[off + j] = i * kMaxEntries + j;
+    }
+    base += scan[kWeightsBlock - 1];
+    __syncthreads();
   }
-  d.n_entries = n;
-  d.sum = sum;
-  d.has_seeds = n > 0 && sum > 0 ? 1u : 0u;
-  if (!d.has_seeds) return;
-  d.b = sum / (double)(n + d.n_prev);
+  const uint32_t n = base;
+  double sum = 0;
+  for (uint32_t hi = n; hi > 0;) {
+    const uint32_t lo = hi > kWeightsChunk ? hi - kWeightsChunk : 0u;
+    for (uint32_t g = lo + t; g < hi; g += kWeightsBlock) buf[g - lo] = B.cdf[g];
+    __syncthreads();
+    if (t == 0)
+      for (uint32_t g = hi; g-- > lo;) sum += buf[g - lo];
+    __syncthreads();
+    hi = lo;
+  }
+  if (t == 0) sum_sh = sum;
+  __syncthreads();
+  sum = sum_sh;
+  const bool found = n > 0 && sum > 0;
+  if (t == 0) {
+    d.n_entries = n;
+    d.sum = sum;
+    d.has_seeds = found ? 1u : 0u;
+    if (found) d.b = sum / (double)(n + d.n_prev);
+  }
+  if (!found) return;
   double acc = 0;
-  uint32_t g = 0;
-  d.first_pos = ~0u;
-  d.last_pos = 0;
-  for (uint32_t i = N; i--;) {
-    const uint32_t c = B.seed_cnt[i];
-    for (uint32_t j = 0; j < c; ++j, ++g) {
-      const double w = B.ent_w[i * kMaxEntries + j] / sum;
-      acc += w;
-      B.cdf[g] = acc;
-      B.order[g] = i * kMaxEntries + j;
-      if (w > 0) {
-        if (d.first_pos == ~0u) d.first_pos = g;
-        d.last_pos = g;
+  uint32_t first = ~0u, last = 0;
+  for (uint32_t lo = 0; lo < n; lo += kWeightsChunk) {
+    const uint32_t hi = min(n, lo + kWeightsChunk);
+    for (uint32_t g = lo + t; g < hi; g += kWeightsBlock) buf[g - lo] = B.cdf[g] / sum;
+    __syncthreads();
+    if (t == 0) {
+      for (uint32_t g = lo; g < hi; ++g) {
+        const double w = buf[g - lo];
+        acc += w;
+        buf[g - lo] = acc;
+        if (w > 0) {
+          if (first == ~0u) first = g;
+          last = g;
+        }
       }
     }
+    __syncthreads();
+    for (uint32_t g = lo + t; g < hi; g += kWeightsBlock) B.cdf[g] = buf[g - lo];
+    __syncthreads();
+  }
+  if (t == 0) {
+    d.first_pos = first;
+    d.last_pos = last;
   }
 }
 

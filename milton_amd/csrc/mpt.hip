@@ -2002,7 +2002,7 @@ int render_mlt(mpt_scene *s, const mpt_job *job, float4 *film, mpt_render_stats 
     MPT_HIP(ctx, hipMemcpyAsync(dd, &d, sizeof(d), hipMemcpyHostToDevice, st));
     mlt::k_mlt_seed<<<(N + mlt::kBlock - 1) / mlt::kBlock, mlt::kBlock, 0, st>>>(s->dev, dd, B);
     MPT_LAUNCHED(ctx, "k_mlt_seed");
-    mlt::k_mlt_weights<<<1, 64, 0, st>>>(dd, B);
+    mlt::k_mlt_weights<<<1, mlt::kWeightsBlock, 0, st>>>(dd, B);
     MPT_LAUNCHED(ctx, "k_mlt_weights");
     MPT_HIP(ctx, hipMemcpyAsync(&d, dd, sizeof(d), hipMemcpyDeviceToHost, st));
     MPT_HIP(ctx, hipStreamSynchronize(st));
