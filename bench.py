@@ -447,11 +447,12 @@ def bench_mlt(m, ctx, args, threads):
     arrays = m.generate(m.MPT_SCENE_ROOM)
     scene = m.Scene(ctx, arrays)
     film = m.Film(ctx, MLT_W, MLT_H)
+    nch = args.mlt_chains or MLT_CHAINS
     job = m.make_job(arrays.camera, MLT_W, MLT_H, 1, 1, seed=1, renderer=m.MPT_RENDERER_MLT,
-                     mlt=dict(chains=MLT_CHAINS, mutations=MLT_MUTATIONS))
+                     mlt=dict(chains=nch, mutations=MLT_MUTATIONS))
     # the seed phase alone: seeds, chain starts and one mutation
     one = m.make_job(arrays.camera, MLT_W, MLT_H, 1, 1, seed=1, renderer=m.MPT_RENDERER_MLT,
-                     mlt=dict(chains=MLT_CHAINS, mutations=1))
+                     mlt=dict(chains=nch, mutations=1))
     film.render(scene, job)   # warm-up
     t = time.perf_counter()
     st1 = film.render(scene, one)
@@ -463,9 +464,9 @@ def bench_mlt(m, ctx, args, threads):
         st = film.render(scene, job)
     ctx.sync()
     el = (time.perf_counter() - t) / steps
-    muts = MLT_CHAINS * MLT_MUTATIONS
+    muts = nch * MLT_MUTATIONS
     out = {"renderer": "Metropolis light transport (MLTRenderer.cpp:70-235)", "scene": "room",
-           "width": MLT_W, "height": MLT_H, "chains": MLT_CHAINS, "mutations_per_chain": MLT_MUTATIONS,
+           "width": MLT_W, "height": MLT_H, "chains": nch, "mutations_per_chain": MLT_MUTATIONS,
            "ms_per_job": 1000 * el, "mutations_per_s": muts / el,
            "seed_phase_ms": 1000 * seed_s,
            "mutations_per_s_after_seeds": muts / max(1e-9, el - seed_s),
@@ -558,6 +559,7 @@ def main():
     ap.add_argument("--no-systems", action="store_true", help="skip the batched and serving workloads")
     ap.add_argument("--no-bidir", action="store_true", help="skip the bidirectional renderer's room jobs")
     ap.add_argument("--no-mlt", action="store_true", help="skip the Metropolis light transport room job")
+    ap.add_argument("--mlt-chains", type=int, default=0, help="Markov chains of the Metropolis job (0: the default)")
     ap.add_argument("--batch-paths", type=int, default=0)
     ap.add_argument("--stack", type=int, default=0, help="traversal short-stack rows (8, 12, 16, 24)")
     ap.add_argument("--persistent", type=int, default=-1,

@@ -46,6 +46,10 @@ constexpr uint32_t kCap = kMaxLen + 1;   // vertex records per path buffer: a su
 // splits (k, s) of one seed path of kMaxLen vertices: sum over k = 2..16 of k + 1
 constexpr uint32_t kMaxEntries = (kMaxLen + 1) * (kMaxLen + 2) / 2 - 3;
 constexpr uint32_t kSlots = 12;      // float4 per vertex record
+constexpr uint32_t kWave = 64;       // lanes that share a run of path records
+constexpr uint64_t kRun = (uint64_t)kCap * kSlots * kWave;   // float4 of one wave's path buffer
+// runs of path records for n lanes
+__host__ __device__ constexpr uint64_t wave_runs(uint64_t n) { return (n + kWave - 1) / kWave; }
 constexpr uint32_t kBlock = 128;     // threads per block of the chain kernels
 constexpr uint32_t kStackRows = 8;   // traversal ring rows per lane
 constexpr double kFix = 16777216.0;  // film sums: units of b / 2^24
@@ -86,8 +90,10 @@ struct MVtx {
 };
 
 // A chain's (or seed lane's) path buffer: slot k of vertex i at
-// base[(i * kSlots + k) * stride], lanes interleaved so a wave's loads of
-// one slot coalesce.
+// base[(i * kSlots + k) * stride].  Buffers are laid out by wave: the 64
+// lanes of a wave interleave (stride 64), so a wave's load of one slot is
+// 1 KiB contiguous, and each wave's records are one contiguous run
+// (kCap * kSlots * 1 KiB), which keeps a wave's accesses within a few pages.
 struct Store {
   float4 *base;
   uint32_t stride;
@@ -145,6 +151,11 @@ struct Store {
     for (uint32_t k = 0; k < kSlots; ++k) at(dst, k) = src.at(i, k);
   }
 };
+
+// Lane `lane`'s records in a buffer of runs of 64 lanes.
+__device__ __forceinline__ Store lane_store(float4 *buf, uint32_t lane) {
+  return Store{buf + (uint64_t)(lane / kWave) * kRun + lane % kWave, kWave};
+}
 
 // Sequential uniforms of one stream (the oracle's Seq).
 struct Seq {
@@ -1117,7 +1128,7 @@ struct ChainState {
 };
 
 struct MltBufs {
-  // seed paths: vertex records (stride seed_paths), length and attempt per
+  // seed paths: vertex records (runs of 64 paths), length and attempt per
   // path; per entry (path-major, kMaxEntries per path): weight, stream
   // position of its join, (k, s)
   float4 *seed_v;
@@ -1131,7 +1142,7 @@ struct MltBufs {
   // order position's (path, entry)
   double *cdf;
   uint32_t *order;
-  // chains: three path buffers (stride chains), states, stratification
+  // chains: three path buffers (runs of 64 chains each), states, stratification
   float4 *chain_v;
   ChainState *st;
   uint16_t *strat;
@@ -1169,8 +1180,8 @@ __global__ __launch_bounds__(kBlock) void k_mlt_seed(Scene sc, const MltDesc *__
   cx.pA_film = (1.0 / d.W) * (1.0 / d.H);
   cx.stk = stack + threadIdx.x;
   cx.n_ext = cx.n_vis = 0;
-  Sub G{Store{B.seed_v + i, N}, 0};
-  Sub T{Store{B.seed_tmp + i, N}, 0};
+  Sub G{lane_store(B.seed_v, i), 0};
+  Sub T{lane_store(B.seed_tmp, i), 0};
   const uint32_t idx = d.round * N + i;
   uint32_t cnt = 0;
   for (uint32_t a = 0; a < kAttempts; ++a) {
@@ -1186,7 +1197,7 @@ __global__ __launch_bounds__(kBlock) void k_mlt_seed(Scene sc, const MltDesc *__
         const uint32_t e = i * kMaxEntries + cnt;
         B.ent_pos[e] = rng.n;
         B.ent_ks[e] = (k << 8) | s;
-        Sub P{Store{B.seed_tmp2 + i, N}, s};
+        Sub P{lane_store(B.seed_tmp2, i), s};
         for (uint32_t j = 0; j < s; ++j) P.s.copy(j, G.s, j);
         Sub R{T.s, t};
         for (uint32_t j = 0; j < t; ++j) R.s.copy(j, G.s, L - 1 - j);
@@ -1300,7 +1311,7 @@ __device__ __forceinline__ Ctx chain_ctx(const MltDesc &d, uint2 *stk) {
 }
 
 __device__ __forceinline__ Store chain_store(const MltBufs &B, const MltDesc &d, uint32_t which, uint32_t c) {
-  return Store{B.chain_v + (uint64_t)which * kCap * kSlots * d.chains + c, d.chains};
+  return Store{B.chain_v + ((uint64_t)which * wave_runs(d.chains) + c / kWave) * kRun + c % kWave, kWave};
 }
 
 // A chain's start (MLTRenderer.cpp:100-112): its seed entry by the weights
@@ -1327,7 +1338,7 @@ __global__ __launch_bounds__(kBlock) void k_mlt_start(Scene sc, const MltDesc *_
   const uint32_t i = e / kMaxEntries;
   const uint32_t ks = B.ent_ks[e], k = ks >> 8, s = ks & 255u, t = k - s;
   const uint32_t N = d.seed_paths;
-  const Store G{B.seed_v + i, N};
+  const Store G = lane_store(B.seed_v, i);
   const uint32_t L = B.seed_len[i];
   Sub X{chain_store(B, d, 0, c), s};
   for (uint32_t j = 0; j < s; ++j) X.s.copy(j, G, j);

@@ -1949,10 +1949,11 @@ int render_mlt(mpt_scene *s, const mpt_job *job, float4 *film, mpt_render_stats 
     for (uint32_t i = d.n_mut; i--;) sum += d.mut_w[i];
     for (uint32_t i = d.n_mut; i--;) d.mut_w[i] /= sum;
   }
-  const uint64_t rec_len = (uint64_t)mlt::kCap * mlt::kSlots;
-  MPT_HIP(ctx, w.seed_v.reserve(rec_len * N));
-  MPT_HIP(ctx, w.seed_tmp.reserve(rec_len * N));
-  MPT_HIP(ctx, w.seed_tmp2.reserve(rec_len * N));
+  // path records in runs of 64 lanes (mlt::Store)
+  const uint64_t seed_recs = mlt::wave_runs(N) * mlt::kRun;
+  MPT_HIP(ctx, w.seed_v.reserve(seed_recs));
+  MPT_HIP(ctx, w.seed_tmp.reserve(seed_recs));
+  MPT_HIP(ctx, w.seed_tmp2.reserve(seed_recs));
   MPT_HIP(ctx, w.seed_len.reserve(N));
   MPT_HIP(ctx, w.seed_att.reserve(N));
   MPT_HIP(ctx, w.seed_cnt.reserve(N));
@@ -1961,7 +1962,7 @@ int render_mlt(mpt_scene *s, const mpt_job *job, float4 *film, mpt_render_stats 
   MPT_HIP(ctx, w.ent_ks.reserve((uint64_t)N * mlt::kMaxEntries));
   MPT_HIP(ctx, w.cdf.reserve((uint64_t)N * mlt::kMaxEntries));
   MPT_HIP(ctx, w.order.reserve((uint64_t)N * mlt::kMaxEntries));
-  MPT_HIP(ctx, w.chain_v.reserve(3 * rec_len * C));
+  MPT_HIP(ctx, w.chain_v.reserve(3 * mlt::wave_runs(C) * mlt::kRun));
   MPT_HIP(ctx, w.states.reserve(sizeof(mlt::ChainState) * (uint64_t)C));
   MPT_HIP(ctx, w.strat.reserve((uint64_t)C * P));
   MPT_HIP(ctx, w.num.reserve(3ull * P));
