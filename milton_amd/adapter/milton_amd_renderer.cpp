@@ -314,10 +314,13 @@ class AmdPathTracer : public PointSampleRenderer {
     ok_ = false;
     static const char *kKeys[] = {"noSuperSamples", "maxDepth", "seed", "noDirectSamples", "kdMinPrimitives",
                                   "kdMaxDepth", "kdCostTraversal", "kdEmptyBias", "kdNoThreads", "device",
-                                  "batchPaths"};
+                                  "batchPaths", "bidirectional", "mlt", "noRenderThreads", "mltNoInitialPaths",
+                                  "mltMaxDepth", "mltMaxConsequtiveRejections", "mltBidirPathMutationProb",
+                                  "mltLensSubpathMutationProb", "mltPerturbationPathMutationProb", "mltMutations"};
     for (const char *k : kKeys)
       if (contains(k)) pt_->setValue(k, getValue<real_t>(k, 0));
     if (contains("efficientDirect")) pt_->setValue("efficientDirect", getValue<bool>("efficientDirect", true));
+    if (contains("mltFilterProposed")) pt_->setValue("mltFilterProposed", getValue<bool>("mltFilterProposed", true));
     if (!flatten(m_scene, flat_)) {
       std::cerr << "AmdPathTracer: " << flat_.error << std::endl;
       return;
