@@ -1565,6 +1565,10 @@ int check_job(mpt_context *ctx, const mpt_job *job) {
     if ((uint64_t)job->width * job->height < 2) return fail(ctx, MPT_E_INVALID, "the film needs two or more pixels");
     if (job->filter) return fail(ctx, MPT_E_INVALID, "Metropolis light transport bins samples (filter 0)");
     if (job->mlt_seed_paths > (1u << 22)) return fail(ctx, MPT_E_INVALID, "mlt_seed_paths above 2^22");
+    // a bin's proposal count is 32-bit and its fixed-point sums 64-bit
+    // (at most 2^33 samples of at most 14 b each in units of b / 2^24)
+    if ((uint64_t)job->mlt_chains * job->mlt_mutations >= (1ull << 32))
+      return fail(ctx, MPT_E_INVALID, "mlt_chains x mlt_mutations at or above 2^32");
     for (int k = 0; k < 3; ++k)
       if (!std::isfinite(job->mlt_probs[k])) return fail(ctx, MPT_E_INVALID, "mlt_probs must be finite");
     const uint64_t strat = (uint64_t)job->mlt_chains * job->width * job->height * sizeof(uint16_t);

@@ -117,7 +117,8 @@ def test_refused_jobs(ctx):
     with pytest.raises(m.MptError, match="pinhole"):
         sr.render(m.make_job(lens, 8, 8, 1, 1, renderer=MLT, mlt=dict(chains=4, mutations=4)))
     for bad, what in ((dict(max_depth=17), "mlt_max_depth"), (dict(max_depth=1), "mlt_max_depth"),
-                      (dict(chains=0), "mlt_chains"), (dict(mutations=40000), "32767")):
+                      (dict(chains=0), "mlt_chains"), (dict(mutations=40000), "32767"),
+                      (dict(chains=1 << 18, mutations=1 << 14), "2\\^32")):
         p = dict(chains=4, mutations=4, seed_paths=16)
         p.update(bad)
         with pytest.raises(m.MptError, match=what):
