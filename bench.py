@@ -435,12 +435,13 @@ def main_systems(args):
     print(json.dumps(out), flush=True)
 
 
-MLT_W, MLT_H, MLT_CHAINS, MLT_MUTATIONS = 256, 256, 65536, 256
+# two waves of chains per SIMD of the 256 CUs, the step kernel's occupancy
+MLT_W, MLT_H, MLT_CHAINS, MLT_MUTATIONS = 256, 256, 131072, 128
 
 
 def bench_mlt(m, ctx, args, threads):
     """Metropolis light transport (MLTRenderer, renderers/mlt/) of the room:
-    a 256^2 film, 65,536 chains of 256 mutations (16.8 M mutations), the
+    a 256^2 film, 131,072 chains of 128 mutations (16.8 M mutations), the
     reference's defaults otherwise (10,000 seed paths, mltMaxDepth 10, all
     three mutations, the proposal filter).  The oracle's rate on the host
     threads comes from 8,192 chains of the same job, less its seed phase."""
@@ -448,8 +449,9 @@ def bench_mlt(m, ctx, args, threads):
     scene = m.Scene(ctx, arrays)
     film = m.Film(ctx, MLT_W, MLT_H)
     nch = args.mlt_chains or MLT_CHAINS
+    nmut = args.mlt_mutations or MLT_MUTATIONS
     job = m.make_job(arrays.camera, MLT_W, MLT_H, 1, 1, seed=1, renderer=m.MPT_RENDERER_MLT,
-                     mlt=dict(chains=nch, mutations=MLT_MUTATIONS))
+                     mlt=dict(chains=nch, mutations=nmut))
     # the seed phase alone: seeds, chain starts and one mutation
     one = m.make_job(arrays.camera, MLT_W, MLT_H, 1, 1, seed=1, renderer=m.MPT_RENDERER_MLT,
                      mlt=dict(chains=nch, mutations=1))
@@ -464,9 +466,9 @@ def bench_mlt(m, ctx, args, threads):
         st = film.render(scene, job)
     ctx.sync()
     el = (time.perf_counter() - t) / steps
-    muts = nch * MLT_MUTATIONS
+    muts = nch * nmut
     out = {"renderer": "Metropolis light transport (MLTRenderer.cpp:70-235)", "scene": "room",
-           "width": MLT_W, "height": MLT_H, "chains": nch, "mutations_per_chain": MLT_MUTATIONS,
+           "width": MLT_W, "height": MLT_H, "chains": nch, "mutations_per_chain": nmut,
            "ms_per_job": 1000 * el, "mutations_per_s": muts / el,
            "seed_phase_ms": 1000 * seed_s,
            "mutations_per_s_after_seeds": muts / max(1e-9, el - seed_s),
@@ -478,7 +480,7 @@ def bench_mlt(m, ctx, args, threads):
         from oracle import pyoracle as po
         chains = 8192
         cjob = m.make_job(arrays.camera, MLT_W, MLT_H, 1, 1, seed=1, renderer=m.MPT_RENDERER_MLT,
-                          mlt=dict(chains=chains, mutations=MLT_MUTATIONS))
+                          mlt=dict(chains=chains, mutations=nmut))
         osc = po.OracleScene(arrays, threads=threads)
         osc.mlt_seeds(cjob, threads=threads)   # warm-up
         t = time.perf_counter()
@@ -488,10 +490,10 @@ def bench_mlt(m, ctx, args, threads):
         osc.mlt_render(cjob, threads=threads)
         cel = time.perf_counter() - t
         osc.close()
-        cm = chains * MLT_MUTATIONS
+        cm = chains * nmut
         out["cpu_baseline"] = {"value": cm / max(1e-9, cel - cseed), "unit": "mutations/s", "cores": threads,
                                "kind": "port",
-                               "sample": f"{chains} chains x {MLT_MUTATIONS} mutations of the same job "
+                               "sample": f"{chains} chains x {nmut} mutations of the same job "
                                          f"({cel:.1f} s on {threads} threads, less its {cseed:.2f} s seed phase)"}
         out["speedup_vs_cpu"] = out["mutations_per_s_after_seeds"] / out["cpu_baseline"]["value"]
         log(f"mlt cpu baseline: {out['cpu_baseline']}")
@@ -510,8 +512,8 @@ def main_mlt(args):
     out = {"metric": "mutations/s", "value": r["mutations_per_s"], "unit": "mutations/s", "n_gpus": 1,
            "steps": 3, "warmup": 1, "ms_per_step": r["ms_per_job"], "higher_is_better": True, "scaling": "weak",
            "vs_baseline": None, "dtype": "f64", "data": "synthetic",
-           "config": {"workload": f"Metropolis light transport of the room, {MLT_W}x{MLT_H}, {MLT_CHAINS} chains x "
-                                  f"{MLT_MUTATIONS} mutations", "engine": engine_settings(ctx, m)},
+           "config": {"workload": f"Metropolis light transport of the room, {MLT_W}x{MLT_H}, {r['chains']} chains x "
+                                  f"{r['mutations_per_chain']} mutations", "engine": engine_settings(ctx, m)},
            "cpu_baseline": r.get("cpu_baseline"), "mlt": r}
     ctx.close()
     print(json.dumps(out), flush=True)
@@ -560,6 +562,8 @@ def main():
     ap.add_argument("--no-bidir", action="store_true", help="skip the bidirectional renderer's room jobs")
     ap.add_argument("--no-mlt", action="store_true", help="skip the Metropolis light transport room job")
     ap.add_argument("--mlt-chains", type=int, default=0, help="Markov chains of the Metropolis job (0: the default)")
+    ap.add_argument("--mlt-mutations", type=int, default=0,
+                    help="mutations per chain of the Metropolis job (0: the default)")
     ap.add_argument("--batch-paths", type=int, default=0)
     ap.add_argument("--stack", type=int, default=0, help="traversal short-stack rows (8, 12, 16, 24)")
     ap.add_argument("--persistent", type=int, default=-1,

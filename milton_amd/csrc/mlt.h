@@ -1377,9 +1377,13 @@ __device__ __forceinline__ void add_sample(const MltBufs &B, double b, uint32_t 
 
 // Mutations [j0, j1) of each chain (MLTMarkovProcess::run, :38-112).  rec
 // (optional): per chain and mutation, the parity record (mutation, alpha,
-// accepted, Y length, tentative bin, X length after).
-__global__ __launch_bounds__(kBlock) void k_mlt_step(Scene sc, const MltDesc *__restrict__ dd, MltBufs B,
-                                                    uint32_t j0, uint32_t j1, double *rec, uint32_t rec_steps) {
+// accepted, Y length, tentative bin, X length after).  The step waits on
+// dependent path-record loads and kd walks most of its cycles; held to 256
+// registers (about 60 spilled), two waves share each SIMD to hide that, which
+// pays once a job has two waves of chains per SIMD (131,072 on 256 CUs).
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2)))
+void k_mlt_step(Scene sc, const MltDesc *__restrict__ dd, MltBufs B, uint32_t j0, uint32_t j1, double *rec,
+                uint32_t rec_steps) {
   __shared__ uint2 stack[kStackRows * kBlock];
   const MltDesc &d = *dd;
   const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
