@@ -444,7 +444,7 @@ def bench_mlt(m, ctx, args, threads):
     a 256^2 film, 131,072 chains of 128 mutations (16.8 M mutations), the
     reference's defaults otherwise (10,000 seed paths, mltMaxDepth 10, all
     three mutations, the proposal filter).  The oracle's rate on the host
-    threads comes from 8,192 chains of the same job, less its seed phase."""
+    threads comes from the same job, less its seed phase."""
     arrays = m.generate(m.MPT_SCENE_ROOM)
     scene = m.Scene(ctx, arrays)
     film = m.Film(ctx, MLT_W, MLT_H)
@@ -478,7 +478,7 @@ def bench_mlt(m, ctx, args, threads):
     log(f"mlt room: {out}")
     if not args.no_cpu:
         from oracle import pyoracle as po
-        chains = 8192
+        chains = nch   # the whole job: about 2 s on 16 threads
         cjob = m.make_job(arrays.camera, MLT_W, MLT_H, 1, 1, seed=1, renderer=m.MPT_RENDERER_MLT,
                           mlt=dict(chains=chains, mutations=nmut))
         osc = po.OracleScene(arrays, threads=threads)
