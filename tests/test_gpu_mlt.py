@@ -61,6 +61,39 @@ def test_room_traces_match_oracle(ctx):
     _compare_traces(ctx, a, job, 16, "room", 0.9)
 
 
+@pytest.mark.parametrize("what,kw", [
+    ("bidirectional mutations only", dict(probs=(1.0, 0.0, 0.0))),
+    ("lens mutations only", dict(probs=(0.0, 1.0, 0.0))),
+    ("perturbations only", dict(probs=(0.0, 0.0, 1.0))),
+])
+def test_room_traces_per_setting(ctx, what, kw):
+    """Each mutation of the aggregate alone."""
+    a = m.generate(m.MPT_SCENE_ROOM)
+    job = _job(a, 32, 24, chains=256, mutations=16, seed_paths=2000, max_depth=10, **kw)
+    _compare_traces(ctx, a, job, 16, f"room, {what}", 0.9)
+
+
+@pytest.mark.parametrize("what,kw", [("no proposal filter", dict(filter_proposed=False)),
+                                     ("3 rejections", dict(max_rejections=3))])
+def test_room_film_per_setting(ctx, what, kw):
+    """mltFilterProposed off (the film is the numerators times 2 / samples)
+    and mltMaxConsequtiveRejections 3 (MLTMarkovProcess.cpp:97-118: a chain
+    that has stayed on its path that many times in a row adds no samples
+    until it moves), against the oracle's film and sample count."""
+    a = m.generate(m.MPT_SCENE_ROOM)
+    job = _job(a, 24, 16, chains=128, mutations=64, seed_paths=2000, **kw)
+    film, st = m.Scene(ctx, a).render(job)
+    orc, _, _, ost = po.OracleScene(a).mlt_render(job)
+    print(f"[mlt room film, {what}] dev {film.mean(axis=(0, 1))} oracle {orc.mean(axis=(0, 1))} "
+          f"samples {st['mlt_samples']} / {ost['samples']}", flush=True)
+    if "max_rejections" in kw:
+        # the limit withholds most samples of the default job (limit 500)
+        base = po.OracleScene(a).mlt_render(_job(a, 24, 16, chains=128, mutations=64, seed_paths=2000))[3]
+        assert ost["samples"] < 0.5 * base["samples"]
+    assert abs(st["mlt_samples"] - ost["samples"]) <= 0.02 * ost["samples"]
+    assert np.allclose(film.mean(axis=(0, 1)), orc.mean(axis=(0, 1)), rtol=0.05)
+
+
 def test_room_film_matches_oracle_and_is_deterministic(ctx):
     a = m.generate(m.MPT_SCENE_ROOM)
     job = _job(a, 24, 16, chains=128, mutations=64, seed_paths=2000)
