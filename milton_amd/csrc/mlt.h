@@ -1254,8 +1254,19 @@ __global__ __launch_bounds__(kWeightsBlock) void k_mlt_weights(MltDesc *dd, MltB
     const uint32_t lo = hi > kWeightsChunk ? hi - kWeightsChunk : 0u;
     for (uint32_t g = lo + t; g < hi; g += kWeightsBlock) buf[g - lo] = B.cdf[g];
     __syncthreads();
-    if (t == 0)
-      for (uint32_t g = hi; g-- > lo;) sum += buf[g - lo];
+    if (t == 0) {
+      // the same adds in the same order, the LDS reads of eight issued
+      // before them
+      uint32_t g = hi - lo;
+      for (; g >= 8; g -= 8) {
+        double v[8];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) v[q] = buf[g - 1 - q];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) sum += v[q];
+      }
+      while (g > 0) sum += buf[--g];
+    }
     __syncthreads();
     hi = lo;
   }
@@ -1270,21 +1281,41 @@ __global__ __launch_bounds__(kWeightsBlock) void k_mlt_weights(MltDesc *dd, MltB
     if (found) d.b = sum / (double)(n + d.n_prev);
   }
   if (!found) return;
+  // the first and last entries of positive weight, found by every thread
+  // (scan and first_last share no storage with the sums)
+  uint32_t *first_last = scan;
+  if (t == 0) {
+    first_last[0] = ~0u;
+    first_last[1] = 0;
+  }
+  __syncthreads();
   double acc = 0;
-  uint32_t first = ~0u, last = 0;
   for (uint32_t lo = 0; lo < n; lo += kWeightsChunk) {
     const uint32_t hi = min(n, lo + kWeightsChunk);
-    for (uint32_t g = lo + t; g < hi; g += kWeightsBlock) buf[g - lo] = B.cdf[g] / sum;
+    for (uint32_t g = lo + t; g < hi; g += kWeightsBlock) {
+      const double w = B.cdf[g] / sum;
+      buf[g - lo] = w;
+      if (w > 0) {
+        atomicMin(first_last, g);
+        atomicMax(first_last + 1, g);
+      }
+    }
     __syncthreads();
     if (t == 0) {
-      for (uint32_t g = lo; g < hi; ++g) {
-        const double w = buf[g - lo];
-        acc += w;
-        buf[g - lo] = acc;
-        if (w > 0) {
-          if (first == ~0u) first = g;
-          last = g;
+      uint32_t g = 0;
+      for (; g + 8 <= hi - lo; g += 8) {
+        double w[8];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) w[q] = buf[g + q];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+          acc += w[q];
+          buf[g + q] = acc;
         }
+      }
+      for (; g < hi - lo; ++g) {
+        acc += buf[g];
+        buf[g] = acc;
       }
     }
     __syncthreads();
@@ -1292,8 +1323,8 @@ __global__ __launch_bounds__(kWeightsBlock) void k_mlt_weights(MltDesc *dd, MltB
     __syncthreads();
   }
   if (t == 0) {
-    d.first_pos = first;
-    d.last_pos = last;
+    d.first_pos = first_last[0];
+    d.last_pos = first_last[1];
   }
 }
 
