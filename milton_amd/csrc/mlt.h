@@ -51,6 +51,11 @@ constexpr uint64_t kRun = (uint64_t)kCap * kSlots * kWave;   // float4 of one wa
 // runs of path records for n lanes
 __host__ __device__ constexpr uint64_t wave_runs(uint64_t n) { return (n + kWave - 1) / kWave; }
 constexpr uint32_t kBlock = 128;     // threads per block of the chain kernels
+// threads per block of the seed kernel: its few waves (10,000 seed paths
+// by default) spread over more CUs; the kernels' LDS rings are laid out for
+// kBlock lanes, so any smaller block stays within them
+constexpr uint32_t kSeedBlock = 64;
+static_assert(kSeedBlock <= kBlock, "the seed kernel's ring rows hold kBlock lanes");
 constexpr uint32_t kStackRows = 8;   // traversal ring rows per lane
 constexpr double kFix = 16777216.0;  // film sums: units of b / 2^24
 

@@ -2005,7 +2005,7 @@ int render_mlt(mpt_scene *s, const mpt_job *job, float4 *film, mpt_render_stats 
   for (uint32_t round = 0; round < mlt::kRounds && !found; ++round) {
     d.round = round;
     MPT_HIP(ctx, hipMemcpyAsync(dd, &d, sizeof(d), hipMemcpyHostToDevice, st));
-    mlt::k_mlt_seed<<<(N + mlt::kBlock - 1) / mlt::kBlock, mlt::kBlock, 0, st>>>(s->dev, dd, B);
+    mlt::k_mlt_seed<<<(N + mlt::kSeedBlock - 1) / mlt::kSeedBlock, mlt::kSeedBlock, 0, st>>>(s->dev, dd, B);
     MPT_LAUNCHED(ctx, "k_mlt_seed");
     mlt::k_mlt_weights<<<1, mlt::kWeightsBlock, 0, st>>>(dd, B);
     MPT_LAUNCHED(ctx, "k_mlt_weights");
